@@ -1,0 +1,238 @@
+"""Synthetic MPM scenes with seeded inputs.
+
+A scene is a plain dict of numpy arrays laid out exactly as the reference
+engine binds them to a solver (include/solver/particle_data.h:44-58,
+include/solver/material_data.h:50-64, include/engine/shape.h:58-89,
+include/engine/geometry.h:65-118):
+
+  particles: xm float32[N,4] (position, mass), v float32[N,4], params float32[N,4],
+             type int32[N], mask uint8[N]
+  shapes:    ids int32[S], type int32[cap], geom int32[cap], and float32[cap,4]
+             tables pos, rot (quaternion x,y,z,w), invScale, linVel, angVel,
+             comInvMass, inertia0, inertia1, params0 (smooth, drag, friction, fatten)
+  geometry:  gtype int32[G], gparams float32[G,4]
+  sdf:       sdf_dims int32[K,4], sdf_lbcs float32[K,4], sdf_data float32[D,4]
+
+Random numbers come from SplitMix64; a float is (u >> 40) * 2^-24.
+"""
+import math
+
+import numpy as np
+
+M64 = (1 << 64) - 1
+
+# enums (include/solver/mpm_solver.h:51-61, include/engine/geometry.h:42-55,
+# include/engine/shape.h:43-48, include/solver/material_data.h:41-46)
+MLS, PB, STD = 0, 1, 2
+BOX, SPHERE, PLANE, CAPSULE, TRIMESH, QUAD_SLICER, TRIMESH_SLICER, LINES, ARC = range(9)
+STATIC, KINEMATIC, DYNAMIC = range(3)
+COROTATIONAL, NEOHOOKEAN, LINEAR = range(3)
+
+
+class SplitMix64:
+    """SplitMix64 stream (Steele, Lea, Flood 2014)."""
+
+    def __init__(self, seed):
+        self.state = np.uint64(seed & M64)
+
+    def next_u64(self, n):
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + k * np.uint64(0x9E3779B97F4A7C15)
+            self.state = self.state + np.uint64(n) * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n, lo=0.0, hi=1.0):
+        u = (self.next_u64(n) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+        return lo + (hi - lo) * u
+
+
+def lame(E, nu):
+    """Lame parameters as the reference's tests compute them (tests/test_benchmark/main.cu:111-114)."""
+    lam = (E * nu) / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    return lam, mu
+
+
+def new_scene(kind, dx, lo, hi, n_particles, dt, gravity=(0.0, -9.81, 0.0), iterations=10):
+    return dict(kind=kind, dx=float(dx), lo=np.asarray(lo, np.float32), hi=np.asarray(hi, np.float32),
+                gravity=tuple(float(g) for g in gravity), dt=float(dt), iterations=int(iterations),
+                contact_model=0,
+                xm=np.zeros((n_particles, 4), np.float32), v=np.zeros((n_particles, 4), np.float32),
+                params=np.zeros((n_particles, 4), np.float32), type=np.zeros(n_particles, np.int32),
+                mask=np.ones(n_particles, np.uint8), shape_list=[], geoms=[],
+                sdf_dims=np.zeros((1, 4), np.int32), sdf_lbcs=np.zeros((1, 4), np.float32),
+                sdf_data=np.zeros((1, 4), np.float32), frames=[])
+
+
+def grid_dims(scene):
+    """Node count per axis, int(extent/dx + 1) in FP32 (src/solver/gpu_mls_mpm_solver.cu:48-50)."""
+    ext = scene["hi"] - scene["lo"]
+    return tuple(int(np.float32(e) / np.float32(scene["dx"]) + np.float32(1.0)) for e in ext)
+
+
+def add_shape(scene, gtype, gparams, pos=(0, 0, 0), rot=(0, 0, 0, 1), stype=STATIC, params0=(0.0, 1.0, 0.0, 0.0),
+              inv_scale=(1, 1, 1), lin_vel=(0, 0, 0), ang_vel=(0, 0, 0), com_inv_mass=(0, 0, 0, 0),
+              inertia0=(0, 0, 0, 0), inertia1=(0, 0, 0, 0)):
+    """Append a geometry and a shape that uses it; returns the shape id."""
+    scene["geoms"].append((int(gtype), tuple(float(x) for x in gparams)))
+    sid = len(scene["shape_list"])
+    scene["shape_list"].append(dict(type=int(stype), geom=len(scene["geoms"]) - 1, pos=tuple(pos) + (0.0,),
+                                    rot=tuple(rot), invScale=tuple(inv_scale) + (0.0,),
+                                    linVel=tuple(lin_vel) + (0.0,), angVel=tuple(ang_vel) + (0.0,),
+                                    comInvMass=tuple(com_inv_mass), inertia0=tuple(inertia0),
+                                    inertia1=tuple(inertia1), params0=tuple(params0)))
+    return sid
+
+
+SHAPE_F4 = ("pos", "rot", "invScale", "linVel", "angVel", "comInvMass", "inertia0", "inertia1", "params0")
+
+
+def shape_tables(scene):
+    """ShapeData / GeometryData tables (at least one row each, so no pointer is null)."""
+    sl = scene["shape_list"]
+    cap = max(1, len(sl))
+    t = dict(ids=np.arange(len(sl), dtype=np.int32), type=np.zeros(cap, np.int32), geom=np.zeros(cap, np.int32))
+    for k in SHAPE_F4:
+        t[k] = np.zeros((cap, 4), np.float32)
+    t["rot"][:, 3] = 1.0
+    for i, s in enumerate(sl):
+        t["type"][i] = s["type"]
+        t["geom"][i] = s["geom"]
+        for k in SHAPE_F4:
+            t[k][i] = s[k]
+    g = scene["geoms"] or [(BOX, (0.0, 0.0, 0.0, 0.0))]
+    t["gtype"] = np.array([x[0] for x in g], np.int32)
+    t["gparams"] = np.array([x[1] for x in g], np.float32)
+    return t
+
+
+def apply_frame(scene, frame):
+    """Kinematic target for one frame: teleport to the frame's pose, set its velocity."""
+    s = scene["shape_list"][frame["shape"]]
+    s["pos"] = tuple(frame["pos"]) + (0.0,)
+    s["linVel"] = tuple(frame["lin_vel"]) + (0.0,)
+
+
+def lattice_block(rng, n, origin, spacing, jitter):
+    """n = (nx, ny, nz) lattice, x fastest; jitter is the half-width of a uniform offset per axis."""
+    nx, ny, nz = n
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = np.stack([ii.transpose(2, 1, 0).ravel(), jj.transpose(2, 1, 0).ravel(), kk.transpose(2, 1, 0).ravel()], 1)
+    pts = np.asarray(origin, np.float64) + (idx + 0.5) * np.asarray(spacing, np.float64)
+    if jitter:
+        pts = pts + rng.uniform(pts.size, -1.0, 1.0).reshape(pts.shape) * np.asarray(jitter, np.float64)
+    return pts
+
+
+def config1(n_side=32, steps=200, seed=20261203 + 1):
+    """Plain MLS elastic cube on an SDF ground plane: 32^3 = 32,768 particles,
+    64^3-cell unit box, StVK (co-rotational falls through to it), dt = 1e-4."""
+    rng = SplitMix64(seed)
+    dx = 1.0 / 64
+    spacing = 1.0 / 128
+    side = n_side * spacing
+    N = n_side ** 3
+    sc = new_scene(MLS, dx, (0, 0, 0), (1, 1, 1), N, 1e-4)
+    origin = (0.5 - side / 2, 0.1 + dx / 2, 0.5 - side / 2)
+    pts = lattice_block(rng, (n_side,) * 3, origin, (spacing,) * 3, 0.25 * dx)
+    sc["xm"][:, :3] = pts
+    sc["xm"][:, 3] = 1000.0 * spacing ** 3
+    sc["v"][:, 1] = -0.5
+    lam, mu = lame(1e4, 0.3)
+    sc["params"][:, 0] = lam
+    sc["params"][:, 1] = mu
+    sc["type"][:] = COROTATIONAL
+    add_shape(sc, PLANE, (0, 0, 0, 0), pos=(0, 0.1, 0), params0=(0.0, 1.0, 0.0, 0.0))
+    sc["steps"] = steps
+    return sc
+
+
+def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, frame_substeps=10):
+    """Plain PB-MPM tissue slab 1.0 x 0.2 x 1.0 m: stratified 128x16x128 = 262,144
+    particles on a 128x64x128-cell grid (dx = 1/128), Neo-Hookean with PB
+    parameters (alpha, r) = (0.5/dt, 0.9), kinematic sphere indenter r = 0.1 m
+    moving down at 0.05 m/s, friction 0.3, dt = 2e-3, 20 iterations."""
+    rng = SplitMix64(seed)
+    dx = 1.0 / 128
+    dt = 2e-3
+    N = nx * ny * nz
+    sc = new_scene(PB, dx, (0, 0, 0), (1, 0.5, 1), N, dt, iterations=iterations)
+    slab = (1.0, 0.2, 1.0)
+    y0 = 2.5 * dx
+    cell = (slab[0] / nx, slab[1] / ny, slab[2] / nz)
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = np.stack([ii.transpose(2, 1, 0).ravel(), jj.transpose(2, 1, 0).ravel(), kk.transpose(2, 1, 0).ravel()], 1)
+    u = rng.uniform(3 * N).reshape(N, 3)
+    pts = np.asarray((0.0, y0, 0.0)) + (idx + u) * np.asarray(cell)
+    sc["xm"][:, :3] = pts
+    sc["xm"][:, 3] = 1000.0 * (slab[0] * slab[1] * slab[2]) / N
+    sc["params"][:, 0] = 0.5 / dt
+    sc["params"][:, 1] = 0.9
+    sc["type"][:] = NEOHOOKEAN
+    top = y0 + slab[1]
+    r = 0.1
+    c0 = (0.5, top + 0.005 + r, 0.5)
+    add_shape(sc, SPHERE, (r, 0, 0, 0), pos=c0, stype=KINEMATIC, params0=(0.0, 1.0, 0.3, 0.0))
+    sc["steps"] = steps
+    # Kinematic target per frame of `frame_substeps` substeps: the shape
+    # teleports to the frame's end pose and moves at (0, -0.05, 0) during it
+    # (src/engine/shape_impl.cpp:72-107).
+    speed = 0.05
+    frames = []
+    for f in range((steps + frame_substeps - 1) // frame_substeps):
+        y = c0[1] - speed * dt * frame_substeps * (f + 1)
+        frames.append(dict(substeps=min(frame_substeps, steps - f * frame_substeps),
+                           shape=0, pos=(c0[0], y, c0[2]), lin_vel=(0.0, -speed, 0.0)))
+    sc["frames"] = frames
+    return sc
+
+
+def small_scene(kind, seed, n_side=8, shapes="plane", dx=1.0 / 32, jitter=0.25, v0=(0.0, -0.5, 0.0),
+                material=COROTATIONAL, E=1e4, nu=0.3):
+    """Small randomized block used by the parity tests (8^3 particles by default)."""
+    rng = SplitMix64(seed)
+    spacing = dx / 2
+    N = n_side ** 3
+    dt = 1e-4 if kind != PB else 1e-3
+    sc = new_scene(kind, dx, (0, 0, 0), (1, 1, 1), N, dt, iterations=4)
+    side = n_side * spacing
+    origin = (0.5 - side / 2, 0.2, 0.5 - side / 2)
+    pts = lattice_block(rng, (n_side,) * 3, origin, (spacing,) * 3, jitter * spacing)
+    sc["xm"][:, :3] = pts
+    sc["xm"][:, 3] = 1000.0 * spacing ** 3
+    vr = rng.uniform(3 * N, -0.1, 0.1).reshape(N, 3)
+    sc["v"][:, :3] = np.asarray(v0) + vr
+    if kind == PB:
+        sc["params"][:, 0] = 0.5 / dt
+        sc["params"][:, 1] = 0.9
+        sc["type"][:] = NEOHOOKEAN if material == COROTATIONAL else material
+    else:
+        lam, mu = lame(E, nu)
+        sc["params"][:, 0] = lam
+        sc["params"][:, 1] = mu
+        sc["type"][:] = material
+    if "plane" in shapes:
+        add_shape(sc, PLANE, (0, 0, 0, 0), pos=(0, 0.19, 0), params0=(0.02, 1.0, 0.3, 0.0))
+    if "sphere" in shapes:
+        add_shape(sc, SPHERE, (0.05, 0, 0, 0), pos=(0.5, 0.2 + side + 0.04, 0.5), stype=KINEMATIC,
+                  params0=(0.01, 0.9, 0.5, 0.0), lin_vel=(0.0, -0.3, 0.0))
+    if "box" in shapes:
+        a = math.radians(20.0)
+        add_shape(sc, BOX, (0.03, 0.02, 0.04, 0), pos=(0.5 + side / 2, 0.22, 0.5),
+                  rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), stype=KINEMATIC,
+                  params0=(0.01, 1.0, 0.2, 0.002), lin_vel=(-0.2, 0.0, 0.0), ang_vel=(0, 0.5, 0))
+    if "capsule" in shapes:
+        add_shape(sc, CAPSULE, (0.01, 0.03, 0, 0), pos=(0.5, 0.2 + side * 0.8, 0.5 - side / 2), stype=DYNAMIC,
+                  params0=(0.01, 1.0, 0.1, 0.0), lin_vel=(0, 0, 0.2), inv_scale=(1.0, 0.8, 1.25))
+    if "slicer" in shapes:
+        a = math.radians(90.0)
+        # vertical quad slicer through the block centre (local y is the cut normal)
+        add_shape(sc, QUAD_SLICER, (side / 3, side, 0, 0), pos=(0.5 + 0.3 * spacing, 0.2 + side, 0.5),
+                  rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), stype=KINEMATIC,
+                  params0=(0.0, 1.0, 0.0, 0.001))
+    sc["steps"] = 10
+    return sc
