@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Substep throughput of the MI355X PB-MPM solver on the 262,144-particle slab.
+
+Workload: a 1.0 x 0.2 x 1.0 m Neo-Hookean slab of 128 x 16 x 128 = 262,144
+particles on a 128 x 64 x 128-cell grid (129 x 65 x 129 nodes, dx = 1/128),
+PB-MPM with 20 solver iterations per substep, dt = 2e-3, a kinematic sphere
+indenter (r = 0.1 m) moving down at 0.05 m/s with friction 0.3, its target
+updated every 10 substeps.  Inputs are synthetic (SplitMix64-seeded) and
+already resident in HBM when timing starts.  A step is one substep.
+
+Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
+(the fused G2P+P2G iteration kernel) measured with HIP events on the solver
+stream over the timed region, and a CPU baseline timed on this host.
+"""
+import argparse
+import copy
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cressim-mpm_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import crmpm_scenes as S  # noqa: E402
+
+METRIC = "particle-substeps/s at 262k PB-MPM; batched & fwd+adj vs plain; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+# Minimal FP32 state traffic per particle for one G2P + one P2G of PB-MPM:
+# P2G reads x, v, F, C, (alpha, r) = 12+12+36+36+8 = 104 B; G2P reads x and
+# writes v, C = 12+12+36 = 60 B.  Grid: P2G writes and G2P reads 16 B per
+# touched node.
+BYTES_PER_PARTICLE_FUSED = 164
+BYTES_PER_TOUCHED_NODE_FUSED = 32
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
+
+
+def cpu_baseline(scene, substeps):
+    """Reference CPU solver (compiled from its sources) or the oracle, one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as CC
+
+    which = "ref" if CC.ref_available() else "oracle"
+    sc = copy.deepcopy(scene)
+    o = CC.CpuSolver(sc, which)
+    o.compute_initial_data()
+    S.apply_frame(sc, sc["frames"][0])
+    o.upload_shapes()
+    t0 = time.perf_counter()
+    o.step(substeps)
+    dt = time.perf_counter() - t0
+    o.close()
+    n = len(scene["xm"])
+    return {
+        "value": n * substeps / dt,
+        "unit": "particle-substeps/s",
+        "cores": 1,
+        "kind": "reference" if which == "ref" else "port",
+        "sample": f"{substeps} substep(s) of the full 262,144-particle scene, 20 iterations, "
+                  f"{dt:.1f} s on 1 core of {platform.processor() or platform.machine()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: "
+              "this solver has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import crmpm_hip as H
+
+    K, W = args.steps, args.warmup
+    scene = S.config2(steps=K + W)
+    n = len(scene["xm"])
+    solver = H.HipSolver(copy.deepcopy(scene))
+    solver.compute_initial_data()
+    frames = solver.scene["frames"]
+    fsub = frames[0]["substeps"]
+
+    def run(first, count):
+        # substeps [first, first + count) of the scene, kinematic target per frame
+        s = first
+        end = first + count
+        while s < end:
+            f = s // fsub
+            S.apply_frame(solver.scene, frames[f])
+            solver._upload_shapes()
+            k = min((f + 1) * fsub, end) - s
+            solver.step(k)
+            s += k
+
+    run(0, W)
+    torch.cuda.synchronize()
+    # touched nodes of one substep (grid nodes holding mass), for the grid term of the roofline
+    grid = solver.grid()
+    touched = int(np.count_nonzero(grid[:, 3] > 0))
+
+    solver.profile_begin(max_launches=K * (2 * scene["iterations"] + 8) + 64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(W, K)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = solver.profile_end()
+    elapsed = t1 - t0
+    value = n * K / elapsed
+
+    fused_ms, fused_n = prof["pb_fused"]
+    avg_fused_s = fused_ms / max(fused_n, 1) / 1e3
+    bytes_fused = BYTES_PER_PARTICLE_FUSED * n + BYTES_PER_TOUCHED_NODE_FUSED * touched
+    achieved = bytes_fused / avg_fused_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pb_mpm_neohookean_slab_sphere_indenter", "particles": n,
+                   "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
+                   "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu"},
+        "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1> (fused G2P+P2G)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
+                     "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched},
+        "kernel_ms_per_step": {k: v[0] / K for k, v in prof.items() if v[1]},
+    }
+    solver.close()
+    if args.cpu_substeps > 0:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

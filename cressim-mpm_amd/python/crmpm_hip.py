@@ -1,0 +1,302 @@
+"""ctypes binding of the MI355X MPM solver C-ABI (include/crmpm_hip.h).
+
+PyTorch only provides device memory and the stream; every solver operation
+runs in libcrmpm_hip.so.  Loading fails loudly when the library is missing:
+there is no CPU fallback.
+
+`HipSolver` plays the role of the reference engine for one scene: it owns
+the engine-side device buffers (particles, shape/geometry/SDF tables) in the
+reference's layouts, binds them to a solver handle and drives substeps,
+applying kinematic targets between frames like SceneImpl::_advanceTaskFn
+(src/engine/scene_impl.cpp:308-330).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libcrmpm_hip.so")
+
+import crmpm_scenes as S  # noqa: E402
+
+MLS, PB, STANDARD = 1, 3, 5
+KIND_FROM_SCENE = {S.MLS: MLS, S.PB: PB, S.STD: STANDARD}
+
+_vp = ctypes.c_void_p
+_f3 = ctypes.c_float * 3
+
+
+class ShapeData(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("type", "geometryIdx", "position", "rotation", "invScale", "linearVelocity",
+                                    "angularVelocity", "comInvMass", "inertiaInv0", "inertiaInv1", "force", "torque",
+                                    "params0")] + [("size", ctypes.c_int)]
+
+
+class GeometryData(ctypes.Structure):
+    _fields_ = [("type", _vp), ("params0", _vp), ("size", ctypes.c_int)]
+
+
+class GeometrySdfData(ctypes.Structure):
+    _fields_ = [("dimension", _vp), ("lowerBoundCellSize", _vp), ("gradientSignedDistance", _vp)]
+
+
+EXPORTS = {
+    "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
+    "crHipSolverDestroy": [_vp],
+    "crHipSolverGetGridDims": [_vp, ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverSetStream": [_vp, _vp],
+    "crHipSolverSetGravity": [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float],
+    "crHipSolverSetIntegrationStepSize": [_vp, ctypes.c_float],
+    "crHipSolverSetSolverIterations": [_vp, ctypes.c_int],
+    "crHipSolverSetShapeContactModel": [_vp, ctypes.c_int],
+    "crHipSolverBindParticleData": [_vp, _vp, _vp, ctypes.c_int],
+    "crHipSolverBindParticleMaterials": [_vp, _vp, _vp, ctypes.c_int],
+    "crHipSolverBindActiveMask": [_vp, _vp],
+    "crHipSolverSetNumActiveParticles": [_vp, ctypes.c_uint],
+    "crHipSolverSetNumShapes": [_vp, ctypes.c_int],
+    "crHipSolverBindShapeIds": [_vp, _vp],
+    "crHipSolverBindShapeData": [_vp, ctypes.POINTER(ShapeData)],
+    "crHipSolverBindGeometryData": [_vp, ctypes.POINTER(GeometryData)],
+    "crHipSolverBindGeometrySdfData": [_vp, ctypes.POINTER(GeometrySdfData)],
+    "crHipSolverInitialize": [_vp],
+    "crHipSolverComputeInitialData": [_vp, ctypes.c_uint, _vp],
+    "crHipSolverStep": [_vp, ctypes.POINTER(ctypes.c_float)],
+    "crHipSolverFetchResults": [_vp],
+    "crHipSolverGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                  ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverCopyGrid": [_vp, _vp],
+    "crHipSolverBinParticles": [_vp, _vp, _vp],
+    "crHipSolverProfileBegin": [_vp, ctypes.c_int],
+    "crHipSolverProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
+}
+
+PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g", "mls_g2p", "std_p2g", "std_g2p",
+                   "binning")
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libcrmpm_hip.so and declare every C-ABI entry point."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"libcrmpm_hip.so not found at {path}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, args in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with status {rc}")
+
+
+class HipSolver:
+    """One scene on the GPU: engine-side buffers + a solver handle."""
+
+    def __init__(self, scene, device="cuda:0", stream=None):
+        import torch
+
+        self.torch = torch
+        self.lib = load_library()
+        self.scene = scene
+        self.device = torch.device(device)
+        self.n = n = len(scene["xm"])
+        lib = self.lib
+        h = _vp()
+        check(lib.crHipSolverCreate(KIND_FROM_SCENE[scene["kind"]], n, scene["dx"], _f3(*scene["lo"]),
+                                    _f3(*scene["hi"]), ctypes.byref(h)), "create")
+        self.h = h
+        dims = (ctypes.c_int * 3)()
+        check(lib.crHipSolverGetGridDims(h, dims), "dims")
+        self.dims = tuple(dims)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        check(lib.crHipSolverSetStream(h, _vp(self.stream.cuda_stream)), "stream")
+        check(lib.crHipSolverSetGravity(h, *scene["gravity"]), "gravity")
+        check(lib.crHipSolverSetIntegrationStepSize(h, scene["dt"]), "dt")
+        check(lib.crHipSolverSetSolverIterations(h, scene["iterations"]), "iterations")
+        check(lib.crHipSolverSetShapeContactModel(h, scene.get("contact_model", 0)), "contact model")
+        dev = self.device
+        T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
+        self.xm = T(scene["xm"], np.float32)
+        self.v = T(scene["v"], np.float32)
+        self.params = T(scene["params"], np.float32)
+        self.type = T(scene["type"], np.int32)
+        self.mask = T(scene["mask"], np.uint8)
+        check(lib.crHipSolverBindParticleData(h, self.xm.data_ptr(), self.v.data_ptr(), n), "bind particles")
+        check(lib.crHipSolverBindParticleMaterials(h, self.params.data_ptr(), self.type.data_ptr(), n), "bind materials")
+        check(lib.crHipSolverBindActiveMask(h, self.mask.data_ptr()), "bind mask")
+        self._upload_shapes(first=True)
+        check(lib.crHipSolverInitialize(h), "initialize")
+        check(lib.crHipSolverSetNumActiveParticles(h, n), "num active")
+        F, C, V0, stride = _vp(), _vp(), _vp(), ctypes.c_int()
+        check(lib.crHipSolverGetStateArrays(h, ctypes.byref(F), ctypes.byref(C), ctypes.byref(V0), ctypes.byref(stride)),
+              "state arrays")
+        self._state_ptrs = (F.value, C.value, V0.value, stride.value)
+
+    def _upload_shapes(self, first=False):
+        torch = self.torch
+        sc = self.scene
+        t = S.shape_tables(sc)
+        dev = self.device
+        T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
+        if first or not hasattr(self, "sh"):
+            self.sh = {k: T(t[k], np.float32) for k in S.SHAPE_F4}
+            self.sh["type"] = T(t["type"], np.int32)
+            self.sh["geom"] = T(t["geom"], np.int32)
+            self.sh["ids"] = T(np.maximum(t["ids"], 0) if len(t["ids"]) else np.zeros(1, np.int32), np.int32)
+            cap = len(t["type"])
+            self.force = torch.zeros((cap, 4), dtype=torch.float32, device=dev)
+            self.torque = torch.zeros((cap, 4), dtype=torch.float32, device=dev)
+            self.gtype = T(t["gtype"], np.int32)
+            self.gparams = T(t["gparams"], np.float32)
+            self.sdf_dims = T(sc["sdf_dims"], np.int32)
+            self.sdf_lbcs = T(sc["sdf_lbcs"], np.float32)
+            self.sdf_data = T(sc["sdf_data"], np.float32)
+        else:
+            # pinned staging keeps the per-frame shape update asynchronous on the stream
+            for k in S.SHAPE_F4:
+                self.sh[k].copy_(torch.from_numpy(t[k]).pin_memory(), non_blocking=True)
+        p = lambda x: x.data_ptr()
+        # field order: type, geometryIdx, position, rotation, invScale, linearVelocity, angularVelocity,
+        # comInvMass, inertiaInv0, inertiaInv1, force, torque, params0
+        sd = ShapeData(p(self.sh["type"]), p(self.sh["geom"]), p(self.sh["pos"]), p(self.sh["rot"]),
+                       p(self.sh["invScale"]), p(self.sh["linVel"]), p(self.sh["angVel"]), p(self.sh["comInvMass"]),
+                       p(self.sh["inertia0"]), p(self.sh["inertia1"]), p(self.force), p(self.torque),
+                       p(self.sh["params0"]), len(t["type"]))
+        gd = GeometryData(p(self.gtype), p(self.gparams), len(t["gtype"]))
+        sdf = GeometrySdfData(p(self.sdf_dims), p(self.sdf_lbcs), p(self.sdf_data))
+        lib, h = self.lib, self.h
+        check(lib.crHipSolverBindShapeData(h, ctypes.byref(sd)), "bind shapes")
+        check(lib.crHipSolverBindGeometryData(h, ctypes.byref(gd)), "bind geometry")
+        check(lib.crHipSolverBindGeometrySdfData(h, ctypes.byref(sdf)), "bind sdf")
+        check(lib.crHipSolverBindShapeIds(h, p(self.sh["ids"])), "bind shape ids")
+        check(lib.crHipSolverSetNumShapes(h, len(t["ids"])), "num shapes")
+
+    def compute_initial_data(self, idx=None):
+        torch = self.torch
+        if idx is None:
+            idx = np.arange(self.n, dtype=np.uint32)
+        self._idx = torch.from_numpy(np.ascontiguousarray(idx, np.uint32).view(np.int32)).to(self.device)
+        check(self.lib.crHipSolverComputeInitialData(self.h, len(idx), self._idx.data_ptr()), "initial data")
+
+    def step(self, n=1):
+        dt = ctypes.c_float()
+        for _ in range(n):
+            check(self.lib.crHipSolverStep(self.h, ctypes.byref(dt)), "step")
+
+    def run(self, steps=None):
+        sc = self.scene
+        steps = sc.get("steps", 1) if steps is None else steps
+        if not sc["frames"]:
+            self.step(steps)
+            return
+        done = 0
+        for fr in sc["frames"]:
+            if done >= steps:
+                break
+            S.apply_frame(sc, fr)
+            self._upload_shapes()
+            k = min(fr["substeps"], steps - done)
+            self.step(k)
+            done += k
+
+    def sync(self):
+        check(self.lib.crHipSolverFetchResults(self.h), "fetch results")
+
+    def state(self):
+        torch = self.torch
+        self.sync()
+        F, C, V0, stride = self._state_ptrs
+        n = self.n
+        out = dict(xm=self.xm.cpu().numpy().astype(np.float64), v=self.v[:, :3].cpu().numpy().astype(np.float64))
+        for name, ptr, rows in (("F", F, 9), ("C", C, 9), ("V0", V0, 1)):
+            t = torch.empty(rows * stride, dtype=torch.float32, device=self.device)
+            hip_memcpy_d2d(t.data_ptr(), ptr, rows * stride * 4, self.stream.cuda_stream)
+            a = t.cpu().numpy().reshape(rows, stride)[:, :n].T.astype(np.float64)
+            out[name] = a if rows > 1 else a[:, 0]
+        return out
+
+    def grid(self):
+        torch = self.torch
+        total = self.dims[0] * self.dims[1] * self.dims[2]
+        g = torch.empty((total, 4), dtype=torch.float32, device=self.device)
+        check(self.lib.crHipSolverCopyGrid(self.h, g.data_ptr()), "copy grid")
+        self.sync()
+        return g.cpu().numpy().astype(np.float64)
+
+    def binning(self):
+        """Sorted block keys and the particle order of the current state."""
+        torch = self.torch
+        keys = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        order = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        check(self.lib.crHipSolverBinParticles(self.h, keys.data_ptr(), order.data_ptr()), "bin particles")
+        self.sync()
+        return keys.cpu().numpy().view(np.uint32), order.cpu().numpy().view(np.uint32)
+
+    def profile_begin(self, max_launches):
+        check(self.lib.crHipSolverProfileBegin(self.h, max_launches), "profile begin")
+
+    def profile_end(self):
+        n = len(PROF_CATEGORIES)
+        ms = (ctypes.c_float * n)()
+        cnt = (ctypes.c_int * n)()
+        check(self.lib.crHipSolverProfileEnd(self.h, ms, cnt, n), "profile end")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_CATEGORIES)}
+
+    def shape_force(self):
+        self.sync()
+        return self.force.cpu().numpy().astype(np.float64), self.torque.cpu().numpy().astype(np.float64)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.crHipSolverDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_HIPRT = None
+
+
+def hip_runtime():
+    """The HIP runtime library that torch itself loaded."""
+    global _HIPRT
+    if _HIPRT is None:
+        import glob
+
+        import torch
+
+        cands = sorted(glob.glob(os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so*")))
+        cands += ["libamdhip64.so"]
+        for c in cands:
+            try:
+                _HIPRT = ctypes.CDLL(c)
+                break
+            except OSError:
+                continue
+        if _HIPRT is None:
+            raise RuntimeError("HIP runtime library not found")
+        _HIPRT.hipMemcpyAsync.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]
+        _HIPRT.hipMemcpyAsync.restype = ctypes.c_int
+        _HIPRT.hipStreamSynchronize.argtypes = [_vp]
+        _HIPRT.hipStreamSynchronize.restype = ctypes.c_int
+    return _HIPRT
+
+
+def hip_memcpy_d2d(dst, src, nbytes, stream_handle):
+    rt = hip_runtime()
+    check(rt.hipMemcpyAsync(_vp(dst), _vp(src), nbytes, 3, _vp(stream_handle)), "hipMemcpyAsync")  # 3: device to device
+    check(rt.hipStreamSynchronize(_vp(stream_handle)), "hipStreamSynchronize")

@@ -152,29 +152,34 @@ def config1(n_side=32, steps=200, seed=20261203 + 1):
 
 
 def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, frame_substeps=10):
-    """Plain PB-MPM tissue slab 1.0 x 0.2 x 1.0 m: stratified 128x16x128 = 262,144
-    particles on a 128x64x128-cell grid (dx = 1/128), Neo-Hookean with PB
-    parameters (alpha, r) = (0.5/dt, 0.9), kinematic sphere indenter r = 0.1 m
-    moving down at 0.05 m/s, friction 0.3, dt = 2e-3, 20 iterations."""
+    """Plain PB-MPM tissue slab: stratified nx x ny x nz particles, one per
+    (1/128, 0.0125, 1/128) m cell, i.e. the full 128 x 16 x 128 = 262,144-particle
+    slab of 1.0 x 0.2 x 1.0 m; smaller counts give a smaller slab at the same
+    spacing, centred in x/z.  Grid [0,1] x [0,0.5] x [0,1], dx = 1/128
+    (129 x 65 x 129 nodes).  Neo-Hookean with PB parameters (alpha, r) =
+    (0.5/dt, 0.9); a kinematic sphere indenter of radius 0.1 m (scaled with the
+    slab) starts 5 mm above the top and moves down at 0.05 m/s, friction 0.3;
+    dt = 2e-3, 20 iterations per substep."""
     rng = SplitMix64(seed)
     dx = 1.0 / 128
     dt = 2e-3
     N = nx * ny * nz
     sc = new_scene(PB, dx, (0, 0, 0), (1, 0.5, 1), N, dt, iterations=iterations)
-    slab = (1.0, 0.2, 1.0)
+    cell = (1.0 / 128, 0.2 / 16, 1.0 / 128)
+    slab = (nx * cell[0], ny * cell[1], nz * cell[2])
     y0 = 2.5 * dx
-    cell = (slab[0] / nx, slab[1] / ny, slab[2] / nz)
+    origin = (0.5 - slab[0] / 2, y0, 0.5 - slab[2] / 2)
     ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     idx = np.stack([ii.transpose(2, 1, 0).ravel(), jj.transpose(2, 1, 0).ravel(), kk.transpose(2, 1, 0).ravel()], 1)
     u = rng.uniform(3 * N).reshape(N, 3)
-    pts = np.asarray((0.0, y0, 0.0)) + (idx + u) * np.asarray(cell)
+    pts = np.asarray(origin) + (idx + u) * np.asarray(cell)
     sc["xm"][:, :3] = pts
-    sc["xm"][:, 3] = 1000.0 * (slab[0] * slab[1] * slab[2]) / N
+    sc["xm"][:, 3] = 1000.0 * (cell[0] * cell[1] * cell[2])
     sc["params"][:, 0] = 0.5 / dt
     sc["params"][:, 1] = 0.9
     sc["type"][:] = NEOHOOKEAN
     top = y0 + slab[1]
-    r = 0.1
+    r = 0.1 * min(nx, nz) / 128
     c0 = (0.5, top + 0.005 + r, 0.5)
     add_shape(sc, SPHERE, (r, 0, 0, 0), pos=c0, stype=KINEMATIC, params0=(0.0, 1.0, 0.3, 0.0))
     sc["steps"] = steps

@@ -1,0 +1,172 @@
+/*
+ * crmpm_hip.h -- C-ABI of the MI355X (gfx950) MPM substep solver.
+ *
+ * One opaque handle per solver instance replaces one GPU solver object of the
+ * reference (GpuMlsMpmSolver / GpuPbMpmSolver / GpuStandardMpmSolver behind
+ * crmpm::MpmSolver, include/solver/mpm_solver.h:63-190 and
+ * src/solver/mpm_solver_gpu.h:43-57).  Each entry point below names the
+ * reference member it stands in for.  Device pointers are plain pointers into
+ * HBM with the reference's element layouts:
+ *   positionMass  float4[N]  (x, y, z, mass)        include/solver/particle_data.h:48
+ *   velocity      float4[N]  (Vec3f, 16 B)           include/solver/particle_data.h:49
+ *   params0       float4[N], type int32[N]           include/solver/material_data.h:53-54
+ *   activeMask    uint8[N]                           include/solver/mpm_solver.h:133
+ * All calls return 0 on success or a CRMPM_HIP_* error code; none throws.
+ * Handles are independent: no global mutable state, so different handles may
+ * be driven from different host threads.  Calls on one handle must not overlap.
+ */
+#ifndef CRMPM_HIP_H
+#define CRMPM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Solver kinds: the values of crmpm::MpmSolverType for the GPU solvers
+ * (include/solver/mpm_solver.h:51-61). */
+#define CRMPM_HIP_MLS 1
+#define CRMPM_HIP_PB 3
+#define CRMPM_HIP_STANDARD 5
+
+/* crmpm::ShapeContactModel (include/solver/mpm_solver.h:45-49) */
+#define CRMPM_HIP_CONTACT_KINEMATIC 0
+#define CRMPM_HIP_CONTACT_EFFECTIVE_MASS 1
+
+#define CRMPM_HIP_OK 0
+#define CRMPM_HIP_ERR_ARG 1
+#define CRMPM_HIP_ERR_STATE 2
+#define CRMPM_HIP_ERR_ALLOC 3
+#define CRMPM_HIP_ERR_LAUNCH 4
+#define CRMPM_HIP_ERR_CAPACITY 5
+
+typedef struct CrHipSolver CrHipSolver;
+
+/* ShapeData SoA of device pointers (include/engine/shape.h:58-89).  Vec3f and
+ * Quat entries are 16-byte float4 records. */
+typedef struct CrHipShapeData
+{
+    int *type;           /* ShapeType: 0 static, 1 kinematic, 2 dynamic */
+    int *geometryIdx;
+    float *position;     /* float4 */
+    float *rotation;     /* float4 quaternion (x, y, z, w) */
+    float *invScale;     /* float4 */
+    float *linearVelocity;
+    float *angularVelocity;
+    float *comInvMass;   /* float4: centre of mass, inverse mass */
+    float *inertiaInv0;  /* float4: Ixx, Iyy, Izz */
+    float *inertiaInv1;  /* float4: Ixy, Ixz, Iyz */
+    float *force;        /* float4, accumulated coupling force */
+    float *torque;       /* float4, accumulated coupling torque */
+    float *params0;      /* float4: sdf smooth distance, drag, friction, fatten */
+    int size;
+} CrHipShapeData;
+
+/* GeometryData SoA (include/engine/geometry.h:65-108) */
+typedef struct CrHipGeometryData
+{
+    int *type;      /* GeometryType 0..8 */
+    float *params0; /* float4 */
+    int size;
+} CrHipGeometryData;
+
+/* GeometrySdfData SoA (include/engine/geometry.h:113-118) */
+typedef struct CrHipGeometrySdfData
+{
+    int *dimension;               /* int4 (Vec3i) per SDF */
+    float *lowerBoundCellSize;    /* float4 per SDF */
+    float *gradientSignedDistance;/* float4 per SDF sample */
+} CrHipGeometrySdfData;
+
+/* Constructor GpuMlsMpmSolver(int numParticles, float cellSize, Bounds3 gridBound)
+ * (src/solver/gpu_mls_mpm_solver.cu:40-52; PB :40-52; standard :37-49).
+ * kind is CRMPM_HIP_MLS, CRMPM_HIP_PB or CRMPM_HIP_STANDARD. */
+int crHipSolverCreate(int kind, int numMaxParticles, float cellSize, const float boundMin[3],
+                      const float boundMax[3], CrHipSolver **out);
+/* Releasable::release -> _release (src/solver/gpu_mls_mpm_solver.cu:111-124) */
+int crHipSolverDestroy(CrHipSolver *s);
+
+/* Grid node counts per axis, int(extent / cellSize + 1) (src/solver/gpu_mls_mpm_solver.cu:48-50) */
+int crHipSolverGetGridDims(const CrHipSolver *s, int dims[3]);
+
+/* MpmSolverGpu::setCudaStream (src/solver/mpm_solver_gpu.cpp:47-50); stream is a hipStream_t */
+int crHipSolverSetStream(CrHipSolver *s, void *stream);
+/* MpmSolver::setGravity (include/solver/mpm_solver.h:84) */
+int crHipSolverSetGravity(CrHipSolver *s, float gx, float gy, float gz);
+/* MpmSolver::setIntegrationStepSize (include/solver/mpm_solver.h:89) */
+int crHipSolverSetIntegrationStepSize(CrHipSolver *s, float dt);
+/* MpmSolver::setSolverIterations (include/solver/mpm_solver.h:96; PB only) */
+int crHipSolverSetSolverIterations(CrHipSolver *s, int n);
+/* MpmSolver::setShapeContactModel (include/solver/mpm_solver.h:182) */
+int crHipSolverSetShapeContactModel(CrHipSolver *s, int model);
+
+/* MpmSolver::bindParticleData (include/solver/mpm_solver.h:112) */
+int crHipSolverBindParticleData(CrHipSolver *s, float *positionMass, float *velocity, int size);
+/* MpmSolver::bindParticleMaterials (include/solver/mpm_solver.h:119) */
+int crHipSolverBindParticleMaterials(CrHipSolver *s, float *params0, int *type, int size);
+/* MpmSolver::bindActiveMask (include/solver/mpm_solver.h:136) */
+int crHipSolverBindActiveMask(CrHipSolver *s, unsigned char *activeMask);
+/* MpmSolver::setNumActiveParticles (include/solver/mpm_solver.h:127) */
+int crHipSolverSetNumActiveParticles(CrHipSolver *s, unsigned int n);
+/* MpmSolver::setNumShapes / bindShapeIds (include/solver/mpm_solver.h:141,147) */
+int crHipSolverSetNumShapes(CrHipSolver *s, int n);
+int crHipSolverBindShapeIds(CrHipSolver *s, int *shapeIds);
+/* MpmSolver::bindShapeData / bindGeometryData / bindGeometrySdfData
+ * (include/solver/mpm_solver.h:153,159,165); the structs are copied. */
+int crHipSolverBindShapeData(CrHipSolver *s, const CrHipShapeData *d);
+int crHipSolverBindGeometryData(CrHipSolver *s, const CrHipGeometryData *d);
+int crHipSolverBindGeometrySdfData(CrHipSolver *s, const CrHipGeometrySdfData *d);
+
+/* MpmSolver::initialize (include/solver/mpm_solver.h:70): allocates F, C, V0
+ * and the grid, zeroed (src/solver/gpu_mls_mpm_solver.cu:54-83). */
+int crHipSolverInitialize(CrHipSolver *s);
+/* MpmSolver::computeInitialData (include/solver/mpm_solver.h:104): F := I and
+ * V0 from the mass already on the grid plus these particles' mass; indices is
+ * a device array of n particle indices. */
+int crHipSolverComputeInitialData(CrHipSolver *s, unsigned int n, const unsigned int *indices);
+/* MpmSolver::step (include/solver/mpm_solver.h:75): enqueue one substep on the
+ * solver stream; *dtOut (may be NULL) receives the step size. */
+int crHipSolverStep(CrHipSolver *s, float *dtOut);
+/* MpmSolver::fetchResults (src/solver/mpm_solver_gpu.cpp:37-40): stream sync */
+int crHipSolverFetchResults(CrHipSolver *s);
+
+/* Solver-owned per-particle state, which the reference keeps private
+ * (src/solver/gpu_mls_mpm_solver.cuh:61-77).  Device pointers into the
+ * solver's arrays, structure-of-arrays: element k of F (column-major, k =
+ * 3 * column + row) for particle p is F[k * stride + p]; same for C; V0[p].
+ * Used for checkpoints, the adjoint and parity checks. */
+int crHipSolverGetStateArrays(CrHipSolver *s, float **F, float **C, float **V0, int *stride);
+/* Binning stage of a substep, exposed for verification; it has no counterpart
+ * in the reference, which scatters in particle order.  For the current
+ * positions of the numActive particles: keysOut[k] is the k-th smallest
+ * 4x4x4-cell block key (bx + nbx * (by + nby * bz) of the base cell
+ * trunc((x - min) / dx - 0.5) >> 2, clamped to the grid; inactive particles
+ * get the key nbx * nby * nbz) and orderOut[k] the particle index, ties in
+ * index order (a stable sort).  Both are device arrays of numActive entries. */
+int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int *orderOut);
+/* Launch timing, an addition of this solver (the reference times nothing on
+ * the device).  Between Begin and End every particle-transfer, grid-update and
+ * binning launch of crHipSolverStep is bracketed by a pair of hipEvents on the
+ * solver stream (at most maxLaunches launches are recorded).  End synchronizes
+ * the stream and returns the summed milliseconds and launch counts per
+ * category (CRMPM_HIP_PROF_*). */
+#define CRMPM_HIP_PROF_PB_FUSED 0  /* PB: G2P of iteration k-1 fused with P2G of iteration k */
+#define CRMPM_HIP_PROF_PB_P2G 1    /* PB: P2G of the first iteration */
+#define CRMPM_HIP_PROF_PB_G2P 2    /* PB: last G2P fused with the particle integration */
+#define CRMPM_HIP_PROF_UPDATE 3    /* grid update, all solvers */
+#define CRMPM_HIP_PROF_MLS_P2G 4
+#define CRMPM_HIP_PROF_MLS_G2P 5
+#define CRMPM_HIP_PROF_STD_P2G 6
+#define CRMPM_HIP_PROF_STD_G2P 7
+#define CRMPM_HIP_PROF_BINNING 8   /* binning, radix sort, tiling, PB staging */
+#define CRMPM_HIP_PROF_CATEGORIES 9
+int crHipSolverProfileBegin(CrHipSolver *s, int maxLaunches);
+int crHipSolverProfileEnd(CrHipSolver *s, float *msPerCategory, int *launchesPerCategory, int numCategories);
+/* Copies the grid (mass-weighted momentum or velocity in xyz, mass in w) to a
+ * device float4 array in node order x + y * nx + z * nx * ny. */
+int crHipSolverCopyGrid(CrHipSolver *s, float *dst);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRMPM_HIP_H */

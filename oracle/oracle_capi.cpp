@@ -230,6 +230,23 @@ extern "C"
         });
     }
 
+    // Base cell of each particle, trunc((x - min) * invdx - 0.5) per axis
+    // (src/solver/mls_mpm_algorithm.h:172-174).
+    void orc_base_cells(void *h, int n, int *base3)
+    {
+        with(h, [&](auto &s) {
+            for (int p = 0; p < n; ++p)
+            {
+                orc::V3<int> b;
+                std::decay_t<decltype(s.getX(0))> fx;
+                s.cellOf(s.getX(p), b, fx);
+                base3[3 * p] = b.x;
+                base3[3 * p + 1] = b.y;
+                base3[3 * p + 2] = b.z;
+            }
+        });
+    }
+
     void orc_get_grid(void *h, double *g4)
     {
         with(h, [&](auto &s) {

@@ -48,12 +48,15 @@ def _declare(lib, pre):
         "get_particles": (None, [_vp, ctypes.c_int, _d, _d, _d, _d, _d]),
         "set_deformation": (None, [_vp, ctypes.c_int, _f, _f]),
         "get_grid": (None, [_vp, _d]),
+        "base_cells": (None, [_vp, ctypes.c_int, _i]),
         "svd3": (None, [ctypes.c_int, ctypes.c_int, _f, _d, _d, _d]),
         "stress": (None, [ctypes.c_int, _f, _i, _f, _d]),
         "pb_constraint": (None, [ctypes.c_int, ctypes.c_float, _f, _i, _f, _f, _d]),
     }
     fns = {}
     for name, (res, args) in sig.items():
+        if not hasattr(lib, pre + name):
+            continue
         fn = getattr(lib, pre + name)
         fn.restype = res
         fn.argtypes = args
@@ -182,6 +185,11 @@ class CpuSolver:
         g = np.zeros((self.dims[0] * self.dims[1] * self.dims[2], 4))
         self.fn["get_grid"](self.h, P(g, _d))
         return g
+
+    def base_cells(self):
+        b = np.zeros((self.n, 3), np.int32)
+        self.fn["base_cells"](self.h, self.n, P(b, _i))
+        return b
 
     def shape_force(self):
         f = np.zeros((self.cap, 4))

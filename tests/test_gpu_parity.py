@@ -1,0 +1,183 @@
+"""GPU parity: the HIP solver (through the C-ABI) against the CPU oracle.
+
+Tolerances (FP32):
+  * binning keys and sort permutation: bit-exact against the oracle's base cells
+    and a stable sort;
+  * positions, velocities, F after 100 substeps: relative error <= 1e-4 (max
+    abs difference over the field's max magnitude); after 10 substeps 2e-5;
+  * the HIP path must be bitwise reproducible run to run.
+The FP64 replay error of both FP32 paths is printed for the record.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import cpu_checkers as CC
+import crmpm_scenes as S
+
+pytestmark = pytest.mark.gpu
+
+TOL_100 = 1e-4
+TOL_10 = 2e-5
+
+
+def rel(a, b):
+    scale = max(float(np.abs(b).max()), 1e-30)
+    return float(np.abs(a - b).max()) / scale
+
+
+def hip_run(sc, steps):
+    import crmpm_hip as H
+
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    g.run(steps)
+    st = g.state()
+    g.close()
+    return st
+
+
+def cpu_run(sc, steps, fp64=False):
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle", fp64=fp64)
+    o.compute_initial_data()
+    o.run(steps)
+    st = o.state()
+    o.close()
+    return st
+
+
+def report(label, gs, os_, od=None):
+    errs = {k: rel(gs[k], os_[k]) for k in ("xm", "v", "F", "C", "V0")}
+    line = f"{label}: gpu-vs-oracle " + " ".join(f"{k}={v:.2e}" for k, v in errs.items())
+    if od is not None:
+        line += " | fp64 replay: gpu " + " ".join(f"{k}={rel(gs[k], od[k]):.2e}" for k in ("xm", "v", "F"))
+        line += " cpu " + " ".join(f"{k}={rel(os_[k], od[k]):.2e}" for k in ("xm", "v", "F"))
+    print(line, flush=True)
+    return errs
+
+
+KINDS = {"mls": S.MLS, "pb": S.PB, "std": S.STD}
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_binning_bit_exact(kind):
+    import crmpm_hip as H
+
+    sc = S.small_scene(KINDS[kind], 3, n_side=12, shapes="plane", jitter=0.45)
+    # a few particles outside the grid and a masked one exercise the clamping and the sentinel
+    sc["xm"][0, 0] = -0.01
+    sc["xm"][1, 1] = 1.2
+    sc["mask"][5] = 0
+    print(f"binning {kind} n={len(sc['xm'])}", flush=True)
+    g = H.HipSolver(copy.deepcopy(sc))
+    keys, order = g.binning()
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    base = o.base_cells().astype(np.int64)
+    dims = np.array(g.dims)
+    nb = (dims + 3) // 4
+    b = np.clip(base, 0, dims - 1) >> 2
+    ref_keys = b[:, 0] + nb[0] * (b[:, 1] + nb[1] * b[:, 2])
+    if KINDS[kind] != S.STD:
+        ref_keys[sc["mask"] == 0] = np.prod(nb)
+    ref_order = np.argsort(ref_keys, kind="stable")
+    assert np.array_equal(order.astype(np.int64), ref_order)
+    assert np.array_equal(keys.astype(np.int64), ref_keys[ref_order])
+    g.close()
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_initial_volume(kind):
+    sc = S.small_scene(KINDS[kind], 11, n_side=10, shapes="plane")
+    print(f"initial volume {kind}", flush=True)
+    gs = hip_run(sc, 0)
+    os_ = cpu_run(sc, 0)
+    assert rel(gs["V0"], os_["V0"]) < 1e-6
+    assert np.array_equal(gs["F"], os_["F"])
+
+
+SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer"]
+
+
+@pytest.mark.parametrize("shapes", SHAPES)
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_parity_10_substeps(kind, shapes):
+    sc = S.small_scene(KINDS[kind], 7 + KINDS[kind], shapes=shapes)
+    gs = hip_run(sc, 10)
+    os_ = cpu_run(sc, 10)
+    errs = report(f"{kind} {shapes} 10 steps", gs, os_)
+    for k in ("xm", "v", "F"):
+        assert errs[k] <= TOL_10, (k, errs[k])
+
+
+def test_parity_mls_cube_100_substeps():
+    """Elastic cube on the ground plane, 16^3 particles, 100 substeps."""
+    sc = S.config1(n_side=16, steps=100)
+    gs = hip_run(sc, 100)
+    os_ = cpu_run(sc, 100)
+    od = cpu_run(sc, 100, fp64=True)
+    errs = report("mls cube 16^3 100 steps", gs, os_, od)
+    for k in ("xm", "v", "F"):
+        assert errs[k] <= TOL_100, (k, errs[k])
+
+
+def test_parity_pb_slab_100_substeps():
+    """PB-MPM slab with the sphere indenter, 32x8x32 particles, 20 iterations, 100 substeps."""
+    sc = S.config2(nx=32, ny=8, nz=32, steps=100, iterations=20)
+    gs = hip_run(sc, 100)
+    os_ = cpu_run(sc, 100)
+    od = cpu_run(sc, 100, fp64=True)
+    errs = report("pb slab 32x8x32 100 steps", gs, os_, od)
+    for k in ("xm", "v", "F"):
+        assert errs[k] <= TOL_100, (k, errs[k])
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_bitwise_reproducible(kind):
+    sc = S.small_scene(KINDS[kind], 5, shapes="plane+sphere+box")
+    a = hip_run(sc, 5)
+    b = hip_run(sc, 5)
+    for k in ("xm", "v", "F", "C"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_grid_after_one_step():
+    import crmpm_hip as H
+
+    sc = S.small_scene(S.MLS, 21, shapes="plane+sphere")
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    g.step(1)
+    gg = g.grid()
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    o.step(1)
+    og = o.grid()
+    assert rel(gg[:, 3], og[:, 3]) < 1e-6
+    assert rel(gg[:, :3], og[:, :3]) < 1e-5
+    g.close()
+
+
+def test_masked_and_partial_active():
+    import crmpm_hip as H
+
+    sc = S.small_scene(S.MLS, 31, shapes="plane")
+    sc["mask"][::7] = 0
+    n = len(sc["xm"])
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, n - 40), "num active")
+    g.step(5)
+    gs = g.state()
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    o.fn["set_num_active"](o.h, n - 40)
+    o.step(5)
+    os_ = o.state()
+    assert rel(gs["xm"], os_["xm"]) < TOL_10
+    # inactive particles are untouched
+    inactive = np.zeros(n, bool)
+    inactive[::7] = True
+    inactive[n - 40:] = True
+    assert np.array_equal(gs["xm"][inactive], sc["xm"][inactive].astype(np.float64))
+    g.close()
