@@ -147,6 +147,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched},
         "kernel_ms_per_step": {k: v[0] / K for k, v in prof.items() if v[1]},
     }
+    st = solver.state()
+    out["state_check"] = {"nonfinite_particles": int((~np.isfinite(st["xm"])).any(axis=1).sum()),
+                          "max_speed": float(np.abs(st["v"]).max()),
+                          "y_range": [float(st["xm"][:, 1].min()), float(st["xm"][:, 1].max())]}
     solver.close()
     if args.cpu_substeps > 0:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps)

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #define CR_DEV __device__ __forceinline__
+#define CR_DEV_NOINLINE __device__ __noinline__
 
 namespace crhip
 {
@@ -786,7 +787,7 @@ namespace crhip
     // checkOnDifferentSides (src/solver/mpm_rigid_contact.h:401-478); the
     // normal stays in the slicer's local frame.
     template <bool ComputeNormal>
-    CR_DEV bool different_sides(const ShapeView &sv, int nSlicers, V3 p1, V3 p2, V3 &normal)
+    CR_DEV_NOINLINE bool different_sides(const ShapeView &sv, int nSlicers, V3 p1, V3 p2, V3 &normal)
     {
         for (int k = 0; k < nSlicers; ++k)
         {

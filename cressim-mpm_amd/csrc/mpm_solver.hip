@@ -451,6 +451,63 @@ namespace
         return lx + kTN * (ly + kTN * lz);
     }
 
+    CR_DEV float wsel(const Weights &W, int i, int axis)
+    {
+        return i == 0 ? W.w[0][axis] : (i == 1 ? W.w[1][axis] : W.w[2][axis]);
+    }
+
+    // All 27 nodes of base cell (bx, by, bz) inside the grid: the tile box then
+    // holds them all and the node loop needs no checks.
+    CR_DEV bool nodes_inside(const GridGeom &g, int bx, int by, int bz)
+    {
+        return bx >= 0 && by >= 0 && bz >= 0 && bx + 2 < g.nx && by + 2 < g.ny && bz + 2 < g.nz;
+    }
+
+    // Visit the 27 nodes of a particle in the reference's (i, j, k) order:
+    // fn(li, a, b, c, dist, wt) with li the node's index in the tile box,
+    // dist = ((a, b, c) - fx) * dx and wt the B-spline weight.  Fast: fully
+    // unrolled and unchecked (requires nodes_inside).  Checked: out-of-grid
+    // nodes skipped, loops left rolled to keep the code small.
+    template <bool Fast, class Fn>
+    CR_DEV void for_each_node(const GridGeom &g, const TileInfo &ti, int bx, int by, int bz, V3 fx, const Weights &W, Fn &&fn)
+    {
+        if constexpr (Fast)
+        {
+            const int l0 = (bx - ti.bx * kTB) + kTN * ((by - ti.by * kTB) + kTN * (bz - ti.bz * kTB));
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
+                        const float wt = W.w[a][0] * W.w[b][1] * W.w[cc][2];
+                        fn(l0 + a + kTN * (b + kTN * cc), a, b, cc, dist, wt);
+                    }
+        }
+        else
+        {
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+                for (int b = 0; b < 3; ++b)
+#pragma unroll 1
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const int nx = bx + a, ny = by + b, nz = bz + cc;
+                        if (!in_grid(g, nx, ny, nz))
+                            continue;
+                        const int li = tile_local(ti, nx, ny, nz);
+                        if (li < 0)
+                            continue;
+                        const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
+                        const float wt = wsel(W, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2);
+                        fn(li, a, b, cc, dist, wt);
+                    }
+        }
+    }
+
     // ------------------------------------------------------------------
     // MLS-MPM P2G (src/solver/mls_mpm_algorithm.h:147-241)
     // ------------------------------------------------------------------
@@ -506,30 +563,22 @@ namespace
             const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
             const M3 affine = stress * sc + Cm * m;
             const V3 vm = vel * m;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
-                    {
-                        const int nx = bx + a, ny = by + b, nz = bz + cc;
-                        if (!in_grid(g, nx, ny, nz))
-                            continue;
-                        const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
-                        V3 nrm;
-                        if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                            continue;
-                        const int li = tile_local(ti, nx, ny, nz);
-                        if (li < 0)
-                            continue;
-                        const float wt = W.w[a][0] * W.w[b][1] * W.w[cc][2];
-                        const V3 contrib = (vm + mul(affine, dist)) * wt;
-                        atomicAdd(&A[3][li], wt * m);
-                        atomicAdd(&A[0][li], contrib.x);
-                        atomicAdd(&A[1][li], contrib.y);
-                        atomicAdd(&A[2][li], contrib.z);
-                    }
+            auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
+                const V3 contrib = (vm + mul(affine, dist)) * wt;
+                atomicAdd(&A[3][li], wt * m);
+                atomicAdd(&A[0][li], contrib.x);
+                atomicAdd(&A[1][li], contrib.y);
+                atomicAdd(&A[2][li], contrib.z);
+            };
+            if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
+                for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
+            else
+                for_each_node<false>(g, ti, bx, by, bz, fx, W, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                    V3 nrm;
+                    if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                        return;
+                    scatter(li, a, b, cc, dist, wt);
+                });
         }
         wave_sync();
         float4 *out = c.tilebuf + (size_t)t * kTileNodes;
@@ -627,29 +676,22 @@ namespace
         weights_of(fx, W);
         V3 vacc{0, 0, 0};
         M3 aacc{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                {
-                    const int nx = bx + a, ny = by + b, nz = bz + cc;
-                    if (!in_grid(g, nx, ny, nz))
-                        continue;
-                    const int li = tile_local(ti, nx, ny, nz);
-                    if (li < 0)
-                        continue;
-                    V3 nv = v3(G[li]);
-                    const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
-                    V3 n;
-                    if (nSlicers > 0 && different_sides<true>(c.sv, nSlicers, x, x + dist, n))
-                        nv = vp - n * dot(vp, n) * 2.0f;
-                    const float wt = W.w[a][0] * W.w[b][1] * W.w[cc][2];
-                    const V3 nw = nv * wt;
-                    vacc = vacc + nw;
-                    aacc = aacc + outer_ref(nw, dist);
-                }
+        auto gather = [&](V3 nv, V3 dist, float wt) {
+            const V3 nw = nv * wt;
+            vacc = vacc + nw;
+            aacc = aacc + outer_ref(nw, dist);
+        };
+        if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
+            for_each_node<true>(g, ti, bx, by, bz, fx, W,
+                                [&](int li, int, int, int, V3 dist, float wt) { gather(v3(G[li]), dist, wt); });
+        else
+            for_each_node<false>(g, ti, bx, by, bz, fx, W, [&](int li, int, int, int, V3 dist, float wt) {
+                V3 nv = v3(G[li]);
+                V3 n;
+                if (nSlicers > 0 && different_sides<true>(c.sv, nSlicers, x, x + dist, n))
+                    nv = vp - n * dot(vp, n) * 2.0f;
+                gather(nv, dist, wt);
+            });
         vOut = vacc;
         cOut = aacc * (4.0f * g.invdx);
     }
@@ -742,30 +784,23 @@ namespace
         cell_of(g, x, bx, by, bz, fx);
         Weights W;
         weights_of(fx, W);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                {
-                    const int nx = bx + a, ny = by + b, nz = bz + cc;
-                    if (!in_grid(g, nx, ny, nz))
-                        continue;
-                    const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
-                    V3 nrm;
-                    if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                        continue;
-                    const int li = tile_local(ti, nx, ny, nz);
-                    if (li < 0)
-                        continue;
-                    const float wm = (W.w[a][0] * W.w[b][1] * W.w[cc][2]) * m;
-                    const V3 contrib = (vel + mul(Cm, dist)) * wm;
-                    atomicAdd(&A[3][li], wm);
-                    atomicAdd(&A[0][li], contrib.x);
-                    atomicAdd(&A[1][li], contrib.y);
-                    atomicAdd(&A[2][li], contrib.z);
-                }
+        auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
+            const float wm = wt * m;
+            const V3 contrib = (vel + mul(Cm, dist)) * wm;
+            atomicAdd(&A[3][li], wm);
+            atomicAdd(&A[0][li], contrib.x);
+            atomicAdd(&A[1][li], contrib.y);
+            atomicAdd(&A[2][li], contrib.z);
+        };
+        if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
+            for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
+        else
+            for_each_node<false>(g, ti, bx, by, bz, fx, W, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                V3 nrm;
+                if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                    return;
+                scatter(li, a, b, cc, dist, wt);
+            });
     }
 
     // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
@@ -898,35 +933,26 @@ namespace
             const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_m3(c.S.F, c.S.stride, p));
             const V3 vel = v3(c.P.v[p]);
             const float nv0 = -c.S.V0[p];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
-                    {
-                        const int nx = bx + a, ny = by + b, nz = bz + cc;
-                        if (!in_grid(g, nx, ny, nz))
-                            continue;
-                        const int li = tile_local(ti, nx, ny, nz);
-                        if (li < 0)
-                            continue;
-                        const float wt = W.w[a][0] * W.w[b][1] * W.w[cc][2];
-                        const V3 wg{D.w[a][0] * W.w[b][1] * W.w[cc][2], W.w[a][0] * D.w[b][1] * W.w[cc][2],
-                                    W.w[a][0] * W.w[b][1] * D.w[cc][2]};
-                        const float wm = wt * m;
-                        const V3 ext = grav * wm;
-                        const V3 inr = mul(stress, wg) * nv0;
-                        const V3 mom = vel * wm;
-                        const V3 f = ext + inr;
-                        atomicAdd(&A[3][li], wm);
-                        atomicAdd(&A[0][li], mom.x);
-                        atomicAdd(&A[1][li], mom.y);
-                        atomicAdd(&A[2][li], mom.z);
-                        atomicAdd(&A[4][li], f.x);
-                        atomicAdd(&A[5][li], f.y);
-                        atomicAdd(&A[6][li], f.z);
-                    }
+            auto scatter = [&](int li, int a, int b, int cc, V3, float wt) {
+                const V3 wg{wsel(D, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2), wsel(W, a, 0) * wsel(D, b, 1) * wsel(W, cc, 2),
+                            wsel(W, a, 0) * wsel(W, b, 1) * wsel(D, cc, 2)};
+                const float wm = wt * m;
+                const V3 ext = grav * wm;
+                const V3 inr = mul(stress, wg) * nv0;
+                const V3 mom = vel * wm;
+                const V3 f = ext + inr;
+                atomicAdd(&A[3][li], wm);
+                atomicAdd(&A[0][li], mom.x);
+                atomicAdd(&A[1][li], mom.y);
+                atomicAdd(&A[2][li], mom.z);
+                atomicAdd(&A[4][li], f.x);
+                atomicAdd(&A[5][li], f.y);
+                atomicAdd(&A[6][li], f.z);
+            };
+            if (nodes_inside(g, bx, by, bz))
+                for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
+            else
+                for_each_node<false>(g, ti, bx, by, bz, fx, W, scatter);
         }
         wave_sync();
         float4 *out = c.tilebuf + (size_t)t * kTileNodes;
@@ -967,26 +993,17 @@ namespace
             bspline_dw(fx.z, D.w[0][2], D.w[1][2], D.w[2][2]);
             V3 vacc{0, 0, 0};
             M3 gv{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
-                    {
-                        const int nx = bx + a, ny = by + b, nz = bz + cc;
-                        if (!in_grid(g, nx, ny, nz))
-                            continue;
-                        const int li = tile_local(ti, nx, ny, nz);
-                        if (li < 0)
-                            continue;
-                        const float wt = W.w[a][0] * W.w[b][1] * W.w[cc][2];
-                        const V3 nv = v3(G[li]);
-                        vacc = vacc + nv * wt;
-                        const V3 wg{D.w[a][0] * W.w[b][1] * W.w[cc][2], W.w[a][0] * D.w[b][1] * W.w[cc][2],
-                                    W.w[a][0] * W.w[b][1] * D.w[cc][2]};
-                        gv = gv + outer_ref(wg, nv);
-                    }
+            auto gather = [&](int li, int a, int b, int cc, V3, float wt) {
+                const V3 nv = v3(G[li]);
+                vacc = vacc + nv * wt;
+                const V3 wg{wsel(D, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2), wsel(W, a, 0) * wsel(D, b, 1) * wsel(W, cc, 2),
+                            wsel(W, a, 0) * wsel(W, b, 1) * wsel(D, cc, 2)};
+                gv = gv + outer_ref(wg, nv);
+            };
+            if (nodes_inside(g, bx, by, bz))
+                for_each_node<true>(g, ti, bx, by, bz, fx, W, gather);
+            else
+                for_each_node<false>(g, ti, bx, by, bz, fx, W, gather);
             V3 x = clamp_bounds(g, v3(xm) + vacc * c.dt);
             float4 vv = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
             resolve_rigid<true>(c.sv, nShapes, vv, x);

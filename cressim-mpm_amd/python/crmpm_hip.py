@@ -85,6 +85,10 @@ def load_library(path=LIB_PATH):
         return _LIB
     if not os.path.exists(path):
         raise RuntimeError(f"libcrmpm_hip.so not found at {path}; run __graft_entry__.build()")
+    # torch first: its HIP runtime (soname libamdhip64.so.7) then satisfies the
+    # library's dependency, so one runtime serves both in this process.
+    import torch  # noqa: F401
+
     lib = ctypes.CDLL(path)
     for name, args in EXPORTS.items():
         fn = getattr(lib, name)

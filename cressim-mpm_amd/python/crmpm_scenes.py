@@ -152,20 +152,25 @@ def config1(n_side=32, steps=200, seed=20261203 + 1):
 
 
 def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, frame_substeps=10):
-    """Plain PB-MPM tissue slab: stratified nx x ny x nz particles, one per
-    (1/128, 0.0125, 1/128) m cell, i.e. the full 128 x 16 x 128 = 262,144-particle
-    slab of 1.0 x 0.2 x 1.0 m; smaller counts give a smaller slab at the same
-    spacing, centred in x/z.  Grid [0,1] x [0,0.5] x [0,1], dx = 1/128
-    (129 x 65 x 129 nodes).  Neo-Hookean with PB parameters (alpha, r) =
-    (0.5/dt, 0.9); a kinematic sphere indenter of radius 0.1 m (scaled with the
-    slab) starts 5 mm above the top and moves down at 0.05 m/s, friction 0.3;
-    dt = 2e-3, 20 iterations per substep."""
+    """Plain PB-MPM tissue slab: stratified nx x ny x nz particles, the full
+    128 x 16 x 128 = 262,144-particle slab 0.961 x 0.2 x 0.961 m, smaller
+    counts give a smaller slab at the same spacing, centred in x/z.
+    Grid [0,1] x [0,0.5] x [0,1], dx = 1/128 (129 x 65 x 129 nodes).
+    The slab keeps 2.5 dx from every lower grid face: a particle closer than
+    dx/2 to a lower face gets base cell trunc(s - 0.5) = 0 with fx < 0.5 and
+    weights that do not sum to one (src/solver/mls_mpm_algorithm.h:172-174),
+    and a slab filling the whole 1 m box blows up in the reference solver
+    itself within 3 substeps.  Neo-Hookean with PB parameters (alpha, r) =
+    (0.5/dt, 0.9); a kinematic sphere indenter of radius 0.1 m (scaled with
+    the slab) starts 5 mm above the top and moves down at 0.05 m/s, friction
+    0.3; dt = 2e-3, 20 iterations per substep."""
     rng = SplitMix64(seed)
     dx = 1.0 / 128
     dt = 2e-3
     N = nx * ny * nz
     sc = new_scene(PB, dx, (0, 0, 0), (1, 0.5, 1), N, dt, iterations=iterations)
-    cell = (1.0 / 128, 0.2 / 16, 1.0 / 128)
+    span = 1.0 - 5 * dx
+    cell = (span / 128, 0.2 / 16, span / 128)
     slab = (nx * cell[0], ny * cell[1], nz * cell[2])
     y0 = 2.5 * dx
     origin = (0.5 - slab[0] / 2, y0, 0.5 - slab[2] / 2)

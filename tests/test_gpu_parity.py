@@ -181,3 +181,16 @@ def test_masked_and_partial_active():
     inactive[n - 40:] = True
     assert np.array_equal(gs["xm"][inactive], sc["xm"][inactive].astype(np.float64))
     g.close()
+
+
+def test_cpp_solver_interface():
+    """The C++ MpmSolverHip interface drives a scene like the reference engine does."""
+    import os
+    import subprocess
+
+    exe = os.path.join(CC.ROOT, "tests", "cpp", "solver_api_test")
+    assert os.path.exists(exe), "build() compiles tests/cpp/solver_api_test"
+    print("cpp solver_api_test", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
