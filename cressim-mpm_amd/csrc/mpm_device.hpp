@@ -2,10 +2,11 @@
 //
 // Scalar FP32 helpers, column-major 3x3 matrices, quaternion rotation, the
 // 3x3 SVD, analytic/grid SDFs, rigid contact and slicer connectivity.  The
-// library is compiled with -ffp-contract=off and IEEE division/sqrt so the
-// per-particle arithmetic follows the reference's evaluation order term for
-// term; only summation order (tile reductions) differs from the reference
-// CPU path.  Reference semantics reproduced (file:line of the reference):
+// library is compiled with -ffp-contract=off and IEEE division/sqrt, so the
+// per-particle arithmetic follows the reference CPU path's evaluation order
+// term for term (rsqrtf is 1/sqrt there, include/utils/cuda_compat.h:55);
+// only the summation order of the tile reductions differs.
+// Reference semantics reproduced (file:line of the reference):
 //   Vec3::outerProduct(a, b)(r, c) = a_c * b_r     include/math/mat33.h:287-306
 //   Mat3x3f::inverse() = inverse^T, 0 if |det|<1e-6 include/math/mat33.h:256-283
 //   svd3 factors A^T (u * vt = R(A)^T)              include/math/svd3.h:1265-1275
@@ -787,7 +788,7 @@ namespace crhip
     // checkOnDifferentSides (src/solver/mpm_rigid_contact.h:401-478); the
     // normal stays in the slicer's local frame.
     template <bool ComputeNormal>
-    CR_DEV_NOINLINE bool different_sides(const ShapeView &sv, int nSlicers, V3 p1, V3 p2, V3 &normal)
+    CR_DEV bool different_sides(const ShapeView &sv, int nSlicers, V3 p1, V3 p2, V3 &normal)
     {
         for (int k = 0; k < nSlicers; ++k)
         {

@@ -451,9 +451,17 @@ namespace
         return lx + kTN * (ly + kTN * lz);
     }
 
-    CR_DEV float wsel(const Weights &W, int i, int axis)
+    // Weight i (0, 1, 2) of fraction f and its grid-unit gradient, evaluated
+    // directly (same expressions as bspline_w / bspline_dw) so that a rolled
+    // node loop needs no indexed weight table, which would live in scratch.
+    CR_DEV float w_of(float f, int i)
     {
-        return i == 0 ? W.w[0][axis] : (i == 1 ? W.w[1][axis] : W.w[2][axis]);
+        const float a = 1.5f - f, b = f - 1.0f, c = f - 0.5f;
+        return i == 0 ? 0.5f * (a * a) : (i == 1 ? 0.75f - (b * b) : 0.5f * (c * c));
+    }
+    CR_DEV float dw_of(float f, int i)
+    {
+        return i == 0 ? (float)((double)f - 1.5) : (i == 1 ? 2.0f * (1.0f - f) : f - 0.5f);
     }
 
     // All 27 nodes of base cell (bx, by, bz) inside the grid: the tile box then
@@ -502,10 +510,101 @@ namespace
                         if (li < 0)
                             continue;
                         const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
-                        const float wt = wsel(W, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2);
+                        const float wt = w_of(fx.x, a) * w_of(fx.y, b) * w_of(fx.z, cc);
                         fn(li, a, b, cc, dist, wt);
                     }
         }
+    }
+
+    // ------------------------------------------------------------------
+    // Conflict-free LDS scatter.  Two lanes can only hit the same node in one
+    // node-loop instruction if their particles share a base cell, so lanes are
+    // grouped by base cell and each pass lets one lane per cell scatter with a
+    // plain read-modify-write (ds_read_b128 + ds_write_b128 per node) instead
+    // of four ds_add_f32 atomics.  Lanes sharing a cell take turns in lane
+    // order, so the summation order is fixed and the result reproducible.
+    // ------------------------------------------------------------------
+    CR_DEV void compiler_fence() { asm volatile("" ::: "memory"); }
+
+    // Makes values opaque to the optimiser at this point (no instruction), so
+    // the per-node work of a scatter pass is not hoisted out of the pass loop.
+    CR_DEV void opaque(float &v) { asm volatile("" : "+v"(v)); }
+    CR_DEV void opaque(V3 &v)
+    {
+        opaque(v.x);
+        opaque(v.y);
+        opaque(v.z);
+    }
+    CR_DEV void opaque(M3 &m)
+    {
+        opaque(m.c0);
+        opaque(m.c1);
+        opaque(m.c2);
+    }
+    CR_DEV void opaque(Weights &W)
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                opaque(W.w[i][a]);
+    }
+
+    // Key of the base cell inside the tile box: 3 bits per axis of the local
+    // base offset, clamped to [-2, 5] (a particle whose offset is clamped has
+    // no node in the box on that side, so sharing a key only serialises it).
+    CR_DEV unsigned cell_key(const TileInfo &ti, int bx, int by, int bz)
+    {
+        const int lx = min(max(bx - ti.bx * kTB + 2, 0), 7);
+        const int ly = min(max(by - ti.by * kTB + 2, 0), 7);
+        const int lz = min(max(bz - ti.bz * kTB + 2, 0), 7);
+        return (unsigned)(lx | (ly << 3) | (lz << 6));
+    }
+
+    // Active lanes whose 9-bit key equals this lane's.
+    CR_DEV unsigned long long match_lanes(unsigned key)
+    {
+        unsigned long long peers = __ballot(1);
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+        {
+            const bool set = (key >> b) & 1u;
+            const unsigned long long bal = __ballot(set);
+            peers &= set ? bal : ~bal;
+        }
+        return peers;
+    }
+
+    // Runs body() on every active lane, in passes in which no two lanes share a key.
+    template <class Body>
+    CR_DEV void in_conflict_free_passes(unsigned key, Body &&body)
+    {
+        const unsigned long long peers = match_lanes(key);
+        const int lane = __lane_id();
+        bool pending = true;
+        for (;;)
+        {
+            const unsigned long long pend = __ballot(pending);
+            if (pend == 0ull)
+                break;
+            if (pending && lane == __ffsll((unsigned long long)(pend & peers)) - 1)
+            {
+                body();
+                pending = false;
+            }
+            compiler_fence();
+        }
+    }
+
+    CR_DEV void lds_add(float4 *A, int li, float x, float y, float z, float w)
+    {
+        float4 cur = A[li];
+        cur.x += x;
+        cur.y += y;
+        cur.z += z;
+        cur.w += w;
+        A[li] = cur;
+        compiler_fence();
     }
 
     // ------------------------------------------------------------------
@@ -530,15 +629,15 @@ namespace
 
     __global__ __launch_bounds__(kWG) void k_mls_p2g(StepCommon c)
     {
-        __shared__ float acc[kWaves][4][kTileNodes];
+        __shared__ float4 acc[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
         if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
             return;
-        float(*A)[kTileNodes] = acc[wave];
-        for (int i = lane; i < 4 * kTileNodes; i += 64)
-            (&A[0][0])[i] = 0.0f;
+        float4 *A = acc[wave];
+        for (int i = lane; i < kTileNodes; i += 64)
+            A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         wave_sync();
         const int nSlicers = c.sv.counts[1];
         const GridGeom &g = c.g;
@@ -563,27 +662,34 @@ namespace
             const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
             const M3 affine = stress * sc + Cm * m;
             const V3 vm = vel * m;
-            auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
-                const V3 contrib = (vm + mul(affine, dist)) * wt;
-                atomicAdd(&A[3][li], wt * m);
-                atomicAdd(&A[0][li], contrib.x);
-                atomicAdd(&A[1][li], contrib.y);
-                atomicAdd(&A[2][li], contrib.z);
-            };
-            if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
-                for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
-            else
-                for_each_node<false>(g, ti, bx, by, bz, fx, W, [&](int li, int a, int b, int cc, V3 dist, float wt) {
-                    V3 nrm;
-                    if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                        return;
-                    scatter(li, a, b, cc, dist, wt);
-                });
+            const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
+            in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                M3 af = affine;
+                V3 v_m = vm, f = fx;
+                Weights w = W;
+                opaque(af);
+                opaque(v_m);
+                opaque(f);
+                opaque(w);
+                auto sc = [&](int li, int, int, int, V3 dist, float wt) {
+                    const V3 contrib = (v_m + mul(af, dist)) * wt;
+                    lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
+                };
+                if (fast)
+                    for_each_node<true>(g, ti, bx, by, bz, f, w, sc);
+                else
+                    for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                        V3 nrm;
+                        if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                            return;
+                        sc(li, a, b, cc, dist, wt);
+                    });
+            });
         }
         wave_sync();
         float4 *out = c.tilebuf + (size_t)t * kTileNodes;
         for (int i = lane; i < kTileNodes; i += 64)
-            out[i] = make_float4(A[0][i], A[1][i], A[2][i], A[3][i]);
+            out[i] = A[i];
     }
 
     // ------------------------------------------------------------------
@@ -775,8 +881,8 @@ namespace
     }
 
     // Scatter of one PB particle into the LDS tile (pbMpmParticleToGrid loop)
-    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float (*A)[kTileNodes], int nSlicers, V3 x, float m,
-                           V3 vel, const M3 &Cm)
+    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A, int nSlicers, V3 x, float m, V3 vel,
+                           const M3 &Cm)
     {
         const GridGeom &g = c.g;
         int bx, by, bz;
@@ -784,43 +890,50 @@ namespace
         cell_of(g, x, bx, by, bz, fx);
         Weights W;
         weights_of(fx, W);
-        auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
-            const float wm = wt * m;
-            const V3 contrib = (vel + mul(Cm, dist)) * wm;
-            atomicAdd(&A[3][li], wm);
-            atomicAdd(&A[0][li], contrib.x);
-            atomicAdd(&A[1][li], contrib.y);
-            atomicAdd(&A[2][li], contrib.z);
-        };
-        if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
-            for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
-        else
-            for_each_node<false>(g, ti, bx, by, bz, fx, W, [&](int li, int a, int b, int cc, V3 dist, float wt) {
-                V3 nrm;
-                if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                    return;
-                scatter(li, a, b, cc, dist, wt);
-            });
+        const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
+        in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+            M3 cm = Cm;
+            V3 v = vel, f = fx;
+            Weights w = W;
+            opaque(cm);
+            opaque(v);
+            opaque(f);
+            opaque(w);
+            auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
+                const float wm = wt * m;
+                const V3 contrib = (v + mul(cm, dist)) * wm;
+                lds_add(A, li, contrib.x, contrib.y, contrib.z, wm);
+            };
+            if (fast)
+                for_each_node<true>(g, ti, bx, by, bz, f, w, scatter);
+            else
+                for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                    V3 nrm;
+                    if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                        return;
+                    scatter(li, a, b, cc, dist, wt);
+                });
+        });
     }
 
     // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
     // iteration fused with P2G of this one.  Mode 2: last G2P fused with
     // pbMpmIntegrateParticle, results written back in particle order.
     template <int Mode>
-    __global__ __launch_bounds__(kWG) void k_pb_tile(StepCommon c, PbStage st, float3 gravity)
+    __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(StepCommon c, PbStage st, float3 gravity)
     {
-        __shared__ float acc[(Mode < 2) ? kWaves : 1][4][kTileNodes];
+        __shared__ float4 acc[(Mode < 2) ? kWaves : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
         if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
             return;
-        float(*A)[kTileNodes] = acc[(Mode < 2) ? wave : 0];
+        float4 *A = acc[(Mode < 2) ? wave : 0];
         float4 *G = lg[(Mode > 0) ? wave : 0];
         if (Mode < 2)
-            for (int i = lane; i < 4 * kTileNodes; i += 64)
-                (&A[0][0])[i] = 0.0f;
+            for (int i = lane; i < kTileNodes; i += 64)
+                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (Mode > 0)
             load_tile_grid(c.g, ti, c.grid, G, lane);
         wave_sync();
@@ -874,7 +987,7 @@ namespace
             wave_sync();
             float4 *out = c.tilebuf + (size_t)t * kTileNodes;
             for (int i = lane; i < kTileNodes; i += 64)
-                out[i] = make_float4(A[0][i], A[1][i], A[2][i], A[3][i]);
+                out[i] = A[i];
         }
     }
 
@@ -904,15 +1017,19 @@ namespace
     // ------------------------------------------------------------------
     __global__ __launch_bounds__(kWG) void k_std_p2g(StepCommon c, float4 *tileForce)
     {
-        __shared__ float acc[kWaves][7][kTileNodes];
+        __shared__ float4 acc[kWaves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
         if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
             return;
-        float(*A)[kTileNodes] = acc[wave];
-        for (int i = lane; i < 7 * kTileNodes; i += 64)
-            (&A[0][0])[i] = 0.0f;
+        float4 *A = acc[wave][0];
+        float4 *AF = acc[wave][1];
+        for (int i = lane; i < kTileNodes; i += 64)
+        {
+            A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            AF[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         wave_sync();
         const GridGeom &g = c.g;
         const V3 grav = v3(c.gravity.x, c.gravity.y, c.gravity.z);
@@ -924,43 +1041,45 @@ namespace
             int bx, by, bz;
             V3 fx;
             cell_of(g, v3(xm), bx, by, bz, fx);
-            Weights W, D;
+            Weights W;
             weights_of(fx, W);
-            bspline_dw(fx.x, D.w[0][0], D.w[1][0], D.w[2][0]);
-            bspline_dw(fx.y, D.w[0][1], D.w[1][1], D.w[2][1]);
-            bspline_dw(fx.z, D.w[0][2], D.w[1][2], D.w[2][2]);
             const float4 prm = c.P.params[p];
             const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_m3(c.S.F, c.S.stride, p));
             const V3 vel = v3(c.P.v[p]);
             const float nv0 = -c.S.V0[p];
-            auto scatter = [&](int li, int a, int b, int cc, V3, float wt) {
-                const V3 wg{wsel(D, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2), wsel(W, a, 0) * wsel(D, b, 1) * wsel(W, cc, 2),
-                            wsel(W, a, 0) * wsel(W, b, 1) * wsel(D, cc, 2)};
-                const float wm = wt * m;
-                const V3 ext = grav * wm;
-                const V3 inr = mul(stress, wg) * nv0;
-                const V3 mom = vel * wm;
-                const V3 f = ext + inr;
-                atomicAdd(&A[3][li], wm);
-                atomicAdd(&A[0][li], mom.x);
-                atomicAdd(&A[1][li], mom.y);
-                atomicAdd(&A[2][li], mom.z);
-                atomicAdd(&A[4][li], f.x);
-                atomicAdd(&A[5][li], f.y);
-                atomicAdd(&A[6][li], f.z);
-            };
-            if (nodes_inside(g, bx, by, bz))
-                for_each_node<true>(g, ti, bx, by, bz, fx, W, scatter);
-            else
-                for_each_node<false>(g, ti, bx, by, bz, fx, W, scatter);
+            const bool fast = nodes_inside(g, bx, by, bz);
+            in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                M3 st = stress;
+                V3 v = vel, f = fx;
+                Weights w = W;
+                opaque(st);
+                opaque(v);
+                opaque(f);
+                opaque(w);
+                auto scatter = [&](int li, int a, int b, int cc, V3, float wt) {
+                    const float wa = w_of(f.x, a), wb = w_of(f.y, b), wc = w_of(f.z, cc);
+                    const V3 wg{dw_of(f.x, a) * wb * wc, wa * dw_of(f.y, b) * wc, wa * wb * dw_of(f.z, cc)};
+                    const float wm = wt * m;
+                    const V3 ext = grav * wm;
+                    const V3 inr = mul(st, wg) * nv0;
+                    const V3 mom = v * wm;
+                    const V3 fo = ext + inr;
+                    lds_add(A, li, mom.x, mom.y, mom.z, wm);
+                    lds_add(AF, li, fo.x, fo.y, fo.z, 0.0f);
+                };
+                if (fast)
+                    for_each_node<true>(g, ti, bx, by, bz, f, w, scatter);
+                else
+                    for_each_node<false>(g, ti, bx, by, bz, f, w, scatter);
+            });
         }
         wave_sync();
         float4 *out = c.tilebuf + (size_t)t * kTileNodes;
         float4 *outF = tileForce + (size_t)t * kTileNodes;
         for (int i = lane; i < kTileNodes; i += 64)
         {
-            out[i] = make_float4(A[0][i], A[1][i], A[2][i], A[3][i]);
-            outF[i] = make_float4(A[4][i], A[5][i], A[6][i], 0.0f);
+            out[i] = A[i];
+            outF[i] = AF[i];
         }
     }
 
@@ -986,18 +1105,15 @@ namespace
             int bx, by, bz;
             V3 fx;
             cell_of(g, v3(xm), bx, by, bz, fx);
-            Weights W, D;
+            Weights W;
             weights_of(fx, W);
-            bspline_dw(fx.x, D.w[0][0], D.w[1][0], D.w[2][0]);
-            bspline_dw(fx.y, D.w[0][1], D.w[1][1], D.w[2][1]);
-            bspline_dw(fx.z, D.w[0][2], D.w[1][2], D.w[2][2]);
             V3 vacc{0, 0, 0};
             M3 gv{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
             auto gather = [&](int li, int a, int b, int cc, V3, float wt) {
                 const V3 nv = v3(G[li]);
                 vacc = vacc + nv * wt;
-                const V3 wg{wsel(D, a, 0) * wsel(W, b, 1) * wsel(W, cc, 2), wsel(W, a, 0) * wsel(D, b, 1) * wsel(W, cc, 2),
-                            wsel(W, a, 0) * wsel(W, b, 1) * wsel(D, cc, 2)};
+                const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, cc);
+                const V3 wg{dw_of(fx.x, a) * wb * wc, wa * dw_of(fx.y, b) * wc, wa * wb * dw_of(fx.z, cc)};
                 gv = gv + outer_ref(wg, nv);
             };
             if (nodes_inside(g, bx, by, bz))
