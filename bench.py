@@ -36,6 +36,85 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_FUSED = 164
 BYTES_PER_TOUCHED_NODE_FUSED = 32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
+FUSED_KERNEL = "k_pb_tile<1>"
+COPY_CAL_BYTES = 256 << 20
+
+
+def copy_calibration():
+    """Three device-to-device copies of 256 MiB: the known-size kernel that
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+
+    x = torch.ones(COPY_CAL_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copy calibration: 3 copies of {COPY_CAL_BYTES} bytes", flush=True)
+
+
+def _short_name(name):
+    import re
+
+    m = re.search(r"(k_\w+(?:<\d+>)?|__amd_rocclr_\w+)", name)
+    return m.group(1) if m else name.split("(")[0][-60:]
+
+
+def _read_csv(directory, suffix):
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _counter_per_launch(directory):
+    # kernel -> mean counter value (KiB) per dispatch
+    acc = {}
+    for r in _read_csv(directory, "counter_collection.csv"):
+        k = _short_name(r["Kernel_Name"])
+        acc.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}, acc
+
+
+def summarize(out_dir, title, commands):
+    """Turn the rocprofv3 outputs of one profiling session into the committed
+    summary: kernel-trace statistics, FETCH_SIZE / WRITE_SIZE per dispatch of
+    each kernel, and the copy-kernel calibration that scales them to bytes."""
+    stats = _read_csv(os.path.join(out_dir, "trace"), "kernel_stats.csv")
+    fetch, _ = _counter_per_launch(os.path.join(out_dir, "fetch"))
+    write, _ = _counter_per_launch(os.path.join(out_dir, "write"))
+    _, fetch_cp = _counter_per_launch(os.path.join(out_dir, "fetch_copy"))
+    _, write_cp = _counter_per_launch(os.path.join(out_dir, "write_copy"))
+    kib = COPY_CAL_BYTES / 1024.0
+    f_fac = kib / max(fetch_cp["__amd_rocclr_copyBuffer"])
+    w_fac = kib / max(write_cp["__amd_rocclr_copyBuffer"])
+    lines = [f"# {title}", "", "Commands (repository root, rocprofv3 on one MI355X):", "", "```"]
+    lines += commands + ["```", "",
+                         "| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM MB/dispatch (calibrated) |",
+                         "|---|---|---|---|---|---|---|"]
+    merged = {}
+    for r in stats:
+        k = _short_name(r["Name"])
+        c, t = merged.get(k, (0, 0.0))
+        merged[k] = (c + int(r["Calls"]), t + float(r["TotalDurationNs"]))
+    total = sum(t for _, t in merged.values())
+    for k, (c, t) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
+        fr, wr = fetch.get(k), write.get(k)
+        mb = "" if fr is None or wr is None else f"{(fr * f_fac + wr * w_fac) * 1024 / 1e6:.2f}"
+        lines.append(f"| `{k}` | {c} | {t / c / 1e3:.1f} | {100 * t / total:.2f} | "
+                     f"{'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} |")
+    lines += ["", f"Calibration: a device-to-device copy of {COPY_CAL_BYTES >> 20} MiB reads FETCH_SIZE "
+              f"{max(fetch_cp['__amd_rocclr_copyBuffer']):,.0f} KiB and WRITE_SIZE {max(write_cp['__amd_rocclr_copyBuffer']):,.0f} KiB "
+              f"per dispatch, so bytes = FETCH_SIZE x {f_fac:.3f} KiB + WRITE_SIZE x {w_fac:.3f} KiB."]
+    result = {"kernel": FUSED_KERNEL, "fetch_kib_raw": fetch.get(FUSED_KERNEL), "write_kib_raw": write.get(FUSED_KERNEL),
+              "fetch_factor": f_fac, "write_factor": w_fac}
+    if result["fetch_kib_raw"] is not None and result["write_kib_raw"] is not None:
+        result["bytes_per_launch"] = 1024.0 * (result["fetch_kib_raw"] * f_fac + result["write_kib_raw"] * w_fac)
+    return "\n".join(lines) + "\n", result
 
 
 def cpu_baseline(scene, substeps):
@@ -70,7 +149,28 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
+    ap.add_argument("--summarize", metavar="DIR",
+                    help="write the profile summary from rocprofv3 outputs under DIR/{trace,fetch,write,fetch_copy,write_copy}")
     args = ap.parse_args()
+    if args.summarize:
+        cmds = ["rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o pb -- python3 bench.py --steps 30 --warmup 5 --cpu-substeps 0"]
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = c.split("_")[0].lower()
+            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o pb -- python3 bench.py --steps 3 --warmup 1 --cpu-substeps 0")
+            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- python3 bench.py --copy-calibration")
+        cmds.append(f"python3 bench.py --summarize DIR")
+        md, traffic = summarize(args.summarize, "Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations", cmds)
+        with open(os.path.join(ROOT, "profiles", "pb_slab262k_kernel_trace.md"), "w") as f:
+            f.write(md)
+        with open(TRAFFIC_FILE, "w") as f:
+            json.dump(traffic, f, indent=1)
+        print(md)
+        return
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: "
@@ -107,7 +207,7 @@ def main():
     grid = solver.grid()
     touched = int(np.count_nonzero(grid[:, 3] > 0))
 
-    solver.profile_begin(max_launches=K * (2 * scene["iterations"] + 8) + 64)
+    solver.profile_begin(max_launches=K * (2 * scene["iterations"] + 16) + 64)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     run(W, K)

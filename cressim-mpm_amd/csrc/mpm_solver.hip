@@ -19,6 +19,7 @@
 //                 -> G2P (one wavefront per tile, 6^3 nodes staged in LDS).
 // P2G never issues global atomics and every sum has a fixed order, so a substep is bitwise
 // reproducible run to run.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -147,15 +148,74 @@ namespace
     // Per-step bookkeeping
     // ------------------------------------------------------------------
 
-    // Zero the grid blocks written since the last step, reset the block maps
-    // and the tile / active-block counters.  One wavefront per block.
-    __global__ __launch_bounds__(kWG) void k_prepare(GridGeom g, float4 *grid, float4 *gridForce, unsigned char *dirty,
-                                                     int *tileOf, int *activeFlag, int *counters)
+    // Copy one bound shape into a compact record (shapeIds order).
+    CR_DEV void cache_shape(int s, const int *shapeIds, const CrHipShapeData &sd, const CrHipGeometryData &gd, ShapeRec *recs)
     {
-        const int b = blockIdx.x * kWaves + (threadIdx.x >> 6);
+        const int id = shapeIds[s];
+        const int gi = sd.geometryIdx[id];
+        ShapeRec r;
+        r.type = sd.type[id];
+        r.geomType = gd.type[gi];
+        r.shapeId = id;
+        r.pad = 0;
+        const float4 *f4 = nullptr;
+        r.pos = reinterpret_cast<const float4 *>(sd.position)[id];
+        r.rot = reinterpret_cast<const float4 *>(sd.rotation)[id];
+        r.invScale = reinterpret_cast<const float4 *>(sd.invScale)[id];
+        r.linVel = reinterpret_cast<const float4 *>(sd.linearVelocity)[id];
+        r.angVel = reinterpret_cast<const float4 *>(sd.angularVelocity)[id];
+        r.params0 = reinterpret_cast<const float4 *>(sd.params0)[id];
+        r.gparams = reinterpret_cast<const float4 *>(gd.params0)[gi];
+        f4 = reinterpret_cast<const float4 *>(sd.comInvMass);
+        r.comInvMass = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
+        f4 = reinterpret_cast<const float4 *>(sd.inertiaInv0);
+        r.inertia0 = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
+        f4 = reinterpret_cast<const float4 *>(sd.inertiaInv1);
+        r.inertia1 = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
+        recs[s] = r;
+    }
+
+    struct ShapeTables
+    {
+        int numShapes;
+        const int *shapeIds;
+        CrHipShapeData sd;
+        CrHipGeometryData gd;
+        ShapeRec *recs;
+        int *counts;
+        int *slicers;
+    };
+
+    // Start of a step.  Workgroups [0, blockWGs): zero the grid blocks written
+    // since the last step and mark every pblock empty (one wavefront per
+    // block).  The workgroups after them copy the bound shape tables into a
+    // compact record array in shapeIds order; the first of those also lists
+    // the slicers (shapes whose geometry is a slicer).
+    __global__ __launch_bounds__(kWG) void k_prepare(GridGeom g, int blockWGs, float4 *grid, float4 *gridForce,
+                                                     unsigned char *dirty, int *segStart, int *segEnd, ShapeTables st)
+    {
         const int lane = threadIdx.x & 63;
-        if (blockIdx.x == 0 && threadIdx.x < 4)
-            counters[threadIdx.x] = 0;
+        if ((int)blockIdx.x >= blockWGs)
+        {
+            const int sw = blockIdx.x - blockWGs;
+            const int s = sw * kWG + threadIdx.x;
+            if (s < st.numShapes)
+                cache_shape(s, st.shapeIds, st.sd, st.gd, st.recs);
+            if (sw == 0 && threadIdx.x == 0)
+            {
+                int ns = 0;
+                for (int k = 0; k < st.numShapes; ++k)
+                {
+                    const int gt = st.gd.type[st.sd.geometryIdx[st.shapeIds[k]]];
+                    if (gt == gQuadSlicer || gt == gTriangleMeshSlicer)
+                        st.slicers[ns++] = k;
+                }
+                st.counts[0] = st.numShapes;
+                st.counts[1] = ns;
+            }
+            return;
+        }
+        const int b = blockIdx.x * kWaves + (threadIdx.x >> 6);
         if (b >= g.numBlocks)
             return;
         if (dirty[b])
@@ -168,74 +228,9 @@ namespace
         if (lane == 0)
         {
             dirty[b] = 0;
-            tileOf[b] = -1;
-            activeFlag[b] = 0;
+            segStart[b] = 0;
+            segEnd[b] = 0;
         }
-    }
-
-    // Copy the bound shape tables into a compact record array in shapeIds
-    // order and list the slicers (shapes whose geometry is a slicer).
-    __global__ void k_shape_cache(int numShapes, const int *shapeIds, CrHipShapeData sd, CrHipGeometryData gd,
-                                  ShapeRec *recs, int *counts, int *slicers)
-    {
-        const int s = threadIdx.x + blockIdx.x * blockDim.x;
-        if (s < numShapes)
-        {
-            const int id = shapeIds[s];
-            const int gi = sd.geometryIdx[id];
-            ShapeRec r;
-            r.type = sd.type[id];
-            r.geomType = gd.type[gi];
-            r.shapeId = id;
-            r.pad = 0;
-            const float4 *f4 = nullptr;
-            r.pos = reinterpret_cast<const float4 *>(sd.position)[id];
-            r.rot = reinterpret_cast<const float4 *>(sd.rotation)[id];
-            r.invScale = reinterpret_cast<const float4 *>(sd.invScale)[id];
-            r.linVel = reinterpret_cast<const float4 *>(sd.linearVelocity)[id];
-            r.angVel = reinterpret_cast<const float4 *>(sd.angularVelocity)[id];
-            r.params0 = reinterpret_cast<const float4 *>(sd.params0)[id];
-            r.gparams = reinterpret_cast<const float4 *>(gd.params0)[gi];
-            f4 = reinterpret_cast<const float4 *>(sd.comInvMass);
-            r.comInvMass = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
-            f4 = reinterpret_cast<const float4 *>(sd.inertiaInv0);
-            r.inertia0 = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
-            f4 = reinterpret_cast<const float4 *>(sd.inertiaInv1);
-            r.inertia1 = f4 ? f4[id] : make_float4(0.f, 0.f, 0.f, 0.f);
-            recs[s] = r;
-        }
-        __syncthreads();
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-        {
-            int ns = 0;
-            for (int k = 0; k < numShapes; ++k)
-            {
-                const int gt = gd.type[sd.geometryIdx[shapeIds[k]]];
-                if (gt == gQuadSlicer || gt == gTriangleMeshSlicer)
-                    slicers[ns++] = k;
-            }
-            counts[0] = numShapes;
-            counts[1] = ns;
-        }
-    }
-
-    // Key per particle: its pblock, or the sentinel numBlocks when it does not take part.
-    __global__ void k_bin(GridGeom g, int n, ParticleBind P, int useMask, unsigned *keys, unsigned *vals)
-    {
-        const int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= n)
-            return;
-        unsigned key = (unsigned)g.numBlocks;
-        if (!useMask || P.mask[i])
-        {
-            const float4 x = P.xm[i];
-            int bx, by, bz;
-            V3 fx;
-            cell_of(g, v3(x), bx, by, bz, fx);
-            key = (unsigned)pblock_key(g, bx, by, bz);
-        }
-        keys[i] = key;
-        vals[i] = (unsigned)i;
     }
 
     // ---------------- stable LSD radix sort, 8-bit digits ----------------
@@ -284,6 +279,40 @@ namespace
             const int idx = base + r * kWG + threadIdx.x;
             if (idx < n)
                 atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        counts[threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
+    }
+
+    // Key per particle (its pblock, or the sentinel numBlocks when it does
+    // not take part), the identity permutation, and the histogram of the
+    // first radix digit: k_radix_hist fused with the binning pass.
+    __global__ __launch_bounds__(kWG) void k_bin_hist(GridGeom g, int n, ParticleBind P, int useMask, unsigned *keys,
+                                                      unsigned *vals, unsigned *counts, int numTiles)
+    {
+        __shared__ unsigned h[256];
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const int base = blockIdx.x * kRadixTile;
+#pragma unroll
+        for (int r = 0; r < kRadixItems; ++r)
+        {
+            const int i = base + r * kWG + threadIdx.x;
+            if (i < n)
+            {
+                unsigned key = (unsigned)g.numBlocks;
+                if (!useMask || P.mask[i])
+                {
+                    const float4 x = P.xm[i];
+                    int bx, by, bz;
+                    V3 fx;
+                    cell_of(g, v3(x), bx, by, bz, fx);
+                    key = (unsigned)pblock_key(g, bx, by, bz);
+                }
+                keys[i] = key;
+                vals[i] = (unsigned)i;
+                atomicAdd(&h[key & 255u], 1u);
+            }
         }
         __syncthreads();
         counts[threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
@@ -371,9 +400,10 @@ namespace
         }
     }
 
-    // Segment boundaries of the sorted keys -> tiles.
-    __global__ void k_segments(int n, int numBlocks, const unsigned *keys, int *segStart, int *segEnd, int *tileKey,
-                               int *tileOf, int *counters)
+    // Segment boundaries of the sorted keys: pblock k holds the sorted
+    // particles [segStart[k], segEnd[k]); both stay 0 for an empty pblock.
+    // The tile of pblock k is tile k, so no tile list is built.
+    __global__ void k_segments(int n, int numBlocks, const unsigned *keys, int *segStart, int *segEnd)
     {
         const int k = blockIdx.x * blockDim.x + threadIdx.x;
         if (k >= n)
@@ -382,34 +412,9 @@ namespace
         if (key >= (unsigned)numBlocks)
             return;
         if (k == 0 || keys[k - 1] != key)
-        {
             segStart[key] = k;
-            const int t = atomicAdd(&counters[0], 1);
-            tileKey[t] = (int)key;
-            tileOf[key] = t;
-        }
         if (k == n - 1 || keys[k + 1] != key)
             segEnd[key] = k + 1;
-    }
-
-    // Node blocks touched by tile t: its own block and the +1 neighbours.
-    __global__ void k_mark_active(GridGeom g, const int *tileKey, int *activeFlag, int *activeList, int *counters)
-    {
-        const int t = blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= counters[0])
-            return;
-        int bx, by, bz;
-        decode_block(g, tileKey[t], bx, by, bz);
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
-        {
-            const int x = bx + (d & 1), y = by + ((d >> 1) & 1), z = bz + (d >> 2);
-            if (x >= g.nbx || y >= g.nby || z >= g.nbz)
-                continue;
-            const int nb = block_key(g, x, y, z);
-            if (atomicExch(&activeFlag[nb], 1) == 0)
-                activeList[atomicAdd(&counters[1], 1)] = nb;
-        }
     }
 
     // ------------------------------------------------------------------
@@ -419,15 +424,16 @@ namespace
     {
         int bx, by, bz, start, end;
     };
-    CR_DEV bool tile_info(const GridGeom &g, int t, const int *counters, const int *tileKey, const int *segStart,
-                          const int *segEnd, TileInfo &ti)
+    // Tile t is pblock t; false when it holds no particle.
+    CR_DEV bool tile_info(const GridGeom &g, int t, const int *segStart, const int *segEnd, TileInfo &ti)
     {
-        if (t >= counters[0])
+        if (t >= g.numBlocks)
             return false;
-        const int key = tileKey[t];
-        decode_block(g, key, ti.bx, ti.by, ti.bz);
-        ti.start = segStart[key];
-        ti.end = segEnd[key];
+        ti.start = segStart[t];
+        ti.end = segEnd[t];
+        if (ti.end <= ti.start)
+            return false;
+        decode_block(g, t, ti.bx, ti.by, ti.bz);
         return true;
     }
 
@@ -618,8 +624,6 @@ namespace
         ParticleBind P;
         SolverState S;
         ShapeView sv;
-        const int *counters;
-        const int *tileKey;
         const int *segStart;
         const int *segEnd;
         const unsigned *order; // sorted particle indices
@@ -633,7 +637,7 @@ namespace
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
+        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
             return;
         float4 *A = acc[wave];
         for (int i = lane; i < kTileNodes; i += 64)
@@ -698,45 +702,57 @@ namespace
     // (src/solver/mls_mpm_algorithm.h:243-294, pb 233-284, standard 217-272).
     // ------------------------------------------------------------------
     template <int Kind> // 0 MLS, 1 PB, 2 standard
-    __global__ __launch_bounds__(kWG) void k_update(StepCommon c, const int *activeList, const int *tileOf,
-                                                    const float4 *tileForce, unsigned char *dirty)
+    __global__ __launch_bounds__(kWG) void k_update(StepCommon c, const float4 *tileForce, unsigned char *dirty)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int a = blockIdx.x * kWaves + wave;
-        if (a >= c.counters[1])
-            return;
+        const int nb = blockIdx.x * kWaves + wave;
         const GridGeom &g = c.g;
-        const int nb = activeList[a];
+        if (nb >= g.numBlocks)
+            return;
         int bx, by, bz;
         decode_block(g, nb, bx, by, bz);
+        // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz).
+        // All occupancy loads are issued before any partial is read.
+        bool src[8];
+        bool any = false;
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+        {
+            const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
+            const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
+            src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
+            any = any || src[d];
+        }
+        if (!any)
+            return;
         const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+        float4 q[8], qf[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+        {
+            const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+            const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
+            const bool use = src[d] && ux < kTN && uy < kTN && uz < kTN;
+            const size_t at = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
+            q[d] = use ? c.tilebuf[at] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (Kind == 2)
+                qf[d] = use ? tileForce[at] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        // fixed summation order d = 0..7 (bitwise reproducible)
         float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
         float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int d = 0; d < 8; ++d)
         {
-            const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
-            const int px = bx - dx, py = by - dy, pz = bz - dz;
-            if (px < 0 || py < 0 || pz < 0)
-                continue;
-            const int t = tileOf[block_key(g, px, py, pz)];
-            if (t < 0)
-                continue;
-            const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
-            if (ux >= kTN || uy >= kTN || uz >= kTN)
-                continue;
-            const int li = ux + kTN * (uy + kTN * uz);
-            const float4 q = c.tilebuf[(size_t)t * kTileNodes + li];
-            s.x += q.x;
-            s.y += q.y;
-            s.z += q.z;
-            s.w += q.w;
+            s.x += q[d].x;
+            s.y += q[d].y;
+            s.z += q[d].z;
+            s.w += q[d].w;
             if (Kind == 2)
             {
-                const float4 qf = tileForce[(size_t)t * kTileNodes + li];
-                f.x += qf.x;
-                f.y += qf.y;
-                f.z += qf.z;
+                f.x += qf[d].x;
+                f.y += qf[d].y;
+                f.z += qf[d].z;
             }
         }
         if (lane == 0)
@@ -815,7 +831,7 @@ namespace
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
+        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
             return;
         float4 *G = lg[wave];
         load_tile_grid(c.g, ti, c.grid, G, lane);
@@ -927,7 +943,7 @@ namespace
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
+        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
             return;
         float4 *A = acc[(Mode < 2) ? wave : 0];
         float4 *G = lg[(Mode > 0) ? wave : 0];
@@ -1021,7 +1037,7 @@ namespace
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
+        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
             return;
         float4 *A = acc[wave][0];
         float4 *AF = acc[wave][1];
@@ -1089,7 +1105,7 @@ namespace
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         TileInfo ti;
         const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.counters, c.tileKey, c.segStart, c.segEnd, ti))
+        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
             return;
         float4 *G = lg[wave];
         load_tile_grid(c.g, ti, c.grid, G, lane);
@@ -1230,9 +1246,7 @@ struct CrHipSolver
     float *F = nullptr, *C = nullptr, *V0 = nullptr;
     float4 *grid = nullptr, *gridForce = nullptr;
     unsigned char *dirty = nullptr;
-    int *tileOf = nullptr, *activeFlag = nullptr, *activeList = nullptr;
-    int *segStart = nullptr, *segEnd = nullptr, *tileKey = nullptr;
-    int *counters = nullptr;
+    int *segStart = nullptr, *segEnd = nullptr;
     unsigned *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
     unsigned *radixCounts = nullptr, *digitTotals = nullptr;
     float4 *tilebuf = nullptr, *tileForce = nullptr;
@@ -1304,13 +1318,28 @@ namespace
         return v;
     }
 
+    // Kernel launch on the solver stream.  While profiling, the dispatch itself
+    // records a start and a stop event (hipExtLaunchKernelGGL), so the timing
+    // covers the kernel's execution only.
+    template <typename... Params, typename... Args>
+    void launch(CrHipSolver *s, int cat, void (*kernel)(Params...), dim3 grid, dim3 block, Args... args)
+    {
+        if (s->profEvents && s->profUsed < s->profMax)
+        {
+            const int slot = s->profUsed++;
+            s->profCat[slot] = cat;
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, s->stream, s->profEvents[2 * slot], s->profEvents[2 * slot + 1], 0,
+                                  static_cast<Params>(args)...);
+        }
+        else
+            hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, static_cast<Params>(args)...);
+    }
+
     // Keys per particle and a stable LSD radix sort of (key, index).  Returns
     // the sorted keys and particle order (one of the two ping-pong buffers).
     int sort_particles(CrHipSolver *s, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
     {
-        hipStream_t st = s->stream;
         const GridGeom &g = s->g;
-        k_bin<<<grid1d(n, 256), 256, 0, st>>>(g, n, s->P, useMask, s->keysA, s->valsA);
         int bits = 0;
         while ((1ll << bits) <= (long long)g.numBlocks)
             ++bits;
@@ -1320,9 +1349,14 @@ namespace
         for (int pass = 0; pass < passes; ++pass)
         {
             const int shift = 8 * pass;
-            k_radix_hist<<<numTiles, kWG, 0, st>>>(kin, n, shift, s->radixCounts, numTiles);
-            k_radix_scan<<<256, kWG, 0, st>>>(s->radixCounts, numTiles, s->digitTotals);
-            k_radix_scatter<<<numTiles, kWG, 0, st>>>(kin, vin, n, shift, s->radixCounts, numTiles, s->digitTotals, kout, vout);
+            if (pass == 0)
+                launch(s, CRMPM_HIP_PROF_BINNING, k_bin_hist, numTiles, kWG, g, n, s->P, useMask, kin, vin, s->radixCounts,
+                       numTiles);
+            else
+                launch(s, CRMPM_HIP_PROF_BINNING, k_radix_hist, numTiles, kWG, kin, n, shift, s->radixCounts, numTiles);
+            launch(s, CRMPM_HIP_PROF_BINNING, k_radix_scan, 256, kWG, s->radixCounts, numTiles, s->digitTotals);
+            launch(s, CRMPM_HIP_PROF_BINNING, k_radix_scatter, numTiles, kWG, kin, vin, n, shift, s->radixCounts, numTiles,
+                   s->digitTotals, kout, vout);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
@@ -1331,41 +1365,16 @@ namespace
         return check_launch();
     }
 
-    // Bin, sort and tile the particles taking part in this step.
+    // Bin, sort and segment the particles taking part in this step.
     int build_tiles(CrHipSolver *s, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
     {
         int rc = sort_particles(s, n, useMask, orderOut, keysOut);
         if (rc)
             return rc;
-        hipStream_t st = s->stream;
-        const GridGeom &g = s->g;
-        k_segments<<<grid1d(n, 256), 256, 0, st>>>(n, g.numBlocks, *keysOut, s->segStart, s->segEnd, s->tileKey, s->tileOf,
-                                                   s->counters);
-        const long long maxT = std::min<long long>(s->maxTiles, n);
-        k_mark_active<<<grid1d(maxT, 256), 256, 0, st>>>(g, s->tileKey, s->activeFlag, s->activeList, s->counters);
+        launch(s, CRMPM_HIP_PROF_BINNING, k_segments, grid1d(n, 256), 256, n, s->g.numBlocks, *keysOut, s->segStart,
+               s->segEnd);
         return check_launch();
     }
-
-    // Event pair around one launch when profiling is on.
-    struct ProfScope
-    {
-        CrHipSolver *s;
-        int slot = -1;
-        ProfScope(CrHipSolver *sv, int cat) : s(sv)
-        {
-            if (s->profEvents && s->profUsed < s->profMax)
-            {
-                slot = s->profUsed++;
-                s->profCat[slot] = cat;
-                (void)hipEventRecord(s->profEvents[2 * slot], s->stream);
-            }
-        }
-        ~ProfScope()
-        {
-            if (slot >= 0)
-                (void)hipEventRecord(s->profEvents[2 * slot + 1], s->stream);
-        }
-    };
 
     StepCommon common(CrHipSolver *s, const unsigned *order)
     {
@@ -1376,8 +1385,6 @@ namespace
         c.P = s->P;
         c.S = SolverState{s->F, s->C, s->V0, s->maxParticles};
         c.sv = shape_view(s);
-        c.counters = s->counters;
-        c.tileKey = s->tileKey;
         c.segStart = s->segStart;
         c.segEnd = s->segEnd;
         c.order = order;
@@ -1438,13 +1445,8 @@ int crHipSolverDestroy(CrHipSolver *s)
     dfree(s->grid);
     dfree(s->gridForce);
     dfree(s->dirty);
-    dfree(s->tileOf);
-    dfree(s->activeFlag);
-    dfree(s->activeList);
     dfree(s->segStart);
     dfree(s->segEnd);
-    dfree(s->tileKey);
-    dfree(s->counters);
     dfree(s->keysA);
     dfree(s->keysB);
     dfree(s->valsA);
@@ -1594,7 +1596,8 @@ int crHipSolverInitialize(CrHipSolver *s)
     const size_t N = (size_t)std::max(s->maxParticles, 1);
     const GridGeom &g = s->g;
     const size_t nodes = (size_t)g.numBlocks * 64;
-    s->maxTiles = std::min<long long>(g.numBlocks, (long long)N);
+    // tile k is pblock k, so the partial buffers hold one 6^3 box per pblock
+    s->maxTiles = g.numBlocks;
     const int radixTiles = grid1d((long long)N, kRadixTile);
     int rc = 0;
     rc |= dalloc(&s->F, 9 * N);
@@ -1607,13 +1610,8 @@ int crHipSolverInitialize(CrHipSolver *s)
         rc |= dalloc(&s->tileForce, (size_t)s->maxTiles * kTileNodes);
     }
     rc |= dalloc(&s->dirty, g.numBlocks);
-    rc |= dalloc(&s->tileOf, g.numBlocks);
-    rc |= dalloc(&s->activeFlag, g.numBlocks);
-    rc |= dalloc(&s->activeList, g.numBlocks);
     rc |= dalloc(&s->segStart, g.numBlocks);
     rc |= dalloc(&s->segEnd, g.numBlocks);
-    rc |= dalloc(&s->tileKey, g.numBlocks);
-    rc |= dalloc(&s->counters, 4);
     rc |= dalloc(&s->keysA, N);
     rc |= dalloc(&s->keysB, N);
     rc |= dalloc(&s->valsA, N);
@@ -1654,9 +1652,8 @@ int crHipSolverInitialize(CrHipSolver *s)
     if (s->gridForce)
         rc |= hipMemsetAsync(s->gridForce, 0, nodes * sizeof(float4), st);
     rc |= hipMemsetAsync(s->dirty, 0, g.numBlocks, st);
-    rc |= hipMemsetAsync(s->tileOf, 0xff, g.numBlocks * sizeof(int), st);
-    rc |= hipMemsetAsync(s->activeFlag, 0, g.numBlocks * sizeof(int), st);
-    rc |= hipMemsetAsync(s->counters, 0, 4 * sizeof(int), st);
+    rc |= hipMemsetAsync(s->segStart, 0, g.numBlocks * sizeof(int), st);
+    rc |= hipMemsetAsync(s->segEnd, 0, g.numBlocks * sizeof(int), st);
     rc |= hipMemsetAsync(s->shapeCounts, 0, 4 * sizeof(int), st);
     if (rc)
         return CRMPM_HIP_ERR_LAUNCH;
@@ -1688,94 +1685,54 @@ int crHipSolverStep(CrHipSolver *s, float *dtOut)
         return CRMPM_HIP_ERR_STATE;
     if (dtOut)
         *dtOut = s->dt;
-    hipStream_t st = s->stream;
     const GridGeom &g = s->g;
-    k_prepare<<<grid1d(g.numBlocks, kWaves), kWG, 0, st>>>(g, s->grid, s->gridForce, s->dirty, s->tileOf, s->activeFlag,
-                                                           s->counters);
-    k_shape_cache<<<std::max(1, grid1d(s->numShapes, 256)), 256, 0, st>>>(s->numShapes, s->shapeIds, s->shapes, s->geoms, s->shapeRecs, s->shapeCounts,
-                                     s->slicerList);
+    const int blockWGs = grid1d(g.numBlocks, kWaves);
+    const ShapeTables tables{s->numShapes, s->shapeIds, s->shapes, s->geoms, s->shapeRecs, s->shapeCounts, s->slicerList};
+    launch(s, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + std::max(1, grid1d(s->numShapes, kWG)), kWG, g, blockWGs,
+           s->grid, s->gridForce, s->dirty, s->segStart, s->segEnd, tables);
     const int n = (int)s->numActive;
     if (n == 0)
         return check_launch();
     const unsigned *order = nullptr, *keys = nullptr;
-    int rc;
-    {
-        ProfScope ps(s, CRMPM_HIP_PROF_BINNING);
-        // The standard solver's P2G ignores the active mask (src/solver/cpu_standard_mpm_solver.cpp:101-120).
-        rc = build_tiles(s, n, s->kind != CRMPM_HIP_STANDARD, &order, &keys);
-    }
+    // The standard solver's P2G ignores the active mask (src/solver/cpu_standard_mpm_solver.cpp:101-120).
+    int rc = build_tiles(s, n, s->kind != CRMPM_HIP_STANDARD, &order, &keys);
     if (rc)
         return rc;
     StepCommon c = common(s, order);
-    const int tileWGs = grid1d(std::min<long long>(s->maxTiles, n), kWaves);
-    const int updWGs = grid1d(std::min<long long>(g.numBlocks, 8ll * std::min<long long>(s->maxTiles, n)), kWaves);
+    const int tileWGs = grid1d(g.numBlocks, kWaves);
+    const int updWGs = grid1d(g.numBlocks, kWaves);
+    const float4 *noForce = nullptr;
     if (s->kind == CRMPM_HIP_MLS)
     {
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_MLS_P2G);
-            k_mls_p2g<<<tileWGs, kWG, 0, st>>>(c);
-        }
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_UPDATE);
-            k_update<0><<<updWGs, kWG, 0, st>>>(c, s->activeList, s->tileOf, nullptr, s->dirty);
-        }
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_MLS_G2P);
-            k_mls_g2p<<<tileWGs, kWG, 0, st>>>(c);
-        }
+        launch(s, CRMPM_HIP_PROF_MLS_P2G, k_mls_p2g, tileWGs, kWG, c);
+        launch(s, CRMPM_HIP_PROF_UPDATE, k_update<0>, updWGs, kWG, c, noForce, s->dirty);
+        launch(s, CRMPM_HIP_PROF_MLS_G2P, k_mls_g2p, tileWGs, kWG, c);
     }
     else if (s->kind == CRMPM_HIP_STANDARD)
     {
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_STD_P2G);
-            k_std_p2g<<<tileWGs, kWG, 0, st>>>(c, s->tileForce);
-        }
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_UPDATE);
-            k_update<2><<<updWGs, kWG, 0, st>>>(c, s->activeList, s->tileOf, s->tileForce, s->dirty);
-        }
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_STD_G2P);
-            k_std_g2p<<<tileWGs, kWG, 0, st>>>(c);
-        }
+        launch(s, CRMPM_HIP_PROF_STD_P2G, k_std_p2g, tileWGs, kWG, c, s->tileForce);
+        launch(s, CRMPM_HIP_PROF_UPDATE, k_update<2>, updWGs, kWG, c, s->tileForce, s->dirty);
+        launch(s, CRMPM_HIP_PROF_STD_G2P, k_std_g2p, tileWGs, kWG, c);
     }
     else
     {
         SolverState S{s->F, s->C, s->V0, s->maxParticles};
-        {
-            ProfScope ps(s, CRMPM_HIP_PROF_BINNING);
-            k_pb_stage<<<grid1d(n, 256), 256, 0, st>>>(n, g.numBlocks, keys, order, s->P, S, s->st);
-        }
+        launch(s, CRMPM_HIP_PROF_BINNING, k_pb_stage, grid1d(n, 256), 256, n, g.numBlocks, keys, order, s->P, S, s->st);
         const int K = s->iterations;
         if (K == 0)
         {
-            k_pb_integrate_only<<<grid1d(n, 256), 256, 0, st>>>(c, n, keys, s->st, s->gravity);
+            launch(s, CRMPM_HIP_PROF_PB_G2P, k_pb_integrate_only, grid1d(n, 256), 256, c, n, keys, s->st, s->gravity);
         }
         else
         {
-            {
-                ProfScope ps(s, CRMPM_HIP_PROF_PB_P2G);
-                k_pb_tile<0><<<tileWGs, kWG, 0, st>>>(c, s->st, s->gravity);
-            }
-            {
-                ProfScope ps(s, CRMPM_HIP_PROF_UPDATE);
-                k_update<1><<<updWGs, kWG, 0, st>>>(c, s->activeList, s->tileOf, nullptr, s->dirty);
-            }
+            launch(s, CRMPM_HIP_PROF_PB_P2G, k_pb_tile<0>, tileWGs, kWG, c, s->st, s->gravity);
+            launch(s, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, c, noForce, s->dirty);
             for (int it = 1; it < K; ++it)
             {
-                {
-                    ProfScope ps(s, CRMPM_HIP_PROF_PB_FUSED);
-                    k_pb_tile<1><<<tileWGs, kWG, 0, st>>>(c, s->st, s->gravity);
-                }
-                {
-                    ProfScope ps(s, CRMPM_HIP_PROF_UPDATE);
-                    k_update<1><<<updWGs, kWG, 0, st>>>(c, s->activeList, s->tileOf, nullptr, s->dirty);
-                }
+                launch(s, CRMPM_HIP_PROF_PB_FUSED, k_pb_tile<1>, tileWGs, kWG, c, s->st, s->gravity);
+                launch(s, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, c, noForce, s->dirty);
             }
-            {
-                ProfScope ps(s, CRMPM_HIP_PROF_PB_G2P);
-                k_pb_tile<2><<<tileWGs, kWG, 0, st>>>(c, s->st, s->gravity);
-            }
+            launch(s, CRMPM_HIP_PROF_PB_G2P, k_pb_tile<2>, tileWGs, kWG, c, s->st, s->gravity);
         }
     }
     return check_launch();
