@@ -10,7 +10,8 @@ already resident in HBM when timing starts.  A step is one substep.
 
 Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
 (the fused G2P+P2G iteration kernel) measured with HIP events on the solver
-stream over the timed region, and a CPU baseline timed on this host.
+stream over the timed region (the events ride on the dispatches of every 10th
+substep), and a CPU baseline timed on this host.
 """
 import argparse
 import copy
@@ -37,6 +38,7 @@ BYTES_PER_PARTICLE_FUSED = 164
 BYTES_PER_TOUCHED_NODE_FUSED = 32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
 FUSED_KERNEL = "k_pb_tile<1>"
+PROFILE_EVERY = 10
 COPY_CAL_BYTES = 256 << 20
 
 
@@ -71,13 +73,25 @@ def _read_csv(directory, suffix):
         return list(csv.DictReader(f))
 
 
-def _counter_per_launch(directory):
-    # kernel -> mean counter value (KiB) per dispatch
+def _counter_per_launch(directory, counter=None):
+    # kernel -> mean counter value per dispatch (FETCH_SIZE / WRITE_SIZE: KiB)
     acc = {}
     for r in _read_csv(directory, "counter_collection.csv"):
+        if counter is not None and r["Counter_Name"] != counter:
+            continue
         k = _short_name(r["Kernel_Name"])
         acc.setdefault(k, []).append(float(r["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in acc.items()}, acc
+
+
+# VALU issue model: a wave64 VALU instruction occupies its SIMD for 4 cycles;
+# MI355X has 256 CUs x 4 SIMDs at up to 2.4 GHz.
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
+
+
+def valu_issue_us(instructions):
+    return instructions * 4.0 / SIMDS / CLOCK_HZ * 1e6
 
 
 def summarize(out_dir, title, commands):
@@ -89,13 +103,16 @@ def summarize(out_dir, title, commands):
     write, _ = _counter_per_launch(os.path.join(out_dir, "write"))
     _, fetch_cp = _counter_per_launch(os.path.join(out_dir, "fetch_copy"))
     _, write_cp = _counter_per_launch(os.path.join(out_dir, "write_copy"))
+    sq_dir = os.path.join(out_dir, "sq")
+    valu = _counter_per_launch(sq_dir, "SQ_INSTS_VALU")[0] if os.path.isdir(sq_dir) else {}
     kib = COPY_CAL_BYTES / 1024.0
     f_fac = kib / max(fetch_cp["__amd_rocclr_copyBuffer"])
     w_fac = kib / max(write_cp["__amd_rocclr_copyBuffer"])
     lines = [f"# {title}", "", "Commands (repository root, rocprofv3 on one MI355X):", "", "```"]
     lines += commands + ["```", "",
-                         "| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM MB/dispatch (calibrated) |",
-                         "|---|---|---|---|---|---|---|"]
+                         "| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
+                         "| HBM MB/dispatch (calibrated) | SQ_INSTS_VALU/dispatch | VALU issue µs |",
+                         "|---|---|---|---|---|---|---|---|---|"]
     merged = {}
     for r in stats:
         k = _short_name(r["Name"])
@@ -105,13 +122,17 @@ def summarize(out_dir, title, commands):
     for k, (c, t) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
         fr, wr = fetch.get(k), write.get(k)
         mb = "" if fr is None or wr is None else f"{(fr * f_fac + wr * w_fac) * 1024 / 1e6:.2f}"
+        vi = valu.get(k)
         lines.append(f"| `{k}` | {c} | {t / c / 1e3:.1f} | {100 * t / total:.2f} | "
-                     f"{'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} |")
+                     f"{'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} | "
+                     f"{'' if vi is None else f'{vi:,.0f}'} | {'' if vi is None else f'{valu_issue_us(vi):.1f}'} |")
     lines += ["", f"Calibration: a device-to-device copy of {COPY_CAL_BYTES >> 20} MiB reads FETCH_SIZE "
               f"{max(fetch_cp['__amd_rocclr_copyBuffer']):,.0f} KiB and WRITE_SIZE {max(write_cp['__amd_rocclr_copyBuffer']):,.0f} KiB "
-              f"per dispatch, so bytes = FETCH_SIZE x {f_fac:.3f} KiB + WRITE_SIZE x {w_fac:.3f} KiB."]
+              f"per dispatch, so bytes = FETCH_SIZE x {f_fac:.3f} KiB + WRITE_SIZE x {w_fac:.3f} KiB.",
+              "", "VALU issue µs = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz): the time the VALU pipes need "
+              "for the kernel's vector instructions; a kernel whose duration is close to it is bound by VALU issue, not HBM."]
     result = {"kernel": FUSED_KERNEL, "fetch_kib_raw": fetch.get(FUSED_KERNEL), "write_kib_raw": write.get(FUSED_KERNEL),
-              "fetch_factor": f_fac, "write_factor": w_fac}
+              "fetch_factor": f_fac, "write_factor": w_fac, "valu_instructions": valu.get(FUSED_KERNEL)}
     if result["fetch_kib_raw"] is not None and result["write_kib_raw"] is not None:
         result["bytes_per_launch"] = 1024.0 * (result["fetch_kib_raw"] * f_fac + result["write_kib_raw"] * w_fac)
     return "\n".join(lines) + "\n", result
@@ -160,7 +181,10 @@ def main():
             d = c.split("_")[0].lower()
             cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o pb -- python3 bench.py --steps 3 --warmup 1 --cpu-substeps 0")
             cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- python3 bench.py --copy-calibration")
-        cmds.append(f"python3 bench.py --summarize DIR")
+        cmds.append("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
+                    "SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o pb -- python3 bench.py --steps 3 "
+                    "--warmup 1 --cpu-substeps 0")
+        cmds.append("python3 bench.py --summarize DIR")
         md, traffic = summarize(args.summarize, "Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations", cmds)
         with open(os.path.join(ROOT, "profiles", "pb_slab262k_kernel_trace.md"), "w") as f:
             f.write(md)
@@ -189,17 +213,15 @@ def main():
     frames = solver.scene["frames"]
     fsub = frames[0]["substeps"]
 
-    def run(first, count):
+    def run(first, count, sample=False):
         # substeps [first, first + count) of the scene, kinematic target per frame
-        s = first
-        end = first + count
-        while s < end:
-            f = s // fsub
-            S.apply_frame(solver.scene, frames[f])
-            solver._upload_shapes()
-            k = min((f + 1) * fsub, end) - s
-            solver.step(k)
-            s += k
+        for s in range(first, first + count):
+            if s % fsub == 0 or s == first:
+                S.apply_frame(solver.scene, frames[s // fsub])
+                solver._upload_shapes()
+            if sample:
+                solver.profile_mask(H.PROF_CATEGORIES if (s - first) % PROFILE_EVERY == 0 else [])
+            solver.step(1)
 
     run(0, W)
     torch.cuda.synchronize()
@@ -207,10 +229,15 @@ def main():
     grid = solver.grid()
     touched = int(np.count_nonzero(grid[:, 3] > 0))
 
-    solver.profile_begin(max_launches=K * (2 * scene["iterations"] + 16) + 64)
+    # Every PROFILE_EVERY-th substep records HIP events around each of its
+    # launches (a recorded launch costs a few microseconds of dispatch, so
+    # recording every launch would slow the run it measures; recording whole
+    # substeps keeps each kernel's start event behind its predecessor's end).
+    solver.profile_begin(max_launches=(K // PROFILE_EVERY + 1) * (2 * scene["iterations"] + 16) + 64)
+    solver.profile_mask([])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(W, K)
+    run(W, K, sample=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = solver.profile_end()
@@ -222,9 +249,12 @@ def main():
     bytes_fused = BYTES_PER_PARTICLE_FUSED * n + BYTES_PER_TOUCHED_NODE_FUSED * touched
     achieved = bytes_fused / avg_fused_s / 1e9
     traffic = None
+    valu_insts = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            prof_summary = json.load(f)
+        traffic = prof_summary.get("bytes_per_launch")
+        valu_insts = prof_summary.get("valu_instructions")
     out = {
         "metric": METRIC,
         "value": value,
@@ -244,8 +274,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1> (fused G2P+P2G)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
-                     "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched},
-        "kernel_ms_per_step": {k: v[0] / K for k, v in prof.items() if v[1]},
+                     "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched,
+                     # the fused kernel's VALU issue time (rocprofv3 SQ_INSTS_VALU of this workload) over its duration
+                     "valu_issue_frac": None if valu_insts is None else valu_issue_us(valu_insts) / (avg_fused_s * 1e6)},
     }
     st = solver.state()
     out["state_check"] = {"nonfinite_particles": int((~np.isfinite(st["xm"])).any(axis=1).sum()),

@@ -5,7 +5,9 @@
 // library is compiled with -ffp-contract=off and IEEE division/sqrt, so the
 // per-particle arithmetic follows the reference CPU path's evaluation order
 // term for term (rsqrtf is 1/sqrt there, include/utils/cuda_compat.h:55);
-// only the summation order of the tile reductions differs.
+// the summation order of the tile reductions differs, and the SVD (used by
+// PB-MPM once per particle per iteration) is contracted to FMAs with the
+// hardware rsqrt, as in the reference's CUDA build.
 // Reference semantics reproduced (file:line of the reference):
 //   Vec3::outerProduct(a, b)(r, c) = a_c * b_r     include/math/mat33.h:287-306
 //   Mat3x3f::inverse() = inverse^T, 0 if |det|<1e-6 include/math/mat33.h:256-283
@@ -31,6 +33,11 @@ namespace crhip
     enum : int { mCoRotational = 0, mNeoHookean, mLinearElastic };
 
     CR_DEV float rsq(float x) { return 1.0f / sqrtf(x); }
+    // Hardware reciprocal square root (v_rsq_f32, about 1 ulp).  Used only in
+    // the SVD's Jacobi and Givens steps, whose McAdams formulation is built for
+    // an approximate rsqrt and refines it with a Newton step where it matters;
+    // the reference's CUDA build takes the fast rsqrtf there as well.
+    CR_DEV float rsq_svd(float x) { return __builtin_amdgcn_rsqf(x); }
 
     struct V3
     {
@@ -124,6 +131,7 @@ namespace crhip
     CR_DEV void svd_conj(float &sPP, float &sQQ, float &sPQ, float &sRR, float &sRP, float &sRQ,
                          float &qs, float &qR, float &qP, float &qQ)
     {
+#pragma clang fp contract(fast)
         float sh = sPQ * 0.5f;
         float t5 = sPP - sQQ;
         float t2 = sh * sh;
@@ -133,7 +141,7 @@ namespace crhip
         float t1 = sh * sh;
         t2 = ch * ch;
         float t3 = t1 + t2;
-        float t4 = rsq(t3);
+        float t4 = rsq_svd(t3);
         sh = t4 * sh;
         ch = t4 * ch;
         t1 = 5.82842712474619f * t1;
@@ -187,7 +195,8 @@ namespace crhip
 
     CR_DEV float svd_refined_rsq(float t2)
     {
-        float t1 = rsq(t2);
+#pragma clang fp contract(fast)
+        float t1 = rsq_svd(t2);
         float t4 = t1 * 0.5f;
         float t3 = t1 * t4;
         t3 = t1 * t3;
@@ -200,6 +209,7 @@ namespace crhip
     // columns of u are passed explicitly so everything stays in registers.
     CR_DEV void svd_givens(float app, float aqp, float &c, float &s)
     {
+#pragma clang fp contract(fast)
         float sh = aqp * aqp;
         sh = (sh >= 1.e-12f) ? aqp : 0.0f;
         float ch = 0.0f - app;
@@ -230,6 +240,7 @@ namespace crhip
     }
     CR_DEV void svd_rot(float &a, float &b, float c, float s)
     {
+#pragma clang fp contract(fast)
         const float t1 = s * a;
         const float t2 = s * b;
         a = c * a;
@@ -249,6 +260,7 @@ namespace crhip
     CR_DEV void svd_kernel(float a11, float a12, float a13, float a21, float a22, float a23, float a31, float a32, float a33,
                            float (&U)[3][3], float (&V)[3][3], float (&sig)[3])
     {
+#pragma clang fp contract(fast)
         float s11 = a11 * a11;
         s11 = a21 * a21 + s11;
         s11 = a31 * a31 + s11;

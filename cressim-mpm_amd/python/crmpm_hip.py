@@ -70,6 +70,7 @@ EXPORTS = {
     "crHipSolverBinParticles": [_vp, _vp, _vp],
     "crHipSolverProfileBegin": [_vp, ctypes.c_int],
     "crHipSolverProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
+    "crHipSolverProfileMask": [_vp, ctypes.c_uint],
 }
 
 PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g", "mls_g2p", "std_p2g", "std_g2p",
@@ -152,8 +153,14 @@ class HipSolver:
         t = S.shape_tables(sc)
         dev = self.device
         T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
+        nf4 = len(S.SHAPE_F4)
         if first or not hasattr(self, "sh"):
-            self.sh = {k: T(t[k], np.float32) for k in S.SHAPE_F4}
+            # the float4 shape fields live in one device block (one view per
+            # field) so a frame's kinematic update is a single pinned copy
+            cap = len(t["type"])
+            self.sh_block = torch.from_numpy(np.ascontiguousarray(np.stack([t[k] for k in S.SHAPE_F4]))).to(dev)
+            self.sh_stage = torch.empty((nf4, cap, 4), dtype=torch.float32).pin_memory()
+            self.sh = {k: self.sh_block[i] for i, k in enumerate(S.SHAPE_F4)}
             self.sh["type"] = T(t["type"], np.int32)
             self.sh["geom"] = T(t["geom"], np.int32)
             self.sh["ids"] = T(np.maximum(t["ids"], 0) if len(t["ids"]) else np.zeros(1, np.int32), np.int32)
@@ -166,9 +173,15 @@ class HipSolver:
             self.sdf_lbcs = T(sc["sdf_lbcs"], np.float32)
             self.sdf_data = T(sc["sdf_data"], np.float32)
         else:
-            # pinned staging keeps the per-frame shape update asynchronous on the stream
-            for k in S.SHAPE_F4:
-                self.sh[k].copy_(torch.from_numpy(t[k]).pin_memory(), non_blocking=True)
+            # pinned staging keeps the per-frame shape update asynchronous on
+            # the stream; wait for the previous frame's copy before refilling it
+            if getattr(self, "sh_copied", None) is not None:
+                self.sh_copied.synchronize()
+            self.sh_stage.numpy()[:] = np.stack([t[k] for k in S.SHAPE_F4])
+            with torch.cuda.stream(self.stream):
+                self.sh_block.copy_(self.sh_stage, non_blocking=True)
+                self.sh_copied = torch.cuda.Event()
+                self.sh_copied.record(self.stream)
         p = lambda x: x.data_ptr()
         # field order: type, geometryIdx, position, rotation, invScale, linearVelocity, angularVelocity,
         # comInvMass, inertiaInv0, inertiaInv1, force, torque, params0
@@ -248,6 +261,13 @@ class HipSolver:
 
     def profile_begin(self, max_launches):
         check(self.lib.crHipSolverProfileBegin(self.h, max_launches), "profile begin")
+
+    def profile_mask(self, categories):
+        """Record events only for launches of the named categories (empty: pause)."""
+        mask = 0
+        for c in categories:
+            mask |= 1 << PROF_CATEGORIES.index(c)
+        check(self.lib.crHipSolverProfileMask(self.h, mask), "profile mask")
 
     def profile_end(self):
         n = len(PROF_CATEGORIES)

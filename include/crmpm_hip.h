@@ -161,6 +161,11 @@ int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int 
 #define CRMPM_HIP_PROF_CATEGORIES 9
 int crHipSolverProfileBegin(CrHipSolver *s, int maxLaunches);
 int crHipSolverProfileEnd(CrHipSolver *s, float *msPerCategory, int *launchesPerCategory, int numCategories);
+/* Between ProfileBegin and ProfileEnd, record events only for launches whose
+ * category bit (1 << CRMPM_HIP_PROF_*) is set in mask; 0 pauses recording.
+ * ProfileBegin sets the mask to all categories.  Each recorded launch costs a
+ * few microseconds of dispatch time, so timing runs sample a subset. */
+int crHipSolverProfileMask(CrHipSolver *s, unsigned int mask);
 /* Copies the grid (mass-weighted momentum or velocity in xyz, mass in w) to a
  * device float4 array in node order x + y * nx + z * nx * ny. */
 int crHipSolverCopyGrid(CrHipSolver *s, float *dst);
