@@ -173,22 +173,25 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
-                    help="write the profile summary from rocprofv3 outputs under DIR/{trace,fetch,write,fetch_copy,write_copy}")
+                    help="write the profile summary from rocprofv3 outputs under DIR/{trace,fetch,write,fetch_copy,write_copy,sq}")
+    ap.add_argument("--summary-out", metavar="DIR", default=os.path.join(ROOT, "profiles"),
+                    help="where --summarize writes its .md and .json (default: profiles/)")
     args = ap.parse_args()
     if args.summarize:
-        cmds = ["rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o pb -- python3 bench.py --steps 30 --warmup 5 --cpu-substeps 0"]
+        run = "python3 bench.py --cpu-substeps 0"
+        cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o pb -- {run}"]
         for c in ("FETCH_SIZE", "WRITE_SIZE"):
             d = c.split("_")[0].lower()
-            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o pb -- python3 bench.py --steps 3 --warmup 1 --cpu-substeps 0")
+            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o pb -- {run}")
             cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- python3 bench.py --copy-calibration")
         cmds.append("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
-                    "SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o pb -- python3 bench.py --steps 3 "
-                    "--warmup 1 --cpu-substeps 0")
+                    f"SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o pb -- {run}")
         cmds.append("python3 bench.py --summarize DIR")
         md, traffic = summarize(args.summarize, "Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations", cmds)
-        with open(os.path.join(ROOT, "profiles", "pb_slab262k_kernel_trace.md"), "w") as f:
+        os.makedirs(args.summary_out, exist_ok=True)
+        with open(os.path.join(args.summary_out, "pb_slab262k_kernel_trace.md"), "w") as f:
             f.write(md)
-        with open(TRAFFIC_FILE, "w") as f:
+        with open(os.path.join(args.summary_out, os.path.basename(TRAFFIC_FILE)), "w") as f:
             json.dump(traffic, f, indent=1)
         print(md)
         return
