@@ -164,12 +164,59 @@ def cpu_baseline(scene, substeps):
     }
 
 
+def batched(kind_name, steps, warmup, tune=True, graph=True):
+    """64 scenes of uneven size (config3) advanced as one batch, against the sum
+    of the same scenes run one at a time through the single-scene solver
+    (same kernels).  Returns the batch throughput and the speedup."""
+    import torch
+
+    import crmpm_hip as H
+
+    kind = {"mls": S.MLS, "pb": S.PB}[kind_name]
+    scenes = S.config3(kind=kind, steps=steps + warmup)
+    n_total = sum(len(sc["xm"]) for sc in scenes)
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    b.step(warmup)
+    tile_waves = update_waves = 0
+    if tune:
+        tile_waves, update_waves = b.autotune(steps_per_trial=2)
+    b.set_graph_mode(graph)
+    b.step(1)  # capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    finite = all(np.isfinite(b.state(i)["xm"]).all() for i in (0, len(scenes) - 1))
+    b.close()
+    t_plain = 0.0
+    for sc in scenes:
+        g = H.HipSolver(copy.deepcopy(sc))
+        g.compute_initial_data()
+        g.step(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        g.close()
+    return {"workload": f"{len(scenes)} {kind_name.upper()} scenes, log-uniform 2k-200k particles, grids 32^3-128^3",
+            "particles": n_total, "substeps": steps, "iterations": scenes[0]["iterations"] if kind == S.PB else None,
+            "value": n_total * steps / t_batch, "unit": "particle-substeps/s",
+            "batch_ms_per_substep": t_batch / steps * 1e3, "sum_of_plain_ms_per_substep": t_plain / steps * 1e3,
+            "speedup_vs_sum_of_plain": t_plain / t_batch, "autotuned_waves": [tile_waves, update_waves],
+            "hip_graph": graph, "finite": bool(finite)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched"], default="all",
+                    help="all: the PB slab headline line plus the batched comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
@@ -208,6 +255,9 @@ def main():
 
     import crmpm_hip as H
 
+    if args.workload == "batched":
+        print(json.dumps({"batched_mls": batched("mls", 100, 5), "batched_pb": batched("pb", 20, 2)}), flush=True)
+        return
     K, W = args.steps, args.warmup
     scene = S.config2(steps=K + W)
     n = len(scene["xm"])
@@ -290,6 +340,13 @@ def main():
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps)
     else:
         out["cpu_baseline"] = None
+    if args.workload == "all":
+        # the batched form beside the plain solver (64 scenes, MLS 100 substeps, PB 20 substeps)
+        for key, kind, steps, warm in (("batched_mls", "mls", 100, 5), ("batched_pb", "pb", 20, 2)):
+            try:
+                out[key] = batched(kind, steps, warm)
+            except Exception as e:  # reported, not fatal: the headline line still prints
+                out[key] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

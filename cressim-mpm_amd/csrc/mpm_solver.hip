@@ -68,6 +68,108 @@ namespace
         int stride;
     };
 
+    // One scene as the kernels see it: its grid geometry and parameters, and
+    // the batch-wide arrays with the scene's pblock-indexed ones (segments,
+    // tile partials, grid) offset to the scene's first pblock, so tile and
+    // block indices below are scene-local.
+    struct StepCommon
+    {
+        GridGeom g;
+        float3 gravity;
+        float dt;
+        ParticleBind P;
+        SolverState S;
+        ShapeView sv;
+        const int *segStart;
+        const int *segEnd;
+        const unsigned *order; // sorted particle indices (global)
+        float4 *tilebuf;
+        float4 *tileForce;
+        float4 *grid;
+    };
+
+    // Per-scene record in device memory.  A batch is a list of scenes whose
+    // pblocks are numbered consecutively (scene s owns global pblocks
+    // [blockOffset, blockOffset + g.numBlocks)) and whose particles are
+    // slices of the bound particle arrays ([particleOffset, + numActive)).
+    struct SceneRec
+    {
+        GridGeom g;
+        float3 gravity;
+        float dt;
+        ShapeView sv;
+        int blockOffset;
+        int particleOffset;
+        int numActive;
+        int activeOffset; // sort item of the scene's first active particle
+    };
+
+    // Shape tables bound to a scene (engine-owned), read when a step starts.
+    struct SceneShapes
+    {
+        int numShapes;
+        const int *shapeIds;
+        CrHipShapeData sd;
+        CrHipGeometryData gd;
+    };
+
+    // Kernel argument of the batched launches: every scene in one launch.
+    struct BatchArgs
+    {
+        const SceneRec *scenes;
+        const int *blockScene; // global pblock -> scene
+        int numScenes;
+        int totalBlocks;
+        ParticleBind P;
+        SolverState S;
+        const int *segStart; // global pblock
+        const int *segEnd;
+        const unsigned *order;
+        float4 *tilebuf;
+        float4 *tileForce;
+        float4 *grid;
+    };
+
+    CR_DEV StepCommon scene_view_of(const BatchArgs &b, int s)
+    {
+        const SceneRec &r = b.scenes[s];
+        StepCommon c;
+        c.g = r.g;
+        c.gravity = r.gravity;
+        c.dt = r.dt;
+        c.P = b.P;
+        c.S = b.S;
+        c.sv = r.sv;
+        c.segStart = b.segStart + r.blockOffset;
+        c.segEnd = b.segEnd + r.blockOffset;
+        c.order = b.order;
+        c.tilebuf = b.tilebuf + (size_t)r.blockOffset * kTileNodes;
+        c.tileForce = b.tileForce ? b.tileForce + (size_t)r.blockOffset * kTileNodes : nullptr;
+        c.grid = b.grid + (size_t)r.blockOffset * 64;
+        return c;
+    }
+
+    // View of the scene owning global pblock tg; t receives the scene-local pblock.
+    CR_DEV StepCommon scene_view(const BatchArgs &b, int tg, int &t)
+    {
+        const int s = __builtin_amdgcn_readfirstlane(b.blockScene[tg]);
+        t = tg - b.scenes[s].blockOffset;
+        return scene_view_of(b, s);
+    }
+
+    // Persistent tile loop: wavefront w of the launch visits global pblocks
+    // w, w + W, w + 2W, ... (W = wavefronts in the launch), so a launch of any
+    // size covers the batch and the autotuner picks the size.
+    CR_DEV int first_tile(int wave) { return __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWaves + wave); }
+    CR_DEV int tile_stride() { return (int)gridDim.x * kWaves; }
+    // Next pblock of a tile loop; a non-persistent launch (one wavefront per
+    // pblock) compiles to straight-line code, which needs fewer registers.
+    template <bool Persistent>
+    CR_DEV int next_tile(int tg, int totalBlocks)
+    {
+        return Persistent ? tg + tile_stride() : totalBlocks;
+    }
+
     CR_DEV int block_key(const GridGeom &g, int bx, int by, int bz) { return bx + g.nbx * (by + g.nby * bz); }
 
     CR_DEV int grid_index(const GridGeom &g, int x, int y, int z)
@@ -175,54 +277,47 @@ namespace
         recs[s] = r;
     }
 
-    struct ShapeTables
-    {
-        int numShapes;
-        const int *shapeIds;
-        CrHipShapeData sd;
-        CrHipGeometryData gd;
-        ShapeRec *recs;
-        int *counts;
-        int *slicers;
-    };
-
     // Start of a step.  Workgroups [0, blockWGs): zero the grid blocks written
     // since the last step and mark every pblock empty (one wavefront per
-    // block).  The workgroups after them copy the bound shape tables into a
-    // compact record array in shapeIds order; the first of those also lists
-    // the slicers (shapes whose geometry is a slicer).
-    __global__ __launch_bounds__(kWG) void k_prepare(GridGeom g, int blockWGs, float4 *grid, float4 *gridForce,
-                                                     unsigned char *dirty, int *segStart, int *segEnd, ShapeTables st)
+    // block).  Workgroup blockWGs + s copies scene s's bound shape tables into
+    // its compact record array in shapeIds order and lists its slicers
+    // (shapes whose geometry is a slicer).
+    __global__ __launch_bounds__(kWG) void k_prepare(int totalBlocks, int blockWGs, float4 *grid, float4 *gridForce,
+                                                     unsigned char *dirty, int *segStart, int *segEnd,
+                                                     const SceneShapes *shapes, const SceneRec *scenes)
     {
         const int lane = threadIdx.x & 63;
         if ((int)blockIdx.x >= blockWGs)
         {
-            const int sw = blockIdx.x - blockWGs;
-            const int s = sw * kWG + threadIdx.x;
-            if (s < st.numShapes)
-                cache_shape(s, st.shapeIds, st.sd, st.gd, st.recs);
-            if (sw == 0 && threadIdx.x == 0)
+            const int sc = blockIdx.x - blockWGs;
+            const SceneShapes st = shapes[sc];
+            ShapeRec *recs = const_cast<ShapeRec *>(scenes[sc].sv.recs);
+            int *counts = const_cast<int *>(scenes[sc].sv.counts);
+            int *slicers = const_cast<int *>(scenes[sc].sv.slicers);
+            for (int s = threadIdx.x; s < st.numShapes; s += kWG)
+                cache_shape(s, st.shapeIds, st.sd, st.gd, recs);
+            if (threadIdx.x == 0)
             {
                 int ns = 0;
                 for (int k = 0; k < st.numShapes; ++k)
                 {
                     const int gt = st.gd.type[st.sd.geometryIdx[st.shapeIds[k]]];
                     if (gt == gQuadSlicer || gt == gTriangleMeshSlicer)
-                        st.slicers[ns++] = k;
+                        slicers[ns++] = k;
                 }
-                st.counts[0] = st.numShapes;
-                st.counts[1] = ns;
+                counts[0] = st.numShapes;
+                counts[1] = ns;
             }
             return;
         }
         const int b = blockIdx.x * kWaves + (threadIdx.x >> 6);
-        if (b >= g.numBlocks)
+        if (b >= totalBlocks)
             return;
         if (dirty[b])
         {
-            grid[b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            grid[(size_t)b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gridForce)
-                gridForce[b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                gridForce[(size_t)b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         wave_sync();
         if (lane == 0)
@@ -284,11 +379,28 @@ namespace
         counts[threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
     }
 
-    // Key per particle (its pblock, or the sentinel numBlocks when it does
-    // not take part), the identity permutation, and the histogram of the
-    // first radix digit: k_radix_hist fused with the binning pass.
-    __global__ __launch_bounds__(kWG) void k_bin_hist(GridGeom g, int n, ParticleBind P, int useMask, unsigned *keys,
-                                                      unsigned *vals, unsigned *counts, int numTiles)
+    // Sort item i of a step is active particle i - activeOffset of the scene
+    // whose [activeOffset, activeOffset + numActive) holds i.
+    CR_DEV int scene_of_item(const int *activePrefix, int numScenes, int i)
+    {
+        int lo = 0, hi = numScenes - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (activePrefix[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
+
+    // Key per sort item (the global pblock of its particle, or the sentinel
+    // totalBlocks when the particle does not take part), the particle index,
+    // and the histogram of the first radix digit: k_radix_hist fused with the
+    // binning pass.
+    __global__ __launch_bounds__(kWG) void k_bin_hist(BatchArgs b, const int *activePrefix, int n, int useMask,
+                                                      unsigned *keys, unsigned *vals, unsigned *counts, int numTiles)
     {
         __shared__ unsigned h[256];
         h[threadIdx.x] = 0;
@@ -300,17 +412,20 @@ namespace
             const int i = base + r * kWG + threadIdx.x;
             if (i < n)
             {
-                unsigned key = (unsigned)g.numBlocks;
-                if (!useMask || P.mask[i])
+                const int sc = scene_of_item(activePrefix, b.numScenes, i);
+                const SceneRec &rec = b.scenes[sc];
+                const int p = rec.particleOffset + (i - activePrefix[sc]);
+                unsigned key = (unsigned)b.totalBlocks;
+                if (!useMask || b.P.mask[p])
                 {
-                    const float4 x = P.xm[i];
+                    const float4 x = b.P.xm[p];
                     int bx, by, bz;
                     V3 fx;
-                    cell_of(g, v3(x), bx, by, bz, fx);
-                    key = (unsigned)pblock_key(g, bx, by, bz);
+                    cell_of(rec.g, v3(x), bx, by, bz, fx);
+                    key = (unsigned)(rec.blockOffset + pblock_key(rec.g, bx, by, bz));
                 }
                 keys[i] = key;
-                vals[i] = (unsigned)i;
+                vals[i] = (unsigned)p;
                 atomicAdd(&h[key & 255u], 1u);
             }
         }
@@ -623,62 +738,68 @@ namespace
     // instead of one pass per particle of its fullest cell.  One wavefront per
     // tile; tmp holds (rank << 6 | cell) per sorted particle.
     // ------------------------------------------------------------------
-    __global__ __launch_bounds__(kWG) void k_cell_rank_order(GridGeom g, const int *segStart, const int *segEnd,
-                                                             const float4 *xm, const unsigned *order, unsigned *tmp,
+    __global__ __launch_bounds__(kWG) void k_cell_rank_order(BatchArgs b, const unsigned *order, unsigned *tmp,
                                                              unsigned *orderOut)
     {
         __shared__ int cnt[kWaves][64];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(g, t, segStart, segEnd, ti))
-            return;
-        int *C = cnt[wave];
-        C[lane] = 0;
-        wave_sync();
-        const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-        for (int base = ti.start; base < ti.end; base += 64)
+        const float4 *xm = b.P.xm;
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg += tile_stride())
         {
-            const int k = base + lane;
-            const bool valid = k < ti.end;
-            unsigned cell = 64u;
-            if (valid)
+            int t;
+            const StepCommon sv = scene_view(b, tg, t);
+            const GridGeom &g = sv.g;
+            TileInfo ti;
+            if (!tile_info(g, t, sv.segStart, sv.segEnd, ti))
+                continue;
+            wave_sync();
+            int *C = cnt[wave];
+            C[lane] = 0;
+            wave_sync();
+            const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+            for (int base = ti.start; base < ti.end; base += 64)
             {
-                int bx, by, bz;
-                V3 fx;
-                cell_of(g, v3(xm[order[k]]), bx, by, bz, fx);
-                bx = min(max(bx, 0), g.nx - 1);
-                by = min(max(by, 0), g.ny - 1);
-                bz = min(max(bz, 0), g.nz - 1);
-                cell = (unsigned)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
+                const int k = base + lane;
+                const bool valid = k < ti.end;
+                unsigned cell = 64u;
+                if (valid)
+                {
+                    int bx, by, bz;
+                    V3 fx;
+                    cell_of(g, v3(xm[order[k]]), bx, by, bz, fx);
+                    bx = min(max(bx, 0), g.nx - 1);
+                    by = min(max(by, 0), g.ny - 1);
+                    bz = min(max(bz, 0), g.nz - 1);
+                    cell = (unsigned)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
+                }
+                const unsigned long long peers = match_lanes(cell);
+                const int below = __popcll(peers & lt);
+                const int r = valid ? C[cell] + below : 0;
+                wave_sync();
+                if (valid && below == 0)
+                    C[cell] += __popcll(peers);
+                wave_sync();
+                if (valid)
+                    tmp[k] = ((unsigned)r << 6) | cell;
             }
-            const unsigned long long peers = match_lanes(cell);
-            const int below = __popcll(peers & lt);
-            const int r = valid ? C[cell] + below : 0;
-            wave_sync();
-            if (valid && below == 0)
-                C[cell] += __popcll(peers);
-            wave_sync();
-            if (valid)
-                tmp[k] = ((unsigned)r << 6) | cell;
-        }
-        const int mine = C[lane];
-        for (int base = ti.start; base < ti.end; base += 64)
-        {
-            const int k = base + lane;
-            const bool valid = k < ti.end;
-            const unsigned rc = valid ? tmp[k] : 0u;
-            const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
-            // particles of lower rank, then rank-r particles of lower cells
-            int pos = 0;
+            const int mine = C[lane];
+            for (int base = ti.start; base < ti.end; base += 64)
+            {
+                const int k = base + lane;
+                const bool valid = k < ti.end;
+                const unsigned rc = valid ? tmp[k] : 0u;
+                const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
+                // particles of lower rank, then rank-r particles of lower cells
+                int pos = 0;
 #pragma unroll 16
-            for (int c = 0; c < 64; ++c)
-            {
-                const int nc = __builtin_amdgcn_readlane(mine, c);
-                pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                for (int c = 0; c < 64; ++c)
+                {
+                    const int nc = __builtin_amdgcn_readlane(mine, c);
+                    pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                }
+                if (valid)
+                    orderOut[ti.start + pos] = order[k];
             }
-            if (valid)
-                orderOut[ti.start + pos] = order[k];
         }
     }
 
@@ -807,84 +928,76 @@ namespace
     // ------------------------------------------------------------------
     // MLS-MPM P2G (src/solver/mls_mpm_algorithm.h:147-241)
     // ------------------------------------------------------------------
-    struct StepCommon
-    {
-        GridGeom g;
-        float3 gravity;
-        float dt;
-        ParticleBind P;
-        SolverState S;
-        ShapeView sv;
-        const int *segStart;
-        const int *segEnd;
-        const unsigned *order; // sorted particle indices
-        float4 *tilebuf;
-        float4 *grid;
-    };
 
-    __global__ __launch_bounds__(kWG) void k_mls_p2g(StepCommon c)
+    template <bool Persistent>
+    __global__ __launch_bounds__(kWG) void k_mls_p2g(BatchArgs b)
     {
         __shared__ float4 acc[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
-            return;
-        float4 *A = acc[wave];
-        for (int i = lane; i < kTileNodes; i += 64)
-            A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        wave_sync();
-        const int nSlicers = c.sv.counts[1];
-        const GridGeom &g = c.g;
-        for (int k = ti.start + lane; k < ti.end; k += 64)
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
         {
-            const int p = (int)c.order[k];
-            const float4 xm = c.P.xm[p];
-            const V3 x = v3(xm);
-            const float m = xm.w;
-            int bx, by, bz;
-            V3 fx;
-            cell_of(g, x, bx, by, bz, fx);
-            Weights W;
-            weights_of(fx, W);
-            float4 vv = c.P.v[p];
-            const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-            c.P.v[p] = make_float4(vel.x, vel.y, vel.z, vv.w);
-            const M3 F = load_m3(c.S.F, c.S.stride, p);
-            const M3 Cm = load_m3(c.S.C, c.S.stride, p);
-            const float4 prm = c.P.params[p];
-            const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], F);
-            const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
-            const M3 affine = stress * sc + Cm * m;
-            const V3 vm = vel * m;
-            const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
-            in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
-                M3 af = affine;
-                V3 v_m = vm, f = fx;
-                Weights w = W;
-                opaque(af);
-                opaque(v_m);
-                opaque(f);
-                opaque(w);
-                auto sc = [&](int li, int, int, int, V3 dist, float wt) {
-                    const V3 contrib = (v_m + mul(af, dist)) * wt;
-                    lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
-                };
-                if (fast)
-                    for_each_node<true>(g, ti, bx, by, bz, f, w, sc);
-                else
-                    for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
-                        V3 nrm;
-                        if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                            return;
-                        sc(li, a, b, cc, dist, wt);
-                    });
-            });
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            wave_sync(); // the previous tile's LDS reads are done
+            float4 *A = acc[wave];
+            for (int i = lane; i < kTileNodes; i += 64)
+                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            wave_sync();
+            const int nSlicers = c.sv.counts[1];
+            const GridGeom &g = c.g;
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                const float4 xm = c.P.xm[p];
+                const V3 x = v3(xm);
+                const float m = xm.w;
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, x, bx, by, bz, fx);
+                Weights W;
+                weights_of(fx, W);
+                float4 vv = c.P.v[p];
+                const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                c.P.v[p] = make_float4(vel.x, vel.y, vel.z, vv.w);
+                const M3 F = load_m3(c.S.F, c.S.stride, p);
+                const M3 Cm = load_m3(c.S.C, c.S.stride, p);
+                const float4 prm = c.P.params[p];
+                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], F);
+                const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
+                const M3 affine = stress * sc + Cm * m;
+                const V3 vm = vel * m;
+                const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
+                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                    M3 af = affine;
+                    V3 v_m = vm, f = fx;
+                    Weights w = W;
+                    opaque(af);
+                    opaque(v_m);
+                    opaque(f);
+                    opaque(w);
+                    auto sc = [&](int li, int, int, int, V3 dist, float wt) {
+                        const V3 contrib = (v_m + mul(af, dist)) * wt;
+                        lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
+                    };
+                    if (fast)
+                        for_each_node<true>(g, ti, bx, by, bz, f, w, sc);
+                    else
+                        for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                            V3 nrm;
+                            if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                                return;
+                            sc(li, a, b, cc, dist, wt);
+                        });
+                });
+            }
+            wave_sync();
+            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+            for (int i = lane; i < kTileNodes; i += 64)
+                out[i] = A[i];
         }
-        wave_sync();
-        float4 *out = c.tilebuf + (size_t)t * kTileNodes;
-        for (int i = lane; i < kTileNodes; i += 64)
-            out[i] = A[i];
     }
 
     // ------------------------------------------------------------------
@@ -893,88 +1006,90 @@ namespace
     // (src/solver/mls_mpm_algorithm.h:243-294, pb 233-284, standard 217-272).
     // ------------------------------------------------------------------
     template <int Kind> // 0 MLS, 1 PB, 2 standard
-    __global__ __launch_bounds__(kWG) void k_update(StepCommon c, const float4 *tileForce, unsigned char *dirty)
+    __global__ __launch_bounds__(kWG) void k_update(BatchArgs b, unsigned char *dirty)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int nb = blockIdx.x * kWaves + wave;
-        const GridGeom &g = c.g;
-        if (nb >= g.numBlocks)
-            return;
-        int bx, by, bz;
-        decode_block(g, nb, bx, by, bz);
-        // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz).
-        // All occupancy loads are issued before any partial is read.
-        bool src[8];
-        bool any = false;
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
+        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg += tile_stride())
         {
-            const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
-            const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
-            src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
-            any = any || src[d];
-        }
-        if (!any)
-            return;
-        const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
-        float4 q[8], qf[8];
+            int nb;
+            const StepCommon c = scene_view(b, nbg, nb);
+            const GridGeom &g = c.g;
+            int bx, by, bz;
+            decode_block(g, nb, bx, by, bz);
+            // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz).
+            // All occupancy loads are issued before any partial is read.
+            bool src[8];
+            bool any = false;
 #pragma unroll
-        for (int d = 0; d < 8; ++d)
-        {
-            const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
-            const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
-            const bool use = src[d] && ux < kTN && uy < kTN && uz < kTN;
-            const size_t at = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
-            q[d] = use ? c.tilebuf[at] : make_float4(0.f, 0.f, 0.f, 0.f);
-            if (Kind == 2)
-                qf[d] = use ? tileForce[at] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        // fixed summation order d = 0..7 (bitwise reproducible)
-        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
-        {
-            s.x += q[d].x;
-            s.y += q[d].y;
-            s.z += q[d].z;
-            s.w += q[d].w;
-            if (Kind == 2)
+            for (int d = 0; d < 8; ++d)
             {
-                f.x += qf[d].x;
-                f.y += qf[d].y;
-                f.z += qf[d].z;
+                const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
+                const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
+                src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
+                any = any || src[d];
             }
-        }
-        if (lane == 0)
-            dirty[nb] = 1;
-        const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
-        if (!in_grid(g, i, j, k))
-            return;
-        const int gi = nb * 64 + lane;
-        const float cutoff = (Kind == 1) ? 1e-8f : 1e-4f;
-        if (!(s.w < cutoff))
-        {
-            if (Kind == 2)
+            if (!any)
+                continue;
+            const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+            float4 q[8], qf[8];
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
             {
-                s.x += f.x * c.dt;
-                s.y += f.y * c.dt;
-                s.z += f.z * c.dt;
+                const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+                const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
+                const bool use = src[d] && ux < kTN && uy < kTN && uz < kTN;
+                const size_t at = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
+                q[d] = use ? c.tilebuf[at] : make_float4(0.f, 0.f, 0.f, 0.f);
+                if (Kind == 2)
+                    qf[d] = use ? c.tileForce[at] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            s.x /= s.w;
-            s.y /= s.w;
-            s.z /= s.w;
-            V3 np = V3{(float)i * g.dx, (float)j * g.dx, (float)k * g.dx} + v3(g.bmin.x, g.bmin.y, g.bmin.z);
-            np = np + v3(s) * c.dt;
-            resolve_rigid<false>(c.sv, c.sv.counts[0], s, np);
-            if (i < 2 || i > g.nx - 3)
-                s.x = 0.0f;
-            if (j < 2 || j > g.ny - 3)
-                s.y = 0.0f;
-            if (k < 2 || k > g.nz - 3)
-                s.z = 0.0f;
+            // fixed summation order d = 0..7 (bitwise reproducible)
+            float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+            {
+                s.x += q[d].x;
+                s.y += q[d].y;
+                s.z += q[d].z;
+                s.w += q[d].w;
+                if (Kind == 2)
+                {
+                    f.x += qf[d].x;
+                    f.y += qf[d].y;
+                    f.z += qf[d].z;
+                }
+            }
+            if (lane == 0)
+                dirty[nbg] = 1;
+            const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
+            if (!in_grid(g, i, j, k))
+                continue;
+            const int gi = nb * 64 + lane;
+            const float cutoff = (Kind == 1) ? 1e-8f : 1e-4f;
+            if (!(s.w < cutoff))
+            {
+                if (Kind == 2)
+                {
+                    s.x += f.x * c.dt;
+                    s.y += f.y * c.dt;
+                    s.z += f.z * c.dt;
+                }
+                s.x /= s.w;
+                s.y /= s.w;
+                s.z /= s.w;
+                V3 np = V3{(float)i * g.dx, (float)j * g.dx, (float)k * g.dx} + v3(g.bmin.x, g.bmin.y, g.bmin.z);
+                np = np + v3(s) * c.dt;
+                resolve_rigid<false>(c.sv, c.sv.counts[0], s, np);
+                if (i < 2 || i > g.nx - 3)
+                    s.x = 0.0f;
+                if (j < 2 || j > g.ny - 3)
+                    s.y = 0.0f;
+                if (k < 2 || k > g.nz - 3)
+                    s.z = 0.0f;
+            }
+            c.grid[gi] = s;
         }
-        c.grid[gi] = s;
     }
 
     // Gather of v and C from the LDS tile (src/solver/mls_mpm_algorithm.h:318-371)
@@ -1017,35 +1132,41 @@ namespace
     }
 
     // MLS G2P (src/solver/mls_mpm_algorithm.h:296-390)
-    __global__ __launch_bounds__(kWG) void k_mls_g2p(StepCommon c)
+    template <bool Persistent>
+    __global__ __launch_bounds__(kWG) void k_mls_g2p(BatchArgs b)
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
-            return;
-        float4 *G = lg[wave];
-        load_tile_grid(c.g, ti, c.grid, G, lane);
-        wave_sync();
-        const int nSlicers = c.sv.counts[1];
-        const int nShapes = c.sv.counts[0];
-        for (int k = ti.start + lane; k < ti.end; k += 64)
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
         {
-            const int p = (int)c.order[k];
-            const float4 xm = c.P.xm[p];
-            V3 x = v3(xm);
-            V3 vel;
-            M3 Cm;
-            gather_vc(c, ti, G, nSlicers, x, v3(c.P.v[p]), vel, Cm);
-            x = clamp_bounds(c.g, x + vel * c.dt);
-            float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
-            resolve_rigid<true>(c.sv, nShapes, vv, x);
-            c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
-            c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-            store_m3(c.S.C, c.S.stride, p, Cm);
-            const M3 F = load_m3(c.S.F, c.S.stride, p);
-            store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            wave_sync(); // the previous tile's LDS reads are done
+            float4 *G = lg[wave];
+            load_tile_grid(c.g, ti, c.grid, G, lane);
+            wave_sync();
+            const int nSlicers = c.sv.counts[1];
+            const int nShapes = c.sv.counts[0];
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                const float4 xm = c.P.xm[p];
+                V3 x = v3(xm);
+                V3 vel;
+                M3 Cm;
+                gather_vc(c, ti, G, nSlicers, x, v3(c.P.v[p]), vel, Cm);
+                x = clamp_bounds(c.g, x + vel * c.dt);
+                float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                resolve_rigid<true>(c.sv, nShapes, vv, x);
+                c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+                c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                store_m3(c.S.C, c.S.stride, p, Cm);
+                const M3 F = load_m3(c.S.F, c.S.stride, p);
+                store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+            }
         }
     }
 
@@ -1133,91 +1254,97 @@ namespace
     // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
     // iteration fused with P2G of this one.  Mode 2: last G2P fused with
     // pbMpmIntegrateParticle, results written back in particle order.
-    template <int Mode>
-    __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(StepCommon c, PbStage st, float3 gravity)
+    template <int Mode, bool Persistent>
+    __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
         __shared__ float4 acc[(Mode < 2) ? kWaves : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
-            return;
-        float4 *A = acc[(Mode < 2) ? wave : 0];
-        float4 *G = lg[(Mode > 0) ? wave : 0];
-        if (Mode < 2)
-            for (int i = lane; i < kTileNodes; i += 64)
-                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (Mode > 0)
-            load_tile_grid(c.g, ti, c.grid, G, lane);
-        wave_sync();
-        const int nSlicers = c.sv.counts[1];
-        const int nShapes = c.sv.counts[0];
-        const int S = st.stride;
-        for (int k = ti.start + lane; k < ti.end; k += 64)
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
         {
-            const float4 xm = st.xs[k];
-            V3 x = v3(xm);
-            V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
-            M3 Cm;
-            if (Mode == 0)
-                Cm = load_m3(st.Cs, S, k);
-            else
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            wave_sync(); // the previous tile's LDS reads are done
+            float4 *A = acc[(Mode < 2) ? wave : 0];
+            float4 *G = lg[(Mode > 0) ? wave : 0];
+            if (Mode < 2)
+                for (int i = lane; i < kTileNodes; i += 64)
+                    A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (Mode > 0)
+                load_tile_grid(c.g, ti, c.grid, G, lane);
+            wave_sync();
+            const int nSlicers = c.sv.counts[1];
+            const int nShapes = c.sv.counts[0];
+            const int S = st.stride;
+            for (int k = ti.start + lane; k < ti.end; k += 64)
             {
-                V3 vnew;
-                gather_vc(c, ti, G, nSlicers, x, vel, vnew, Cm);
-                vel = vnew;
+                const float4 xm = st.xs[k];
+                V3 x = v3(xm);
+                V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
+                M3 Cm;
+                if (Mode == 0)
+                    Cm = load_m3(st.Cs, S, k);
+                else
+                {
+                    V3 vnew;
+                    gather_vc(c, ti, G, nSlicers, x, vel, vnew, Cm);
+                    vel = vnew;
+                }
+                if (Mode < 2)
+                {
+                    const M3 F = load_m3(st.Fs, S, k);
+                    const float2 pr = st.ps[k];
+                    pb_constraint(c.dt, pr.x, pr.y, st.ts[k], F, Cm);
+                    if (Mode == 1 && nSlicers > 0)
+                    {
+                        st.vs[k] = vel.x;
+                        st.vs[S + k] = vel.y;
+                        st.vs[2 * S + k] = vel.z;
+                    }
+                    pb_scatter(c, ti, A, nSlicers, x, xm.w, vel, Cm);
+                }
+                else
+                {
+                    // pbMpmIntegrateParticle (src/solver/pb_mpm_algorithm.h:363-408)
+                    const int p = (int)c.order[k];
+                    x = clamp_bounds(c.g, x + vel * c.dt);
+                    vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                    float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                    resolve_rigid<true>(c.sv, nShapes, vv, x);
+                    c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+                    c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                    store_m3(c.S.C, c.S.stride, p, Cm);
+                    const M3 F = load_m3(st.Fs, S, k);
+                    store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+                }
             }
             if (Mode < 2)
             {
-                const M3 F = load_m3(st.Fs, S, k);
-                const float2 pr = st.ps[k];
-                pb_constraint(c.dt, pr.x, pr.y, st.ts[k], F, Cm);
-                if (Mode == 1 && nSlicers > 0)
-                {
-                    st.vs[k] = vel.x;
-                    st.vs[S + k] = vel.y;
-                    st.vs[2 * S + k] = vel.z;
-                }
-                pb_scatter(c, ti, A, nSlicers, x, xm.w, vel, Cm);
+                wave_sync();
+                float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+                for (int i = lane; i < kTileNodes; i += 64)
+                    out[i] = A[i];
             }
-            else
-            {
-                // pbMpmIntegrateParticle (src/solver/pb_mpm_algorithm.h:363-408)
-                const int p = (int)c.order[k];
-                x = clamp_bounds(c.g, x + vel * c.dt);
-                vel = vel + v3(gravity.x, gravity.y, gravity.z) * c.dt;
-                float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                resolve_rigid<true>(c.sv, nShapes, vv, x);
-                c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
-                c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                store_m3(c.S.C, c.S.stride, p, Cm);
-                const M3 F = load_m3(st.Fs, S, k);
-                store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
-            }
-        }
-        if (Mode < 2)
-        {
-            wave_sync();
-            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
-            for (int i = lane; i < kTileNodes; i += 64)
-                out[i] = A[i];
         }
     }
 
     // Zero iterations: integrate with the current v and C.
-    __global__ void k_pb_integrate_only(StepCommon c, int n, const unsigned *keys, PbStage st, float3 gravity)
+    __global__ void k_pb_integrate_only(BatchArgs b, int n, const unsigned *keys, PbStage st)
     {
         const int k = blockIdx.x * blockDim.x + threadIdx.x;
-        if (k >= n || keys[k] >= (unsigned)c.g.numBlocks)
+        if (k >= n || keys[k] >= (unsigned)b.totalBlocks)
             return;
+        const StepCommon c = scene_view_of(b, b.blockScene[keys[k]]);
         const int S = st.stride;
         const int p = (int)c.order[k];
         const float4 xm = st.xs[k];
         V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
         const M3 Cm = load_m3(st.Cs, S, k);
         V3 x = clamp_bounds(c.g, v3(xm) + vel * c.dt);
-        vel = vel + v3(gravity.x, gravity.y, gravity.z) * c.dt;
+        vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
         float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
         resolve_rigid<true>(c.sv, c.sv.counts[0], vv, x);
         c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
@@ -1229,118 +1356,130 @@ namespace
     // ------------------------------------------------------------------
     // Standard MPM (src/solver/standard_mpm_algorithm.h:146-352)
     // ------------------------------------------------------------------
-    __global__ __launch_bounds__(kWG) void k_std_p2g(StepCommon c, float4 *tileForce)
+    template <bool Persistent>
+    __global__ __launch_bounds__(kWG) void k_std_p2g(BatchArgs b)
     {
         __shared__ float4 acc[kWaves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
-            return;
-        float4 *A = acc[wave][0];
-        float4 *AF = acc[wave][1];
-        for (int i = lane; i < kTileNodes; i += 64)
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
         {
-            A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            AF[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        wave_sync();
-        const GridGeom &g = c.g;
-        const V3 grav = v3(c.gravity.x, c.gravity.y, c.gravity.z);
-        for (int k = ti.start + lane; k < ti.end; k += 64)
-        {
-            const int p = (int)c.order[k];
-            const float4 xm = c.P.xm[p];
-            const float m = xm.w;
-            int bx, by, bz;
-            V3 fx;
-            cell_of(g, v3(xm), bx, by, bz, fx);
-            Weights W;
-            weights_of(fx, W);
-            const float4 prm = c.P.params[p];
-            const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_m3(c.S.F, c.S.stride, p));
-            const V3 vel = v3(c.P.v[p]);
-            const float nv0 = -c.S.V0[p];
-            const bool fast = nodes_inside(g, bx, by, bz);
-            in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
-                M3 st = stress;
-                V3 v = vel, f = fx;
-                Weights w = W;
-                opaque(st);
-                opaque(v);
-                opaque(f);
-                opaque(w);
-                auto scatter = [&](int li, int a, int b, int cc, V3, float wt) {
-                    const float wa = w_of(f.x, a), wb = w_of(f.y, b), wc = w_of(f.z, cc);
-                    const V3 wg{dw_of(f.x, a) * wb * wc, wa * dw_of(f.y, b) * wc, wa * wb * dw_of(f.z, cc)};
-                    const float wm = wt * m;
-                    const V3 ext = grav * wm;
-                    const V3 inr = mul(st, wg) * nv0;
-                    const V3 mom = v * wm;
-                    const V3 fo = ext + inr;
-                    lds_add(A, li, mom.x, mom.y, mom.z, wm);
-                    lds_add(AF, li, fo.x, fo.y, fo.z, 0.0f);
-                };
-                if (fast)
-                    for_each_node<true>(g, ti, bx, by, bz, f, w, scatter);
-                else
-                    for_each_node<false>(g, ti, bx, by, bz, f, w, scatter);
-            });
-        }
-        wave_sync();
-        float4 *out = c.tilebuf + (size_t)t * kTileNodes;
-        float4 *outF = tileForce + (size_t)t * kTileNodes;
-        for (int i = lane; i < kTileNodes; i += 64)
-        {
-            out[i] = A[i];
-            outF[i] = AF[i];
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            wave_sync(); // the previous tile's LDS reads are done
+            float4 *A = acc[wave][0];
+            float4 *AF = acc[wave][1];
+            for (int i = lane; i < kTileNodes; i += 64)
+            {
+                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                AF[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            wave_sync();
+            const GridGeom &g = c.g;
+            const V3 grav = v3(c.gravity.x, c.gravity.y, c.gravity.z);
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                const float4 xm = c.P.xm[p];
+                const float m = xm.w;
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, v3(xm), bx, by, bz, fx);
+                Weights W;
+                weights_of(fx, W);
+                const float4 prm = c.P.params[p];
+                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_m3(c.S.F, c.S.stride, p));
+                const V3 vel = v3(c.P.v[p]);
+                const float nv0 = -c.S.V0[p];
+                const bool fast = nodes_inside(g, bx, by, bz);
+                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                    M3 st = stress;
+                    V3 v = vel, f = fx;
+                    Weights w = W;
+                    opaque(st);
+                    opaque(v);
+                    opaque(f);
+                    opaque(w);
+                    auto scatter = [&](int li, int a, int b, int cc, V3, float wt) {
+                        const float wa = w_of(f.x, a), wb = w_of(f.y, b), wc = w_of(f.z, cc);
+                        const V3 wg{dw_of(f.x, a) * wb * wc, wa * dw_of(f.y, b) * wc, wa * wb * dw_of(f.z, cc)};
+                        const float wm = wt * m;
+                        const V3 ext = grav * wm;
+                        const V3 inr = mul(st, wg) * nv0;
+                        const V3 mom = v * wm;
+                        const V3 fo = ext + inr;
+                        lds_add(A, li, mom.x, mom.y, mom.z, wm);
+                        lds_add(AF, li, fo.x, fo.y, fo.z, 0.0f);
+                    };
+                    if (fast)
+                        for_each_node<true>(g, ti, bx, by, bz, f, w, scatter);
+                    else
+                        for_each_node<false>(g, ti, bx, by, bz, f, w, scatter);
+                });
+            }
+            wave_sync();
+            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+            float4 *outF = c.tileForce + (size_t)t * kTileNodes;
+            for (int i = lane; i < kTileNodes; i += 64)
+            {
+                out[i] = A[i];
+                outF[i] = AF[i];
+            }
         }
     }
 
-    __global__ __launch_bounds__(kWG) void k_std_g2p(StepCommon c)
+    template <bool Persistent>
+    __global__ __launch_bounds__(kWG) void k_std_g2p(BatchArgs b)
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        TileInfo ti;
-        const int t = blockIdx.x * kWaves + wave;
-        if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
-            return;
-        float4 *G = lg[wave];
-        load_tile_grid(c.g, ti, c.grid, G, lane);
-        wave_sync();
-        const GridGeom &g = c.g;
-        const int nShapes = c.sv.counts[0];
-        for (int k = ti.start + lane; k < ti.end; k += 64)
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
         {
-            const int p = (int)c.order[k];
-            if (!c.P.mask[p])
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
                 continue;
-            const float4 xm = c.P.xm[p];
-            int bx, by, bz;
-            V3 fx;
-            cell_of(g, v3(xm), bx, by, bz, fx);
-            Weights W;
-            weights_of(fx, W);
-            V3 vacc{0, 0, 0};
-            M3 gv{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
-            auto gather = [&](int li, int a, int b, int cc, V3, float wt) {
-                const V3 nv = v3(G[li]);
-                vacc = vacc + nv * wt;
-                const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, cc);
-                const V3 wg{dw_of(fx.x, a) * wb * wc, wa * dw_of(fx.y, b) * wc, wa * wb * dw_of(fx.z, cc)};
-                gv = gv + outer_ref(wg, nv);
-            };
-            if (nodes_inside(g, bx, by, bz))
-                for_each_node<true>(g, ti, bx, by, bz, fx, W, gather);
-            else
-                for_each_node<false>(g, ti, bx, by, bz, fx, W, gather);
-            V3 x = clamp_bounds(g, v3(xm) + vacc * c.dt);
-            float4 vv = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
-            resolve_rigid<true>(c.sv, nShapes, vv, x);
-            c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
-            c.P.v[p] = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
-            const M3 F = load_m3(c.S.F, c.S.stride, p);
-            store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + gv * c.dt, F));
+            wave_sync(); // the previous tile's LDS reads are done
+            float4 *G = lg[wave];
+            load_tile_grid(c.g, ti, c.grid, G, lane);
+            wave_sync();
+            const GridGeom &g = c.g;
+            const int nShapes = c.sv.counts[0];
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                if (!c.P.mask[p])
+                    continue;
+                const float4 xm = c.P.xm[p];
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, v3(xm), bx, by, bz, fx);
+                Weights W;
+                weights_of(fx, W);
+                V3 vacc{0, 0, 0};
+                M3 gv{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
+                auto gather = [&](int li, int a, int b, int cc, V3, float wt) {
+                    const V3 nv = v3(G[li]);
+                    vacc = vacc + nv * wt;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, cc);
+                    const V3 wg{dw_of(fx.x, a) * wb * wc, wa * dw_of(fx.y, b) * wc, wa * wb * dw_of(fx.z, cc)};
+                    gv = gv + outer_ref(wg, nv);
+                };
+                if (nodes_inside(g, bx, by, bz))
+                    for_each_node<true>(g, ti, bx, by, bz, fx, W, gather);
+                else
+                    for_each_node<false>(g, ti, bx, by, bz, fx, W, gather);
+                V3 x = clamp_bounds(g, v3(xm) + vacc * c.dt);
+                float4 vv = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
+                resolve_rigid<true>(c.sv, nShapes, vv, x);
+                c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+                c.P.v[p] = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
+                const M3 F = load_m3(c.S.F, c.S.stride, p);
+                store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + gv * c.dt, F));
+            }
         }
     }
 
@@ -1348,42 +1487,69 @@ namespace
     // computeInitialData (src/solver/mls_mpm_algorithm.h:46-145): mass onto the
     // grid as it stands (global atomics, rarely called), then V0.
     // ------------------------------------------------------------------
-    __global__ void k_init_mass(GridGeom g, unsigned n, const unsigned *idx, const float4 *xm, SolverState S, float4 *grid,
-                                unsigned char *dirty)
+    // computeInitialData's mass scatter (src/solver/mls_mpm_algorithm.h:46-101)
+    // in 2^-40 kg fixed point: integer atomics add in any order to the same
+    // sum, so the initial volumes are bitwise reproducible; the sum is added
+    // to the grid's existing mass once per node by k_init_apply.
+    constexpr double kMassFixed = 1099511627776.0; // 2^40
+
+    // idx: scene-local particle indices of scene s; acc: one counter per node
+    // of the scene (scene-local node index), zeroed by the caller.
+    __global__ void k_init_mass(BatchArgs b, int s, unsigned n, const unsigned *idx, unsigned long long *acc)
     {
         const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
-        const int p = (int)idx[i];
-        store_m3(S.F, S.stride, p, m3_identity());
-        const float4 x = xm[p];
+        const StepCommon c = scene_view_of(b, s);
+        const GridGeom &g = c.g;
+        const int p = b.scenes[s].particleOffset + (int)idx[i];
+        store_m3(c.S.F, c.S.stride, p, m3_identity());
+        const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
         cell_of(g, v3(x), bx, by, bz, fx);
         Weights W;
         weights_of(fx, W);
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                for (int c = 0; c < 3; ++c)
+            for (int bb = 0; bb < 3; ++bb)
+                for (int cc = 0; cc < 3; ++cc)
                 {
-                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    const int nx = bx + a, ny = by + bb, nz = bz + cc;
                     if (!in_grid(g, nx, ny, nz))
                         continue;
-                    const float wt = W.w[a][0] * W.w[b][1] * W.w[c][2];
-                    atomicAdd(&grid[grid_index(g, nx, ny, nz)].w, wt * x.w);
-                    dirty[block_key(g, nx >> 2, ny >> 2, nz >> 2)] = 1;
+                    const float wt = W.w[a][0] * W.w[bb][1] * W.w[cc][2];
+                    const long long q = llrint((double)(wt * x.w) * kMassFixed);
+                    atomicAdd(&acc[grid_index(g, nx, ny, nz)], (unsigned long long)q);
                 }
     }
 
-    __global__ void k_init_volume(GridGeom g, unsigned n, const unsigned *idx, const float4 *xm, SolverState S,
-                                  const float4 *grid)
+    // Add the fixed-point mass sums to the grid (one wavefront per node block).
+    __global__ __launch_bounds__(kWG) void k_init_apply(BatchArgs b, int s, const unsigned long long *acc,
+                                                        unsigned char *dirty)
+    {
+        const StepCommon c = scene_view_of(b, s);
+        const int blk = blockIdx.x * kWaves + (threadIdx.x >> 6);
+        if (blk >= c.g.numBlocks)
+            return;
+        const int gi = blk * 64 + (threadIdx.x & 63);
+        const long long q = (long long)acc[gi];
+        const bool touched = q != 0;
+        if (touched)
+            c.grid[gi].w += (float)((double)q / kMassFixed);
+        if (__ballot(touched) != 0ull && (threadIdx.x & 63) == 0)
+            dirty[b.scenes[s].blockOffset + blk] = 1;
+    }
+
+    __global__ void k_init_volume(BatchArgs b, int s, unsigned n, const unsigned *idx)
     {
         const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= n)
             return;
-        const int p = (int)idx[i];
-        store_m3(S.F, S.stride, p, m3_identity());
-        const float4 x = xm[p];
+        const StepCommon c = scene_view_of(b, s);
+        const GridGeom &g = c.g;
+        const int p = b.scenes[s].particleOffset + (int)idx[i];
+        store_m3(c.S.F, c.S.stride, p, m3_identity());
+        const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
         cell_of(g, v3(x), bx, by, bz, fx);
@@ -1391,54 +1557,97 @@ namespace
         weights_of(fx, W);
         float density = 0.0f;
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                for (int c = 0; c < 3; ++c)
+            for (int bb = 0; bb < 3; ++bb)
+                for (int cc = 0; cc < 3; ++cc)
                 {
-                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    const int nx = bx + a, ny = by + bb, nz = bz + cc;
                     if (!in_grid(g, nx, ny, nz))
                         continue;
-                    const float wt = W.w[a][0] * W.w[b][1] * W.w[c][2];
-                    const float nd = grid[grid_index(g, nx, ny, nz)].w / g.cellVol;
+                    const float wt = W.w[a][0] * W.w[bb][1] * W.w[cc][2];
+                    const float nd = c.grid[grid_index(g, nx, ny, nz)].w / g.cellVol;
                     density += nd * wt;
                 }
-        S.V0[p] = (density > 1e-8f) ? x.w / density : 1e-8f;
+        c.S.V0[p] = (density > 1e-8f) ? x.w / density : 1e-8f;
     }
 
-    __global__ void k_copy_grid(GridGeom g, const float4 *grid, float4 *dst)
+    __global__ void k_copy_grid(BatchArgs b, int s, float4 *dst)
     {
+        const StepCommon c = scene_view_of(b, s);
+        const GridGeom &g = c.g;
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
         const int total = g.nx * g.ny * g.nz;
         if (i >= total)
             return;
         const int x = i % g.nx, y = (i / g.nx) % g.ny, z = i / (g.nx * g.ny);
-        dst[i] = grid[grid_index(g, x, y, z)];
+        dst[i] = c.grid[grid_index(g, x, y, z)];
     }
 
     int grid1d(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 } // namespace
 
+
 // ----------------------------------------------------------------------
-// Solver object behind the C-ABI
+// Batch object behind the C-ABI.  A batch is a list of scenes of one solver
+// kind advanced together: every phase of a substep is one launch over all
+// scenes.  The single-scene solver (CrHipSolver) is a batch of one.
 // ----------------------------------------------------------------------
-struct CrHipSolver
+namespace
+{
+    struct HostScene
+    {
+        GridGeom g{};
+        float3 gravity = make_float3(0.f, 0.f, 0.f);
+        float dt = 0.002f; // mpm_solver_base.h default
+        int contactModel = CRMPM_HIP_CONTACT_KINEMATIC;
+        int maxParticles = 0;
+        int particleOffset = 0;
+        unsigned numActive = 0;
+        int numShapes = 0;
+        int *shapeIds = nullptr;
+        CrHipShapeData shapes{};
+        CrHipGeometryData geoms{};
+        CrHipGeometrySdfData sdf{};
+        int blockOffset = 0;
+        int shapeOffset = 0, shapeCap = 0;
+    };
+
+    constexpr int kUploadSlots = 4;
+
+    bool grid_geom(float cellSize, const float lo[3], const float hi[3], GridGeom &g)
+    {
+        g.bmin = make_float3(lo[0], lo[1], lo[2]);
+        g.bmax = make_float3(hi[0], hi[1], hi[2]);
+        g.dx = cellSize;
+        g.invdx = 1 / cellSize;
+        g.cellVol = cellSize * cellSize * cellSize;
+        int n[3];
+        for (int a = 0; a < 3; ++a)
+            n[a] = (int)((hi[a] - lo[a]) / cellSize + 1.0f);
+        g.nx = n[0];
+        g.ny = n[1];
+        g.nz = n[2];
+        if (g.nx < 1 || g.ny < 1 || g.nz < 1)
+            return false;
+        g.nbx = (g.nx + 3) / 4;
+        g.nby = (g.ny + 3) / 4;
+        g.nbz = (g.nz + 3) / 4;
+        g.numBlocks = g.nbx * g.nby * g.nbz;
+        return true;
+    }
+} // namespace
+
+struct CrHipBatch
 {
     int kind = CRMPM_HIP_MLS;
-    int maxParticles = 0;
-    GridGeom g{};
-    float3 gravity = make_float3(0.f, 0.f, 0.f);
-    float dt = 0.002f; // mpm_solver_base.h default
     int iterations = 10;
-    int contactModel = CRMPM_HIP_CONTACT_KINEMATIC;
+    HostScene *scenes = nullptr;
+    int numScenes = 0;
+    long long totalBlocks = 0;
+    int capacity = 0; // particles in the bound arrays
     hipStream_t stream = nullptr;
     bool initialized = false;
-
+    bool scenesDirty = true;
     ParticleBind P{};
-    unsigned numActive = 0;
-    int numShapes = 0;
-    int *shapeIds = nullptr;
-    CrHipShapeData shapes{};
-    CrHipGeometryData geoms{};
-    CrHipGeometrySdfData sdf{};
 
     // owned
     float *F = nullptr, *C = nullptr, *V0 = nullptr;
@@ -1448,18 +1657,44 @@ struct CrHipSolver
     unsigned *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
     unsigned *radixCounts = nullptr, *digitTotals = nullptr;
     float4 *tilebuf = nullptr, *tileForce = nullptr;
-    long long maxTiles = 0;
     ShapeRec *shapeRecs = nullptr;
     int *shapeCounts = nullptr, *slicerList = nullptr;
-    int shapeCap = 0;
+    int shapeCapTotal = 0;
+    SceneRec *dScenes = nullptr;
+    SceneShapes *dShapes = nullptr;
+    int *dActivePrefix = nullptr;
+    int *dBlockScene = nullptr;
     PbStage st{};
     float *stageMem = nullptr;
 
-    // optional per-launch event timing (crHipSolverProfileBegin/End)
+    // pinned staging of the per-scene tables (a ring, so an upload never
+    // overwrites one the stream has not consumed yet)
+    char *hStage = nullptr;
+    size_t stageBytes = 0;
+    hipEvent_t stageDone[kUploadSlots] = {};
+    int stageSlot = 0;
+    int totalActive = 0;
+
+    // launch sizes in wavefronts; 0 = one wavefront per pblock
+    int tileWaves = 0, updateWaves = 0;
+
+    // hipGraph replay of the substep
+    bool graphMode = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    long long graphKey[5] = {-1, -1, -1, -1, -1};
+    long long bindEpoch = 0;
+
+    // optional per-launch event timing (ProfileBegin/End)
     int profMax = 0, profUsed = 0;
     unsigned profMask = ~0u;
     hipEvent_t *profEvents = nullptr;
     int *profCat = nullptr;
+};
+
+struct CrHipSolver
+{
+    CrHipBatch *b = nullptr;
 };
 
 namespace
@@ -1489,73 +1724,163 @@ namespace
         return hipGetLastError() == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
 
-    int ensure_shape_capacity(CrHipSolver *s, int n)
+    bool valid_scene(const CrHipBatch *b, int s) { return b && s >= 0 && s < b->numScenes; }
+
+    // Shape record regions: scene s owns [shapeOffset, shapeOffset + shapeCap).
+    int ensure_shape_capacity(CrHipBatch *b)
     {
-        if (n <= s->shapeCap && s->shapeRecs)
+        bool grow = !b->shapeRecs;
+        for (int s = 0; s < b->numScenes; ++s)
+            grow = grow || b->scenes[s].numShapes > b->scenes[s].shapeCap;
+        if (!grow)
             return CRMPM_HIP_OK;
-        const int cap = std::max(n, 16);
-        dfree(s->shapeRecs);
-        dfree(s->slicerList);
-        if (dalloc(&s->shapeRecs, cap) || dalloc(&s->slicerList, cap))
+        if (b->initialized)
+            (void)hipStreamSynchronize(b->stream);
+        int total = 0;
+        for (int s = 0; s < b->numScenes; ++s)
+        {
+            HostScene &h = b->scenes[s];
+            h.shapeCap = std::max(h.shapeCap, std::max(h.numShapes, 16));
+            h.shapeOffset = total;
+            total += h.shapeCap;
+        }
+        dfree(b->shapeRecs);
+        dfree(b->slicerList);
+        if (dalloc(&b->shapeRecs, total) || dalloc(&b->slicerList, total))
             return CRMPM_HIP_ERR_ALLOC;
-        s->shapeCap = cap;
+        b->shapeCapTotal = total;
+        b->scenesDirty = true;
+        ++b->bindEpoch;
         return CRMPM_HIP_OK;
     }
 
-    ShapeView shape_view(const CrHipSolver *s)
+    ShapeView shape_view(const CrHipBatch *b, int s)
     {
+        const HostScene &h = b->scenes[s];
         ShapeView v;
-        v.recs = s->shapeRecs;
-        v.counts = s->shapeCounts;
-        v.slicers = s->slicerList;
-        v.force = reinterpret_cast<float4 *>(s->shapes.force);
-        v.torque = reinterpret_cast<float4 *>(s->shapes.torque);
-        v.sdfDims = reinterpret_cast<const int4 *>(s->sdf.dimension);
-        v.sdfLbcs = reinterpret_cast<const float4 *>(s->sdf.lowerBoundCellSize);
-        v.sdfData = reinterpret_cast<const float4 *>(s->sdf.gradientSignedDistance);
-        v.effectiveMass = s->contactModel == CRMPM_HIP_CONTACT_EFFECTIVE_MASS;
+        v.recs = b->shapeRecs + h.shapeOffset;
+        v.counts = b->shapeCounts + 4 * s;
+        v.slicers = b->slicerList + h.shapeOffset;
+        v.force = reinterpret_cast<float4 *>(h.shapes.force);
+        v.torque = reinterpret_cast<float4 *>(h.shapes.torque);
+        v.sdfDims = reinterpret_cast<const int4 *>(h.sdf.dimension);
+        v.sdfLbcs = reinterpret_cast<const float4 *>(h.sdf.lowerBoundCellSize);
+        v.sdfData = reinterpret_cast<const float4 *>(h.sdf.gradientSignedDistance);
+        v.effectiveMass = h.contactModel == CRMPM_HIP_CONTACT_EFFECTIVE_MASS;
         return v;
     }
 
-    // Kernel launch on the solver stream.  While profiling, the dispatch itself
+    // Upload the per-scene records, shape bindings and active-particle prefix
+    // when any of them changed since the last step.
+    int upload_scenes(CrHipBatch *b)
+    {
+        if (!b->scenesDirty)
+            return CRMPM_HIP_OK;
+        int rc = ensure_shape_capacity(b);
+        if (rc)
+            return rc;
+        const int ns = b->numScenes;
+        char *slot = b->hStage + (size_t)b->stageSlot * b->stageBytes;
+        if (hipEventSynchronize(b->stageDone[b->stageSlot]) != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
+        SceneRec *rec = reinterpret_cast<SceneRec *>(slot);
+        SceneShapes *sh = reinterpret_cast<SceneShapes *>(slot + ns * sizeof(SceneRec));
+        int *prefix = reinterpret_cast<int *>(slot + ns * (sizeof(SceneRec) + sizeof(SceneShapes)));
+        int active = 0;
+        for (int s = 0; s < ns; ++s)
+        {
+            const HostScene &h = b->scenes[s];
+            rec[s].g = h.g;
+            rec[s].gravity = h.gravity;
+            rec[s].dt = h.dt;
+            rec[s].sv = shape_view(b, s);
+            rec[s].blockOffset = h.blockOffset;
+            rec[s].particleOffset = h.particleOffset;
+            rec[s].numActive = (int)h.numActive;
+            rec[s].activeOffset = active;
+            sh[s].numShapes = h.numShapes;
+            sh[s].shapeIds = h.shapeIds;
+            sh[s].sd = h.shapes;
+            sh[s].gd = h.geoms;
+            prefix[s] = active;
+            active += (int)h.numActive;
+        }
+        prefix[ns] = active;
+        b->totalActive = active;
+        if (hipMemcpyAsync(b->dScenes, rec, ns * sizeof(SceneRec), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hipMemcpyAsync(b->dShapes, sh, ns * sizeof(SceneShapes), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hipMemcpyAsync(b->dActivePrefix, prefix, (ns + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hipEventRecord(b->stageDone[b->stageSlot], b->stream) != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
+        b->stageSlot = (b->stageSlot + 1) % kUploadSlots;
+        b->scenesDirty = false;
+        return CRMPM_HIP_OK;
+    }
+
+    // Kernel launch on the batch stream.  While profiling, the dispatch itself
     // records a start and a stop event (hipExtLaunchKernelGGL), so the timing
     // covers the kernel's execution only.
     template <typename... Params, typename... Args>
-    void launch(CrHipSolver *s, int cat, void (*kernel)(Params...), dim3 grid, dim3 block, Args... args)
+    void launch(CrHipBatch *b, int cat, void (*kernel)(Params...), dim3 grid, dim3 block, Args... args)
     {
-        if (s->profEvents && ((s->profMask >> cat) & 1u) && s->profUsed < s->profMax)
+        if (b->profEvents && ((b->profMask >> cat) & 1u) && b->profUsed < b->profMax)
         {
-            const int slot = s->profUsed++;
-            s->profCat[slot] = cat;
-            hipExtLaunchKernelGGL(kernel, grid, block, 0, s->stream, s->profEvents[2 * slot], s->profEvents[2 * slot + 1], 0,
+            const int slot = b->profUsed++;
+            b->profCat[slot] = cat;
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, b->stream, b->profEvents[2 * slot], b->profEvents[2 * slot + 1], 0,
                                   static_cast<Params>(args)...);
         }
         else
-            hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, static_cast<Params>(args)...);
+            hipLaunchKernelGGL(kernel, grid, block, 0, b->stream, static_cast<Params>(args)...);
     }
 
-    // Keys per particle and a stable LSD radix sort of (key, index).  Returns
-    // the sorted keys and particle order (one of the two ping-pong buffers).
-    int sort_particles(CrHipSolver *s, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
+    BatchArgs batch_args(const CrHipBatch *b, const unsigned *order)
     {
-        const GridGeom &g = s->g;
+        BatchArgs a;
+        a.scenes = b->dScenes;
+        a.blockScene = b->dBlockScene;
+        a.numScenes = b->numScenes;
+        a.totalBlocks = (int)b->totalBlocks;
+        a.P = b->P;
+        a.S = SolverState{b->F, b->C, b->V0, b->capacity};
+        a.segStart = b->segStart;
+        a.segEnd = b->segEnd;
+        a.order = order;
+        a.tilebuf = b->tilebuf;
+        a.tileForce = b->tileForce;
+        a.grid = b->grid;
+        return a;
+    }
+
+    // Workgroups of a persistent pblock launch (waves = 0: one wavefront per pblock).
+    int pblock_wgs(const CrHipBatch *b, int waves)
+    {
+        const long long all = grid1d(b->totalBlocks, kWaves);
+        return (int)std::max(1ll, waves > 0 ? std::min<long long>(all, grid1d(waves, kWaves)) : all);
+    }
+
+    // Keys per particle and a stable LSD radix sort of (key, particle index).
+    // Returns the sorted keys and particle order (one of the ping-pong buffers).
+    int sort_particles(CrHipBatch *b, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
+    {
         int bits = 0;
-        while ((1ll << bits) <= (long long)g.numBlocks)
+        while ((1ll << bits) <= b->totalBlocks)
             ++bits;
         const int passes = (bits + 7) / 8;
         const int numTiles = grid1d(n, kRadixTile);
-        unsigned *kin = s->keysA, *vin = s->valsA, *kout = s->keysB, *vout = s->valsB;
+        const BatchArgs a = batch_args(b, nullptr);
+        unsigned *kin = b->keysA, *vin = b->valsA, *kout = b->keysB, *vout = b->valsB;
         for (int pass = 0; pass < passes; ++pass)
         {
             const int shift = 8 * pass;
             if (pass == 0)
-                launch(s, CRMPM_HIP_PROF_BINNING, k_bin_hist, numTiles, kWG, g, n, s->P, useMask, kin, vin, s->radixCounts,
-                       numTiles);
+                launch(b, CRMPM_HIP_PROF_BINNING, k_bin_hist, numTiles, kWG, a, b->dActivePrefix, n, useMask, kin, vin,
+                       b->radixCounts, numTiles);
             else
-                launch(s, CRMPM_HIP_PROF_BINNING, k_radix_hist, numTiles, kWG, kin, n, shift, s->radixCounts, numTiles);
-            launch(s, CRMPM_HIP_PROF_BINNING, k_radix_scan, 256, kWG, s->radixCounts, numTiles, s->digitTotals);
-            launch(s, CRMPM_HIP_PROF_BINNING, k_radix_scatter, numTiles, kWG, kin, vin, n, shift, s->radixCounts, numTiles,
-                   s->digitTotals, kout, vout);
+                launch(b, CRMPM_HIP_PROF_BINNING, k_radix_hist, numTiles, kWG, kin, n, shift, b->radixCounts, numTiles);
+            launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scan, 256, kWG, b->radixCounts, numTiles, b->digitTotals);
+            launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scatter, numTiles, kWG, kin, vin, n, shift, b->radixCounts, numTiles,
+                   b->digitTotals, kout, vout);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
@@ -1564,76 +1889,831 @@ namespace
         return check_launch();
     }
 
-    // Bin, sort and segment the particles taking part in this step.
-    int build_tiles(CrHipSolver *s, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
+    // Bin, sort and segment the particles taking part in this step, then
+    // order each tile by (cell rank, cell).
+    int build_tiles(CrHipBatch *b, int n, int useMask, const unsigned **orderOut, const unsigned **keysOut)
     {
-        int rc = sort_particles(s, n, useMask, orderOut, keysOut);
+        int rc = sort_particles(b, n, useMask, orderOut, keysOut);
         if (rc)
             return rc;
-        launch(s, CRMPM_HIP_PROF_BINNING, k_segments, grid1d(n, 256), 256, n, s->g.numBlocks, *keysOut, s->segStart,
-               s->segEnd);
+        launch(b, CRMPM_HIP_PROF_BINNING, k_segments, grid1d(n, 256), 256, n, (int)b->totalBlocks, *keysOut, b->segStart,
+               b->segEnd);
         // the sort's spare ping-pong buffers take the cell ranks and the tile-ordered permutation
-        unsigned *tmp = (*keysOut == s->keysA) ? s->keysB : s->keysA;
-        unsigned *order2 = (*orderOut == s->valsA) ? s->valsB : s->valsA;
-        launch(s, CRMPM_HIP_PROF_BINNING, k_cell_rank_order, grid1d(s->g.numBlocks, kWaves), kWG, s->g, s->segStart,
-               s->segEnd, s->P.xm, *orderOut, tmp, order2);
+        unsigned *tmp = (*keysOut == b->keysA) ? b->keysB : b->keysA;
+        unsigned *order2 = (*orderOut == b->valsA) ? b->valsB : b->valsA;
+        launch(b, CRMPM_HIP_PROF_BINNING, k_cell_rank_order, pblock_wgs(b, b->tileWaves), kWG, batch_args(b, nullptr),
+               *orderOut, tmp, order2);
         *orderOut = order2;
         return check_launch();
     }
 
-    StepCommon common(CrHipSolver *s, const unsigned *order)
+    int validate_for_step(const CrHipBatch *b)
     {
-        StepCommon c;
-        c.g = s->g;
-        c.gravity = s->gravity;
-        c.dt = s->dt;
-        c.P = s->P;
-        c.S = SolverState{s->F, s->C, s->V0, s->maxParticles};
-        c.sv = shape_view(s);
-        c.segStart = s->segStart;
-        c.segEnd = s->segEnd;
-        c.order = order;
-        c.tilebuf = s->tilebuf;
-        c.grid = s->grid;
-        return c;
+        if (!b->initialized || !b->P.xm || !b->P.v || !b->P.params || !b->P.type || !b->P.mask)
+            return CRMPM_HIP_ERR_STATE;
+        for (int s = 0; s < b->numScenes; ++s)
+        {
+            const HostScene &h = b->scenes[s];
+            if (h.numShapes > 0 && (!h.shapeIds || !h.shapes.type || !h.geoms.type))
+                return CRMPM_HIP_ERR_STATE;
+        }
+        return CRMPM_HIP_OK;
+    }
+
+    // The launches of one substep (all scenes).
+    int enqueue_step(CrHipBatch *b)
+    {
+        const int blockWGs = grid1d(b->totalBlocks, kWaves);
+        launch(b, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + b->numScenes, kWG, (int)b->totalBlocks, blockWGs, b->grid,
+               b->gridForce, b->dirty, b->segStart, b->segEnd, (const SceneShapes *)b->dShapes, (const SceneRec *)b->dScenes);
+        const int n = b->totalActive;
+        if (n == 0)
+            return check_launch();
+        const unsigned *order = nullptr, *keys = nullptr;
+        // The standard solver's P2G ignores the active mask (src/solver/cpu_standard_mpm_solver.cpp:101-120).
+        int rc = build_tiles(b, n, b->kind != CRMPM_HIP_STANDARD, &order, &keys);
+        if (rc)
+            return rc;
+        const BatchArgs a = batch_args(b, order);
+        const int tileWGs = pblock_wgs(b, b->tileWaves);
+        const int updWGs = pblock_wgs(b, b->updateWaves);
+        const bool pers = b->tileWaves > 0;
+        if (b->kind == CRMPM_HIP_MLS)
+        {
+            launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
+            launch(b, CRMPM_HIP_PROF_UPDATE, k_update<0>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, tileWGs, kWG, a);
+        }
+        else if (b->kind == CRMPM_HIP_STANDARD)
+        {
+            launch(b, CRMPM_HIP_PROF_STD_P2G, pers ? k_std_p2g<true> : k_std_p2g<false>, tileWGs, kWG, a);
+            launch(b, CRMPM_HIP_PROF_UPDATE, k_update<2>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, tileWGs, kWG, a);
+        }
+        else
+        {
+            launch(b, CRMPM_HIP_PROF_BINNING, k_pb_stage, grid1d(n, 256), 256, n, (int)b->totalBlocks, keys, order, b->P,
+                   a.S, b->st);
+            const int K = b->iterations;
+            if (K == 0)
+            {
+                launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_integrate_only, grid1d(n, 256), 256, a, n, keys, b->st);
+            }
+            else
+            {
+                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_tile<0, true> : k_pb_tile<0, false>, tileWGs, kWG, a, b->st);
+                launch(b, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, a, b->dirty);
+                for (int it = 1; it < K; ++it)
+                {
+                    launch(b, CRMPM_HIP_PROF_PB_FUSED, pers ? k_pb_tile<1, true> : k_pb_tile<1, false>, tileWGs, kWG, a,
+                           b->st);
+                    launch(b, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, a, b->dirty);
+                }
+                launch(b, CRMPM_HIP_PROF_PB_G2P, pers ? k_pb_tile<2, true> : k_pb_tile<2, false>, tileWGs, kWG, a, b->st);
+            }
+        }
+        return check_launch();
+    }
+
+    void drop_graph(CrHipBatch *b)
+    {
+        if (b->graphExec)
+            (void)hipGraphExecDestroy(b->graphExec);
+        if (b->graph)
+            (void)hipGraphDestroy(b->graph);
+        b->graphExec = nullptr;
+        b->graph = nullptr;
+        b->graphKey[0] = -1;
+    }
+
+    // One substep: uploads scene records if needed, then either enqueues the
+    // launches or replays the captured graph of them.  The graph holds only
+    // kernel nodes (no memset or copy nodes); it is recaptured whenever the
+    // launch configuration changes.
+    int batch_step(CrHipBatch *b)
+    {
+        int rc = validate_for_step(b);
+        if (rc)
+            return rc;
+        rc = upload_scenes(b);
+        if (rc)
+            return rc;
+        const bool useGraph = b->graphMode && !b->profEvents && b->stream != nullptr;
+        if (!useGraph)
+            return enqueue_step(b);
+        const long long key[5] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch};
+        if (!b->graphExec || !std::equal(key, key + 5, b->graphKey))
+        {
+            drop_graph(b);
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return CRMPM_HIP_ERR_LAUNCH;
+            const int erc = enqueue_step(b);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+            if (erc || ec != hipSuccess || !g)
+            {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return erc ? erc : CRMPM_HIP_ERR_LAUNCH;
+            }
+            b->graph = g;
+            if (hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0) != hipSuccess)
+            {
+                drop_graph(b);
+                return CRMPM_HIP_ERR_LAUNCH;
+            }
+            std::copy(key, key + 5, b->graphKey);
+        }
+        return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+    }
+
+    int batch_destroy(CrHipBatch *b)
+    {
+        if (b->initialized)
+            (void)hipStreamSynchronize(b->stream);
+        drop_graph(b);
+        dfree(b->F);
+        dfree(b->C);
+        dfree(b->V0);
+        dfree(b->grid);
+        dfree(b->gridForce);
+        dfree(b->dirty);
+        dfree(b->segStart);
+        dfree(b->segEnd);
+        dfree(b->keysA);
+        dfree(b->keysB);
+        dfree(b->valsA);
+        dfree(b->valsB);
+        dfree(b->radixCounts);
+        dfree(b->digitTotals);
+        dfree(b->tilebuf);
+        dfree(b->tileForce);
+        dfree(b->shapeRecs);
+        dfree(b->shapeCounts);
+        dfree(b->slicerList);
+        dfree(b->dScenes);
+        dfree(b->dShapes);
+        dfree(b->dActivePrefix);
+        dfree(b->dBlockScene);
+        dfree(b->stageMem);
+        for (int i = 0; i < kUploadSlots; ++i)
+            if (b->stageDone[i])
+                (void)hipEventDestroy(b->stageDone[i]);
+        if (b->hStage)
+            (void)hipHostFree(b->hStage);
+        if (b->profEvents)
+        {
+            for (int i = 0; i < 2 * b->profMax; ++i)
+                (void)hipEventDestroy(b->profEvents[i]);
+            delete[] b->profEvents;
+            delete[] b->profCat;
+        }
+        delete[] b->scenes;
+        delete b;
+        return CRMPM_HIP_OK;
+    }
+
+    CrHipBatch *batch_new(int kind, int numScenes)
+    {
+        CrHipBatch *b = new (std::nothrow) CrHipBatch();
+        if (!b)
+            return nullptr;
+        b->scenes = new (std::nothrow) HostScene[numScenes];
+        if (!b->scenes)
+        {
+            delete b;
+            return nullptr;
+        }
+        b->kind = kind;
+        b->numScenes = numScenes;
+        return b;
+    }
+
+    int batch_initialize(CrHipBatch *b)
+    {
+        if (b->initialized)
+            return CRMPM_HIP_ERR_STATE;
+        const size_t N = (size_t)std::max(b->capacity, 1);
+        const size_t blocks = (size_t)b->totalBlocks;
+        const size_t nodes = blocks * 64;
+        const int radixTiles = grid1d((long long)N, kRadixTile);
+        const int ns = b->numScenes;
+        int rc = 0;
+        rc |= dalloc(&b->F, 9 * N);
+        rc |= dalloc(&b->C, 9 * N);
+        rc |= dalloc(&b->V0, N);
+        rc |= dalloc(&b->grid, nodes);
+        if (b->kind == CRMPM_HIP_STANDARD)
+        {
+            rc |= dalloc(&b->gridForce, nodes);
+            rc |= dalloc(&b->tileForce, blocks * kTileNodes);
+        }
+        rc |= dalloc(&b->dirty, blocks);
+        rc |= dalloc(&b->segStart, blocks);
+        rc |= dalloc(&b->segEnd, blocks);
+        rc |= dalloc(&b->keysA, N);
+        rc |= dalloc(&b->keysB, N);
+        rc |= dalloc(&b->valsA, N);
+        rc |= dalloc(&b->valsB, N);
+        rc |= dalloc(&b->radixCounts, (size_t)radixTiles * 256);
+        rc |= dalloc(&b->digitTotals, 256);
+        // tile k is pblock k: one 6^3 partial box per pblock
+        rc |= dalloc(&b->tilebuf, blocks * kTileNodes);
+        rc |= dalloc(&b->shapeCounts, 4 * (size_t)ns);
+        rc |= dalloc(&b->dScenes, ns);
+        rc |= dalloc(&b->dShapes, ns);
+        rc |= dalloc(&b->dActivePrefix, ns + 1);
+        rc |= dalloc(&b->dBlockScene, blocks);
+        if (b->kind == CRMPM_HIP_PB)
+        {
+            // xs (4) + vs (3) + Fs (9) + Cs (9) + ps (2) + ts (1) floats per particle
+            rc |= dalloc(&b->stageMem, 28 * N);
+            if (!rc)
+            {
+                float *m = b->stageMem;
+                b->st.stride = (int)N;
+                b->st.xs = reinterpret_cast<float4 *>(m);
+                m += 4 * N;
+                b->st.vs = m;
+                m += 3 * N;
+                b->st.Fs = m;
+                m += 9 * N;
+                b->st.Cs = m;
+                m += 9 * N;
+                b->st.ps = reinterpret_cast<float2 *>(m);
+                m += 2 * N;
+                b->st.ts = reinterpret_cast<int *>(m);
+            }
+        }
+        b->stageBytes = ((ns * (sizeof(SceneRec) + sizeof(SceneShapes)) + (ns + 1) * sizeof(int)) + 255) & ~size_t(255);
+        if (!rc && hipHostMalloc((void **)&b->hStage, b->stageBytes * kUploadSlots, hipHostMallocDefault) != hipSuccess)
+        {
+            b->hStage = nullptr;
+            rc = CRMPM_HIP_ERR_ALLOC;
+        }
+        for (int i = 0; i < kUploadSlots && !rc; ++i)
+            if (hipEventCreateWithFlags(&b->stageDone[i], hipEventDisableTiming) != hipSuccess)
+                rc = CRMPM_HIP_ERR_ALLOC;
+        if (!rc)
+            rc = ensure_shape_capacity(b);
+        if (rc)
+            return CRMPM_HIP_ERR_ALLOC;
+        // pblock -> scene map
+        int *map = new (std::nothrow) int[blocks];
+        if (!map)
+            return CRMPM_HIP_ERR_ALLOC;
+        for (int s = 0; s < ns; ++s)
+            for (int k = 0; k < b->scenes[s].g.numBlocks; ++k)
+                map[b->scenes[s].blockOffset + k] = s;
+        const hipError_t mc = hipMemcpy(b->dBlockScene, map, blocks * sizeof(int), hipMemcpyHostToDevice);
+        delete[] map;
+        if (mc != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
+        hipStream_t st = b->stream;
+        rc |= hipMemsetAsync(b->F, 0, 9 * N * sizeof(float), st);
+        rc |= hipMemsetAsync(b->C, 0, 9 * N * sizeof(float), st);
+        rc |= hipMemsetAsync(b->V0, 0, N * sizeof(float), st);
+        rc |= hipMemsetAsync(b->grid, 0, nodes * sizeof(float4), st);
+        if (b->gridForce)
+            rc |= hipMemsetAsync(b->gridForce, 0, nodes * sizeof(float4), st);
+        rc |= hipMemsetAsync(b->dirty, 0, blocks, st);
+        rc |= hipMemsetAsync(b->segStart, 0, blocks * sizeof(int), st);
+        rc |= hipMemsetAsync(b->segEnd, 0, blocks * sizeof(int), st);
+        rc |= hipMemsetAsync(b->shapeCounts, 0, 4 * ns * sizeof(int), st);
+        if (rc)
+            return CRMPM_HIP_ERR_LAUNCH;
+        b->initialized = true;
+        b->scenesDirty = true;
+        ++b->bindEpoch;
+        return CRMPM_HIP_OK;
+    }
+
+    int batch_compute_initial_data(CrHipBatch *b, int s, unsigned n, const unsigned *indices)
+    {
+        if (!valid_scene(b, s) || (n > 0 && !indices))
+            return CRMPM_HIP_ERR_ARG;
+        if (!b->initialized || !b->P.xm)
+            return CRMPM_HIP_ERR_STATE;
+        if (n == 0)
+            return CRMPM_HIP_OK;
+        int rc = upload_scenes(b);
+        if (rc)
+            return rc;
+        const BatchArgs a = batch_args(b, nullptr);
+        // the scene's tile partial region holds its fixed-point mass counters
+        // (216 float4 per pblock >= 64 counters per node block)
+        const HostScene &h = b->scenes[s];
+        unsigned long long *acc = reinterpret_cast<unsigned long long *>(b->tilebuf + (size_t)h.blockOffset * kTileNodes);
+        if (hipMemsetAsync(acc, 0, (size_t)h.g.numBlocks * 64 * sizeof(unsigned long long), b->stream) != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
+        k_init_mass<<<grid1d(n, 256), 256, 0, b->stream>>>(a, s, n, indices, acc);
+        k_init_apply<<<grid1d(h.g.numBlocks, kWaves), kWG, 0, b->stream>>>(a, s, acc, b->dirty);
+        k_init_volume<<<grid1d(n, 256), 256, 0, b->stream>>>(a, s, n, indices);
+        return check_launch();
+    }
+
+    int batch_copy_grid(CrHipBatch *b, int s, float *dst)
+    {
+        if (!valid_scene(b, s) || !dst)
+            return CRMPM_HIP_ERR_ARG;
+        if (!b->initialized)
+            return CRMPM_HIP_ERR_STATE;
+        int rc = upload_scenes(b);
+        if (rc)
+            return rc;
+        const GridGeom &g = b->scenes[s].g;
+        const long long total = (long long)g.nx * g.ny * g.nz;
+        k_copy_grid<<<grid1d(total, 256), 256, 0, b->stream>>>(batch_args(b, nullptr), s, reinterpret_cast<float4 *>(dst));
+        return check_launch();
+    }
+
+    int profile_begin(CrHipBatch *b, int maxLaunches)
+    {
+        if (!b || maxLaunches <= 0)
+            return CRMPM_HIP_ERR_ARG;
+        if (b->profEvents)
+            return CRMPM_HIP_ERR_STATE;
+        b->profEvents = new (std::nothrow) hipEvent_t[2 * (size_t)maxLaunches]();
+        b->profCat = new (std::nothrow) int[maxLaunches];
+        if (!b->profEvents || !b->profCat)
+            return CRMPM_HIP_ERR_ALLOC;
+        b->profMax = maxLaunches;
+        for (int i = 0; i < 2 * maxLaunches; ++i)
+            if (hipEventCreate(&b->profEvents[i]) != hipSuccess)
+                return CRMPM_HIP_ERR_ALLOC;
+        b->profUsed = 0;
+        b->profMask = ~0u;
+        return CRMPM_HIP_OK;
+    }
+
+    int profile_end(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories)
+    {
+        if (!b || !b->profEvents)
+            return CRMPM_HIP_ERR_STATE;
+        int rc = CRMPM_HIP_OK;
+        if (hipStreamSynchronize(b->stream) != hipSuccess)
+            rc = CRMPM_HIP_ERR_LAUNCH;
+        for (int c = 0; c < numCategories; ++c)
+        {
+            if (msPerCategory)
+                msPerCategory[c] = 0.0f;
+            if (launchesPerCategory)
+                launchesPerCategory[c] = 0;
+        }
+        for (int i = 0; i < b->profUsed && rc == CRMPM_HIP_OK; ++i)
+        {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, b->profEvents[2 * i], b->profEvents[2 * i + 1]) != hipSuccess)
+            {
+                rc = CRMPM_HIP_ERR_LAUNCH;
+                break;
+            }
+            const int c = b->profCat[i];
+            if (c >= 0 && c < numCategories)
+            {
+                if (msPerCategory)
+                    msPerCategory[c] += ms;
+                if (launchesPerCategory)
+                    launchesPerCategory[c] += 1;
+            }
+        }
+        for (int i = 0; i < 2 * b->profMax; ++i)
+            (void)hipEventDestroy(b->profEvents[i]);
+        delete[] b->profEvents;
+        delete[] b->profCat;
+        b->profEvents = nullptr;
+        b->profCat = nullptr;
+        b->profMax = b->profUsed = 0;
+        return rc;
+    }
+
+    // Autotuner state copy: everything a substep changes.
+    struct TuneBackup
+    {
+        void *mem = nullptr;
+        size_t bytes = 0;
+    };
+
+    struct Span
+    {
+        void *p;
+        size_t bytes;
+    };
+
+    int tune_spans(CrHipBatch *b, Span *sp, int maxSpans)
+    {
+        const size_t N = (size_t)std::max(b->capacity, 1);
+        int k = 0;
+        auto add = [&](void *p, size_t bytes) {
+            if (p && bytes && k < maxSpans)
+                sp[k++] = Span{p, bytes};
+        };
+        add(b->P.xm, N * sizeof(float4));
+        add(b->P.v, N * sizeof(float4));
+        add(b->F, 9 * N * sizeof(float));
+        add(b->C, 9 * N * sizeof(float));
+        add(b->V0, N * sizeof(float));
+        add(b->grid, (size_t)b->totalBlocks * 64 * sizeof(float4));
+        add(b->gridForce, (size_t)b->totalBlocks * 64 * sizeof(float4));
+        add(b->dirty, (size_t)b->totalBlocks);
+        for (int s = 0; s < b->numScenes; ++s)
+        {
+            const HostScene &h = b->scenes[s];
+            if (h.numShapes > 0)
+            {
+                add(h.shapes.force, (size_t)h.shapes.size * sizeof(float4));
+                add(h.shapes.torque, (size_t)h.shapes.size * sizeof(float4));
+            }
+        }
+        return k;
+    }
+
+    int tune_copy(CrHipBatch *b, char *backup, bool save)
+    {
+        Span sp[64];
+        const int k = tune_spans(b, sp, 64);
+        size_t off = 0;
+        for (int i = 0; i < k; ++i)
+        {
+            void *dst = save ? (void *)(backup + off) : sp[i].p;
+            const void *src = save ? sp[i].p : (const void *)(backup + off);
+            if (hipMemcpyAsync(dst, src, sp[i].bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+                return CRMPM_HIP_ERR_LAUNCH;
+            off += (sp[i].bytes + 255) & ~size_t(255);
+        }
+        return CRMPM_HIP_OK;
+    }
+
+    // Time `steps` substeps from the saved state with the given launch sizes.
+    int tune_trial(CrHipBatch *b, char *backup, int steps, int tileWaves, int updateWaves, float *ms)
+    {
+        b->tileWaves = tileWaves;
+        b->updateWaves = updateWaves;
+        int rc = tune_copy(b, backup, false);
+        if (rc)
+            return rc;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess)
+            return CRMPM_HIP_ERR_ALLOC;
+        if (hipEventCreate(&e1) != hipSuccess)
+        {
+            (void)hipEventDestroy(e0);
+            return CRMPM_HIP_ERR_ALLOC;
+        }
+        rc = enqueue_step(b); // warm-up substep
+        if (!rc && hipEventRecord(e0, b->stream) != hipSuccess)
+            rc = CRMPM_HIP_ERR_LAUNCH;
+        for (int i = 0; i < steps && !rc; ++i)
+            rc = enqueue_step(b);
+        if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+            rc = CRMPM_HIP_ERR_LAUNCH;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+
+    int batch_autotune(CrHipBatch *b, int steps, int *tileWavesOut, int *updateWavesOut)
+    {
+        if (!b || steps <= 0)
+            return CRMPM_HIP_ERR_ARG;
+        int rc = validate_for_step(b);
+        if (rc)
+            return rc;
+        if (b->profEvents)
+            return CRMPM_HIP_ERR_STATE;
+        rc = upload_scenes(b);
+        if (rc)
+            return rc;
+        Span sp[64];
+        const int k = tune_spans(b, sp, 64);
+        size_t bytes = 0;
+        for (int i = 0; i < k; ++i)
+            bytes += (sp[i].bytes + 255) & ~size_t(255);
+        char *backup = nullptr;
+        if (dalloc(&backup, bytes))
+            return CRMPM_HIP_ERR_ALLOC;
+        rc = tune_copy(b, backup, true);
+        // candidates: one wavefront per pblock, or a persistent launch of
+        // 2, 4, 8 or 16 thousand wavefronts looping over the pblocks
+        const int cand[] = {0, 2048, 4096, 8192, 16384};
+        int bestT = 0, bestU = 0;
+        float best = 3.4e38f;
+        for (int c : cand)
+        {
+            float ms = 0.0f;
+            if (rc || (c > 0 && c >= b->totalBlocks))
+                continue;
+            rc = tune_trial(b, backup, steps, c, 0, &ms);
+            if (!rc && ms < best)
+            {
+                best = ms;
+                bestT = c;
+            }
+        }
+        best = 3.4e38f;
+        for (int c : cand)
+        {
+            float ms = 0.0f;
+            if (rc || (c > 0 && c >= b->totalBlocks))
+                continue;
+            rc = tune_trial(b, backup, steps, bestT, c, &ms);
+            if (!rc && ms < best)
+            {
+                best = ms;
+                bestU = c;
+            }
+        }
+        if (!rc)
+            rc = tune_copy(b, backup, false);
+        (void)hipStreamSynchronize(b->stream);
+        dfree(backup);
+        b->tileWaves = bestT;
+        b->updateWaves = bestU;
+        if (tileWavesOut)
+            *tileWavesOut = bestT;
+        if (updateWavesOut)
+            *updateWavesOut = bestU;
+        return rc;
     }
 } // namespace
 
 extern "C" {
 
-int crHipSolverCreate(int kind, int numMaxParticles, float cellSize, const float boundMin[3], const float boundMax[3],
-                      CrHipSolver **out)
+// ---------------------------- batch ----------------------------
+
+int crHipBatchCreate(int kind, int numScenes, const CrHipSceneDesc *scenes, CrHipBatch **out)
 {
-    if (!out || !boundMin || !boundMax || numMaxParticles < 0 || !(cellSize > 0.0f))
+    if (!out || numScenes <= 0 || !scenes)
         return CRMPM_HIP_ERR_ARG;
     if (kind != CRMPM_HIP_MLS && kind != CRMPM_HIP_PB && kind != CRMPM_HIP_STANDARD)
         return CRMPM_HIP_ERR_ARG;
+    CrHipBatch *b = batch_new(kind, numScenes);
+    if (!b)
+        return CRMPM_HIP_ERR_ALLOC;
+    long long blocks = 0;
+    long long cap = 0;
+    for (int s = 0; s < numScenes; ++s)
+    {
+        const CrHipSceneDesc &d = scenes[s];
+        HostScene &h = b->scenes[s];
+        if (d.numMaxParticles < 0 || d.particleOffset < 0 || !(d.cellSize > 0.0f) ||
+            !grid_geom(d.cellSize, d.boundMin, d.boundMax, h.g))
+        {
+            batch_destroy(b);
+            return CRMPM_HIP_ERR_ARG;
+        }
+        h.maxParticles = d.numMaxParticles;
+        h.particleOffset = d.particleOffset;
+        h.blockOffset = (int)blocks;
+        blocks += h.g.numBlocks;
+        cap = std::max(cap, (long long)d.particleOffset + d.numMaxParticles);
+    }
+    if (blocks >= (1ll << 31) - 1 || cap >= (1ll << 31))
+    {
+        batch_destroy(b);
+        return CRMPM_HIP_ERR_CAPACITY;
+    }
+    b->totalBlocks = blocks;
+    b->capacity = (int)cap;
+    *out = b;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchDestroy(CrHipBatch *b)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    return batch_destroy(b);
+}
+
+int crHipBatchSetStream(CrHipBatch *b, void *stream)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    b->stream = (hipStream_t)stream;
+    drop_graph(b);
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSolverIterations(CrHipBatch *b, int n)
+{
+    if (!b || n < 0)
+        return CRMPM_HIP_ERR_ARG;
+    b->iterations = n;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchBindParticleData(CrHipBatch *b, float *positionMass, float *velocity, int size)
+{
+    if (!b || !positionMass || !velocity || size < b->capacity)
+        return CRMPM_HIP_ERR_ARG;
+    b->P.xm = reinterpret_cast<float4 *>(positionMass);
+    b->P.v = reinterpret_cast<float4 *>(velocity);
+    ++b->bindEpoch;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchBindParticleMaterials(CrHipBatch *b, float *params0, int *type, int size)
+{
+    if (!b || !params0 || !type || size < b->capacity)
+        return CRMPM_HIP_ERR_ARG;
+    b->P.params = reinterpret_cast<const float4 *>(params0);
+    b->P.type = type;
+    ++b->bindEpoch;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchBindActiveMask(CrHipBatch *b, unsigned char *activeMask)
+{
+    if (!b || !activeMask)
+        return CRMPM_HIP_ERR_ARG;
+    b->P.mask = activeMask;
+    ++b->bindEpoch;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchInitialize(CrHipBatch *b)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    return batch_initialize(b);
+}
+
+int crHipBatchGetGridDims(const CrHipBatch *b, int scene, int dims[3])
+{
+    if (!valid_scene(b, scene) || !dims)
+        return CRMPM_HIP_ERR_ARG;
+    dims[0] = b->scenes[scene].g.nx;
+    dims[1] = b->scenes[scene].g.ny;
+    dims[2] = b->scenes[scene].g.nz;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSceneGravity(CrHipBatch *b, int scene, float gx, float gy, float gz)
+{
+    if (!valid_scene(b, scene))
+        return CRMPM_HIP_ERR_ARG;
+    b->scenes[scene].gravity = make_float3(gx, gy, gz);
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSceneStepSize(CrHipBatch *b, int scene, float dt)
+{
+    if (!valid_scene(b, scene))
+        return CRMPM_HIP_ERR_ARG;
+    b->scenes[scene].dt = dt;
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSceneContactModel(CrHipBatch *b, int scene, int model)
+{
+    if (!valid_scene(b, scene) || (model != CRMPM_HIP_CONTACT_KINEMATIC && model != CRMPM_HIP_CONTACT_EFFECTIVE_MASS))
+        return CRMPM_HIP_ERR_ARG;
+    b->scenes[scene].contactModel = model;
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSceneNumActiveParticles(CrHipBatch *b, int scene, unsigned int n)
+{
+    if (!valid_scene(b, scene) || n > (unsigned)b->scenes[scene].maxParticles)
+        return CRMPM_HIP_ERR_ARG;
+    b->scenes[scene].numActive = n;
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchBindSceneShapes(CrHipBatch *b, int scene, int numShapes, int *shapeIds, const CrHipShapeData *shapes,
+                              const CrHipGeometryData *geometries, const CrHipGeometrySdfData *sdf)
+{
+    if (!valid_scene(b, scene) || numShapes < 0)
+        return CRMPM_HIP_ERR_ARG;
+    HostScene &h = b->scenes[scene];
+    h.numShapes = numShapes;
+    h.shapeIds = shapeIds;
+    if (shapes)
+        h.shapes = *shapes;
+    if (geometries)
+        h.geoms = *geometries;
+    if (sdf)
+        h.sdf = *sdf;
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchComputeInitialData(CrHipBatch *b, int scene, unsigned int n, const unsigned int *indices)
+{
+    return batch_compute_initial_data(b, scene, n, indices);
+}
+
+int crHipBatchStep(CrHipBatch *b)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    return batch_step(b);
+}
+
+int crHipBatchSetGraphMode(CrHipBatch *b, int enabled)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    b->graphMode = enabled != 0;
+    if (!b->graphMode)
+        drop_graph(b);
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves)
+{
+    return batch_autotune(b, stepsPerTrial, tileWaves, updateWaves);
+}
+
+int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves)
+{
+    if (!b || tileWaves < 0 || updateWaves < 0)
+        return CRMPM_HIP_ERR_ARG;
+    b->tileWaves = tileWaves;
+    b->updateWaves = updateWaves;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchFetchResults(CrHipBatch *b)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    return hipStreamSynchronize(b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+}
+
+int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, int *stride)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    if (!b->initialized)
+        return CRMPM_HIP_ERR_STATE;
+    if (F)
+        *F = b->F;
+    if (C)
+        *C = b->C;
+    if (V0)
+        *V0 = b->V0;
+    if (stride)
+        *stride = b->capacity;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchCopyGrid(CrHipBatch *b, int scene, float *dst)
+{
+    return batch_copy_grid(b, scene, dst);
+}
+
+int crHipBatchProfileBegin(CrHipBatch *b, int maxLaunches)
+{
+    return profile_begin(b, maxLaunches);
+}
+
+int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask)
+{
+    if (!b)
+        return CRMPM_HIP_ERR_ARG;
+    if (!b->profEvents)
+        return CRMPM_HIP_ERR_STATE;
+    b->profMask = mask;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories)
+{
+    return profile_end(b, msPerCategory, launchesPerCategory, numCategories);
+}
+
+// ------------------------- single scene -------------------------
+
+int crHipSolverCreate(int kind, int numMaxParticles, float cellSize, const float boundMin[3], const float boundMax[3],
+                      CrHipSolver **out)
+{
+    if (!out || !boundMin || !boundMax)
+        return CRMPM_HIP_ERR_ARG;
+    CrHipSceneDesc d{};
+    d.numMaxParticles = numMaxParticles;
+    d.particleOffset = 0;
+    d.cellSize = cellSize;
+    for (int a = 0; a < 3; ++a)
+    {
+        d.boundMin[a] = boundMin[a];
+        d.boundMax[a] = boundMax[a];
+    }
     CrHipSolver *s = new (std::nothrow) CrHipSolver();
     if (!s)
         return CRMPM_HIP_ERR_ALLOC;
-    s->kind = kind;
-    s->maxParticles = numMaxParticles;
-    GridGeom &g = s->g;
-    g.bmin = make_float3(boundMin[0], boundMin[1], boundMin[2]);
-    g.bmax = make_float3(boundMax[0], boundMax[1], boundMax[2]);
-    g.dx = cellSize;
-    g.invdx = 1 / cellSize;
-    g.cellVol = cellSize * cellSize * cellSize;
-    int n[3];
-    for (int a = 0; a < 3; ++a)
-        n[a] = (int)((boundMax[a] - boundMin[a]) / cellSize + 1.0f);
-    g.nx = n[0];
-    g.ny = n[1];
-    g.nz = n[2];
-    if (g.nx < 1 || g.ny < 1 || g.nz < 1)
+    const int rc = crHipBatchCreate(kind, 1, &d, &s->b);
+    if (rc)
     {
         delete s;
-        return CRMPM_HIP_ERR_ARG;
+        return rc;
     }
-    g.nbx = (g.nx + 3) / 4;
-    g.nby = (g.ny + 3) / 4;
-    g.nbz = (g.nz + 3) / 4;
-    g.numBlocks = g.nbx * g.nby * g.nbz;
     *out = s;
     return CRMPM_HIP_OK;
 }
@@ -1642,129 +2722,76 @@ int crHipSolverDestroy(CrHipSolver *s)
 {
     if (!s)
         return CRMPM_HIP_ERR_ARG;
-    if (s->initialized)
-        (void)hipStreamSynchronize(s->stream);
-    dfree(s->F);
-    dfree(s->C);
-    dfree(s->V0);
-    dfree(s->grid);
-    dfree(s->gridForce);
-    dfree(s->dirty);
-    dfree(s->segStart);
-    dfree(s->segEnd);
-    dfree(s->keysA);
-    dfree(s->keysB);
-    dfree(s->valsA);
-    dfree(s->valsB);
-    dfree(s->radixCounts);
-    dfree(s->digitTotals);
-    dfree(s->tilebuf);
-    dfree(s->tileForce);
-    dfree(s->shapeRecs);
-    dfree(s->shapeCounts);
-    dfree(s->slicerList);
-    dfree(s->stageMem);
+    const int rc = batch_destroy(s->b);
     delete s;
-    return CRMPM_HIP_OK;
+    return rc;
 }
 
 int crHipSolverGetGridDims(const CrHipSolver *s, int dims[3])
 {
-    if (!s || !dims)
-        return CRMPM_HIP_ERR_ARG;
-    dims[0] = s->g.nx;
-    dims[1] = s->g.ny;
-    dims[2] = s->g.nz;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchGetGridDims(s->b, 0, dims) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetStream(CrHipSolver *s, void *stream)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    s->stream = (hipStream_t)stream;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetStream(s->b, stream) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetGravity(CrHipSolver *s, float gx, float gy, float gz)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    s->gravity = make_float3(gx, gy, gz);
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetSceneGravity(s->b, 0, gx, gy, gz) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetIntegrationStepSize(CrHipSolver *s, float dt)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    s->dt = dt;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetSceneStepSize(s->b, 0, dt) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetSolverIterations(CrHipSolver *s, int n)
 {
-    if (!s || n < 0)
-        return CRMPM_HIP_ERR_ARG;
-    s->iterations = n;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetSolverIterations(s->b, n) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetShapeContactModel(CrHipSolver *s, int model)
 {
-    if (!s || (model != CRMPM_HIP_CONTACT_KINEMATIC && model != CRMPM_HIP_CONTACT_EFFECTIVE_MASS))
-        return CRMPM_HIP_ERR_ARG;
-    s->contactModel = model;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetSceneContactModel(s->b, 0, model) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverBindParticleData(CrHipSolver *s, float *positionMass, float *velocity, int size)
 {
-    if (!s || !positionMass || !velocity || size < s->maxParticles)
-        return CRMPM_HIP_ERR_ARG;
-    s->P.xm = reinterpret_cast<float4 *>(positionMass);
-    s->P.v = reinterpret_cast<float4 *>(velocity);
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchBindParticleData(s->b, positionMass, velocity, size) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverBindParticleMaterials(CrHipSolver *s, float *params0, int *type, int size)
 {
-    if (!s || !params0 || !type || size < s->maxParticles)
-        return CRMPM_HIP_ERR_ARG;
-    s->P.params = reinterpret_cast<const float4 *>(params0);
-    s->P.type = type;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchBindParticleMaterials(s->b, params0, type, size) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverBindActiveMask(CrHipSolver *s, unsigned char *activeMask)
 {
-    if (!s || !activeMask)
-        return CRMPM_HIP_ERR_ARG;
-    s->P.mask = activeMask;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchBindActiveMask(s->b, activeMask) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetNumActiveParticles(CrHipSolver *s, unsigned int n)
 {
-    if (!s || n > (unsigned)s->maxParticles)
-        return CRMPM_HIP_ERR_ARG;
-    s->numActive = n;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchSetSceneNumActiveParticles(s->b, 0, n) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverSetNumShapes(CrHipSolver *s, int n)
 {
     if (!s || n < 0)
         return CRMPM_HIP_ERR_ARG;
-    s->numShapes = n;
-    return ensure_shape_capacity(s, n);
+    s->b->scenes[0].numShapes = n;
+    s->b->scenesDirty = true;
+    return CRMPM_HIP_OK;
 }
 
 int crHipSolverBindShapeIds(CrHipSolver *s, int *shapeIds)
 {
     if (!s)
         return CRMPM_HIP_ERR_ARG;
-    s->shapeIds = shapeIds;
+    s->b->scenes[0].shapeIds = shapeIds;
+    s->b->scenesDirty = true;
     return CRMPM_HIP_OK;
 }
 
@@ -1772,7 +2799,8 @@ int crHipSolverBindShapeData(CrHipSolver *s, const CrHipShapeData *d)
 {
     if (!s || !d)
         return CRMPM_HIP_ERR_ARG;
-    s->shapes = *d;
+    s->b->scenes[0].shapes = *d;
+    s->b->scenesDirty = true;
     return CRMPM_HIP_OK;
 }
 
@@ -1780,7 +2808,8 @@ int crHipSolverBindGeometryData(CrHipSolver *s, const CrHipGeometryData *d)
 {
     if (!s || !d)
         return CRMPM_HIP_ERR_ARG;
-    s->geoms = *d;
+    s->b->scenes[0].geoms = *d;
+    s->b->scenesDirty = true;
     return CRMPM_HIP_OK;
 }
 
@@ -1788,283 +2817,81 @@ int crHipSolverBindGeometrySdfData(CrHipSolver *s, const CrHipGeometrySdfData *d
 {
     if (!s || !d)
         return CRMPM_HIP_ERR_ARG;
-    s->sdf = *d;
+    s->b->scenes[0].sdf = *d;
+    s->b->scenesDirty = true;
     return CRMPM_HIP_OK;
 }
 
 int crHipSolverInitialize(CrHipSolver *s)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    if (s->initialized)
-        return CRMPM_HIP_ERR_STATE;
-    const size_t N = (size_t)std::max(s->maxParticles, 1);
-    const GridGeom &g = s->g;
-    const size_t nodes = (size_t)g.numBlocks * 64;
-    // tile k is pblock k, so the partial buffers hold one 6^3 box per pblock
-    s->maxTiles = g.numBlocks;
-    const int radixTiles = grid1d((long long)N, kRadixTile);
-    int rc = 0;
-    rc |= dalloc(&s->F, 9 * N);
-    rc |= dalloc(&s->C, 9 * N);
-    rc |= dalloc(&s->V0, N);
-    rc |= dalloc(&s->grid, nodes);
-    if (s->kind == CRMPM_HIP_STANDARD)
-    {
-        rc |= dalloc(&s->gridForce, nodes);
-        rc |= dalloc(&s->tileForce, (size_t)s->maxTiles * kTileNodes);
-    }
-    rc |= dalloc(&s->dirty, g.numBlocks);
-    rc |= dalloc(&s->segStart, g.numBlocks);
-    rc |= dalloc(&s->segEnd, g.numBlocks);
-    rc |= dalloc(&s->keysA, N);
-    rc |= dalloc(&s->keysB, N);
-    rc |= dalloc(&s->valsA, N);
-    rc |= dalloc(&s->valsB, N);
-    rc |= dalloc(&s->radixCounts, (size_t)radixTiles * 256);
-    rc |= dalloc(&s->digitTotals, 256);
-    rc |= dalloc(&s->tilebuf, (size_t)s->maxTiles * kTileNodes);
-    rc |= dalloc(&s->shapeCounts, 4);
-    rc |= ensure_shape_capacity(s, std::max(s->numShapes, 1));
-    if (s->kind == CRMPM_HIP_PB)
-    {
-        // xs (4) + vs (3) + Fs (9) + Cs (9) + ps (2) + ts (1) floats per particle
-        rc |= dalloc(&s->stageMem, 28 * N);
-        if (!rc)
-        {
-            float *m = s->stageMem;
-            s->st.stride = (int)N;
-            s->st.xs = reinterpret_cast<float4 *>(m);
-            m += 4 * N;
-            s->st.vs = m;
-            m += 3 * N;
-            s->st.Fs = m;
-            m += 9 * N;
-            s->st.Cs = m;
-            m += 9 * N;
-            s->st.ps = reinterpret_cast<float2 *>(m);
-            m += 2 * N;
-            s->st.ts = reinterpret_cast<int *>(m);
-        }
-    }
-    if (rc)
-        return CRMPM_HIP_ERR_ALLOC;
-    hipStream_t st = s->stream;
-    rc |= hipMemsetAsync(s->F, 0, 9 * N * sizeof(float), st);
-    rc |= hipMemsetAsync(s->C, 0, 9 * N * sizeof(float), st);
-    rc |= hipMemsetAsync(s->V0, 0, N * sizeof(float), st);
-    rc |= hipMemsetAsync(s->grid, 0, nodes * sizeof(float4), st);
-    if (s->gridForce)
-        rc |= hipMemsetAsync(s->gridForce, 0, nodes * sizeof(float4), st);
-    rc |= hipMemsetAsync(s->dirty, 0, g.numBlocks, st);
-    rc |= hipMemsetAsync(s->segStart, 0, g.numBlocks * sizeof(int), st);
-    rc |= hipMemsetAsync(s->segEnd, 0, g.numBlocks * sizeof(int), st);
-    rc |= hipMemsetAsync(s->shapeCounts, 0, 4 * sizeof(int), st);
-    if (rc)
-        return CRMPM_HIP_ERR_LAUNCH;
-    s->initialized = true;
-    return CRMPM_HIP_OK;
+    return s ? batch_initialize(s->b) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverComputeInitialData(CrHipSolver *s, unsigned int n, const unsigned int *indices)
 {
-    if (!s || (n > 0 && !indices))
-        return CRMPM_HIP_ERR_ARG;
-    if (!s->initialized || !s->P.xm)
-        return CRMPM_HIP_ERR_STATE;
-    if (n == 0)
-        return CRMPM_HIP_OK;
-    SolverState S{s->F, s->C, s->V0, s->maxParticles};
-    k_init_mass<<<grid1d(n, 256), 256, 0, s->stream>>>(s->g, n, indices, s->P.xm, S, s->grid, s->dirty);
-    k_init_volume<<<grid1d(n, 256), 256, 0, s->stream>>>(s->g, n, indices, s->P.xm, S, s->grid);
-    return check_launch();
+    return s ? batch_compute_initial_data(s->b, 0, n, indices) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverStep(CrHipSolver *s, float *dtOut)
 {
     if (!s)
         return CRMPM_HIP_ERR_ARG;
-    if (!s->initialized || !s->P.xm || !s->P.v || !s->P.params || !s->P.type || !s->P.mask)
-        return CRMPM_HIP_ERR_STATE;
-    if (s->numShapes > 0 && (!s->shapeIds || !s->shapes.type || !s->geoms.type))
-        return CRMPM_HIP_ERR_STATE;
     if (dtOut)
-        *dtOut = s->dt;
-    const GridGeom &g = s->g;
-    const int blockWGs = grid1d(g.numBlocks, kWaves);
-    const ShapeTables tables{s->numShapes, s->shapeIds, s->shapes, s->geoms, s->shapeRecs, s->shapeCounts, s->slicerList};
-    launch(s, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + std::max(1, grid1d(s->numShapes, kWG)), kWG, g, blockWGs,
-           s->grid, s->gridForce, s->dirty, s->segStart, s->segEnd, tables);
-    const int n = (int)s->numActive;
-    if (n == 0)
-        return check_launch();
-    const unsigned *order = nullptr, *keys = nullptr;
-    // The standard solver's P2G ignores the active mask (src/solver/cpu_standard_mpm_solver.cpp:101-120).
-    int rc = build_tiles(s, n, s->kind != CRMPM_HIP_STANDARD, &order, &keys);
-    if (rc)
-        return rc;
-    StepCommon c = common(s, order);
-    const int tileWGs = grid1d(g.numBlocks, kWaves);
-    const int updWGs = grid1d(g.numBlocks, kWaves);
-    const float4 *noForce = nullptr;
-    if (s->kind == CRMPM_HIP_MLS)
-    {
-        launch(s, CRMPM_HIP_PROF_MLS_P2G, k_mls_p2g, tileWGs, kWG, c);
-        launch(s, CRMPM_HIP_PROF_UPDATE, k_update<0>, updWGs, kWG, c, noForce, s->dirty);
-        launch(s, CRMPM_HIP_PROF_MLS_G2P, k_mls_g2p, tileWGs, kWG, c);
-    }
-    else if (s->kind == CRMPM_HIP_STANDARD)
-    {
-        launch(s, CRMPM_HIP_PROF_STD_P2G, k_std_p2g, tileWGs, kWG, c, s->tileForce);
-        launch(s, CRMPM_HIP_PROF_UPDATE, k_update<2>, updWGs, kWG, c, s->tileForce, s->dirty);
-        launch(s, CRMPM_HIP_PROF_STD_G2P, k_std_g2p, tileWGs, kWG, c);
-    }
-    else
-    {
-        SolverState S{s->F, s->C, s->V0, s->maxParticles};
-        launch(s, CRMPM_HIP_PROF_BINNING, k_pb_stage, grid1d(n, 256), 256, n, g.numBlocks, keys, order, s->P, S, s->st);
-        const int K = s->iterations;
-        if (K == 0)
-        {
-            launch(s, CRMPM_HIP_PROF_PB_G2P, k_pb_integrate_only, grid1d(n, 256), 256, c, n, keys, s->st, s->gravity);
-        }
-        else
-        {
-            launch(s, CRMPM_HIP_PROF_PB_P2G, k_pb_tile<0>, tileWGs, kWG, c, s->st, s->gravity);
-            launch(s, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, c, noForce, s->dirty);
-            for (int it = 1; it < K; ++it)
-            {
-                launch(s, CRMPM_HIP_PROF_PB_FUSED, k_pb_tile<1>, tileWGs, kWG, c, s->st, s->gravity);
-                launch(s, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, c, noForce, s->dirty);
-            }
-            launch(s, CRMPM_HIP_PROF_PB_G2P, k_pb_tile<2>, tileWGs, kWG, c, s->st, s->gravity);
-        }
-    }
-    return check_launch();
+        *dtOut = s->b->scenes[0].dt;
+    return batch_step(s->b);
 }
 
 int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int *orderOut)
 {
     if (!s || !keysOut || !orderOut)
         return CRMPM_HIP_ERR_ARG;
-    if (!s->initialized || !s->P.xm || !s->P.mask)
+    CrHipBatch *b = s->b;
+    if (!b->initialized || !b->P.xm || !b->P.mask)
         return CRMPM_HIP_ERR_STATE;
-    const int n = (int)s->numActive;
+    int rc = upload_scenes(b);
+    if (rc)
+        return rc;
+    const int n = b->totalActive;
     if (n == 0)
         return CRMPM_HIP_OK;
     const unsigned *order = nullptr, *keys = nullptr;
-    int rc = sort_particles(s, n, s->kind != CRMPM_HIP_STANDARD, &order, &keys);
+    rc = sort_particles(b, n, b->kind != CRMPM_HIP_STANDARD, &order, &keys);
     if (rc)
         return rc;
-    if (hipMemcpyAsync(keysOut, keys, n * sizeof(unsigned), hipMemcpyDeviceToDevice, s->stream) != hipSuccess ||
-        hipMemcpyAsync(orderOut, order, n * sizeof(unsigned), hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+    if (hipMemcpyAsync(keysOut, keys, n * sizeof(unsigned), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(orderOut, order, n * sizeof(unsigned), hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
         return CRMPM_HIP_ERR_LAUNCH;
     return CRMPM_HIP_OK;
 }
 
 int crHipSolverProfileBegin(CrHipSolver *s, int maxLaunches)
 {
-    if (!s || maxLaunches <= 0)
-        return CRMPM_HIP_ERR_ARG;
-    if (s->profEvents)
-        return CRMPM_HIP_ERR_STATE;
-    s->profEvents = new (std::nothrow) hipEvent_t[2 * (size_t)maxLaunches];
-    s->profCat = new (std::nothrow) int[maxLaunches];
-    if (!s->profEvents || !s->profCat)
-        return CRMPM_HIP_ERR_ALLOC;
-    for (int i = 0; i < 2 * maxLaunches; ++i)
-        if (hipEventCreate(&s->profEvents[i]) != hipSuccess)
-            return CRMPM_HIP_ERR_ALLOC;
-    s->profMax = maxLaunches;
-    s->profUsed = 0;
-    s->profMask = ~0u;
-    return CRMPM_HIP_OK;
+    return s ? profile_begin(s->b, maxLaunches) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverProfileMask(CrHipSolver *s, unsigned int mask)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    if (!s->profEvents)
-        return CRMPM_HIP_ERR_STATE;
-    s->profMask = mask;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchProfileMask(s->b, mask) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverProfileEnd(CrHipSolver *s, float *msPerCategory, int *launchesPerCategory, int numCategories)
 {
-    if (!s || !s->profEvents)
-        return CRMPM_HIP_ERR_STATE;
-    int rc = CRMPM_HIP_OK;
-    if (hipStreamSynchronize(s->stream) != hipSuccess)
-        rc = CRMPM_HIP_ERR_LAUNCH;
-    for (int c = 0; c < numCategories; ++c)
-    {
-        if (msPerCategory)
-            msPerCategory[c] = 0.0f;
-        if (launchesPerCategory)
-            launchesPerCategory[c] = 0;
-    }
-    for (int i = 0; i < s->profUsed && rc == CRMPM_HIP_OK; ++i)
-    {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s->profEvents[2 * i], s->profEvents[2 * i + 1]) != hipSuccess)
-        {
-            rc = CRMPM_HIP_ERR_LAUNCH;
-            break;
-        }
-        const int c = s->profCat[i];
-        if (c >= 0 && c < numCategories)
-        {
-            if (msPerCategory)
-                msPerCategory[c] += ms;
-            if (launchesPerCategory)
-                launchesPerCategory[c] += 1;
-        }
-    }
-    for (int i = 0; i < 2 * s->profMax; ++i)
-        (void)hipEventDestroy(s->profEvents[i]);
-    delete[] s->profEvents;
-    delete[] s->profCat;
-    s->profEvents = nullptr;
-    s->profCat = nullptr;
-    s->profMax = s->profUsed = 0;
-    return rc;
+    return s ? profile_end(s->b, msPerCategory, launchesPerCategory, numCategories) : CRMPM_HIP_ERR_STATE;
 }
 
 int crHipSolverFetchResults(CrHipSolver *s)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    return hipStreamSynchronize(s->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+    return s ? crHipBatchFetchResults(s->b) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverGetStateArrays(CrHipSolver *s, float **F, float **C, float **V0, int *stride)
 {
-    if (!s)
-        return CRMPM_HIP_ERR_ARG;
-    if (!s->initialized)
-        return CRMPM_HIP_ERR_STATE;
-    if (F)
-        *F = s->F;
-    if (C)
-        *C = s->C;
-    if (V0)
-        *V0 = s->V0;
-    if (stride)
-        *stride = s->maxParticles;
-    return CRMPM_HIP_OK;
+    return s ? crHipBatchGetStateArrays(s->b, F, C, V0, stride) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverCopyGrid(CrHipSolver *s, float *dst)
 {
-    if (!s || !dst)
-        return CRMPM_HIP_ERR_ARG;
-    if (!s->initialized)
-        return CRMPM_HIP_ERR_STATE;
-    const long long total = (long long)s->g.nx * s->g.ny * s->g.nz;
-    k_copy_grid<<<grid1d(total, 256), 256, 0, s->stream>>>(s->g, s->grid, reinterpret_cast<float4 *>(dst));
-    return check_launch();
+    return s ? batch_copy_grid(s->b, 0, dst) : CRMPM_HIP_ERR_ARG;
 }
 
 } // extern "C"

@@ -42,6 +42,11 @@ class GeometrySdfData(ctypes.Structure):
     _fields_ = [("dimension", _vp), ("lowerBoundCellSize", _vp), ("gradientSignedDistance", _vp)]
 
 
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("numMaxParticles", ctypes.c_int), ("particleOffset", ctypes.c_int), ("cellSize", ctypes.c_float),
+                ("boundMin", _f3), ("boundMax", _f3)]
+
+
 EXPORTS = {
     "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
     "crHipSolverDestroy": [_vp],
@@ -71,6 +76,33 @@ EXPORTS = {
     "crHipSolverProfileBegin": [_vp, ctypes.c_int],
     "crHipSolverProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
     "crHipSolverProfileMask": [_vp, ctypes.c_uint],
+    "crHipBatchCreate": [ctypes.c_int, ctypes.c_int, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)],
+    "crHipBatchDestroy": [_vp],
+    "crHipBatchSetStream": [_vp, _vp],
+    "crHipBatchSetSolverIterations": [_vp, ctypes.c_int],
+    "crHipBatchBindParticleData": [_vp, _vp, _vp, ctypes.c_int],
+    "crHipBatchBindParticleMaterials": [_vp, _vp, _vp, ctypes.c_int],
+    "crHipBatchBindActiveMask": [_vp, _vp],
+    "crHipBatchInitialize": [_vp],
+    "crHipBatchGetGridDims": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchSetSceneGravity": [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float],
+    "crHipBatchSetSceneStepSize": [_vp, ctypes.c_int, ctypes.c_float],
+    "crHipBatchSetSceneContactModel": [_vp, ctypes.c_int, ctypes.c_int],
+    "crHipBatchSetSceneNumActiveParticles": [_vp, ctypes.c_int, ctypes.c_uint],
+    "crHipBatchBindSceneShapes": [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(ShapeData),
+                                  ctypes.POINTER(GeometryData), ctypes.POINTER(GeometrySdfData)],
+    "crHipBatchComputeInitialData": [_vp, ctypes.c_int, ctypes.c_uint, _vp],
+    "crHipBatchStep": [_vp],
+    "crHipBatchSetGraphMode": [_vp, ctypes.c_int],
+    "crHipBatchAutotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchSetLaunchSizes": [_vp, ctypes.c_int, ctypes.c_int],
+    "crHipBatchFetchResults": [_vp],
+    "crHipBatchGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchCopyGrid": [_vp, ctypes.c_int, _vp],
+    "crHipBatchProfileBegin": [_vp, ctypes.c_int],
+    "crHipBatchProfileMask": [_vp, ctypes.c_uint],
+    "crHipBatchProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
 }
 
 PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g", "mls_g2p", "std_p2g", "std_g2p",
@@ -283,6 +315,170 @@ class HipSolver:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.crHipSolverDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceShapeTables:
+    """A scene's ShapeData / GeometryData / GeometrySdfData tables in device memory."""
+
+    def __init__(self, scene, device):
+        import torch
+
+        t = S.shape_tables(scene)
+        T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(device)
+        self.sh = {k: T(t[k], np.float32) for k in S.SHAPE_F4}
+        self.sh["type"] = T(t["type"], np.int32)
+        self.sh["geom"] = T(t["geom"], np.int32)
+        self.ids = T(np.maximum(t["ids"], 0) if len(t["ids"]) else np.zeros(1, np.int32), np.int32)
+        cap = len(t["type"])
+        self.force = torch.zeros((cap, 4), dtype=torch.float32, device=device)
+        self.torque = torch.zeros((cap, 4), dtype=torch.float32, device=device)
+        self.gtype = T(t["gtype"], np.int32)
+        self.gparams = T(t["gparams"], np.float32)
+        self.sdf_dims = T(scene["sdf_dims"], np.int32)
+        self.sdf_lbcs = T(scene["sdf_lbcs"], np.float32)
+        self.sdf_data = T(scene["sdf_data"], np.float32)
+        self.num_shapes = len(t["ids"])
+        p = lambda x: x.data_ptr()
+        self.sd = ShapeData(p(self.sh["type"]), p(self.sh["geom"]), p(self.sh["pos"]), p(self.sh["rot"]),
+                            p(self.sh["invScale"]), p(self.sh["linVel"]), p(self.sh["angVel"]),
+                            p(self.sh["comInvMass"]), p(self.sh["inertia0"]), p(self.sh["inertia1"]), p(self.force),
+                            p(self.torque), p(self.sh["params0"]), cap)
+        self.gd = GeometryData(p(self.gtype), p(self.gparams), len(t["gtype"]))
+        self.sdf = GeometrySdfData(p(self.sdf_dims), p(self.sdf_lbcs), p(self.sdf_data))
+
+
+class HipBatch:
+    """Many scenes of one solver kind advanced together (crHipBatch*): one
+    launch per phase over all scenes.  Scene particles are slices of one set
+    of device arrays, like the engine's per-scene ranges of the factory's
+    particle block.  Scenes with kinematic frames are not supported here."""
+
+    def __init__(self, scenes, device="cuda:0", stream=None):
+        import torch
+
+        self.torch = torch
+        self.lib = lib = load_library()
+        self.scenes = scenes
+        self.device = dev = torch.device(device)
+        kinds = {sc["kind"] for sc in scenes}
+        iters = {sc["iterations"] for sc in scenes}
+        if len(kinds) != 1 or len(iters) != 1:
+            raise ValueError("a batch holds scenes of one solver kind and one iteration count")
+        if any(sc["frames"] for sc in scenes):
+            raise ValueError("HipBatch does not apply kinematic frames")
+        self.counts = [len(sc["xm"]) for sc in scenes]
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        ns = len(scenes)
+        descs = (SceneDesc * ns)()
+        for i, sc in enumerate(scenes):
+            descs[i] = SceneDesc(self.counts[i], int(self.offsets[i]), sc["dx"], _f3(*sc["lo"]), _f3(*sc["hi"]))
+        h = _vp()
+        check(lib.crHipBatchCreate(KIND_FROM_SCENE[scenes[0]["kind"]], ns, descs, ctypes.byref(h)), "batch create")
+        self.h = h
+        self.dims = []
+        for i in range(ns):
+            d = (ctypes.c_int * 3)()
+            check(lib.crHipBatchGetGridDims(h, i, d), "dims")
+            self.dims.append(tuple(d))
+        self.stream = stream if stream is not None else torch.cuda.current_stream(dev)
+        check(lib.crHipBatchSetStream(h, _vp(self.stream.cuda_stream)), "stream")
+        check(lib.crHipBatchSetSolverIterations(h, scenes[0]["iterations"]), "iterations")
+        cat = lambda key, dt: torch.from_numpy(np.ascontiguousarray(np.concatenate([sc[key] for sc in scenes]), dt)).to(dev)
+        self.xm = cat("xm", np.float32)
+        self.v = cat("v", np.float32)
+        self.params = cat("params", np.float32)
+        self.type = cat("type", np.int32)
+        self.mask = cat("mask", np.uint8)
+        n = int(self.offsets[-1])
+        self.n = n
+        check(lib.crHipBatchBindParticleData(h, self.xm.data_ptr(), self.v.data_ptr(), n), "bind particles")
+        check(lib.crHipBatchBindParticleMaterials(h, self.params.data_ptr(), self.type.data_ptr(), n), "bind materials")
+        check(lib.crHipBatchBindActiveMask(h, self.mask.data_ptr()), "bind mask")
+        self.tables = []
+        for i, sc in enumerate(scenes):
+            check(lib.crHipBatchSetSceneGravity(h, i, *sc["gravity"]), "gravity")
+            check(lib.crHipBatchSetSceneStepSize(h, i, sc["dt"]), "dt")
+            check(lib.crHipBatchSetSceneContactModel(h, i, sc.get("contact_model", 0)), "contact model")
+            t = DeviceShapeTables(sc, dev)
+            self.tables.append(t)
+            check(lib.crHipBatchBindSceneShapes(h, i, t.num_shapes, t.ids.data_ptr(), ctypes.byref(t.sd),
+                                                ctypes.byref(t.gd), ctypes.byref(t.sdf)), "bind shapes")
+        check(lib.crHipBatchInitialize(h), "initialize")
+        for i in range(ns):
+            check(lib.crHipBatchSetSceneNumActiveParticles(h, i, self.counts[i]), "num active")
+        F, C, V0, stride = _vp(), _vp(), _vp(), ctypes.c_int()
+        check(lib.crHipBatchGetStateArrays(h, ctypes.byref(F), ctypes.byref(C), ctypes.byref(V0), ctypes.byref(stride)),
+              "state arrays")
+        self._state_ptrs = (F.value, C.value, V0.value, stride.value)
+
+    def compute_initial_data(self):
+        torch = self.torch
+        self._idx = []
+        for i, c in enumerate(self.counts):
+            idx = torch.arange(c, dtype=torch.int32, device=self.device)
+            self._idx.append(idx)
+            check(self.lib.crHipBatchComputeInitialData(self.h, i, c, idx.data_ptr()), "initial data")
+
+    def step(self, n=1):
+        for _ in range(n):
+            check(self.lib.crHipBatchStep(self.h), "batch step")
+
+    def set_graph_mode(self, on):
+        check(self.lib.crHipBatchSetGraphMode(self.h, 1 if on else 0), "graph mode")
+
+    def autotune(self, steps_per_trial=3):
+        t, u = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipBatchAutotune(self.h, steps_per_trial, ctypes.byref(t), ctypes.byref(u)), "autotune")
+        return t.value, u.value
+
+    def set_launch_sizes(self, tile_waves, update_waves):
+        check(self.lib.crHipBatchSetLaunchSizes(self.h, tile_waves, update_waves), "launch sizes")
+
+    def sync(self):
+        check(self.lib.crHipBatchFetchResults(self.h), "fetch results")
+
+    def state(self, i):
+        """State of scene i (as HipSolver.state)."""
+        torch = self.torch
+        self.sync()
+        F, C, V0, stride = self._state_ptrs
+        a, e = int(self.offsets[i]), int(self.offsets[i + 1])
+        out = dict(xm=self.xm[a:e].cpu().numpy().astype(np.float64), v=self.v[a:e, :3].cpu().numpy().astype(np.float64))
+        for name, ptr, rows in (("F", F, 9), ("C", C, 9), ("V0", V0, 1)):
+            t = torch.empty(rows * stride, dtype=torch.float32, device=self.device)
+            hip_memcpy_d2d(t.data_ptr(), ptr, rows * stride * 4, self.stream.cuda_stream)
+            arr = t.cpu().numpy().reshape(rows, stride)[:, a:e].T.astype(np.float64)
+            out[name] = arr if rows > 1 else arr[:, 0]
+        return out
+
+    def grid(self, i):
+        torch = self.torch
+        d = self.dims[i]
+        g = torch.empty((d[0] * d[1] * d[2], 4), dtype=torch.float32, device=self.device)
+        check(self.lib.crHipBatchCopyGrid(self.h, i, g.data_ptr()), "copy grid")
+        self.sync()
+        return g.cpu().numpy().astype(np.float64)
+
+    def profile_begin(self, max_launches):
+        check(self.lib.crHipBatchProfileBegin(self.h, max_launches), "profile begin")
+
+    def profile_end(self):
+        n = len(PROF_CATEGORIES)
+        ms = (ctypes.c_float * n)()
+        cnt = (ctypes.c_int * n)()
+        check(self.lib.crHipBatchProfileEnd(self.h, ms, cnt, n), "profile end")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_CATEGORIES)}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.crHipBatchDestroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -246,3 +246,75 @@ def small_scene(kind, seed, n_side=8, shapes="plane", dx=1.0 / 32, jitter=0.25, 
                   params0=(0.0, 1.0, 0.0, 0.001))
     sc["steps"] = 10
     return sc
+
+
+def config3(n_scenes=64, kind=MLS, steps=100, seed=20261203 + 3, n_min=2000, n_max=200000, iterations=20):
+    """Batched scenes: n_scenes independent soft bodies with particle counts
+    log-uniform in [n_min, n_max] and grids of G = 4 round(U(8, 32)) cells per
+    axis over the unit box.  Each body is a box or a sphere (equal odds) at
+    8 particles per cell (lattice spacing dx / 2, enlarged when the body would
+    exceed half the box), E ~ U(5e3, 5e4), nu ~ U(0.3, 0.45), falling at
+    0.5 m/s onto one static SDF box (half extents U(0.05, 0.2)) or capsule
+    (radius U(0.02, 0.08), half height U(0.05, 0.2)) below it.  MLS uses the
+    co-rotational model (StVK, as the reference evaluates it); PB uses
+    Neo-Hookean with (alpha, r) = (0.5 / dt, 0.9) and `iterations` iterations.
+    dt = 1e-4."""
+    rng = SplitMix64(seed)
+    dt = 1e-4
+    scenes = []
+    for s in range(n_scenes):
+        u = rng.uniform(9)
+        n_target = int(round(math.exp(math.log(n_min) + u[0] * (math.log(n_max) - math.log(n_min)))))
+        G = 4 * int(round(8 + u[1] * 24))
+        dx = 1.0 / G
+        is_sphere = u[2] < 0.5
+        E = 5e3 + u[3] * 4.5e4
+        nu = 0.3 + u[4] * 0.15
+        h = dx / 2
+        if is_sphere:
+            k = (3.0 * n_target / (4.0 * math.pi)) ** (1.0 / 3.0)
+            radius = min(k * h, 0.25)
+            h = radius / k
+            m = int(math.ceil(2 * radius / h)) + 1
+            ii, jj, kk = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+            off = (np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1) + 0.5) * h - m * h / 2
+            pts = off[np.linalg.norm(off, axis=1) <= radius]
+            half = (radius, radius, radius)
+        else:
+            n_side = max(2, int(round(n_target ** (1.0 / 3.0))))
+            h = min(h, 0.5 / n_side)
+            ii, jj, kk = np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij")
+            pts = (np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1) + 0.5) * h - n_side * h / 2
+            half = (n_side * h / 2,) * 3
+        N = len(pts)
+        jit = rng.uniform(3 * N, -0.25, 0.25).reshape(N, 3) * h
+        sc = new_scene(kind, dx, (0, 0, 0), (1, 1, 1), N, dt, iterations=iterations)
+        # obstacle centred under the body, its top at y = 0.3
+        if u[5] < 0.5:
+            hx, hy, hz = 0.05 + u[6] * 0.15, 0.05 + u[7] * 0.15, 0.05 + u[8] * 0.15
+            obs_top = 0.3
+            add_shape(sc, BOX, (hx, hy, hz, 0), pos=(0.5, obs_top - hy, 0.5), params0=(0.0, 1.0, 0.3, 0.0))
+        else:
+            r = 0.02 + u[6] * 0.06
+            hh = 0.05 + u[7] * 0.15
+            obs_top = 0.3
+            # capsule axis along local y rotated to lie along x
+            a = math.radians(90.0)
+            add_shape(sc, CAPSULE, (r, hh, 0, 0), pos=(0.5, obs_top - r, 0.5),
+                      rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), params0=(0.0, 1.0, 0.3, 0.0))
+        centre = np.array([0.5, obs_top + 2 * dx + half[1], 0.5])
+        sc["xm"][:, :3] = centre + pts + jit
+        sc["xm"][:, 3] = 1000.0 * h ** 3
+        sc["v"][:, 1] = -0.5
+        if kind == PB:
+            sc["params"][:, 0] = 0.5 / dt
+            sc["params"][:, 1] = 0.9
+            sc["type"][:] = NEOHOOKEAN
+        else:
+            lam, mu = lame(E, nu)
+            sc["params"][:, 0] = lam
+            sc["params"][:, 1] = mu
+            sc["type"][:] = COROTATIONAL
+        sc["steps"] = steps
+        scenes.append(sc)
+    return scenes

@@ -170,6 +170,70 @@ int crHipSolverProfileMask(CrHipSolver *s, unsigned int mask);
  * device float4 array in node order x + y * nx + z * nx * ny. */
 int crHipSolverCopyGrid(CrHipSolver *s, float *dst);
 
+/* ------------------------------------------------------------------------
+ * Batched scenes.  A batch advances many scenes of one solver kind together:
+ * each phase of a substep (binning, sort, P2G, grid update, G2P; PB: its
+ * iteration kernels) is one launch over the pblocks of every scene, so small
+ * scenes share the GPU instead of each paying its own launch sequence.  The
+ * scenes' particles are slices of one set of bound particle arrays, as the
+ * engine's per-scene particle ranges are slices of the factory's particle
+ * block (src/engine/scene_impl.h:143-160); scenes may differ in particle
+ * count, grid and parameters.  Replaces SimulationFactory::advanceAll's one
+ * solver step per scene (src/engine/simulation_factory_impl.cpp:1106-1150).
+ * A CrHipSolver is a batch of one scene. */
+typedef struct CrHipBatch CrHipBatch;
+
+typedef struct CrHipSceneDesc
+{
+    int numMaxParticles; /* particles of the scene (its slice of the bound arrays) */
+    int particleOffset;  /* index of the scene's first particle in the bound arrays */
+    float cellSize;      /* SceneDesc::gridCellSize */
+    float boundMin[3];   /* SceneDesc::gridBounds */
+    float boundMax[3];
+} CrHipSceneDesc;
+
+int crHipBatchCreate(int kind, int numScenes, const CrHipSceneDesc *scenes, CrHipBatch **out);
+int crHipBatchDestroy(CrHipBatch *b);
+int crHipBatchSetStream(CrHipBatch *b, void *stream);
+/* PB-MPM iterations, one count for the whole batch */
+int crHipBatchSetSolverIterations(CrHipBatch *b, int n);
+/* Bound arrays hold every scene's slice: size >= max(particleOffset + numMaxParticles). */
+int crHipBatchBindParticleData(CrHipBatch *b, float *positionMass, float *velocity, int size);
+int crHipBatchBindParticleMaterials(CrHipBatch *b, float *params0, int *type, int size);
+int crHipBatchBindActiveMask(CrHipBatch *b, unsigned char *activeMask);
+int crHipBatchInitialize(CrHipBatch *b);
+int crHipBatchGetGridDims(const CrHipBatch *b, int scene, int dims[3]);
+/* Per-scene settings (MpmSolverBase setters of each scene's solver) */
+int crHipBatchSetSceneGravity(CrHipBatch *b, int scene, float gx, float gy, float gz);
+int crHipBatchSetSceneStepSize(CrHipBatch *b, int scene, float dt);
+int crHipBatchSetSceneContactModel(CrHipBatch *b, int scene, int model);
+/* Particles [particleOffset, particleOffset + n) of the scene take part. */
+int crHipBatchSetSceneNumActiveParticles(CrHipBatch *b, int scene, unsigned int n);
+/* A scene's shapes: setNumShapes + bindShapeIds + bindShapeData + bindGeometryData
+ * + bindGeometrySdfData of its solver (null table pointers keep the previous binding). */
+int crHipBatchBindSceneShapes(CrHipBatch *b, int scene, int numShapes, int *shapeIds, const CrHipShapeData *shapes,
+                              const CrHipGeometryData *geometries, const CrHipGeometrySdfData *sdf);
+/* computeInitialData of one scene; indices are scene-local particle indices (device). */
+int crHipBatchComputeInitialData(CrHipBatch *b, int scene, unsigned int n, const unsigned int *indices);
+/* One substep of every scene, enqueued on the batch stream. */
+int crHipBatchStep(CrHipBatch *b);
+/* With graph mode on (and a non-null stream), crHipBatchStep captures its
+ * launches into a hipGraph once and replays it while the launch configuration
+ * (active particle count, iterations, launch sizes, bindings) is unchanged. */
+int crHipBatchSetGraphMode(CrHipBatch *b, int enabled);
+/* Autotuner: times substeps of the batch's current state (saved and restored
+ * around the trials, so the state is unchanged afterwards) for several launch
+ * sizes of the pblock kernels and keeps the fastest.  Launch sizes are in
+ * wavefronts; 0 means one wavefront per pblock. */
+int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves);
+int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
+int crHipBatchFetchResults(CrHipBatch *b);
+int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, int *stride);
+int crHipBatchCopyGrid(CrHipBatch *b, int scene, float *dst);
+int crHipBatchProfileBegin(CrHipBatch *b, int maxLaunches);
+int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask);
+int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories);
+
 #ifdef __cplusplus
 }
 #endif

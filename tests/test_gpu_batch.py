@@ -1,0 +1,126 @@
+"""Batched scenes (crHipBatch*): one launch per phase over all scenes.
+
+A scene in a batch runs the same per-tile arithmetic in the same order as
+when it runs alone, so its trajectory must match the single-scene solver's;
+the tolerance (relative 1e-6 after 10 substeps) only absorbs the float
+atomics of computeInitialData's mass scatter, whose order is not fixed.
+Graph replay and autotuned launch sizes must not change results at all.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import crmpm_scenes as S
+
+pytestmark = pytest.mark.gpu
+
+KINDS = {"mls": S.MLS, "pb": S.PB, "std": S.STD}
+FIELDS = ("xm", "v", "F", "C", "V0")
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-30)
+
+
+def mixed_scenes(kind):
+    """Five scenes of different sizes, grids and shapes."""
+    specs = [(6, 1 / 32, "plane"), (10, 1 / 24, "plane+sphere+box"), (4, 1 / 40, "plane+capsule"),
+             (8, 1 / 32, "plane+slicer"), (12, 1 / 28, "plane")]
+    out = []
+    for i, (n_side, dx, shapes) in enumerate(specs):
+        sc = S.small_scene(kind, 100 + i, n_side=n_side, dx=dx, shapes=shapes)
+        out.append(sc)
+    return out
+
+
+def single_runs(scenes, steps):
+    import crmpm_hip as H
+
+    res = []
+    for sc in scenes:
+        g = H.HipSolver(copy.deepcopy(sc))
+        g.compute_initial_data()
+        g.step(steps)
+        res.append(g.state())
+        g.close()
+    return res
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_batch_matches_single_scenes(kind):
+    import crmpm_hip as H
+
+    scenes = mixed_scenes(KINDS[kind])
+    print(f"batch {kind}: {len(scenes)} scenes, {sum(len(s['xm']) for s in scenes)} particles", flush=True)
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    b.step(10)
+    alone = single_runs(scenes, 10)
+    for i in range(len(scenes)):
+        st = b.state(i)
+        errs = {k: rel(st[k], alone[i][k]) for k in FIELDS}
+        print(f"  scene {i}: " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+        for k in FIELDS:
+            assert errs[k] <= 1e-6, (i, k, errs[k])
+    b.close()
+
+
+@pytest.mark.parametrize("kind", ["mls", "pb"])
+def test_graph_replay_matches_eager(kind):
+    """Each replay of the captured substep equals the eager substep, with eager
+    work and synchronizations between replays."""
+    import torch
+
+    import crmpm_hip as H
+
+    scenes = mixed_scenes(KINDS[kind])
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        eager = H.HipBatch(copy.deepcopy(scenes), stream=stream)
+        graph = H.HipBatch(copy.deepcopy(scenes), stream=stream)
+        eager.compute_initial_data()
+        graph.compute_initial_data()
+        graph.set_graph_mode(True)
+        scratch = torch.zeros(1 << 16, device="cuda")
+        for r in range(4):
+            print(f"graph {kind} replay {r}", flush=True)
+            eager.step(1)
+            graph.step(1)
+            scratch.add_(1.0)  # unrelated eager work between replays
+            torch.cuda.synchronize()
+            for i in range(len(scenes)):
+                a, b = eager.state(i), graph.state(i)
+                for k in FIELDS:
+                    assert rel(b[k], a[k]) <= 1e-6, (r, i, k)
+        eager.close()
+        graph.close()
+
+
+def test_autotune_keeps_state_and_results():
+    import crmpm_hip as H
+
+    scenes = mixed_scenes(S.PB)
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    b.step(2)
+    before = [b.state(i) for i in range(len(scenes))]
+    t, u = b.autotune(steps_per_trial=2)
+    print(f"autotune picked tile waves {t}, update waves {u}", flush=True)
+    after = [b.state(i) for i in range(len(scenes))]
+    for i in range(len(scenes)):
+        for k in FIELDS:
+            assert np.array_equal(before[i][k], after[i][k]), (i, k)
+    # every launch size gives the same arithmetic: persistent launches against one wavefront per pblock
+    ref = H.HipBatch(copy.deepcopy(scenes))
+    ref.compute_initial_data()
+    ref.step(2)
+    b.set_launch_sizes(2048, 2048)
+    b.step(3)
+    ref.step(3)
+    for i in range(len(scenes)):
+        x, y = b.state(i), ref.state(i)
+        for k in FIELDS:
+            assert rel(x[k], y[k]) <= 1e-6, (i, k)
+    b.close()
+    ref.close()

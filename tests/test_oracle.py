@@ -265,3 +265,17 @@ def test_capi_argument_errors_without_gpu():
     assert lib.crHipSolverSetNumActiveParticles(h, 11) == 1
     assert lib.crHipSolverSetShapeContactModel(h, 3) == 1
     assert lib.crHipSolverDestroy(h) == 0
+
+
+def test_config3_generator():
+    """Batched workload generator: deterministic, 64 scenes, counts in range, bodies inside the box."""
+    a = S.config3()
+    b = S.config3()
+    assert len(a) == 64
+    for x, y in zip(a, b):
+        assert np.array_equal(x["xm"], y["xm"])
+        n = len(x["xm"])
+        assert 1000 <= n <= 200000
+        assert x["xm"][:, :3].min() > 3 * x["dx"] and x["xm"][:, :3].max() < 1 - 3 * x["dx"]
+        G = round(1 / x["dx"])
+        assert G % 4 == 0 and 32 <= G <= 128
