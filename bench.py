@@ -209,14 +209,83 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
             "hip_graph": graph, "finite": bool(finite)}
 
 
+def adjoint(steps=100, every=10):
+    """Differentiable MLS block (config4: 65,536 particles, 96^3 grid, GRF log E):
+    time of a forward rollout with checkpoints, of forward + adjoint, and of the
+    plain solver's forward, for the same substeps."""
+    import torch
+
+    import crmpm_hip as H
+
+    sc, log_e = S.config4(steps=steps)
+    hidden, log_e_true = S.config4(steps=steps, field_seed=20261203 + 44)
+    sim = H.DifferentiableMls(sc, steps, checkpoint_every=every)
+    dev = sim.solver.device
+    nu = 0.3
+    v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
+
+    def lame_t(le):
+        E = torch.exp(torch.as_tensor(le, dtype=torch.float64, device=dev))
+        return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+    lt, mt = lame_t(log_e_true)
+    target = sim.forward(lt, mt, v0).double()
+    lam, mu = lame_t(log_e)
+    n = sim.n
+
+    def fwd():
+        return sim.forward(lam, mu, v0)
+
+    def fwd_adj():
+        x = sim.forward(lam, mu, v0)
+        gx = 2.0 * (x.double() - target) / n
+        return sim.backward(gx)
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    t_f = timed(fwd)
+    t_fa = timed(fwd_adj)
+    x = fwd().double()
+    loss = float(((x - target) ** 2).sum(1).mean())
+    dlam, dmu, dv0 = fwd_adj()
+    g = (dlam * lam + dmu * mu)
+    finite = bool(torch.isfinite(g).all() and torch.isfinite(dv0).all())
+    sim.close()
+    plain = H.HipSolver(copy.deepcopy(sc))
+    plain.compute_initial_data()
+    plain.step(5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.step(steps)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t0
+    plain.close()
+    return {"workload": "MLS block 64x32x32 = 65,536 particles, 96^3 grid, GRF log E, loss MSE to a hidden-field rollout",
+            "particles": n, "substeps": steps, "checkpoint_every": every,
+            "plain_forward_ms_per_substep": t_p / steps * 1e3,
+            "checkpointed_forward_ms_per_substep": t_f / steps * 1e3,
+            "forward_adjoint_ms_per_substep": t_fa / steps * 1e3,
+            "forward_adjoint_vs_plain_forward": t_fa / t_p, "loss": loss,
+            "grad_logE_abs_max": float(g.abs().max()), "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched"], default="all",
-                    help="all: the PB slab headline line plus the batched comparison")
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint"], default="all",
+                    help="all: the PB slab headline line plus the batched and adjoint comparisons")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
@@ -255,6 +324,9 @@ def main():
 
     import crmpm_hip as H
 
+    if args.workload == "adjoint":
+        print(json.dumps({"adjoint": adjoint()}), flush=True)
+        return
     if args.workload == "batched":
         print(json.dumps({"batched_mls": batched("mls", 100, 5), "batched_pb": batched("pb", 20, 2)}), flush=True)
         return
@@ -347,6 +419,10 @@ def main():
                 out[key] = batched(kind, steps, warm)
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
+        try:
+            out["adjoint"] = adjoint()
+        except Exception as e:
+            out["adjoint"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

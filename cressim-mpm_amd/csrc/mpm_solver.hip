@@ -1587,6 +1587,394 @@ namespace
 
 
 // ----------------------------------------------------------------------
+// Adjoint of the MLS-MPM substep (reverse mode), for one scene without
+// rigid shapes.  Forward per substep (src/solver/mls_mpm_algorithm.h):
+//   P2G   v' = v + g dt, A = s tau(F) + m C (s = -4 dt V0 / dx^2),
+//         p_i = sum_p w (m v' + A d),  m_i = sum_p w m          (d = x_i - x_p)
+//   grid  u_i = p_i / m_i with the wall band zeroed, or u_i = p_i when m_i < 1e-4
+//   G2P   v+ = sum w u_i, C+ = (4 / dx) sum w d u_i^T, x+ = clamp(x + dt v+),
+//         F+ = (I + dt C+) F
+// The kernels below apply the transposed Jacobians in the opposite order:
+// k_adj_g2p (per particle, scatters the grid velocity adjoint through the
+// LDS tile), k_adj_grid (per node), k_adj_p2g (per particle, gathers the
+// momentum and mass adjoints, then the stress adjoint).  Parameter adjoints
+// (lambda, mu) are accumulated per particle in FP64.
+// ----------------------------------------------------------------------
+namespace
+{
+    CR_DEV float mdot(const M3 &a, const M3 &b) { return dot(a.c0, b.c0) + dot(a.c1, b.c1) + dot(a.c2, b.c2); }
+
+    // Particle state (x|m, v, F, C) in the solver's layouts.
+    struct StateRef
+    {
+        const float4 *x;
+        const float4 *v;
+        const float *F;
+        const float *C;
+        int stride;
+    };
+
+    // Adjoint of the particle state, indexed by particle.
+    struct AdjRef
+    {
+        float4 *x;
+        float4 *v;
+        float *F;
+        float *C;
+        float4 *xTmp; // G2P-adjoint part of the x adjoint
+        float *FTmp;  // G2P-adjoint part of the F adjoint
+        double *lambdaBar;
+        double *muBar;
+        int stride;
+    };
+
+    struct NodeW
+    {
+        float w[3][3], dw[3][3]; // [i][axis]: weight and d weight / d fraction
+    };
+    CR_DEV void node_weights(V3 fx, NodeW &n)
+    {
+        const float f[3] = {fx.x, fx.y, fx.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            bspline_w(f[a], n.w[0][a], n.w[1][a], n.w[2][a]);
+            bspline_dw(f[a], n.dw[0][a], n.dw[1][a], n.dw[2][a]);
+        }
+    }
+
+    // Visit the 27 nodes with weight, position gradient of the weight
+    // (d w / d x_p) and d = x_i - x_p; out-of-grid nodes are skipped.
+    template <class Fn>
+    CR_DEV void for_each_node_grad(const GridGeom &g, const TileInfo &ti, int bx, int by, int bz, V3 fx, const NodeW &n,
+                                   Fn &&fn)
+    {
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+            for (int b = 0; b < 3; ++b)
+#pragma unroll 3
+                for (int c = 0; c < 3; ++c)
+                {
+                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    if (!in_grid(g, nx, ny, nz))
+                        continue;
+                    const int li = tile_local(ti, nx, ny, nz);
+                    if (li < 0)
+                        continue;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, c);
+                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, b), dc = dw_of(fx.z, c);
+                    const float wt = wa * wb * wc;
+                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                    const V3 dist = (V3{(float)a, (float)b, (float)c} - fx) * g.dx;
+                    fn(li, dist, wt, gw);
+                }
+    }
+
+    // G2P adjoint of one substep.  s0: state at the substep's start, s1: at
+    // its end; grid: the substep's updated grid; the adjoint of s1 comes in
+    // through ad and the G2P parts of the x and F adjoints go to ad.xTmp /
+    // ad.FTmp; the grid velocity adjoint goes to the tile partials.
+    __global__ __launch_bounds__(kWG) void k_adj_g2p(BatchArgs b, StateRef s0, StateRef s1, const float4 *grid, AdjRef ad)
+    {
+        __shared__ float4 acc[kWaves][kTileNodes];
+        __shared__ float4 lg[kWaves][kTileNodes];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<false>(tg, b.totalBlocks))
+        {
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            const GridGeom &g = c.g;
+            float4 *A = acc[wave];
+            float4 *G = lg[wave];
+            for (int i = lane; i < kTileNodes; i += 64)
+                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            load_tile_grid(g, ti, grid + (c.grid - b.grid), G, lane);
+            wave_sync();
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                const V3 x = v3(s0.x[p]);
+                const V3 vn = v3(s1.v[p]);
+                const M3 Cn = load_m3(s1.C, s1.stride, p);
+                const M3 F = load_m3(s0.F, s0.stride, p);
+                const V3 xb = v3(ad.x[p]);
+                const V3 vb = v3(ad.v[p]);
+                const M3 Fb = load_m3(ad.F, ad.stride, p);
+                const M3 Cb = load_m3(ad.C, ad.stride, p);
+                // x+ = clamp(x + dt v+): the adjoint passes where the clamp is inactive
+                const V3 xp = x + vn * c.dt;
+                const V3 keep{(xp.x > g.bmin.x && xp.x < g.bmax.x) ? 1.f : 0.f,
+                              (xp.y > g.bmin.y && xp.y < g.bmax.y) ? 1.f : 0.f,
+                              (xp.z > g.bmin.z && xp.z < g.bmax.z) ? 1.f : 0.f};
+                const V3 xbk = xb * keep;
+                const V3 vaccb = vb + xbk * c.dt;
+                // F+ = (I + dt C+) F
+                const M3 Bb = (Cb + mul(Fb, transpose(F)) * c.dt) * (4.0f * g.invdx);
+                store_m3(ad.FTmp, ad.stride, p, mul(transpose(m3_identity() + Cn * c.dt), Fb));
+                V3 xacc = xbk;
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, x, bx, by, bz, fx);
+                NodeW nw;
+                node_weights(fx, nw);
+                const M3 BbT = transpose(Bb);
+                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                    for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
+                        const V3 u = v3(G[li]);
+                        const V3 ub = (vaccb + mul(BbT, dist)) * wt;
+                        lds_add(A, li, ub.x, ub.y, ub.z, 0.0f);
+                        const V3 Bu = mul(Bb, u);
+                        const float wbar = dot(vaccb, u) + dot(dist, Bu);
+                        xacc = xacc + gw * wbar - Bu * wt;
+                    });
+                });
+                ad.xTmp[p] = make_float4(xacc.x, xacc.y, xacc.z, 0.0f);
+            }
+            wave_sync();
+            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+            for (int i = lane; i < kTileNodes; i += 64)
+                out[i] = A[i];
+        }
+    }
+
+    // Grid adjoint: sum the tile partials of the velocity adjoint (fixed
+    // order), then through u = p / m (or u = p below the mass cutoff) to the
+    // momentum and mass adjoints (p_bar xyz, m_bar w).
+    __global__ __launch_bounds__(kWG) void k_adj_grid(BatchArgs b, const float4 *grid, float4 *gridAdj)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg += tile_stride())
+        {
+            int nb;
+            const StepCommon c = scene_view(b, nbg, nb);
+            const GridGeom &g = c.g;
+            int bx, by, bz;
+            decode_block(g, nb, bx, by, bz);
+            bool src[8];
+            bool any = false;
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+            {
+                const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
+                const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
+                src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
+                any = any || src[d];
+            }
+            if (!any)
+                continue;
+            const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+            V3 ub{0.f, 0.f, 0.f};
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+            {
+                const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+                const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
+                if (src[d] && ux < kTN && uy < kTN && uz < kTN)
+                {
+                    const size_t at =
+                        (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
+                    ub = ub + v3(c.tilebuf[at]);
+                }
+            }
+            const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
+            if (!in_grid(g, i, j, k))
+                continue;
+            const size_t gi = (size_t)(c.grid - b.grid) + (size_t)nb * 64 + lane;
+            const float4 u = grid[gi];
+            float4 r;
+            if (u.w < 1e-4f)
+                r = make_float4(ub.x, ub.y, ub.z, 0.0f);
+            else
+            {
+                const bool kx = !(i < 2 || i > g.nx - 3), ky = !(j < 2 || j > g.ny - 3), kz = !(k < 2 || k > g.nz - 3);
+                r.x = kx ? ub.x / u.w : 0.0f;
+                r.y = ky ? ub.y / u.w : 0.0f;
+                r.z = kz ? ub.z / u.w : 0.0f;
+                r.w = -(ub.x * u.x + ub.y * u.y + ub.z * u.z) / u.w;
+            }
+            gridAdj[gi] = r;
+        }
+    }
+
+    // FP64 3x3 matrices for the affine and stress adjoints (row-major d[r][c]).
+    struct D3
+    {
+        double d[3][3];
+    };
+    CR_DEV D3 d3_zero()
+    {
+        D3 a;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                a.d[r][c] = 0.0;
+        return a;
+    }
+    CR_DEV D3 d3_of(const M3 &m)
+    {
+        const V3 col[3] = {m.c0, m.c1, m.c2};
+        D3 a;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            a.d[0][c] = col[c].x;
+            a.d[1][c] = col[c].y;
+            a.d[2][c] = col[c].z;
+        }
+        return a;
+    }
+    CR_DEV M3 m3_of(const D3 &a)
+    {
+        return M3{V3{(float)a.d[0][0], (float)a.d[1][0], (float)a.d[2][0]},
+                  V3{(float)a.d[0][1], (float)a.d[1][1], (float)a.d[2][1]},
+                  V3{(float)a.d[0][2], (float)a.d[1][2], (float)a.d[2][2]}};
+    }
+    CR_DEV D3 d3_mul(const D3 &a, const D3 &b)
+    {
+        D3 o;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                o.d[r][c] = a.d[r][0] * b.d[0][c] + a.d[r][1] * b.d[1][c] + a.d[r][2] * b.d[2][c];
+        return o;
+    }
+    CR_DEV D3 d3_t(const D3 &a)
+    {
+        D3 o;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                o.d[r][c] = a.d[c][r];
+        return o;
+    }
+    CR_DEV double d3_tr(const D3 &a) { return a.d[0][0] + a.d[1][1] + a.d[2][2]; }
+    CR_DEV double d3_dot(const D3 &a, const D3 &b)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                s += a.d[r][c] * b.d[r][c];
+        return s;
+    }
+    // a x + b y + z I
+    CR_DEV D3 d3_comb(const D3 &a, double x, const D3 &b, double y, double z)
+    {
+        D3 o;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                o.d[r][c] = a.d[r][c] * x + b.d[r][c] * y + (r == c ? z : 0.0);
+        return o;
+    }
+
+    // P2G adjoint of one substep: gathers (p_bar, m_bar) from the LDS tile,
+    // then the affine, stress and parameter adjoints; writes the adjoint of
+    // the substep's start state.
+    __global__ __launch_bounds__(kWG) void k_adj_p2g(BatchArgs b, StateRef s0, const float4 *gridAdj, AdjRef ad)
+    {
+        __shared__ float4 lg[kWaves][kTileNodes];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<false>(tg, b.totalBlocks))
+        {
+            int t;
+            const StepCommon c = scene_view(b, tg, t);
+            TileInfo ti;
+            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+                continue;
+            const GridGeom &g = c.g;
+            float4 *G = lg[wave];
+            load_tile_grid(g, ti, gridAdj + (c.grid - b.grid), G, lane);
+            wave_sync();
+            for (int k = ti.start + lane; k < ti.end; k += 64)
+            {
+                const int p = (int)c.order[k];
+                const float4 xm = s0.x[p];
+                const V3 x = v3(xm);
+                const float m = xm.w;
+                const V3 vel = v3(s0.v[p]) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                const M3 F = load_m3(s0.F, s0.stride, p);
+                const M3 Cm = load_m3(s0.C, s0.stride, p);
+                const float4 prm = c.P.params[p];
+                const int type = c.P.type[p];
+                const M3 tau = stress_kirchhoff(prm.x, prm.y, type, F);
+                const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
+                const M3 A = tau * sc + Cm * m;
+                const M3 AT = transpose(A);
+                const V3 vm = vel * m;
+                V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
+                D3 Ab = d3_zero(); // affine adjoint in FP64: its trace is a first moment with cancellation
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, x, bx, by, bz, fx);
+                NodeW nw;
+                node_weights(fx, nw);
+                for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
+                    const float4 q = G[li];
+                    const V3 pb = v3(q);
+                    vpb = vpb + pb * (wt * m);
+                    const double pr[3] = {pb.x, pb.y, pb.z}, dc[3] = {dist.x, dist.y, dist.z};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc)
+                            Ab.d[r][cc] += (double)wt * pr[r] * dc[cc];
+                    const float wbar = dot(pb, vm + mul(A, dist)) + q.w * m;
+                    xacc = xacc + gw * wbar - mul(AT, pb) * wt;
+                });
+                // A = s tau + m C; the stress and parameter adjoints in FP64
+                const D3 taub = d3_comb(Ab, (double)sc, Ab, 0.0, 0.0);
+                const D3 Fd = d3_of(F), FdT = d3_t(Fd);
+                const double lam = prm.x, muv = prm.y;
+                D3 Fb = d3_of(load_m3(ad.FTmp, ad.stride, p));
+                double lb = 0.0, mb = 0.0;
+                if (type == mNeoHookean)
+                {
+                    // tau = mu (F F^T - I) + lambda ln(J) I, J = max(det F, 1e-6)
+                    const float Jr = det(F);
+                    const double J = Jr < 1e-6f ? 1e-6 : (double)Jr;
+                    Fb = d3_comb(Fb, 1.0, d3_mul(d3_comb(taub, 1.0, d3_t(taub), 1.0, 0.0), Fd), muv, 0.0);
+                    if (Jr >= 1e-6f) // d ln(J) / dF = F^-T (inverse_t)
+                        Fb = d3_comb(Fb, 1.0, d3_of(inverse_t(F)), lam * d3_tr(taub), 0.0);
+                    mb = d3_dot(taub, d3_comb(d3_mul(Fd, FdT), 1.0, Fd, 0.0, -1.0));
+                    lb = d3_tr(taub) * log(J);
+                }
+                else if (type == mCoRotational || type == mLinearElastic)
+                {
+                    // tau = F S F^T, S = lambda tr(E) I + 2 mu E, E = (F^T F - I) / 2
+                    const D3 E = d3_comb(d3_mul(FdT, Fd), 0.5, Fd, 0.0, -0.5);
+                    const D3 S = d3_comb(E, 2.0 * muv, E, 0.0, lam * d3_tr(E));
+                    const D3 Sb = d3_mul(d3_mul(FdT, taub), Fd);
+                    const D3 Eb = d3_comb(Sb, 2.0 * muv, Sb, 0.0, lam * d3_tr(Sb));
+                    const D3 t1 = d3_mul(d3_mul(taub, Fd), d3_t(S));
+                    const D3 t2 = d3_mul(d3_mul(d3_t(taub), Fd), S);
+                    const D3 t3 = d3_mul(Fd, d3_comb(Eb, 0.5, d3_t(Eb), 0.5, 0.0));
+                    Fb = d3_comb(d3_comb(Fb, 1.0, t1, 1.0, 0.0), 1.0, d3_comb(t2, 1.0, t3, 1.0, 0.0), 1.0, 0.0);
+                    lb = d3_tr(E) * d3_tr(Sb);
+                    mb = 2.0 * d3_dot(Sb, E);
+                }
+                const V3 xt = v3(ad.xTmp[p]) + xacc;
+                ad.x[p] = make_float4(xt.x, xt.y, xt.z, 0.0f);
+                ad.v[p] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
+                store_m3(ad.F, ad.stride, p, m3_of(Fb));
+                store_m3(ad.C, ad.stride, p, m3_of(d3_comb(Ab, (double)m, Ab, 0.0, 0.0)));
+                ad.lambdaBar[p] += lb;
+                ad.muBar[p] += mb;
+            }
+        }
+    }
+} // namespace
+
+// ----------------------------------------------------------------------
 // Batch object behind the C-ABI.  A batch is a list of scenes of one solver
 // kind advanced together: every phase of a substep is one launch over all
 // scenes.  The single-scene solver (CrHipSolver) is a batch of one.
@@ -1677,6 +2065,8 @@ struct CrHipBatch
 
     // launch sizes in wavefronts; 0 = one wavefront per pblock
     int tileWaves = 0, updateWaves = 0;
+    // tile-ordered particle permutation of the last enqueued substep
+    const unsigned *lastOrder = nullptr;
 
     // hipGraph replay of the substep
     bool graphMode = false;
@@ -1934,6 +2324,7 @@ namespace
         int rc = build_tiles(b, n, b->kind != CRMPM_HIP_STANDARD, &order, &keys);
         if (rc)
             return rc;
+        b->lastOrder = order;
         const BatchArgs a = batch_args(b, order);
         const int tileWGs = pblock_wgs(b, b->tileWaves);
         const int updWGs = pblock_wgs(b, b->updateWaves);
@@ -2895,3 +3286,260 @@ int crHipSolverCopyGrid(CrHipSolver *s, float *dst)
 }
 
 } // extern "C"
+
+// ----------------------------------------------------------------------
+// Adjoint driver: checkpointed forward, segment-wise recomputation and
+// reverse sweep (one scene, MLS-MPM, no rigid shapes).
+// ----------------------------------------------------------------------
+namespace
+{
+    struct StateBuf
+    {
+        float4 *x = nullptr, *v = nullptr;
+        float *F = nullptr, *C = nullptr;
+    };
+} // namespace
+
+struct CrHipAdjoint
+{
+    CrHipSolver *solver = nullptr;
+    int steps = 0, every = 0, segments = 0;
+    bool forwardDone = false;
+    StateBuf *ckpt = nullptr;  // state at substeps 0, K, 2K, ... and the final state
+    StateBuf *seg = nullptr;   // states of the substeps of one segment (K + 1)
+    float4 **segGrid = nullptr;
+    unsigned **segOrder = nullptr;
+    int **segStart = nullptr, **segEnd = nullptr;
+    float4 *gridAdj = nullptr;
+    float4 *xTmp = nullptr;
+    float *FTmp = nullptr;
+};
+
+namespace
+{
+    int state_alloc(StateBuf &s, size_t N)
+    {
+        int rc = dalloc(&s.x, N);
+        rc |= dalloc(&s.v, N);
+        rc |= dalloc(&s.F, 9 * N);
+        rc |= dalloc(&s.C, 9 * N);
+        return rc;
+    }
+    void state_free(StateBuf &s)
+    {
+        dfree(s.x);
+        dfree(s.v);
+        dfree(s.F);
+        dfree(s.C);
+    }
+    // live particle state <-> buffer (whole arrays: x|m, v, F, C)
+    int state_copy(CrHipBatch *b, StateBuf &s, bool save)
+    {
+        const size_t N = (size_t)std::max(b->capacity, 1);
+        auto cp = [&](void *live, void *buf, size_t bytes) {
+            return hipMemcpyAsync(save ? buf : live, save ? live : buf, bytes, hipMemcpyDeviceToDevice, b->stream) ==
+                   hipSuccess;
+        };
+        const bool ok = cp(b->P.xm, s.x, N * sizeof(float4)) && cp(b->P.v, s.v, N * sizeof(float4)) &&
+                        cp(b->F, s.F, 9 * N * sizeof(float)) && cp(b->C, s.C, 9 * N * sizeof(float));
+        return ok ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+    }
+    StateRef state_ref(const StateBuf &s, int stride) { return StateRef{s.x, s.v, s.F, s.C, stride}; }
+
+    int adjoint_destroy(CrHipAdjoint *a)
+    {
+        if (a->solver && a->solver->b->initialized)
+            (void)hipStreamSynchronize(a->solver->b->stream);
+        for (int i = 0; a->ckpt && i < a->segments + 1; ++i)
+            state_free(a->ckpt[i]);
+        for (int i = 0; a->seg && i < a->every + 1; ++i)
+            state_free(a->seg[i]);
+        for (int i = 0; i < a->every; ++i)
+        {
+            if (a->segGrid)
+                dfree(a->segGrid[i]);
+            if (a->segOrder)
+                dfree(a->segOrder[i]);
+            if (a->segStart)
+                dfree(a->segStart[i]);
+            if (a->segEnd)
+                dfree(a->segEnd[i]);
+        }
+        delete[] a->ckpt;
+        delete[] a->seg;
+        delete[] a->segGrid;
+        delete[] a->segOrder;
+        delete[] a->segStart;
+        delete[] a->segEnd;
+        dfree(a->gridAdj);
+        dfree(a->xTmp);
+        dfree(a->FTmp);
+        delete a;
+        return CRMPM_HIP_OK;
+    }
+} // namespace
+
+extern "C" {
+
+int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipAdjoint **out)
+{
+    if (!s || !out || numSteps <= 0 || checkpointEvery <= 0)
+        return CRMPM_HIP_ERR_ARG;
+    CrHipBatch *b = s->b;
+    if (b->kind != CRMPM_HIP_MLS || !b->initialized)
+        return CRMPM_HIP_ERR_STATE;
+    CrHipAdjoint *a = new (std::nothrow) CrHipAdjoint();
+    if (!a)
+        return CRMPM_HIP_ERR_ALLOC;
+    a->solver = s;
+    a->steps = numSteps;
+    a->every = std::min(checkpointEvery, numSteps);
+    a->segments = (numSteps + a->every - 1) / a->every;
+    const size_t N = (size_t)std::max(b->capacity, 1);
+    const size_t nodes = (size_t)b->totalBlocks * 64;
+    a->ckpt = new (std::nothrow) StateBuf[a->segments + 1];
+    a->seg = new (std::nothrow) StateBuf[a->every + 1];
+    a->segGrid = new (std::nothrow) float4 *[a->every]();
+    a->segOrder = new (std::nothrow) unsigned *[a->every]();
+    a->segStart = new (std::nothrow) int *[a->every]();
+    a->segEnd = new (std::nothrow) int *[a->every]();
+    int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd) ? 0 : CRMPM_HIP_ERR_ALLOC;
+    for (int i = 0; !rc && i < a->segments + 1; ++i)
+        rc |= state_alloc(a->ckpt[i], N);
+    for (int i = 0; !rc && i < a->every + 1; ++i)
+        rc |= state_alloc(a->seg[i], N);
+    for (int i = 0; !rc && i < a->every; ++i)
+    {
+        rc |= dalloc(&a->segGrid[i], nodes);
+        rc |= dalloc(&a->segOrder[i], N);
+        rc |= dalloc(&a->segStart[i], (size_t)b->totalBlocks);
+        rc |= dalloc(&a->segEnd[i], (size_t)b->totalBlocks);
+    }
+    if (!rc)
+    {
+        rc |= dalloc(&a->gridAdj, nodes);
+        rc |= dalloc(&a->xTmp, N);
+        rc |= dalloc(&a->FTmp, 9 * N);
+    }
+    if (rc)
+    {
+        adjoint_destroy(a);
+        return CRMPM_HIP_ERR_ALLOC;
+    }
+    *out = a;
+    return CRMPM_HIP_OK;
+}
+
+int crHipAdjointDestroy(CrHipAdjoint *a)
+{
+    if (!a)
+        return CRMPM_HIP_ERR_ARG;
+    return adjoint_destroy(a);
+}
+
+int crHipAdjointForward(CrHipAdjoint *a)
+{
+    if (!a)
+        return CRMPM_HIP_ERR_ARG;
+    CrHipBatch *b = a->solver->b;
+    int rc = validate_for_step(b);
+    if (rc)
+        return rc;
+    if (b->scenes[0].numShapes > 0)
+        return CRMPM_HIP_ERR_STATE;
+    for (int t = 0; t < a->steps && !rc; ++t)
+    {
+        if (t % a->every == 0)
+            rc = state_copy(b, a->ckpt[t / a->every], true);
+        if (!rc)
+            rc = batch_step(b);
+    }
+    if (!rc)
+        rc = state_copy(b, a->ckpt[a->segments], true);
+    a->forwardDone = rc == CRMPM_HIP_OK;
+    return rc;
+}
+
+int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, double *dLdLambda, double *dLdMu,
+                         float *dLdx0, float *dLdv0)
+{
+    if (!a || !dLdx || !dLdLambda || !dLdMu || !dLdv0)
+        return CRMPM_HIP_ERR_ARG;
+    if (!a->forwardDone)
+        return CRMPM_HIP_ERR_STATE;
+    CrHipBatch *b = a->solver->b;
+    if (b->profEvents)
+        return CRMPM_HIP_ERR_STATE;
+    const size_t N = (size_t)std::max(b->capacity, 1);
+    hipStream_t st = b->stream;
+    // Adjoint of the state: x and v in the caller's output arrays (x in a
+    // scratch array when dL/dx0 is not wanted), F and C in scratch arrays.
+    // The final state's adjoint is (dL/dx, dL/dv, 0, 0).
+    float4 *xOwn = nullptr;
+    float *Fb = nullptr, *Cb = nullptr;
+    int rc = CRMPM_HIP_OK;
+    if (dalloc(&Fb, 9 * N) || dalloc(&Cb, 9 * N) || (!dLdx0 && dalloc(&xOwn, N)))
+        rc = CRMPM_HIP_ERR_ALLOC;
+    float4 *xb = dLdx0 ? reinterpret_cast<float4 *>(dLdx0) : xOwn;
+    float4 *vb = reinterpret_cast<float4 *>(dLdv0);
+    if (!rc && (hipMemcpyAsync(xb, dLdx, N * sizeof(float4), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                (dLdv ? hipMemcpyAsync(vb, dLdv, N * sizeof(float4), hipMemcpyDeviceToDevice, st)
+                      : hipMemsetAsync(vb, 0, N * sizeof(float4), st)) != hipSuccess ||
+                hipMemsetAsync(Fb, 0, 9 * N * sizeof(float), st) != hipSuccess ||
+                hipMemsetAsync(Cb, 0, 9 * N * sizeof(float), st) != hipSuccess ||
+                hipMemsetAsync(dLdLambda, 0, N * sizeof(double), st) != hipSuccess ||
+                hipMemsetAsync(dLdMu, 0, N * sizeof(double), st) != hipSuccess))
+        rc = CRMPM_HIP_ERR_LAUNCH;
+    const AdjRef ad{xb, vb, Fb, Cb, a->xTmp, a->FTmp, dLdLambda, dLdMu, b->capacity};
+    const int stride = b->capacity;
+    for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
+    {
+        const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
+        // recompute the segment from its checkpoint, keeping every substep's
+        // start state, updated grid and tiles
+        rc = state_copy(b, a->ckpt[sgm], false);
+        for (int t = t0; t < t1 && !rc; ++t)
+        {
+            const int j = t - t0;
+            rc = state_copy(b, a->seg[j], true);
+            if (!rc)
+                rc = batch_step(b);
+            if (!rc &&
+                (hipMemcpyAsync(a->segGrid[j], b->grid, (size_t)b->totalBlocks * 64 * sizeof(float4), hipMemcpyDeviceToDevice,
+                                st) != hipSuccess ||
+                 hipMemcpyAsync(a->segOrder[j], b->lastOrder, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                 hipMemcpyAsync(a->segStart[j], b->segStart, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
+                                st) != hipSuccess ||
+                 hipMemcpyAsync(a->segEnd[j], b->segEnd, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
+                                st) != hipSuccess))
+                rc = CRMPM_HIP_ERR_LAUNCH;
+        }
+        if (!rc)
+            rc = state_copy(b, a->seg[t1 - t0], true);
+        // reverse sweep over the segment
+        for (int t = t1 - 1; t >= t0 && !rc; --t)
+        {
+            const int j = t - t0;
+            BatchArgs args = batch_args(b, a->segOrder[j]);
+            args.segStart = a->segStart[j];
+            args.segEnd = a->segEnd[j];
+            const int wgs = grid1d(b->totalBlocks, kWaves);
+            k_adj_g2p<<<wgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
+                                            a->segGrid[j], ad);
+            k_adj_grid<<<wgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
+            k_adj_p2g<<<wgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
+            rc = check_launch();
+        }
+    }
+    // leave the solver in the state the forward pass ended in
+    if (!rc)
+        rc = state_copy(b, a->ckpt[a->segments], false);
+    (void)hipStreamSynchronize(st);
+    dfree(Fb);
+    dfree(Cb);
+    dfree(xOwn);
+    return rc;
+}
+
+} // extern "C"
+

@@ -103,6 +103,10 @@ EXPORTS = {
     "crHipBatchProfileBegin": [_vp, ctypes.c_int],
     "crHipBatchProfileMask": [_vp, ctypes.c_uint],
     "crHipBatchProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
+    "crHipAdjointCreate": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    "crHipAdjointForward": [_vp],
+    "crHipAdjointBackward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "crHipAdjointDestroy": [_vp],
 }
 
 PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g", "mls_g2p", "std_p2g", "std_g2p",
@@ -486,6 +490,120 @@ class HipBatch:
             self.close()
         except Exception:
             pass
+
+
+class DifferentiableMls:
+    """Differentiable rollout of one MLS scene without rigid shapes: `steps`
+    substeps from the scene's initial positions, with per-particle Lame
+    parameters and initial velocities as inputs and final positions as the
+    output.  Gradients come from the hand-written adjoint kernels
+    (crHipAdjoint*), checkpointing every `checkpoint_every` substeps.
+    V0 comes from computeInitialData on the initial positions once; every
+    rollout restarts from those positions with F = I and C = 0."""
+
+    def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0"):
+        import torch
+
+        if scene["kind"] != S.MLS or scene["shape_list"]:
+            raise ValueError("the adjoint covers MLS scenes without rigid shapes")
+        self.torch = torch
+        self.solver = HipSolver(scene, device=device)
+        self.solver.compute_initial_data()
+        self.steps = steps
+        self.n = self.solver.n
+        h = _vp()
+        check(self.solver.lib.crHipAdjointCreate(self.solver.h, steps, checkpoint_every, ctypes.byref(h)), "adjoint create")
+        self.h = h
+        self.x0 = self.solver.xm.clone()
+        F, C, V0, stride = self.solver._state_ptrs
+        self.stride = stride
+        dev = self.solver.device
+        eye = torch.zeros((9, stride), dtype=torch.float32, device=dev)
+        eye[0] = eye[4] = eye[8] = 1.0
+        self._eye = eye
+        self._zero = torch.zeros((9, stride), dtype=torch.float32, device=dev)
+
+    def _reset(self, lam, mu, v0):
+        s = self.solver
+        torch = self.torch
+        s.xm.copy_(self.x0)
+        s.v[:, :3] = v0.detach().to(torch.float32)
+        s.v[:, 3] = 0.0
+        s.params[:, 0] = lam.detach().to(torch.float32)
+        s.params[:, 1] = mu.detach().to(torch.float32)
+        F, C, _, _ = s._state_ptrs
+        sh = s.stream.cuda_stream
+        hip_memcpy_d2d(F, self._eye.data_ptr(), self._eye.numel() * 4, sh)
+        hip_memcpy_d2d(C, self._zero.data_ptr(), self._zero.numel() * 4, sh)
+
+    def forward(self, lam, mu, v0):
+        """Final positions (N, 3), float32; no graph (use rollout() for autograd)."""
+        self._reset(lam, mu, v0)
+        check(self.solver.lib.crHipAdjointForward(self.h), "adjoint forward")
+        return self.solver.xm[:, :3].clone()
+
+    def backward(self, grad_x, grad_v=None):
+        """dL/dlambda, dL/dmu (float64) and dL/dv0 (float32) for dL/dx of the last forward."""
+        torch = self.torch
+        dev = self.solver.device
+        n = self.n
+        gx = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+        gx[:, :3] = grad_x.to(torch.float32)
+        gv = None
+        if grad_v is not None:
+            gv = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+            gv[:, :3] = grad_v.to(torch.float32)
+        dlam = torch.empty(n, dtype=torch.float64, device=dev)
+        dmu = torch.empty(n, dtype=torch.float64, device=dev)
+        dv0 = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        check(self.solver.lib.crHipAdjointBackward(self.h, gx.data_ptr(), gv.data_ptr() if gv is not None else None,
+                                                   dlam.data_ptr(), dmu.data_ptr(), None, dv0.data_ptr()),
+              "adjoint backward")
+        return dlam, dmu, dv0[:, :3]
+
+    def rollout(self, lam, mu, v0):
+        """Final positions as a differentiable function of (lambda, mu, v0)."""
+        return _MlsRollout.apply(lam, mu, v0, self)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.solver.lib.crHipAdjointDestroy(self.h)
+            self.h = None
+        self.solver.close()
+
+
+def _rollout_function():
+    import torch
+
+    class MlsRollout(torch.autograd.Function):
+        """torch.autograd.Function over the GPU forward and adjoint kernels."""
+
+        @staticmethod
+        def forward(ctx, lam, mu, v0, sim):
+            ctx.sim = sim
+            ctx.dtypes = (lam.dtype, mu.dtype, v0.dtype)
+            return sim.forward(lam, mu, v0)
+
+        @staticmethod
+        def backward(ctx, grad_x):
+            dlam, dmu, dv0 = ctx.sim.backward(grad_x)
+            tl, tm, tv = ctx.dtypes
+            return dlam.to(tl), dmu.to(tm), dv0.to(tv), None
+
+    return MlsRollout
+
+
+class _LazyRollout:
+    fn = None
+
+    @classmethod
+    def apply(cls, *args):
+        if cls.fn is None:
+            cls.fn = _rollout_function()
+        return cls.fn.apply(*args)
+
+
+_MlsRollout = _LazyRollout
 
 
 _HIPRT = None

@@ -318,3 +318,49 @@ def config3(n_scenes=64, kind=MLS, steps=100, seed=20261203 + 3, n_min=2000, n_m
         sc["steps"] = steps
         scenes.append(sc)
     return scenes
+
+
+def grf(rng, points, length=0.1, features=256):
+    """Smooth Gaussian random field at the points (random Fourier features,
+    Rahimi and Recht 2007): sqrt(2/K) sum_k cos(w_k . x + b_k), w_k ~ N(0, I/l^2),
+    b_k ~ U(0, 2 pi); normals by Box-Muller from the SplitMix64 stream."""
+    u1 = rng.uniform(3 * features)
+    u2 = rng.uniform(3 * features)
+    z = np.sqrt(-2.0 * np.log(np.maximum(u1, 1e-300))) * np.cos(2.0 * np.pi * u2)
+    w = z.reshape(features, 3) / length
+    b = rng.uniform(features, 0.0, 2.0 * np.pi)
+    return np.sqrt(2.0 / features) * np.cos(np.asarray(points, np.float64) @ w.T + b).sum(1)
+
+
+def config4(nx=64, ny=32, nz=32, steps=100, seed=20261203 + 4, field_seed=None):
+    """Differentiable MLS block: nx x ny x nz = 65,536 particles at spacing dx/2
+    on a 96^3-cell unit box (dx = 1/96), per-particle log E from a smooth
+    Gaussian random field (length scale 0.1 m) mapped to [ln 5e3, ln 5e4],
+    nu = 0.3, co-rotational (StVK as the reference evaluates it), v0 =
+    (0, -0.5, 0) + 0.05 * GRF per component, dt = 1e-4.  field_seed picks
+    the log E field (the hidden ground truth uses another seed); returns the
+    scene and the per-particle log E."""
+    rng = SplitMix64(seed)
+    dx = 1.0 / 96
+    h = dx / 2
+    N = nx * ny * nz
+    sc = new_scene(MLS, dx, (0, 0, 0), (1, 1, 1), N, 1e-4)
+    size = np.array([nx, ny, nz]) * h
+    origin = np.array([0.5, 0.5, 0.5]) - size / 2
+    pts = lattice_block(rng, (nx, ny, nz), origin, (h, h, h), 0.0)
+    sc["xm"][:, :3] = pts
+    sc["xm"][:, 3] = 1000.0 * h ** 3
+    frng = SplitMix64(seed + 1000 if field_seed is None else field_seed)
+    f = grf(frng, pts)
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-12)
+    log_e = math.log(5e3) + f * (math.log(5e4) - math.log(5e3))
+    vrng = SplitMix64(seed + 2000)
+    v0 = np.stack([grf(vrng, pts) for _ in range(3)], 1) * 0.05
+    v0[:, 1] += -0.5
+    sc["v"][:, :3] = v0
+    lam, mu = lame(np.exp(log_e), 0.3)
+    sc["params"][:, 0] = lam
+    sc["params"][:, 1] = mu
+    sc["type"][:] = COROTATIONAL
+    sc["steps"] = steps
+    return sc, log_e

@@ -234,6 +234,25 @@ int crHipBatchProfileBegin(CrHipBatch *b, int maxLaunches);
 int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask);
 int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories);
 
+/* ------------------------------------------------------------------------
+ * Differentiable MLS-MPM (one scene, no rigid shapes).  The reference has no
+ * adjoint; this is the differentiable form of its MLS substep
+ * (src/solver/mls_mpm_algorithm.h:147-390, same arithmetic and quirks).
+ * Forward runs numSteps substeps from the solver's current state and keeps
+ * the state every checkpointEvery substeps; Backward takes dL/dx (and
+ * optionally dL/dv) of the final state, recomputes each segment from its
+ * checkpoint and sweeps the adjoint kernels back to substep 0.  Outputs
+ * (device arrays, particle order): dL/dlambda and dL/dmu per particle
+ * (FP64, summed over the substeps), dL/dv and dL/dx of the initial state
+ * (float4, xyz used; dLdx0 may be null).  Backward leaves the solver in the
+ * state Forward ended in. */
+typedef struct CrHipAdjoint CrHipAdjoint;
+int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipAdjoint **out);
+int crHipAdjointForward(CrHipAdjoint *a);
+int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, double *dLdLambda, double *dLdMu,
+                         float *dLdx0, float *dLdv0);
+int crHipAdjointDestroy(CrHipAdjoint *a);
+
 #ifdef __cplusplus
 }
 #endif
