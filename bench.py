@@ -178,9 +178,9 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
     b = H.HipBatch(copy.deepcopy(scenes))
     b.compute_initial_data()
     b.step(warmup)
-    tile_waves = update_waves = 0
+    tuned = None
     if tune:
-        tile_waves, update_waves = b.autotune(steps_per_trial=2)
+        tuned = b.autotune(steps_per_trial=2)
     b.set_graph_mode(graph)
     b.step(1)  # capture
     torch.cuda.synchronize()
@@ -205,7 +205,8 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
             "particles": n_total, "substeps": steps, "iterations": scenes[0]["iterations"] if kind == S.PB else None,
             "value": n_total * steps / t_batch, "unit": "particle-substeps/s",
             "batch_ms_per_substep": t_batch / steps * 1e3, "sum_of_plain_ms_per_substep": t_plain / steps * 1e3,
-            "speedup_vs_sum_of_plain": t_plain / t_batch, "autotuned_waves": [tile_waves, update_waves],
+            "speedup_vs_sum_of_plain": t_plain / t_batch,
+            "autotuned": None if tuned is None else dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned)),
             "hip_graph": graph, "finite": bool(finite)}
 
 
@@ -219,7 +220,7 @@ def adjoint(steps=100, every=10):
 
     sc, log_e = S.config4(steps=steps)
     hidden, log_e_true = S.config4(steps=steps, field_seed=20261203 + 44)
-    sim = H.DifferentiableMls(sc, steps, checkpoint_every=every)
+    sim = H.DifferentiableMls(sc, steps, checkpoint_every=every, tune=True)
     dev = sim.solver.device
     nu = 0.3
     v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
@@ -252,6 +253,7 @@ def adjoint(steps=100, every=10):
             best = min(best, time.perf_counter() - t0)
         return best
 
+    sim_tuned = sim.tuned
     t_f = timed(fwd)
     t_fa = timed(fwd_adj)
     x = fwd().double()
@@ -262,6 +264,7 @@ def adjoint(steps=100, every=10):
     sim.close()
     plain = H.HipSolver(copy.deepcopy(sc))
     plain.compute_initial_data()
+    plain.autotune(steps_per_trial=2)
     plain.step(5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -275,7 +278,89 @@ def adjoint(steps=100, every=10):
             "checkpointed_forward_ms_per_substep": t_f / steps * 1e3,
             "forward_adjoint_ms_per_substep": t_fa / steps * 1e3,
             "forward_adjoint_vs_plain_forward": t_fa / t_p, "loss": loss,
+            "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), sim_tuned)),
             "grad_logE_abs_max": float(g.abs().max()), "finite": finite}
+
+
+def mls_run(scene, steps, warmup, tune=True):
+    """Run an MLS scene (applying its kinematic frames) and time `steps`
+    substeps; every 10th substep records HIP events on all its launches."""
+    import torch
+
+    import crmpm_hip as H
+
+    solver = H.HipSolver(copy.deepcopy(scene))
+    solver.compute_initial_data()
+    frames = solver.scene["frames"]
+    fsub = frames[0]["substeps"] if frames else 0
+
+    def run(first, count, sample=False):
+        for s in range(first, first + count):
+            if frames and (s % fsub == 0 or s == first):
+                S.apply_frame(solver.scene, frames[s // fsub])
+                solver._upload_shapes()
+            if sample:
+                solver.profile_mask(H.PROF_CATEGORIES if (s - first) % PROFILE_EVERY == 0 else [])
+            solver.step(1)
+
+    run(0, warmup)
+    tuned = solver.autotune(steps_per_trial=2) if tune else None
+    torch.cuda.synchronize()
+    grid = solver.grid()
+    touched = int(np.count_nonzero(grid[:, 3] > 0))
+    solver.profile_begin(max_launches=(steps // PROFILE_EVERY + 1) * 24 + 64)
+    solver.profile_mask([])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(warmup, steps, sample=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = solver.profile_end()
+    st = solver.state()
+    solver.close()
+    return elapsed, prof, touched, tuned, st
+
+
+def mls_kernel_roofline(n, touched, prof, slicer):
+    """Achieved HBM rate of the MLS P2G and G2P kernels on their algorithmic
+    bytes: P2G reads x, v, F, C, V0, (lambda, mu) = 108 B per particle and
+    writes 16 B per touched node; G2P reads x, F (and x again per slicer test,
+    12 B) and writes x, v, F, C = 96 B per particle and reads 16 B per node."""
+    out = {}
+    for name, per_p in (("mls_p2g", 108), ("mls_g2p", 48 + (12 if slicer else 0) + 96)):
+        ms, k = prof[name]
+        if not k:
+            continue
+        avg = ms / k / 1e3
+        achieved = (per_p * n + 16 * touched) / avg / 1e9
+        out[name] = {"avg_launch_us": avg * 1e6, "launches": k, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_launch": per_p * n + 16 * touched}
+    return out
+
+
+def mls_cutting(steps=500, warmup=10):
+    """MLS at scale with cutting (config5: 2,097,152 particles, 257 x 129 x 257 nodes)."""
+    sc = S.config5(steps=steps + warmup)
+    n = len(sc["xm"])
+    elapsed, prof, touched, tuned, st = mls_run(sc, steps, warmup)
+    return {"workload": "MLS block 256x32x256 = 2,097,152 particles, 257x129x257 nodes, capsule tool + growing quad "
+                        "slicer cutting at 0.02 m/s", "particles": n, "substeps": steps,
+            "value": n * steps / elapsed, "unit": "particle-substeps/s", "ms_per_substep": elapsed / steps * 1e3,
+            "touched_nodes": touched, "roofline": mls_kernel_roofline(n, touched, prof, slicer=True),
+            "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned)),
+            "finite": bool(np.isfinite(st["xm"]).all())}
+
+
+def mls_cube(steps=200, warmup=5):
+    """The reference's CPU-runnable case (config1: 32^3 = 32,768 particles, 64^3 cells, 200 substeps)."""
+    sc = S.config1(steps=steps + warmup)
+    n = len(sc["xm"])
+    elapsed, prof, touched, tuned, st = mls_run(sc, steps, warmup)
+    return {"workload": "MLS elastic cube 32^3 = 32,768 particles on 65^3 nodes, ground plane", "particles": n,
+            "substeps": steps, "value": n * steps / elapsed, "unit": "particle-substeps/s",
+            "ms_per_substep": elapsed / steps * 1e3, "roofline": mls_kernel_roofline(n, touched, prof, slicer=False),
+            "finite": bool(np.isfinite(st["xm"]).all())}
 
 
 def main():
@@ -284,8 +369,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint"], default="all",
-                    help="all: the PB slab headline line plus the batched and adjoint comparisons")
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls"], default="all",
+                    help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
@@ -324,6 +409,9 @@ def main():
 
     import crmpm_hip as H
 
+    if args.workload == "mls":
+        print(json.dumps({"mls_cutting": mls_cutting(), "mls_cube": mls_cube()}), flush=True)
+        return
     if args.workload == "adjoint":
         print(json.dumps({"adjoint": adjoint()}), flush=True)
         return
@@ -349,6 +437,8 @@ def main():
             solver.step(1)
 
     run(0, W)
+    # the library's autotuner picks the tile split and launch sizes on this state (which it leaves unchanged)
+    tuned = solver.autotune(steps_per_trial=2)
     torch.cuda.synchronize()
     # touched nodes of one substep (grid nodes holding mass), for the grid term of the roofline
     grid = solver.grid()
@@ -395,7 +485,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pb_mpm_neohookean_slab_sphere_indenter", "particles": n,
                    "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
-                   "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu"},
+                   "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu",
+                   "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned))},
         "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1> (fused G2P+P2G)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
@@ -419,10 +510,11 @@ def main():
                 out[key] = batched(kind, steps, warm)
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
-        try:
-            out["adjoint"] = adjoint()
-        except Exception as e:
-            out["adjoint"] = {"error": repr(e)}
+        for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube)):
+            try:
+                out[key] = fn()
+            except Exception as e:
+                out[key] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -828,6 +828,28 @@ namespace crhip
         return false;
     }
 
+    // True when no particle-node pair of the particle at x can straddle a
+    // slicer, so different_sides is false for all 27 nodes: nodes lie within
+    // 1.5 sqrt(3) dx of x, and a quad slicer only cuts a pair whose particle
+    // is inside its extent with the node on the other side of its plane.
+    // Triangle-mesh slicers are never ruled out here.
+    CR_DEV bool slicer_free(const ShapeView &sv, int nSlicers, V3 x, float dx)
+    {
+        const float reach = 2.5980762f * dx; // 1.5 * sqrt(3) * dx
+        for (int k = 0; k < nSlicers; ++k)
+        {
+            const ShapeRec &r = sv.recs[sv.slicers[k]];
+            if (r.geomType != gQuadSlicer)
+                return false;
+            const V3 is = v3(r.invScale);
+            const V3 l = qrot_inv(r.rot, x - v3(r.pos)) * is;
+            const bool inside = !(l.x < -r.gparams.x || l.x > r.gparams.x || l.z < -r.gparams.y || l.z > r.gparams.y);
+            if (inside && fabsf(l.y) <= reach * fabsf(is.y) * 1.0001f)
+                return false;
+        }
+        return true;
+    }
+
     CR_DEV void add_momentum(float4 *force, float4 *torque, int id, V3 mom, V3 arm)
     {
         const V3 tq = cross(arm, mom);

@@ -86,6 +86,7 @@ namespace
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
+        int subSize, subMax, subStride; // sub-tile split (see BatchArgs)
     };
 
     // Per-scene record in device memory.  A batch is a list of scenes whose
@@ -128,11 +129,26 @@ namespace
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
+        // A pblock's particles are split into sub-tiles of subSize particles
+        // (a multiple of 64; the last of at most subMax sub-tiles takes the
+        // rest), each run by its own wavefront with its own partial box at
+        // slot pblock * subStride + j.
+        int subSize, subMax, subStride;
     };
+
+    // Scene records are never written by a kernel: reading them through the
+    // constant address space makes the loads invariant, so the compiler may
+    // reload a field instead of holding it in registers for the whole kernel.
+    // (The attribute only exists in the device compilation pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstSceneRec = const __attribute__((address_space(4))) SceneRec;
+#else
+    using ConstSceneRec = const SceneRec;
+#endif
 
     CR_DEV StepCommon scene_view_of(const BatchArgs &b, int s)
     {
-        const SceneRec &r = b.scenes[s];
+        ConstSceneRec &r = ((ConstSceneRec *)(size_t)b.scenes)[s];
         StepCommon c;
         c.g = r.g;
         c.gravity = r.gravity;
@@ -143,9 +159,12 @@ namespace
         c.segStart = b.segStart + r.blockOffset;
         c.segEnd = b.segEnd + r.blockOffset;
         c.order = b.order;
-        c.tilebuf = b.tilebuf + (size_t)r.blockOffset * kTileNodes;
-        c.tileForce = b.tileForce ? b.tileForce + (size_t)r.blockOffset * kTileNodes : nullptr;
+        c.tilebuf = b.tilebuf + (size_t)r.blockOffset * b.subStride * kTileNodes;
+        c.tileForce = b.tileForce ? b.tileForce + (size_t)r.blockOffset * b.subStride * kTileNodes : nullptr;
         c.grid = b.grid + (size_t)r.blockOffset * 64;
+        c.subSize = b.subSize;
+        c.subMax = b.subMax;
+        c.subStride = b.subStride;
         return c;
     }
 
@@ -153,7 +172,7 @@ namespace
     CR_DEV StepCommon scene_view(const BatchArgs &b, int tg, int &t)
     {
         const int s = __builtin_amdgcn_readfirstlane(b.blockScene[tg]);
-        t = tg - b.scenes[s].blockOffset;
+        t = tg - ((ConstSceneRec *)(size_t)b.scenes)[s].blockOffset;
         return scene_view_of(b, s);
     }
 
@@ -162,6 +181,16 @@ namespace
     // size covers the batch and the autotuner picks the size.
     CR_DEV int first_tile(int wave) { return __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWaves + wave); }
     CR_DEV int tile_stride() { return (int)gridDim.x * kWaves; }
+    CR_DEV int num_subtiles(const BatchArgs &b) { return b.totalBlocks * b.subMax; }
+    // Whether sub-tile tg (global pblock tg / subMax, index tg % subMax) has
+    // particles, from the global segment arrays alone (an empty sub-tile
+    // skips the scene record loads).
+    CR_DEV bool subtile_used(const BatchArgs &b, int tg)
+    {
+        const int pb = tg / b.subMax, j = tg - pb * b.subMax;
+        const int count = b.segEnd[pb] - b.segStart[pb];
+        return count > 0 && (j == 0 || count > j * b.subSize);
+    }
     // Next pblock of a tile loop; a non-persistent launch (one wavefront per
     // pblock) compiles to straight-line code, which needs fewer registers.
     template <bool Persistent>
@@ -538,6 +567,7 @@ namespace
     struct TileInfo
     {
         int bx, by, bz, start, end;
+        int slot; // partial box of this (sub-)tile, scene-local
     };
     // Tile t is pblock t; false when it holds no particle.
     CR_DEV bool tile_info(const GridGeom &g, int t, const int *segStart, const int *segEnd, TileInfo &ti)
@@ -549,6 +579,28 @@ namespace
         if (ti.end <= ti.start)
             return false;
         decode_block(g, t, ti.bx, ti.by, ti.bz);
+        ti.slot = t;
+        return true;
+    }
+
+    // Number of sub-tiles of a pblock holding `count` particles.
+    CR_DEV int subtiles_of(int count, int subSize, int subMax)
+    {
+        return count <= 0 ? 0 : min((count + subSize - 1) / subSize, subMax);
+    }
+
+    // Sub-tile j of scene-local pblock t: false when the pblock is empty or
+    // has fewer sub-tiles.
+    CR_DEV bool subtile_info(const StepCommon &c, int t, int j, TileInfo &ti)
+    {
+        const int start = c.segStart[t], end = c.segEnd[t];
+        const int ns = subtiles_of(end - start, c.subSize, c.subMax);
+        if (j >= ns)
+            return false;
+        ti.start = start + j * c.subSize;
+        ti.end = (j == ns - 1) ? end : ti.start + c.subSize;
+        decode_block(c.g, t, ti.bx, ti.by, ti.bz);
+        ti.slot = t * c.subStride + j;
         return true;
     }
 
@@ -741,7 +793,10 @@ namespace
     __global__ __launch_bounds__(kWG) void k_cell_rank_order(BatchArgs b, const unsigned *order, unsigned *tmp,
                                                              unsigned *orderOut)
     {
+        constexpr int kLevels = 32;
         __shared__ int cnt[kWaves][64];
+        __shared__ unsigned long long levelMask[kWaves][kLevels];
+        __shared__ int levelStart[kWaves][kLevels];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const float4 *xm = b.P.xm;
         for (int tg = first_tile(wave); tg < b.totalBlocks; tg += tile_stride())
@@ -783,19 +838,47 @@ namespace
                     tmp[k] = ((unsigned)r << 6) | cell;
             }
             const int mine = C[lane];
+            // Position of (rank r, cell) = particles of lower rank + rank-r
+            // particles of lower cells.  Per rank level up to kLevels: the mask
+            // of cells holding a rank-r particle and the level's start; deeper
+            // levels (very dense cells) take the per-cell sum.
+            int top = mine;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+                top = max(top, __shfl_xor(top, o, 64));
+            const int levels = min(top, kLevels);
+            unsigned long long *LM = levelMask[wave];
+            int *LS = levelStart[wave];
+            int start = 0;
+            for (int r = 0; r < levels; ++r)
+            {
+                const unsigned long long m = __ballot(mine > r);
+                if (lane == 0)
+                {
+                    LM[r] = m;
+                    LS[r] = start;
+                }
+                start += __popcll(m);
+            }
+            wave_sync();
             for (int base = ti.start; base < ti.end; base += 64)
             {
                 const int k = base + lane;
                 const bool valid = k < ti.end;
                 const unsigned rc = valid ? tmp[k] : 0u;
                 const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
-                // particles of lower rank, then rank-r particles of lower cells
-                int pos = 0;
-#pragma unroll 16
-                for (int c = 0; c < 64; ++c)
+                const unsigned long long below = (cell == 0) ? 0ull : ((~0ull) >> (64 - cell));
+                int pos;
+                if (r < levels)
+                    pos = LS[r] + __popcll(LM[r] & below);
+                else
                 {
-                    const int nc = __builtin_amdgcn_readlane(mine, c);
-                    pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                    pos = 0;
+                    for (int c = 0; c < 64; ++c)
+                    {
+                        const int nc = __builtin_amdgcn_readlane(mine, c);
+                        pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                    }
                 }
                 if (valid)
                     orderOut[ti.start + pos] = order[k];
@@ -875,7 +958,7 @@ namespace
     struct ScatterTerms
     {
         V3 va[3], py[3], pz[3];
-        float wxm[3], wy[3], wz[3]; // x weights times the particle mass
+        float wxm[3], wy[3], wz[3]; // x weights times m (the caller passes m = 1 to keep them plain)
     };
     CR_DEV ScatterTerms scatter_terms(V3 v, const M3 &Cm, float m, const Weights &W, const AxisOffsets &o)
     {
@@ -905,8 +988,8 @@ namespace
             opaque(t.wz[i]);
         }
     }
-    // Scatter (term * w m, w m) to the 27 nodes.
-    CR_DEV void scatter_separable(float4 *A, int l0, const ScatterTerms &t)
+    // Scatter (term * w, w * mfac) to the 27 nodes, w = wxm[a] wy[b] wz[c].
+    CR_DEV void scatter_separable(float4 *A, int l0, const ScatterTerms &t, float mfac)
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -920,7 +1003,7 @@ namespace
                 {
                     const float wm = t.wxm[a] * wbc;
                     const V3 contrib = (t.va[a] + q) * wm;
-                    lds_add(A, node_li(l0, a, b, c), contrib.x, contrib.y, contrib.z, wm);
+                    lds_add(A, node_li(l0, a, b, c), contrib.x, contrib.y, contrib.z, wm * mfac);
                 }
             }
     }
@@ -934,12 +1017,14 @@ namespace
     {
         __shared__ float4 acc[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *A = acc[wave];
@@ -969,8 +1054,18 @@ namespace
                 const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
                 const M3 affine = stress * sc + Cm * m;
                 const V3 vm = vel * m;
-                const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
+                const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
+                // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
+                const ScatterTerms terms = scatter_terms(vm, affine, 1.0f, W, axis_offsets(fx, g.dx));
+                const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
                 in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                    if (fast)
+                    {
+                        ScatterTerms tt = terms;
+                        opaque(tt);
+                        scatter_separable(A, l0, tt, m);
+                        return;
+                    }
                     M3 af = affine;
                     V3 v_m = vm, f = fx;
                     Weights w = W;
@@ -982,19 +1077,16 @@ namespace
                         const V3 contrib = (v_m + mul(af, dist)) * wt;
                         lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
                     };
-                    if (fast)
-                        for_each_node<true>(g, ti, bx, by, bz, f, w, sc);
-                    else
-                        for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
-                            V3 nrm;
-                            if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                                return;
-                            sc(li, a, b, cc, dist, wt);
-                        });
+                    for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                        V3 nrm;
+                        if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                            return;
+                        sc(li, a, b, cc, dist, wt);
+                    });
                 });
             }
             wave_sync();
-            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
             for (int i = lane; i < kTileNodes; i += 64)
                 out[i] = A[i];
         }
@@ -1005,7 +1097,9 @@ namespace
     // mlsMpmUpdateGrid / pbMpmUpdateGrid / standardMpmUpdateGrid
     // (src/solver/mls_mpm_algorithm.h:243-294, pb 233-284, standard 217-272).
     // ------------------------------------------------------------------
-    template <int Kind> // 0 MLS, 1 PB, 2 standard
+    // Multi: pblocks may have more than one sub-tile (subMax > 1); the
+    // single-sub-tile variant needs fewer registers.
+    template <int Kind, bool Multi> // Kind: 0 MLS, 1 PB, 2 standard
     __global__ __launch_bounds__(kWG) void k_update(BatchArgs b, unsigned char *dirty)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1018,32 +1112,38 @@ namespace
             decode_block(g, nb, bx, by, bz);
             // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz).
             // All occupancy loads are issued before any partial is read.
-            bool src[8];
+            int nsub[8];
             bool any = false;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
             {
                 const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
                 const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
-                src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
-                any = any || src[d];
+                nsub[d] = key >= 0 ? subtiles_of(c.segEnd[key] - c.segStart[key], c.subSize, c.subMax) : 0;
+                any = any || nsub[d] > 0;
             }
             if (!any)
                 continue;
             const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+            // First sub-tile of every source pblock: all loads issued before any
+            // is summed.  Box pointers and sub-tile counts are wave-uniform; only
+            // the node offset inside a box is per lane.
             float4 q[8], qf[8];
+            bool more = false;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
             {
                 const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
                 const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
-                const bool use = src[d] && ux < kTN && uy < kTN && uz < kTN;
-                const size_t at = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
-                q[d] = use ? c.tilebuf[at] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const bool use = nsub[d] > 0 && ux < kTN && uy < kTN && uz < kTN;
+                const size_t box = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * c.subStride * kTileNodes;
+                const int li = ux + kTN * (uy + kTN * uz);
+                more = more || nsub[d] > 1;
+                q[d] = use ? (c.tilebuf + box)[li] : make_float4(0.f, 0.f, 0.f, 0.f);
                 if (Kind == 2)
-                    qf[d] = use ? c.tileForce[at] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    qf[d] = use ? (c.tileForce + box)[li] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            // fixed summation order d = 0..7 (bitwise reproducible)
+            // fixed summation order: d = 0..7, each followed by its further sub-tiles (bitwise reproducible)
             float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
             float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -1059,6 +1159,26 @@ namespace
                     f.y += qf[d].y;
                     f.z += qf[d].z;
                 }
+                const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+                const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
+                if (Multi && more && ux < kTN && uy < kTN && uz < kTN)
+                    for (int j = 1; j < nsub[d]; ++j)
+                    {
+                        const size_t at = ((size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * c.subStride + j) * kTileNodes +
+                                          (ux + kTN * (uy + kTN * uz));
+                        const float4 r = c.tilebuf[at];
+                        s.x += r.x;
+                        s.y += r.y;
+                        s.z += r.z;
+                        s.w += r.w;
+                        if (Kind == 2)
+                        {
+                            const float4 rf = c.tileForce[at];
+                            f.x += rf.x;
+                            f.y += rf.y;
+                            f.z += rf.z;
+                        }
+                    }
             }
             if (lane == 0)
                 dirty[nbg] = 1;
@@ -1109,7 +1229,7 @@ namespace
             vacc = vacc + nw;
             aacc = aacc + outer_ref(nw, dist);
         };
-        if (nSlicers == 0 && nodes_inside(g, bx, by, bz))
+        if (nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx)))
         {
             gather_separable(G, base_li(ti, bx, by, bz), W, axis_offsets(fx, g.dx), vacc, aacc);
         }
@@ -1137,12 +1257,14 @@ namespace
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *G = lg[wave];
@@ -1219,7 +1341,7 @@ namespace
         cell_of(g, x, bx, by, bz, fx);
         Weights W;
         weights_of(fx, W);
-        const bool fast = nSlicers == 0 && nodes_inside(g, bx, by, bz);
+        const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
         const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
         in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
@@ -1227,7 +1349,7 @@ namespace
             {
                 ScatterTerms tt = terms;
                 opaque(tt);
-                scatter_separable(A, l0, tt);
+                scatter_separable(A, l0, tt, 1.0f);
                 return;
             }
             M3 cm = Cm;
@@ -1260,12 +1382,14 @@ namespace
         __shared__ float4 acc[(Mode < 2) ? kWaves : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *A = acc[(Mode < 2) ? wave : 0];
@@ -1324,7 +1448,7 @@ namespace
             if (Mode < 2)
             {
                 wave_sync();
-                float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+                float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
                 for (int i = lane; i < kTileNodes; i += 64)
                     out[i] = A[i];
             }
@@ -1361,12 +1485,14 @@ namespace
     {
         __shared__ float4 acc[kWaves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *A = acc[wave][0];
@@ -1420,8 +1546,8 @@ namespace
                 });
             }
             wave_sync();
-            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
-            float4 *outF = c.tileForce + (size_t)t * kTileNodes;
+            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
+            float4 *outF = c.tileForce + (size_t)ti.slot * kTileNodes;
             for (int i = lane; i < kTileNodes; i += 64)
             {
                 out[i] = A[i];
@@ -1435,12 +1561,14 @@ namespace
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<Persistent>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *G = lg[wave];
@@ -1680,12 +1808,14 @@ namespace
         __shared__ float4 acc[kWaves][kTileNodes];
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<false>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             const GridGeom &g = c.g;
             float4 *A = acc[wave];
@@ -1735,7 +1865,7 @@ namespace
                 ad.xTmp[p] = make_float4(xacc.x, xacc.y, xacc.z, 0.0f);
             }
             wave_sync();
-            float4 *out = c.tilebuf + (size_t)t * kTileNodes;
+            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
             for (int i = lane; i < kTileNodes; i += 64)
                 out[i] = A[i];
         }
@@ -1754,15 +1884,15 @@ namespace
             const GridGeom &g = c.g;
             int bx, by, bz;
             decode_block(g, nb, bx, by, bz);
-            bool src[8];
+            int nsub[8];
             bool any = false;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
             {
                 const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
                 const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
-                src[d] = key >= 0 && c.segEnd[key] > c.segStart[key];
-                any = any || src[d];
+                nsub[d] = key >= 0 ? subtiles_of(c.segEnd[key] - c.segStart[key], c.subSize, c.subMax) : 0;
+                any = any || nsub[d] > 0;
             }
             if (!any)
                 continue;
@@ -1773,12 +1903,12 @@ namespace
             {
                 const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
                 const int ux = lx + 4 * dx, uy = ly + 4 * dy, uz = lz + 4 * dz;
-                if (src[d] && ux < kTN && uy < kTN && uz < kTN)
-                {
-                    const size_t at =
-                        (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * kTileNodes + (ux + kTN * (uy + kTN * uz));
-                    ub = ub + v3(c.tilebuf[at]);
-                }
+                if (!(ux < kTN && uy < kTN && uz < kTN))
+                    continue;
+                const size_t box = (size_t)(nb - dx - g.nbx * (dy + g.nby * dz)) * c.subStride;
+                const int li = ux + kTN * (uy + kTN * uz);
+                for (int j = 0; j < nsub[d]; ++j)
+                    ub = ub + v3(c.tilebuf[(box + j) * kTileNodes + li]);
             }
             const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
             if (!in_grid(g, i, j, k))
@@ -1884,12 +2014,14 @@ namespace
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg = next_tile<false>(tg, b.totalBlocks))
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
+            if (!subtile_used(b, tg))
+                continue;
             int t;
-            const StepCommon c = scene_view(b, tg, t);
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
             TileInfo ti;
-            if (!tile_info(c.g, t, c.segStart, c.segEnd, ti))
+            if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             const GridGeom &g = c.g;
             float4 *G = lg[wave];
@@ -2063,8 +2195,11 @@ struct CrHipBatch
     int stageSlot = 0;
     int totalActive = 0;
 
-    // launch sizes in wavefronts; 0 = one wavefront per pblock
+    // launch sizes in wavefronts; 0 = one wavefront per (sub-)tile
     int tileWaves = 0, updateWaves = 0;
+    // sub-tile split: particles per sub-tile, sub-tiles per pblock (<= subStride,
+    // the allocated partial boxes per pblock)
+    int subSize = 64, subMax = 1, subStride = 1;
     // tile-ordered particle permutation of the last enqueued substep
     const unsigned *lastOrder = nullptr;
 
@@ -2072,7 +2207,7 @@ struct CrHipBatch
     bool graphMode = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    long long graphKey[5] = {-1, -1, -1, -1, -1};
+    long long graphKey[7] = {-1, -1, -1, -1, -1, -1, -1};
     long long bindEpoch = 0;
 
     // optional per-launch event timing (ProfileBegin/End)
@@ -2239,13 +2374,17 @@ namespace
         a.tilebuf = b->tilebuf;
         a.tileForce = b->tileForce;
         a.grid = b->grid;
+        a.subSize = b->subSize;
+        a.subMax = b->subMax;
+        a.subStride = b->subStride;
         return a;
     }
 
-    // Workgroups of a persistent pblock launch (waves = 0: one wavefront per pblock).
-    int pblock_wgs(const CrHipBatch *b, int waves)
+    // Workgroups of a launch over `items` (pblocks or sub-tiles); waves = 0:
+    // one wavefront per item, otherwise a persistent launch of that many.
+    int pblock_wgs(long long items, int waves)
     {
-        const long long all = grid1d(b->totalBlocks, kWaves);
+        const long long all = grid1d(items, kWaves);
         return (int)std::max(1ll, waves > 0 ? std::min<long long>(all, grid1d(waves, kWaves)) : all);
     }
 
@@ -2291,8 +2430,8 @@ namespace
         // the sort's spare ping-pong buffers take the cell ranks and the tile-ordered permutation
         unsigned *tmp = (*keysOut == b->keysA) ? b->keysB : b->keysA;
         unsigned *order2 = (*orderOut == b->valsA) ? b->valsB : b->valsA;
-        launch(b, CRMPM_HIP_PROF_BINNING, k_cell_rank_order, pblock_wgs(b, b->tileWaves), kWG, batch_args(b, nullptr),
-               *orderOut, tmp, order2);
+        launch(b, CRMPM_HIP_PROF_BINNING, k_cell_rank_order, pblock_wgs(b->totalBlocks, b->tileWaves), kWG,
+               batch_args(b, nullptr), *orderOut, tmp, order2);
         *orderOut = order2;
         return check_launch();
     }
@@ -2326,19 +2465,21 @@ namespace
             return rc;
         b->lastOrder = order;
         const BatchArgs a = batch_args(b, order);
-        const int tileWGs = pblock_wgs(b, b->tileWaves);
-        const int updWGs = pblock_wgs(b, b->updateWaves);
+        const int tileWGs = pblock_wgs(b->totalBlocks * b->subMax, b->tileWaves);
+        const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
+        const bool multi = b->subMax > 1;
+        auto upd1 = multi ? k_update<1, true> : k_update<1, false>;
         if (b->kind == CRMPM_HIP_MLS)
         {
             launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
-            launch(b, CRMPM_HIP_PROF_UPDATE, k_update<0>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_UPDATE, multi ? k_update<0, true> : k_update<0, false>, updWGs, kWG, a, b->dirty);
             launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, tileWGs, kWG, a);
         }
         else if (b->kind == CRMPM_HIP_STANDARD)
         {
             launch(b, CRMPM_HIP_PROF_STD_P2G, pers ? k_std_p2g<true> : k_std_p2g<false>, tileWGs, kWG, a);
-            launch(b, CRMPM_HIP_PROF_UPDATE, k_update<2>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_UPDATE, multi ? k_update<2, true> : k_update<2, false>, updWGs, kWG, a, b->dirty);
             launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, tileWGs, kWG, a);
         }
         else
@@ -2353,12 +2494,12 @@ namespace
             else
             {
                 launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_tile<0, true> : k_pb_tile<0, false>, tileWGs, kWG, a, b->st);
-                launch(b, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, a, b->dirty);
+                launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, pers ? k_pb_tile<1, true> : k_pb_tile<1, false>, tileWGs, kWG, a,
                            b->st);
-                    launch(b, CRMPM_HIP_PROF_UPDATE, k_update<1>, updWGs, kWG, a, b->dirty);
+                    launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 }
                 launch(b, CRMPM_HIP_PROF_PB_G2P, pers ? k_pb_tile<2, true> : k_pb_tile<2, false>, tileWGs, kWG, a, b->st);
             }
@@ -2392,8 +2533,9 @@ namespace
         const bool useGraph = b->graphMode && !b->profEvents && b->stream != nullptr;
         if (!useGraph)
             return enqueue_step(b);
-        const long long key[5] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch};
-        if (!b->graphExec || !std::equal(key, key + 5, b->graphKey))
+        const long long key[7] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
+                                  b->subSize, b->subMax};
+        if (!b->graphExec || !std::equal(key, key + 7, b->graphKey))
         {
             drop_graph(b);
             if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -2413,7 +2555,7 @@ namespace
                 drop_graph(b);
                 return CRMPM_HIP_ERR_LAUNCH;
             }
-            std::copy(key, key + 5, b->graphKey);
+            std::copy(key, key + 7, b->graphKey);
         }
         return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
@@ -2495,10 +2637,7 @@ namespace
         rc |= dalloc(&b->V0, N);
         rc |= dalloc(&b->grid, nodes);
         if (b->kind == CRMPM_HIP_STANDARD)
-        {
             rc |= dalloc(&b->gridForce, nodes);
-            rc |= dalloc(&b->tileForce, blocks * kTileNodes);
-        }
         rc |= dalloc(&b->dirty, blocks);
         rc |= dalloc(&b->segStart, blocks);
         rc |= dalloc(&b->segEnd, blocks);
@@ -2508,8 +2647,14 @@ namespace
         rc |= dalloc(&b->valsB, N);
         rc |= dalloc(&b->radixCounts, (size_t)radixTiles * 256);
         rc |= dalloc(&b->digitTotals, 256);
-        // tile k is pblock k: one 6^3 partial box per pblock
-        rc |= dalloc(&b->tilebuf, blocks * kTileNodes);
+        // partial boxes: subStride per pblock (up to 8, fewer for very large
+        // batches so the boxes stay within 8 GiB)
+        const size_t box = (size_t)kTileNodes * sizeof(float4);
+        b->subStride = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t(8) << 30) / std::max<size_t>(blocks * box, 1)));
+        b->subMax = std::min(b->subMax, b->subStride);
+        rc |= dalloc(&b->tilebuf, blocks * b->subStride * kTileNodes);
+        if (b->kind == CRMPM_HIP_STANDARD)
+            rc |= dalloc(&b->tileForce, blocks * b->subStride * kTileNodes);
         rc |= dalloc(&b->shapeCounts, 4 * (size_t)ns);
         rc |= dalloc(&b->dScenes, ns);
         rc |= dalloc(&b->dShapes, ns);
@@ -2594,7 +2739,8 @@ namespace
         // the scene's tile partial region holds its fixed-point mass counters
         // (216 float4 per pblock >= 64 counters per node block)
         const HostScene &h = b->scenes[s];
-        unsigned long long *acc = reinterpret_cast<unsigned long long *>(b->tilebuf + (size_t)h.blockOffset * kTileNodes);
+        unsigned long long *acc =
+            reinterpret_cast<unsigned long long *>(b->tilebuf + (size_t)h.blockOffset * b->subStride * kTileNodes);
         if (hipMemsetAsync(acc, 0, (size_t)h.g.numBlocks * 64 * sizeof(unsigned long long), b->stream) != hipSuccess)
             return CRMPM_HIP_ERR_LAUNCH;
         k_init_mass<<<grid1d(n, 256), 256, 0, b->stream>>>(a, s, n, indices, acc);
@@ -2735,11 +2881,15 @@ namespace
         return CRMPM_HIP_OK;
     }
 
-    // Time `steps` substeps from the saved state with the given launch sizes.
-    int tune_trial(CrHipBatch *b, char *backup, int steps, int tileWaves, int updateWaves, float *ms)
+    // Time `steps` substeps from the saved state with the given launch sizes
+    // and sub-tile split.
+    int tune_trial(CrHipBatch *b, char *backup, int steps, int tileWaves, int updateWaves, int subSize, int subMax,
+                   float *ms)
     {
         b->tileWaves = tileWaves;
         b->updateWaves = updateWaves;
+        b->subSize = subSize;
+        b->subMax = subMax;
         int rc = tune_copy(b, backup, false);
         if (rc)
             return rc;
@@ -2764,7 +2914,8 @@ namespace
         return rc;
     }
 
-    int batch_autotune(CrHipBatch *b, int steps, int *tileWavesOut, int *updateWavesOut)
+    int batch_autotune(CrHipBatch *b, int steps, int *tileWavesOut, int *updateWavesOut, int *subSizeOut,
+                       int *subMaxOut)
     {
         if (!b || steps <= 0)
             return CRMPM_HIP_ERR_ARG;
@@ -2785,17 +2936,35 @@ namespace
         if (dalloc(&backup, bytes))
             return CRMPM_HIP_ERR_ALLOC;
         rc = tune_copy(b, backup, true);
-        // candidates: one wavefront per pblock, or a persistent launch of
-        // 2, 4, 8 or 16 thousand wavefronts looping over the pblocks
+        // 1. sub-tile split: 64, 128 or 256 particles per sub-tile, or whole
+        //    pblocks (one sub-tile each)
+        const int splits[4][2] = {{64, b->subStride}, {128, b->subStride}, {256, b->subStride}, {1 << 20, 1}};
+        int bestS = b->subSize, bestM = b->subMax;
+        float best = 3.4e38f;
+        for (const auto &sp : splits)
+        {
+            float ms = 0.0f;
+            if (rc)
+                break;
+            rc = tune_trial(b, backup, steps, 0, 0, sp[0], sp[1], &ms);
+            if (!rc && ms < best)
+            {
+                best = ms;
+                bestS = sp[0];
+                bestM = sp[1];
+            }
+        }
+        // 2. launch sizes: one wavefront per item, or a persistent launch of
+        //    2, 4, 8 or 16 thousand wavefronts looping over the items
         const int cand[] = {0, 2048, 4096, 8192, 16384};
         int bestT = 0, bestU = 0;
-        float best = 3.4e38f;
+        best = 3.4e38f;
         for (int c : cand)
         {
             float ms = 0.0f;
-            if (rc || (c > 0 && c >= b->totalBlocks))
+            if (rc || (c > 0 && c >= b->totalBlocks * bestM))
                 continue;
-            rc = tune_trial(b, backup, steps, c, 0, &ms);
+            rc = tune_trial(b, backup, steps, c, 0, bestS, bestM, &ms);
             if (!rc && ms < best)
             {
                 best = ms;
@@ -2808,7 +2977,7 @@ namespace
             float ms = 0.0f;
             if (rc || (c > 0 && c >= b->totalBlocks))
                 continue;
-            rc = tune_trial(b, backup, steps, bestT, c, &ms);
+            rc = tune_trial(b, backup, steps, bestT, c, bestS, bestM, &ms);
             if (!rc && ms < best)
             {
                 best = ms;
@@ -2821,10 +2990,16 @@ namespace
         dfree(backup);
         b->tileWaves = bestT;
         b->updateWaves = bestU;
+        b->subSize = bestS;
+        b->subMax = bestM;
         if (tileWavesOut)
             *tileWavesOut = bestT;
         if (updateWavesOut)
             *updateWavesOut = bestU;
+        if (subSizeOut)
+            *subSizeOut = bestS;
+        if (subMaxOut)
+            *subMaxOut = bestM;
         return rc;
     }
 } // namespace
@@ -3017,9 +3192,20 @@ int crHipBatchSetGraphMode(CrHipBatch *b, int enabled)
     return CRMPM_HIP_OK;
 }
 
-int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves)
+int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax)
 {
-    return batch_autotune(b, stepsPerTrial, tileWaves, updateWaves);
+    return batch_autotune(b, stepsPerTrial, tileWaves, updateWaves, subSize, subMax);
+}
+
+int crHipBatchSetTileSplit(CrHipBatch *b, int subSize, int subMax)
+{
+    if (!b || subSize < 64 || subSize % 64 != 0 || subMax < 1 || subMax > 8)
+        return CRMPM_HIP_ERR_ARG;
+    if (b->initialized && subMax > b->subStride)
+        return CRMPM_HIP_ERR_CAPACITY;
+    b->subSize = subSize;
+    b->subMax = subMax;
+    return CRMPM_HIP_OK;
 }
 
 int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves)
@@ -3230,6 +3416,11 @@ int crHipSolverStep(CrHipSolver *s, float *dtOut)
     if (dtOut)
         *dtOut = s->b->scenes[0].dt;
     return batch_step(s->b);
+}
+
+int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax)
+{
+    return s ? batch_autotune(s->b, stepsPerTrial, tileWaves, updateWaves, subSize, subMax) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int *orderOut)
@@ -3523,11 +3714,12 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             BatchArgs args = batch_args(b, a->segOrder[j]);
             args.segStart = a->segStart[j];
             args.segEnd = a->segEnd[j];
-            const int wgs = grid1d(b->totalBlocks, kWaves);
-            k_adj_g2p<<<wgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
-                                            a->segGrid[j], ad);
-            k_adj_grid<<<wgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
-            k_adj_p2g<<<wgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
+            const int subWgs = grid1d(b->totalBlocks * b->subMax, kWaves);
+            const int blockWgs = grid1d(b->totalBlocks, kWaves);
+            k_adj_g2p<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
+                                               a->segGrid[j], ad);
+            k_adj_grid<<<blockWgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
+            k_adj_p2g<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
             rc = check_launch();
         }
     }

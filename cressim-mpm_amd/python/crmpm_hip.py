@@ -76,6 +76,8 @@ EXPORTS = {
     "crHipSolverProfileBegin": [_vp, ctypes.c_int],
     "crHipSolverProfileEnd": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int],
     "crHipSolverProfileMask": [_vp, ctypes.c_uint],
+    "crHipSolverAutotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipBatchCreate": [ctypes.c_int, ctypes.c_int, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)],
     "crHipBatchDestroy": [_vp],
     "crHipBatchSetStream": [_vp, _vp],
@@ -94,7 +96,9 @@ EXPORTS = {
     "crHipBatchComputeInitialData": [_vp, ctypes.c_int, ctypes.c_uint, _vp],
     "crHipBatchStep": [_vp],
     "crHipBatchSetGraphMode": [_vp, ctypes.c_int],
-    "crHipBatchAutotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchAutotune": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchSetTileSplit": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchSetLaunchSizes": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchFetchResults": [_vp],
     "crHipBatchGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -205,6 +209,7 @@ class HipSolver:
             self.torque = torch.zeros((cap, 4), dtype=torch.float32, device=dev)
             self.gtype = T(t["gtype"], np.int32)
             self.gparams = T(t["gparams"], np.float32)
+            self._gparams_host = np.array(t["gparams"], np.float32)
             self.sdf_dims = T(sc["sdf_dims"], np.int32)
             self.sdf_lbcs = T(sc["sdf_lbcs"], np.float32)
             self.sdf_data = T(sc["sdf_data"], np.float32)
@@ -216,6 +221,10 @@ class HipSolver:
             self.sh_stage.numpy()[:] = np.stack([t[k] for k in S.SHAPE_F4])
             with torch.cuda.stream(self.stream):
                 self.sh_block.copy_(self.sh_stage, non_blocking=True)
+                if self.gparams.shape == t["gparams"].shape and not np.array_equal(self._gparams_host, t["gparams"]):
+                    # Geometry::setParams between frames
+                    self._gparams_host = t["gparams"].copy()
+                    self.gparams.copy_(torch.from_numpy(self._gparams_host))
                 self.sh_copied = torch.cuda.Event()
                 self.sh_copied.record(self.stream)
         p = lambda x: x.data_ptr()
@@ -297,6 +306,14 @@ class HipSolver:
 
     def profile_begin(self, max_launches):
         check(self.lib.crHipSolverProfileBegin(self.h, max_launches), "profile begin")
+
+    def autotune(self, steps_per_trial=2):
+        """Tile split and launch sizes for this scene (state unchanged);
+        returns (tile waves, update waves, sub-tile size, sub-tiles per pblock)."""
+        t, u, ss, sm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipSolverAutotune(self.h, steps_per_trial, ctypes.byref(t), ctypes.byref(u), ctypes.byref(ss),
+                                           ctypes.byref(sm)), "autotune")
+        return t.value, u.value, ss.value, sm.value
 
     def profile_mask(self, categories):
         """Record events only for launches of the named categories (empty: pause)."""
@@ -438,9 +455,14 @@ class HipBatch:
         check(self.lib.crHipBatchSetGraphMode(self.h, 1 if on else 0), "graph mode")
 
     def autotune(self, steps_per_trial=3):
-        t, u = ctypes.c_int(), ctypes.c_int()
-        check(self.lib.crHipBatchAutotune(self.h, steps_per_trial, ctypes.byref(t), ctypes.byref(u)), "autotune")
-        return t.value, u.value
+        """Returns (tile waves, update waves, sub-tile size, sub-tiles per pblock)."""
+        t, u, ss, sm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipBatchAutotune(self.h, steps_per_trial, ctypes.byref(t), ctypes.byref(u), ctypes.byref(ss),
+                                          ctypes.byref(sm)), "autotune")
+        return t.value, u.value, ss.value, sm.value
+
+    def set_tile_split(self, sub_size, sub_max):
+        check(self.lib.crHipBatchSetTileSplit(self.h, sub_size, sub_max), "tile split")
 
     def set_launch_sizes(self, tile_waves, update_waves):
         check(self.lib.crHipBatchSetLaunchSizes(self.h, tile_waves, update_waves), "launch sizes")
@@ -501,7 +523,7 @@ class DifferentiableMls:
     V0 comes from computeInitialData on the initial positions once; every
     rollout restarts from those positions with F = I and C = 0."""
 
-    def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0"):
+    def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0", tune=False):
         import torch
 
         if scene["kind"] != S.MLS or scene["shape_list"]:
@@ -509,6 +531,8 @@ class DifferentiableMls:
         self.torch = torch
         self.solver = HipSolver(scene, device=device)
         self.solver.compute_initial_data()
+        if tune:
+            self.tuned = self.solver.autotune(steps_per_trial=2)
         self.steps = steps
         self.n = self.solver.n
         h = _vp()

@@ -111,10 +111,17 @@ def shape_tables(scene):
 
 
 def apply_frame(scene, frame):
-    """Kinematic target for one frame: teleport to the frame's pose, set its velocity."""
-    s = scene["shape_list"][frame["shape"]]
-    s["pos"] = tuple(frame["pos"]) + (0.0,)
-    s["linVel"] = tuple(frame["lin_vel"]) + (0.0,)
+    """Kinematic target for one frame: teleport to the frame's pose, set its
+    velocity (src/engine/shape_impl.cpp:72-107).  A frame may move several
+    shapes ("moves": [(shape, pos, lin_vel)]) and change geometry parameters
+    ("geoms": [(geometry, params)], Geometry::setParams)."""
+    moves = frame.get("moves") or [(frame["shape"], frame["pos"], frame["lin_vel"])]
+    for sid, pos, vel in moves:
+        s = scene["shape_list"][sid]
+        s["pos"] = tuple(pos) + (0.0,)
+        s["linVel"] = tuple(vel) + (0.0,)
+    for gi, params in frame.get("geoms", []):
+        scene["geoms"][gi] = (scene["geoms"][gi][0], tuple(float(x) for x in params))
 
 
 def lattice_block(rng, n, origin, spacing, jitter):
@@ -364,3 +371,51 @@ def config4(nx=64, ny=32, nz=32, steps=100, seed=20261203 + 4, field_seed=None):
     sc["type"][:] = COROTATIONAL
     sc["steps"] = steps
     return sc, log_e
+
+
+def config5(nx=256, ny=32, nz=256, steps=500, seed=20261203 + 5, frame_substeps=10, speed=0.02):
+    """MLS at scale with cutting: a 256 x 32 x 256 = 2,097,152-particle tissue
+    block (spacing 5 mm, 8 particles per cell, 1.28 x 0.16 x 1.28 m) on a
+    [0, 2.56] x [0, 1.28] x [0, 2.56] m grid with dx = 0.01 (257 x 129 x 257
+    nodes), co-rotational (StVK) E = 2e4, nu = 0.3, dt = 1e-4.  A kinematic
+    vertical capsule tool (radius 5 mm) 2 cm deep in the top surface sweeps
+    along +x at 0.02 m/s through the block's mid plane z = 1.28; a kinematic
+    quad slicer in that plane spans the swept part of the cut (its half
+    extent grows with the swept length, one Geometry::setParams per frame).
+    The block keeps 2.5 dx from the lower faces (base-cell weights)."""
+    rng = SplitMix64(seed)
+    dx = 0.01
+    h = 0.005
+    dt = 1e-4
+    N = nx * ny * nz
+    sc = new_scene(MLS, dx, (0, 0, 0), (2.56, 1.28, 2.56), N, dt)
+    size = np.array([nx, ny, nz]) * h
+    origin = np.array([1.28 - size[0] / 2, 2.5 * dx, 1.28 - size[2] / 2])
+    sc["xm"][:, :3] = lattice_block(rng, (nx, ny, nz), origin, (h, h, h), 0.25 * h)
+    sc["xm"][:, 3] = 1000.0 * h ** 3
+    lam, mu = lame(2e4, 0.3)
+    sc["params"][:, 0] = lam
+    sc["params"][:, 1] = mu
+    sc["type"][:] = COROTATIONAL
+    top = origin[1] + size[1]
+    depth, r, hh = 0.02, 0.005, 0.05
+    x0 = origin[0] - 2 * r
+    zc = 1.28
+    tool = add_shape(sc, CAPSULE, (r, hh, 0, 0), pos=(x0, top - depth + r + hh, zc), stype=KINEMATIC,
+                     params0=(0.0, 1.0, 0.3, 0.0), lin_vel=(speed, 0, 0))
+    # quad in the local xz plane; +90 degrees about x puts its normal (local y) along z
+    q = (math.sin(math.pi / 4), 0.0, 0.0, math.cos(math.pi / 4))
+    slicer = add_shape(sc, QUAD_SLICER, (0.0, depth / 2 + 0.005, 0, 0), pos=(x0, top - depth / 2, zc), rot=q,
+                       stype=KINEMATIC, params0=(0.0, 1.0, 0.0, 0.0))
+    g_slicer = sc["shape_list"][slicer]["geom"]
+    frames = []
+    for f in range((steps + frame_substeps - 1) // frame_substeps):
+        t_end = dt * frame_substeps * (f + 1)
+        xt = x0 + speed * t_end
+        frames.append(dict(substeps=min(frame_substeps, steps - f * frame_substeps),
+                           moves=[(tool, (xt, top - depth + r + hh, zc), (speed, 0.0, 0.0)),
+                                  (slicer, ((x0 + xt) / 2, top - depth / 2, zc), (speed / 2, 0.0, 0.0))],
+                           geoms=[(g_slicer, ((xt - x0) / 2, depth / 2 + 0.005, 0.0, 0.0))]))
+    sc["frames"] = frames
+    sc["steps"] = steps
+    return sc

@@ -143,6 +143,9 @@ int crHipSolverGetStateArrays(CrHipSolver *s, float **F, float **C, float **V0, 
  * get the key nbx * nby * nbz) and orderOut[k] the particle index, ties in
  * index order (a stable sort).  Both are device arrays of numActive entries. */
 int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int *orderOut);
+/* crHipBatchAutotune for the solver's one-scene batch (no counterpart in the
+ * reference): picks the tile split and launch sizes on the current state. */
+int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
 /* Launch timing, an addition of this solver (the reference times nothing on
  * the device).  Between Begin and End every particle-transfer, grid-update and
  * binning launch of crHipSolverStep is bracketed by a pair of hipEvents on the
@@ -221,11 +224,19 @@ int crHipBatchStep(CrHipBatch *b);
  * launches into a hipGraph once and replays it while the launch configuration
  * (active particle count, iterations, launch sizes, bindings) is unchanged. */
 int crHipBatchSetGraphMode(CrHipBatch *b, int enabled);
+/* Tile split: each pblock's particles run as up to subMax sub-tiles of subSize
+ * particles (a multiple of 64; the last sub-tile takes the rest), one
+ * wavefront and one partial box each, so dense pblocks spread over several
+ * wavefronts.  subMax <= 8 (fewer when the partial boxes of a very large
+ * batch would exceed 8 GiB: CRMPM_HIP_ERR_CAPACITY).  Default: whole pblocks
+ * (subMax 1). */
+int crHipBatchSetTileSplit(CrHipBatch *b, int subSize, int subMax);
 /* Autotuner: times substeps of the batch's current state (saved and restored
- * around the trials, so the state is unchanged afterwards) for several launch
- * sizes of the pblock kernels and keeps the fastest.  Launch sizes are in
- * wavefronts; 0 means one wavefront per pblock. */
-int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves);
+ * around the trials, so the state is unchanged afterwards) for several tile
+ * splits and launch sizes and keeps the fastest.  Launch sizes are in
+ * wavefronts; 0 means one wavefront per (sub-)tile or node block.  Any
+ * output pointer may be null. */
+int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
 int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
 int crHipBatchFetchResults(CrHipBatch *b);
 int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, int *stride);

@@ -105,16 +105,18 @@ def test_autotune_keeps_state_and_results():
     b.compute_initial_data()
     b.step(2)
     before = [b.state(i) for i in range(len(scenes))]
-    t, u = b.autotune(steps_per_trial=2)
-    print(f"autotune picked tile waves {t}, update waves {u}", flush=True)
+    t, u, ss, sm = b.autotune(steps_per_trial=2)
+    print(f"autotune picked tile waves {t}, update waves {u}, sub-tiles {sm} x {ss}", flush=True)
     after = [b.state(i) for i in range(len(scenes))]
     for i in range(len(scenes)):
         for k in FIELDS:
             assert np.array_equal(before[i][k], after[i][k]), (i, k)
-    # every launch size gives the same arithmetic: persistent launches against one wavefront per pblock
+    # every launch size gives the same arithmetic (the tile split fixes the
+    # summation order): persistent launches against one wavefront per sub-tile
     ref = H.HipBatch(copy.deepcopy(scenes))
     ref.compute_initial_data()
     ref.step(2)
+    ref.set_tile_split(ss, sm)
     b.set_launch_sizes(2048, 2048)
     b.step(3)
     ref.step(3)

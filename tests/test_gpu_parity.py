@@ -132,6 +132,18 @@ def test_parity_pb_slab_100_substeps():
         assert errs[k] <= TOL_100, (k, errs[k])
 
 
+def test_parity_mls_cutting_30_substeps():
+    """The cutting scene at reduced size: a 24 x 8 x 24 block, the capsule tool
+    and the growing quad slicer sweeping at 1 m/s (so the cut crosses many
+    particles within 30 substeps), on the full 257 x 129 x 257-node grid."""
+    sc = S.config5(nx=24, ny=8, nz=24, steps=30, speed=1.0)
+    gs = hip_run(sc, 30)
+    os_ = cpu_run(sc, 30)
+    errs = report("mls cutting 24x8x24 30 steps", gs, os_)
+    for k in ("xm", "v", "F"):
+        assert errs[k] <= TOL_10, (k, errs[k])
+
+
 @pytest.mark.parametrize("kind", list(KINDS))
 def test_bitwise_reproducible(kind):
     sc = S.small_scene(KINDS[kind], 5, shapes="plane+sphere+box")

@@ -279,3 +279,16 @@ def test_config3_generator():
         assert x["xm"][:, :3].min() > 3 * x["dx"] and x["xm"][:, :3].max() < 1 - 3 * x["dx"]
         G = round(1 / x["dx"])
         assert G % 4 == 0 and 32 <= G <= 128
+
+
+def test_config5_generator():
+    """Cutting scene: 2,097,152 particles on 257 x 129 x 257 nodes, the block
+    inset from the lower faces, the slicer's half extent growing by the swept
+    length each frame."""
+    sc = S.config5()
+    assert len(sc["xm"]) == 256 * 32 * 256
+    assert S.grid_dims(sc) == (257, 129, 257)
+    assert sc["xm"][:, 1].min() > 2 * sc["dx"]
+    halves = [f["geoms"][0][1][0] for f in sc["frames"]]
+    assert np.all(np.diff(halves) > 0)
+    assert abs(halves[-1] - 0.02 * 500 * 1e-4 / 2) < 1e-9
