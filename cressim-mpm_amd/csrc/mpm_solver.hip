@@ -60,10 +60,15 @@ namespace
         const unsigned char *mask;
     };
 
+    // Solver-owned particle state, particle order.  F and C are three float4
+    // columns per particle (the reference's Mat3x3f layout), so a particle's
+    // matrix is 48 contiguous bytes: the tile kernels visit particles in
+    // sorted order, and per-particle records keep those scattered accesses
+    // to few cache lines.
     struct SolverState
     {
-        float *F;   // [9][stride]
-        float *C;   // [9][stride]
+        float4 *F;  // [3 * stride]: column c of particle p at 3 p + c
+        float4 *C;
         float *V0;
         int stride;
     };
@@ -239,6 +244,18 @@ namespace
         bz = key / (g.nbx * g.nby);
     }
 
+    CR_DEV M3 load_cols(const float4 *m, int p)
+    {
+        const float4 a = m[3 * p], b = m[3 * p + 1], c = m[3 * p + 2];
+        return M3{V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}};
+    }
+    CR_DEV void store_cols(float4 *m, int p, const M3 &v)
+    {
+        m[3 * p] = make_float4(v.c0.x, v.c0.y, v.c0.z, 0.0f);
+        m[3 * p + 1] = make_float4(v.c1.x, v.c1.y, v.c1.z, 0.0f);
+        m[3 * p + 2] = make_float4(v.c2.x, v.c2.y, v.c2.z, 0.0f);
+    }
+    // Planes [9][stride] (element 3 column + row): the PB staging in sorted order.
     CR_DEV M3 load_m3(const float *plane, int stride, int p)
     {
         return M3{V3{plane[0 * stride + p], plane[1 * stride + p], plane[2 * stride + p]},
@@ -1047,8 +1064,8 @@ namespace
                 float4 vv = c.P.v[p];
                 const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
                 c.P.v[p] = make_float4(vel.x, vel.y, vel.z, vv.w);
-                const M3 F = load_m3(c.S.F, c.S.stride, p);
-                const M3 Cm = load_m3(c.S.C, c.S.stride, p);
+                const M3 F = load_cols(c.S.F, p);
+                const M3 Cm = load_cols(c.S.C, p);
                 const float4 prm = c.P.params[p];
                 const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], F);
                 const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
@@ -1285,9 +1302,9 @@ namespace
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
                 c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                 c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                store_m3(c.S.C, c.S.stride, p, Cm);
-                const M3 F = load_m3(c.S.F, c.S.stride, p);
-                store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+                store_cols(c.S.C, p, Cm);
+                const M3 F = load_cols(c.S.F, p);
+                store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
             }
         }
     }
@@ -1320,12 +1337,8 @@ namespace
         st.vs[k] = v.x;
         st.vs[st.stride + k] = v.y;
         st.vs[2 * st.stride + k] = v.z;
-#pragma unroll
-        for (int e = 0; e < 9; ++e)
-        {
-            st.Fs[e * st.stride + k] = S.F[e * S.stride + p];
-            st.Cs[e * st.stride + k] = S.C[e * S.stride + p];
-        }
+        store_m3(st.Fs, st.stride, k, load_cols(S.F, p));
+        store_m3(st.Cs, st.stride, k, load_cols(S.C, p));
         const float4 prm = P.params[p];
         st.ps[k] = make_float2(prm.x, prm.y);
         st.ts[k] = P.type[p];
@@ -1440,9 +1453,9 @@ namespace
                     resolve_rigid<true>(c.sv, nShapes, vv, x);
                     c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                     c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                    store_m3(c.S.C, c.S.stride, p, Cm);
+                    store_cols(c.S.C, p, Cm);
                     const M3 F = load_m3(st.Fs, S, k);
-                    store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+                    store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
                 }
             }
             if (Mode < 2)
@@ -1474,7 +1487,7 @@ namespace
         c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
         c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
         const M3 F = load_m3(st.Fs, S, k);
-        store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + Cm * c.dt, F));
+        store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
     }
 
     // ------------------------------------------------------------------
@@ -1516,7 +1529,7 @@ namespace
                 Weights W;
                 weights_of(fx, W);
                 const float4 prm = c.P.params[p];
-                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_m3(c.S.F, c.S.stride, p));
+                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_cols(c.S.F, p));
                 const V3 vel = v3(c.P.v[p]);
                 const float nv0 = -c.S.V0[p];
                 const bool fast = nodes_inside(g, bx, by, bz);
@@ -1605,8 +1618,8 @@ namespace
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
                 c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                 c.P.v[p] = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
-                const M3 F = load_m3(c.S.F, c.S.stride, p);
-                store_m3(c.S.F, c.S.stride, p, mul(m3_identity() + gv * c.dt, F));
+                const M3 F = load_cols(c.S.F, p);
+                store_cols(c.S.F, p, mul(m3_identity() + gv * c.dt, F));
             }
         }
     }
@@ -1631,7 +1644,7 @@ namespace
         const StepCommon c = scene_view_of(b, s);
         const GridGeom &g = c.g;
         const int p = b.scenes[s].particleOffset + (int)idx[i];
-        store_m3(c.S.F, c.S.stride, p, m3_identity());
+        store_cols(c.S.F, p, m3_identity());
         const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
@@ -1676,7 +1689,7 @@ namespace
         const StepCommon c = scene_view_of(b, s);
         const GridGeom &g = c.g;
         const int p = b.scenes[s].particleOffset + (int)idx[i];
-        store_m3(c.S.F, c.S.stride, p, m3_identity());
+        store_cols(c.S.F, p, m3_identity());
         const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
@@ -1737,8 +1750,8 @@ namespace
     {
         const float4 *x;
         const float4 *v;
-        const float *F;
-        const float *C;
+        const float4 *F; // columns, as SolverState
+        const float4 *C;
         int stride;
     };
 
@@ -1747,10 +1760,10 @@ namespace
     {
         float4 *x;
         float4 *v;
-        float *F;
-        float *C;
+        float4 *F; // columns, as SolverState
+        float4 *C;
         float4 *xTmp; // G2P-adjoint part of the x adjoint
-        float *FTmp;  // G2P-adjoint part of the F adjoint
+        float4 *FTmp; // G2P-adjoint part of the F adjoint
         double *lambdaBar;
         double *muBar;
         int stride;
@@ -1829,12 +1842,12 @@ namespace
                 const int p = (int)c.order[k];
                 const V3 x = v3(s0.x[p]);
                 const V3 vn = v3(s1.v[p]);
-                const M3 Cn = load_m3(s1.C, s1.stride, p);
-                const M3 F = load_m3(s0.F, s0.stride, p);
+                const M3 Cn = load_cols(s1.C, p);
+                const M3 F = load_cols(s0.F, p);
                 const V3 xb = v3(ad.x[p]);
                 const V3 vb = v3(ad.v[p]);
-                const M3 Fb = load_m3(ad.F, ad.stride, p);
-                const M3 Cb = load_m3(ad.C, ad.stride, p);
+                const M3 Fb = load_cols(ad.F, p);
+                const M3 Cb = load_cols(ad.C, p);
                 // x+ = clamp(x + dt v+): the adjoint passes where the clamp is inactive
                 const V3 xp = x + vn * c.dt;
                 const V3 keep{(xp.x > g.bmin.x && xp.x < g.bmax.x) ? 1.f : 0.f,
@@ -1844,7 +1857,7 @@ namespace
                 const V3 vaccb = vb + xbk * c.dt;
                 // F+ = (I + dt C+) F
                 const M3 Bb = (Cb + mul(Fb, transpose(F)) * c.dt) * (4.0f * g.invdx);
-                store_m3(ad.FTmp, ad.stride, p, mul(transpose(m3_identity() + Cn * c.dt), Fb));
+                store_cols(ad.FTmp, p, mul(transpose(m3_identity() + Cn * c.dt), Fb));
                 V3 xacc = xbk;
                 int bx, by, bz;
                 V3 fx;
@@ -2034,8 +2047,8 @@ namespace
                 const V3 x = v3(xm);
                 const float m = xm.w;
                 const V3 vel = v3(s0.v[p]) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                const M3 F = load_m3(s0.F, s0.stride, p);
-                const M3 Cm = load_m3(s0.C, s0.stride, p);
+                const M3 F = load_cols(s0.F, p);
+                const M3 Cm = load_cols(s0.C, p);
                 const float4 prm = c.P.params[p];
                 const int type = c.P.type[p];
                 const M3 tau = stress_kirchhoff(prm.x, prm.y, type, F);
@@ -2067,7 +2080,7 @@ namespace
                 const D3 taub = d3_comb(Ab, (double)sc, Ab, 0.0, 0.0);
                 const D3 Fd = d3_of(F), FdT = d3_t(Fd);
                 const double lam = prm.x, muv = prm.y;
-                D3 Fb = d3_of(load_m3(ad.FTmp, ad.stride, p));
+                D3 Fb = d3_of(load_cols(ad.FTmp, p));
                 double lb = 0.0, mb = 0.0;
                 if (type == mNeoHookean)
                 {
@@ -2097,8 +2110,8 @@ namespace
                 const V3 xt = v3(ad.xTmp[p]) + xacc;
                 ad.x[p] = make_float4(xt.x, xt.y, xt.z, 0.0f);
                 ad.v[p] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
-                store_m3(ad.F, ad.stride, p, m3_of(Fb));
-                store_m3(ad.C, ad.stride, p, m3_of(d3_comb(Ab, (double)m, Ab, 0.0, 0.0)));
+                store_cols(ad.F, p, m3_of(Fb));
+                store_cols(ad.C, p, m3_of(d3_comb(Ab, (double)m, Ab, 0.0, 0.0)));
                 ad.lambdaBar[p] += lb;
                 ad.muBar[p] += mb;
             }
@@ -2170,7 +2183,8 @@ struct CrHipBatch
     ParticleBind P{};
 
     // owned
-    float *F = nullptr, *C = nullptr, *V0 = nullptr;
+    float4 *F = nullptr, *C = nullptr;
+    float *V0 = nullptr;
     float4 *grid = nullptr, *gridForce = nullptr;
     unsigned char *dirty = nullptr;
     int *segStart = nullptr, *segEnd = nullptr;
@@ -2632,8 +2646,8 @@ namespace
         const int radixTiles = grid1d((long long)N, kRadixTile);
         const int ns = b->numScenes;
         int rc = 0;
-        rc |= dalloc(&b->F, 9 * N);
-        rc |= dalloc(&b->C, 9 * N);
+        rc |= dalloc(&b->F, 3 * N);
+        rc |= dalloc(&b->C, 3 * N);
         rc |= dalloc(&b->V0, N);
         rc |= dalloc(&b->grid, nodes);
         if (b->kind == CRMPM_HIP_STANDARD)
@@ -2706,8 +2720,8 @@ namespace
         if (mc != hipSuccess)
             return CRMPM_HIP_ERR_LAUNCH;
         hipStream_t st = b->stream;
-        rc |= hipMemsetAsync(b->F, 0, 9 * N * sizeof(float), st);
-        rc |= hipMemsetAsync(b->C, 0, 9 * N * sizeof(float), st);
+        rc |= hipMemsetAsync(b->F, 0, 3 * N * sizeof(float4), st);
+        rc |= hipMemsetAsync(b->C, 0, 3 * N * sizeof(float4), st);
         rc |= hipMemsetAsync(b->V0, 0, N * sizeof(float), st);
         rc |= hipMemsetAsync(b->grid, 0, nodes * sizeof(float4), st);
         if (b->gridForce)
@@ -2847,8 +2861,8 @@ namespace
         };
         add(b->P.xm, N * sizeof(float4));
         add(b->P.v, N * sizeof(float4));
-        add(b->F, 9 * N * sizeof(float));
-        add(b->C, 9 * N * sizeof(float));
+        add(b->F, 3 * N * sizeof(float4));
+        add(b->C, 3 * N * sizeof(float4));
         add(b->V0, N * sizeof(float));
         add(b->grid, (size_t)b->totalBlocks * 64 * sizeof(float4));
         add(b->gridForce, (size_t)b->totalBlocks * 64 * sizeof(float4));
@@ -3231,9 +3245,9 @@ int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, in
     if (!b->initialized)
         return CRMPM_HIP_ERR_STATE;
     if (F)
-        *F = b->F;
+        *F = reinterpret_cast<float *>(b->F);
     if (C)
-        *C = b->C;
+        *C = reinterpret_cast<float *>(b->C);
     if (V0)
         *V0 = b->V0;
     if (stride)
@@ -3487,7 +3501,7 @@ namespace
     struct StateBuf
     {
         float4 *x = nullptr, *v = nullptr;
-        float *F = nullptr, *C = nullptr;
+        float4 *F = nullptr, *C = nullptr;
     };
 } // namespace
 
@@ -3503,7 +3517,7 @@ struct CrHipAdjoint
     int **segStart = nullptr, **segEnd = nullptr;
     float4 *gridAdj = nullptr;
     float4 *xTmp = nullptr;
-    float *FTmp = nullptr;
+    float4 *FTmp = nullptr;
 };
 
 namespace
@@ -3512,8 +3526,8 @@ namespace
     {
         int rc = dalloc(&s.x, N);
         rc |= dalloc(&s.v, N);
-        rc |= dalloc(&s.F, 9 * N);
-        rc |= dalloc(&s.C, 9 * N);
+        rc |= dalloc(&s.F, 3 * N);
+        rc |= dalloc(&s.C, 3 * N);
         return rc;
     }
     void state_free(StateBuf &s)
@@ -3532,7 +3546,7 @@ namespace
                    hipSuccess;
         };
         const bool ok = cp(b->P.xm, s.x, N * sizeof(float4)) && cp(b->P.v, s.v, N * sizeof(float4)) &&
-                        cp(b->F, s.F, 9 * N * sizeof(float)) && cp(b->C, s.C, 9 * N * sizeof(float));
+                        cp(b->F, s.F, 3 * N * sizeof(float4)) && cp(b->C, s.C, 3 * N * sizeof(float4));
         return ok ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
     StateRef state_ref(const StateBuf &s, int stride) { return StateRef{s.x, s.v, s.F, s.C, stride}; }
@@ -3610,7 +3624,7 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
     {
         rc |= dalloc(&a->gridAdj, nodes);
         rc |= dalloc(&a->xTmp, N);
-        rc |= dalloc(&a->FTmp, 9 * N);
+        rc |= dalloc(&a->FTmp, 3 * N);
     }
     if (rc)
     {
@@ -3667,17 +3681,17 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
     // scratch array when dL/dx0 is not wanted), F and C in scratch arrays.
     // The final state's adjoint is (dL/dx, dL/dv, 0, 0).
     float4 *xOwn = nullptr;
-    float *Fb = nullptr, *Cb = nullptr;
+    float4 *Fb = nullptr, *Cb = nullptr;
     int rc = CRMPM_HIP_OK;
-    if (dalloc(&Fb, 9 * N) || dalloc(&Cb, 9 * N) || (!dLdx0 && dalloc(&xOwn, N)))
+    if (dalloc(&Fb, 3 * N) || dalloc(&Cb, 3 * N) || (!dLdx0 && dalloc(&xOwn, N)))
         rc = CRMPM_HIP_ERR_ALLOC;
     float4 *xb = dLdx0 ? reinterpret_cast<float4 *>(dLdx0) : xOwn;
     float4 *vb = reinterpret_cast<float4 *>(dLdv0);
     if (!rc && (hipMemcpyAsync(xb, dLdx, N * sizeof(float4), hipMemcpyDeviceToDevice, st) != hipSuccess ||
                 (dLdv ? hipMemcpyAsync(vb, dLdv, N * sizeof(float4), hipMemcpyDeviceToDevice, st)
                       : hipMemsetAsync(vb, 0, N * sizeof(float4), st)) != hipSuccess ||
-                hipMemsetAsync(Fb, 0, 9 * N * sizeof(float), st) != hipSuccess ||
-                hipMemsetAsync(Cb, 0, 9 * N * sizeof(float), st) != hipSuccess ||
+                hipMemsetAsync(Fb, 0, 3 * N * sizeof(float4), st) != hipSuccess ||
+                hipMemsetAsync(Cb, 0, 3 * N * sizeof(float4), st) != hipSuccess ||
                 hipMemsetAsync(dLdLambda, 0, N * sizeof(double), st) != hipSuccess ||
                 hipMemsetAsync(dLdMu, 0, N * sizeof(double), st) != hipSuccess))
         rc = CRMPM_HIP_ERR_LAUNCH;

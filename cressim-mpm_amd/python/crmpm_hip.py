@@ -144,6 +144,21 @@ def check(rc, what):
         raise RuntimeError(f"{what} failed with status {rc}")
 
 
+def _read_state(torch, device, stream, F, C, V0, stride, a, e):
+    """F, C as (n, 9) with element 3 * column + row, and V0 (n,), of particles
+    a..e-1, float64.  The solver keeps F and C as three float4 columns per
+    particle and V0 as one float per particle."""
+    out = {}
+    for name, ptr in (("F", F), ("C", C)):
+        t = torch.empty((stride, 3, 4), dtype=torch.float32, device=device)
+        hip_memcpy_d2d(t.data_ptr(), ptr, t.numel() * 4, stream)
+        out[name] = t[a:e, :, :3].reshape(e - a, 9).cpu().numpy().astype(np.float64)
+    t = torch.empty(stride, dtype=torch.float32, device=device)
+    hip_memcpy_d2d(t.data_ptr(), V0, stride * 4, stream)
+    out["V0"] = t[a:e].cpu().numpy().astype(np.float64)
+    return out
+
+
 class HipSolver:
     """One scene on the GPU: engine-side buffers + a solver handle."""
 
@@ -280,11 +295,7 @@ class HipSolver:
         F, C, V0, stride = self._state_ptrs
         n = self.n
         out = dict(xm=self.xm.cpu().numpy().astype(np.float64), v=self.v[:, :3].cpu().numpy().astype(np.float64))
-        for name, ptr, rows in (("F", F, 9), ("C", C, 9), ("V0", V0, 1)):
-            t = torch.empty(rows * stride, dtype=torch.float32, device=self.device)
-            hip_memcpy_d2d(t.data_ptr(), ptr, rows * stride * 4, self.stream.cuda_stream)
-            a = t.cpu().numpy().reshape(rows, stride)[:, :n].T.astype(np.float64)
-            out[name] = a if rows > 1 else a[:, 0]
+        out.update(_read_state(torch, self.device, self.stream.cuda_stream, F, C, V0, stride, 0, n))
         return out
 
     def grid(self):
@@ -477,11 +488,7 @@ class HipBatch:
         F, C, V0, stride = self._state_ptrs
         a, e = int(self.offsets[i]), int(self.offsets[i + 1])
         out = dict(xm=self.xm[a:e].cpu().numpy().astype(np.float64), v=self.v[a:e, :3].cpu().numpy().astype(np.float64))
-        for name, ptr, rows in (("F", F, 9), ("C", C, 9), ("V0", V0, 1)):
-            t = torch.empty(rows * stride, dtype=torch.float32, device=self.device)
-            hip_memcpy_d2d(t.data_ptr(), ptr, rows * stride * 4, self.stream.cuda_stream)
-            arr = t.cpu().numpy().reshape(rows, stride)[:, a:e].T.astype(np.float64)
-            out[name] = arr if rows > 1 else arr[:, 0]
+        out.update(_read_state(torch, self.device, self.stream.cuda_stream, F, C, V0, stride, a, e))
         return out
 
     def grid(self, i):
@@ -542,10 +549,10 @@ class DifferentiableMls:
         F, C, V0, stride = self.solver._state_ptrs
         self.stride = stride
         dev = self.solver.device
-        eye = torch.zeros((9, stride), dtype=torch.float32, device=dev)
-        eye[0] = eye[4] = eye[8] = 1.0
+        eye = torch.zeros((stride, 3, 4), dtype=torch.float32, device=dev)
+        eye[:, 0, 0] = eye[:, 1, 1] = eye[:, 2, 2] = 1.0
         self._eye = eye
-        self._zero = torch.zeros((9, stride), dtype=torch.float32, device=dev)
+        self._zero = torch.zeros((stride, 3, 4), dtype=torch.float32, device=dev)
 
     def _reset(self, lam, mu, v0):
         s = self.solver

@@ -131,9 +131,11 @@ int crHipSolverFetchResults(CrHipSolver *s);
 
 /* Solver-owned per-particle state, which the reference keeps private
  * (src/solver/gpu_mls_mpm_solver.cuh:61-77).  Device pointers into the
- * solver's arrays, structure-of-arrays: element k of F (column-major, k =
- * 3 * column + row) for particle p is F[k * stride + p]; same for C; V0[p].
- * Used for checkpoints, the adjoint and parity checks. */
+ * solver's arrays.  F and C hold three float4 columns per particle, as the
+ * reference's Mat3x3f: element (row r, column c) of particle p is
+ * F[12 * p + 4 * c + r] (the fourth float of a column is padding); same for
+ * C; V0[p].  stride is the number of particle slots.  Used for checkpoints,
+ * the adjoint and parity checks. */
 int crHipSolverGetStateArrays(CrHipSolver *s, float **F, float **C, float **V0, int *stride);
 /* Binning stage of a substep, exposed for verification; it has no counterpart
  * in the reference, which scatters in particle order.  For the current
