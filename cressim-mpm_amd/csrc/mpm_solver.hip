@@ -176,7 +176,8 @@ namespace
     // View of the scene owning global pblock tg; t receives the scene-local pblock.
     CR_DEV StepCommon scene_view(const BatchArgs &b, int tg, int &t)
     {
-        const int s = __builtin_amdgcn_readfirstlane(b.blockScene[tg]);
+        // one scene: no map load in front of the dependent loads that follow
+        const int s = b.numScenes == 1 ? 0 : __builtin_amdgcn_readfirstlane(b.blockScene[tg]);
         t = tg - ((ConstSceneRec *)(size_t)b.scenes)[s].blockOffset;
         return scene_view_of(b, s);
     }
@@ -1115,12 +1116,15 @@ namespace
     // (src/solver/mls_mpm_algorithm.h:243-294, pb 233-284, standard 217-272).
     // ------------------------------------------------------------------
     // Multi: pblocks may have more than one sub-tile (subMax > 1); the
-    // single-sub-tile variant needs fewer registers.
-    template <int Kind, bool Multi> // Kind: 0 MLS, 1 PB, 2 standard
-    __global__ __launch_bounds__(kWG) void k_update(BatchArgs b, unsigned char *dirty)
+    // single-sub-tile variant needs fewer registers.  The kernel is bound by
+    // the latency of its dependent loads (segment counts, then partials), so
+    // it is capped at 80 VGPRs for 6 wavefronts per SIMD.
+    template <int Kind, bool Multi, bool Persistent> // Kind: 0 MLS, 1 PB, 2 standard
+    __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k_update(BatchArgs b,
+                                                                                            unsigned char *dirty)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg += tile_stride())
+        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg = next_tile<Persistent>(nbg, b.totalBlocks))
         {
             int nb;
             const StepCommon c = scene_view(b, nbg, nb);
@@ -1136,7 +1140,8 @@ namespace
             {
                 const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
                 const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
-                nsub[d] = key >= 0 ? subtiles_of(c.segEnd[key] - c.segStart[key], c.subSize, c.subMax) : 0;
+                const int count = key >= 0 ? c.segEnd[key] - c.segStart[key] : 0;
+                nsub[d] = Multi ? subtiles_of(count, c.subSize, c.subMax) : (count > 0 ? 1 : 0);
                 any = any || nsub[d] > 0;
             }
             if (!any)
@@ -2463,6 +2468,13 @@ namespace
         return CRMPM_HIP_OK;
     }
 
+    template <int Kind>
+    auto update_kernel(bool multi, bool persistent)
+    {
+        return multi ? (persistent ? k_update<Kind, true, true> : k_update<Kind, true, false>)
+                     : (persistent ? k_update<Kind, false, true> : k_update<Kind, false, false>);
+    }
+
     // The launches of one substep (all scenes).
     int enqueue_step(CrHipBatch *b)
     {
@@ -2482,18 +2494,18 @@ namespace
         const int tileWGs = pblock_wgs(b->totalBlocks * b->subMax, b->tileWaves);
         const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
-        const bool multi = b->subMax > 1;
-        auto upd1 = multi ? k_update<1, true> : k_update<1, false>;
+        const bool multi = b->subMax > 1, persU = b->updateWaves > 0;
+        auto upd1 = update_kernel<1>(multi, persU);
         if (b->kind == CRMPM_HIP_MLS)
         {
             launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
-            launch(b, CRMPM_HIP_PROF_UPDATE, multi ? k_update<0, true> : k_update<0, false>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, a, b->dirty);
             launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, tileWGs, kWG, a);
         }
         else if (b->kind == CRMPM_HIP_STANDARD)
         {
             launch(b, CRMPM_HIP_PROF_STD_P2G, pers ? k_std_p2g<true> : k_std_p2g<false>, tileWGs, kWG, a);
-            launch(b, CRMPM_HIP_PROF_UPDATE, multi ? k_update<2, true> : k_update<2, false>, updWGs, kWG, a, b->dirty);
+            launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<2>(multi, persU), updWGs, kWG, a, b->dirty);
             launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, tileWGs, kWG, a);
         }
         else
