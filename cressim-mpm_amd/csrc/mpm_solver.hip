@@ -710,10 +710,17 @@ namespace
     // ------------------------------------------------------------------
     // Conflict-free LDS scatter.  Two lanes can only hit the same node in one
     // node-loop instruction if their particles share a base cell, so lanes are
-    // grouped by base cell and each pass lets one lane per cell scatter with a
-    // plain read-modify-write (ds_read_b128 + ds_write_b128 per node) instead
-    // of four ds_add_f32 atomics.  Lanes sharing a cell take turns in lane
-    // order, so the summation order is fixed and the result reproducible.
+    // grouped by base cell and each pass lets one lane per cell and
+    // accumulation tile scatter with a plain read-modify-write (ds_read_b128 +
+    // ds_write_b128 per node) instead of four ds_add_f32 atomics (which
+    // measure 1.5x slower even when only the second and later lanes of a cell
+    // use them, 4.5x when all lanes do).  The r-th lane of a cell adds into
+    // tile r % Tiles in pass r / Tiles: with several tiles per wavefront, a
+    // sparse tile (cells holding 0-3 particles) scatters in one or two passes
+    // instead of one per particle of its fullest cell, and the tiles are
+    // summed in a fixed order when written out.  Lanes of a cell take turns
+    // in lane order, so the summation order is fixed and the result
+    // reproducible.
     // ------------------------------------------------------------------
     CR_DEV void compiler_fence() { asm volatile("" ::: "memory"); }
 
@@ -766,23 +773,22 @@ namespace
         return peers;
     }
 
-    // Runs body() on every active lane, in passes in which no two lanes share a key.
-    template <class Body>
+    // Runs body(tile) on every active lane, in passes in which no two lanes
+    // with the same tile share a key: the lane of rank r among the active
+    // lanes of its key (in lane order) runs in pass r / Tiles with tile
+    // r % Tiles.
+    template <int Tiles, class Body>
     CR_DEV void in_conflict_free_passes(unsigned key, Body &&body)
     {
         const unsigned long long peers = match_lanes(key);
         const int lane = __lane_id();
-        bool pending = true;
-        for (;;)
+        const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+        const int rank = __popcll(peers & below);
+        const int pass = rank / Tiles, tile = rank - pass * Tiles;
+        for (int p = 0; __ballot(pass >= p) != 0ull; ++p)
         {
-            const unsigned long long pend = __ballot(pending);
-            if (pend == 0ull)
-                break;
-            if (pending && lane == __ffsll((unsigned long long)(pend & peers)) - 1)
-            {
-                body();
-                pending = false;
-            }
+            if (pass == p)
+                body(tile);
             compiler_fence();
         }
     }
@@ -1076,7 +1082,7 @@ namespace
                 // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
                 const ScatterTerms terms = scatter_terms(vm, affine, 1.0f, W, axis_offsets(fx, g.dx));
                 const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
                     if (fast)
                     {
                         ScatterTerms tt = terms;
@@ -1349,8 +1355,12 @@ namespace
         st.ts[k] = P.type[p];
     }
 
-    // Scatter of one PB particle into the LDS tile (pbMpmParticleToGrid loop)
-    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A, int nSlicers, V3 x, float m, V3 vel,
+    // Accumulation tiles per wavefront of the PB kernels (see in_conflict_free_passes).
+    constexpr int kPbTiles = 2;
+
+    // Scatter of one PB particle into the LDS tiles A0[0 .. kPbTiles)
+    // (pbMpmParticleToGrid loop)
+    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A0, int nSlicers, V3 x, float m, V3 vel,
                            const M3 &Cm)
     {
         const GridGeom &g = c.g;
@@ -1362,7 +1372,8 @@ namespace
         const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
         const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-        in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+        in_conflict_free_passes<kPbTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
+            float4 *A = A0 + tile * kTileNodes;
             if (fast)
             {
                 ScatterTerms tt = terms;
@@ -1397,7 +1408,7 @@ namespace
     template <int Mode, bool Persistent>
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
-        __shared__ float4 acc[(Mode < 2) ? kWaves : 1][kTileNodes];
+        __shared__ float4 acc[(Mode < 2) ? kWaves * kPbTiles : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
@@ -1410,10 +1421,10 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            float4 *A = acc[(Mode < 2) ? wave : 0];
+            float4 *A = acc[(Mode < 2) ? wave * kPbTiles : 0];
             float4 *G = lg[(Mode > 0) ? wave : 0];
             if (Mode < 2)
-                for (int i = lane; i < kTileNodes; i += 64)
+                for (int i = lane; i < kPbTiles * kTileNodes; i += 64)
                     A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (Mode > 0)
                 load_tile_grid(c.g, ti, c.grid, G, lane);
@@ -1468,7 +1479,19 @@ namespace
                 wave_sync();
                 float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
                 for (int i = lane; i < kTileNodes; i += 64)
-                    out[i] = A[i];
+                {
+                    float4 v = A[i];
+#pragma unroll
+                    for (int j = 1; j < kPbTiles; ++j)
+                    {
+                        const float4 u = A[j * kTileNodes + i];
+                        v.x += u.x;
+                        v.y += u.y;
+                        v.z += u.z;
+                        v.w += u.w;
+                    }
+                    out[i] = v;
+                }
             }
         }
     }
@@ -1538,7 +1561,7 @@ namespace
                 const V3 vel = v3(c.P.v[p]);
                 const float nv0 = -c.S.V0[p];
                 const bool fast = nodes_inside(g, bx, by, bz);
-                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
                     M3 st = stress;
                     V3 v = vel, f = fx;
                     Weights w = W;
@@ -1870,7 +1893,7 @@ namespace
                 NodeW nw;
                 node_weights(fx, nw);
                 const M3 BbT = transpose(Bb);
-                in_conflict_free_passes(cell_key(ti, bx, by, bz), [&]() {
+                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
                     for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
                         const V3 u = v3(G[li]);
                         const V3 ub = (vaccb + mul(BbT, dist)) * wt;
