@@ -793,6 +793,35 @@ namespace
         }
     }
 
+    // Accumulation tiles per wavefront of the MLS and PB P2G kernels.
+    constexpr int kScatterTiles = 2;
+
+    template <int Tiles>
+    CR_DEV void zero_tiles(float4 *A, int lane)
+    {
+        for (int i = lane; i < Tiles * kTileNodes; i += 64)
+            A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // out = sum of the Tiles accumulation tiles, in tile order.
+    template <int Tiles>
+    CR_DEV void store_tile_sum(const float4 *A, float4 *out, int lane)
+    {
+        for (int i = lane; i < kTileNodes; i += 64)
+        {
+            float4 v = A[i];
+#pragma unroll
+            for (int j = 1; j < Tiles; ++j)
+            {
+                const float4 u = A[j * kTileNodes + i];
+                v.x += u.x;
+                v.y += u.y;
+                v.z += u.z;
+                v.w += u.w;
+            }
+            out[i] = v;
+        }
+    }
+
     CR_DEV void lds_add(float4 *A, int li, float x, float y, float z, float w)
     {
         float4 cur = A[li];
@@ -1039,7 +1068,7 @@ namespace
     template <bool Persistent>
     __global__ __launch_bounds__(kWG) void k_mls_p2g(BatchArgs b)
     {
-        __shared__ float4 acc[kWaves][kTileNodes];
+        __shared__ float4 acc[kWaves * kScatterTiles][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
         {
@@ -1051,9 +1080,8 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            float4 *A = acc[wave];
-            for (int i = lane; i < kTileNodes; i += 64)
-                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 *A0 = acc[wave * kScatterTiles];
+            zero_tiles<kScatterTiles>(A0, lane);
             wave_sync();
             const int nSlicers = c.sv.counts[1];
             const GridGeom &g = c.g;
@@ -1082,7 +1110,8 @@ namespace
                 // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
                 const ScatterTerms terms = scatter_terms(vm, affine, 1.0f, W, axis_offsets(fx, g.dx));
                 const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
+                in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
+                    float4 *A = A0 + tile * kTileNodes;
                     if (fast)
                     {
                         ScatterTerms tt = terms;
@@ -1110,9 +1139,7 @@ namespace
                 });
             }
             wave_sync();
-            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
-            for (int i = lane; i < kTileNodes; i += 64)
-                out[i] = A[i];
+            store_tile_sum<kScatterTiles>(A0, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
         }
     }
 
@@ -1355,10 +1382,7 @@ namespace
         st.ts[k] = P.type[p];
     }
 
-    // Accumulation tiles per wavefront of the PB kernels (see in_conflict_free_passes).
-    constexpr int kPbTiles = 2;
-
-    // Scatter of one PB particle into the LDS tiles A0[0 .. kPbTiles)
+    // Scatter of one PB particle into the LDS tiles A0[0 .. kScatterTiles)
     // (pbMpmParticleToGrid loop)
     CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A0, int nSlicers, V3 x, float m, V3 vel,
                            const M3 &Cm)
@@ -1372,7 +1396,7 @@ namespace
         const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
         const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-        in_conflict_free_passes<kPbTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
+        in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
             float4 *A = A0 + tile * kTileNodes;
             if (fast)
             {
@@ -1408,7 +1432,7 @@ namespace
     template <int Mode, bool Persistent>
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
-        __shared__ float4 acc[(Mode < 2) ? kWaves * kPbTiles : 1][kTileNodes];
+        __shared__ float4 acc[(Mode < 2) ? kWaves * kScatterTiles : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
@@ -1421,11 +1445,10 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            float4 *A = acc[(Mode < 2) ? wave * kPbTiles : 0];
+            float4 *A = acc[(Mode < 2) ? wave * kScatterTiles : 0];
             float4 *G = lg[(Mode > 0) ? wave : 0];
             if (Mode < 2)
-                for (int i = lane; i < kPbTiles * kTileNodes; i += 64)
-                    A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                zero_tiles<kScatterTiles>(A, lane);
             if (Mode > 0)
                 load_tile_grid(c.g, ti, c.grid, G, lane);
             wave_sync();
@@ -1477,21 +1500,7 @@ namespace
             if (Mode < 2)
             {
                 wave_sync();
-                float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
-                for (int i = lane; i < kTileNodes; i += 64)
-                {
-                    float4 v = A[i];
-#pragma unroll
-                    for (int j = 1; j < kPbTiles; ++j)
-                    {
-                        const float4 u = A[j * kTileNodes + i];
-                        v.x += u.x;
-                        v.y += u.y;
-                        v.z += u.z;
-                        v.w += u.w;
-                    }
-                    out[i] = v;
-                }
+                store_tile_sum<kScatterTiles>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
             }
         }
     }
