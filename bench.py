@@ -37,7 +37,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_FUSED = 164
 BYTES_PER_TOUCHED_NODE_FUSED = 32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
-FUSED_KERNEL = "k_pb_tile<1>"
+# the fused iteration kernel (template Mode 1; either launch form)
+FUSED_KERNEL = "k_pb_tile<1,"
 PROFILE_EVERY = 10
 COPY_CAL_BYTES = 256 << 20
 
@@ -58,7 +59,8 @@ def copy_calibration():
 def _short_name(name):
     import re
 
-    m = re.search(r"(k_\w+(?:<\d+>)?|__amd_rocclr_\w+)", name)
+    # kernel name with its template arguments, e.g. k_pb_tile<1, false>
+    m = re.search(r"(k_\w+(?:<[^<>()]*>)?|__amd_rocclr_\w+)", name)
     return m.group(1) if m else name.split("(")[0][-60:]
 
 
@@ -94,25 +96,69 @@ def valu_issue_us(instructions):
     return instructions * 4.0 / SIMDS / CLOCK_HZ * 1e6
 
 
-def summarize(out_dir, title, commands):
+# What --summarize covers per profiled workload: the bench invocation that
+# rocprofv3 wraps, the summary's title and file name, and whether the HBM
+# counter passes (FETCH_SIZE, WRITE_SIZE with their copy calibration, SQ
+# instruction counts) belong to it or the kernel trace alone.
+PROFILE_TARGETS = {
+    "pb_slab": dict(run="python3 bench.py --workload pb_slab --cpu-substeps 0", counters=True,
+                    title="Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations, 500 substeps",
+                    md="pb_slab262k_kernel_trace.md"),
+    "mls": dict(run="python3 bench.py --workload mls", counters=True,
+                title="Kernel trace and HBM traffic: MLS-MPM cutting block, 2,097,152 particles, 500 substeps; "
+                      "then the 32,768-particle cube, 200 substeps",
+                md="mls_cutting2m_kernel_trace.md"),
+    "batched": dict(run="python3 bench.py --workload batched", counters=False,
+                    title="Kernel trace: 64 batched scenes (MLS 100 substeps, PB 20 substeps) and the same scenes "
+                          "one at a time",
+                    md="batched64_kernel_trace.md"),
+    "adjoint": dict(run="python3 bench.py --workload adjoint", counters=False,
+                    title="Kernel trace: differentiable MLS block, 65,536 particles, 100 substeps, forward + adjoint",
+                    md="adjoint65k_kernel_trace.md"),
+}
+
+
+def profile_commands(target):
+    t = PROFILE_TARGETS[target]
+    run = t["run"]
+    cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o run -- {run}"]
+    if t["counters"]:
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = c.split("_")[0].lower()
+            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o run -- {run}")
+            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- "
+                        "python3 bench.py --copy-calibration")
+        cmds.append("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
+                    f"SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o run -- {run}")
+    cmds.append(f"python3 bench.py --summarize DIR --profile-target {target}")
+    return cmds
+
+
+def summarize(out_dir, title, commands, counters=True):
     """Turn the rocprofv3 outputs of one profiling session into the committed
-    summary: kernel-trace statistics, FETCH_SIZE / WRITE_SIZE per dispatch of
-    each kernel, and the copy-kernel calibration that scales them to bytes."""
+    summary: kernel-trace statistics and, with counters, FETCH_SIZE /
+    WRITE_SIZE per dispatch of each kernel, the copy-kernel calibration that
+    scales them to bytes, and the VALU instruction count."""
     stats = _read_csv(os.path.join(out_dir, "trace"), "kernel_stats.csv")
-    fetch, _ = _counter_per_launch(os.path.join(out_dir, "fetch"))
-    write, _ = _counter_per_launch(os.path.join(out_dir, "write"))
-    _, fetch_cp = _counter_per_launch(os.path.join(out_dir, "fetch_copy"))
-    _, write_cp = _counter_per_launch(os.path.join(out_dir, "write_copy"))
-    sq_dir = os.path.join(out_dir, "sq")
-    valu = _counter_per_launch(sq_dir, "SQ_INSTS_VALU")[0] if os.path.isdir(sq_dir) else {}
-    kib = COPY_CAL_BYTES / 1024.0
-    f_fac = kib / max(fetch_cp["__amd_rocclr_copyBuffer"])
-    w_fac = kib / max(write_cp["__amd_rocclr_copyBuffer"])
+    fetch = write = valu = {}
+    f_fac = w_fac = None
+    if counters:
+        fetch, _ = _counter_per_launch(os.path.join(out_dir, "fetch"))
+        write, _ = _counter_per_launch(os.path.join(out_dir, "write"))
+        _, fetch_cp = _counter_per_launch(os.path.join(out_dir, "fetch_copy"))
+        _, write_cp = _counter_per_launch(os.path.join(out_dir, "write_copy"))
+        valu = _counter_per_launch(os.path.join(out_dir, "sq"), "SQ_INSTS_VALU")[0]
+        kib = COPY_CAL_BYTES / 1024.0
+        f_fac = kib / max(fetch_cp["__amd_rocclr_copyBuffer"])
+        w_fac = kib / max(write_cp["__amd_rocclr_copyBuffer"])
     lines = [f"# {title}", "", "Commands (repository root, rocprofv3 on one MI355X):", "", "```"]
-    lines += commands + ["```", "",
-                         "| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
-                         "| HBM MB/dispatch (calibrated) | SQ_INSTS_VALU/dispatch | VALU issue µs |",
-                         "|---|---|---|---|---|---|---|---|---|"]
+    lines += commands + ["```", ""]
+    if counters:
+        lines += ["| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
+                  "| HBM MB/dispatch (calibrated) | SQ_INSTS_VALU/dispatch | VALU issue µs |",
+                  "|---|---|---|---|---|---|---|---|---|"]
+    else:
+        lines += ["| kernel | calls | avg µs | total % |", "|---|---|---|---|"]
     merged = {}
     for r in stats:
         k = _short_name(r["Name"])
@@ -123,16 +169,23 @@ def summarize(out_dir, title, commands):
         fr, wr = fetch.get(k), write.get(k)
         mb = "" if fr is None or wr is None else f"{(fr * f_fac + wr * w_fac) * 1024 / 1e6:.2f}"
         vi = valu.get(k)
-        lines.append(f"| `{k}` | {c} | {t / c / 1e3:.1f} | {100 * t / total:.2f} | "
-                     f"{'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} | "
-                     f"{'' if vi is None else f'{vi:,.0f}'} | {'' if vi is None else f'{valu_issue_us(vi):.1f}'} |")
+        row = f"| `{k}` | {c} | {t / c / 1e3:.1f} | {100 * t / total:.2f} |"
+        if counters:
+            row += (f" {'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} | "
+                    f"{'' if vi is None else f'{vi:,.0f}'} | {'' if vi is None else f'{valu_issue_us(vi):.1f}'} |")
+        lines.append(row)
+    lines += ["", "Kernel names carry their template arguments: the tile kernels' last one selects the persistent "
+              "launch form, which appears in the autotuner's trial substeps."]
+    if not counters:
+        return "\n".join(lines) + "\n", None
     lines += ["", f"Calibration: a device-to-device copy of {COPY_CAL_BYTES >> 20} MiB reads FETCH_SIZE "
               f"{max(fetch_cp['__amd_rocclr_copyBuffer']):,.0f} KiB and WRITE_SIZE {max(write_cp['__amd_rocclr_copyBuffer']):,.0f} KiB "
               f"per dispatch, so bytes = FETCH_SIZE x {f_fac:.3f} KiB + WRITE_SIZE x {w_fac:.3f} KiB.",
               "", "VALU issue µs = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz): the time the VALU pipes need "
               "for the kernel's vector instructions; a kernel whose duration is close to it is bound by VALU issue, not HBM."]
-    result = {"kernel": FUSED_KERNEL, "fetch_kib_raw": fetch.get(FUSED_KERNEL), "write_kib_raw": write.get(FUSED_KERNEL),
-              "fetch_factor": f_fac, "write_factor": w_fac, "valu_instructions": valu.get(FUSED_KERNEL)}
+    fused = max((k for k in merged if k.startswith(FUSED_KERNEL)), key=lambda k: merged[k][0], default=None)
+    result = {"kernel": fused, "fetch_kib_raw": fetch.get(fused), "write_kib_raw": write.get(fused),
+              "fetch_factor": f_fac, "write_factor": w_fac, "valu_instructions": valu.get(fused)}
     if result["fetch_kib_raw"] is not None and result["write_kib_raw"] is not None:
         result["bytes_per_launch"] = 1024.0 * (result["fetch_kib_raw"] * f_fac + result["write_kib_raw"] * w_fac)
     return "\n".join(lines) + "\n", result
@@ -375,25 +428,20 @@ def main():
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
                     help="write the profile summary from rocprofv3 outputs under DIR/{trace,fetch,write,fetch_copy,write_copy,sq}")
+    ap.add_argument("--profile-target", choices=sorted(PROFILE_TARGETS), default="pb_slab",
+                    help="which profiled workload --summarize describes")
     ap.add_argument("--summary-out", metavar="DIR", default=os.path.join(ROOT, "profiles"),
                     help="where --summarize writes its .md and .json (default: profiles/)")
     args = ap.parse_args()
     if args.summarize:
-        run = "python3 bench.py --cpu-substeps 0"
-        cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o pb -- {run}"]
-        for c in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = c.split("_")[0].lower()
-            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o pb -- {run}")
-            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- python3 bench.py --copy-calibration")
-        cmds.append("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
-                    f"SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o pb -- {run}")
-        cmds.append("python3 bench.py --summarize DIR")
-        md, traffic = summarize(args.summarize, "Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations", cmds)
+        tgt = PROFILE_TARGETS[args.profile_target]
+        md, traffic = summarize(args.summarize, tgt["title"], profile_commands(args.profile_target), tgt["counters"])
         os.makedirs(args.summary_out, exist_ok=True)
-        with open(os.path.join(args.summary_out, "pb_slab262k_kernel_trace.md"), "w") as f:
+        with open(os.path.join(args.summary_out, tgt["md"]), "w") as f:
             f.write(md)
-        with open(os.path.join(args.summary_out, os.path.basename(TRAFFIC_FILE)), "w") as f:
-            json.dump(traffic, f, indent=1)
+        if args.profile_target == "pb_slab":
+            with open(os.path.join(args.summary_out, os.path.basename(TRAFFIC_FILE)), "w") as f:
+                json.dump(traffic, f, indent=1)
         print(md)
         return
     if args.copy_calibration:
@@ -487,7 +535,7 @@ def main():
                    "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
                    "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu",
                    "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned))},
-        "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1> (fused G2P+P2G)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1, ...> (fused G2P+P2G)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
                      "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched,

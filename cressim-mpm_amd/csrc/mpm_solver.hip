@@ -87,7 +87,8 @@ namespace
         ShapeView sv;
         const int *segStart;
         const int *segEnd;
-        const unsigned *order; // sorted particle indices (global)
+        const unsigned *order;       // particle indices by tile, each tile in (cell rank, cell) order (global)
+        const unsigned *gatherOrder; // the same tiles, each in particle index order (kernels that do not scatter)
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
@@ -131,6 +132,7 @@ namespace
         const int *segStart; // global pblock
         const int *segEnd;
         const unsigned *order;
+        const unsigned *gatherOrder;
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
@@ -164,6 +166,7 @@ namespace
         c.segStart = b.segStart + r.blockOffset;
         c.segEnd = b.segEnd + r.blockOffset;
         c.order = b.order;
+        c.gatherOrder = b.gatherOrder;
         c.tilebuf = b.tilebuf + (size_t)r.blockOffset * b.subStride * kTileNodes;
         c.tileForce = b.tileForce ? b.tileForce + (size_t)r.blockOffset * b.subStride * kTileNodes : nullptr;
         c.grid = b.grid + (size_t)r.blockOffset * 64;
@@ -1329,7 +1332,7 @@ namespace
             const int nShapes = c.sv.counts[0];
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
-                const int p = (int)c.order[k];
+                const int p = (int)c.gatherOrder[k];
                 const float4 xm = c.P.xm[p];
                 V3 x = v3(xm);
                 V3 vel;
@@ -1628,7 +1631,7 @@ namespace
             const int nShapes = c.sv.counts[0];
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
-                const int p = (int)c.order[k];
+                const int p = (int)c.gatherOrder[k];
                 if (!c.P.mask[p])
                     continue;
                 const float4 xm = c.P.xm[p];
@@ -2253,6 +2256,7 @@ struct CrHipBatch
     int subSize = 64, subMax = 1, subStride = 1;
     // tile-ordered particle permutation of the last enqueued substep
     const unsigned *lastOrder = nullptr;
+    const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
 
     // hipGraph replay of the substep
     bool graphMode = false;
@@ -2422,6 +2426,7 @@ namespace
         a.segStart = b->segStart;
         a.segEnd = b->segEnd;
         a.order = order;
+        a.gatherOrder = b->gatherOrder;
         a.tilebuf = b->tilebuf;
         a.tileForce = b->tileForce;
         a.grid = b->grid;
@@ -2483,6 +2488,9 @@ namespace
         unsigned *order2 = (*orderOut == b->valsA) ? b->valsB : b->valsA;
         launch(b, CRMPM_HIP_PROF_BINNING, k_cell_rank_order, pblock_wgs(b->totalBlocks, b->tileWaves), kWG,
                batch_args(b, nullptr), *orderOut, tmp, order2);
+        // G2P visits each tile in particle index order, the sort's own order:
+        // neighbouring lanes then write neighbouring particles' records.
+        b->gatherOrder = *orderOut;
         *orderOut = order2;
         return check_launch();
     }
@@ -3770,6 +3778,7 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
         {
             const int j = t - t0;
             BatchArgs args = batch_args(b, a->segOrder[j]);
+            args.gatherOrder = args.order;
             args.segStart = a->segStart[j];
             args.segEnd = a->segEnd[j];
             const int subWgs = grid1d(b->totalBlocks * b->subMax, kWaves);
