@@ -1088,10 +1088,20 @@ namespace
             wave_sync();
             const int nSlicers = c.sv.counts[1];
             const GridGeom &g = c.g;
+            // the next chunk's particle index is loaded one chunk ahead, and all
+            // of a particle's state is loaded before the velocity store
+            int pNext = (ti.start + lane < ti.end) ? (int)c.order[ti.start + lane] : 0;
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
-                const int p = (int)c.order[k];
+                const int p = pNext;
+                pNext = (k + 64 < ti.end) ? (int)c.order[k + 64] : 0;
                 const float4 xm = c.P.xm[p];
+                const float4 vv = c.P.v[p];
+                const M3 F = load_cols(c.S.F, p);
+                const M3 Cm = load_cols(c.S.C, p);
+                const float4 prm = c.P.params[p];
+                const int type = c.P.type[p];
+                const float v0 = c.S.V0[p];
                 const V3 x = v3(xm);
                 const float m = xm.w;
                 int bx, by, bz;
@@ -1099,14 +1109,10 @@ namespace
                 cell_of(g, x, bx, by, bz, fx);
                 Weights W;
                 weights_of(fx, W);
-                float4 vv = c.P.v[p];
                 const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
                 c.P.v[p] = make_float4(vel.x, vel.y, vel.z, vv.w);
-                const M3 F = load_cols(c.S.F, p);
-                const M3 Cm = load_cols(c.S.C, p);
-                const float4 prm = c.P.params[p];
-                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], F);
-                const float sc = -c.dt * 4.0f * g.invdx * g.invdx * c.S.V0[p];
+                const M3 stress = stress_kirchhoff(prm.x, prm.y, type, F);
+                const float sc = -c.dt * 4.0f * g.invdx * g.invdx * v0;
                 const M3 affine = stress * sc + Cm * m;
                 const V3 vm = vel * m;
                 const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
@@ -1330,21 +1336,26 @@ namespace
             wave_sync();
             const int nSlicers = c.sv.counts[1];
             const int nShapes = c.sv.counts[0];
+            // the next chunk's particle index is loaded one chunk ahead, and a
+            // particle's x, v and F are loaded together before any store
+            int pNext = (ti.start + lane < ti.end) ? (int)c.gatherOrder[ti.start + lane] : 0;
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
-                const int p = (int)c.gatherOrder[k];
+                const int p = pNext;
+                pNext = (k + 64 < ti.end) ? (int)c.gatherOrder[k + 64] : 0;
                 const float4 xm = c.P.xm[p];
+                const float4 vp = c.P.v[p];
+                const M3 F = load_cols(c.S.F, p);
                 V3 x = v3(xm);
                 V3 vel;
                 M3 Cm;
-                gather_vc(c, ti, G, nSlicers, x, v3(c.P.v[p]), vel, Cm);
+                gather_vc(c, ti, G, nSlicers, x, v3(vp), vel, Cm);
                 x = clamp_bounds(c.g, x + vel * c.dt);
                 float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
                 c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                 c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
                 store_cols(c.S.C, p, Cm);
-                const M3 F = load_cols(c.S.F, p);
                 store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
             }
         }
@@ -1635,6 +1646,7 @@ namespace
                 if (!c.P.mask[p])
                     continue;
                 const float4 xm = c.P.xm[p];
+                const M3 F = load_cols(c.S.F, p); // loaded with x, before the stores
                 int bx, by, bz;
                 V3 fx;
                 cell_of(g, v3(xm), bx, by, bz, fx);
@@ -1658,7 +1670,6 @@ namespace
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
                 c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                 c.P.v[p] = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
-                const M3 F = load_cols(c.S.F, p);
                 store_cols(c.S.F, p, mul(m3_identity() + gv * c.dt, F));
             }
         }
