@@ -88,6 +88,54 @@ def add_shape(scene, gtype, gparams, pos=(0, 0, 0), rot=(0, 0, 0, 1), stype=STAT
     return sid
 
 
+def int_bits(i):
+    """An int32 carried in a float32 geometry parameter (CR_FLOAT_AS_INT,
+    include/math/type_punning.h:52-58)."""
+    return float(np.array([i], np.int32).view(np.float32)[0])
+
+
+def add_sdf_grid(scene, dims, lower, cell, data):
+    """Append one SDF table (GeometrySdfData, include/engine/geometry.h:113-118):
+    dims (x, y, z), lower bound and cell size, and `data` float4 rows (gradient,
+    value).  Returns (offset of the first row in sdf_data, table id)."""
+    off, sid = len(scene["sdf_data"]), len(scene["sdf_dims"])
+    scene["sdf_dims"] = np.vstack([scene["sdf_dims"], np.array([[dims[0], dims[1], dims[2], 0]], np.int32)])
+    scene["sdf_lbcs"] = np.vstack([scene["sdf_lbcs"], np.array([[lower[0], lower[1], lower[2], cell]], np.float32)])
+    scene["sdf_data"] = np.vstack([scene["sdf_data"], np.asarray(data, np.float32).reshape(-1, 4)])
+    return off, sid
+
+
+def _grid_points(dims, lower, cell):
+    # node (i, j, k) at row i + j nx + k nx ny (getGridIndex, include/geometry_utils/sdf.h)
+    k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    return np.stack([i.ravel(), j.ravel(), k.ravel()], 1) * np.float32(cell) + np.asarray(lower, np.float32)
+
+
+def sphere_sdf_grid(radius, cell, n):
+    """Sampled SDF of a sphere as a triangle-mesh SDF table: n^3 nodes centred
+    on the origin, rows (unit gradient, |p| - radius)."""
+    lower = (-(n - 1) / 2 * cell,) * 3
+    pts = _grid_points((n, n, n), lower, cell).astype(np.float64)
+    r = np.linalg.norm(pts, axis=1)
+    g = np.where(r[:, None] > 0, pts / np.maximum(r, 1e-12)[:, None], np.array([0.0, 1.0, 0.0]))
+    return (n, n, n), lower, cell, np.concatenate([g, (r - radius)[:, None]], 1)
+
+
+def planar_slicer_grid(half_x, half_z, cell, spine):
+    """Mesh-slicer table of a planar sheet y = 0 (|x| <= half_x): a spine row
+    (normal, offset) then the grid of signed distances y with gradient
+    (0, sign y, 0), NaN beyond the sheet's x extent (no surface there)."""
+    nx, ny, nz = int(round(2 * (half_x + 2 * cell) / cell)) + 1, 7, int(round(2 * (half_z + 2 * cell) / cell)) + 1
+    lower = (-(nx - 1) / 2 * cell, -(ny - 1) / 2 * cell, -(nz - 1) / 2 * cell)
+    pts = _grid_points((nx, ny, nz), lower, cell).astype(np.float64)
+    val = pts[:, 1].copy()
+    grad = np.zeros((len(pts), 3))
+    grad[:, 1] = np.where(val >= 0, 1.0, -1.0)
+    val[np.abs(pts[:, 0]) > half_x] = np.nan
+    rows = np.concatenate([grad, val[:, None]], 1)
+    return (nx, ny, nz), lower, cell, np.vstack([np.asarray(spine, np.float64)[None, :], rows])
+
+
 SHAPE_F4 = ("pos", "rot", "invScale", "linVel", "angVel", "comInvMass", "inertia0", "inertia1", "params0")
 
 
@@ -245,10 +293,33 @@ def small_scene(kind, seed, n_side=8, shapes="plane", dx=1.0 / 32, jitter=0.25, 
     if "capsule" in shapes:
         add_shape(sc, CAPSULE, (0.01, 0.03, 0, 0), pos=(0.5, 0.2 + side * 0.8, 0.5 - side / 2), stype=DYNAMIC,
                   params0=(0.01, 1.0, 0.1, 0.0), lin_vel=(0, 0, 0.2), inv_scale=(1.0, 0.8, 1.25))
-    if "slicer" in shapes:
+    parts = shapes.split("+")
+    if "slicer" in parts:
         a = math.radians(90.0)
         # vertical quad slicer through the block centre (local y is the cut normal)
         add_shape(sc, QUAD_SLICER, (side / 3, side, 0, 0), pos=(0.5 + 0.3 * spacing, 0.2 + side, 0.5),
+                  rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), stype=KINEMATIC,
+                  params0=(0.0, 1.0, 0.0, 0.001))
+    if "mesh" in parts:
+        # triangle-mesh SDF table (a sampled sphere) pressing on the block, a
+        # polyline under it and an arc around one side at a node row (the
+        # last two act on grid nodes only)
+        off, tid = add_sdf_grid(sc, *sphere_sdf_grid(0.05, 0.01, 17))
+        add_shape(sc, TRIMESH, (int_bits(off), int_bits(tid), 0, 0), pos=(0.5, 0.2 + side + 0.04, 0.5),
+                  stype=KINEMATIC, params0=(0.01, 0.9, 0.5, 0.0), lin_vel=(0.0, -0.3, 0.0))
+        y = 0.197
+        pts = np.array([[0.5 - side, y, 0.5 - side / 4], [0.5, y - 0.002, 0.5], [0.5 + side, y, 0.5 + side / 4]])
+        lo, hi = pts.min(0) - 0.03, pts.max(0) + 0.03
+        off, tid = add_sdf_grid(sc, (0, 0, 0), lo, 0.0, np.vstack([np.append(hi, 0.0), np.c_[pts, np.zeros(3)]]))
+        add_shape(sc, LINES, (int_bits(off), int_bits(tid), int_bits(4), 0), params0=(0.03, 1.0, 0.4, 0.003))
+        r = side / 2 + 0.01
+        add_shape(sc, ARC, (-math.pi / 2, math.pi / 2, 0, 0), pos=(0.5, 0.25, 0.5),
+                  inv_scale=(1 / r, 1 / r, 1 / r), params0=(0.02, 1.0, 0.2, 0.002))
+    if "meshslicer" in parts:
+        # a mesh slicer (planar sheet table with a spine) through the block centre
+        a = math.radians(90.0)
+        off, tid = add_sdf_grid(sc, *planar_slicer_grid(side / 3, side, spacing, (0.0, 0.0, 1.5, -side / 2)))
+        add_shape(sc, TRIMESH_SLICER, (int_bits(off), int_bits(tid), 0, 0), pos=(0.5 + 0.3 * spacing, 0.2 + side, 0.5),
                   rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), stype=KINEMATIC,
                   params0=(0.0, 1.0, 0.0, 0.001))
     sc["steps"] = 10

@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "cressim-mpm_amd", "python"))
 import cpu_checkers as CC  # noqa: E402
 import crmpm_scenes as S  # noqa: E402
 
-SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer"]
+SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer", "plane+mesh", "plane+meshslicer"]
 STEPS = 10
 
 
@@ -40,6 +40,9 @@ def input_digest(sc):
     h.update(repr(sc["shape_list"]).encode())
     h.update(repr(sc["geoms"]).encode())
     h.update(repr(sc["frames"]).encode())
+    if len(sc["sdf_data"]) > 1:  # SDF tables beyond the empty placeholder row
+        for k in ("sdf_dims", "sdf_lbcs", "sdf_data"):
+            h.update(np.ascontiguousarray(sc[k]).tobytes())
     return h.hexdigest()
 
 

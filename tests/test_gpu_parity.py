@@ -20,6 +20,12 @@ pytestmark = pytest.mark.gpu
 
 TOL_100 = 1e-4
 TOL_10 = 2e-5
+# Where the scene amplifies rounding (cut pairs reflecting velocities, PB
+# iterations), the FP32 oracle itself departs from its FP64 replay by more than
+# TOL_10.  Two FP32 evaluations that differ only in summation order may then
+# differ by up to the sum of their errors: the bound for such a field is twice
+# the FP32 oracle's own error against FP64.
+FP32_NOISE_FACTOR = 2.0
 
 
 def rel(a, b):
@@ -96,7 +102,7 @@ def test_initial_volume(kind):
     assert np.array_equal(gs["F"], os_["F"])
 
 
-SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer"]
+SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer", "plane+mesh", "plane+meshslicer"]
 
 
 @pytest.mark.parametrize("shapes", SHAPES)
@@ -105,9 +111,11 @@ def test_parity_10_substeps(kind, shapes):
     sc = S.small_scene(KINDS[kind], 7 + KINDS[kind], shapes=shapes)
     gs = hip_run(sc, 10)
     os_ = cpu_run(sc, 10)
-    errs = report(f"{kind} {shapes} 10 steps", gs, os_)
+    od = cpu_run(sc, 10, fp64=True)
+    errs = report(f"{kind} {shapes} 10 steps", gs, os_, od)
     for k in ("xm", "v", "F"):
-        assert errs[k] <= TOL_10, (k, errs[k])
+        tol = max(TOL_10, FP32_NOISE_FACTOR * rel(os_[k], od[k]))
+        assert errs[k] <= tol, (k, errs[k], tol)
 
 
 def test_parity_mls_cube_100_substeps():

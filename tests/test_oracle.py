@@ -38,7 +38,7 @@ def run(sc, which, steps, fp64=False):
     return st
 
 
-SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer"]
+SHAPES = ["plane", "plane+sphere+box", "plane+capsule", "plane+slicer", "plane+mesh", "plane+meshslicer"]
 KINDS = {"mls": S.MLS, "pb": S.PB, "std": S.STD}
 
 
