@@ -133,6 +133,11 @@ namespace
         const int *segEnd;
         const unsigned *order;
         const unsigned *gatherOrder;
+        // Node blocks that receive tile partials this substep, compacted:
+        // entry (global block << 8 | mask of its 8 source pblocks holding
+        // particles), activeCount entries.
+        const unsigned *activeList;
+        const int *activeCount;
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
@@ -334,7 +339,8 @@ namespace
     // (shapes whose geometry is a slicer).
     __global__ __launch_bounds__(kWG) void k_prepare(int totalBlocks, int blockWGs, float4 *grid, float4 *gridForce,
                                                      unsigned char *dirty, int *segStart, int *segEnd,
-                                                     const SceneShapes *shapes, const SceneRec *scenes)
+                                                     const SceneShapes *shapes, const SceneRec *scenes,
+                                                     int *activeCount)
     {
         const int lane = threadIdx.x & 63;
         if ((int)blockIdx.x >= blockWGs)
@@ -361,6 +367,8 @@ namespace
             return;
         }
         const int b = blockIdx.x * kWaves + (threadIdx.x >> 6);
+        if (b == 0 && lane == 0)
+            *activeCount = 0;
         if (b >= totalBlocks)
             return;
         if (dirty[b])
@@ -1157,37 +1165,84 @@ namespace
     // mlsMpmUpdateGrid / pbMpmUpdateGrid / standardMpmUpdateGrid
     // (src/solver/mls_mpm_algorithm.h:243-294, pb 233-284, standard 217-272).
     // ------------------------------------------------------------------
+    // Node blocks whose 6^3-box sources (the pblocks nb - (dx, dy, dz), d in
+    // {0, 1}^3) hold particles this substep, listed once per substep for the
+    // grid updates that follow (20 per substep in PB-MPM).  One thread per
+    // global block; each wavefront appends its blocks with one atomic, so the
+    // list's order varies between runs while each block's result does not.
+    __global__ __launch_bounds__(256) void k_active_blocks(BatchArgs b, unsigned *list, int *count)
+    {
+        const int nbg = blockIdx.x * blockDim.x + threadIdx.x;
+        const int lane = threadIdx.x & 63;
+        unsigned mask = 0u;
+        if (nbg < b.totalBlocks)
+        {
+            const int s = (b.numScenes == 1) ? 0 : b.blockScene[nbg];
+            ConstSceneRec &r = ((ConstSceneRec *)(size_t)b.scenes)[s];
+            const GridGeom g = r.g;
+            const int off = r.blockOffset;
+            int bx, by, bz;
+            decode_block(g, nbg - off, bx, by, bz);
+#pragma unroll
+            for (int d = 0; d < 8; ++d)
+            {
+                const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
+                if (px >= 0 && py >= 0 && pz >= 0)
+                {
+                    const int key = off + block_key(g, px, py, pz);
+                    if (b.segEnd[key] > b.segStart[key])
+                        mask |= 1u << d;
+                }
+            }
+        }
+        const unsigned long long act = __ballot(mask != 0u);
+        int base = 0;
+        if (lane == 0 && act != 0ull)
+            base = atomicAdd(count, __popcll(act));
+        base = __shfl(base, 0, 64);
+        if (mask != 0u)
+        {
+            const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+            list[base + __popcll(act & below)] = ((unsigned)nbg << 8) | mask;
+        }
+    }
+
     // Multi: pblocks may have more than one sub-tile (subMax > 1); the
     // single-sub-tile variant needs fewer registers.  The kernel is bound by
-    // the latency of its dependent loads (segment counts, then partials), so
-    // it is capped at 80 VGPRs for 6 wavefronts per SIMD.
+    // the latency of its dependent loads (list entry, then partials), so it
+    // is capped at 80 VGPRs for 6 wavefronts per SIMD.  It visits the listed
+    // blocks only; unlisted blocks receive no partial and keep their cleared
+    // state.
     template <int Kind, bool Multi, bool Persistent> // Kind: 0 MLS, 1 PB, 2 standard
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(6))) void k_update(BatchArgs b,
                                                                                             unsigned char *dirty)
     {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg = next_tile<Persistent>(nbg, b.totalBlocks))
+        const int nActive = __builtin_amdgcn_readfirstlane(*b.activeCount);
+        for (int item = first_tile(wave); item < nActive; item = next_tile<Persistent>(item, nActive))
         {
+            const unsigned entry = __builtin_amdgcn_readfirstlane(b.activeList[item]);
+            const int nbg = (int)(entry >> 8);
             int nb;
             const StepCommon c = scene_view(b, nbg, nb);
             const GridGeom &g = c.g;
             int bx, by, bz;
             decode_block(g, nb, bx, by, bz);
-            // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz).
-            // All occupancy loads are issued before any partial is read.
+            // Tiles whose 6^3 box covers this block: pblocks nb - (dx, dy, dz),
+            // those holding particles flagged in the entry.
             int nsub[8];
-            bool any = false;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
             {
-                const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
-                const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
-                const int count = key >= 0 ? c.segEnd[key] - c.segStart[key] : 0;
-                nsub[d] = Multi ? subtiles_of(count, c.subSize, c.subMax) : (count > 0 ? 1 : 0);
-                any = any || nsub[d] > 0;
+                const bool src = (entry >> d) & 1u;
+                if (Multi)
+                {
+                    const int key = src ? block_key(g, bx - (d & 1), by - ((d >> 1) & 1), bz - (d >> 2)) : 0;
+                    nsub[d] = src ? subtiles_of(c.segEnd[key] - c.segStart[key], c.subSize, c.subMax) : 0;
+                }
+                else
+                    nsub[d] = src ? 1 : 0;
             }
-            if (!any)
-                continue;
             const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
             // First sub-tile of every source pblock: all loads issued before any
             // is summed.  Box pointers and sub-tile counts are wave-uniform; only
@@ -2239,6 +2294,8 @@ struct CrHipBatch
     float4 *grid = nullptr, *gridForce = nullptr;
     unsigned char *dirty = nullptr;
     int *segStart = nullptr, *segEnd = nullptr;
+    unsigned *activeList = nullptr; // node blocks with partials this substep
+    int *activeCount = nullptr;
     unsigned *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
     unsigned *radixCounts = nullptr, *digitTotals = nullptr;
     float4 *tilebuf = nullptr, *tileForce = nullptr;
@@ -2438,6 +2495,8 @@ namespace
         a.segEnd = b->segEnd;
         a.order = order;
         a.gatherOrder = b->gatherOrder;
+        a.activeList = b->activeList;
+        a.activeCount = b->activeCount;
         a.tilebuf = b->tilebuf;
         a.tileForce = b->tileForce;
         a.grid = b->grid;
@@ -2494,6 +2553,8 @@ namespace
             return rc;
         launch(b, CRMPM_HIP_PROF_BINNING, k_segments, grid1d(n, 256), 256, n, (int)b->totalBlocks, *keysOut, b->segStart,
                b->segEnd);
+        launch(b, CRMPM_HIP_PROF_BINNING, k_active_blocks, grid1d(b->totalBlocks, 256), 256, batch_args(b, nullptr),
+               b->activeList, b->activeCount);
         // the sort's spare ping-pong buffers take the cell ranks and the tile-ordered permutation
         unsigned *tmp = (*keysOut == b->keysA) ? b->keysB : b->keysA;
         unsigned *order2 = (*orderOut == b->valsA) ? b->valsB : b->valsA;
@@ -2531,7 +2592,8 @@ namespace
     {
         const int blockWGs = grid1d(b->totalBlocks, kWaves);
         launch(b, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + b->numScenes, kWG, (int)b->totalBlocks, blockWGs, b->grid,
-               b->gridForce, b->dirty, b->segStart, b->segEnd, (const SceneShapes *)b->dShapes, (const SceneRec *)b->dScenes);
+               b->gridForce, b->dirty, b->segStart, b->segEnd, (const SceneShapes *)b->dShapes, (const SceneRec *)b->dScenes,
+               b->activeCount);
         const int n = b->totalActive;
         if (n == 0)
             return check_launch();
@@ -2648,6 +2710,8 @@ namespace
         dfree(b->grid);
         dfree(b->gridForce);
         dfree(b->dirty);
+        dfree(b->activeList);
+        dfree(b->activeCount);
         dfree(b->segStart);
         dfree(b->segEnd);
         dfree(b->keysA);
@@ -2716,6 +2780,8 @@ namespace
         if (b->kind == CRMPM_HIP_STANDARD)
             rc |= dalloc(&b->gridForce, nodes);
         rc |= dalloc(&b->dirty, blocks);
+        rc |= dalloc(&b->activeList, blocks);
+        rc |= dalloc(&b->activeCount, 1);
         rc |= dalloc(&b->segStart, blocks);
         rc |= dalloc(&b->segEnd, blocks);
         rc |= dalloc(&b->keysA, N);
@@ -3112,7 +3178,8 @@ int crHipBatchCreate(int kind, int numScenes, const CrHipSceneDesc *scenes, CrHi
         blocks += h.g.numBlocks;
         cap = std::max(cap, (long long)d.particleOffset + d.numMaxParticles);
     }
-    if (blocks >= (1ll << 31) - 1 || cap >= (1ll << 31))
+    // the grid update's block list packs a block index in 24 bits
+    if (blocks >= (1ll << 24) || cap >= (1ll << 31))
     {
         batch_destroy(b);
         return CRMPM_HIP_ERR_CAPACITY;

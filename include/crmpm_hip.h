@@ -197,6 +197,8 @@ typedef struct CrHipSceneDesc
     float boundMax[3];
 } CrHipSceneDesc;
 
+/* CRMPM_HIP_ERR_CAPACITY when the scenes' grids total 2^24 node blocks of
+ * 4x4x4 nodes (about 1.07e9 nodes) or more, or their particles 2^31. */
 int crHipBatchCreate(int kind, int numScenes, const CrHipSceneDesc *scenes, CrHipBatch **out);
 int crHipBatchDestroy(CrHipBatch *b);
 int crHipBatchSetStream(CrHipBatch *b, void *stream);
