@@ -405,6 +405,43 @@ def mls_cutting(steps=500, warmup=10):
             "finite": bool(np.isfinite(st["xm"]).all())}
 
 
+def lattice(reps=5):
+    """Particle-object lattice on the device (ParticleObjectImpl::initialize):
+    a sphere of radius 0.3 m at 2 mm spacing (304^3 lattice points), against
+    the oracle's restatement of the engine's host loop over the same lattice."""
+    import torch
+
+    import crmpm_hip as H
+
+    obj = S.lattice_object(S.SPHERE, (0.3, 0, 0, 0), 0.002, 8e-6, position=(0.5, 0.5, 0.5))
+    n, pts = H.particle_lattice(obj)  # warm-up, sizes the output
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m, _ = H.particle_lattice(obj)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    dims = [int((np.float32(0.6) + np.float32(0.008)) / np.float32(0.002))] * 3
+    points = dims[0] * dims[1] * dims[2]
+    out = {"workload": "sphere r = 0.3 m at 2 mm spacing: lattice points -> particles (float4 position|mass)",
+           "lattice_points": points, "particles": n, "seconds": best, "points_per_s": points / best,
+           "particles_per_s": n / best,
+           # two SDF passes over the lattice, one 16-byte write per particle
+           "hbm_write_GBps": 16.0 * n / best / 1e9}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as CC
+
+    t0 = time.perf_counter()
+    k, _ = CC.particle_lattice(obj, cap=0)  # one pass: count only
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": points / dt, "unit": "lattice points/s", "cores": 1, "kind": "port",
+                           "sample": f"the full {points}-point lattice ({k} particles, count pass only) "
+                                     f"in {dt:.2f} s on 1 core"}
+    out["finite"] = bool(torch.isfinite(pts).all())
+    return out
+
+
 def mls_cube(steps=200, warmup=5):
     """The reference's CPU-runnable case (config1: 32^3 = 32,768 particles, 64^3 cells, 200 substeps)."""
     sc = S.config1(steps=steps + warmup)
@@ -422,7 +459,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls"], default="all",
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
@@ -459,6 +496,9 @@ def main():
 
     if args.workload == "mls":
         print(json.dumps({"mls_cutting": mls_cutting(), "mls_cube": mls_cube()}), flush=True)
+        return
+    if args.workload == "lattice":
+        print(json.dumps({"lattice": lattice()}), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"adjoint": adjoint()}), flush=True)
@@ -558,7 +598,8 @@ def main():
                 out[key] = batched(kind, steps, warm)
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
-        for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube)):
+        for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube),
+                        ("lattice", lattice)):
             try:
                 out[key] = fn()
             except Exception as e:

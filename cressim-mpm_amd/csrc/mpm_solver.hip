@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -3147,6 +3148,140 @@ namespace
     }
 } // namespace
 
+namespace
+{
+    // ------------------------------------------------------------------
+    // Particle lattice of a particle object: ParticleObjectImpl::initialize
+    // (src/engine/particle_object_impl.cpp:36-88).  Lattice point (ix, iy, iz)
+    // of the fattened geometry bounds is q = (ix, iy, iz) spacing invScale +
+    // min in the geometry's frame; it becomes a particle where the geometry's
+    // SDF is <= 0, at rotate(q / invScale) + position.  Particles come out in
+    // the reference's loop order (ix outer, iz inner): a count pass per
+    // 2048-point block, a scan of the block counts, and a write pass that
+    // ranks the inside points of each block.
+    // ------------------------------------------------------------------
+    struct LatticeArgs
+    {
+        ShapeView sv; // SDF tables only
+        float4 p0;    // geometry params0
+        float4 rot;   // quaternion (x, y, z, w)
+        float3 lo, invScale, pos;
+        float spacing, mass;
+        int gtype, ny, nz;
+        long long total; // nx * ny * nz
+    };
+
+    CR_DEV bool lattice_point(const LatticeArgs &a, long long L, V3 &q)
+    {
+        const long long yz = (long long)a.ny * a.nz;
+        const int ix = (int)(L / yz);
+        const int rem = (int)(L - (long long)ix * yz);
+        const int iy = rem / a.nz, iz = rem - iy * a.nz;
+        q = V3{((float)ix * a.spacing) * a.invScale.x + a.lo.x, ((float)iy * a.spacing) * a.invScale.y + a.lo.y,
+               ((float)iz * a.spacing) * a.invScale.z + a.lo.z};
+        V3 g, tangent{0, 0, 0};
+        float d;
+        geometry_sdf(a.sv, q, a.gtype, a.p0, g, d, tangent);
+        return d <= 0.0f;
+    }
+
+    // Each thread takes kLatticePts consecutive lattice points, a workgroup
+    // kLatticeBlock of them.
+    constexpr int kLatticePts = 8;
+    constexpr int kLatticeBlock = kWG * kLatticePts;
+
+    // Exclusive prefix of v over the workgroup's threads (in thread order)
+    // and the workgroup total.
+    CR_DEV unsigned wg_exclusive_scan(unsigned v, unsigned &total)
+    {
+        __shared__ unsigned waveSum[kWaves];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        unsigned inc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const unsigned t = __shfl_up(inc, o, 64);
+            if (lane >= o)
+                inc += t;
+        }
+        if (lane == 63)
+            waveSum[wave] = inc;
+        __syncthreads();
+        unsigned before = 0;
+        total = 0;
+        for (int w = 0; w < kWaves; ++w)
+        {
+            before += (w < wave) ? waveSum[w] : 0u;
+            total += waveSum[w];
+        }
+        __syncthreads(); // waveSum is reused by the next call
+        return before + inc - v;
+    }
+
+    // Inside flags of this thread's points (bit j: point first + j).
+    CR_DEV unsigned lattice_flags(const LatticeArgs &a, long long first)
+    {
+        unsigned f = 0;
+        for (int j = 0; j < kLatticePts; ++j)
+        {
+            V3 q;
+            if (first + j < a.total && lattice_point(a, first + j, q))
+                f |= 1u << j;
+        }
+        return f;
+    }
+
+    __global__ __launch_bounds__(kWG) void k_lattice_count(LatticeArgs a, unsigned *blockCounts)
+    {
+        const long long first = (long long)blockIdx.x * kLatticeBlock + (long long)threadIdx.x * kLatticePts;
+        unsigned total;
+        wg_exclusive_scan((unsigned)__popc(lattice_flags(a, first)), total);
+        if (threadIdx.x == 0)
+            blockCounts[blockIdx.x] = total;
+    }
+
+    // Exclusive scan of n block counts in place (one workgroup); total to counts[n].
+    __global__ __launch_bounds__(kWG) void k_lattice_scan(int n, unsigned *counts)
+    {
+        unsigned carry = 0;
+        for (int base = 0; base < n; base += kWG)
+        {
+            const int i = base + (int)threadIdx.x;
+            const unsigned v = (i < n) ? counts[i] : 0u;
+            unsigned total;
+            const unsigned ex = wg_exclusive_scan(v, total);
+            if (i < n)
+                counts[i] = carry + ex;
+            carry += total;
+        }
+        if (threadIdx.x == 0)
+            counts[n] = carry;
+    }
+
+    __global__ __launch_bounds__(kWG) void k_lattice_write(LatticeArgs a, const unsigned *blockOffsets, float4 *out,
+                                                           unsigned capacity)
+    {
+        const long long first = (long long)blockIdx.x * kLatticeBlock + (long long)threadIdx.x * kLatticePts;
+        const unsigned f = lattice_flags(a, first);
+        unsigned total;
+        unsigned idx = blockOffsets[blockIdx.x] + wg_exclusive_scan((unsigned)__popc(f), total);
+        for (int j = 0; j < kLatticePts; ++j)
+        {
+            if (!((f >> j) & 1u))
+                continue;
+            if (idx < capacity)
+            {
+                V3 q;
+                lattice_point(a, first + j, q);
+                const V3 pl{q.x / a.invScale.x, q.y / a.invScale.y, q.z / a.invScale.z};
+                const V3 pw = qrot(a.rot, pl) + v3(a.pos.x, a.pos.y, a.pos.z);
+                out[idx] = make_float4(pw.x, pw.y, pw.z, a.mass);
+            }
+            ++idx;
+        }
+    }
+} // namespace
+
 extern "C" {
 
 // ---------------------------- batch ----------------------------
@@ -3875,6 +4010,78 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
     dfree(Fb);
     dfree(Cb);
     dfree(xOwn);
+    return rc;
+}
+
+
+int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf, const float boundsMin[3],
+                         const float boundsMax[3], float particleSpacing, const float position[3],
+                         const float rotation[4], const float invScale[3], float particleMass, float *positionMassOut,
+                         unsigned int capacity, unsigned int *numParticles, void *stream)
+{
+    if (!params0 || !boundsMin || !boundsMax || !position || !rotation || !invScale || !numParticles ||
+        (capacity > 0 && !positionMassOut) || geomType < gBox || geomType > gArc || !(particleSpacing > 0.0f))
+        return CRMPM_HIP_ERR_ARG;
+    const bool table = geomType == gTriangleMesh || geomType == gTriangleMeshSlicer || geomType == gLines;
+    if (table && (!sdf || !sdf->dimension || !sdf->lowerBoundCellSize || !sdf->gradientSignedDistance))
+        return CRMPM_HIP_ERR_ARG;
+    // bounds fattened by 2 spacings (Bounds3::fattenFast, include/math/bounds3.h:85-94); lattice
+    // dimensions ((max - min) / spacing / invScale).cast<int>()
+    float lo[3], dimf[3];
+    int dims[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        if (!(invScale[k] > 0.0f) || !std::isfinite(boundsMin[k]) || !std::isfinite(boundsMax[k]))
+            return CRMPM_HIP_ERR_ARG;
+        const float d = 2.0f * particleSpacing;
+        lo[k] = boundsMin[k] - d;
+        const float hi = boundsMax[k] + d;
+        dimf[k] = (hi - lo[k]) / particleSpacing / invScale[k];
+        if (!(dimf[k] < 2147483647.0f))
+            return CRMPM_HIP_ERR_CAPACITY;
+        dims[k] = std::max(0, (int)dimf[k]);
+    }
+    const long long total = (long long)dims[0] * dims[1] * dims[2];
+    if (total >= (1ll << 31))
+        return CRMPM_HIP_ERR_CAPACITY;
+    *numParticles = 0;
+    if (total == 0)
+        return CRMPM_HIP_OK;
+    LatticeArgs a{};
+    if (table)
+    {
+        a.sv.sdfDims = reinterpret_cast<const int4 *>(sdf->dimension);
+        a.sv.sdfLbcs = reinterpret_cast<const float4 *>(sdf->lowerBoundCellSize);
+        a.sv.sdfData = reinterpret_cast<const float4 *>(sdf->gradientSignedDistance);
+    }
+    a.p0 = make_float4(params0[0], params0[1], params0[2], params0[3]);
+    a.rot = make_float4(rotation[0], rotation[1], rotation[2], rotation[3]);
+    a.lo = make_float3(lo[0], lo[1], lo[2]);
+    a.invScale = make_float3(invScale[0], invScale[1], invScale[2]);
+    a.pos = make_float3(position[0], position[1], position[2]);
+    a.spacing = particleSpacing;
+    a.mass = particleMass;
+    a.gtype = geomType;
+    a.ny = dims[1];
+    a.nz = dims[2];
+    a.total = total;
+    const int blocks = (int)((total + kLatticeBlock - 1) / kLatticeBlock);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned *counts = nullptr;
+    if (dalloc(&counts, (size_t)blocks + 1))
+        return CRMPM_HIP_ERR_ALLOC;
+    int rc = CRMPM_HIP_OK;
+    k_lattice_count<<<blocks, kWG, 0, st>>>(a, counts);
+    k_lattice_scan<<<1, kWG, 0, st>>>(blocks, counts);
+    k_lattice_write<<<blocks, kWG, 0, st>>>(a, counts, reinterpret_cast<float4 *>(positionMassOut), capacity);
+    unsigned n = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&n, counts + blocks, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        rc = CRMPM_HIP_ERR_LAUNCH;
+    dfree(counts);
+    if (!rc)
+        *numParticles = n;
     return rc;
 }
 

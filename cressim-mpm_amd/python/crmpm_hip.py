@@ -47,7 +47,11 @@ class SceneDesc(ctypes.Structure):
                 ("boundMin", _f3), ("boundMax", _f3)]
 
 
+_f4 = ctypes.c_float * 4
+
 EXPORTS = {
+    "crHipParticleLattice": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3, _f4, _f3,
+                             ctypes.c_float, _vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), _vp],
     "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
     "crHipSolverDestroy": [_vp],
     "crHipSolverGetGridDims": [_vp, ctypes.POINTER(ctypes.c_int)],
@@ -142,6 +146,29 @@ def load_library(path=LIB_PATH):
 def check(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed with status {rc}")
+
+
+def particle_lattice(obj, device="cuda:0", capacity=None):
+    """ParticleObjectImpl::initialize on the device (crHipParticleLattice).
+    obj: a dict from crmpm_scenes.lattice_object.  Returns (count, (m, 4)
+    float32 tensor of position|mass) with m = min(count, capacity)."""
+    import torch
+
+    lib = load_library()
+    dev = torch.device(device)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    sdf_t = [torch.as_tensor(np.ascontiguousarray(obj[k], dt), device=dev)
+             for k, dt in (("sdf_dims", np.int32), ("sdf_lbcs", np.float32), ("sdf_data", np.float32))]
+    sdf = GeometrySdfData(sdf_t[0].data_ptr(), sdf_t[1].data_ptr(), sdf_t[2].data_ptr())
+    args = (int(obj["type"]), _f4(*obj["params0"]), ctypes.byref(sdf), _f3(*obj["bounds_min"]),
+            _f3(*obj["bounds_max"]), float(obj["spacing"]), _f3(*obj["position"]), _f4(*obj["rotation"]),
+            _f3(*obj["inv_scale"]), float(obj["mass"]))
+    n = ctypes.c_uint(0)
+    check(lib.crHipParticleLattice(*args, None, 0, ctypes.byref(n), _vp(stream)), "particle lattice count")
+    m = n.value if capacity is None else min(n.value, capacity)
+    out = torch.empty((max(m, 1), 4), dtype=torch.float32, device=dev)
+    check(lib.crHipParticleLattice(*args, _vp(out.data_ptr()), m, ctypes.byref(n), _vp(stream)), "particle lattice")
+    return n.value, out[:m]
 
 
 def _read_state(torch, device, stream, F, C, V0, stride, a, e):

@@ -136,6 +136,42 @@ def planar_slicer_grid(half_x, half_z, cell, spine):
     return (nx, ny, nz), lower, cell, np.vstack([np.asarray(spine, np.float64)[None, :], rows])
 
 
+def geometry_bounds(gtype, gparams, sdf=None):
+    """Geometry::getBounds as SimulationFactoryImpl::createGeometry sets it
+    (src/engine/simulation_factory_impl.cpp:521-577): box, sphere, capsule,
+    quad slicer; a triangle-mesh table (sdf = (dims, lower, cell, rows)) spans
+    its nodes (lower bound + (dims - 1) cell)."""
+    p = [float(x) for x in gparams]
+    if gtype == BOX:
+        return (-p[0], -p[1], -p[2]), (p[0], p[1], p[2])
+    if gtype == SPHERE:
+        return (-p[0],) * 3, (p[0],) * 3
+    if gtype == CAPSULE:
+        return (-p[0], -p[0] - p[1], -p[0]), (p[0], p[0] + p[1], p[0])
+    if gtype == QUAD_SLICER:
+        return (-p[0], 0.0, -p[1]), (p[0], 0.0, p[1])
+    if gtype == TRIMESH and sdf is not None:
+        dims, lower, cell = sdf[0], sdf[1], sdf[2]
+        return tuple(lower), tuple(lower[k] + (dims[k] - 1) * cell for k in range(3))
+    raise ValueError("unbounded geometry")
+
+
+def lattice_object(gtype, gparams, spacing, mass, position=(0, 0, 0), rotation=(0, 0, 0, 1), inv_scale=(1, 1, 1),
+                   sdf=None):
+    """A particle object's inputs (ParticleObjectDesc, include/engine/particle_object.h):
+    geometry, spacing, mass and pose.  sdf: a table from sphere_sdf_grid for
+    a triangle-mesh geometry; gparams then carries its offset and id."""
+    o = dict(type=int(gtype), spacing=float(spacing), mass=float(mass), position=tuple(position),
+             rotation=tuple(rotation), inv_scale=tuple(inv_scale), sdf_dims=np.zeros((1, 4), np.int32),
+             sdf_lbcs=np.zeros((1, 4), np.float32), sdf_data=np.zeros((1, 4), np.float32))
+    if sdf is not None:
+        off, tid = add_sdf_grid(o, *sdf)
+        gparams = (int_bits(off), int_bits(tid), 0.0, 0.0)
+    o["params0"] = tuple(float(x) for x in gparams)
+    o["bounds_min"], o["bounds_max"] = geometry_bounds(gtype, gparams, sdf)
+    return o
+
+
 SHAPE_F4 = ("pos", "rot", "invScale", "linVel", "angVel", "comInvMass", "inertia0", "inertia1", "params0")
 
 

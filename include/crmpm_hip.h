@@ -250,6 +250,30 @@ int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask);
 int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories);
 
 /* ------------------------------------------------------------------------
+ * Particle lattice of a particle object on the device:
+ * ParticleObjectImpl::initialize (src/engine/particle_object_impl.cpp:36-88),
+ * which SimulationFactoryImpl::createParticleObject runs on the host
+ * (src/engine/simulation_factory_impl.cpp:776-783).  boundsMin/boundsMax are
+ * the geometry's Geometry::getBounds(); the lattice covers them fattened by
+ * two spacings, with ((max - min) / spacing / invScale) points per axis
+ * (truncated).  Point q = (ix, iy, iz) * spacing * invScale + min becomes a
+ * particle where the geometry's SDF at q is <= 0, at rotation.rotate(q /
+ * invScale) + position with mass particleMass.  Particles are written as
+ * float4 position|mass in the reference's order (ix outer, iz inner), at most
+ * capacity of them, to the device array positionMassOut (e.g. the bound
+ * positionMass array at the object's offset); *numParticles receives the
+ * full count (call with capacity 0 to size an allocation).  Synchronous on
+ * stream.  sdf is required for the table geometries (triangle mesh, mesh
+ * slicer, line segments) and may be NULL otherwise.  Errors:
+ * CRMPM_HIP_ERR_ARG for a missing pointer, a geometry type outside 0..8,
+ * spacing <= 0, invScale <= 0 or unbounded bounds (plane, arc);
+ * CRMPM_HIP_ERR_CAPACITY for 2^31 lattice points or more. */
+int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf, const float boundsMin[3],
+                         const float boundsMax[3], float particleSpacing, const float position[3],
+                         const float rotation[4], const float invScale[3], float particleMass, float *positionMassOut,
+                         unsigned int capacity, unsigned int *numParticles, void *stream);
+
+/* ------------------------------------------------------------------------
  * Differentiable MLS-MPM (one scene, no rigid shapes).  The reference has no
  * adjoint; this is the differentiable form of its MLS substep
  * (src/solver/mls_mpm_algorithm.h:147-390, same arithmetic and quirks).

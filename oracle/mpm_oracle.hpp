@@ -1521,4 +1521,50 @@ namespace orc
             }
         }
     };
+    // ------------------------------------------------------------------
+    // ParticleObjectImpl::initialize (src/engine/particle_object_impl.cpp:36-88):
+    // lattice over the geometry bounds fattened by two spacings
+    // (Bounds3::fattenFast, include/math/bounds3.h:85-94), the points whose
+    // SDF is <= 0 (GeometryImpl::queryPointSdf, src/engine/geometry_impl.cpp:37-45),
+    // mapped to the world, in loop order.  FP32 only, as in the engine.
+    // ------------------------------------------------------------------
+    inline unsigned particle_lattice(const Scene &sc, int type, const float *p0, const float *bmin, const float *bmax,
+                                     float spacing, const float *pos, const float *rot, const float *invScale, float mass,
+                                     std::vector<float> &out)
+    {
+        const float fat = 2 * spacing;
+        float lo[3], hi[3];
+        int dim[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = bmin[k] - fat;
+            hi[k] = bmax[k] + fat;
+            // ((maximum - minimum) / particleSpacing / invScale).cast<int>()
+            dim[k] = static_cast<int>((hi[k] - lo[k]) / spacing / invScale[k]);
+        }
+        unsigned n = 0;
+        for (int ix = 0; ix < dim[0]; ++ix)
+            for (int iy = 0; iy < dim[1]; ++iy)
+                for (int iz = 0; iz < dim[2]; ++iz)
+                {
+                    // Vec3f(ix, iy, iz) * particleSpacing * invScale + minimum
+                    const V3<float> q(float(ix) * spacing * invScale[0] + lo[0], float(iy) * spacing * invScale[1] + lo[1],
+                                      float(iz) * spacing * invScale[2] + lo[2]);
+                    Sdf4<float> sd;
+                    geometry_sdf(sc, q, type, p0, sd, static_cast<V3<float> *>(nullptr));
+                    if (sd.d <= 0)
+                    {
+                        V3<float> w(q.x / invScale[0], q.y / invScale[1], q.z / invScale[2]);
+                        w = qrot(rot, w);
+                        w = V3<float>(w.x + pos[0], w.y + pos[1], w.z + pos[2]);
+                        out.push_back(w.x);
+                        out.push_back(w.y);
+                        out.push_back(w.z);
+                        out.push_back(mass);
+                        ++n;
+                    }
+                }
+        return n;
+    }
+
 } // namespace orc

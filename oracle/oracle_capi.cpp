@@ -255,6 +255,23 @@ extern "C"
         });
     }
 
+    // ParticleObjectImpl::initialize on a geometry (params0, SDF tables) with
+    // bounds [bmin, bmax]; writes up to cap float4 particles, returns the count.
+    unsigned orc_particle_lattice(int type, const float *p0, int numSdf, const int *sdfDims4, const float *sdfLbcs4,
+                                  int sdfRows, const float *sdfData4, const float *bmin, const float *bmax,
+                                  float spacing, const float *pos, const float *rot4, const float *invScale,
+                                  float mass, float *out4, unsigned cap)
+    {
+        orc::Scene sc;
+        sc.sdfDims.assign(sdfDims4, sdfDims4 + 4 * numSdf);
+        sc.sdfLbcs.assign(sdfLbcs4, sdfLbcs4 + 4 * numSdf);
+        sc.sdfData.assign(sdfData4, sdfData4 + 4 * sdfRows);
+        std::vector<float> out;
+        const unsigned n = orc::particle_lattice(sc, type, p0, bmin, bmax, spacing, pos, rot4, invScale, mass, out);
+        std::copy(out.begin(), out.begin() + 4 * std::min(n, cap), out4);
+        return n;
+    }
+
     // svd3 on n column-major 3x3 matrices (include/math/svd3.h:1265-1275)
     void orc_svd3(int fp64, int n, const float *A9, double *U9, double *Vt9, double *S3)
     {

@@ -50,6 +50,8 @@ def _declare(lib, pre):
         "get_grid": (None, [_vp, _d]),
         "base_cells": (None, [_vp, ctypes.c_int, _i]),
         "svd3": (None, [ctypes.c_int, ctypes.c_int, _f, _d, _d, _d]),
+        "particle_lattice": (ctypes.c_uint, [ctypes.c_int, _f, ctypes.c_int, _i, _f, ctypes.c_int, _f, _f, _f,
+                                             ctypes.c_float, _f, _f, _f, ctypes.c_float, _f, ctypes.c_uint]),
         "stress": (None, [ctypes.c_int, _f, _i, _f, _d]),
         "pb_constraint": (None, [ctypes.c_int, ctypes.c_float, _f, _i, _f, _f, _d]),
     }
@@ -207,6 +209,28 @@ class CpuSolver:
             self.close()
         except Exception:
             pass
+
+
+def particle_lattice(obj, cap=None):
+    """ParticleObjectImpl::initialize through the oracle: obj is a dict from
+    crmpm_scenes.lattice_object; returns (count, float4 particles)."""
+    fn = load("oracle")
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)  # noqa: E731
+    dims = np.ascontiguousarray(obj["sdf_dims"], np.int32)
+    lbcs, data = f32(obj["sdf_lbcs"]), f32(obj["sdf_data"])
+    p0, lo, hi = f32(obj["params0"]), f32(obj["bounds_min"]), f32(obj["bounds_max"])
+    pos, rot, inv = f32(obj["position"]), f32(obj["rotation"]), f32(obj["inv_scale"])
+    n = fn["particle_lattice"](int(obj["type"]), P(p0, _f), len(dims), P(dims, _i), P(lbcs, _f), len(data), P(data, _f),
+                               P(lo, _f), P(hi, _f), float(obj["spacing"]), P(pos, _f), P(rot, _f), P(inv, _f),
+                               float(obj["mass"]), None, 0)
+    m = n if cap is None else min(n, cap)
+    if m == 0:
+        return n, np.zeros((0, 4), np.float32)
+    out = np.zeros((m, 4), np.float32)
+    fn["particle_lattice"](int(obj["type"]), P(p0, _f), len(dims), P(dims, _i), P(lbcs, _f), len(data), P(data, _f),
+                           P(lo, _f), P(hi, _f), float(obj["spacing"]), P(pos, _f), P(rot, _f), P(inv, _f),
+                           float(obj["mass"]), P(out, _f), m)
+    return n, out[:m]
 
 
 def svd3(which, A, fp64=False):

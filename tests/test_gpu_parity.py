@@ -214,3 +214,41 @@ def test_cpp_solver_interface():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_reset_subset_mid_run(kind):
+    """ParticleObject::reset semantics (src/engine/particle_object_impl.cpp:131-157):
+    the engine restores x of the object's particles and recomputes their
+    initial data (F = I, V0 from the mass already on the grid, Q10); v and C
+    keep their values (Q19).  GPU and oracle run 12 substeps, reset every
+    third particle, run 8 more."""
+    import crmpm_hip as H
+
+    sc = S.small_scene(KINDS[kind], 40 + KINDS[kind], shapes="plane")
+    idx = np.arange(0, len(sc["xm"]), 3, dtype=np.uint32)
+    x0 = np.asarray(sc["xm"], np.float32)
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    g.step(12)
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    o.step(12)
+    import torch
+
+    g.xm[torch.as_tensor(idx.astype(np.int64), device=g.xm.device)] = torch.as_tensor(x0[idx], device=g.xm.device)
+    g.compute_initial_data(idx)
+    xo = o.state()["xm"].astype(np.float32)
+    xo[idx] = x0[idx]
+    o.fn["set_particles"](o.h, len(xo), CC.P(np.ascontiguousarray(xo), CC._f), None, None, None, None)
+    o.compute_initial_data(idx)
+    g.step(8)
+    o.step(8)
+    gs, os_ = g.state(), o.state()
+    g.close()
+    o.close()
+    errs = report(f"{kind} reset 12+8 steps", gs, os_)
+    # the reset particles restart from F = I
+    assert np.abs(gs["F"][idx] - os_["F"][idx]).max() < 1e-3
+    for k in ("xm", "v", "F", "V0"):
+        assert errs[k] <= TOL_100, (k, errs[k])

@@ -12,6 +12,7 @@ A planted one-ulp change in the oracle's input is caught by (1)/(2).
 """
 import copy
 import glob
+import math
 import os
 import re
 
@@ -292,3 +293,72 @@ def test_config5_generator():
     halves = [f["geoms"][0][1][0] for f in sc["frames"]]
     assert np.all(np.diff(halves) > 0)
     assert abs(halves[-1] - 0.02 * 500 * 1e-4 / 2) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# Particle lattice of a particle object (ParticleObjectImpl::initialize,
+# src/engine/particle_object_impl.cpp:36-88), through the oracle.
+# ---------------------------------------------------------------------------
+def _axis_points(lo_b, hi_b, spacing, inv):
+    """Lattice coordinates of one axis in FP32, as the engine forms them:
+    bounds fattened by 2 spacings, (hi - lo) / spacing / invScale points,
+    point i at i * spacing * invScale + lo."""
+    f = np.float32
+    s, inv = f(spacing), f(inv)
+    lo, hi = f(lo_b) - f(2) * s, f(hi_b) + f(2) * s
+    n = int(((hi - lo) / s) / inv)
+    i = np.arange(n, dtype=np.float32)
+    return (i * s) * inv + lo
+
+
+def test_lattice_box_is_the_product_of_axis_lattices():
+    h = (0.1, 0.05, 0.08)
+    o = S.lattice_object(S.BOX, h + (0.0,), 0.01, 2e-3, position=(0.3, 0.2, 0.1))
+    n, pts = CC.particle_lattice(o)
+    axes = [_axis_points(-h[k], h[k], 0.01, 1.0) for k in range(3)]
+    keep = [a[np.abs(a) <= np.float32(h[k])] for k, a in enumerate(axes)]
+    # loop order: x outer, z inner
+    X, Y, Z = np.meshgrid(*keep, indexing="ij")
+    exp = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    assert n == len(exp) == len(keep[0]) * len(keep[1]) * len(keep[2])
+    np.testing.assert_array_equal(pts[:, :3], (exp + np.asarray([0.3, 0.2, 0.1], np.float32)).astype(np.float32))
+    assert (pts[:, 3] == np.float32(2e-3)).all()
+
+
+def test_lattice_sphere_count_order_and_pose():
+    r, s = 0.1, 0.0125
+    inv = (0.5, 1.0, 2.0)
+    a = math.radians(30.0)
+    ax = np.array([1.0, 2.0, 2.0]) / 3.0
+    quat = tuple(ax * math.sin(a / 2)) + (math.cos(a / 2),)
+    pos = (0.5, 0.4, 0.3)
+    o = S.lattice_object(S.SPHERE, (r, 0, 0, 0), s, 1e-3, position=pos, rotation=quat, inv_scale=inv)
+    n, pts = CC.particle_lattice(o)
+    # inside test in the geometry frame: |q| - r <= 0 in FP32
+    axes = [_axis_points(-r, r, s, inv[k]) for k in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    nrm = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2])
+    q = q[(nrm - np.float32(r)) <= 0]
+    assert n == len(q)
+    # world pose: rotate(q / invScale) + position, against a float64 rotation matrix
+    x, y, z, w = quat
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    world = (q.astype(np.float64) / np.asarray(inv)) @ R.T + np.asarray(pos)
+    np.testing.assert_allclose(pts[:, :3], world, rtol=0, atol=2e-6)
+
+
+def test_lattice_mesh_table_matches_sphere_within_a_cell():
+    # the sampled-sphere SDF table gives the sphere's particles up to the
+    # trilinear interpolation error near the surface
+    s = 0.01
+    o = S.lattice_object(S.TRIMESH, (0, 0, 0, 0), s, 1e-3, sdf=S.sphere_sdf_grid(0.05, 0.01, 17))
+    n, pts = CC.particle_lattice(o)
+    rad = np.linalg.norm(pts[:, :3].astype(np.float64), axis=1)
+    assert n > 0 and rad.max() <= 0.05 + 0.5 * s
+    exact = S.lattice_object(S.SPHERE, (0.05, 0, 0, 0), s, 1e-3, position=(0, 0, 0))
+    exact["bounds_min"], exact["bounds_max"] = o["bounds_min"], o["bounds_max"]
+    m, _ = CC.particle_lattice(exact)
+    assert abs(n - m) <= 0.1 * m
