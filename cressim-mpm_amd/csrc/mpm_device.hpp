@@ -850,22 +850,42 @@ namespace crhip
         return true;
     }
 
-    CR_DEV void add_momentum(float4 *force, float4 *torque, int id, V3 mom, V3 arm)
+    // Momentum a dynamic shape receives from one wavefront's nodes: the lanes'
+    // (momentum, arm) pairs are summed over the wavefront in a fixed shuffle
+    // tree and lane 0 adds the sums with one set of atomics, instead of one
+    // set per node (addMomentumToRigidBody, src/solver/mpm_rigid_contact.h:59-81).
+    // All 64 lanes must call it.
+    CR_DEV void wave_add_momentum(float4 *force, float4 *torque, int id, V3 mom, V3 arm)
     {
         const V3 tq = cross(arm, mom);
-        atomicAdd(&force[id].x, mom.x);
-        atomicAdd(&force[id].y, mom.y);
-        atomicAdd(&force[id].z, mom.z);
-        atomicAdd(&torque[id].x, tq.x);
-        atomicAdd(&torque[id].y, tq.y);
-        atomicAdd(&torque[id].z, tq.z);
+        float v[6] = {mom.x, mom.y, mom.z, tq.x, tq.y, tq.z};
+        const bool any = (v[0] != 0.0f) || (v[1] != 0.0f) || (v[2] != 0.0f) || (v[3] != 0.0f) || (v[4] != 0.0f) ||
+                         (v[5] != 0.0f);
+        if (__ballot(any) == 0ull)
+            return;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1)
+                v[k] += __shfl_xor(v[k], o, 64);
+        if (__lane_id() == 0)
+        {
+            atomicAdd(&force[id].x, v[0]);
+            atomicAdd(&force[id].y, v[1]);
+            atomicAdd(&force[id].z, v[2]);
+            atomicAdd(&torque[id].x, v[3]);
+            atomicAdd(&torque[id].y, v[4]);
+            atomicAdd(&torque[id].z, v[5]);
+        }
     }
 
     // resolveRigidCollision (src/solver/mpm_rigid_contact.h:101-399).
-    // Nodes: vel = (v, mass), pos = predicted node position.
+    // Nodes: vel = (v, mass), pos = predicted node position; every lane of
+    // the wavefront calls it (active false: a node without contact work), so
+    // a dynamic shape's coupling momentum is summed per wavefront.
     // Particles: pushes pos out of shapes; vel is untouched.
     template <bool IsParticle>
-    CR_DEV void resolve_rigid(const ShapeView &sv, int nShapes, float4 &vel, V3 &pos)
+    CR_DEV void resolve_rigid(const ShapeView &sv, int nShapes, float4 &vel, V3 &pos, bool active = true)
     {
         for (int si = 0; si < nShapes; ++si)
         {
@@ -873,107 +893,114 @@ namespace crhip
             const int gtype = r.geomType;
             if (IsParticle && (gtype == gLines || gtype == gArc))
                 continue;
-            const V3 is = v3(r.invScale);
-            const V3 spos = v3(r.pos);
-            const V3 worldRel = pos - spos;
-            const V3 local = qrot_inv(r.rot, worldRel) * is;
-            V3 gradient, tangent{0, 0, 0};
-            float distance;
-            const bool onSpine = geometry_sdf(sv, local, gtype, r.gparams, gradient, distance, tangent);
-            if (isnan(distance))
-                continue;
-            gradient = gradient * is;
-            const float inv = rsq(sqn(gradient));
-            distance = distance * inv;
-            gradient = gradient * inv;
-            distance -= r.params0.w;
-            const float smooth = IsParticle ? 0.0f : r.params0.x;
-            if (!(distance < smooth))
-                continue;
-            const V3 normal = qrot(r.rot, gradient);
-            const V3 rigidVel = (r.type == sStatic) ? V3{0, 0, 0} : (v3(r.linVel) + cross(v3(r.angVel), worldRel));
-            const V3 relVel = v3(vel) - rigidVel;
-            const float vn = dot(relVel, normal);
-            const V3 nComp = normal * vn;
-            if (IsParticle)
-            {
-                const float inside = fminf(distance, 0.0f);
-                const V3 corr = normal * inside;
-                pos.x -= corr.x;
-                pos.y -= corr.y;
-                pos.z -= corr.z;
-                continue;
-            }
-            if (vn >= 0.0f)
-                continue;
-            const float friction = onSpine ? 1e2f : r.params0.z;
-            const float sticky = r.params0.y;
-            V3 response{0, 0, 0};
-            if (sv.effectiveMass && r.type == sDynamic)
-            {
-                const float gridMass = vel.w;
-                const float invMassRigid = r.comInvMass.w;
-                const float rigidMass = invMassRigid > 0.0f ? 1.0f / invMassRigid : 0.0f;
-                if (rigidMass > 0.0f)
+            V3 mom{0, 0, 0}, arm{0, 0, 0}; // this node's coupling momentum for a dynamic shape
+            [&]() {
+                if (!active)
+                    return;
+                const V3 is = v3(r.invScale);
+                const V3 spos = v3(r.pos);
+                const V3 worldRel = pos - spos;
+                const V3 local = qrot_inv(r.rot, worldRel) * is;
+                V3 gradient, tangent{0, 0, 0};
+                float distance;
+                const bool onSpine = geometry_sdf(sv, local, gtype, r.gparams, gradient, distance, tangent);
+                if (isnan(distance))
+                    return;
+                gradient = gradient * is;
+                const float inv = rsq(sqn(gradient));
+                distance = distance * inv;
+                gradient = gradient * inv;
+                distance -= r.params0.w;
+                const float smooth = IsParticle ? 0.0f : r.params0.x;
+                if (!(distance < smooth))
+                    return;
+                const V3 normal = qrot(r.rot, gradient);
+                const V3 rigidVel = (r.type == sStatic) ? V3{0, 0, 0} : (v3(r.linVel) + cross(v3(r.angVel), worldRel));
+                const V3 relVel = v3(vel) - rigidVel;
+                const float vn = dot(relVel, normal);
+                const V3 nComp = normal * vn;
+                if (IsParticle)
                 {
-                    const V3 rr = worldRel - qrot(r.rot, v3(r.comInvMass));
-                    const V3 rxnl = qrot_inv(r.rot, cross(rr, normal));
-                    const float4 i0 = r.inertia0, i1 = r.inertia1;
-                    const V3 wl{i0.x * rxnl.x + i1.x * rxnl.y + i1.y * rxnl.z,
-                                i1.x * rxnl.x + i0.y * rxnl.y + i1.z * rxnl.z,
-                                i1.y * rxnl.x + i1.z * rxnl.y + i0.z * rxnl.z};
-                    const V3 t = cross(qrot(r.rot, wl), rr);
-                    const float invMassRigidN = invMassRigid + dot(normal, t);
-                    const float invMassGrid = 1.0f / gridMass;
-                    const float invMassComb = invMassGrid + invMassRigidN;
-                    if (invMassComb > 0.0f)
+                    const float inside = fminf(distance, 0.0f);
+                    const V3 corr = normal * inside;
+                    pos.x -= corr.x;
+                    pos.y -= corr.y;
+                    pos.z -= corr.z;
+                    return;
+                }
+                if (vn >= 0.0f)
+                    return;
+                const float friction = onSpine ? 1e2f : r.params0.z;
+                const float sticky = r.params0.y;
+                V3 response{0, 0, 0};
+                if (sv.effectiveMass && r.type == sDynamic)
+                {
+                    const float gridMass = vel.w;
+                    const float invMassRigid = r.comInvMass.w;
+                    const float rigidMass = invMassRigid > 0.0f ? 1.0f / invMassRigid : 0.0f;
+                    if (rigidMass > 0.0f)
                     {
-                        const float Jn = -1.0f * vn / invMassComb;
-                        const V3 impN = normal * Jn;
-                        const V3 vto = relVel - (normal * vn);
-                        const float vtn = norm(vto);
-                        V3 impT{0, 0, 0};
-                        if (vtn > kFltEps)
+                        const V3 rr = worldRel - qrot(r.rot, v3(r.comInvMass));
+                        const V3 rxnl = qrot_inv(r.rot, cross(rr, normal));
+                        const float4 i0 = r.inertia0, i1 = r.inertia1;
+                        const V3 wl{i0.x * rxnl.x + i1.x * rxnl.y + i1.y * rxnl.z,
+                                    i1.x * rxnl.x + i0.y * rxnl.y + i1.z * rxnl.z,
+                                    i1.y * rxnl.x + i1.z * rxnl.y + i0.z * rxnl.z};
+                        const V3 t = cross(qrot(r.rot, wl), rr);
+                        const float invMassRigidN = invMassRigid + dot(normal, t);
+                        const float invMassGrid = 1.0f / gridMass;
+                        const float invMassComb = invMassGrid + invMassRigidN;
+                        if (invMassComb > 0.0f)
                         {
-                            const float stopJt = vtn / invMassComb;
-                            const float maxJt = friction * Jn;
-                            const float fJt = fminf(maxJt, stopJt);
-                            const float left = vtn - (fJt * invMassComb);
-                            const float fin = left * sticky;
-                            const float totJt = (vtn - fin) / invMassComb;
-                            impT = (vto * -1.0f / vtn) * totJt;
+                            const float Jn = -1.0f * vn / invMassComb;
+                            const V3 impN = normal * Jn;
+                            const V3 vto = relVel - (normal * vn);
+                            const float vtn = norm(vto);
+                            V3 impT{0, 0, 0};
+                            if (vtn > kFltEps)
+                            {
+                                const float stopJt = vtn / invMassComb;
+                                const float maxJt = friction * Jn;
+                                const float fJt = fminf(maxJt, stopJt);
+                                const float left = vtn - (fJt * invMassComb);
+                                const float fin = left * sticky;
+                                const float totJt = (vtn - fin) / invMassComb;
+                                impT = (vto * -1.0f / vtn) * totJt;
+                            }
+                            const V3 totImp = impN + impT;
+                            response = v3(vel) + (totImp * invMassGrid);
+                            mom = totImp * -1.0f;
+                            arm = rr;
                         }
-                        const V3 totImp = impN + impT;
-                        response = v3(vel) + (totImp * invMassGrid);
-                        add_momentum(sv.force, sv.torque, r.shapeId, totImp * -1.0f, rr);
                     }
                 }
-            }
-            else
-            {
-                V3 tang = relVel - nComp;
-                if (gtype == gLines || gtype == gArc)
+                else
                 {
-                    tangent = tangent * is;
-                    normalize(tangent);
-                    tangent = qrot(r.rot, tangent);
-                    tang = tangent * dot(tang, tangent);
+                    V3 tang = relVel - nComp;
+                    if (gtype == gLines || gtype == gArc)
+                    {
+                        tangent = tangent * is;
+                        normalize(tangent);
+                        tangent = qrot(r.rot, tangent);
+                        tang = tangent * dot(tang, tangent);
+                    }
+                    const float tn = norm(tang);
+                    const float fc = fmaxf(friction * vn / (tn + kFltEps), -1.0f);
+                    response = tang + tang * fc;
+                    response = response * sticky;
+                    response = response + rigidVel;
+                    if (!sv.effectiveMass && r.type == sDynamic)
+                    {
+                        mom = (v3(vel) - response) * vel.w;
+                        arm = pos - spos;
+                    }
                 }
-                const float tn = norm(tang);
-                const float fc = fmaxf(friction * vn / (tn + kFltEps), -1.0f);
-                response = tang + tang * fc;
-                response = response * sticky;
-                response = response + rigidVel;
-                if (!sv.effectiveMass && r.type == sDynamic)
-                {
-                    V3 mom = v3(vel) - response;
-                    mom = mom * vel.w;
-                    add_momentum(sv.force, sv.torque, r.shapeId, mom, pos - spos);
-                }
-            }
-            vel.x = response.x;
-            vel.y = response.y;
-            vel.z = response.z;
+                vel.x = response.x;
+                vel.y = response.y;
+                vel.z = response.z;
+            }();
+            if (!IsParticle && r.type == sDynamic)
+                wave_add_momentum(sv.force, sv.torque, r.shapeId, mom, arm);
         }
     }
 } // namespace crhip

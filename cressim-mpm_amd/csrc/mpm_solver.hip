@@ -1303,11 +1303,11 @@ namespace
             if (lane == 0)
                 dirty[nbg] = 1;
             const int i = bx * 4 + lx, j = by * 4 + ly, k = bz * 4 + lz;
-            if (!in_grid(g, i, j, k))
-                continue;
-            const int gi = nb * 64 + lane;
+            const bool inGrid = in_grid(g, i, j, k);
             const float cutoff = (Kind == 1) ? 1e-8f : 1e-4f;
-            if (!(s.w < cutoff))
+            const bool heavy = inGrid && !(s.w < cutoff);
+            V3 np{0, 0, 0};
+            if (heavy)
             {
                 if (Kind == 2)
                 {
@@ -1318,9 +1318,13 @@ namespace
                 s.x /= s.w;
                 s.y /= s.w;
                 s.z /= s.w;
-                V3 np = V3{(float)i * g.dx, (float)j * g.dx, (float)k * g.dx} + v3(g.bmin.x, g.bmin.y, g.bmin.z);
+                np = V3{(float)i * g.dx, (float)j * g.dx, (float)k * g.dx} + v3(g.bmin.x, g.bmin.y, g.bmin.z);
                 np = np + v3(s) * c.dt;
-                resolve_rigid<false>(c.sv, c.sv.counts[0], s, np);
+            }
+            // all lanes take part (coupling momentum is summed per wavefront); light nodes are left as they are
+            resolve_rigid<false>(c.sv, c.sv.counts[0], s, np, heavy);
+            if (heavy)
+            {
                 if (i < 2 || i > g.nx - 3)
                     s.x = 0.0f;
                 if (j < 2 || j > g.ny - 3)
@@ -1328,7 +1332,8 @@ namespace
                 if (k < 2 || k > g.nz - 3)
                     s.z = 0.0f;
             }
-            c.grid[gi] = s;
+            if (inGrid)
+                c.grid[nb * 64 + lane] = s;
         }
     }
 

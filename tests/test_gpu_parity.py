@@ -252,3 +252,40 @@ def test_reset_subset_mid_run(kind):
     assert np.abs(gs["F"][idx] - os_["F"][idx]).max() < 1e-3
     for k in ("xm", "v", "F", "V0"):
         assert errs[k] <= TOL_100, (k, errs[k])
+
+
+def _coupled_scene(kind, contact_model):
+    sc = S.small_scene(KINDS[kind], 77 + KINDS[kind], shapes="plane+capsule")
+    sc["contact_model"] = contact_model
+    sc["shape_list"][1]["comInvMass"] = (0.0, 0.01, 0.0, 2.0)
+    sc["shape_list"][1]["inertia0"] = (50.0, 40.0, 50.0, 0.0)
+    sc["shape_list"][1]["inertia1"] = (1.0, 0.5, 0.2, 0.0)
+    return sc
+
+
+@pytest.mark.parametrize("contact_model", [0, 1])
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_dynamic_shape_coupling_force(kind, contact_model):
+    """Force and torque a dynamic shape receives from the grid
+    (addMomentumToRigidBody, src/solver/mpm_rigid_contact.h:219-348,378-389),
+    accumulated over 10 substeps, against the oracle (which sums in FP64)."""
+    import crmpm_hip as H
+
+    sc = _coupled_scene(kind, contact_model)
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    g.step(10)
+    gf, gt = g.shape_force()
+    gs = g.state()
+    g.close()
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    o.step(10)
+    of, ot = o.shape_force()
+    os_ = o.state()
+    o.close()
+    assert np.abs(of[1, :3]).max() > 0, "the capsule receives no force"
+    ef, et = rel(gf[1, :3], of[1, :3]), rel(gt[1, :3], ot[1, :3])
+    print(f"{kind} contact model {contact_model}: force rel {ef:.2e} torque rel {et:.2e}", flush=True)
+    assert ef <= 1e-4 and et <= 1e-4, (ef, et)
+    assert rel(gs["v"], os_["v"]) <= TOL_100
