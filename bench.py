@@ -39,6 +39,9 @@ BYTES_PER_TOUCHED_NODE_FUSED = 32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
 # the fused iteration kernel (template Mode 1; either launch form)
 FUSED_KERNEL = "k_pb_tile<1,"
+# what the library's autotuner picked: launch sizes, the P2G sub-tile split
+# and the G2P (gather) split
+TUNED_KEYS = ("tile_waves", "update_waves", "sub_size", "sub_max", "gather_sub_size", "gather_sub_max")
 PROFILE_EVERY = 10
 COPY_CAL_BYTES = 256 << 20
 
@@ -233,7 +236,7 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
     b.step(warmup)
     tuned = None
     if tune:
-        tuned = b.autotune(steps_per_trial=2)
+        tuned = b.autotune(steps_per_trial=2) + b.gather_split()
     b.set_graph_mode(graph)
     b.step(1)  # capture
     torch.cuda.synchronize()
@@ -259,7 +262,7 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
             "value": n_total * steps / t_batch, "unit": "particle-substeps/s",
             "batch_ms_per_substep": t_batch / steps * 1e3, "sum_of_plain_ms_per_substep": t_plain / steps * 1e3,
             "speedup_vs_sum_of_plain": t_plain / t_batch,
-            "autotuned": None if tuned is None else dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned)),
+            "autotuned": None if tuned is None else dict(zip(TUNED_KEYS, tuned)),
             "hip_graph": graph, "finite": bool(finite)}
 
 
@@ -331,7 +334,7 @@ def adjoint(steps=100, every=10):
             "checkpointed_forward_ms_per_substep": t_f / steps * 1e3,
             "forward_adjoint_ms_per_substep": t_fa / steps * 1e3,
             "forward_adjoint_vs_plain_forward": t_fa / t_p, "loss": loss,
-            "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), sim_tuned)),
+            "autotuned": dict(zip(TUNED_KEYS, sim_tuned)),
             "grad_logE_abs_max": float(g.abs().max()), "finite": finite}
 
 
@@ -357,7 +360,7 @@ def mls_run(scene, steps, warmup, tune=True):
             solver.step(1)
 
     run(0, warmup)
-    tuned = solver.autotune(steps_per_trial=2) if tune else None
+    tuned = solver.autotune(steps_per_trial=2) + solver.gather_split() if tune else None
     torch.cuda.synchronize()
     grid = solver.grid()
     touched = int(np.count_nonzero(grid[:, 3] > 0))
@@ -401,7 +404,7 @@ def mls_cutting(steps=500, warmup=10):
                         "slicer cutting at 0.02 m/s", "particles": n, "substeps": steps,
             "value": n * steps / elapsed, "unit": "particle-substeps/s", "ms_per_substep": elapsed / steps * 1e3,
             "touched_nodes": touched, "roofline": mls_kernel_roofline(n, touched, prof, slicer=True),
-            "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned)),
+            "autotuned": dict(zip(TUNED_KEYS, tuned)),
             "finite": bool(np.isfinite(st["xm"]).all())}
 
 
@@ -526,7 +529,7 @@ def main():
 
     run(0, W)
     # the library's autotuner picks the tile split and launch sizes on this state (which it leaves unchanged)
-    tuned = solver.autotune(steps_per_trial=2)
+    tuned = solver.autotune(steps_per_trial=2) + solver.gather_split()
     torch.cuda.synchronize()
     # touched nodes of one substep (grid nodes holding mass), for the grid term of the roofline
     grid = solver.grid()
@@ -574,7 +577,7 @@ def main():
         "config": {"workload": "pb_mpm_neohookean_slab_sphere_indenter", "particles": n,
                    "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
                    "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu",
-                   "autotuned": dict(zip(("tile_waves", "update_waves", "sub_size", "sub_max"), tuned))},
+                   "autotuned": dict(zip(TUNED_KEYS, tuned))},
         "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1, ...> (fused G2P+P2G)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,

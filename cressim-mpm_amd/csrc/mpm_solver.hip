@@ -147,6 +147,9 @@ namespace
         // rest), each run by its own wavefront with its own partial box at
         // slot pblock * subStride + j.
         int subSize, subMax, subStride;
+        // Split of the gather kernels (G2P), which write no partials: a
+        // pblock's particles in sub-tiles of gSubSize, at most gSubMax.
+        int gSubSize, gSubMax;
     };
 
     // Scene records are never written by a kernel: reading them through the
@@ -200,12 +203,13 @@ namespace
     // Whether sub-tile tg (global pblock tg / subMax, index tg % subMax) has
     // particles, from the global segment arrays alone (an empty sub-tile
     // skips the scene record loads).
-    CR_DEV bool subtile_used(const BatchArgs &b, int tg)
+    CR_DEV bool split_used(const BatchArgs &b, int tg, int subSize, int subMax)
     {
-        const int pb = tg / b.subMax, j = tg - pb * b.subMax;
+        const int pb = tg / subMax, j = tg - pb * subMax;
         const int count = b.segEnd[pb] - b.segStart[pb];
-        return count > 0 && (j == 0 || count > j * b.subSize);
+        return count > 0 && (j == 0 || count > j * subSize);
     }
+    CR_DEV bool subtile_used(const BatchArgs &b, int tg) { return split_used(b, tg, b.subSize, b.subMax); }
     // Next pblock of a tile loop; a non-persistent launch (one wavefront per
     // pblock) compiles to straight-line code, which needs fewer registers.
     template <bool Persistent>
@@ -621,17 +625,21 @@ namespace
 
     // Sub-tile j of scene-local pblock t: false when the pblock is empty or
     // has fewer sub-tiles.
-    CR_DEV bool subtile_info(const StepCommon &c, int t, int j, TileInfo &ti)
+    CR_DEV bool split_info(const StepCommon &c, int t, int j, int subSize, int subMax, TileInfo &ti)
     {
         const int start = c.segStart[t], end = c.segEnd[t];
-        const int ns = subtiles_of(end - start, c.subSize, c.subMax);
+        const int ns = subtiles_of(end - start, subSize, subMax);
         if (j >= ns)
             return false;
-        ti.start = start + j * c.subSize;
-        ti.end = (j == ns - 1) ? end : ti.start + c.subSize;
+        ti.start = start + j * subSize;
+        ti.end = (j == ns - 1) ? end : ti.start + subSize;
         decode_block(c.g, t, ti.bx, ti.by, ti.bz);
         ti.slot = t * c.subStride + j;
         return true;
+    }
+    CR_DEV bool subtile_info(const StepCommon &c, int t, int j, TileInfo &ti)
+    {
+        return split_info(c, t, j, c.subSize, c.subMax, ti);
     }
 
     // Stage the grid nodes of a tile in LDS (xyz velocity, w mass).
@@ -1382,14 +1390,15 @@ namespace
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
+        const int items = b.totalBlocks * b.gSubMax;
+        for (int tg = first_tile(wave); tg < items; tg = next_tile<Persistent>(tg, items))
         {
-            if (!subtile_used(b, tg))
+            if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
             int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
+            const StepCommon c = scene_view(b, tg / b.gSubMax, t);
             TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
+            if (!split_info(c, t, tg % b.gSubMax, b.gSubSize, b.gSubMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *G = lg[wave];
@@ -1686,14 +1695,15 @@ namespace
     {
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
+        const int items = b.totalBlocks * b.gSubMax;
+        for (int tg = first_tile(wave); tg < items; tg = next_tile<Persistent>(tg, items))
         {
-            if (!subtile_used(b, tg))
+            if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
             int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
+            const StepCommon c = scene_view(b, tg / b.gSubMax, t);
             TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
+            if (!split_info(c, t, tg % b.gSubMax, b.gSubSize, b.gSubMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *G = lg[wave];
@@ -2328,6 +2338,7 @@ struct CrHipBatch
     // sub-tile split: particles per sub-tile, sub-tiles per pblock (<= subStride,
     // the allocated partial boxes per pblock)
     int subSize = 64, subMax = 1, subStride = 1;
+    int gSubSize = 1 << 20, gSubMax = 1; // gather kernels' split
     // tile-ordered particle permutation of the last enqueued substep
     const unsigned *lastOrder = nullptr;
     const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
@@ -2336,7 +2347,7 @@ struct CrHipBatch
     bool graphMode = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    long long graphKey[7] = {-1, -1, -1, -1, -1, -1, -1};
+    long long graphKey[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
     long long bindEpoch = 0;
 
     // optional per-launch event timing (ProfileBegin/End)
@@ -2509,6 +2520,8 @@ namespace
         a.subSize = b->subSize;
         a.subMax = b->subMax;
         a.subStride = b->subStride;
+        a.gSubSize = b->gSubSize;
+        a.gSubMax = b->gSubMax;
         return a;
     }
 
@@ -2611,6 +2624,7 @@ namespace
         b->lastOrder = order;
         const BatchArgs a = batch_args(b, order);
         const int tileWGs = pblock_wgs(b->totalBlocks * b->subMax, b->tileWaves);
+        const int gatherWGs = pblock_wgs(b->totalBlocks * b->gSubMax, b->tileWaves);
         const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
         const bool multi = b->subMax > 1, persU = b->updateWaves > 0;
@@ -2619,13 +2633,13 @@ namespace
         {
             launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, a, b->dirty);
-            launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, tileWGs, kWG, a);
+            launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, gatherWGs, kWG, a);
         }
         else if (b->kind == CRMPM_HIP_STANDARD)
         {
             launch(b, CRMPM_HIP_PROF_STD_P2G, pers ? k_std_p2g<true> : k_std_p2g<false>, tileWGs, kWG, a);
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<2>(multi, persU), updWGs, kWG, a, b->dirty);
-            launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, tileWGs, kWG, a);
+            launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, gatherWGs, kWG, a);
         }
         else
         {
@@ -2678,9 +2692,9 @@ namespace
         const bool useGraph = b->graphMode && !b->profEvents && b->stream != nullptr;
         if (!useGraph)
             return enqueue_step(b);
-        const long long key[7] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
-                                  b->subSize, b->subMax};
-        if (!b->graphExec || !std::equal(key, key + 7, b->graphKey))
+        const long long key[9] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
+                                  b->subSize,       b->subMax,     b->gSubSize,  b->gSubMax};
+        if (!b->graphExec || !std::equal(key, key + 9, b->graphKey))
         {
             drop_graph(b);
             if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -2700,7 +2714,7 @@ namespace
                 drop_graph(b);
                 return CRMPM_HIP_ERR_LAUNCH;
             }
-            std::copy(key, key + 7, b->graphKey);
+            std::copy(key, key + 9, b->graphKey);
         }
         return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
@@ -3033,12 +3047,14 @@ namespace
     // Time `steps` substeps from the saved state with the given launch sizes
     // and sub-tile split.
     int tune_trial(CrHipBatch *b, char *backup, int steps, int tileWaves, int updateWaves, int subSize, int subMax,
-                   float *ms)
+                   int gSubSize, int gSubMax, float *ms)
     {
         b->tileWaves = tileWaves;
         b->updateWaves = updateWaves;
         b->subSize = subSize;
         b->subMax = subMax;
+        b->gSubSize = gSubSize;
+        b->gSubMax = gSubMax;
         int rc = tune_copy(b, backup, false);
         if (rc)
             return rc;
@@ -3095,7 +3111,7 @@ namespace
             float ms = 0.0f;
             if (rc)
                 break;
-            rc = tune_trial(b, backup, steps, 0, 0, sp[0], sp[1], &ms);
+            rc = tune_trial(b, backup, steps, 0, 0, sp[0], sp[1], 1 << 20, 1, &ms);
             if (!rc && ms < best)
             {
                 best = ms;
@@ -3103,7 +3119,25 @@ namespace
                 bestM = sp[1];
             }
         }
-        // 2. launch sizes: one wavefront per item, or a persistent launch of
+        // 2. gather (G2P) split: whole pblocks, or sub-tiles of 256 or 128
+        //    particles (no partials to sum, only more wavefronts)
+        const int gsplits[3][2] = {{1 << 20, 1}, {256, 4}, {128, 8}};
+        int bestGS = 1 << 20, bestGM = 1;
+        best = 3.4e38f;
+        for (const auto &gs : gsplits)
+        {
+            float ms = 0.0f;
+            if (rc)
+                break;
+            rc = tune_trial(b, backup, steps, 0, 0, bestS, bestM, gs[0], gs[1], &ms);
+            if (!rc && ms < best)
+            {
+                best = ms;
+                bestGS = gs[0];
+                bestGM = gs[1];
+            }
+        }
+        // 3. launch sizes: one wavefront per item, or a persistent launch of
         //    2, 4, 8 or 16 thousand wavefronts looping over the items
         const int cand[] = {0, 2048, 4096, 8192, 16384};
         int bestT = 0, bestU = 0;
@@ -3113,7 +3147,7 @@ namespace
             float ms = 0.0f;
             if (rc || (c > 0 && c >= b->totalBlocks * bestM))
                 continue;
-            rc = tune_trial(b, backup, steps, c, 0, bestS, bestM, &ms);
+            rc = tune_trial(b, backup, steps, c, 0, bestS, bestM, bestGS, bestGM, &ms);
             if (!rc && ms < best)
             {
                 best = ms;
@@ -3126,7 +3160,7 @@ namespace
             float ms = 0.0f;
             if (rc || (c > 0 && c >= b->totalBlocks))
                 continue;
-            rc = tune_trial(b, backup, steps, bestT, c, bestS, bestM, &ms);
+            rc = tune_trial(b, backup, steps, bestT, c, bestS, bestM, bestGS, bestGM, &ms);
             if (!rc && ms < best)
             {
                 best = ms;
@@ -3141,6 +3175,8 @@ namespace
         b->updateWaves = bestU;
         b->subSize = bestS;
         b->subMax = bestM;
+        b->gSubSize = bestGS;
+        b->gSubMax = bestGM;
         if (tileWavesOut)
             *tileWavesOut = bestT;
         if (updateWavesOut)
@@ -3492,6 +3528,24 @@ int crHipBatchSetTileSplit(CrHipBatch *b, int subSize, int subMax)
     return CRMPM_HIP_OK;
 }
 
+int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax)
+{
+    if (!b || subSize < 64 || subSize % 64 != 0 || subMax < 1 || subMax > 64)
+        return CRMPM_HIP_ERR_ARG;
+    b->gSubSize = subSize;
+    b->gSubMax = subMax;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax)
+{
+    if (!b || !subSize || !subMax)
+        return CRMPM_HIP_ERR_ARG;
+    *subSize = b->gSubSize;
+    *subMax = b->gSubMax;
+    return CRMPM_HIP_OK;
+}
+
 int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves)
 {
     if (!b || tileWaves < 0 || updateWaves < 0)
@@ -3705,6 +3759,11 @@ int crHipSolverStep(CrHipSolver *s, float *dtOut)
 int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax)
 {
     return s ? batch_autotune(s->b, stepsPerTrial, tileWaves, updateWaves, subSize, subMax) : CRMPM_HIP_ERR_ARG;
+}
+
+int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax)
+{
+    return s ? crHipBatchGetGatherSplit(s->b, subSize, subMax) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int *orderOut)

@@ -104,6 +104,9 @@ EXPORTS = {
                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipBatchSetTileSplit": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchSetLaunchSizes": [_vp, ctypes.c_int, ctypes.c_int],
+    "crHipBatchSetGatherSplit": [_vp, ctypes.c_int, ctypes.c_int],
+    "crHipBatchGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipBatchFetchResults": [_vp],
     "crHipBatchGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  ctypes.POINTER(ctypes.c_int)],
@@ -353,6 +356,12 @@ class HipSolver:
                                            ctypes.byref(sm)), "autotune")
         return t.value, u.value, ss.value, sm.value
 
+    def gather_split(self):
+        """(sub-tile size, sub-tiles per pblock) of the G2P kernels."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipSolverGetGatherSplit(self.h, ctypes.byref(a), ctypes.byref(b)), "gather split")
+        return a.value, b.value
+
     def profile_mask(self, categories):
         """Record events only for launches of the named categories (empty: pause)."""
         mask = 0
@@ -501,6 +510,14 @@ class HipBatch:
 
     def set_tile_split(self, sub_size, sub_max):
         check(self.lib.crHipBatchSetTileSplit(self.h, sub_size, sub_max), "tile split")
+
+    def set_gather_split(self, sub_size, sub_max):
+        check(self.lib.crHipBatchSetGatherSplit(self.h, sub_size, sub_max), "gather split")
+
+    def gather_split(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipBatchGetGatherSplit(self.h, ctypes.byref(a), ctypes.byref(b)), "gather split")
+        return a.value, b.value
 
     def set_launch_sizes(self, tile_waves, update_waves):
         check(self.lib.crHipBatchSetLaunchSizes(self.h, tile_waves, update_waves), "launch sizes")

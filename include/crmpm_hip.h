@@ -148,6 +148,9 @@ int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int 
 /* crHipBatchAutotune for the solver's one-scene batch (no counterpart in the
  * reference): picks the tile split and launch sizes on the current state. */
 int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
+/* crHipBatchGetGatherSplit for the solver's one-scene batch (the split the
+ * autotuner picked for the G2P kernels). */
+int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax);
 /* Launch timing, an addition of this solver (the reference times nothing on
  * the device).  Between Begin and End every particle-transfer, grid-update and
  * binning launch of crHipSolverStep is bracketed by a pair of hipEvents on the
@@ -237,11 +240,17 @@ int crHipBatchSetGraphMode(CrHipBatch *b, int enabled);
 int crHipBatchSetTileSplit(CrHipBatch *b, int subSize, int subMax);
 /* Autotuner: times substeps of the batch's current state (saved and restored
  * around the trials, so the state is unchanged afterwards) for several tile
- * splits and launch sizes and keeps the fastest.  Launch sizes are in
+ * splits, gather splits and launch sizes and keeps the fastest.  Launch sizes are in
  * wavefronts; 0 means one wavefront per (sub-)tile or node block.  Any
  * output pointer may be null. */
 int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
 int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
+/* Split of the gather kernels (G2P), which write no partial boxes: a
+ * pblock's particles in sub-tiles of subSize (a multiple of 64), at most
+ * subMax (1..64) of them, the last taking the rest.  Default whole pblocks;
+ * crHipBatchAutotune also picks it.  No counterpart in the reference. */
+int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax);
+int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax);
 int crHipBatchFetchResults(CrHipBatch *b);
 int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, int *stride);
 int crHipBatchCopyGrid(CrHipBatch *b, int scene, float *dst);

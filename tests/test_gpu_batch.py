@@ -126,3 +126,25 @@ def test_autotune_keeps_state_and_results():
             assert rel(x[k], y[k]) <= 1e-6, (i, k)
     b.close()
     ref.close()
+
+
+@pytest.mark.parametrize("kind", ["mls", "std"])
+def test_gather_split_is_bitwise_neutral(kind):
+    """The G2P kernels' split only regroups particles into wavefronts: each
+    particle's gather is the same arithmetic, so results are bitwise equal."""
+    import crmpm_hip as H
+
+    scenes = mixed_scenes(KINDS[kind])
+    runs = []
+    for split in ((1 << 20, 1), (64, 8), (128, 2)):
+        b = H.HipBatch(copy.deepcopy(scenes))
+        b.set_gather_split(*split)
+        assert b.gather_split() == split
+        b.compute_initial_data()
+        b.step(6)
+        runs.append([b.state(i) for i in range(len(scenes))])
+        b.close()
+    for r in runs[1:]:
+        for i in range(len(scenes)):
+            for k in FIELDS:
+                assert np.array_equal(r[i][k], runs[0][i][k]), (i, k)
