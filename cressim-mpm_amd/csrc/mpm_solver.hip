@@ -1126,8 +1126,11 @@ namespace
                 cell_of(g, x, bx, by, bz, fx);
                 Weights W;
                 weights_of(fx, W);
+                // v += g dt before the scatter (Q9).  The reference writes it
+                // back; only G2P's slicer reflection reads it before G2P
+                // overwrites v, so G2P forms the same sum instead and P2G
+                // issues no global store until its tile flush.
                 const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                c.P.v[p] = make_float4(vel.x, vel.y, vel.z, vv.w);
                 const M3 stress = stress_kirchhoff(prm.x, prm.y, type, F);
                 const float sc = -c.dt * 4.0f * g.invdx * g.invdx * v0;
                 const M3 affine = stress * sc + Cm * m;
@@ -1419,7 +1422,9 @@ namespace
                 V3 x = v3(xm);
                 V3 vel;
                 M3 Cm;
-                gather_vc(c, ti, G, nSlicers, x, v3(vp), vel, Cm);
+                // the particle velocity after P2G's gravity update (P2G does not store it)
+                const V3 vpg = v3(vp) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                gather_vc(c, ti, G, nSlicers, x, vpg, vel, Cm);
                 x = clamp_bounds(c.g, x + vel * c.dt);
                 float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
