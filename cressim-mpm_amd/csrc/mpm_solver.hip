@@ -813,7 +813,7 @@ namespace
         }
     }
 
-    // Accumulation tiles per wavefront of the MLS and PB P2G kernels.
+    // Accumulation tiles per wavefront of MLS P2G and the fused PB iteration kernel.
     constexpr int kScatterTiles = 2;
 
     template <int Tiles>
@@ -1473,6 +1473,7 @@ namespace
 
     // Scatter of one PB particle into the LDS tiles A0[0 .. kScatterTiles)
     // (pbMpmParticleToGrid loop)
+    template <int Tiles>
     CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A0, int nSlicers, V3 x, float m, V3 vel,
                            const M3 &Cm)
     {
@@ -1485,7 +1486,7 @@ namespace
         const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
         const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-        in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
+        in_conflict_free_passes<Tiles>(cell_key(ti, bx, by, bz), [&](int tile) {
             float4 *A = A0 + tile * kTileNodes;
             if (fast)
             {
@@ -1518,10 +1519,14 @@ namespace
     // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
     // iteration fused with P2G of this one.  Mode 2: last G2P fused with
     // pbMpmIntegrateParticle, results written back in particle order.
+    // Mode 0 scatters into 3 accumulation tiles (it stages no grid tile),
+    // Mode 1 into 2 (a third one, and its occupancy and register cost, did not
+    // pay for the scatter passes it saves there).
     template <int Mode, bool Persistent>
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
-        __shared__ float4 acc[(Mode < 2) ? kWaves * kScatterTiles : 1][kTileNodes];
+        constexpr int kAcc = (Mode == 0) ? 3 : kScatterTiles;
+        __shared__ float4 acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
@@ -1534,10 +1539,10 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            float4 *A = acc[(Mode < 2) ? wave * kScatterTiles : 0];
+            float4 *A = acc[(Mode < 2) ? wave * kAcc : 0];
             float4 *G = lg[(Mode > 0) ? wave : 0];
             if (Mode < 2)
-                zero_tiles<kScatterTiles>(A, lane);
+                zero_tiles<kAcc>(A, lane);
             if (Mode > 0)
                 load_tile_grid(c.g, ti, c.grid, G, lane);
             wave_sync();
@@ -1569,7 +1574,7 @@ namespace
                         st.vs[S + k] = vel.y;
                         st.vs[2 * S + k] = vel.z;
                     }
-                    pb_scatter(c, ti, A, nSlicers, x, xm.w, vel, Cm);
+                    pb_scatter<kAcc>(c, ti, A, nSlicers, x, xm.w, vel, Cm);
                 }
                 else
                 {
@@ -1589,7 +1594,7 @@ namespace
             if (Mode < 2)
             {
                 wave_sync();
-                store_tile_sum<kScatterTiles>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
+                store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
             }
         }
     }
