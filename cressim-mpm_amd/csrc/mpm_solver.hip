@@ -2699,7 +2699,8 @@ namespace
         rc = upload_scenes(b);
         if (rc)
             return rc;
-        const bool useGraph = b->graphMode && !b->profEvents && b->stream != nullptr;
+        // substeps whose launches record profiling events run eagerly
+        const bool useGraph = b->graphMode && (!b->profEvents || b->profMask == 0u) && b->stream != nullptr;
         if (!useGraph)
             return enqueue_step(b);
         const long long key[9] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
@@ -3769,6 +3770,11 @@ int crHipSolverStep(CrHipSolver *s, float *dtOut)
 int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax)
 {
     return s ? batch_autotune(s->b, stepsPerTrial, tileWaves, updateWaves, subSize, subMax) : CRMPM_HIP_ERR_ARG;
+}
+
+int crHipSolverSetGraphMode(CrHipSolver *s, int enabled)
+{
+    return s ? crHipBatchSetGraphMode(s->b, enabled) : CRMPM_HIP_ERR_ARG;
 }
 
 int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax)

@@ -107,6 +107,7 @@ EXPORTS = {
     "crHipBatchSetGatherSplit": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipSolverGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverSetGraphMode": [_vp, ctypes.c_int],
     "crHipBatchFetchResults": [_vp],
     "crHipBatchGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                  ctypes.POINTER(ctypes.c_int)],
@@ -355,6 +356,10 @@ class HipSolver:
         check(self.lib.crHipSolverAutotune(self.h, steps_per_trial, ctypes.byref(t), ctypes.byref(u), ctypes.byref(ss),
                                            ctypes.byref(sm)), "autotune")
         return t.value, u.value, ss.value, sm.value
+
+    def set_graph_mode(self, on):
+        """Replay each substep as a hipGraph (captured on first use)."""
+        check(self.lib.crHipSolverSetGraphMode(self.h, 1 if on else 0), "graph mode")
 
     def gather_split(self):
         """(sub-tile size, sub-tiles per pblock) of the G2P kernels."""

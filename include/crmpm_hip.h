@@ -148,6 +148,9 @@ int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int 
 /* crHipBatchAutotune for the solver's one-scene batch (no counterpart in the
  * reference): picks the tile split and launch sizes on the current state. */
 int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
+/* crHipBatchSetGraphMode for the solver's one-scene batch: replay the
+ * substep as a hipGraph (no counterpart in the reference). */
+int crHipSolverSetGraphMode(CrHipSolver *s, int enabled);
 /* crHipBatchGetGatherSplit for the solver's one-scene batch (the split the
  * autotuner picked for the G2P kernels). */
 int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax);

@@ -148,3 +148,38 @@ def test_gather_split_is_bitwise_neutral(kind):
         for i in range(len(scenes)):
             for k in FIELDS:
                 assert np.array_equal(r[i][k], runs[0][i][k]), (i, k)
+
+
+def test_single_solver_graph_with_kinematic_frames():
+    """The single-scene solver replaying its substep as a hipGraph, with the
+    indenter's target uploaded between frames and every third substep run
+    eagerly with profiling events, matches eager stepping bitwise after every
+    frame."""
+    import crmpm_hip as H
+
+    sc = S.config2(nx=24, ny=6, nz=24, steps=40)
+    runs = {}
+    for graph in (False, True):
+        g = H.HipSolver(copy.deepcopy(sc))
+        g.compute_initial_data()
+        if graph:
+            g.set_graph_mode(True)
+            g.profile_begin(max_launches=4096)
+        frames = g.scene["frames"]
+        states = []
+        for s in range(40):
+            if s % frames[0]["substeps"] == 0:
+                S.apply_frame(g.scene, frames[s // frames[0]["substeps"]])
+                g._upload_shapes()
+            if graph:
+                g.profile_mask(H.PROF_CATEGORIES if s % 3 == 0 else [])
+            g.step(1)
+            if s % 10 == 9:
+                states.append(g.state())
+        if graph:
+            g.profile_end()
+        runs[graph] = states
+        g.close()
+    for a, b in zip(runs[False], runs[True]):
+        for k in FIELDS:
+            assert np.array_equal(a[k], b[k]), k
