@@ -3850,6 +3850,9 @@ namespace
     };
 } // namespace
 
+// Particles per sub-tile of the reverse (adjoint) kernels.
+constexpr int kAdjSubSize = 128;
+
 struct CrHipAdjoint
 {
     CrHipSolver *solver = nullptr;
@@ -4074,7 +4077,13 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             args.gatherOrder = args.order;
             args.segStart = a->segStart[j];
             args.segEnd = a->segEnd[j];
-            const int subWgs = grid1d(b->totalBlocks * b->subMax, kWaves);
+            // The reverse kernels use their own split, as fine as the partial
+            // slots allow: their per-particle work is several times the
+            // forward's, and the forward's tuned split (often whole pblocks)
+            // would leave them one wavefront per dense pblock.
+            args.subSize = kAdjSubSize;
+            args.subMax = b->subStride;
+            const int subWgs = grid1d(b->totalBlocks * args.subMax, kWaves);
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
             k_adj_g2p<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
                                                a->segGrid[j], ad);
