@@ -3851,7 +3851,7 @@ namespace
 } // namespace
 
 // Particles per sub-tile of the reverse (adjoint) kernels.
-constexpr int kAdjSubSize = 128;
+constexpr int kAdjSubSize = 64;
 
 struct CrHipAdjoint
 {
