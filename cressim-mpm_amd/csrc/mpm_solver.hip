@@ -3138,7 +3138,7 @@ namespace
         for (const auto &gs : gsplits)
         {
             float ms = 0.0f;
-            if (rc)
+            if (rc || b->kind == CRMPM_HIP_PB) // PB has no separate gather kernel
                 break;
             rc = tune_trial(b, backup, steps, 0, 0, bestS, bestM, gs[0], gs[1], &ms);
             if (!rc && ms < best)
