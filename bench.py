@@ -461,7 +461,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-substeps", type=int, default=1, help="substeps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-substeps", type=int, default=4,
+                    help="substeps in the CPU baseline sample, about 3 s each on one core (0 = skip)")
     ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--copy-calibration", action="store_true",
