@@ -338,8 +338,8 @@ namespace
     }
 
     // Start of a step.  Workgroups [0, blockWGs): zero the grid blocks written
-    // since the last step and mark every pblock empty (one wavefront per
-    // block).  Workgroup blockWGs + s copies scene s's bound shape tables into
+    // since the last step and mark every pblock empty (64 blocks per
+    // wavefront).  Workgroup blockWGs + s copies scene s's bound shape tables into
     // its compact record array in shapeIds order and lists its slicers
     // (shapes whose geometry is a slicer).
     __global__ __launch_bounds__(kWG) void k_prepare(int totalBlocks, int blockWGs, float4 *grid, float4 *gridForce,
@@ -371,23 +371,29 @@ namespace
             }
             return;
         }
-        const int b = blockIdx.x * kWaves + (threadIdx.x >> 6);
-        if (b == 0 && lane == 0)
+        // a wavefront takes 64 node blocks: one lane per block resets its flags
+        // and segment, then the whole wavefront clears each dirty block
+        const int b0 = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * 64;
+        const int b = b0 + lane;
+        if (b == 0)
             *activeCount = 0;
-        if (b >= totalBlocks)
-            return;
-        if (dirty[b])
+        bool wasDirty = false;
+        if (b < totalBlocks)
         {
-            grid[(size_t)b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gridForce)
-                gridForce[(size_t)b * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        wave_sync();
-        if (lane == 0)
-        {
+            wasDirty = dirty[b] != 0;
             dirty[b] = 0;
             segStart[b] = 0;
             segEnd[b] = 0;
+        }
+        unsigned long long pending = __ballot(wasDirty);
+        while (pending)
+        {
+            const int i = __ffsll(pending) - 1;
+            pending &= pending - 1ull;
+            const size_t node = (size_t)(b0 + i) * 64 + lane;
+            grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gridForce)
+                gridForce[node] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
 
@@ -2619,7 +2625,7 @@ namespace
     // The launches of one substep (all scenes).
     int enqueue_step(CrHipBatch *b)
     {
-        const int blockWGs = grid1d(b->totalBlocks, kWaves);
+        const int blockWGs = grid1d(b->totalBlocks, 64 * kWaves); // 64 node blocks per wavefront
         launch(b, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + b->numScenes, kWG, (int)b->totalBlocks, blockWGs, b->grid,
                b->gridForce, b->dirty, b->segStart, b->segEnd, (const SceneShapes *)b->dShapes, (const SceneRec *)b->dScenes,
                b->activeCount);
