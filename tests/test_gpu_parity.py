@@ -289,3 +289,54 @@ def test_dynamic_shape_coupling_force(kind, contact_model):
     print(f"{kind} contact model {contact_model}: force rel {ef:.2e} torque rel {et:.2e}", flush=True)
     assert ef <= 1e-4 and et <= 1e-4, (ef, et)
     assert rel(gs["v"], os_["v"]) <= TOL_100
+
+
+@pytest.mark.parametrize("kind", ["mls", "pb"])
+def test_empty_and_fully_masked(kind):
+    """No active particle (numActive 0) and every particle masked: a step
+    changes no particle, as in the reference, whose loops skip them."""
+    import crmpm_hip as H
+
+    sc = S.small_scene(KINDS[kind], 55, shapes="plane+sphere")
+    n = len(sc["xm"])
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    before = g.state()
+    H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, 0), "num active")
+    g.step(3)
+    mid = g.state()
+    for k in ("xm", "v", "F", "C"):
+        assert np.array_equal(mid[k], before[k]), k
+    H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, n), "num active")
+    import torch
+
+    g.mask[:] = 0
+    torch.cuda.synchronize()  # the solver runs on its own stream
+    g.step(3)
+    after = g.state()
+    for k in ("xm", "v", "F", "C"):
+        assert np.array_equal(after[k], before[k]), k
+    g.close()
+
+
+def test_batch_with_an_empty_scene():
+    """A scene with no active particle inside a batch leaves the other scenes'
+    trajectories as they are when run alone."""
+    import crmpm_hip as H
+
+    scenes = [S.small_scene(S.MLS, 60 + i, n_side=6 + 2 * i, shapes="plane") for i in range(3)]
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    H.check(b.lib.crHipBatchSetSceneNumActiveParticles(b.h, 1, 0), "num active")
+    b.step(5)
+    for i in (0, 2):
+        g = H.HipSolver(copy.deepcopy(scenes[i]))
+        g.compute_initial_data()
+        g.step(5)
+        gs, bs = g.state(), b.state(i)
+        g.close()
+        for k in ("xm", "v", "F", "C"):
+            assert rel(bs[k], gs[k]) <= 1e-6, (i, k)
+    x1 = b.state(1)["xm"]
+    assert np.array_equal(x1, np.asarray(scenes[1]["xm"], np.float64))
+    b.close()
