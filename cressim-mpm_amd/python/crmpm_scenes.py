@@ -136,6 +136,52 @@ def planar_slicer_grid(half_x, half_z, cell, spine):
     return (nx, ny, nz), lower, cell, np.vstack([np.asarray(spine, np.float64)[None, :], rows])
 
 
+def icosphere(radius, subdivisions=2, center=(0.0, 0.0, 0.0)):
+    """Closed triangle mesh of a sphere: an icosahedron with each face split
+    into 4 `subdivisions` times, vertices projected to the sphere.
+    Returns float32 points (n, 3) and uint32 triangles (m, 3), outward winding."""
+    t = (1.0 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    pts = [np.asarray(x, np.float64) / np.linalg.norm(x) for x in v]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                q = pts[a] + pts[b]
+                pts.append(q / np.linalg.norm(q))
+                mid[key] = len(pts) - 1
+            return mid[key]
+
+        f = [t2 for a, b, c in f for t2 in ((a, m(a, b), m(c, a)), (b, m(b, c), m(a, b)), (c, m(c, a), m(b, c)),
+                                            (m(a, b), m(b, c), m(c, a)))]
+    P = (np.asarray(pts) * radius + np.asarray(center)).astype(np.float32)
+    return P, np.asarray(f, np.uint32)
+
+
+def torus(major, minor, nu=24, nv=12):
+    """Closed triangle mesh of a torus around the y axis, outward winding."""
+    u = np.arange(nu) * 2 * np.pi / nu
+    v = np.arange(nv) * 2 * np.pi / nv
+    U, V = np.meshgrid(u, v, indexing="ij")
+    x = (major + minor * np.cos(V)) * np.cos(U)
+    z = (major + minor * np.cos(V)) * np.sin(U)
+    y = minor * np.sin(V)
+    P = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+    idx = lambda i, j: (i % nu) * nv + (j % nv)  # noqa: E731
+    tris = []
+    for i in range(nu):
+        for j in range(nv):
+            a, b, c, d = idx(i, j), idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)
+            tris += [(a, c, b), (a, d, c)]
+    return P, np.asarray(tris, np.uint32)
+
+
 def geometry_bounds(gtype, gparams, sdf=None):
     """Geometry::getBounds as SimulationFactoryImpl::createGeometry sets it
     (src/engine/simulation_factory_impl.cpp:521-577): box, sphere, capsule,

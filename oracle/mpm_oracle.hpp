@@ -1567,4 +1567,171 @@ namespace orc
         return n;
     }
 
+    // ------------------------------------------------------------------
+    // Triangle mesh -> SDF table, as the engine builds it for a triangle-mesh
+    // geometry (src/engine/simulation_factory_impl.cpp:628-635).  FP32 only.
+    // ------------------------------------------------------------------
+
+    // Closest point of triangle abc to p, region by region (Ericson, Real-Time
+    // Collision Detection 5.1.5), as src/geometry_utils/aabb_tree.cpp:336-439
+    // evaluates it; returns the squared distance.
+    inline float tri_closest(V3<float> p, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp)
+    {
+        const V3<float> ab = b - a, ac = c - a, bc = c - b, ap = p - a, bp = p - b, cpv = p - c;
+        const float d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cpv),
+                    d6 = dot(ac, cpv);
+        const float unom = d4 - d3, udenom = d5 - d6;
+        if (d1 <= 0.0f && d2 <= 0.0f)
+            cp = a;
+        else if (d3 >= 0.0f && d3 >= d4)
+            cp = b;
+        else if (d6 >= 0.0f && d6 >= d5)
+            cp = c;
+        else
+        {
+            const float vc = d1 * d4 - d3 * d2;
+            const float va = d3 * d6 - d5 * d4;
+            const float vb = d5 * d2 - d1 * d6;
+            if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f)
+                cp = a + ab * (d1 / (d1 - d3));
+            else if (va <= 0.0f && d4 >= d3 && d5 >= d6)
+                cp = b + bc * (unom / (unom + udenom));
+            else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f)
+                cp = a + ac * (d2 / (d2 - d6));
+            else
+            {
+                const float den = 1.0f / (va + vb + vc);
+                cp = a + ab * (vb * den) + ac * (vc * den);
+            }
+        }
+        const V3<float> v = p - cp;
+        return dot(v, v);
+    }
+
+    // The same in a unit box: everything shifted by the componentwise
+    // min(max(a, b), min(c, q)) and scaled by 1 / max(1e-12, largest extent)
+    // (aabb_tree.cpp:443-466; the shift is not the box minimum, kept as written).
+    inline float tri_closest_unit(V3<float> q, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp)
+    {
+        auto vmin = [](V3<float> u, V3<float> v) {
+            return V3<float>(std::fmin(u.x, v.x), std::fmin(u.y, v.y), std::fmin(u.z, v.z));
+        };
+        auto vmax = [](V3<float> u, V3<float> v) {
+            return V3<float>(std::fmax(u.x, v.x), std::fmax(u.y, v.y), std::fmax(u.z, v.z));
+        };
+        const V3<float> lo = vmin(vmax(a, b), vmin(c, q));
+        const V3<float> size = vmax(vmax(a, b), vmax(c, q)) - lo;
+        const float inv = std::fmax(1e-12f, std::fmax(size.x, std::fmax(size.y, size.z)));
+        const float s = 1.0f / inv;
+        V3<float> u;
+        const float r = tri_closest((q - lo) * s, (a - lo) * s, (b - lo) * s, (c - lo) * s, u);
+        cp = u * inv + lo;
+        return r * inv * inv;
+    }
+
+    // One triangle's term of the exact winding number: its solid angle over
+    // 4 pi (evaluateExact, include/geometry_utils/winding_number.h:79-93,
+    // the Van Oosterom-Strackee formula), expression order kept.
+    inline float winding_term(V3<float> a, V3<float> b, V3<float> c, V3<float> p)
+    {
+        const float k = float(0.5 / 3.141592653589793238462643383);
+        a = a - p;
+        b = b - p;
+        c = c - p;
+        const float la = norm(a), lb = norm(b), lc = norm(c);
+        const float y = a.x * b.y * c.z - a.x * b.z * c.y - a.y * b.x * c.z + a.y * b.z * c.x + a.z * b.x * c.y -
+                        a.z * b.y * c.x;
+        const float x = (la * lb * lc + (a.x * b.x + a.y * b.y + a.z * b.z) * lc + (b.x * c.x + b.y * c.y + b.z * c.z) * la +
+                         (c.x * a.x + c.y * a.y + c.z * a.z) * lb);
+        return k * std::atan2(y, x);
+    }
+
+    // SdfBuilder::precomputeSdfDesc (src/geometry_utils/sdf_builder.cpp:84-101):
+    // the AABB tree's root box -- the union of the triangles' boxes, each grown
+    // by 1e-4 (buildTree, winding_number.h:148-161) -- grown by fatten;
+    // dims = int(size / cellSize + 1).
+    inline int mesh_sdf_desc(const float *pts, const unsigned *tris, int numTris, float cell, float fatten, int *dims,
+                             float *lower)
+    {
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int t = 0; t < 3 * numTris; ++t)
+            for (int k = 0; k < 3; ++k)
+            {
+                lo[k] = std::fmin(lo[k], pts[3 * tris[t] + k] - 1e-4f);
+                hi[k] = std::fmax(hi[k], pts[3 * tris[t] + k] + 1e-4f);
+            }
+        long count = 1;
+        for (int k = 0; k < 3; ++k)
+        {
+            lower[k] = lo[k] - fatten;
+            const float size = (hi[k] + fatten) - lower[k];
+            dims[k] = static_cast<int>(size / cell + 1.0f);
+            count *= dims[k];
+        }
+        return static_cast<int>(count);
+    }
+
+    // SdfBuilder::computeSdf + computeSdfGradient (sdf_builder.cpp:103-160,
+    // 242-330).  Node (x, y, z) sits at lower + (x, y, z) * cellSize, row
+    // z * dx * dy + y * dx + x.  The closest triangle is found by exhaustive
+    // search in index order with a strict <, where the reference walks its AABB
+    // tree: the minimum is the same, and where several triangles share the
+    // closest point (an edge or vertex) the one picked may differ, with the
+    // distance equal up to rounding.  The sign is -1 where the exact winding
+    // number (computeWindingNumberExact, winding_number.h:136-146) exceeds 0.5;
+    // the reference thresholds its far-field approximation of the same number
+    // (beta = 2, sdf_builder.cpp:150), which differs from it only by the
+    // approximation error, i.e. at nodes on the surface itself.
+    inline void mesh_sdf(const float *pts, const unsigned *tris, int numTris, float cell, const int *dims,
+                         const float *lower, float *grid4)
+    {
+        const int dx = dims[0], dy = dims[1], dz = dims[2];
+        auto P = [&](unsigned i) { return V3<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); };
+        for (int z = 0; z < dz; ++z)
+            for (int y = 0; y < dy; ++y)
+                for (int x = 0; x < dx; ++x)
+                {
+                    const V3<float> q(lower[0] + float(x) * cell, lower[1] + float(y) * cell, lower[2] + float(z) * cell);
+                    float best = FLT_MAX, wn = 0.0f;
+                    V3<float> bestP;
+                    for (int t = 0; t < numTris; ++t)
+                    {
+                        const V3<float> a = P(tris[3 * t]), b = P(tris[3 * t + 1]), c = P(tris[3 * t + 2]);
+                        V3<float> cp;
+                        const float d2 = tri_closest_unit(q, a, b, c, cp);
+                        if (d2 < best)
+                        {
+                            best = d2;
+                            bestP = cp;
+                        }
+                        wn += winding_term(a, b, c, q);
+                    }
+                    const float dist = norm(bestP - q);
+                    grid4[4 * ((z * dy + y) * dx + x) + 3] = wn > 0.5f ? -dist : dist;
+                }
+        const float inv = 1.0f / cell;
+        auto W = [&](int x, int y, int z) { return grid4[4 * ((z * dy + y) * dx + x) + 3]; };
+        auto diff = [&](int i, int n, float lo, float mid, float hi) {
+            if (i > 0 && i < n - 1)
+                return (hi - lo) * 0.5f * inv;
+            return i == 0 ? (hi - mid) * inv : (mid - lo) * inv;
+        };
+        for (int z = 0; z < dz; ++z)
+            for (int y = 0; y < dy; ++y)
+                for (int x = 0; x < dx; ++x)
+                {
+                    const float m = W(x, y, z);
+                    // a one-node axis has no neighbour: the reference then reads
+                    // past the row; such tables are refused by the builders here
+                    V3<float> g(diff(x, dx, x > 0 ? W(x - 1, y, z) : m, m, x < dx - 1 ? W(x + 1, y, z) : m),
+                                diff(y, dy, y > 0 ? W(x, y - 1, z) : m, m, y < dy - 1 ? W(x, y + 1, z) : m),
+                                diff(z, dz, z > 0 ? W(x, y, z - 1) : m, m, z < dz - 1 ? W(x, y, z + 1) : m));
+                    normalize(g);
+                    float *o = grid4 + 4 * ((z * dy + y) * dx + x);
+                    o[0] = g.x;
+                    o[1] = g.y;
+                    o[2] = g.z;
+                }
+    }
+
 } // namespace orc

@@ -272,6 +272,21 @@ extern "C"
         return n;
     }
 
+    // Mesh -> SDF table (src/engine/simulation_factory_impl.cpp:628-635):
+    // writes dims and lower; fills grid4 when the row count fits capacity.
+    // -1 for a vertex index beyond numPoints.
+    int orc_mesh_sdf(const float *pts, int numPoints, const unsigned *tris, int numTris, float cell, float fatten,
+                     int *dims3, float *lower3, float *grid4, int capacity)
+    {
+        for (int t = 0; t < 3 * numTris; ++t)
+            if (tris[t] >= static_cast<unsigned>(numPoints))
+                return -1;
+        const int n = orc::mesh_sdf_desc(pts, tris, numTris, cell, fatten, dims3, lower3);
+        if (grid4 && n <= capacity)
+            orc::mesh_sdf(pts, tris, numTris, cell, dims3, lower3, grid4);
+        return n;
+    }
+
     // svd3 on n column-major 3x3 matrices (include/math/svd3.h:1265-1275)
     void orc_svd3(int fp64, int n, const float *A9, double *U9, double *Vt9, double *S3)
     {

@@ -52,6 +52,8 @@ def _declare(lib, pre):
         "svd3": (None, [ctypes.c_int, ctypes.c_int, _f, _d, _d, _d]),
         "particle_lattice": (ctypes.c_uint, [ctypes.c_int, _f, ctypes.c_int, _i, _f, ctypes.c_int, _f, _f, _f,
                                              ctypes.c_float, _f, _f, _f, ctypes.c_float, _f, ctypes.c_uint]),
+        "mesh_sdf": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i, _f, _f,
+                                    ctypes.c_int]),
         "stress": (None, [ctypes.c_int, _f, _i, _f, _d]),
         "pb_constraint": (None, [ctypes.c_int, ctypes.c_float, _f, _i, _f, _f, _d]),
     }
@@ -231,6 +233,26 @@ def particle_lattice(obj, cap=None):
                            P(lo, _f), P(hi, _f), float(obj["spacing"]), P(pos, _f), P(rot, _f), P(inv, _f),
                            float(obj["mass"]), P(out, _f), m)
     return n, out[:m]
+
+
+def mesh_sdf(points, tris, cell, fatten, which="oracle"):
+    """Triangle mesh -> SDF table through the oracle or the reference's own
+    host builder (oracle/_ref): returns dims, lower bound and the (n, 4)
+    float32 rows (gradient, signed distance)."""
+    fn = load(which)
+    if fn is None:
+        raise RuntimeError("reference CPU build not present")
+    pts = np.ascontiguousarray(points, np.float32)
+    tr = np.ascontiguousarray(tris, np.uint32)
+    dims = np.zeros(3, np.int32)
+    lower = np.zeros(3, np.float32)
+    args = (P(pts, _f), len(pts), P(tr, _u), len(tr), float(cell), float(fatten), P(dims, _i), P(lower, _f))
+    n = fn["mesh_sdf"](*args, None, 0)
+    if n < 0:
+        raise ValueError("triangle vertex index out of range")
+    grid = np.zeros((n, 4), np.float32)
+    fn["mesh_sdf"](*args, P(grid, _f), n)
+    return dims, lower, grid
 
 
 def svd3(which, A, fp64=False):

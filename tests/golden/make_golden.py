@@ -2,8 +2,11 @@
 solvers (oracle/_ref/libcrmpm_ref.so, compiled from the reference sources by
 oracle/Makefile target `ref`).
 
-Each fixture holds only outputs; the inputs are regenerated from the seeded
-scene builders in crmpm_scenes and pinned by a SHA-256 of the input arrays.
+Each substep fixture holds only outputs; the inputs are regenerated from the
+seeded scene builders in crmpm_scenes and pinned by a SHA-256 of the input
+arrays.  The mesh-SDF fixtures (sdf_*.npz) hold the mesh with the table, from
+the reference's host SDF builder; two of the meshes are the reference's own
+test assets (assets/*.obj), read here as data.
 Run from the repository root:  python tests/golden/make_golden.py
 """
 import copy
@@ -31,6 +34,46 @@ def cases():
             yield f"{name}_{shapes.replace('+', '_')}", S.small_scene(kind, 7 + kind, shapes=shapes), STEPS
     yield "mls_cube16_100", S.config1(n_side=16, steps=100), 100
     yield "pb_slab24x6x24_30", S.config2(nx=24, ny=6, nz=24, steps=30), 30
+
+
+REF_ASSETS = "/root/reference/assets"
+
+# The cube of the reference's winding-number test (tests/test_trimesh/main.cpp:162-205).
+CUBE_POINTS = np.array([[1, 1, -1], [1, -1, -1], [1, 1, 1], [1, -1, 1], [-1, 1, -1], [-1, -1, -1], [-1, 1, 1],
+                        [-1, -1, 1]], np.float32)
+CUBE_TRIS = np.array([[4, 2, 0], [2, 7, 3], [6, 5, 7], [1, 7, 5], [0, 3, 1], [4, 1, 5], [4, 6, 2], [2, 6, 7],
+                      [6, 4, 5], [1, 3, 7], [0, 2, 3], [4, 0, 1]], np.uint32)
+
+
+def load_obj(path):
+    """Vertices and triangles of a Wavefront OBJ file (v / f lines, 1-based
+    indices, texture and normal indices dropped)."""
+    pts, tris = [], []
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if not w:
+                continue
+            if w[0] == "v":
+                pts.append([float(x) for x in w[1:4]])
+            elif w[0] == "f":
+                tris.append([int(x.split("/")[0]) - 1 for x in w[1:4]])
+    return np.asarray(pts, np.float32), np.asarray(tris, np.uint32)
+
+
+def sdf_cases():
+    """(name, points, triangles, cellSize, fatten) of the mesh-SDF fixtures:
+    the reference's own test meshes (assets/sphere.obj at the cell size and
+    fattening of tests/test_trimesh/main.cpp:258-260 and a finer one, the open
+    assets/hemisphere.obj, the winding-number test cube) and two generated
+    meshes, a subdivided icosahedron and a torus."""
+    sphere = load_obj(os.path.join(REF_ASSETS, "sphere.obj"))
+    yield "sdf_sphere_obj", *sphere, 0.2, 0.05
+    yield "sdf_sphere_obj_fine", *sphere, 0.1, 0.05
+    yield "sdf_hemisphere_obj", *load_obj(os.path.join(REF_ASSETS, "hemisphere.obj")), 0.1, 0.1
+    yield "sdf_cube", CUBE_POINTS, CUBE_TRIS, 0.2, 0.3
+    yield "sdf_icosphere", *S.icosphere(0.1, 3), 0.01, 0.02
+    yield "sdf_torus", *S.torus(0.1, 0.03, 48, 24), 0.01, 0.01
 
 
 def input_digest(sc):
@@ -69,6 +112,13 @@ def main():
                             force=f.astype(np.float32), torque=t.astype(np.float32),
                             steps=np.int32(steps), digest=np.array(input_digest(sc)))
         print("wrote", out)
+    # Mesh -> SDF tables from the reference's host builder (SdfBuilder,
+    # src/geometry_utils/sdf_builder.cpp), mesh included.
+    for name, pts, tris, cell, fatten in sdf_cases():
+        dims, lower, grid = CC.mesh_sdf(pts, tris, cell, fatten, which="reference")
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), points=pts, tris=tris, cell=np.float32(cell),
+                            fatten=np.float32(fatten), dims=dims, lower=lower, grid=grid)
+        print(f"wrote {name}.npz", tuple(dims))
     # SVD: the matrix of the reference's own SVD test (tests/test_svd/main.cpp:36,
     # column-major constructor) plus random, repeated-singular-value, negative
     # determinant and near-singular matrices.

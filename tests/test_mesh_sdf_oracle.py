@@ -1,0 +1,197 @@
+"""Mesh -> SDF table (SdfBuilder, src/geometry_utils/sdf_builder.cpp), CPU only.
+
+The oracle's restatement (oracle/mpm_oracle.hpp, mesh_sdf) is pinned
+1. against the reference's own host builder, compiled from its sources
+   (oracle/_ref) and committed as fixtures (tests/golden/sdf_*.npz), on the
+   reference's test meshes (assets/sphere.obj at its test's cell size, the
+   open assets/hemisphere.obj, the winding-number test cube) and generated
+   ones (subdivided icosahedron, torus);
+2. against closed forms: the signed distance of a box for the cube mesh and a
+   brute-force sampled distance to a single triangle.
+A mesh with its winding flipped (a planted mistake in the oracle's input)
+flips every sign and is caught by (1).  The library's host-side entry points
+(table size, argument errors) are checked here too, without a GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import cpu_checkers as CC
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = ["sdf_sphere_obj", "sdf_sphere_obj_fine", "sdf_hemisphere_obj", "sdf_cube", "sdf_icosphere", "sdf_torus"]
+OPEN_MESHES = {"sdf_hemisphere_obj"}
+# The reference signs with a far-field approximation of the winding number
+# (beta = 2) and the oracle with the exact sum: on an open mesh they may
+# disagree where the winding number is near 0.5, past the rim; the fixture
+# has 3 such nodes of 6877.
+OPEN_SIGN_MISMATCH_FRACTION = 1e-3
+
+
+def fixture(name):
+    return dict(np.load(os.path.join(GOLDEN, f"{name}.npz")))
+
+
+def node_points(dims, lower, cell):
+    z, y, x = np.meshgrid(*(np.arange(d) for d in dims[::-1]), indexing="ij")
+    ix = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+    return lower.astype(np.float32) + ix * np.float32(cell)
+
+
+def sign_stable(dist, dims):
+    """Nodes whose own sign and their six neighbours' signs are given by
+    `dist` alone: where two tables agree on these, their gradients agree."""
+    s = np.sign(dist).reshape(dims[2], dims[1], dims[0])
+    ok = np.ones_like(s, bool)
+    for ax in range(3):
+        for sh in (1, -1):
+            ok &= np.roll(s, sh, axis=ax) == s
+    return ok.ravel()
+
+
+def compare_tables(name, g, dims, lower, grid, rel=1e-6):
+    """Table against the fixture: same dims and lower bound, |distance| equal
+    up to the rounding of ties between triangles, same sign (all nodes for a
+    closed mesh), same gradient where the signs around a node agree."""
+    assert np.array_equal(dims, g["dims"]) and np.array_equal(lower, g["lower"])
+    ref = g["grid"]
+    scale = max(1.0, float(np.abs(ref[:, 3]).max()))
+    dd = np.abs(np.abs(grid[:, 3]) - np.abs(ref[:, 3]))
+    assert dd.max() <= rel * scale, (name, dd.max())
+    mism = np.sign(grid[:, 3]) != np.sign(ref[:, 3])
+    if name in OPEN_MESHES:
+        assert mism.mean() <= OPEN_SIGN_MISMATCH_FRACTION, (name, mism.sum())
+    else:
+        assert not mism.any(), (name, mism.sum())
+    stable = sign_stable(ref[:, 3], dims) & sign_stable(grid[:, 3], dims)
+    stable &= ~mism
+    gd = np.abs(grid[stable, :3] - ref[stable, :3]).max()
+    assert gd <= 2e-4, (name, gd)
+    return dd
+
+
+needs_ref = pytest.mark.skipif(not CC.ref_available(), reason="reference CPU build (oracle/_ref) not present")
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_matches_reference_fixture(name):
+    g = fixture(name)
+    dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]))
+    dd = compare_tables(name, g, dims, lower, grid)
+    # the same arithmetic: most distances are bit-identical
+    assert np.mean(dd == 0) > 0.95, np.mean(dd == 0)
+
+
+@needs_ref
+def test_reference_build_reproduces_fixture():
+    g = fixture("sdf_sphere_obj")
+    dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]), which="reference")
+    assert np.array_equal(grid, g["grid"]) and np.array_equal(dims, g["dims"])
+
+
+def test_flipped_winding_is_caught():
+    """Planted in the oracle's input: every triangle reversed.  The distances
+    stay, the signs flip, and the fixture comparison fails."""
+    g = fixture("sdf_cube")
+    dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"][:, ::-1], float(g["cell"]), float(g["fatten"]))
+    with pytest.raises(AssertionError):
+        compare_tables("sdf_cube", g, dims, lower, grid)
+
+
+def test_cube_closed_form():
+    """The reference's test cube (half size 1): the box SDF
+    |max(|q| - 1, 0)| + min(max_k(|q_k| - 1), 0), and at nodes facing a face
+    the gradient is that face's normal."""
+    g = fixture("sdf_cube")
+    cell = float(g["cell"])
+    dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"], cell, 1.05)
+    q = node_points(dims, lower, cell).astype(np.float64)
+    e = np.abs(q) - 1.0
+    exact = np.linalg.norm(np.maximum(e, 0), axis=1) + np.minimum(e.max(1), 0)
+    assert np.abs(grid[:, 3] - exact).max() < 2e-6
+    # inside: nodes at (0.5, 0.5, 0.5)-like points have winding number 1
+    assert (grid[exact < -1e-3, 3] < 0).all() and (grid[exact > 1e-3, 3] > 0).all()
+    for ax in range(3):
+        o = [k for k in range(3) if k != ax]
+        face = (np.abs(q[:, o]).max(1) < 1 - 2 * cell) & (q[:, ax] > 1 + 2 * cell)
+        face &= q[:, ax] < q[:, ax].max() - cell / 2  # one-sided rows at the table edge excluded
+        assert face.sum() > 10
+        n = np.zeros(3)
+        n[ax] = 1
+        assert np.abs(grid[face, :3] - n).max() < 1e-5, ax
+
+
+def test_single_triangle_against_sampling():
+    """One triangle: unsigned distance (its winding number stays below 0.5
+    off its plane) against the minimum over a dense barycentric sampling,
+    which lies above the true distance by at most the sampling spacing."""
+    rng = np.random.default_rng(5)
+    tri = rng.uniform(-0.5, 0.5, (3, 3)).astype(np.float32)
+    dims, lower, grid = CC.mesh_sdf(tri, np.array([[0, 1, 2]], np.uint32), 0.1, 0.3)
+    q = node_points(dims, lower, 0.1).astype(np.float64)
+    m = 400
+    u, v = np.meshgrid(np.linspace(0, 1, m + 1), np.linspace(0, 1, m + 1))
+    keep = u + v <= 1
+    bar = np.stack([1 - u[keep] - v[keep], u[keep], v[keep]], 1)
+    pts = bar @ tri.astype(np.float64)
+    edge = max(np.linalg.norm(tri[i] - tri[j]) for i, j in ((0, 1), (1, 2), (0, 2)))
+    sampled = np.full(len(q), np.inf)
+    for a in range(0, len(q), 256):
+        d = np.linalg.norm(q[a:a + 256, None, :] - pts[None], axis=2)
+        sampled[a:a + 256] = d.min(1)
+    assert (grid[:, 3] > 0).all()
+    diff = sampled - grid[:, 3]
+    assert diff.min() > -1e-5 and diff.max() < 1.5 * edge / m, (diff.min(), diff.max())
+
+
+# ---------------- the library's host-side entry points (no GPU) ----------------
+
+def _desc(lib, pts, tris, cell, fatten):
+    import crmpm_hip as H
+
+    dims = (ctypes.c_int * 3)()
+    lower = (ctypes.c_float * 3)()
+    rc = lib.crHipMeshSdfDesc(H._vp(pts.ctypes.data), len(pts), H._vp(tris.ctypes.data), len(tris), cell, fatten,
+                              dims, lower)
+    return rc, np.array(dims[:]), np.array(lower[:], np.float32)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_capi_mesh_sdf_desc_matches_reference(name):
+    import crmpm_hip as H
+
+    g = fixture(name)
+    rc, dims, lower = _desc(H.load_library(), np.ascontiguousarray(g["points"]), np.ascontiguousarray(g["tris"]),
+                            float(g["cell"]), float(g["fatten"]))
+    assert rc == 0 and np.array_equal(dims, g["dims"]) and np.array_equal(lower, g["lower"])
+
+
+def test_capi_mesh_sdf_argument_errors():
+    import crmpm_hip as H
+
+    lib = H.load_library()
+    g = fixture("sdf_cube")
+    pts, tris = np.ascontiguousarray(g["points"]), np.ascontiguousarray(g["tris"])
+    assert _desc(lib, pts, tris, 0.0, 0.1)[0] == 1
+    assert _desc(lib, pts, tris, 0.1, -1.0)[0] == 1
+    bad = tris.copy()
+    bad[3, 1] = 8
+    assert _desc(lib, pts, bad, 0.1, 0.1)[0] == 1
+    nanp = pts.copy()
+    nanp[2, 0] = np.nan
+    assert _desc(lib, nanp, tris, 0.1, 0.1)[0] == 1
+    assert _desc(lib, pts, tris, 1e-9, 0.1)[0] == 5  # 2^31 rows or more
+    dims = (ctypes.c_int * 3)()
+    lower = (ctypes.c_float * 3)()
+    args = (H._vp(pts.ctypes.data), len(pts), H._vp(tris.ctypes.data), len(tris), 0.2, 0.3, dims, lower)
+    # a table row buffer too small: refused before any device work
+    assert lib.crHipMeshSdf(*args, H._vp(16), 10, None) == 5
+    # a flat mesh without fattening has one node across its plane
+    flat = np.ascontiguousarray(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32))
+    one = np.ascontiguousarray(np.array([[0, 1, 2]], np.uint32))
+    rc, d, _ = _desc(lib, flat, one, 0.1, 0.0)
+    assert rc == 0 and d[2] == 1
+    args = (H._vp(flat.ctypes.data), 3, H._vp(one.ctypes.data), 1, 0.1, 0.0, dims, lower)
+    assert lib.crHipMeshSdf(*args, H._vp(16), 1 << 20, None) == 1
