@@ -52,6 +52,10 @@ _f4 = ctypes.c_float * 4
 EXPORTS = {
     "crHipParticleLattice": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3, _f4, _f3,
                              ctypes.c_float, _vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), _vp],
+    "crHipMeshSdfDesc": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
+    "crHipMeshSdf": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong, _vp],
     "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
     "crHipSolverDestroy": [_vp],
     "crHipSolverGetGridDims": [_vp, ctypes.POINTER(ctypes.c_int)],
@@ -173,6 +177,29 @@ def particle_lattice(obj, device="cuda:0", capacity=None):
     out = torch.empty((max(m, 1), 4), dtype=torch.float32, device=dev)
     check(lib.crHipParticleLattice(*args, _vp(out.data_ptr()), m, ctypes.byref(n), _vp(stream)), "particle lattice")
     return n.value, out[:m]
+
+
+def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0"):
+    """SdfBuilder's mesh -> SDF table on the device (crHipMeshSdf).  points
+    (n, 3) float32 and triangles (m, 3) uint32 on the host.  Returns dims,
+    lower bound (numpy) and a (dx*dy*dz, 4) float32 device tensor of rows
+    (gradient, signed distance)."""
+    import torch
+
+    lib = load_library()
+    dev = torch.device(device)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    pts = np.ascontiguousarray(points, np.float32)
+    tris = np.ascontiguousarray(triangles, np.uint32)
+    dims = (ctypes.c_int * 3)()
+    lower = (ctypes.c_float * 3)()
+    args = (_vp(pts.ctypes.data), len(pts), _vp(tris.ctypes.data), len(tris), float(cell_size), float(fatten), dims,
+            lower)
+    check(lib.crHipMeshSdfDesc(*args), "mesh sdf desc")
+    n = dims[0] * dims[1] * dims[2]
+    out = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    check(lib.crHipMeshSdf(*args, _vp(out.data_ptr()), n, _vp(stream)), "mesh sdf")
+    return np.array(dims[:], np.int32), np.array(lower[:], np.float32), out
 
 
 def _read_state(torch, device, stream, F, C, V0, stride, a, e):

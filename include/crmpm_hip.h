@@ -262,6 +262,33 @@ int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask);
 int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories);
 
 /* ------------------------------------------------------------------------
+ * Triangle mesh -> SDF table on the device: the table the engine builds on
+ * the host for a triangle-mesh geometry (SimulationFactoryImpl::createGeometry,
+ * src/engine/simulation_factory_impl.cpp:628-635).
+ *
+ * crHipMeshSdfDesc replaces SdfBuilder::precomputeSdfDesc
+ * (src/geometry_utils/sdf_builder.cpp:84-101): from host arrays points
+ * (float xyz per vertex) and triangles (3 vertex indices each), the table's
+ * dims (int(size / cellSize + 1) per axis) and lower bound (the triangles'
+ * bounding box grown by 1e-4 and by fatten).
+ *
+ * crHipMeshSdf replaces SdfBuilder::computeSdf + computeSdfGradient
+ * (sdf_builder.cpp:103-160, 242-330): writes dims and lowerBound as above
+ * and, when gradientSignedDistance (device float4[count], the geometry's rows
+ * of CrHipGeometrySdfData::gradientSignedDistance) is not NULL, fills row
+ * z*dx*dy + y*dx + x with the normalised gradient (xyz) and signed distance
+ * (w) at lowerBound + (x, y, z) * cellSize; negative inside (winding number
+ * above 0.5).  Synchronous on stream.  Errors: CRMPM_HIP_ERR_ARG for a missing
+ * pointer, no triangles, cellSize <= 0, fatten < 0, a vertex index beyond
+ * numPoints, a non-finite vertex, or a table with one node along an axis;
+ * CRMPM_HIP_ERR_CAPACITY for capacity < dx*dy*dz rows or 2^31 rows or more. */
+int crHipMeshSdfDesc(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                     float cellSize, float fatten, int dims[3], float lowerBound[3]);
+int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles, float cellSize,
+                 float fatten, int dims[3], float lowerBound[3], float *gradientSignedDistance,
+                 unsigned long long capacity, void *stream);
+
+/* ------------------------------------------------------------------------
  * Particle lattice of a particle object on the device:
  * ParticleObjectImpl::initialize (src/engine/particle_object_impl.cpp:36-88),
  * which SimulationFactoryImpl::createParticleObject runs on the host
