@@ -115,6 +115,9 @@ PROFILE_TARGETS = {
                     title="Kernel trace: 64 batched scenes (MLS 100 substeps, PB 20 substeps) and the same scenes "
                           "one at a time",
                     md="batched64_kernel_trace.md"),
+    "meshsdf": dict(run="python3 bench.py --workload meshsdf", counters=True,
+                    title="Kernel trace and counters: mesh -> SDF table, 20,480-triangle icosphere on 128^3 nodes",
+                    md="meshsdf_icosphere128_kernel_trace.md"),
     "adjoint": dict(run="python3 bench.py --workload adjoint", counters=False,
                     title="Kernel trace: differentiable MLS block, 65,536 particles, 100 substeps, forward + adjoint",
                     md="adjoint65k_kernel_trace.md"),
@@ -445,6 +448,47 @@ def lattice(reps=5):
     return out
 
 
+def mesh_sdf(reps=3):
+    """Triangle mesh -> SDF table on the device (SdfBuilder's computeSdf +
+    computeSdfGradient): a subdivided icosahedron of radius 0.1 m, 20,480
+    triangles, on a 128^3 table, against the reference's host builder (or the
+    oracle) on the same mesh at a coarser cell, per table node."""
+    import torch
+
+    import crmpm_hip as H
+
+    pts, tris = S.icosphere(0.1, 5)
+    cell, fatten = 0.0018, 0.0147
+    dims, _, out = H.mesh_sdf(pts, tris, cell, fatten)  # warm-up, loads the code objects
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        H.mesh_sdf(pts, tris, cell, fatten)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    nodes = int(np.prod(dims))
+    pairs = nodes * len(tris)
+    res = {"workload": "icosphere r = 0.1 m, 20,480 triangles -> SDF table of 128^3 nodes (float4 gradient|distance)",
+           "dims": [int(d) for d in dims], "triangles": len(tris), "seconds": best, "nodes_per_s": nodes / best,
+           "node_triangle_pairs_per_s": pairs / best, "finite": bool(torch.isfinite(out).all())}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as CC
+
+    which = "reference" if CC.ref_available() else "oracle"
+    ccell = cell * (4 if which == "reference" else 16)
+    t0 = time.perf_counter()
+    cd, _, _ = CC.mesh_sdf(pts, tris, ccell, fatten, which=which)
+    dt = time.perf_counter() - t0
+    cn = int(np.prod(cd))
+    res["cpu_baseline"] = {"value": cn / dt, "unit": "table nodes/s", "cores": 1,
+                           "kind": "reference" if which == "reference" else "port",
+                           "sample": f"the same mesh on a {tuple(int(d) for d in cd)} table (cell {ccell:.4f} m) in "
+                                     f"{dt:.2f} s on 1 core"}
+    res["speedup_vs_cpu"] = res["nodes_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
 def mls_cube(steps=200, warmup=5):
     """The reference's CPU-runnable case (config1: 32^3 = 32,768 particles, 64^3 cells, 200 substeps)."""
     sc = S.config1(steps=steps + warmup)
@@ -463,7 +507,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=4,
                     help="substeps in the CPU baseline sample, about 3 s each on one core (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice"], default="all",
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice", "meshsdf"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
@@ -503,6 +547,9 @@ def main():
         return
     if args.workload == "lattice":
         print(json.dumps({"lattice": lattice()}), flush=True)
+        return
+    if args.workload == "meshsdf":
+        print(json.dumps({"mesh_sdf": mesh_sdf()}), flush=True)
         return
     if args.workload == "adjoint":
         print(json.dumps({"adjoint": adjoint()}), flush=True)
@@ -603,7 +650,7 @@ def main():
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
         for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube),
-                        ("lattice", lattice)):
+                        ("lattice", lattice), ("mesh_sdf", mesh_sdf)):
             try:
                 out[key] = fn()
             except Exception as e:

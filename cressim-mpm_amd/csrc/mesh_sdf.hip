@@ -12,25 +12,27 @@
 // per node for the closest triangle and a clustered far-field approximation
 // of the winding number for the sign; here a wavefront owns a 4x4x4 brick of
 // nodes and streams the triangles through LDS, each lane evaluating the
-// point-triangle distance and the triangle's solid angle for its node.  The
-// bricks keep the lanes of a wavefront in the same Voronoi region of most
-// triangles, so the region branches rarely diverge.  Small tables split the
-// triangle list into slices over more workgroups; a combine pass takes the
-// slices in order.
+// triangle's solid angle for its node and, unless the triangle's box is
+// provably too far from the brick for any lane to take it, the
+// point-triangle distance.  The bricks keep the lanes of a wavefront in the
+// same Voronoi region of most triangles, so the region branches rarely
+// diverge.  Small tables split the triangle list into slices over more
+// workgroups; a combine pass takes the slices in order.
 //
-// Results against the reference's arithmetic (the library is compiled with
-// -ffp-contract=off and IEEE division and square root):
+// Results against the reference's arithmetic (the distances are evaluated
+// under -ffp-contract=off with IEEE division and square root):
 //   * distance: the closest triangle in index order with a strict <, its
 //     closest point from the same unit-box evaluation as the reference, so
 //     |distance| is the reference's bit for bit wherever the minimum is
 //     unique; where triangles share the closest point the reference's tree
 //     order may pick another one, equal up to rounding;
 //   * sign: the exact winding number (the sum the reference's
-//     computeWindingNumberExact forms, include/geometry_utils/winding_number.h:136-146)
-//     against 0.5; the reference thresholds its beta = 2 approximation of the
-//     same number (sdf_builder.cpp:150), so the two differ only where the
-//     winding number is within that approximation's error of 0.5: on the
-//     surface of a closed mesh, near the rim of an open one;
+//     computeWindingNumberExact forms, include/geometry_utils/winding_number.h:136-146,
+//     here with FMAs and a polynomial atan2, about 1e-7 per triangle from its
+//     FP32 evaluation) against 0.5; the reference thresholds its beta = 2
+//     approximation of the same number (sdf_builder.cpp:150), so the two
+//     differ only where the winding number is within that approximation's
+//     error of 0.5: on the surface of a closed mesh, near the rim of an open one;
 //   * gradient: the reference's differences and normalisation, so equal to it
 //     wherever the neighbouring distances are.
 #include <hip/hip_runtime.h>
@@ -53,9 +55,14 @@ namespace crhip
 
         struct MeshSdfArgs
         {
-            const float4 *tri;  // 3 float4 per triangle: a, b, c (w unused)
+            const float4 *tri;  // 4 float4 per triangle: (a, lo.x), (b, lo.y), (c, lo.z), (hi, cull slack)
+            const float4 *cen;  // per triangle: centroid
             float4 *part;       // per slice and node: closest point, squared distance
             float *partWn;      // per slice and node: winding number of the slice's triangles
+                                // (mesh-slicer table: the closest triangle's index)
+            const int *flags;   // mesh-slicer table: per triangle, boundary edges ab, bc, ac (bits 0-2)
+                                // and boundary vertices a, b, c (bits 3-5)
+            float4 spine;       // mesh-slicer table: spine area (x NaN: none)
             float3 lower;
             float cell;
             int dx, dy, dz;
@@ -70,35 +77,38 @@ namespace crhip
         // Closest point of triangle abc to p, region by region (Ericson,
         // Real-Time Collision Detection 5.1.5), with the reference's tests
         // and expressions (src/geometry_utils/aabb_tree.cpp:336-439).
-        CR_DEV float tri_closest(V3 p, V3 a, V3 b, V3 c, V3 &cp)
+        // Regions of the closest point: vertices a, b, c, edges ab, bc, ac, face.
+        enum : int { kRegA = 0, kRegB, kRegC, kRegAB, kRegBC, kRegAC, kRegFace };
+
+        CR_DEV float tri_closest(V3 p, V3 a, V3 b, V3 c, V3 &cp, int &region)
         {
             const V3 ab = b - a, ac = c - a, bc = c - b, ap = p - a, bp = p - b, cq = p - c;
             const float d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cq),
                         d6 = dot(ac, cq);
             if (d1 <= 0.0f && d2 <= 0.0f)
-                cp = a;
+                cp = a, region = kRegA;
             else if (d3 >= 0.0f && d3 >= d4)
-                cp = b;
+                cp = b, region = kRegB;
             else if (d6 >= 0.0f && d6 >= d5)
-                cp = c;
+                cp = c, region = kRegC;
             else
             {
                 const float vc = d1 * d4 - d3 * d2;
                 const float va = d3 * d6 - d5 * d4;
                 const float vb = d5 * d2 - d1 * d6;
                 if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f)
-                    cp = a + ab * (d1 / (d1 - d3));
+                    cp = a + ab * (d1 / (d1 - d3)), region = kRegAB;
                 else if (va <= 0.0f && d4 >= d3 && d5 >= d6)
                 {
                     const float unom = d4 - d3, udenom = d5 - d6;
-                    cp = b + bc * (unom / (unom + udenom));
+                    cp = b + bc * (unom / (unom + udenom)), region = kRegBC;
                 }
                 else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f)
-                    cp = a + ac * (d2 / (d2 - d6));
+                    cp = a + ac * (d2 / (d2 - d6)), region = kRegAC;
                 else
                 {
                     const float den = 1.0f / (va + vb + vc);
-                    cp = a + ab * (vb * den) + ac * (vc * den);
+                    cp = a + ab * (vb * den) + ac * (vc * den), region = kRegFace;
                 }
             }
             const V3 v = p - cp;
@@ -112,33 +122,105 @@ namespace crhip
         // into a unit box: shifted by the componentwise min(max(a, b), min(c, q))
         // -- not the box minimum, kept as written -- and scaled by
         // 1 / max(1e-12, largest extent) (aabb_tree.cpp:443-466).
-        CR_DEV float tri_closest_unit(V3 q, V3 a, V3 b, V3 c, V3 &cp)
+        CR_DEV float tri_closest_unit(V3 q, V3 a, V3 b, V3 c, V3 &cp, int &region)
         {
             const V3 lo = vmin(vmax(a, b), vmin(c, q));
             const V3 size = vmax(vmax(a, b), vmax(c, q)) - lo;
             const float inv = fmaxf(1e-12f, fmaxf(size.x, fmaxf(size.y, size.z)));
             const float s = 1.0f / inv;
             V3 u;
-            const float r = tri_closest((q - lo) * s, (a - lo) * s, (b - lo) * s, (c - lo) * s, u);
+            const float r = tri_closest((q - lo) * s, (a - lo) * s, (b - lo) * s, (c - lo) * s, u, region);
             cp = u * inv + lo;
             return r * inv * inv;
         }
 
+        CR_DEV float tri_closest_unit(V3 q, V3 a, V3 b, V3 c, V3 &cp)
+        {
+            int region;
+            return tri_closest_unit(q, a, b, c, cp, region);
+        }
+
+        // atan2 from an odd polynomial for atan on [0, 1] and the octant
+        // identities atan(t) = pi/2 - atan(1/t), atan2(y, -x) = pi - atan2(y, x).
+        // Coefficients fitted here (Lawson-reweighted least squares on [0, 1]);
+        // absolute error 1.4e-7 rad evaluated in FP32.  atan2(0, 0) = 0.
+        CR_DEV float atan2_poly(float y, float x)
+        {
+            const float ax = fabsf(x), ay = fabsf(y);
+            const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+            const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+            const float s = t * t;
+            float p = -0.004054564516991377f;
+            p = fmaf(p, s, 0.021862953901290894f);
+            p = fmaf(p, s, -0.055912334471940994f);
+            p = fmaf(p, s, 0.0964219942688942f);
+            p = fmaf(p, s, -0.1390863060951233f);
+            p = fmaf(p, s, 0.19946566224098206f);
+            p = fmaf(p, s, -0.33329862356185913f);
+            p = fmaf(p, s, 0.9999993443489075f);
+            float r = p * t;
+            r = ay > ax ? 1.5707963267948966f - r : r;
+            r = x < 0.0f ? 3.141592653589793f - r : r;
+            return copysignf(r, y);
+        }
+
         // Solid angle of triangle abc seen from p over 4 pi (Van Oosterom and
-        // Strackee), in the reference's expression order (evaluateExact,
-        // include/geometry_utils/winding_number.h:79-93).
+        // Strackee): the term of the exact winding number
+        // (computeWindingNumberExact / evaluateExact,
+        // include/geometry_utils/winding_number.h:79-93, 136-146), evaluated
+        // with FMAs, the hardware square root and atan2_poly.  Only its sum's
+        // side of 0.5 is used, and the sum differs from the reference's FP32
+        // evaluation by about 1e-7 per term.
         CR_DEV float winding_term(V3 a, V3 b, V3 c, V3 p)
         {
             const float k = float(0.5 / 3.141592653589793238462643383);
             a = a - p;
             b = b - p;
             c = c - p;
-            const float la = norm(a), lb = norm(b), lc = norm(c);
-            const float y = a.x * b.y * c.z - a.x * b.z * c.y - a.y * b.x * c.z + a.y * b.z * c.x + a.z * b.x * c.y -
-                            a.z * b.y * c.x;
-            const float x = (la * lb * lc + (a.x * b.x + a.y * b.y + a.z * b.z) * lc +
-                             (b.x * c.x + b.y * c.y + b.z * c.z) * la + (c.x * a.x + c.y * a.y + c.z * a.z) * lb);
-            return k * atan2f(y, x);
+            const float la = __builtin_amdgcn_sqrtf(fmaf(a.x, a.x, fmaf(a.y, a.y, a.z * a.z)));
+            const float lb = __builtin_amdgcn_sqrtf(fmaf(b.x, b.x, fmaf(b.y, b.y, b.z * b.z)));
+            const float lc = __builtin_amdgcn_sqrtf(fmaf(c.x, c.x, fmaf(c.y, c.y, c.z * c.z)));
+            const float bcx = fmaf(b.y, c.z, -b.z * c.y), bcy = fmaf(b.z, c.x, -b.x * c.z),
+                        bcz = fmaf(b.x, c.y, -b.y * c.x);
+            const float y = fmaf(a.x, bcx, fmaf(a.y, bcy, a.z * bcz));
+            const float ab = fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+            const float bc = fmaf(b.x, c.x, fmaf(b.y, c.y, b.z * c.z));
+            const float ca = fmaf(c.x, a.x, fmaf(c.y, a.y, c.z * a.z));
+            const float x = fmaf(ca, lb, fmaf(bc, la, fmaf(ab, lc, la * lb * lc)));
+            return k * atan2_poly(y, x);
+        }
+
+        // Squared gap between two boxes (0 where they overlap): a lower bound
+        // of the squared distance between any points of the two.
+        CR_DEV float box_gap2(V3 lo1, V3 hi1, V3 lo2, V3 hi2)
+        {
+            const float gx = fmaxf(0.0f, fmaxf(lo2.x - hi1.x, lo1.x - hi2.x));
+            const float gy = fmaxf(0.0f, fmaxf(lo2.y - hi1.y, lo1.y - hi2.y));
+            const float gz = fmaxf(0.0f, fmaxf(lo2.z - hi1.z, lo1.z - hi2.z));
+            return gx * gx + gy * gy + gz * gz;
+        }
+
+        CR_DEV float wave_max(float v)
+        {
+            for (int o = 32; o > 0; o >>= 1)
+                v = fmaxf(v, __shfl_xor(v, o));
+            return v;
+        }
+
+        // Smallest d over the wavefront, ties to the smaller index; every lane
+        // ends with the same pair.
+        CR_DEV void wave_argmin(float &d, int &i)
+        {
+            for (int o = 32; o > 0; o >>= 1)
+            {
+                const float od = __shfl_xor(d, o);
+                const int oi = __shfl_xor(i, o);
+                if (od < d || (od == d && oi < i))
+                {
+                    d = od;
+                    i = oi;
+                }
+            }
         }
 
         CR_DEV V3 node_point(const MeshSdfArgs &a, int x, int y, int z)
@@ -149,9 +231,22 @@ namespace crhip
         // One workgroup: 4 bricks (one per wavefront) against one slice of the
         // triangles.  Writes the slice's closest point, squared distance and
         // winding-number sum per node.
+        //
+        // The distance evaluation is skipped, for the whole wavefront, for a
+        // triangle whose box lies farther from the brick's box than the
+        // largest squared distance any lane still has to beat (plus a relative
+        // 1e-5 and an absolute 1e-5 of the triangle's squared extent, far
+        // above the evaluation's rounding): no lane could take it.  The first
+        // bound comes from the triangle whose centroid is nearest the brick's
+        // centre, then from the lanes' running minima after each chunk.  The
+        // triangles are still visited in index order with a strict <, so the
+        // result is the one of the full evaluation.
+        // Slicer: the mesh-slicer table, unsigned, so no winding number; the
+        // closest triangle's index is kept instead.
+        template <bool Slicer>
         __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_slices(MeshSdfArgs a)
         {
-            __shared__ float4 tri[kSdfChunk * 3];
+            __shared__ float4 lds[kSdfChunk * 4];
             const int slice = blockIdx.x / a.groups;
             const int brick = (blockIdx.x - slice * a.groups) * (kSdfWG / 64) + (int)(threadIdx.x >> 6);
             const int lane = threadIdx.x & 63;
@@ -160,37 +255,75 @@ namespace crhip
             const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
             const bool valid = brick < a.nBricks && x < a.dx && y < a.dy && z < a.dz;
             const V3 q = node_point(a, x, y, z);
+            // the box of all 64 lanes' points, valid or not
+            const V3 blo = node_point(a, bx * 4, by * 4, bz * 4), bhi = node_point(a, bx * 4 + 3, by * 4 + 3, bz * 4 + 3);
 
             const int t0 = slice * a.sliceTris;
             const int t1 = min(a.numTris, t0 + a.sliceTris);
+
+            // seed: the slice's triangle with the centroid nearest the brick's centre
+            const V3 mid = (blo + bhi) * 0.5f;
+            float cd = kFltMax;
+            int ci = t0;
+            for (int base = t0; base < t1; base += 4 * kSdfChunk)
+            {
+                const int n = min(4 * kSdfChunk, t1 - base);
+                __syncthreads();
+                for (int i = threadIdx.x; i < n; i += kSdfWG)
+                    lds[i] = a.cen[base + i];
+                __syncthreads();
+                for (int t = lane; t < n; t += 64)
+                {
+                    const float d = sqn(v3(lds[t]) - mid);
+                    if (d < cd)
+                    {
+                        cd = d;
+                        ci = base + t;
+                    }
+                }
+            }
+            wave_argmin(cd, ci);
+            ci = __builtin_amdgcn_readfirstlane(ci);
+            V3 cp;
+            float bound = wave_max(tri_closest_unit(q, v3(a.tri[4 * ci]), v3(a.tri[4 * ci + 1]),
+                                                    v3(a.tri[4 * ci + 2]), cp));
+
             float best = kFltMax, wn = 0.0f;
+            int bestT = t0;
             V3 bestP = q;
             for (int base = t0; base < t1; base += kSdfChunk)
             {
                 const int n = min(kSdfChunk, t1 - base);
                 __syncthreads();
-                for (int i = threadIdx.x; i < 3 * n; i += kSdfWG)
-                    tri[i] = a.tri[3 * (long long)base + i];
+                for (int i = threadIdx.x; i < 4 * n; i += kSdfWG)
+                    lds[i] = a.tri[4 * (long long)base + i];
                 __syncthreads();
                 for (int t = 0; t < n; ++t)
                 {
-                    const V3 ta = v3(tri[3 * t]), tb = v3(tri[3 * t + 1]), tc = v3(tri[3 * t + 2]);
-                    V3 cp;
-                    const float d2 = tri_closest_unit(q, ta, tb, tc, cp);
-                    if (d2 < best)
+                    const float4 r0 = lds[4 * t], r1 = lds[4 * t + 1], r2 = lds[4 * t + 2], r3 = lds[4 * t + 3];
+                    const V3 ta = v3(r0), tb = v3(r1), tc = v3(r2);
+                    const float gap = box_gap2(blo, bhi, v3(r0.w, r1.w, r2.w), v3(r3));
+                    if (!__builtin_amdgcn_readfirstlane((int)(gap > fmaf(bound, 1.00001f, r3.w))))
                     {
-                        best = d2;
-                        bestP = cp;
+                        const float d2 = tri_closest_unit(q, ta, tb, tc, cp);
+                        if (d2 < best)
+                        {
+                            best = d2;
+                            bestP = cp;
+                            bestT = base + t;
+                        }
                     }
-                    wn += winding_term(ta, tb, tc, q);
+                    if (!Slicer)
+                        wn += winding_term(ta, tb, tc, q);
                 }
+                bound = fminf(bound, wave_max(best));
             }
             if (valid)
             {
                 const long long node = ((long long)z * a.dy + y) * a.dx + x;
                 const long long o = (long long)slice * a.count + node;
                 a.part[o] = make_float4(bestP.x, bestP.y, bestP.z, best);
-                a.partWn[o] = wn;
+                a.partWn[o] = Slicer ? __int_as_float(bestT) : wn;
             }
         }
 
@@ -198,7 +331,20 @@ namespace crhip
         // order wins, as in one sequential pass); signed distance
         // |closest - q|, negative where the winding number exceeds 0.5
         // (sdf_builder.cpp:141-157).
-        __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_combine(MeshSdfArgs a, int slices, float *dist)
+        //
+        // Mesh-slicer table: the unsigned distance, and whether the closest
+        // point lies on the surface: not on a boundary edge (one adjacent
+        // triangle) or a vertex of one, as distanceProjectionPointTriangleSquared
+        // classifies it (src/geometry_utils/aabb_tree.cpp:519-688), or the node
+        // is on the positive side of the spine plane (sdf_builder.cpp:226-236).
+        // The region is re-evaluated for the closest triangle with the same
+        // arithmetic.  The reference's tree walk drops the node when any
+        // candidate it accepts on the way falls on the boundary
+        // (aabb_tree.cpp:724-760); here only the closest one counts, so the
+        // nodes kept are a superset of the reference's.
+        template <bool Slicer>
+        __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_combine(MeshSdfArgs a, int slices, float *dist,
+                                                                    unsigned char *onMesh)
         {
             const long long node = (long long)blockIdx.x * kSdfWG + threadIdx.x;
             if (node >= a.count)
@@ -206,6 +352,7 @@ namespace crhip
             const int x = (int)(node % a.dx), y = (int)((node / a.dx) % a.dy), z = (int)(node / ((long long)a.dx * a.dy));
             const V3 q = node_point(a, x, y, z);
             float best = kFltMax, wn = 0.0f;
+            int bestT = 0;
             V3 bestP = q;
             for (int s = 0; s < slices; ++s)
             {
@@ -215,11 +362,32 @@ namespace crhip
                 {
                     best = p.w;
                     bestP = v3(p);
+                    bestT = __float_as_int(a.partWn[o]);
                 }
-                wn += a.partWn[o];
+                if (!Slicer)
+                    wn += a.partWn[o];
             }
             const float d = norm(bestP - q);
-            dist[node] = wn > 0.5f ? -d : d;
+            if (!Slicer)
+            {
+                dist[node] = wn > 0.5f ? -d : d;
+                return;
+            }
+            dist[node] = d;
+            V3 cp;
+            int region;
+            tri_closest_unit(q, v3(a.tri[4 * bestT]), v3(a.tri[4 * bestT + 1]), v3(a.tri[4 * bestT + 2]), cp, region);
+            const int f = a.flags[bestT];
+            bool off = region != kRegFace && ((f >> (region < 3 ? region + 3 : region - 3)) & 1);
+            if (!isnan(a.spine.x))
+            {
+                V3 n = v3(a.spine.x, a.spine.y, a.spine.z);
+                const float s2 = sqn(n);
+                n = s2 > 0.0f ? n * rsq(s2) : v3(0.0f, 0.0f, 0.0f);
+                if (dot(q, n) + a.spine.w > 0.0f)
+                    off = false;
+            }
+            onMesh[node] = off ? 0 : 1;
         }
 
         // computeSdfGradient (sdf_builder.cpp:242-330): central differences
@@ -233,12 +401,22 @@ namespace crhip
             return (w[at] - w[at - step]) * inv;
         }
 
+        // onMesh (mesh-slicer table): rows off the surface become NaN after the
+        // gradient is taken from all distances (postProcessModSdfGradient,
+        // sdf_builder.cpp:332-357).
         __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_gradient(int dx, int dy, int dz, long long count,
-                                                                     float cell, const float *dist, float4 *out)
+                                                                     float cell, const float *dist,
+                                                                     const unsigned char *onMesh, float4 *out)
         {
             const long long node = (long long)blockIdx.x * kSdfWG + threadIdx.x;
             if (node >= count)
                 return;
+            if (onMesh && !onMesh[node])
+            {
+                const float nan = __int_as_float(0x7fc00000);
+                out[node] = make_float4(nan, nan, nan, nan);
+                return;
+            }
             const int x = (int)(node % dx), y = (int)((node / dx) % dy), z = (int)(node / ((long long)dx * dy));
             const float inv = 1.0f / cell;
             V3 g = v3(sdf_diff(x, dx, dist, node, 1, inv), sdf_diff(y, dy, dist, node, dx, inv),
@@ -246,6 +424,51 @@ namespace crhip
             normalize(g);
             out[node] = make_float4(g.x, g.y, g.z, dist[node]);
         }
+
+        // Per triangle: boundary edges ab, bc, ac (bits 0-2) and boundary
+        // vertices a, b, c (bits 3-5); a boundary edge has one adjacent
+        // triangle, a boundary vertex ends one (buildEdgeTriangleMap /
+        // buildEdgeVerticesList, sdf_builder.cpp:45-82).
+        std::vector<int> boundary_flags(const unsigned *tris, int numTris)
+        {
+            auto key = [](unsigned u, unsigned v) {
+                return (unsigned long long)std::min(u, v) << 32 | std::max(u, v);
+            };
+            std::vector<unsigned long long> edges(3 * (size_t)numTris);
+            for (size_t t = 0; t < (size_t)numTris; ++t)
+                for (int j = 0; j < 3; ++j)
+                    edges[3 * t + j] = key(tris[3 * t + j], tris[3 * t + (j + 1) % 3]);
+            std::vector<unsigned long long> sorted(edges);
+            std::sort(sorted.begin(), sorted.end());
+            auto single = [&](unsigned long long e) {
+                auto r = std::equal_range(sorted.begin(), sorted.end(), e);
+                return r.second - r.first < 2;
+            };
+            std::vector<unsigned> bverts;
+            for (size_t i = 0; i < sorted.size(); ++i)
+                if ((i == 0 || sorted[i] != sorted[i - 1]) && single(sorted[i]))
+                {
+                    bverts.push_back((unsigned)(sorted[i] >> 32));
+                    bverts.push_back((unsigned)(sorted[i] & 0xffffffffu));
+                }
+            std::sort(bverts.begin(), bverts.end());
+            std::vector<int> flags(numTris, 0);
+            for (size_t t = 0; t < (size_t)numTris; ++t)
+            {
+                const unsigned *v = tris + 3 * t;
+                int f = 0;
+                // edges in the order ab, bc, ca: bit 2 is edge ac
+                for (int j = 0; j < 3; ++j)
+                    if (single(edges[3 * t + j]))
+                        f |= 1 << j;
+                for (int j = 0; j < 3; ++j)
+                    if (std::binary_search(bverts.begin(), bverts.end(), v[j]))
+                        f |= 8 << j;
+                flags[t] = f;
+            }
+            return flags;
+        }
+
     } // namespace
 } // namespace crhip
 
@@ -290,76 +513,153 @@ int crHipMeshSdfDesc(const float *points, int numPoints, const unsigned int *tri
     return count >= (1ll << 31) ? CRMPM_HIP_ERR_CAPACITY : CRMPM_HIP_OK;
 }
 
+} // extern "C"
+
+namespace
+{
+    // Both tables: the triangle-mesh SDF (slicer false: signed distance,
+    // rows at out) and the mesh-slicer table (spine row at out, then the
+    // grid).  Synchronous on stream.
+    int build_mesh_table(bool slicer, const float *points, int numPoints, const unsigned *triangles,
+                         int numTriangles, float cellSize, float fatten, const float *spine, int dims[3],
+                         float lowerBound[3], float *out, unsigned long long capacity, void *stream)
+    {
+        int rc = crHipMeshSdfDesc(points, numPoints, triangles, numTriangles, cellSize, fatten, dims, lowerBound);
+        if (rc || !out)
+            return rc;
+        // one node along an axis leaves the reference's one-sided difference
+        // reading outside the table; such tables are refused
+        if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2)
+            return CRMPM_HIP_ERR_ARG;
+        const long long count = (long long)dims[0] * dims[1] * dims[2];
+        if (capacity < (unsigned long long)count + (slicer ? 1 : 0))
+            return CRMPM_HIP_ERR_CAPACITY;
+
+        MeshSdfArgs a{};
+        a.lower = make_float3(lowerBound[0], lowerBound[1], lowerBound[2]);
+        a.cell = cellSize;
+        a.dx = dims[0];
+        a.dy = dims[1];
+        a.dz = dims[2];
+        a.bx = (a.dx + 3) / 4;
+        a.by = (a.dy + 3) / 4;
+        a.nBricks = a.bx * a.by * ((a.dz + 3) / 4);
+        a.numTris = numTriangles;
+        a.count = count;
+        a.groups = (a.nBricks + kSdfWG / 64 - 1) / (kSdfWG / 64);
+        if (slicer)
+            a.spine = make_float4(spine[0], spine[1], spine[2], spine[3]);
+        const int chunks = (numTriangles + kSdfChunk - 1) / kSdfChunk;
+        const int slices = std::max(1, std::min(chunks, (kSdfTargetWGs + a.groups - 1) / a.groups));
+        a.sliceTris = ((numTriangles + slices - 1) / slices + 63) / 64 * 64;
+        const int usedSlices = (numTriangles + a.sliceTris - 1) / a.sliceTris;
+
+        // triangle records gathered on the host: corners with the box's lower
+        // corner in .w, then the upper corner and the culling slack (1e-5 of
+        // the squared largest extent); centroids apart
+        std::vector<float4> tri(4 * (size_t)numTriangles), cen(numTriangles);
+        for (size_t t = 0; t < (size_t)numTriangles; ++t)
+        {
+            const float *v[3];
+            for (int k = 0; k < 3; ++k)
+                v[k] = points + 3 * (size_t)triangles[3 * t + k];
+            float lo[3], hi[3], c[3], ext = 0.0f;
+            for (int k = 0; k < 3; ++k)
+            {
+                lo[k] = std::min(v[0][k], std::min(v[1][k], v[2][k]));
+                hi[k] = std::max(v[0][k], std::max(v[1][k], v[2][k]));
+                c[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0f;
+                ext = std::max(ext, hi[k] - lo[k]);
+            }
+            for (int k = 0; k < 3; ++k)
+                tri[4 * t + k] = make_float4(v[k][0], v[k][1], v[k][2], lo[k]);
+            tri[4 * t + 3] = make_float4(hi[0], hi[1], hi[2], 1e-5f * ext * ext);
+            cen[t] = make_float4(c[0], c[1], c[2], 0.0f);
+        }
+        std::vector<int> flags;
+        if (slicer)
+            flags = boundary_flags(triangles, numTriangles);
+        const float spineRow[4] = {slicer ? spine[0] : 0.0f, slicer ? spine[1] : 0.0f, slicer ? spine[2] : 0.0f,
+                                   slicer ? spine[3] : 0.0f};
+
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        float4 *dTri = nullptr, *dCen = nullptr, *dPart = nullptr;
+        float *dWn = nullptr, *dDist = nullptr;
+        int *dFlags = nullptr;
+        unsigned char *dMask = nullptr;
+        rc = CRMPM_HIP_ERR_ALLOC;
+        if (hipMalloc(&dTri, tri.size() * sizeof(float4)) == hipSuccess &&
+            hipMalloc(&dCen, cen.size() * sizeof(float4)) == hipSuccess &&
+            hipMalloc(&dPart, (size_t)usedSlices * count * sizeof(float4)) == hipSuccess &&
+            hipMalloc(&dWn, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
+            hipMalloc(&dDist, (size_t)count * sizeof(float)) == hipSuccess &&
+            (!slicer || (hipMalloc(&dFlags, flags.size() * sizeof(int)) == hipSuccess &&
+                         hipMalloc(&dMask, (size_t)count) == hipSuccess)))
+        {
+            rc = CRMPM_HIP_ERR_LAUNCH;
+            a.tri = dTri;
+            a.cen = dCen;
+            a.part = dPart;
+            a.partWn = dWn;
+            a.flags = dFlags;
+            const int nodeWGs = (int)((count + kSdfWG - 1) / kSdfWG);
+            float4 *rows = reinterpret_cast<float4 *>(out) + (slicer ? 1 : 0);
+            bool ok = hipMemcpyAsync(dTri, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice, st) ==
+                          hipSuccess &&
+                      hipMemcpyAsync(dCen, cen.data(), cen.size() * sizeof(float4), hipMemcpyHostToDevice, st) ==
+                          hipSuccess &&
+                      (!slicer ||
+                       (hipMemcpyAsync(dFlags, flags.data(), flags.size() * sizeof(int), hipMemcpyHostToDevice, st) ==
+                            hipSuccess &&
+                        hipMemcpyAsync(out, spineRow, sizeof(spineRow), hipMemcpyHostToDevice, st) == hipSuccess));
+            if (ok)
+            {
+                if (slicer)
+                {
+                    k_mesh_sdf_slices<true><<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
+                    k_mesh_sdf_combine<true><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, dMask);
+                }
+                else
+                {
+                    k_mesh_sdf_slices<false><<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
+                    k_mesh_sdf_combine<false><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, nullptr);
+                }
+                k_mesh_sdf_gradient<<<nodeWGs, kSdfWG, 0, st>>>(a.dx, a.dy, a.dz, count, cellSize, dDist, dMask, rows);
+                ok = hipGetLastError() == hipSuccess;
+            }
+            // the copies read host memory: wait for them whatever failed
+            if (hipStreamSynchronize(st) == hipSuccess && ok)
+                rc = CRMPM_HIP_OK;
+        }
+        (void)hipFree(dTri);
+        (void)hipFree(dCen);
+        (void)hipFree(dPart);
+        (void)hipFree(dWn);
+        (void)hipFree(dDist);
+        (void)hipFree(dFlags);
+        (void)hipFree(dMask);
+        return rc;
+    }
+} // namespace
+
+extern "C" {
+
 int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles, float cellSize,
                  float fatten, int dims[3], float lowerBound[3], float *gradientSignedDistance,
                  unsigned long long capacity, void *stream)
 {
-    int rc = crHipMeshSdfDesc(points, numPoints, triangles, numTriangles, cellSize, fatten, dims, lowerBound);
-    if (rc || !gradientSignedDistance)
-        return rc;
-    // one node along an axis leaves the reference's one-sided difference
-    // reading outside the table; such tables are refused
-    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2)
+    return build_mesh_table(false, points, numPoints, triangles, numTriangles, cellSize, fatten, nullptr, dims,
+                            lowerBound, gradientSignedDistance, capacity, stream);
+}
+
+int crHipMeshSlicerSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                       float cellSize, float fatten, const float spineArea[4], int dims[3], float lowerBound[3],
+                       float *rows, unsigned long long capacity, void *stream)
+{
+    if (!spineArea)
         return CRMPM_HIP_ERR_ARG;
-    const long long count = (long long)dims[0] * dims[1] * dims[2];
-    if (capacity < (unsigned long long)count)
-        return CRMPM_HIP_ERR_CAPACITY;
-
-    MeshSdfArgs a{};
-    a.lower = make_float3(lowerBound[0], lowerBound[1], lowerBound[2]);
-    a.cell = cellSize;
-    a.dx = dims[0];
-    a.dy = dims[1];
-    a.dz = dims[2];
-    a.bx = (a.dx + 3) / 4;
-    a.by = (a.dy + 3) / 4;
-    a.nBricks = a.bx * a.by * ((a.dz + 3) / 4);
-    a.numTris = numTriangles;
-    a.count = count;
-    a.groups = (a.nBricks + kSdfWG / 64 - 1) / (kSdfWG / 64);
-    const int chunks = (numTriangles + kSdfChunk - 1) / kSdfChunk;
-    const int slices = std::max(1, std::min(chunks, (kSdfTargetWGs + a.groups - 1) / a.groups));
-    a.sliceTris = ((numTriangles + slices - 1) / slices + 63) / 64 * 64;
-    const int usedSlices = (numTriangles + a.sliceTris - 1) / a.sliceTris;
-
-    // triangle corners gathered on the host: 3 float4 per triangle
-    std::vector<float4> tri(3 * (size_t)numTriangles);
-    for (size_t t = 0; t < 3 * (size_t)numTriangles; ++t)
-    {
-        const float *p = points + 3 * (size_t)triangles[t];
-        tri[t] = make_float4(p[0], p[1], p[2], 0.0f);
-    }
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    float4 *dTri = nullptr, *dPart = nullptr;
-    float *dWn = nullptr, *dDist = nullptr;
-    rc = CRMPM_HIP_ERR_ALLOC;
-    if (hipMalloc(&dTri, tri.size() * sizeof(float4)) == hipSuccess &&
-        hipMalloc(&dPart, (size_t)usedSlices * count * sizeof(float4)) == hipSuccess &&
-        hipMalloc(&dWn, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
-        hipMalloc(&dDist, (size_t)count * sizeof(float)) == hipSuccess)
-    {
-        rc = CRMPM_HIP_ERR_LAUNCH;
-        a.tri = dTri;
-        a.part = dPart;
-        a.partWn = dWn;
-        const int nodeWGs = (int)((count + kSdfWG - 1) / kSdfWG);
-        if (hipMemcpyAsync(dTri, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice, st) == hipSuccess)
-        {
-            k_mesh_sdf_slices<<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
-            k_mesh_sdf_combine<<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist);
-            k_mesh_sdf_gradient<<<nodeWGs, kSdfWG, 0, st>>>(a.dx, a.dy, a.dz, count, cellSize, dDist,
-                                                            reinterpret_cast<float4 *>(gradientSignedDistance));
-            const bool launched = hipGetLastError() == hipSuccess;
-            // the copy reads the host vector: wait for it even after a failed launch
-            if (hipStreamSynchronize(st) == hipSuccess && launched)
-                rc = CRMPM_HIP_OK;
-        }
-    }
-    (void)hipFree(dTri);
-    (void)hipFree(dPart);
-    (void)hipFree(dWn);
-    (void)hipFree(dDist);
-    return rc;
+    return build_mesh_table(true, points, numPoints, triangles, numTriangles, cellSize, fatten, spineArea, dims,
+                            lowerBound, rows, capacity, stream);
 }
 
 } // extern "C"

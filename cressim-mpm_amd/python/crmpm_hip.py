@@ -56,6 +56,10 @@ EXPORTS = {
                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "crHipMeshSdf": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong, _vp],
+    "crHipMeshSlicerSdf": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                           ctypes.POINTER(ctypes.c_float),
+                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong,
+                           _vp],
     "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
     "crHipSolverDestroy": [_vp],
     "crHipSolverGetGridDims": [_vp, ctypes.POINTER(ctypes.c_int)],
@@ -199,6 +203,29 @@ def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0"):
     n = dims[0] * dims[1] * dims[2]
     out = torch.empty((n, 4), dtype=torch.float32, device=dev)
     check(lib.crHipMeshSdf(*args, _vp(out.data_ptr()), n, _vp(stream)), "mesh sdf")
+    return np.array(dims[:], np.int32), np.array(lower[:], np.float32), out
+
+
+def mesh_slicer_sdf(points, triangles, cell_size, fatten, spine, device="cuda:0"):
+    """The mesh-slicer table on the device (crHipMeshSlicerSdf): dims, lower
+    bound and a (dx*dy*dz + 1, 4) float32 device tensor, the spine area in
+    row 0."""
+    import torch
+
+    lib = load_library()
+    dev = torch.device(device)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    pts = np.ascontiguousarray(points, np.float32)
+    tris = np.ascontiguousarray(triangles, np.uint32)
+    dims = (ctypes.c_int * 3)()
+    lower = (ctypes.c_float * 3)()
+    check(lib.crHipMeshSdfDesc(_vp(pts.ctypes.data), len(pts), _vp(tris.ctypes.data), len(tris), float(cell_size),
+                               float(fatten), dims, lower), "mesh sdf desc")
+    n = dims[0] * dims[1] * dims[2] + 1
+    out = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    check(lib.crHipMeshSlicerSdf(_vp(pts.ctypes.data), len(pts), _vp(tris.ctypes.data), len(tris), float(cell_size),
+                                 float(fatten), _f4(*spine), dims, lower, _vp(out.data_ptr()), n, _vp(stream)),
+          "mesh slicer sdf")
     return np.array(dims[:], np.int32), np.array(lower[:], np.float32), out
 
 

@@ -288,6 +288,21 @@ int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangl
                  float fatten, int dims[3], float lowerBound[3], float *gradientSignedDistance,
                  unsigned long long capacity, void *stream);
 
+/* The mesh-slicer table the engine builds for a triangle-mesh slicer geometry
+ * (src/engine/simulation_factory_impl.cpp:653-690): SdfBuilder::computeModSdf
+ * + computeSdfGradient + postProcessModSdfGradient (sdf_builder.cpp:162-240,
+ * 242-357).  rows (device float4[dx*dy*dz + 1]) receives the spine area
+ * (normal xyz, offset w; x NaN for none) in row 0, then per node the unsigned
+ * distance (w) and its normalised gradient (xyz), all four NaN where the
+ * closest point lies on a boundary edge or boundary vertex of the mesh (a
+ * boundary edge has one adjacent triangle) unless the node is on the positive
+ * side of the spine plane.  The reference also drops nodes whose tree walk
+ * passed a boundary candidate before the closest one; the rows kept here are
+ * a superset of its.  Errors as crHipMeshSdf; capacity counts the spine row. */
+int crHipMeshSlicerSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                       float cellSize, float fatten, const float spineArea[4], int dims[3], float lowerBound[3],
+                       float *rows, unsigned long long capacity, void *stream);
+
 /* ------------------------------------------------------------------------
  * Particle lattice of a particle object on the device:
  * ParticleObjectImpl::initialize (src/engine/particle_object_impl.cpp:36-88),

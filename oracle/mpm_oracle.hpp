@@ -19,6 +19,7 @@
 //     holding raw momentum (src/solver/mls_mpm_algorithm.h:264-266).
 #pragma once
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -1575,35 +1576,41 @@ namespace orc
     // Closest point of triangle abc to p, region by region (Ericson, Real-Time
     // Collision Detection 5.1.5), as src/geometry_utils/aabb_tree.cpp:336-439
     // evaluates it; returns the squared distance.
-    inline float tri_closest(V3<float> p, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp)
+    // Regions of the closest point: vertices a, b, c, edges ab, bc, ac, face.
+    enum : int { kRegA = 0, kRegB, kRegC, kRegAB, kRegBC, kRegAC, kRegFace };
+
+    inline float tri_closest(V3<float> p, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp, int *region = nullptr)
     {
         const V3<float> ab = b - a, ac = c - a, bc = c - b, ap = p - a, bp = p - b, cpv = p - c;
         const float d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cpv),
                     d6 = dot(ac, cpv);
         const float unom = d4 - d3, udenom = d5 - d6;
+        int r;
         if (d1 <= 0.0f && d2 <= 0.0f)
-            cp = a;
+            cp = a, r = kRegA;
         else if (d3 >= 0.0f && d3 >= d4)
-            cp = b;
+            cp = b, r = kRegB;
         else if (d6 >= 0.0f && d6 >= d5)
-            cp = c;
+            cp = c, r = kRegC;
         else
         {
             const float vc = d1 * d4 - d3 * d2;
             const float va = d3 * d6 - d5 * d4;
             const float vb = d5 * d2 - d1 * d6;
             if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f)
-                cp = a + ab * (d1 / (d1 - d3));
+                cp = a + ab * (d1 / (d1 - d3)), r = kRegAB;
             else if (va <= 0.0f && d4 >= d3 && d5 >= d6)
-                cp = b + bc * (unom / (unom + udenom));
+                cp = b + bc * (unom / (unom + udenom)), r = kRegBC;
             else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f)
-                cp = a + ac * (d2 / (d2 - d6));
+                cp = a + ac * (d2 / (d2 - d6)), r = kRegAC;
             else
             {
                 const float den = 1.0f / (va + vb + vc);
-                cp = a + ab * (vb * den) + ac * (vc * den);
+                cp = a + ab * (vb * den) + ac * (vc * den), r = kRegFace;
             }
         }
+        if (region)
+            *region = r;
         const V3<float> v = p - cp;
         return dot(v, v);
     }
@@ -1611,7 +1618,8 @@ namespace orc
     // The same in a unit box: everything shifted by the componentwise
     // min(max(a, b), min(c, q)) and scaled by 1 / max(1e-12, largest extent)
     // (aabb_tree.cpp:443-466; the shift is not the box minimum, kept as written).
-    inline float tri_closest_unit(V3<float> q, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp)
+    inline float tri_closest_unit(V3<float> q, V3<float> a, V3<float> b, V3<float> c, V3<float> &cp,
+                                  int *region = nullptr)
     {
         auto vmin = [](V3<float> u, V3<float> v) {
             return V3<float>(std::fmin(u.x, v.x), std::fmin(u.y, v.y), std::fmin(u.z, v.z));
@@ -1624,7 +1632,7 @@ namespace orc
         const float inv = std::fmax(1e-12f, std::fmax(size.x, std::fmax(size.y, size.z)));
         const float s = 1.0f / inv;
         V3<float> u;
-        const float r = tri_closest((q - lo) * s, (a - lo) * s, (b - lo) * s, (c - lo) * s, u);
+        const float r = tri_closest((q - lo) * s, (a - lo) * s, (b - lo) * s, (c - lo) * s, u, region);
         cp = u * inv + lo;
         return r * inv * inv;
     }
@@ -1682,6 +1690,9 @@ namespace orc
     // the reference thresholds its far-field approximation of the same number
     // (beta = 2, sdf_builder.cpp:150), which differs from it only by the
     // approximation error, i.e. at nodes on the surface itself.
+    // computeSdfGradient (sdf_builder.cpp:242-330) over the .w column of grid4.
+    inline void sdf_gradient(const int *dims, float cell, float *grid4);
+
     inline void mesh_sdf(const float *pts, const unsigned *tris, int numTris, float cell, const int *dims,
                          const float *lower, float *grid4)
     {
@@ -1709,6 +1720,118 @@ namespace orc
                     const float dist = norm(bestP - q);
                     grid4[4 * ((z * dy + y) * dx + x) + 3] = wn > 0.5f ? -dist : dist;
                 }
+        sdf_gradient(dims, cell, grid4);
+    }
+
+    // The mesh-slicer table (src/engine/simulation_factory_impl.cpp:670-690):
+    // row 0 holds the spine area (a plane: normal xyz, offset w; x NaN for
+    // none), rows 1.. the grid of SdfBuilder::computeModSdf +
+    // computeSdfGradient + postProcessModSdfGradient (sdf_builder.cpp:162-240,
+    // 242-330, 332-357): the unsigned distance to the closest triangle and its
+    // normalised gradient, the whole row NaN where the closest point is not
+    // on the surface -- where it falls on a boundary edge (one adjacent
+    // triangle) or a vertex of one (distanceProjectionPointTriangleSquared,
+    // src/geometry_utils/aabb_tree.cpp:519-688) -- unless the node lies on
+    // the positive side of the spine plane (q . normalize(n) + w > 0).
+    // The closest triangle is the first in index order, as in mesh_sdf.  The
+    // reference marks the node off the surface when any candidate its tree
+    // walk accepts on the way (d <= the best so far, aabb_tree.cpp:724-760)
+    // falls on the boundary, so it may drop a node whose final closest point
+    // is on the surface near the boundary; here only the final one counts.
+    inline void mesh_slicer_sdf(const float *pts, const unsigned *tris, int numTris, float cell, const int *dims,
+                                const float *lower, const float *spine, float *rows4)
+    {
+        const int dx = dims[0], dy = dims[1], dz = dims[2];
+        auto P = [&](unsigned i) { return V3<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); };
+        // boundary edges: one adjacent triangle (buildEdgeTriangleMap /
+        // buildEdgeVerticesList, sdf_builder.cpp:45-82)
+        std::vector<std::pair<uint64_t, int>> edges;
+        edges.reserve(3 * numTris);
+        auto key = [](unsigned u, unsigned v) { return (uint64_t)std::min(u, v) << 32 | std::max(u, v); };
+        for (int t = 0; t < numTris; ++t)
+            for (int j = 0; j < 3; ++j)
+                edges.push_back({key(tris[3 * t + j], tris[3 * t + (j + 1) % 3]), t});
+        std::sort(edges.begin(), edges.end());
+        std::vector<uint64_t> boundaryEdges;
+        std::vector<unsigned> boundaryVerts;
+        for (size_t i = 0; i < edges.size();)
+        {
+            size_t j = i;
+            while (j < edges.size() && edges[j].first == edges[i].first)
+                ++j;
+            if (j - i < 2)
+            {
+                boundaryEdges.push_back(edges[i].first);
+                boundaryVerts.push_back((unsigned)(edges[i].first >> 32));
+                boundaryVerts.push_back((unsigned)(edges[i].first & 0xffffffffu));
+            }
+            i = j;
+        }
+        std::sort(boundaryVerts.begin(), boundaryVerts.end());
+        auto isBE = [&](unsigned u, unsigned v) {
+            return std::binary_search(boundaryEdges.begin(), boundaryEdges.end(), key(u, v));
+        };
+        auto isBV = [&](unsigned u) { return std::binary_search(boundaryVerts.begin(), boundaryVerts.end(), u); };
+        const bool hasSpine = !std::isnan(spine[0]);
+        V3<float> sn(spine[0], spine[1], spine[2]);
+        {
+            const float s2 = sqn(sn);
+            sn = s2 > 0.0f ? sn * rsq(s2) : V3<float>();
+        }
+        for (int k = 0; k < 4; ++k)
+            rows4[k] = spine[k];
+        float *grid4 = rows4 + 4;
+        std::vector<char> onMesh((size_t)dx * dy * dz);
+        for (int z = 0; z < dz; ++z)
+            for (int y = 0; y < dy; ++y)
+                for (int x = 0; x < dx; ++x)
+                {
+                    const V3<float> q(lower[0] + float(x) * cell, lower[1] + float(y) * cell, lower[2] + float(z) * cell);
+                    float best = FLT_MAX;
+                    int bestT = 0;
+                    V3<float> bestP;
+                    for (int t = 0; t < numTris; ++t)
+                    {
+                        V3<float> cp;
+                        const float d2 = tri_closest_unit(q, P(tris[3 * t]), P(tris[3 * t + 1]), P(tris[3 * t + 2]), cp);
+                        if (d2 < best)
+                        {
+                            best = d2;
+                            bestT = t;
+                            bestP = cp;
+                        }
+                    }
+                    const size_t node = ((size_t)z * dy + y) * dx + x;
+                    grid4[4 * node + 3] = norm(bestP - q);
+                    V3<float> cp;
+                    int region;
+                    const unsigned *tv = tris + 3 * bestT;
+                    tri_closest_unit(q, P(tv[0]), P(tv[1]), P(tv[2]), cp, &region);
+                    bool off = false;
+                    switch (region)
+                    {
+                    case kRegA: off = isBV(tv[0]); break;
+                    case kRegB: off = isBV(tv[1]); break;
+                    case kRegC: off = isBV(tv[2]); break;
+                    case kRegAB: off = isBE(tv[0], tv[1]); break;
+                    case kRegBC: off = isBE(tv[1], tv[2]); break;
+                    case kRegAC: off = isBE(tv[0], tv[2]); break;
+                    default: break;
+                    }
+                    if (hasSpine && dot(q, sn) + spine[3] > 0.0f)
+                        off = false;
+                    onMesh[node] = !off;
+                }
+        sdf_gradient(dims, cell, grid4);
+        for (size_t i = 0; i < onMesh.size(); ++i)
+            if (!onMesh[i])
+                for (int k = 0; k < 4; ++k)
+                    grid4[4 * i + k] = std::nanf("0");
+    }
+
+    inline void sdf_gradient(const int *dims, float cell, float *grid4)
+    {
+        const int dx = dims[0], dy = dims[1], dz = dims[2];
         const float inv = 1.0f / cell;
         auto W = [&](int x, int y, int z) { return grid4[4 * ((z * dy + y) * dx + x) + 3]; };
         auto diff = [&](int i, int n, float lo, float mid, float hi) {

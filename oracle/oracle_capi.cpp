@@ -287,6 +287,22 @@ extern "C"
         return n;
     }
 
+    // Mesh-slicer table (src/engine/simulation_factory_impl.cpp:670-690):
+    // dims and lower as above; count + 1 rows (spine area, then the grid)
+    // when they fit capacity.  Returns the grid's node count, -1 for a
+    // vertex index beyond numPoints.
+    int orc_mesh_slicer_sdf(const float *pts, int numPoints, const unsigned *tris, int numTris, float cell,
+                            float fatten, const float *spine4, int *dims3, float *lower3, float *rows4, int capacity)
+    {
+        for (int t = 0; t < 3 * numTris; ++t)
+            if (tris[t] >= static_cast<unsigned>(numPoints))
+                return -1;
+        const int n = orc::mesh_sdf_desc(pts, tris, numTris, cell, fatten, dims3, lower3);
+        if (rows4 && n + 1 <= capacity)
+            orc::mesh_slicer_sdf(pts, tris, numTris, cell, dims3, lower3, spine4, rows4);
+        return n;
+    }
+
     // svd3 on n column-major 3x3 matrices (include/math/svd3.h:1265-1275)
     void orc_svd3(int fp64, int n, const float *A9, double *U9, double *Vt9, double *S3)
     {

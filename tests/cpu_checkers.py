@@ -54,6 +54,8 @@ def _declare(lib, pre):
                                              ctypes.c_float, _f, _f, _f, ctypes.c_float, _f, ctypes.c_uint]),
         "mesh_sdf": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i, _f, _f,
                                     ctypes.c_int]),
+        "mesh_slicer_sdf": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f,
+                                           _i, _f, _f, ctypes.c_int]),
         "stress": (None, [ctypes.c_int, _f, _i, _f, _d]),
         "pb_constraint": (None, [ctypes.c_int, ctypes.c_float, _f, _i, _f, _f, _d]),
     }
@@ -253,6 +255,28 @@ def mesh_sdf(points, tris, cell, fatten, which="oracle"):
     grid = np.zeros((n, 4), np.float32)
     fn["mesh_sdf"](*args, P(grid, _f), n)
     return dims, lower, grid
+
+
+def mesh_slicer_sdf(points, tris, cell, fatten, spine, which="oracle"):
+    """Triangle mesh -> mesh-slicer table through the oracle or the
+    reference's host builder: dims, lower bound and (n + 1, 4) float32 rows
+    (spine area, then gradient|unsigned distance, NaN off the surface)."""
+    fn = load(which)
+    if fn is None:
+        raise RuntimeError("reference CPU build not present")
+    pts = np.ascontiguousarray(points, np.float32)
+    tr = np.ascontiguousarray(tris, np.uint32)
+    sp = np.ascontiguousarray(spine, np.float32)
+    dims = np.zeros(3, np.int32)
+    lower = np.zeros(3, np.float32)
+    args = (P(pts, _f), len(pts), P(tr, _u), len(tr), float(cell), float(fatten), P(sp, _f), P(dims, _i),
+            P(lower, _f))
+    n = fn["mesh_slicer_sdf"](*args, None, 0)
+    if n < 0:
+        raise ValueError("triangle vertex index out of range")
+    rows = np.zeros((n + 1, 4), np.float32)
+    fn["mesh_slicer_sdf"](*args, P(rows, _f), n + 1)
+    return dims, lower, rows
 
 
 def svd3(which, A, fp64=False):

@@ -76,6 +76,20 @@ def sdf_cases():
     yield "sdf_torus", *S.torus(0.1, 0.03, 48, 24), 0.01, 0.01
 
 
+def slicer_cases():
+    """(name, points, triangles, cellSize, fatten, spine area) of the
+    mesh-slicer table fixtures: assets/plane.obj with the cell size, fattening
+    and spine of the reference's slicer-geometry test
+    (tests/test_sdf_geometry/main.cpp:59-63), the open hemisphere without a
+    spine and the curved sheet with one."""
+    plane = load_obj(os.path.join(REF_ASSETS, "plane.obj"))
+    yield "slicer_plane_obj", *plane, 0.1, 0.2, (-1.0, 0.0, 0.0, 0.0)
+    yield "slicer_hemisphere_obj", *load_obj(os.path.join(REF_ASSETS, "hemisphere.obj")), 0.1, 0.1, \
+        (float("nan"), 0.0, 0.0, 0.0)
+    yield "slicer_curved_plane_obj", *load_obj(os.path.join(REF_ASSETS, "curved_plane.obj")), 0.1, 0.1, \
+        (0.0, 0.0, 1.0, -0.5)
+
+
 def input_digest(sc):
     h = hashlib.sha256()
     for k in ("xm", "v", "params", "type", "mask"):
@@ -118,6 +132,12 @@ def main():
         dims, lower, grid = CC.mesh_sdf(pts, tris, cell, fatten, which="reference")
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), points=pts, tris=tris, cell=np.float32(cell),
                             fatten=np.float32(fatten), dims=dims, lower=lower, grid=grid)
+        print(f"wrote {name}.npz", tuple(dims))
+    for name, pts, tris, cell, fatten, spine in slicer_cases():
+        dims, lower, rows = CC.mesh_slicer_sdf(pts, tris, cell, fatten, spine, which="reference")
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), points=pts, tris=tris, cell=np.float32(cell),
+                            fatten=np.float32(fatten), spine=np.asarray(spine, np.float32), dims=dims, lower=lower,
+                            rows=rows)
         print(f"wrote {name}.npz", tuple(dims))
     # SVD: the matrix of the reference's own SVD test (tests/test_svd/main.cpp:36,
     # column-major constructor) plus random, repeated-singular-value, negative

@@ -15,7 +15,8 @@ sys.path.insert(0, os.path.join(os.path.dirname(HERE), "cressim-mpm_amd", "pytho
 
 import cpu_checkers as CC  # noqa: E402
 import crmpm_scenes as S  # noqa: E402
-from test_mesh_sdf_oracle import FIXTURES, compare_tables, fixture, node_points, sign_stable  # noqa: E402
+from test_mesh_sdf_oracle import (FIXTURES, SLICER_FIXTURES, compare_slicer, compare_tables, fixture,  # noqa: E402
+                                  node_points, sign_stable)
 
 pytestmark = pytest.mark.gpu
 
@@ -96,3 +97,21 @@ def test_mesh_sdf_table_drives_the_lattice():
     n, pts = H.particle_lattice(obj(grid))
     assert n == n_ref and n > 0
     np.testing.assert_array_equal(pts.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("name", SLICER_FIXTURES)
+def test_mesh_slicer_sdf_matches_oracle_and_reference(name):
+    """Mesh-slicer table: bit-identical to the oracle (same closest triangle,
+    same boundary test, same NaN rows), and to the reference fixture as the
+    oracle is."""
+    import crmpm_hip as H
+
+    g = fixture(name)
+    cell, fatten = float(g["cell"]), float(g["fatten"])
+    print(f"{name}: {len(g['tris'])} triangles, dims {tuple(g['dims'])}", flush=True)
+    dims, lower, out = H.mesh_slicer_sdf(g["points"], g["tris"], cell, fatten, g["spine"])
+    rows = out.cpu().numpy()
+    compare_slicer(name, g, dims, lower, rows)
+    od, ol, orows = CC.mesh_slicer_sdf(g["points"], g["tris"], cell, fatten, g["spine"])
+    assert np.array_equal(dims, od) and np.array_equal(lower, ol)
+    np.testing.assert_array_equal(rows, orows)

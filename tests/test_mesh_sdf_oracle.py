@@ -1,4 +1,5 @@
-"""Mesh -> SDF table (SdfBuilder, src/geometry_utils/sdf_builder.cpp), CPU only.
+"""Mesh -> SDF tables (SdfBuilder, src/geometry_utils/sdf_builder.cpp), CPU only:
+the triangle-mesh SDF and the mesh-slicer table.
 
 The oracle's restatement (oracle/mpm_oracle.hpp, mesh_sdf) is pinned
 1. against the reference's own host builder, compiled from its sources
@@ -146,6 +147,63 @@ def test_single_triangle_against_sampling():
     assert diff.min() > -1e-5 and diff.max() < 1.5 * edge / m, (diff.min(), diff.max())
 
 
+# ---------------- mesh-slicer tables (computeModSdf) ----------------
+
+SLICER_FIXTURES = ["slicer_plane_obj", "slicer_hemisphere_obj", "slicer_curved_plane_obj"]
+# Rows the reference drops although their closest point is on the surface:
+# its tree walk marks a node off the surface when any candidate accepted on
+# the way lies on the boundary (src/geometry_utils/aabb_tree.cpp:724-760).
+# Measured on the fixtures: 0, 812 and 44 nodes; bounded here per fixture.
+SLICER_EXTRA_ROWS = {"slicer_plane_obj": 0, "slicer_hemisphere_obj": 812, "slicer_curved_plane_obj": 44}
+
+
+def compare_slicer(name, g, dims, lower, rows):
+    """Against the reference fixture: same dims, lower bound and spine row;
+    every row the reference keeps is kept, with |distance| equal up to tie
+    rounding and the same gradient where the neighbourhood is kept; rows kept
+    here but dropped by the reference bounded by SLICER_EXTRA_ROWS."""
+    assert np.array_equal(dims, g["dims"]) and np.array_equal(lower, g["lower"])
+    ref = g["rows"]
+    assert np.array_equal(rows[0], ref[0], equal_nan=True)
+    rn, on = np.isnan(ref[1:, 3]), np.isnan(rows[1:, 3])
+    assert not (on & ~rn).any(), (name, (on & ~rn).sum())
+    assert (rn & ~on).sum() <= SLICER_EXTRA_ROWS[name], (name, (rn & ~on).sum())
+    both = ~rn & ~on
+    assert np.abs(rows[1:, 3][both] - ref[1:, 3][both]).max() <= 1e-6
+    assert np.abs(rows[1:, :3][both] - ref[1:, :3][both]).max() <= 2e-4
+    assert np.isnan(rows[1:][on]).all() and not np.isnan(rows[1:][~on]).any()
+
+
+@pytest.mark.parametrize("name", SLICER_FIXTURES)
+def test_slicer_oracle_matches_reference_fixture(name):
+    g = fixture(name)
+    dims, lower, rows = CC.mesh_slicer_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]), g["spine"])
+    compare_slicer(name, g, dims, lower, rows)
+
+
+def test_slicer_plane_closed_form():
+    """The reference's 2 m square sheet (assets/plane.obj, y = 0) with the
+    spine x < 0 of its slicer test: distance |y| over the square and to its
+    rim beyond, rows kept over the square and on the spine side only."""
+    g = fixture("slicer_plane_obj")
+    cell = float(g["cell"])
+    dims, lower, rows = CC.mesh_slicer_sdf(g["points"], g["tris"], cell, float(g["fatten"]), g["spine"])
+    q = node_points(dims, lower, cell).astype(np.float64)
+    over = (np.abs(q[:, 0]) <= 1) & (np.abs(q[:, 2]) <= 1)
+    keep = over | (q[:, 0] < 0)
+    assert np.array_equal(~np.isnan(rows[1:, 3]), keep)
+    rim = np.maximum(np.abs(q[:, [0, 2]]) - 1, 0)
+    exact = np.sqrt(q[:, 1] ** 2 + (rim ** 2).sum(1))
+    assert np.abs(rows[1:, 3][keep] - exact[keep]).max() < 1e-6
+    # over the square, away from its rim and the table's faces, the gradient is the sheet's normal
+    inner = over & (np.abs(q[:, 0]) < 1 - cell) & (np.abs(q[:, 2]) < 1 - cell) & (np.abs(q[:, 1]) > cell / 2)
+    inner &= (q[:, 1] > q[:, 1].min() + cell / 2) & (q[:, 1] < q[:, 1].max() - cell / 2)
+    assert inner.sum() > 100
+    n = np.zeros((inner.sum(), 3))
+    n[:, 1] = np.sign(q[inner, 1])
+    assert np.abs(rows[1:][inner, :3] - n).max() < 1e-6
+
+
 # ---------------- the library's host-side entry points (no GPU) ----------------
 
 def _desc(lib, pts, tris, cell, fatten):
@@ -195,3 +253,10 @@ def test_capi_mesh_sdf_argument_errors():
     assert rc == 0 and d[2] == 1
     args = (H._vp(flat.ctypes.data), 3, H._vp(one.ctypes.data), 1, 0.1, 0.0, dims, lower)
     assert lib.crHipMeshSdf(*args, H._vp(16), 1 << 20, None) == 1
+    # the slicer table's capacity counts its spine row; no spine is an error
+    sp = H._f4(float("nan"), 0, 0, 0)
+    args = (H._vp(pts.ctypes.data), len(pts), H._vp(tris.ctypes.data), len(tris), 0.2, 0.3)
+    assert lib.crHipMeshSlicerSdf(*args, sp, dims, lower, None, 0, None) == 0
+    count = dims[0] * dims[1] * dims[2]
+    assert lib.crHipMeshSlicerSdf(*args, sp, dims, lower, H._vp(16), count, None) == 5
+    assert lib.crHipMeshSlicerSdf(*args, None, dims, lower, H._vp(16), count + 1, None) == 1
