@@ -52,11 +52,22 @@ namespace crhip
         constexpr int kSdfWG = 256;          // threads: 4 wavefronts, one 4x4x4 node brick each
         constexpr int kSdfChunk = 256;       // triangles staged in LDS per round
         constexpr int kSdfTargetWGs = 2048;  // slice the triangles until about this many workgroups
+        constexpr int kClu = 16;             // triangles per cluster (consecutive in index order)
+        constexpr int kCluPerChunk = kSdfChunk / kClu;
+        // A cluster of radius rho is summed by its far-field term for a brick
+        // at least kFar * rho from its centre.
+        constexpr float kFar = 6.0f;
+        // Winding numbers within this of 0.5 plus the far-field bound are
+        // summed again exactly.
+        constexpr float kWnSlack = 1e-3f;
 
         struct MeshSdfArgs
         {
             const float4 *tri;  // 4 float4 per triangle: (a, lo.x), (b, lo.y), (c, lo.z), (hi, cull slack)
             const float4 *cen;  // per triangle: centroid
+            const float4 *clu;  // 4 float4 per cluster: (centre, radius), (vector area / 4 pi,
+                                // 2 radius area / 4 pi), (box lo, cull slack), (box hi, 0)
+            float *partErr;     // per slice and node: bound on the far-field error of the winding number
             float4 *part;       // per slice and node: closest point, squared distance
             float *partWn;      // per slice and node: winding number of the slice's triangles
                                 // (mesh-slicer table: the closest triangle's index)
@@ -200,6 +211,15 @@ namespace crhip
             return gx * gx + gy * gy + gz * gz;
         }
 
+        // Squared distance from a point to a box (0 inside).
+        CR_DEV float point_box_gap2(V3 p, V3 lo, V3 hi)
+        {
+            const float gx = fmaxf(0.0f, fmaxf(lo.x - p.x, p.x - hi.x));
+            const float gy = fmaxf(0.0f, fmaxf(lo.y - p.y, p.y - hi.y));
+            const float gz = fmaxf(0.0f, fmaxf(lo.z - p.z, p.z - hi.z));
+            return gx * gx + gy * gy + gz * gz;
+        }
+
         CR_DEV float wave_max(float v)
         {
             for (int o = 32; o > 0; o >>= 1)
@@ -246,7 +266,7 @@ namespace crhip
         template <bool Slicer>
         __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_slices(MeshSdfArgs a)
         {
-            __shared__ float4 lds[kSdfChunk * 4];
+            __shared__ float4 lds[kSdfChunk * 4 + kCluPerChunk * 4];
             const int slice = blockIdx.x / a.groups;
             const int brick = (blockIdx.x - slice * a.groups) * (kSdfWG / 64) + (int)(threadIdx.x >> 6);
             const int lane = threadIdx.x & 63;
@@ -288,33 +308,83 @@ namespace crhip
             float bound = wave_max(tri_closest_unit(q, v3(a.tri[4 * ci]), v3(a.tri[4 * ci + 1]),
                                                     v3(a.tri[4 * ci + 2]), cp));
 
-            float best = kFltMax, wn = 0.0f;
+            float best = kFltMax, wn = 0.0f, err = 0.0f;
             int bestT = t0;
             V3 bestP = q;
+            float4 *cl = lds + 4 * kSdfChunk;
             for (int base = t0; base < t1; base += kSdfChunk)
             {
                 const int n = min(kSdfChunk, t1 - base);
+                const int nc = (n + kClu - 1) / kClu;
                 __syncthreads();
-                for (int i = threadIdx.x; i < 4 * n; i += kSdfWG)
-                    lds[i] = a.tri[4 * (long long)base + i];
+                for (int i = threadIdx.x; i < 4 * nc; i += kSdfWG)
+                    cl[i] = a.clu[4 * (long long)(base / kClu) + i];
                 __syncthreads();
-                for (int t = 0; t < n; ++t)
+                // per cluster: is any distance still to be evaluated, and is it
+                // near enough to need the exact winding terms?  Far clusters
+                // add their far-field term and its error bound here.
+                unsigned need = 0, nearMask = 0;
+                for (int k = 0; k < nc; ++k)
                 {
-                    const float4 r0 = lds[4 * t], r1 = lds[4 * t + 1], r2 = lds[4 * t + 2], r3 = lds[4 * t + 3];
-                    const V3 ta = v3(r0), tb = v3(r1), tc = v3(r2);
-                    const float gap = box_gap2(blo, bhi, v3(r0.w, r1.w, r2.w), v3(r3));
-                    if (!__builtin_amdgcn_readfirstlane((int)(gap > fmaf(bound, 1.00001f, r3.w))))
+                    const float4 c0 = cl[4 * k], c1 = cl[4 * k + 1], c2 = cl[4 * k + 2], c3 = cl[4 * k + 3];
+                    const bool dist = !(box_gap2(blo, bhi, v3(c2), v3(c3)) > fmaf(bound, 1.00001f, c2.w));
+                    bool near = false;
+                    if (!Slicer)
                     {
-                        const float d2 = tri_closest_unit(q, ta, tb, tc, cp);
-                        if (d2 < best)
+                        near = !(point_box_gap2(v3(c0), blo, bhi) > kFar * kFar * c0.w * c0.w);
+                        if (!near)
                         {
-                            best = d2;
-                            bestP = cp;
-                            bestT = base + t;
+                            // The cluster's solid angle from q is the sum over its
+                            // triangles of int_T n.(x - q) / |x - q|^3 dA; with x
+                            // replaced by the centre c it is N.(c - q) / R^3 (N the
+                            // vector area), and the integrand's gradient in x is at
+                            // most 2 / r^3, so the error is at most
+                            // 2 rho A / (R - rho)^3 for R = |c - q| > rho.
+                            const V3 v = v3(c0) - q;
+                            const float r2 = fmaf(v.x, v.x, fmaf(v.y, v.y, v.z * v.z));
+                            const float r = __builtin_amdgcn_sqrtf(r2);
+                            const float term = fmaf(c1.x, v.x, fmaf(c1.y, v.y, c1.z * v.z)) *
+                                               __builtin_amdgcn_rcpf(r2 * r);
+                            const float g = r - c0.w;
+                            wn += term;
+                            err += fmaf(c1.w, __builtin_amdgcn_rcpf(g * g * g), 1e-5f * fabsf(term));
                         }
                     }
-                    if (!Slicer)
-                        wn += winding_term(ta, tb, tc, q);
+                    need |= (unsigned)(dist || near) << k;
+                    nearMask |= (unsigned)near << k;
+                }
+                need = __builtin_amdgcn_readfirstlane(need);
+                nearMask = __builtin_amdgcn_readfirstlane(nearMask);
+                if (__syncthreads_or(need != 0))
+                {
+                    for (int i = threadIdx.x; i < 4 * n; i += kSdfWG)
+                        lds[i] = a.tri[4 * (long long)base + i];
+                    __syncthreads();
+                    while (need)
+                    {
+                        const int k = __builtin_ctz(need);
+                        need &= need - 1;
+                        const bool near = (nearMask >> k) & 1;
+                        for (int t = k * kClu; t < min(n, (k + 1) * kClu); ++t)
+                        {
+                            const float4 r0 = lds[4 * t], r1 = lds[4 * t + 1], r2 = lds[4 * t + 2],
+                                         r3 = lds[4 * t + 3];
+                            const V3 ta = v3(r0), tb = v3(r1), tc = v3(r2);
+                            const float gap = box_gap2(blo, bhi, v3(r0.w, r1.w, r2.w), v3(r3));
+                            if (!__builtin_amdgcn_readfirstlane((int)(gap > fmaf(bound, 1.00001f, r3.w))))
+                            {
+                                const float d2 = tri_closest_unit(q, ta, tb, tc, cp);
+                                if (d2 < best)
+                                {
+                                    best = d2;
+                                    bestP = cp;
+                                    bestT = base + t;
+                                }
+                            }
+                            if (!Slicer && near)
+                                wn += winding_term(ta, tb, tc, q);
+                        }
+                    }
                 }
                 bound = fminf(bound, wave_max(best));
             }
@@ -324,6 +394,8 @@ namespace crhip
                 const long long o = (long long)slice * a.count + node;
                 a.part[o] = make_float4(bestP.x, bestP.y, bestP.z, best);
                 a.partWn[o] = Slicer ? __int_as_float(bestT) : wn;
+                if (!Slicer)
+                    a.partErr[o] = err;
             }
         }
 
@@ -351,7 +423,7 @@ namespace crhip
                 return;
             const int x = (int)(node % a.dx), y = (int)((node / a.dx) % a.dy), z = (int)(node / ((long long)a.dx * a.dy));
             const V3 q = node_point(a, x, y, z);
-            float best = kFltMax, wn = 0.0f;
+            float best = kFltMax, wn = 0.0f, err = 0.0f;
             int bestT = 0;
             V3 bestP = q;
             for (int s = 0; s < slices; ++s)
@@ -365,11 +437,22 @@ namespace crhip
                     bestT = __float_as_int(a.partWn[o]);
                 }
                 if (!Slicer)
+                {
                     wn += a.partWn[o];
+                    err += a.partErr[o];
+                }
             }
             const float d = norm(bestP - q);
             if (!Slicer)
             {
+                // the far-field sum cannot decide the side of 0.5: sum every
+                // triangle's term exactly
+                if (!(fabsf(wn - 0.5f) > err + kWnSlack))
+                {
+                    wn = 0.0f;
+                    for (int t = 0; t < a.numTris; ++t)
+                        wn += winding_term(v3(a.tri[4 * t]), v3(a.tri[4 * t + 1]), v3(a.tri[4 * t + 2]), q);
+                }
                 dist[node] = wn > 0.5f ? -d : d;
                 return;
             }
@@ -576,6 +659,43 @@ namespace
             tri[4 * t + 3] = make_float4(hi[0], hi[1], hi[2], 1e-5f * ext * ext);
             cen[t] = make_float4(c[0], c[1], c[2], 0.0f);
         }
+        // clusters of kClu consecutive triangles: centre and radius of their
+        // box, vector area and area (for the far-field winding term and its
+        // error bound, both pre-divided by 4 pi), the box and its cull slack
+        const int numClu = (numTriangles + kClu - 1) / kClu;
+        std::vector<float4> clu(4 * (size_t)numClu);
+        const double k4pi = 1.0 / (4.0 * 3.141592653589793238462643383);
+        for (int k = 0; k < numClu; ++k)
+        {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, N[3] = {0, 0, 0}, A = 0;
+            for (int t = k * kClu; t < std::min(numTriangles, (k + 1) * kClu); ++t)
+            {
+                double v[3][3];
+                for (int j = 0; j < 3; ++j)
+                    for (int c = 0; c < 3; ++c)
+                    {
+                        v[j][c] = points[3 * (size_t)triangles[3 * (size_t)t + j] + c];
+                        lo[c] = std::min(lo[c], v[j][c]);
+                        hi[c] = std::max(hi[c], v[j][c]);
+                    }
+                const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+                const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+                const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                      e1[0] * e2[1] - e1[1] * e2[0]};
+                for (int c = 0; c < 3; ++c)
+                    N[c] += 0.5 * cr[c];
+                A += 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            }
+            const double rho = 0.5 * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                               (hi[2] - lo[2]) * (hi[2] - lo[2])) * (1.0 + 1e-6) + 1e-12;
+            const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+            clu[4 * k] = make_float4((float)(0.5 * (lo[0] + hi[0])), (float)(0.5 * (lo[1] + hi[1])),
+                                     (float)(0.5 * (lo[2] + hi[2])), (float)rho);
+            clu[4 * k + 1] = make_float4((float)(N[0] * k4pi), (float)(N[1] * k4pi), (float)(N[2] * k4pi),
+                                         (float)(2.0 * rho * A * k4pi * (1.0 + 1e-5)));
+            clu[4 * k + 2] = make_float4((float)lo[0], (float)lo[1], (float)lo[2], (float)(1e-5 * ext * ext));
+            clu[4 * k + 3] = make_float4((float)hi[0], (float)hi[1], (float)hi[2], 0.0f);
+        }
         std::vector<int> flags;
         if (slicer)
             flags = boundary_flags(triangles, numTriangles);
@@ -583,13 +703,15 @@ namespace
                                    slicer ? spine[3] : 0.0f};
 
         hipStream_t st = static_cast<hipStream_t>(stream);
-        float4 *dTri = nullptr, *dCen = nullptr, *dPart = nullptr;
-        float *dWn = nullptr, *dDist = nullptr;
+        float4 *dTri = nullptr, *dCen = nullptr, *dClu = nullptr, *dPart = nullptr;
+        float *dWn = nullptr, *dErr = nullptr, *dDist = nullptr;
         int *dFlags = nullptr;
         unsigned char *dMask = nullptr;
         rc = CRMPM_HIP_ERR_ALLOC;
         if (hipMalloc(&dTri, tri.size() * sizeof(float4)) == hipSuccess &&
             hipMalloc(&dCen, cen.size() * sizeof(float4)) == hipSuccess &&
+            hipMalloc(&dClu, clu.size() * sizeof(float4)) == hipSuccess &&
+            hipMalloc(&dErr, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
             hipMalloc(&dPart, (size_t)usedSlices * count * sizeof(float4)) == hipSuccess &&
             hipMalloc(&dWn, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
             hipMalloc(&dDist, (size_t)count * sizeof(float)) == hipSuccess &&
@@ -599,6 +721,8 @@ namespace
             rc = CRMPM_HIP_ERR_LAUNCH;
             a.tri = dTri;
             a.cen = dCen;
+            a.clu = dClu;
+            a.partErr = dErr;
             a.part = dPart;
             a.partWn = dWn;
             a.flags = dFlags;
@@ -607,6 +731,8 @@ namespace
             bool ok = hipMemcpyAsync(dTri, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice, st) ==
                           hipSuccess &&
                       hipMemcpyAsync(dCen, cen.data(), cen.size() * sizeof(float4), hipMemcpyHostToDevice, st) ==
+                          hipSuccess &&
+                      hipMemcpyAsync(dClu, clu.data(), clu.size() * sizeof(float4), hipMemcpyHostToDevice, st) ==
                           hipSuccess &&
                       (!slicer ||
                        (hipMemcpyAsync(dFlags, flags.data(), flags.size() * sizeof(int), hipMemcpyHostToDevice, st) ==
@@ -633,6 +759,8 @@ namespace
         }
         (void)hipFree(dTri);
         (void)hipFree(dCen);
+        (void)hipFree(dClu);
+        (void)hipFree(dErr);
         (void)hipFree(dPart);
         (void)hipFree(dWn);
         (void)hipFree(dDist);
