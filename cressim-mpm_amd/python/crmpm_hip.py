@@ -60,6 +60,25 @@ EXPORTS = {
                            ctypes.POINTER(ctypes.c_float),
                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong,
                            _vp],
+    "crHipEngineCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    "crHipEngineDestroy": [_vp],
+    "crHipEngineCreateScene": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(ctypes.c_int)],
+    "crHipEngineSetSceneParams": [_vp, ctypes.c_int, ctypes.c_float, _f3, ctypes.c_int, ctypes.c_int],
+    "crHipEngineGetSceneHostData": [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+    "crHipEngineMarkSceneDirty": [_vp, ctypes.c_int, ctypes.c_uint],
+    "crHipEngineSetSceneNumParticles": [_vp, ctypes.c_int, ctypes.c_uint],
+    "crHipEngineAddComputeInitialData": [_vp, ctypes.c_int, ctypes.c_uint, _vp],
+    "crHipEngineSetActiveMaskRange": [_vp, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_ubyte],
+    "crHipEngineSetSceneShapes": [_vp, ctypes.c_int, ctypes.c_int, _vp],
+    "crHipEngineGetTables": [_vp, ctypes.c_int, ctypes.POINTER(ShapeData), ctypes.POINTER(GeometryData),
+                             ctypes.POINTER(GeometrySdfData)],
+    "crHipEngineMarkTablesDirty": [_vp, ctypes.c_uint],
+    "crHipEngineAdvanceAll": [_vp, ctypes.c_float],
+    "crHipEngineFetchResultsAll": [_vp],
+    "crHipEngineGetSceneSubsteps": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "crHipEngineGetSceneStateArrays": [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                       ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipSolverCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(_vp)],
     "crHipSolverDestroy": [_vp],
     "crHipSolverGetGridDims": [_vp, ctypes.POINTER(ctypes.c_int)],
@@ -442,6 +461,125 @@ class HipSolver:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.crHipSolverDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SCENE_POSITION_MASS, SCENE_VELOCITY, SCENE_PARAMS0, SCENE_TYPE = 1, 2, 4, 8
+SCENE_NUM_PARTICLES, SCENE_COMPUTE_INITIAL_DATA, SCENE_ACTIVE_MASK = 16, 32, 64
+ENGINE_SHAPES, ENGINE_GEOMETRIES, ENGINE_SDF, ENGINE_SDF_ROWS = 1, 2, 4, 8
+
+
+def _host_view(ptr, shape, ctype):
+    n = int(np.prod(shape))
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,)).reshape(shape)
+
+
+class HipEngine:
+    """The engine runtime (crHipEngine*): the factory's particle block and
+    shape tables as pinned host arrays (numpy views), scenes as slices,
+    dirty flags, advance_all / fetch_results_all."""
+
+    def __init__(self, particle_capacity, shape_capacity=1, geometry_capacity=1, sdf_tables=1, sdf_rows=1):
+        self.lib = load_library()
+        h = _vp()
+        check(self.lib.crHipEngineCreate(particle_capacity, shape_capacity, geometry_capacity, sdf_tables, sdf_rows,
+                                         ctypes.byref(h)), "engine create")
+        self.h = h
+        self.caps = (shape_capacity, geometry_capacity, sdf_tables, sdf_rows)
+        self.sizes = []
+
+    def create_scene(self, kind, num_max, cell, lo, hi, dt=0.002, gravity=(0.0, -9.81, 0.0), contact_model=0,
+                     iterations=10):
+        sid = ctypes.c_int()
+        check(self.lib.crHipEngineCreateScene(self.h, kind, num_max, cell, _f3(*lo), _f3(*hi), ctypes.byref(sid)),
+              "create scene")
+        check(self.lib.crHipEngineSetSceneParams(self.h, sid.value, dt, _f3(*gravity), contact_model, iterations),
+              "scene params")
+        self.sizes.append(num_max)
+        return sid.value
+
+    def host(self, sid):
+        """numpy views of the scene's slice of the pinned host arrays."""
+        p = [_vp() for _ in range(5)]
+        check(self.lib.crHipEngineGetSceneHostData(self.h, sid, *[ctypes.byref(x) for x in p]), "host data")
+        n = self.sizes[sid]
+        return dict(xm=_host_view(p[0].value, (n, 4), ctypes.c_float), v=_host_view(p[1].value, (n, 4), ctypes.c_float),
+                    params=_host_view(p[2].value, (n, 4), ctypes.c_float),
+                    type=_host_view(p[3].value, (n,), ctypes.c_int),
+                    mask=_host_view(p[4].value, (n,), ctypes.c_ubyte))
+
+    def tables(self):
+        """numpy views of the pinned host tables (shape fields by name, geometry and SDF arrays)."""
+        sd, gd, sdf = ShapeData(), GeometryData(), GeometrySdfData()
+        check(self.lib.crHipEngineGetTables(self.h, 0, ctypes.byref(sd), ctypes.byref(gd), ctypes.byref(sdf)), "tables")
+        sc, gc, tc, rc = self.caps
+        f4 = lambda ptr, n: _host_view(ptr, (n, 4), ctypes.c_float)  # noqa: E731
+        out = dict(type=_host_view(sd.type, (sc,), ctypes.c_int), geom=_host_view(sd.geometryIdx, (sc,), ctypes.c_int))
+        for k, field in zip(S.SHAPE_F4, ("position", "rotation", "invScale", "linearVelocity", "angularVelocity",
+                                         "comInvMass", "inertiaInv0", "inertiaInv1", "params0")):
+            out[k] = f4(getattr(sd, field), sc)
+        out["force"], out["torque"] = f4(sd.force, sc), f4(sd.torque, sc)
+        out["gtype"] = _host_view(gd.type, (gc,), ctypes.c_int)
+        out["gparams"] = f4(gd.params0, gc)
+        out["sdf_dims"] = _host_view(sdf.dimension, (tc, 4), ctypes.c_int)
+        out["sdf_lbcs"] = f4(sdf.lowerBoundCellSize, tc)
+        out["sdf_data"] = f4(sdf.gradientSignedDistance, rc)
+        return out
+
+    def mark(self, sid, flags):
+        check(self.lib.crHipEngineMarkSceneDirty(self.h, sid, flags), "mark dirty")
+
+    def mark_tables(self, flags):
+        check(self.lib.crHipEngineMarkTablesDirty(self.h, flags), "mark tables")
+
+    def set_num_particles(self, sid, n):
+        check(self.lib.crHipEngineSetSceneNumParticles(self.h, sid, n), "num particles")
+
+    def add_initial_data(self, sid, idx):
+        self._idx = np.ascontiguousarray(idx, np.uint32)
+        check(self.lib.crHipEngineAddComputeInitialData(self.h, sid, len(self._idx), _vp(self._idx.ctypes.data)),
+              "initial data")
+
+    def set_shapes(self, sid, ids):
+        a = np.ascontiguousarray(ids, np.int32)
+        check(self.lib.crHipEngineSetSceneShapes(self.h, sid, len(a), _vp(a.ctypes.data) if len(a) else None),
+              "scene shapes")
+
+    def advance_all(self, dt):
+        check(self.lib.crHipEngineAdvanceAll(self.h, dt), "advance all")
+
+    def fetch_results_all(self):
+        check(self.lib.crHipEngineFetchResultsAll(self.h), "fetch results all")
+
+    def substeps(self, sid):
+        n = ctypes.c_int()
+        check(self.lib.crHipEngineGetSceneSubsteps(self.h, sid, ctypes.byref(n)), "substeps")
+        return n.value
+
+    def state(self, sid):
+        """F, C (n, 9) and V0 of the scene, float64 (after an advance)."""
+        import torch
+
+        F, C, V0, stride, first = _vp(), _vp(), _vp(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.crHipEngineGetSceneStateArrays(self.h, sid, ctypes.byref(F), ctypes.byref(C), ctypes.byref(V0),
+                                                      ctypes.byref(stride), ctypes.byref(first)), "state arrays")
+        dev = torch.device("cuda:0")
+        stream = torch.cuda.current_stream(dev)
+        a = first.value
+        out = _read_state(torch, dev, stream.cuda_stream, F.value, C.value, V0.value, stride.value, a,
+                          a + self.sizes[sid])
+        stream.synchronize()
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.crHipEngineDestroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -262,6 +262,96 @@ int crHipBatchProfileMask(CrHipBatch *b, unsigned int mask);
 int crHipBatchProfileEnd(CrHipBatch *b, float *msPerCategory, int *launchesPerCategory, int numCategories);
 
 /* ------------------------------------------------------------------------
+ * Engine runtime: the GPU side of SimulationFactoryImpl and SceneImpl
+ * (src/engine/simulation_factory_impl.cpp:897-955, 1106-1150;
+ * src/engine/scene_impl.cpp:180-263).  An engine owns the particle block and
+ * the shape / geometry / SDF tables, each as a pinned host copy and a device
+ * copy; scenes are slices of the particle block.  The caller writes the host
+ * copies and marks what changed; crHipEngineAdvanceAll uploads the marked
+ * arrays, runs the queued computeInitialData, resets the shapes' coupling
+ * force and torque, advances every scene by dt (scenes of one solver kind and
+ * iteration count as one CrHipBatch), and downloads positions, velocities
+ * and the coupling forces into the host copies; it runs on a worker thread
+ * (SceneImpl::advance submits to a thread manager) and
+ * crHipEngineFetchResultsAll waits for it.  While an advance is in flight
+ * every other call returns CRMPM_HIP_ERR_STATE. */
+typedef struct CrHipEngine CrHipEngine;
+
+/* SceneDataDirtyFlags (include/engine/gpu_data_dirty_flags.h:86-99), plus the
+ * active mask, which the reference writes to the device at once. */
+#define CRMPM_HIP_SCENE_POSITION_MASS (1u << 0)
+#define CRMPM_HIP_SCENE_VELOCITY (1u << 1)
+#define CRMPM_HIP_SCENE_PARAMS0 (1u << 2)
+#define CRMPM_HIP_SCENE_TYPE (1u << 3)
+#define CRMPM_HIP_SCENE_NUM_PARTICLES (1u << 4)
+#define CRMPM_HIP_SCENE_COMPUTE_INITIAL_DATA (1u << 5)
+#define CRMPM_HIP_SCENE_ACTIVE_MASK (1u << 6)
+#define CRMPM_HIP_SCENE_ALL 0x7fu
+/* Table groups of SimulationFactoryGpuDataDirtyFlags
+ * (include/engine/gpu_data_dirty_flags.h): every shape array but force and
+ * torque; geometry type and params; SDF dims and lower bound|cell size; SDF rows. */
+#define CRMPM_HIP_ENGINE_SHAPES (1u << 0)
+#define CRMPM_HIP_ENGINE_GEOMETRIES (1u << 1)
+#define CRMPM_HIP_ENGINE_SDF (1u << 2)
+#define CRMPM_HIP_ENGINE_SDF_ROWS (1u << 3)
+
+/* createFactory's capacities (src/engine/simulation_factory_impl.cpp): the
+ * particle block, shapes, geometries, SDF tables and SDF rows. */
+int crHipEngineCreate(int particleCapacity, int shapeCapacity, int geometryCapacity, int sdfTableCapacity,
+                      int sdfRowCapacity, CrHipEngine **out);
+int crHipEngineDestroy(CrHipEngine *e);
+/* SimulationFactory::createScene (simulation_factory_impl.cpp:278-340): a
+ * slice of numMaxParticles particles and a grid; kind CRMPM_HIP_MLS / PB /
+ * STANDARD.  Scenes and their parameters are fixed at the first advance. */
+int crHipEngineCreateScene(CrHipEngine *e, int kind, int numMaxParticles, float cellSize, const float boundMin[3],
+                           const float boundMax[3], int *sceneId);
+/* SceneDesc's solverIntegrationStepSize, gravity, contactModel and
+ * solverIterations (include/engine/simulation_factory.h:49-59; defaults
+ * 0.002, (0, -9.81, 0), kinematic, 10). */
+int crHipEngineSetSceneParams(CrHipEngine *e, int scene, float dt, const float gravity[3], int contactModel,
+                              int iterations);
+/* Scene::getParticleData / getParticleMaterialData / getActiveMask: the
+ * scene's slice of the pinned host copies (position|mass and velocity as
+ * float4, params0 float4, type int, mask u8).  Any output may be NULL. */
+int crHipEngineGetSceneHostData(CrHipEngine *e, int scene, float **positionMass, float **velocity, float **params0,
+                                int **type, unsigned char **activeMask);
+/* Scene::markDirty: CRMPM_HIP_SCENE_* flags */
+int crHipEngineMarkSceneDirty(CrHipEngine *e, int scene, unsigned int flags);
+/* The scene's allocated particle count (setNumActiveParticles at the next advance). */
+int crHipEngineSetSceneNumParticles(CrHipEngine *e, int scene, unsigned int n);
+/* Scene::getComputeInitialDataIndices().pushBack: scene-local indices whose
+ * F, V0 are set up at the next advance. */
+int crHipEngineAddComputeInitialData(CrHipEngine *e, int scene, unsigned int n, const unsigned int *indices);
+/* Scene::setActiveMaskRange (scene_impl.cpp:130-139) */
+int crHipEngineSetActiveMaskRange(CrHipEngine *e, int scene, unsigned int start, unsigned int length,
+                                  unsigned char value);
+/* Scene::addShape: the shape ids (rows of the engine's shape table) the scene collides with. */
+int crHipEngineSetSceneShapes(CrHipEngine *e, int scene, int numShapes, const int *shapeIds);
+/* The engine's tables: device != 0 for the device copies (e.g. to build SDF
+ * rows there with crHipMeshSdf), 0 for the pinned host copies. */
+int crHipEngineGetTables(CrHipEngine *e, int device, CrHipShapeData *shapes, CrHipGeometryData *geometries,
+                         CrHipGeometrySdfData *sdf);
+/* markDirty of the factory's GPU data: CRMPM_HIP_ENGINE_* groups. */
+int crHipEngineMarkTablesDirty(CrHipEngine *e, unsigned int flags);
+/* SimulationFactory::advanceAll(dt) / fetchResultsAll(): each scene takes
+ * the reference's substep count (dt accumulated in FP32 until it reaches
+ * dt); within a batch a scene that has taken its count rests (no active
+ * particle) while the others finish.  A resting scene's grid is cleared, so
+ * computeInitialData queued for it at the next advance adds the new
+ * particles' mass to an empty grid, where the reference's solver still holds
+ * its last step's grid (Q10).  FetchResultsAll returns the advance's status;
+ * CRMPM_HIP_ERR_STATE if no advance is in flight. */
+int crHipEngineAdvanceAll(CrHipEngine *e, float dt);
+int crHipEngineFetchResultsAll(CrHipEngine *e);
+/* Substeps the scene took in the last advance (no counterpart in the reference). */
+int crHipEngineGetSceneSubsteps(const CrHipEngine *e, int scene, int *substeps);
+/* The scene's solver state arrays (as crHipBatchGetStateArrays; the scene's
+ * particles start at index *first).  After the first advance.  No
+ * counterpart in the reference. */
+int crHipEngineGetSceneStateArrays(CrHipEngine *e, int scene, float **F, float **C, float **V0, int *stride,
+                                   int *first);
+
+/* ------------------------------------------------------------------------
  * Triangle mesh -> SDF table on the device: the table the engine builds on
  * the host for a triangle-mesh geometry (SimulationFactoryImpl::createGeometry,
  * src/engine/simulation_factory_impl.cpp:628-635).

@@ -1,0 +1,170 @@
+"""The engine runtime (crHipEngine*, the GPU side of SimulationFactoryImpl /
+SceneImpl: src/engine/simulation_factory_impl.cpp:897-955, 1106-1150,
+src/engine/scene_impl.cpp:180-263) against the single-scene solver driven
+directly through the C-ABI: scenes of two solver kinds and different step
+sizes advanced together, particles edited on the host between frames,
+particles added mid-run through the initial-data queue, coupling forces read
+back after each frame.  Bitwise equal: the same kernels on the same inputs."""
+import copy
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "cressim-mpm_amd", "python"))
+
+import crmpm_scenes as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "rot", "invScale", "linVel", "angVel", "comInvMass", "inertia0", "inertia1", "params0")
+
+
+def _scenes():
+    a = S.small_scene(S.MLS, 7, shapes="plane+sphere+box")
+    b = S.small_scene(S.MLS, 8, shapes="plane")
+    c = S.small_scene(S.MLS, 77, shapes="plane+capsule")
+    c["contact_model"] = 1
+    c["shape_list"][1]["comInvMass"] = (0.0, 0.01, 0.0, 2.0)
+    c["shape_list"][1]["inertia0"] = (50.0, 40.0, 50.0, 0.0)
+    c["shape_list"][1]["inertia1"] = (1.0, 0.5, 0.2, 0.0)
+    c["dt"] = c["dt"] / 2
+    d = S.small_scene(S.PB, 9, shapes="plane+sphere+box")
+    return [a, b, c, d]
+
+
+def substeps(frame_dt, dt):
+    """The reference's `while (t < dt) t += step()` in FP32."""
+    t, n = np.float32(0), 0
+    while t < np.float32(frame_dt):
+        t = np.float32(t + np.float32(dt))
+        n += 1
+    return n
+
+
+def load_engine(H, scenes, held_back):
+    """Every scene into one engine: particles into the host block, shapes and
+    geometries appended to the factory tables (geometry rows offset)."""
+    n_shapes = sum(len(sc["shape_list"]) for sc in scenes)
+    n_geoms = sum(len(sc["geoms"]) for sc in scenes)
+    eng = H.HipEngine(sum(len(sc["xm"]) for sc in scenes), max(1, n_shapes), max(1, n_geoms))
+    tabs = eng.tables()
+    ids, s0, g0 = [], 0, 0
+    for i, sc in enumerate(scenes):
+        n = len(sc["xm"])
+        sid = eng.create_scene(H.KIND_FROM_SCENE[sc["kind"]], n, sc["dx"], sc["lo"], sc["hi"], dt=sc["dt"],
+                               gravity=sc["gravity"], contact_model=sc.get("contact_model", 0),
+                               iterations=sc["iterations"])
+        hd = eng.host(sid)
+        hd["xm"][:] = sc["xm"]
+        hd["v"][:] = sc["v"]
+        hd["params"][:] = sc["params"]
+        hd["type"][:] = sc["type"]
+        hd["mask"][:] = sc["mask"]
+        t = S.shape_tables(sc)
+        k = len(sc["shape_list"])
+        for j in range(k):
+            tabs["type"][s0 + j] = t["type"][j]
+            tabs["geom"][s0 + j] = t["geom"][j] + g0
+            for f in FIELDS:
+                tabs[f][s0 + j] = t[f][j]
+        for j, (gt, gp) in enumerate(sc["geoms"]):
+            tabs["gtype"][g0 + j] = gt
+            tabs["gparams"][g0 + j] = gp
+        eng.set_shapes(sid, np.arange(s0, s0 + k))
+        m = n - held_back.get(i, 0)
+        eng.set_num_particles(sid, m)
+        eng.add_initial_data(sid, np.arange(m))
+        eng.mark(sid, H.SCENE_POSITION_MASS | H.SCENE_VELOCITY | H.SCENE_PARAMS0 | H.SCENE_TYPE |
+                 H.SCENE_ACTIVE_MASK)
+        ids.append((sid, s0, k))
+        s0 += k
+        g0 += len(sc["geoms"])
+    eng.mark_tables(H.ENGINE_SHAPES | H.ENGINE_GEOMETRIES)
+    return eng, ids, tabs
+
+
+def test_engine_advance_all_matches_single_solvers():
+    import crmpm_hip as H
+
+    scenes = _scenes()
+    # scene 2 (half the step size: it never rests) starts with its last 100
+    # particles unallocated; a resting scene's grid is cleared, so initial
+    # data queued for it would not see its last step's grid mass
+    held = {2: 100}
+    frame = 4 * scenes[0]["dt"]
+    eng, ids, tabs = load_engine(H, copy.deepcopy(scenes), held)
+
+    solvers = []
+    for i, sc in enumerate(scenes):
+        g = H.HipSolver(copy.deepcopy(sc))
+        n = len(sc["xm"]) - held.get(i, 0)
+        H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, n), "num active")
+        g.compute_initial_data(np.arange(n))
+        solvers.append(g)
+
+    for f in range(2):
+        if f == 1:
+            # between frames: scene 1's velocities halved on the host; scene
+            # 2's held-back particles allocated and set up; the same on the solvers
+            hd = eng.host(1)
+            hd["v"][:, :3] *= 0.5
+            eng.mark(1, H.SCENE_VELOCITY)
+            solvers[1].v[:, :3] *= 0.5
+            n2 = len(scenes[2]["xm"])
+            eng.set_num_particles(2, n2)
+            eng.add_initial_data(2, np.arange(n2 - 100, n2))
+            g = solvers[2]
+            H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, n2), "num active")
+            g.compute_initial_data(np.arange(n2 - 100, n2))
+            # the solvers' coupling forces restart per frame, as the engine's do
+            for g in solvers:
+                g.force.zero_()
+                g.torque.zero_()
+        eng.advance_all(frame)
+        eng.fetch_results_all()
+        for i, sc in enumerate(scenes):
+            k = substeps(frame, sc["dt"])
+            assert eng.substeps(i) == k, (f, i, eng.substeps(i), k)
+            solvers[i].step(k)
+        for i, (sid, s0, k) in enumerate(ids):
+            st = solvers[i].state()
+            hd = eng.host(sid)
+            print(f"frame {f} scene {i}: {len(hd['xm'])} particles, {eng.substeps(sid)} substeps", flush=True)
+            np.testing.assert_array_equal(hd["xm"], st["xm"].astype(np.float32))
+            np.testing.assert_array_equal(hd["v"][:, :3], st["v"].astype(np.float32))
+            es = eng.state(sid)
+            for key in ("F", "C", "V0"):
+                np.testing.assert_array_equal(es[key], st[key], err_msg=key)
+            gf, gt = solvers[i].shape_force()
+            np.testing.assert_array_equal(tabs["force"][s0:s0 + k], gf[:k].astype(np.float32))
+            np.testing.assert_array_equal(tabs["torque"][s0:s0 + k], gt[:k].astype(np.float32))
+    assert np.abs(tabs["force"][ids[2][1] + 1, :3]).max() > 0, "the dynamic capsule receives no force"
+    for g in solvers:
+        g.close()
+    eng.close()
+
+
+def test_engine_state_errors():
+    import crmpm_hip as H
+
+    sc = S.small_scene(S.MLS, 7, shapes="plane")
+    eng = H.HipEngine(len(sc["xm"]))
+    sid = eng.create_scene(1, len(sc["xm"]), sc["dx"], sc["lo"], sc["hi"])
+    lib = eng.lib
+    assert lib.crHipEngineFetchResultsAll(eng.h) == 2  # nothing in flight
+    assert lib.crHipEngineCreateScene(eng.h, 1, 10, sc["dx"], H._f3(*sc["lo"]), H._f3(*sc["hi"]),
+                                      ctypes.byref(ctypes.c_int())) == 5  # particle block full
+    assert lib.crHipEngineSetSceneNumParticles(eng.h, sid, len(sc["xm"]) + 1) == 1
+    eng.advance_all(0.0)  # no particle allocated: builds the batch, takes no substep
+    assert lib.crHipEngineAdvanceAll(eng.h, 0.01) == 2  # one advance at a time
+    assert lib.crHipEngineMarkSceneDirty(eng.h, sid, 1) == 2
+    eng.fetch_results_all()
+    assert eng.substeps(sid) == 0
+    assert lib.crHipEngineCreateScene(eng.h, 1, 1, sc["dx"], H._f3(*sc["lo"]), H._f3(*sc["hi"]),
+                                      ctypes.byref(ctypes.c_int())) in (2, 5)
+    eng.close()

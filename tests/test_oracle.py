@@ -266,6 +266,13 @@ def test_capi_argument_errors_without_gpu():
     assert lib.crHipSolverSetNumActiveParticles(h, 11) == 1
     assert lib.crHipSolverSetShapeContactModel(h, 3) == 1
     assert lib.crHipSolverDestroy(h) == 0
+    # the engine runtime refuses bad arguments before allocating anything
+    e = ctypes.c_void_p()
+    assert lib.crHipEngineCreate(0, 1, 1, 1, 1, ctypes.byref(e)) == 1
+    assert lib.crHipEngineCreate(10, 1, 1, 1, 1, None) == 1
+    assert lib.crHipEngineDestroy(None) == 1
+    assert lib.crHipEngineAdvanceAll(None, 0.01) == 1
+    assert lib.crHipEngineFetchResultsAll(None) == 1
 
 
 def test_config3_generator():
