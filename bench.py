@@ -269,6 +269,56 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
             "hip_graph": graph, "finite": bool(finite)}
 
 
+def engine(frames=10, substeps=10, warmup=1):
+    """The engine runtime (advanceAll): the 64 MLS scenes of the batched
+    workload held by one engine, advanced by frames of 10 substeps, each frame
+    ending with positions, velocities and coupling forces back in pinned host
+    memory; against the same batch stepped without the engine's per-frame
+    uploads and readback."""
+    import torch
+
+    import crmpm_hip as H
+
+    scenes = S.config3(kind=S.MLS, steps=1)
+    n_total = sum(len(sc["xm"]) for sc in scenes)
+    dt = scenes[0]["dt"]
+    frame = (substeps - 0.5) * dt  # FP32 accumulation reaches it after `substeps` steps
+    eng, _, _ = H.engine_from_scenes(copy.deepcopy(scenes))
+    eng.advance_all(frame * warmup)
+    eng.fetch_results_all()
+    eng.autotune(steps_per_trial=2)
+    eng.set_graph_mode(True)
+    eng.advance_all(frame)
+    eng.fetch_results_all()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        eng.advance_all(frame)
+        eng.fetch_results_all()
+    t_eng = time.perf_counter() - t0
+    k = eng.substeps(0)
+    finite = bool(np.isfinite(eng.host(0)["xm"]).all() and np.isfinite(eng.host(len(scenes) - 1)["xm"]).all())
+    eng.close()
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    b.step(substeps * warmup)
+    b.autotune(steps_per_trial=2)
+    b.set_graph_mode(True)
+    b.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(frames * k)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    return {"workload": f"{len(scenes)} MLS scenes ({n_total} particles) in one engine, frames of {k} substeps with "
+                        "host readback", "particles": n_total, "frames": frames, "substeps_per_frame": k,
+            "value": n_total * k * frames / t_eng, "unit": "particle-substeps/s",
+            "ms_per_frame": t_eng / frames * 1e3, "batch_only_ms_per_frame": t_batch / frames * 1e3,
+            "engine_overhead_frac": t_eng / t_batch - 1.0,
+            # each frame moves 32 B per particle (position|mass and velocity) to the host
+            "readback_GBps": 32.0 * n_total * frames / t_eng / 1e9, "finite": finite}
+
+
 def adjoint(steps=100, every=10):
     """Differentiable MLS block (config4: 65,536 particles, 96^3 grid, GRF log E):
     time of a forward rollout with checkpoints, of forward + adjoint, and of the
@@ -507,7 +557,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=4,
                     help="substeps in the CPU baseline sample, about 3 s each on one core (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice", "meshsdf"], default="all",
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice", "meshsdf", "engine"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
@@ -547,6 +597,9 @@ def main():
         return
     if args.workload == "lattice":
         print(json.dumps({"lattice": lattice()}), flush=True)
+        return
+    if args.workload == "engine":
+        print(json.dumps({"engine": engine()}), flush=True)
         return
     if args.workload == "meshsdf":
         print(json.dumps({"mesh_sdf": mesh_sdf()}), flush=True)
@@ -650,7 +703,7 @@ def main():
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
         for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube),
-                        ("lattice", lattice), ("mesh_sdf", mesh_sdf)):
+                        ("lattice", lattice), ("mesh_sdf", mesh_sdf), ("engine", engine)):
             try:
                 out[key] = fn()
             except Exception as e:

@@ -118,6 +118,7 @@ struct CrHipEngine
     };
     std::vector<Batch> batches;
     bool built = false;
+    bool graphMode = false;
 
     unsigned *dIdx = nullptr;
     size_t dIdxCap = 0;
@@ -169,7 +170,7 @@ namespace
                     (rc = crHipBatchSetSceneContactModel(b, k, s.contactModel)))
                     return rc;
             }
-            if ((rc = crHipBatchInitialize(b)))
+            if ((rc = crHipBatchInitialize(b)) || (rc = crHipBatchSetGraphMode(b, e->graphMode ? 1 : 0)))
                 return rc;
             for (int k = 0; k < (int)bt.scenes.size(); ++k)
                 if ((rc = crHipBatchSetSceneNumActiveParticles(b, k, e->scenes[bt.scenes[k]].numAllocated)))
@@ -646,6 +647,37 @@ int crHipEngineGetSceneSubsteps(const CrHipEngine *e, int scene, int *substeps)
     if (!valid_scene(e, scene) || !substeps)
         return CRMPM_HIP_ERR_ARG;
     *substeps = e->scenes[scene].substeps;
+    return CRMPM_HIP_OK;
+}
+
+int crHipEngineSetGraphMode(CrHipEngine *e, int enabled)
+{
+    if (!e)
+        return CRMPM_HIP_ERR_ARG;
+    if (e->busy)
+        return CRMPM_HIP_ERR_STATE;
+    e->graphMode = enabled != 0;
+    for (auto &bt : e->batches)
+    {
+        const int rc = crHipBatchSetGraphMode(bt.b, enabled);
+        if (rc)
+            return rc;
+    }
+    return CRMPM_HIP_OK;
+}
+
+int crHipEngineAutotune(CrHipEngine *e, int stepsPerTrial)
+{
+    if (!e || stepsPerTrial < 1)
+        return CRMPM_HIP_ERR_ARG;
+    if (e->busy || !e->built)
+        return CRMPM_HIP_ERR_STATE;
+    for (auto &bt : e->batches)
+    {
+        const int rc = crHipBatchAutotune(bt.b, stepsPerTrial, nullptr, nullptr, nullptr, nullptr);
+        if (rc)
+            return rc;
+    }
     return CRMPM_HIP_OK;
 }
 

@@ -76,6 +76,8 @@ EXPORTS = {
     "crHipEngineMarkTablesDirty": [_vp, ctypes.c_uint],
     "crHipEngineAdvanceAll": [_vp, ctypes.c_float],
     "crHipEngineFetchResultsAll": [_vp],
+    "crHipEngineSetGraphMode": [_vp, ctypes.c_int],
+    "crHipEngineAutotune": [_vp, ctypes.c_int],
     "crHipEngineGetSceneSubsteps": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     "crHipEngineGetSceneStateArrays": [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                        ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
@@ -557,6 +559,12 @@ class HipEngine:
     def fetch_results_all(self):
         check(self.lib.crHipEngineFetchResultsAll(self.h), "fetch results all")
 
+    def set_graph_mode(self, on):
+        check(self.lib.crHipEngineSetGraphMode(self.h, 1 if on else 0), "graph mode")
+
+    def autotune(self, steps_per_trial=2):
+        check(self.lib.crHipEngineAutotune(self.h, steps_per_trial), "autotune")
+
     def substeps(self, sid):
         n = ctypes.c_int()
         check(self.lib.crHipEngineGetSceneSubsteps(self.h, sid, ctypes.byref(n)), "substeps")
@@ -587,6 +595,51 @@ class HipEngine:
             self.close()
         except Exception:
             pass
+
+
+def engine_from_scenes(scenes, held_back=None):
+    """An engine holding scene dicts (as HipSolver takes them, analytic
+    geometries only): particles written to the host block, shapes and
+    geometries appended to the factory tables with geometry rows offset,
+    every particle but held_back[i] of scene i allocated and queued for
+    computeInitialData.  Returns (engine, [(scene id, first shape row,
+    shape count)], host tables)."""
+    held_back = held_back or {}
+    n_shapes = sum(len(sc["shape_list"]) for sc in scenes)
+    n_geoms = sum(len(sc["geoms"]) for sc in scenes)
+    eng = HipEngine(sum(len(sc["xm"]) for sc in scenes), max(1, n_shapes), max(1, n_geoms))
+    tabs = eng.tables()
+    ids, s0, g0 = [], 0, 0
+    for i, sc in enumerate(scenes):
+        if len(sc["sdf_data"]) > 1:
+            raise ValueError("engine_from_scenes takes analytic geometries only")
+        n = len(sc["xm"])
+        sid = eng.create_scene(KIND_FROM_SCENE[sc["kind"]], n, sc["dx"], sc["lo"], sc["hi"], dt=sc["dt"],
+                               gravity=sc["gravity"], contact_model=sc.get("contact_model", 0),
+                               iterations=sc["iterations"])
+        hd = eng.host(sid)
+        for k in ("xm", "v", "params", "type", "mask"):
+            hd[k][:] = sc[k]
+        t = S.shape_tables(sc)
+        k = len(sc["shape_list"])
+        for j in range(k):
+            tabs["type"][s0 + j] = t["type"][j]
+            tabs["geom"][s0 + j] = t["geom"][j] + g0
+            for f in S.SHAPE_F4:
+                tabs[f][s0 + j] = t[f][j]
+        for j, (gt, gp) in enumerate(sc["geoms"]):
+            tabs["gtype"][g0 + j] = gt
+            tabs["gparams"][g0 + j] = gp
+        eng.set_shapes(sid, np.arange(s0, s0 + k))
+        m = n - held_back.get(i, 0)
+        eng.set_num_particles(sid, m)
+        eng.add_initial_data(sid, np.arange(m))
+        eng.mark(sid, SCENE_POSITION_MASS | SCENE_VELOCITY | SCENE_PARAMS0 | SCENE_TYPE | SCENE_ACTIVE_MASK)
+        ids.append((sid, s0, k))
+        s0 += k
+        g0 += len(sc["geoms"])
+    eng.mark_tables(ENGINE_SHAPES | ENGINE_GEOMETRIES)
+    return eng, ids, tabs
 
 
 class DeviceShapeTables:

@@ -343,6 +343,11 @@ int crHipEngineMarkTablesDirty(CrHipEngine *e, unsigned int flags);
  * CRMPM_HIP_ERR_STATE if no advance is in flight. */
 int crHipEngineAdvanceAll(CrHipEngine *e, float dt);
 int crHipEngineFetchResultsAll(CrHipEngine *e);
+/* crHipBatchSetGraphMode / crHipBatchAutotune on every batch of the engine
+ * (autotune after the first advance: it times the current state and leaves
+ * it unchanged).  No counterpart in the reference. */
+int crHipEngineSetGraphMode(CrHipEngine *e, int enabled);
+int crHipEngineAutotune(CrHipEngine *e, int stepsPerTrial);
 /* Substeps the scene took in the last advance (no counterpart in the reference). */
 int crHipEngineGetSceneSubsteps(const CrHipEngine *e, int scene, int *substeps);
 /* The scene's solver state arrays (as crHipBatchGetStateArrays; the scene's

@@ -21,9 +21,6 @@ import crmpm_scenes as S  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("pos", "rot", "invScale", "linVel", "angVel", "comInvMass", "inertia0", "inertia1", "params0")
-
-
 def _scenes():
     a = S.small_scene(S.MLS, 7, shapes="plane+sphere+box")
     b = S.small_scene(S.MLS, 8, shapes="plane")
@@ -46,48 +43,6 @@ def substeps(frame_dt, dt):
     return n
 
 
-def load_engine(H, scenes, held_back):
-    """Every scene into one engine: particles into the host block, shapes and
-    geometries appended to the factory tables (geometry rows offset)."""
-    n_shapes = sum(len(sc["shape_list"]) for sc in scenes)
-    n_geoms = sum(len(sc["geoms"]) for sc in scenes)
-    eng = H.HipEngine(sum(len(sc["xm"]) for sc in scenes), max(1, n_shapes), max(1, n_geoms))
-    tabs = eng.tables()
-    ids, s0, g0 = [], 0, 0
-    for i, sc in enumerate(scenes):
-        n = len(sc["xm"])
-        sid = eng.create_scene(H.KIND_FROM_SCENE[sc["kind"]], n, sc["dx"], sc["lo"], sc["hi"], dt=sc["dt"],
-                               gravity=sc["gravity"], contact_model=sc.get("contact_model", 0),
-                               iterations=sc["iterations"])
-        hd = eng.host(sid)
-        hd["xm"][:] = sc["xm"]
-        hd["v"][:] = sc["v"]
-        hd["params"][:] = sc["params"]
-        hd["type"][:] = sc["type"]
-        hd["mask"][:] = sc["mask"]
-        t = S.shape_tables(sc)
-        k = len(sc["shape_list"])
-        for j in range(k):
-            tabs["type"][s0 + j] = t["type"][j]
-            tabs["geom"][s0 + j] = t["geom"][j] + g0
-            for f in FIELDS:
-                tabs[f][s0 + j] = t[f][j]
-        for j, (gt, gp) in enumerate(sc["geoms"]):
-            tabs["gtype"][g0 + j] = gt
-            tabs["gparams"][g0 + j] = gp
-        eng.set_shapes(sid, np.arange(s0, s0 + k))
-        m = n - held_back.get(i, 0)
-        eng.set_num_particles(sid, m)
-        eng.add_initial_data(sid, np.arange(m))
-        eng.mark(sid, H.SCENE_POSITION_MASS | H.SCENE_VELOCITY | H.SCENE_PARAMS0 | H.SCENE_TYPE |
-                 H.SCENE_ACTIVE_MASK)
-        ids.append((sid, s0, k))
-        s0 += k
-        g0 += len(sc["geoms"])
-    eng.mark_tables(H.ENGINE_SHAPES | H.ENGINE_GEOMETRIES)
-    return eng, ids, tabs
-
-
 def test_engine_advance_all_matches_single_solvers():
     import crmpm_hip as H
 
@@ -97,7 +52,7 @@ def test_engine_advance_all_matches_single_solvers():
     # data queued for it would not see its last step's grid mass
     held = {2: 100}
     frame = 4 * scenes[0]["dt"]
-    eng, ids, tabs = load_engine(H, copy.deepcopy(scenes), held)
+    eng, ids, tabs = H.engine_from_scenes(copy.deepcopy(scenes), held)
 
     solvers = []
     for i, sc in enumerate(scenes):
