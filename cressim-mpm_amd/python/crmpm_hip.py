@@ -565,6 +565,14 @@ class HipEngine:
     def autotune(self, steps_per_trial=2):
         check(self.lib.crHipEngineAutotune(self.h, steps_per_trial), "autotune")
 
+    def device_tables(self):
+        """Device pointers of the engine's tables (CrHipShapeData,
+        CrHipGeometryData, CrHipGeometrySdfData)."""
+        sd, gd, sdf = ShapeData(), GeometryData(), GeometrySdfData()
+        check(self.lib.crHipEngineGetTables(self.h, 1, ctypes.byref(sd), ctypes.byref(gd), ctypes.byref(sdf)),
+              "device tables")
+        return sd, gd, sdf
+
     def substeps(self, sid):
         n = ctypes.c_int()
         check(self.lib.crHipEngineGetSceneSubsteps(self.h, sid, ctypes.byref(n)), "substeps")
@@ -597,22 +605,26 @@ class HipEngine:
             pass
 
 
+# geometry types whose params0 (x, y) hold an SDF row offset and table id as int bits
+_TABLE_GEOMETRIES = (S.TRIMESH, S.TRIMESH_SLICER, S.LINES)
+
+
 def engine_from_scenes(scenes, held_back=None):
-    """An engine holding scene dicts (as HipSolver takes them, analytic
-    geometries only): particles written to the host block, shapes and
-    geometries appended to the factory tables with geometry rows offset,
-    every particle but held_back[i] of scene i allocated and queued for
-    computeInitialData.  Returns (engine, [(scene id, first shape row,
-    shape count)], host tables)."""
+    """An engine holding scene dicts (as HipSolver takes them): particles
+    written to the host block; shapes, geometries and SDF tables appended to
+    the factory tables, geometry rows and SDF offsets and ids shifted; every
+    particle but held_back[i] of scene i allocated and queued for
+    computeInitialData.  Returns (engine, [(scene id, first shape row, shape
+    count, first SDF row, first SDF table)], host tables)."""
     held_back = held_back or {}
     n_shapes = sum(len(sc["shape_list"]) for sc in scenes)
     n_geoms = sum(len(sc["geoms"]) for sc in scenes)
-    eng = HipEngine(sum(len(sc["xm"]) for sc in scenes), max(1, n_shapes), max(1, n_geoms))
+    n_tables = sum(len(sc["sdf_dims"]) for sc in scenes)
+    n_rows = sum(len(sc["sdf_data"]) for sc in scenes)
+    eng = HipEngine(sum(len(sc["xm"]) for sc in scenes), max(1, n_shapes), max(1, n_geoms), n_tables, n_rows)
     tabs = eng.tables()
-    ids, s0, g0 = [], 0, 0
+    ids, s0, g0, r0, t0 = [], 0, 0, 0, 0
     for i, sc in enumerate(scenes):
-        if len(sc["sdf_data"]) > 1:
-            raise ValueError("engine_from_scenes takes analytic geometries only")
         n = len(sc["xm"])
         sid = eng.create_scene(KIND_FROM_SCENE[sc["kind"]], n, sc["dx"], sc["lo"], sc["hi"], dt=sc["dt"],
                                gravity=sc["gravity"], contact_model=sc.get("contact_model", 0),
@@ -629,16 +641,27 @@ def engine_from_scenes(scenes, held_back=None):
                 tabs[f][s0 + j] = t[f][j]
         for j, (gt, gp) in enumerate(sc["geoms"]):
             tabs["gtype"][g0 + j] = gt
+            gp = np.array(gp, np.float32)
+            if gt in _TABLE_GEOMETRIES:
+                bits = gp.view(np.int32)
+                bits[0] += r0
+                bits[1] += t0
             tabs["gparams"][g0 + j] = gp
+        nt, nr = len(sc["sdf_dims"]), len(sc["sdf_data"])
+        tabs["sdf_dims"][t0:t0 + nt] = sc["sdf_dims"]
+        tabs["sdf_lbcs"][t0:t0 + nt] = sc["sdf_lbcs"]
+        tabs["sdf_data"][r0:r0 + nr] = sc["sdf_data"]
         eng.set_shapes(sid, np.arange(s0, s0 + k))
         m = n - held_back.get(i, 0)
         eng.set_num_particles(sid, m)
         eng.add_initial_data(sid, np.arange(m))
         eng.mark(sid, SCENE_POSITION_MASS | SCENE_VELOCITY | SCENE_PARAMS0 | SCENE_TYPE | SCENE_ACTIVE_MASK)
-        ids.append((sid, s0, k))
+        ids.append((sid, s0, k, r0, t0))
         s0 += k
         g0 += len(sc["geoms"])
-    eng.mark_tables(ENGINE_SHAPES | ENGINE_GEOMETRIES)
+        r0 += nr
+        t0 += nt
+    eng.mark_tables(ENGINE_SHAPES | ENGINE_GEOMETRIES | ENGINE_SDF | ENGINE_SDF_ROWS)
     return eng, ids, tabs
 
 

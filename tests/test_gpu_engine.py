@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "cressim-mpm_amd", "python"))
 
+import cpu_checkers as CC  # noqa: E402
 import crmpm_scenes as S  # noqa: E402
 
 pytestmark = pytest.mark.gpu
@@ -86,7 +87,7 @@ def test_engine_advance_all_matches_single_solvers():
             k = substeps(frame, sc["dt"])
             assert eng.substeps(i) == k, (f, i, eng.substeps(i), k)
             solvers[i].step(k)
-        for i, (sid, s0, k) in enumerate(ids):
+        for i, (sid, s0, k, _, _) in enumerate(ids):
             st = solvers[i].state()
             hd = eng.host(sid)
             print(f"frame {f} scene {i}: {len(hd['xm'])} particles, {eng.substeps(sid)} substeps", flush=True)
@@ -122,4 +123,63 @@ def test_engine_state_errors():
     assert eng.substeps(sid) == 0
     assert lib.crHipEngineCreateScene(eng.h, 1, 1, sc["dx"], H._f3(*sc["lo"]), H._f3(*sc["hi"]),
                                       ctypes.byref(ctypes.c_int())) in (2, 5)
+    eng.close()
+
+
+def test_engine_scene_with_a_device_built_mesh_table():
+    """A triangle-mesh geometry whose SDF rows are built on the device
+    (crHipMeshSdf) straight into the engine's SDF table, pressing on a block
+    for 10 substeps; against the oracle running the same scene with the table
+    from its own mesh-to-SDF restatement (bit-identical on this closed mesh)."""
+    import ctypes
+
+    import crmpm_hip as H
+
+    pts, tris = S.icosphere(0.05, 3)
+    cell, fatten = 0.01, 0.02
+    dims, lower, rows = CC.mesh_sdf(pts, tris, cell, fatten)
+    sc = S.small_scene(S.MLS, 11, shapes="plane")
+    side = 8 * sc["dx"] / 2
+    off, tid = S.add_sdf_grid(sc, tuple(int(d) for d in dims), tuple(float(x) for x in lower), cell, rows)
+    S.add_shape(sc, S.TRIMESH, (S.int_bits(off), S.int_bits(tid), 0, 0), pos=(0.5, 0.2 + side + 0.03, 0.5),
+                stype=S.KINEMATIC, params0=(0.01, 0.9, 0.5, 0.0), lin_vel=(0.0, -0.3, 0.0))
+    host_sc = copy.deepcopy(sc)
+    host_sc["sdf_data"][off:off + len(rows)] = 0.0  # the host copy does not hold the table
+    eng, ids, _ = H.engine_from_scenes([host_sc])
+    eng.advance_all(0.0)  # uploads the host data, takes no substep
+    eng.fetch_results_all()
+    _, _, sdf = eng.device_tables()
+    r0 = ids[0][3]
+    d3, l3 = (ctypes.c_int * 3)(), (ctypes.c_float * 3)()
+    p32, t32 = np.ascontiguousarray(pts, np.float32), np.ascontiguousarray(tris, np.uint32)
+    H.check(eng.lib.crHipMeshSdf(H._vp(p32.ctypes.data), len(p32), H._vp(t32.ctypes.data), len(t32), cell, fatten,
+                                 d3, l3, H._vp(sdf.gradientSignedDistance + 16 * (r0 + off)), len(rows), None),
+            "mesh sdf into the engine")
+    eng.advance_all(9.5 * sc["dt"])
+    eng.fetch_results_all()
+    assert eng.substeps(0) == 10
+    hd = eng.host(0)
+    st = {"xm": hd["xm"].astype(np.float64), "v": hd["v"][:, :3].astype(np.float64)}
+    st.update(eng.state(0))
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    o.step(10)
+    os_ = o.state()
+    o.close()
+    od = CC.CpuSolver(copy.deepcopy(sc), "oracle", fp64=True)
+    od.compute_initial_data()
+    od.step(10)
+    od_ = od.state()
+    od.close()
+    free = CC.CpuSolver(copy.deepcopy(S.small_scene(S.MLS, 11, shapes="plane")), "oracle")
+    free.compute_initial_data()
+    free.step(10)
+    fs = free.state()
+    free.close()
+    rel = lambda a, b: float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-30)  # noqa: E731
+    assert rel(os_["v"], fs["v"]) > 1e-3, "the mesh shape does not touch the block"
+    for k in ("xm", "v", "F"):
+        err, tol = rel(st[k], os_[k]), max(2e-5, 2.0 * rel(os_[k], od_[k]))
+        print(f"{k}: engine vs oracle {err:.2e} (bound {tol:.2e})", flush=True)
+        assert err <= tol, (k, err, tol)
     eng.close()
