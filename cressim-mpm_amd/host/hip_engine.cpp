@@ -489,9 +489,22 @@ int crHipEngineSetSceneParams(CrHipEngine *e, int scene, float dt, const float g
     if (!valid_scene(e, scene) || !gravity || !(dt > 0.0f) || iterations < 1 ||
         (contactModel != CRMPM_HIP_CONTACT_KINEMATIC && contactModel != CRMPM_HIP_CONTACT_EFFECTIVE_MASS))
         return CRMPM_HIP_ERR_ARG;
-    if (e->busy || e->built)
+    if (e->busy)
         return CRMPM_HIP_ERR_STATE;
     auto &s = e->scenes[scene];
+    if (e->built)
+    {
+        // the batches are formed: step size, gravity and contact model go to
+        // the scene's batch; the iteration count chose the batch and stays
+        if (iterations != s.iterations && s.kind == CRMPM_HIP_PB)
+            return CRMPM_HIP_ERR_STATE;
+        CrHipBatch *b = e->batches[s.batch].b;
+        int rc;
+        if ((rc = crHipBatchSetSceneStepSize(b, s.slot, dt)) ||
+            (rc = crHipBatchSetSceneGravity(b, s.slot, gravity[0], gravity[1], gravity[2])) ||
+            (rc = crHipBatchSetSceneContactModel(b, s.slot, contactModel)))
+            return rc;
+    }
     s.dt = dt;
     std::copy(gravity, gravity + 3, s.gravity);
     s.contactModel = contactModel;

@@ -302,12 +302,14 @@ int crHipEngineCreate(int particleCapacity, int shapeCapacity, int geometryCapac
 int crHipEngineDestroy(CrHipEngine *e);
 /* SimulationFactory::createScene (simulation_factory_impl.cpp:278-340): a
  * slice of numMaxParticles particles and a grid; kind CRMPM_HIP_MLS / PB /
- * STANDARD.  Scenes and their parameters are fixed at the first advance. */
+ * STANDARD.  The set of scenes is fixed at the first advance. */
 int crHipEngineCreateScene(CrHipEngine *e, int kind, int numMaxParticles, float cellSize, const float boundMin[3],
                            const float boundMax[3], int *sceneId);
 /* SceneDesc's solverIntegrationStepSize, gravity, contactModel and
  * solverIterations (include/engine/simulation_factory.h:49-59; defaults
- * 0.002, (0, -9.81, 0), kinematic, 10). */
+ * 0.002, (0, -9.81, 0), kinematic, 10).  After the first advance the step
+ * size, gravity and contact model may still change (the solver setters); a
+ * PB scene's iteration count may not (CRMPM_HIP_ERR_STATE). */
 int crHipEngineSetSceneParams(CrHipEngine *e, int scene, float dt, const float gravity[3], int contactModel,
                               int iterations);
 /* Scene::getParticleData / getParticleMaterialData / getActiveMask: the
