@@ -74,6 +74,12 @@ def sdf_cases():
     yield "sdf_cube", CUBE_POINTS, CUBE_TRIS, 0.2, 0.3
     yield "sdf_icosphere", *S.icosphere(0.1, 3), 0.01, 0.02
     yield "sdf_torus", *S.torus(0.1, 0.03, 48, 24), 0.01, 0.01
+    # an open, curved sheet: its winding number crosses 0.5 away from the surface
+    yield "sdf_curved_plane_obj", *load_obj(os.path.join(REF_ASSETS, "curved_plane.obj")), 0.1, 0.1
+    # duplicated triangles (exact ties) and degenerate ones (a segment, a point)
+    pts, tris = S.icosphere(0.1, 2)
+    tris = np.vstack([tris, tris[:5], [[0, 0, 1]], [[3, 3, 3]]]).astype(np.uint32)
+    yield "sdf_degenerate", pts, tris, 0.02, 0.03
 
 
 def slicer_cases():

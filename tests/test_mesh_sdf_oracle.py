@@ -22,13 +22,14 @@ import pytest
 import cpu_checkers as CC
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FIXTURES = ["sdf_sphere_obj", "sdf_sphere_obj_fine", "sdf_hemisphere_obj", "sdf_cube", "sdf_icosphere", "sdf_torus"]
-OPEN_MESHES = {"sdf_hemisphere_obj"}
+FIXTURES = ["sdf_sphere_obj", "sdf_sphere_obj_fine", "sdf_hemisphere_obj", "sdf_cube", "sdf_icosphere", "sdf_torus",
+            "sdf_curved_plane_obj", "sdf_degenerate"]
 # The reference signs with a far-field approximation of the winding number
 # (beta = 2) and the oracle with the exact sum: on an open mesh they may
-# disagree where the winding number is near 0.5, past the rim; the fixture
-# has 3 such nodes of 6877.
-OPEN_SIGN_MISMATCH_FRACTION = 1e-3
+# disagree where the winding number is near 0.5.  Measured: 3 of 6,877 nodes
+# past the hemisphere's rim; 58 of 4,761 around the curved sheet, whose
+# winding number crosses 0.5 away from the surface on its concave side.
+OPEN_SIGN_MISMATCHES = {"sdf_hemisphere_obj": 3, "sdf_curved_plane_obj": 58}
 
 
 def fixture(name):
@@ -62,10 +63,7 @@ def compare_tables(name, g, dims, lower, grid, rel=1e-6):
     dd = np.abs(np.abs(grid[:, 3]) - np.abs(ref[:, 3]))
     assert dd.max() <= rel * scale, (name, dd.max())
     mism = np.sign(grid[:, 3]) != np.sign(ref[:, 3])
-    if name in OPEN_MESHES:
-        assert mism.mean() <= OPEN_SIGN_MISMATCH_FRACTION, (name, mism.sum())
-    else:
-        assert not mism.any(), (name, mism.sum())
+    assert mism.sum() <= OPEN_SIGN_MISMATCHES.get(name, 0), (name, mism.sum())
     stable = sign_stable(ref[:, 3], dims) & sign_stable(grid[:, 3], dims)
     stable &= ~mism
     gd = np.abs(grid[stable, :3] - ref[stable, :3]).max()
