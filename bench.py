@@ -310,8 +310,25 @@ def engine(frames=10, substeps=10, warmup=1):
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     b.close()
+    # the reference's CPU MLS solver (or the oracle) on the largest scene, a few substeps
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as CC
+
+    which = "ref" if CC.ref_available() else "oracle"
+    big = max(scenes, key=lambda sc: len(sc["xm"]))
+    o = CC.CpuSolver(copy.deepcopy(big), which)
+    o.compute_initial_data()
+    cpu_steps = 30
+    t0 = time.perf_counter()
+    o.step(cpu_steps)
+    dt_cpu = time.perf_counter() - t0
+    o.close()
+    cpu = {"value": len(big["xm"]) * cpu_steps / dt_cpu, "unit": "particle-substeps/s", "cores": 1,
+           "kind": "reference" if which == "ref" else "port",
+           "sample": f"{cpu_steps} substeps of the largest scene ({len(big['xm'])} particles) in {dt_cpu:.1f} s on 1 core"}
     return {"workload": f"{len(scenes)} MLS scenes ({n_total} particles) in one engine, frames of {k} substeps with "
                         "host readback", "particles": n_total, "frames": frames, "substeps_per_frame": k,
+            "cpu_baseline": cpu,
             "value": n_total * k * frames / t_eng, "unit": "particle-substeps/s",
             "ms_per_frame": t_eng / frames * 1e3, "batch_only_ms_per_frame": t_batch / frames * 1e3,
             "engine_overhead_frac": t_eng / t_batch - 1.0,
