@@ -308,24 +308,28 @@ namespace
                 most = std::max(most, n);
             }
             std::vector<int> resting;
-            for (int k = 0; k < most; ++k)
+            for (int k = 0; k < most && !rc; ++k)
             {
                 for (int i : bt.scenes)
                 {
                     auto &s = e->scenes[i];
-                    if (s.substeps == k && s.numAllocated > 0)
+                    if (!rc && s.substeps == k && s.numAllocated > 0)
                     {
-                        if ((rc = crHipBatchSetSceneNumActiveParticles(bt.b, s.slot, 0)))
-                            return rc;
+                        rc = crHipBatchSetSceneNumActiveParticles(bt.b, s.slot, 0);
                         resting.push_back(i);
                     }
                 }
-                if ((rc = crHipBatchStep(bt.b)))
-                    return rc;
+                if (!rc)
+                    rc = crHipBatchStep(bt.b);
             }
+            // resting scenes get their particles back even after a failed step
             for (int i : resting)
-                if ((rc = crHipBatchSetSceneNumActiveParticles(bt.b, e->scenes[i].slot, e->scenes[i].numAllocated)))
-                    return rc;
+            {
+                const int r = crHipBatchSetSceneNumActiveParticles(bt.b, e->scenes[i].slot, e->scenes[i].numAllocated);
+                rc = rc ? rc : r;
+            }
+            if (rc)
+                return rc;
         }
         // SceneImpl::afterAdvance and afterSceneAdvance: positions,
         // velocities and coupling forces back to the host
