@@ -118,6 +118,10 @@ PROFILE_TARGETS = {
     "meshsdf": dict(run="python3 bench.py --workload meshsdf", counters=True,
                     title="Kernel trace and counters: mesh -> SDF table, 20,480-triangle icosphere on 128^3 nodes",
                     md="meshsdf_icosphere128_kernel_trace.md"),
+    "engine": dict(run="python3 bench.py --workload engine", counters=False,
+                   title="Kernel trace: engine advanceAll, 64 MLS scenes in frames of 10 substeps with host readback, "
+                         "then the same batch stepped alone",
+                   md="engine64_kernel_trace.md"),
     "adjoint": dict(run="python3 bench.py --workload adjoint", counters=False,
                     title="Kernel trace: differentiable MLS block, 65,536 particles, 100 substeps, forward + adjoint",
                     md="adjoint65k_kernel_trace.md"),
