@@ -183,3 +183,76 @@ def test_engine_scene_with_a_device_built_mesh_table():
         print(f"{k}: engine vs oracle {err:.2e} (bound {tol:.2e})", flush=True)
         assert err <= tol, (k, err, tol)
     eng.close()
+
+
+def _sheet(half_x, half_z, nx=4, nz=8):
+    xs, zs = np.linspace(-half_x, half_x, nx + 1), np.linspace(-half_z, half_z, nz + 1)
+    pts = np.array([[x, 0.0, z] for z in zs for x in xs], np.float32)
+    tris = []
+    for j in range(nz):
+        for i in range(nx):
+            a, b = j * (nx + 1) + i, j * (nx + 1) + i + 1
+            c, d = a + nx + 1, b + nx + 1
+            tris += [(a, c, b), (b, c, d)]
+    return pts, np.array(tris, np.uint32)
+
+
+def test_engine_scene_with_a_device_built_mesh_slicer():
+    """A mesh-slicer geometry (a triangulated sheet with a spine) whose table,
+    spine row included, is built on the device (crHipMeshSlicerSdf) into the
+    engine's SDF rows, cutting through a block for 10 substeps; against the
+    oracle running the scene with its own restated slicer table."""
+    import ctypes
+    import math
+
+    import crmpm_hip as H
+
+    sc = S.small_scene(S.MLS, 13, shapes="plane")
+    spacing = sc["dx"] / 2
+    side = 8 * spacing
+    pts, tris = _sheet(side / 3, side)
+    cell, fatten, spine = spacing, 2 * spacing, (0.0, 0.0, 1.5, -side / 2)
+    dims, lower, rows = CC.mesh_slicer_sdf(pts, tris, cell, fatten, spine)
+    assert np.isnan(rows[1:, 3]).any() and (~np.isnan(rows[1:, 3])).any()
+    off, tid = S.add_sdf_grid(sc, tuple(int(d) for d in dims), tuple(float(x) for x in lower), cell, rows)
+    a = math.radians(90.0)
+    S.add_shape(sc, S.TRIMESH_SLICER, (S.int_bits(off), S.int_bits(tid), 0, 0),
+                pos=(0.5 + 0.3 * spacing, 0.2 + side, 0.5), rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)),
+                stype=S.KINEMATIC, params0=(0.0, 1.0, 0.0, 0.001))
+    host_sc = copy.deepcopy(sc)
+    host_sc["sdf_data"][off:off + len(rows)] = 0.0  # the host copy does not hold the table
+    eng, ids, _ = H.engine_from_scenes([host_sc])
+    eng.advance_all(0.0)
+    eng.fetch_results_all()
+    _, _, sdf = eng.device_tables()
+    r0 = ids[0][3]
+    d3, l3 = (ctypes.c_int * 3)(), (ctypes.c_float * 3)()
+    H.check(eng.lib.crHipMeshSlicerSdf(H._vp(pts.ctypes.data), len(pts), H._vp(tris.ctypes.data), len(tris), cell,
+                                       fatten, (ctypes.c_float * 4)(*spine), d3, l3,
+                                       H._vp(sdf.gradientSignedDistance + 16 * (r0 + off)), len(rows), None),
+            "mesh slicer sdf into the engine")
+    eng.advance_all(9.5 * sc["dt"])
+    eng.fetch_results_all()
+    hd = eng.host(0)
+    st = {"xm": hd["xm"].astype(np.float64), "v": hd["v"][:, :3].astype(np.float64)}
+    st.update(eng.state(0))
+    runs = []
+    for fp64 in (False, True):
+        o = CC.CpuSolver(copy.deepcopy(sc), "oracle", fp64=fp64)
+        o.compute_initial_data()
+        o.step(10)
+        runs.append(o.state())
+        o.close()
+    free = CC.CpuSolver(copy.deepcopy(S.small_scene(S.MLS, 13, shapes="plane")), "oracle")
+    free.compute_initial_data()
+    free.step(10)
+    fs = free.state()
+    free.close()
+    os_, od_ = runs
+    rel = lambda a, b: float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-30)  # noqa: E731
+    assert rel(os_["v"], fs["v"]) > 1e-4, "the slicer does not act on the block"
+    for k in ("xm", "v", "F"):
+        err, tol = rel(st[k], os_[k]), max(2e-5, 2.0 * rel(os_[k], od_[k]))
+        print(f"{k}: engine vs oracle {err:.2e} (bound {tol:.2e})", flush=True)
+        assert err <= tol, (k, err, tol)
+    eng.close()
