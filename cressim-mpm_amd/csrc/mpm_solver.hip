@@ -860,14 +860,16 @@ namespace
     }
 
     // ------------------------------------------------------------------
-    // Order of the particles inside each tile: by (rank within the base cell,
-    // base cell), ranks counted in sorted order.  The tile kernels run 64
-    // lanes over consecutive particles of a tile, and lanes that share a base
-    // cell take turns in the conflict-free scatter.  In this order each rank
-    // level holds one particle per occupied cell, so a dense tile (several
-    // particles per cell) scatters in one or two passes per 64-lane chunk
-    // instead of one pass per particle of its fullest cell.  One wavefront per
-    // tile; tmp holds (rank << 6 | cell) per sorted particle.
+    // Order of the particles inside each tile: by (rank pair, base cell,
+    // rank), ranks counted in sorted order.  The tile kernels run 64 lanes
+    // over consecutive particles of a tile, and lanes that share a base cell
+    // take turns in the conflict-free scatter.  In this order each level
+    // holds up to two particles per occupied cell (ranks 2p and 2p + 1, which
+    // the two accumulation tiles take in one pass), so a dense tile scatters
+    // in one or two passes per 64-lane chunk instead of one pass per particle
+    // of its fullest cell, and a chunk reads both members of each pair of
+    // neighbouring particles.  One wavefront per tile; tmp holds
+    // (rank << 6 | cell) per sorted particle.
     // ------------------------------------------------------------------
     __global__ __launch_bounds__(kWG) void k_cell_rank_order(BatchArgs b, const unsigned *order, unsigned *tmp,
                                                              unsigned *orderOut)
@@ -948,7 +950,16 @@ namespace
                 const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
                 const unsigned long long below = (cell == 0) ? 0ull : ((~0ull) >> (64 - cell));
                 int pos;
-                if (r < levels)
+                if (r < (levels & ~1))
+                {
+                    // ranks in pairs: level p holds ranks 2p and 2p + 1 of each
+                    // cell side by side, so a 64-lane chunk reads whole runs of
+                    // neighbouring particles and still scatters in one pass of
+                    // the two accumulation tiles
+                    const int r2 = r & ~1;
+                    pos = LS[r2] + __popcll(LM[r2] & below) + __popcll(LM[r2 + 1] & below) + (r & 1);
+                }
+                else if (r < levels)
                     pos = LS[r] + __popcll(LM[r] & below);
                 else
                 {
