@@ -204,6 +204,25 @@ def particle_lattice(obj, device="cuda:0", capacity=None):
     return n.value, out[:m]
 
 
+def load_obj(path):
+    """Triangle mesh from a Wavefront OBJ file, as the reference's ObjLoader
+    reads one (tools/obj_loader.h:62-95): `v` lines give vertices (x y z),
+    `f` lines the first three vertex references of a face, each the 1-based
+    index before any '/'; every other line is skipped.  Returns float32
+    points (n, 3) and uint32 triangles (m, 3), ready for mesh_sdf."""
+    pts, tris = [], []
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if not w:
+                continue
+            if w[0] == "v":
+                pts.append([float(x) for x in w[1:4]])
+            elif w[0] == "f":
+                tris.append([int(t.split("/")[0]) - 1 for t in w[1:4]])
+    return np.asarray(pts, np.float32).reshape(-1, 3), np.asarray(tris, np.uint32).reshape(-1, 3)
+
+
 def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0"):
     """SdfBuilder's mesh -> SDF table on the device (crHipMeshSdf).  points
     (n, 3) float32 and triangles (m, 3) uint32 on the host.  Returns dims,

@@ -45,20 +45,7 @@ CUBE_TRIS = np.array([[4, 2, 0], [2, 7, 3], [6, 5, 7], [1, 7, 5], [0, 3, 1], [4,
                       [6, 4, 5], [1, 3, 7], [0, 2, 3], [4, 0, 1]], np.uint32)
 
 
-def load_obj(path):
-    """Vertices and triangles of a Wavefront OBJ file (v / f lines, 1-based
-    indices, texture and normal indices dropped)."""
-    pts, tris = [], []
-    with open(path) as f:
-        for line in f:
-            w = line.split()
-            if not w:
-                continue
-            if w[0] == "v":
-                pts.append([float(x) for x in w[1:4]])
-            elif w[0] == "f":
-                tris.append([int(x.split("/")[0]) - 1 for x in w[1:4]])
-    return np.asarray(pts, np.float32), np.asarray(tris, np.uint32)
+from crmpm_hip import load_obj  # noqa: E402
 
 
 def sdf_cases():

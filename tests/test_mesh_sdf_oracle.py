@@ -258,3 +258,17 @@ def test_capi_mesh_sdf_argument_errors():
     count = dims[0] * dims[1] * dims[2]
     assert lib.crHipMeshSlicerSdf(*args, sp, dims, lower, H._vp(16), count, None) == 5
     assert lib.crHipMeshSlicerSdf(*args, None, dims, lower, H._vp(16), count + 1, None) == 1
+
+
+def test_load_obj_reads_faces_like_the_reference_loader(tmp_path):
+    """crmpm_hip.load_obj: v lines, the first three references of f lines
+    (1-based, texture/normal indices dropped), everything else skipped."""
+    import crmpm_hip as H
+
+    p = tmp_path / "tri.obj"
+    p.write_text("# a comment\no quad\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvn 0 0 1\n"
+                 "f 1/1/1 2/1/1 3/1/1\nf 1//1 3//1 4//1\nf 4 3 2\n")
+    pts, tris = H.load_obj(str(p))
+    assert pts.dtype == np.float32 and tris.dtype == np.uint32
+    np.testing.assert_array_equal(pts, [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    np.testing.assert_array_equal(tris, [[0, 1, 2], [0, 2, 3], [3, 2, 1]])
