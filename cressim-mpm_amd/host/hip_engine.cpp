@@ -521,6 +521,8 @@ int crHipEngineGetSceneHostData(CrHipEngine *e, int scene, float **positionMass,
 {
     if (!valid_scene(e, scene))
         return CRMPM_HIP_ERR_ARG;
+    if (e->busy)
+        return CRMPM_HIP_ERR_STATE; // the advance writes these arrays
     const size_t off = e->scenes[scene].desc.particleOffset;
     if (positionMass)
         *positionMass = reinterpret_cast<float *>(e->hXm + off);
@@ -604,6 +606,8 @@ int crHipEngineGetTables(CrHipEngine *e, int device, CrHipShapeData *shapes, CrH
 {
     if (!e)
         return CRMPM_HIP_ERR_ARG;
+    if (e->busy)
+        return CRMPM_HIP_ERR_STATE;
     const Tables &t = device ? e->d : e->h;
     if (shapes)
         *shapes = shape_view(t, e->shapeCap);
@@ -663,6 +667,8 @@ int crHipEngineGetSceneSubsteps(const CrHipEngine *e, int scene, int *substeps)
 {
     if (!valid_scene(e, scene) || !substeps)
         return CRMPM_HIP_ERR_ARG;
+    if (e->busy)
+        return CRMPM_HIP_ERR_STATE;
     *substeps = e->scenes[scene].substeps;
     return CRMPM_HIP_OK;
 }

@@ -119,6 +119,7 @@ def test_engine_state_errors():
     eng.advance_all(0.0)  # no particle allocated: builds the batch, takes no substep
     assert lib.crHipEngineAdvanceAll(eng.h, 0.01) == 2  # one advance at a time
     assert lib.crHipEngineMarkSceneDirty(eng.h, sid, 1) == 2
+    assert lib.crHipEngineGetSceneHostData(eng.h, sid, None, None, None, None, None) == 2
     eng.fetch_results_all()
     assert eng.substeps(sid) == 0
     assert lib.crHipEngineCreateScene(eng.h, 1, 1, sc["dx"], H._f3(*sc["lo"]), H._f3(*sc["hi"]),
