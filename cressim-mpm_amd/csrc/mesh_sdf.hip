@@ -445,15 +445,12 @@ namespace crhip
             const float d = norm(bestP - q);
             if (!Slicer)
             {
-                // the far-field sum cannot decide the side of 0.5: sum every
-                // triangle's term exactly
-                if (!(fabsf(wn - 0.5f) > err + kWnSlack))
-                {
-                    wn = 0.0f;
-                    for (int t = 0; t < a.numTris; ++t)
-                        wn += winding_term(v3(a.tri[4 * t]), v3(a.tri[4 * t + 1]), v3(a.tri[4 * t + 2]), q);
-                }
-                dist[node] = wn > 0.5f ? -d : d;
+                // where the far-field sum cannot decide the side of 0.5 the
+                // node is flagged (onMesh carries the flag here) and
+                // k_mesh_sdf_exact sums every triangle's term
+                const bool undecided = !(fabsf(wn - 0.5f) > err + kWnSlack);
+                onMesh[node] = undecided ? 1 : 0;
+                dist[node] = (!undecided && wn > 0.5f) ? -d : d;
                 return;
             }
             dist[node] = d;
@@ -471,6 +468,41 @@ namespace crhip
                     off = false;
             }
             onMesh[node] = off ? 0 : 1;
+        }
+
+        // The exact winding number of the nodes the combine pass left
+        // undecided, bricks as in k_mesh_sdf_slices, every triangle staged
+        // through LDS; workgroups whose four bricks hold no undecided node
+        // leave at once.
+        __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_exact(MeshSdfArgs a, const unsigned char *undecided,
+                                                                  float *dist)
+        {
+            __shared__ float4 lds[kSdfChunk * 3];
+            const int brick = (int)blockIdx.x * (kSdfWG / 64) + (int)(threadIdx.x >> 6);
+            const int lane = threadIdx.x & 63;
+            const int bz = brick / (a.bx * a.by), rem = brick - bz * a.bx * a.by;
+            const int by = rem / a.bx, bx = rem - by * a.bx;
+            const int x = bx * 4 + (lane & 3), y = by * 4 + ((lane >> 2) & 3), z = bz * 4 + (lane >> 4);
+            const bool valid = brick < a.nBricks && x < a.dx && y < a.dy && z < a.dz;
+            const long long node = ((long long)z * a.dy + y) * a.dx + x;
+            const bool need = valid && undecided[node];
+            if (!__syncthreads_or(need))
+                return;
+            const V3 q = node_point(a, x, y, z);
+            float wn = 0.0f;
+            for (int base = 0; base < a.numTris; base += kSdfChunk)
+            {
+                const int n = min(kSdfChunk, a.numTris - base);
+                __syncthreads();
+                for (int i = threadIdx.x; i < 3 * n; i += kSdfWG)
+                    lds[i] = a.tri[4 * (long long)base + 4 * (i / 3) + i % 3];
+                __syncthreads();
+                if (need)
+                    for (int t = 0; t < n; ++t)
+                        wn += winding_term(v3(lds[3 * t]), v3(lds[3 * t + 1]), v3(lds[3 * t + 2]), q);
+            }
+            if (need && wn > 0.5f)
+                dist[node] = -dist[node];
         }
 
         // computeSdfGradient (sdf_builder.cpp:242-330): central differences
@@ -706,7 +738,7 @@ namespace
         float4 *dTri = nullptr, *dCen = nullptr, *dClu = nullptr, *dPart = nullptr;
         float *dWn = nullptr, *dErr = nullptr, *dDist = nullptr;
         int *dFlags = nullptr;
-        unsigned char *dMask = nullptr;
+        unsigned char *dMask = nullptr; // per node: slicer on-surface flag, or SDF sign left undecided
         rc = CRMPM_HIP_ERR_ALLOC;
         if (hipMalloc(&dTri, tri.size() * sizeof(float4)) == hipSuccess &&
             hipMalloc(&dCen, cen.size() * sizeof(float4)) == hipSuccess &&
@@ -715,8 +747,8 @@ namespace
             hipMalloc(&dPart, (size_t)usedSlices * count * sizeof(float4)) == hipSuccess &&
             hipMalloc(&dWn, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
             hipMalloc(&dDist, (size_t)count * sizeof(float)) == hipSuccess &&
-            (!slicer || (hipMalloc(&dFlags, flags.size() * sizeof(int)) == hipSuccess &&
-                         hipMalloc(&dMask, (size_t)count) == hipSuccess)))
+            hipMalloc(&dMask, (size_t)count) == hipSuccess &&
+            (!slicer || hipMalloc(&dFlags, flags.size() * sizeof(int)) == hipSuccess))
         {
             rc = CRMPM_HIP_ERR_LAUNCH;
             a.tri = dTri;
@@ -748,9 +780,12 @@ namespace
                 else
                 {
                     k_mesh_sdf_slices<false><<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
-                    k_mesh_sdf_combine<false><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, nullptr);
+                    k_mesh_sdf_combine<false><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, dMask);
+                    k_mesh_sdf_exact<<<a.groups, kSdfWG, 0, st>>>(a, dMask, dDist);
                 }
-                k_mesh_sdf_gradient<<<nodeWGs, kSdfWG, 0, st>>>(a.dx, a.dy, a.dz, count, cellSize, dDist, dMask, rows);
+                // dMask: the slicer's on-surface flags; for the SDF, the undecided nodes (done with)
+                k_mesh_sdf_gradient<<<nodeWGs, kSdfWG, 0, st>>>(a.dx, a.dy, a.dz, count, cellSize, dDist,
+                                                                slicer ? dMask : nullptr, rows);
                 ok = hipGetLastError() == hipSuccess;
             }
             // the copies read host memory: wait for them whatever failed
