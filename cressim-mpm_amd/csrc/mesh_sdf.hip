@@ -11,13 +11,17 @@
 // Work: every node against every triangle.  The reference walks an AABB tree
 // per node for the closest triangle and a clustered far-field approximation
 // of the winding number for the sign; here a wavefront owns a 4x4x4 brick of
-// nodes and streams the triangles through LDS, each lane evaluating the
-// triangle's solid angle for its node and, unless the triangle's box is
-// provably too far from the brick for any lane to take it, the
-// point-triangle distance.  The bricks keep the lanes of a wavefront in the
+// nodes and walks the triangles in clusters of 16: a cluster far from the
+// brick adds its far-field winding term (with an error bound), a near one
+// each triangle's solid angle, and the triangles' distances are evaluated
+// unless their boxes are provably too far from the brick for any lane to
+// take them; triangles are staged through LDS only for clusters some
+// wavefront of the workgroup needs.  The bricks keep the lanes of a wavefront in the
 // same Voronoi region of most triangles, so the region branches rarely
 // diverge.  Small tables split the triangle list into slices over more
-// workgroups; a combine pass takes the slices in order.
+// workgroups; a combine pass takes the slices in order, and the nodes whose
+// sign the clustered far-field winding sum leaves undecided get the exact sum
+// in one more pass.
 //
 // Results against the reference's arithmetic (the distances are evaluated
 // under -ffp-contract=off with IEEE division and square root):
