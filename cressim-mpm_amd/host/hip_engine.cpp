@@ -332,16 +332,13 @@ namespace
                 return rc;
         }
         // SceneImpl::afterAdvance and afterSceneAdvance: positions,
-        // velocities and coupling forces back to the host
-        for (auto &s : e->scenes)
-        {
-            const size_t off = s.desc.particleOffset, n = s.desc.numMaxParticles;
-            if (hipMemcpyAsync(e->hXm + off, e->dXm + off, n * sizeof(float4), hipMemcpyDeviceToHost, e->stream) !=
-                    hipSuccess ||
-                hipMemcpyAsync(e->hV + off, e->dV + off, n * sizeof(float4), hipMemcpyDeviceToHost, e->stream) !=
-                    hipSuccess)
-                return CRMPM_HIP_ERR_LAUNCH;
-        }
+        // velocities and coupling forces back to the host. Scenes tile
+        // [0, nextOffset) back to back, so one copy per array moves every
+        // scene's range (per-scene copies cost a launch each).
+        const size_t all = (size_t)e->nextOffset;
+        if (hipMemcpyAsync(e->hXm, e->dXm, all * sizeof(float4), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipMemcpyAsync(e->hV, e->dV, all * sizeof(float4), hipMemcpyDeviceToHost, e->stream) != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
         for (int k : {kForce, kTorque})
             if (hipMemcpyAsync(e->h.f4[k], e->d.f4[k], (size_t)e->shapeCap * sizeof(float4), hipMemcpyDeviceToHost,
                                e->stream) != hipSuccess)
