@@ -874,104 +874,151 @@ namespace
     __global__ __launch_bounds__(kWG) void k_cell_rank_order(BatchArgs b, const unsigned *order, unsigned *tmp,
                                                              unsigned *orderOut)
     {
-        constexpr int kLevels = 32;
+        // 128 rank levels cover cells of up to 127 particles (a body packed at
+        // under a quarter of the cell spacing) without the per-cell fallback
+        constexpr int kLevels = 128, kAhead = 4;
         __shared__ int cnt[kWaves][64];
         __shared__ unsigned long long levelMask[kWaves][kLevels];
         __shared__ int levelStart[kWaves][kLevels];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const float4 *xm = b.P.xm;
-        for (int tg = first_tile(wave); tg < b.totalBlocks; tg += tile_stride())
+        // The lanes test the wavefront's next 64 pblocks (tg0, tg0 + W, ...)
+        // for particles at once, and the wavefront visits only the occupied
+        // ones: most of a batch of sparse grids is empty.
+        const int W = tile_stride();
+        for (int tg0 = first_tile(wave); tg0 < b.totalBlocks; tg0 += 64 * W)
         {
-            int t;
-            const StepCommon sv = scene_view(b, tg, t);
-            const GridGeom &g = sv.g;
-            TileInfo ti;
-            if (!tile_info(g, t, sv.segStart, sv.segEnd, ti))
-                continue;
-            wave_sync();
-            int *C = cnt[wave];
-            C[lane] = 0;
-            wave_sync();
-            const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-            for (int base = ti.start; base < ti.end; base += 64)
+            const long long mineTile = (long long)tg0 + (long long)lane * W;
+            const bool used = mineTile < b.totalBlocks && b.segEnd[mineTile] > b.segStart[mineTile];
+            unsigned long long todo = __ballot(used);
+            while (todo != 0ull)
             {
-                const int k = base + lane;
-                const bool valid = k < ti.end;
-                unsigned cell = 64u;
-                if (valid)
-                {
-                    int bx, by, bz;
-                    V3 fx;
-                    cell_of(g, v3(xm[order[k]]), bx, by, bz, fx);
-                    bx = min(max(bx, 0), g.nx - 1);
-                    by = min(max(by, 0), g.ny - 1);
-                    bz = min(max(bz, 0), g.nz - 1);
-                    cell = (unsigned)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
-                }
-                const unsigned long long peers = match_lanes(cell);
-                const int below = __popcll(peers & lt);
-                const int r = valid ? C[cell] + below : 0;
+                const int tg = __builtin_amdgcn_readfirstlane(tg0 + (__ffsll((long long)todo) - 1) * W);
+                todo &= todo - 1ull;
+                int t;
+                const StepCommon sv = scene_view(b, tg, t);
+                const GridGeom &g = sv.g;
+                TileInfo ti;
+                if (!tile_info(g, t, sv.segStart, sv.segEnd, ti))
+                    continue;
                 wave_sync();
-                if (valid && below == 0)
-                    C[cell] += __popcll(peers);
+                int *C = cnt[wave];
+                C[lane] = 0;
                 wave_sync();
-                if (valid)
-                    tmp[k] = ((unsigned)r << 6) | cell;
-            }
-            const int mine = C[lane];
-            // Position of (rank r, cell) = particles of lower rank + rank-r
-            // particles of lower cells.  Per rank level up to kLevels: the mask
-            // of cells holding a rank-r particle and the level's start; deeper
-            // levels (very dense cells) take the per-cell sum.
-            int top = mine;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1)
-                top = max(top, __shfl_xor(top, o, 64));
-            const int levels = min(top, kLevels);
-            unsigned long long *LM = levelMask[wave];
-            int *LS = levelStart[wave];
-            int start = 0;
-            for (int r = 0; r < levels; ++r)
-            {
-                const unsigned long long m = __ballot(mine > r);
-                if (lane == 0)
+                const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+                // kAhead chunks' cells are loaded before their ranks are counted:
+                // the order[k] -> xm gather chains of a dense tile overlap instead
+                // of running one chunk at a time.
+                for (int base = ti.start; base < ti.end; base += 64 * kAhead)
                 {
-                    LM[r] = m;
-                    LS[r] = start;
-                }
-                start += __popcll(m);
-            }
-            wave_sync();
-            for (int base = ti.start; base < ti.end; base += 64)
-            {
-                const int k = base + lane;
-                const bool valid = k < ti.end;
-                const unsigned rc = valid ? tmp[k] : 0u;
-                const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
-                const unsigned long long below = (cell == 0) ? 0ull : ((~0ull) >> (64 - cell));
-                int pos;
-                if (r < (levels & ~1))
-                {
-                    // ranks in pairs: level p holds ranks 2p and 2p + 1 of each
-                    // cell side by side, so a 64-lane chunk reads whole runs of
-                    // neighbouring particles and still scatters in one pass of
-                    // the two accumulation tiles
-                    const int r2 = r & ~1;
-                    pos = LS[r2] + __popcll(LM[r2] & below) + __popcll(LM[r2 + 1] & below) + (r & 1);
-                }
-                else if (r < levels)
-                    pos = LS[r] + __popcll(LM[r] & below);
-                else
-                {
-                    pos = 0;
-                    for (int c = 0; c < 64; ++c)
+                    unsigned cells[kAhead];
+    #pragma unroll
+                    for (int u = 0; u < kAhead; ++u)
                     {
-                        const int nc = __builtin_amdgcn_readlane(mine, c);
-                        pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                        const int k = base + 64 * u + lane;
+                        cells[u] = 64u;
+                        if (k < ti.end)
+                        {
+                            int bx, by, bz;
+                            V3 fx;
+                            cell_of(g, v3(xm[order[k]]), bx, by, bz, fx);
+                            bx = min(max(bx, 0), g.nx - 1);
+                            by = min(max(by, 0), g.ny - 1);
+                            bz = min(max(bz, 0), g.nz - 1);
+                            cells[u] = (unsigned)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
+                        }
+                    }
+    #pragma unroll
+                    for (int u = 0; u < kAhead; ++u)
+                    {
+                        if (base + 64 * u >= ti.end)
+                            break;
+                        const int k = base + 64 * u + lane;
+                        const bool valid = k < ti.end;
+                        const unsigned cell = cells[u];
+                        const unsigned long long peers = match_lanes(cell);
+                        const int below = __popcll(peers & lt);
+                        const int r = valid ? C[cell] + below : 0;
+                        wave_sync();
+                        if (valid && below == 0)
+                            C[cell] += __popcll(peers);
+                        wave_sync();
+                        if (valid)
+                            tmp[k] = ((unsigned)r << 6) | cell;
                     }
                 }
-                if (valid)
-                    orderOut[ti.start + pos] = order[k];
+                const int mine = C[lane];
+                // Position of (rank r, cell) = particles of lower rank + rank-r
+                // particles of lower cells.  Per rank level up to kLevels: the mask
+                // of cells holding a rank-r particle and the level's start; deeper
+                // levels (very dense cells) take the per-cell sum.
+                int top = mine;
+    #pragma unroll
+                for (int o = 32; o > 0; o >>= 1)
+                    top = max(top, __shfl_xor(top, o, 64));
+                const int levels = min(top, kLevels);
+                unsigned long long *LM = levelMask[wave];
+                int *LS = levelStart[wave];
+                int start = 0;
+                for (int r = 0; r < levels; ++r)
+                {
+                    const unsigned long long m = __ballot(mine > r);
+                    if (lane == 0)
+                    {
+                        LM[r] = m;
+                        LS[r] = start;
+                    }
+                    start += __popcll(m);
+                }
+                wave_sync();
+                for (int base = ti.start; base < ti.end; base += 64 * kAhead)
+                {
+                    unsigned rcs[kAhead], src[kAhead];
+    #pragma unroll
+                    for (int u = 0; u < kAhead; ++u)
+                    {
+                        const int k = base + 64 * u + lane;
+                        rcs[u] = 0u;
+                        src[u] = 0u;
+                        if (k < ti.end)
+                        {
+                            rcs[u] = tmp[k];
+                            src[u] = order[k];
+                        }
+                    }
+    #pragma unroll
+                    for (int u = 0; u < kAhead; ++u)
+                    {
+                        const int k = base + 64 * u + lane;
+                        const bool valid = k < ti.end;
+                        const unsigned rc = rcs[u];
+                        const int r = (int)(rc >> 6), cell = (int)(rc & 63u);
+                        const unsigned long long below = (cell == 0) ? 0ull : ((~0ull) >> (64 - cell));
+                        int pos;
+                        if (r < (levels & ~1))
+                        {
+                            // ranks in pairs: level p holds ranks 2p and 2p + 1 of each
+                            // cell side by side, so a 64-lane chunk reads whole runs of
+                            // neighbouring particles and still scatters in one pass of
+                            // the two accumulation tiles
+                            const int r2 = r & ~1;
+                            pos = LS[r2] + __popcll(LM[r2] & below) + __popcll(LM[r2 + 1] & below) + (r & 1);
+                        }
+                        else if (r < levels)
+                            pos = LS[r] + __popcll(LM[r] & below);
+                        else
+                        {
+                            pos = 0;
+                            for (int c = 0; c < 64; ++c)
+                            {
+                                const int nc = __builtin_amdgcn_readlane(mine, c);
+                                pos += min(nc, r) + ((c < cell && nc > r) ? 1 : 0);
+                            }
+                        }
+                        if (valid)
+                            orderOut[ti.start + pos] = src[u];
+                    }
+                }
             }
         }
     }
