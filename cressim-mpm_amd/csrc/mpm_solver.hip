@@ -210,6 +210,28 @@ namespace
         return count > 0 && (j == 0 || count > j * subSize);
     }
     CR_DEV bool subtile_used(const BatchArgs &b, int tg) { return split_used(b, tg, b.subSize, b.subMax); }
+    // First occupied item at or after tg of a persistent tile loop's items
+    // tg, tg + W, tg + 2W, ... (W = wavefronts in the launch): the 64 lanes
+    // test 64 candidates with one load each, so a wavefront passes the empty
+    // pblocks of sparse grids without a dependent load per pblock and visits
+    // the same items in the same order.  A non-persistent launch (one item
+    // per wavefront) keeps its own test.
+    template <bool Persistent>
+    CR_DEV int skip_empty(const BatchArgs &b, int tg, int items, int subSize, int subMax)
+    {
+        if (!Persistent)
+            return tg;
+        const long long W = tile_stride();
+        const int lane = threadIdx.x & 63;
+        for (long long t = tg; t < items; t += 64 * W)
+        {
+            const long long c = t + lane * W;
+            const unsigned long long m = __ballot(c < items && split_used(b, (int)c, subSize, subMax));
+            if (m != 0ull)
+                return __builtin_amdgcn_readfirstlane((int)(t + (__ffsll((long long)m) - 1) * W));
+        }
+        return items;
+    }
     // Next pblock of a tile loop; a non-persistent launch (one wavefront per
     // pblock) compiles to straight-line code, which needs fewer registers.
     template <bool Persistent>
@@ -1154,7 +1176,8 @@ namespace
     {
         __shared__ float4 acc[kWaves * kScatterTiles][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1458,7 +1481,8 @@ namespace
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = first_tile(wave); tg < items; tg = next_tile<Persistent>(tg, items))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubSize, b.gSubMax); tg < items;
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubSize, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
@@ -1593,7 +1617,8 @@ namespace
         __shared__ float4 acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1693,7 +1718,8 @@ namespace
     {
         __shared__ float4 acc[kWaves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<Persistent>(tg, num_subtiles(b)))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1770,7 +1796,8 @@ namespace
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = first_tile(wave); tg < items; tg = next_tile<Persistent>(tg, items))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubSize, b.gSubMax); tg < items;
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubSize, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
