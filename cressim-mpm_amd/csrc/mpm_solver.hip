@@ -214,21 +214,23 @@ namespace
     // tg, tg + W, tg + 2W, ... (W = wavefronts in the launch): the 64 lanes
     // test 64 candidates with one load each, so a wavefront passes the empty
     // pblocks of sparse grids without a dependent load per pblock and visits
-    // the same items in the same order.  A non-persistent launch (one item
-    // per wavefront) keeps its own test.
+    // the same items in the same order.  Taken when items are whole pblocks
+    // (subMax 1) and W <= 2^24, where t + 64 W stays below 2^32; otherwise,
+    // and in a non-persistent launch (one item per wavefront), the loop body
+    // keeps its own test.
     template <bool Persistent>
-    CR_DEV int skip_empty(const BatchArgs &b, int tg, int items, int subSize, int subMax)
+    CR_DEV int skip_empty(const BatchArgs &b, int tg, int items, int subMax)
     {
-        if (!Persistent)
+        if (!Persistent || subMax != 1 || tile_stride() > (1 << 24))
             return tg;
-        const long long W = tile_stride();
-        const int lane = threadIdx.x & 63;
-        for (long long t = tg; t < items; t += 64 * W)
+        const unsigned W = (unsigned)tile_stride(), n = (unsigned)items;
+        const unsigned lane = threadIdx.x & 63u;
+        for (unsigned t = (unsigned)tg; t < n; t += 64u * W)
         {
-            const long long c = t + lane * W;
-            const unsigned long long m = __ballot(c < items && split_used(b, (int)c, subSize, subMax));
+            const unsigned c = t + lane * W;
+            const unsigned long long m = __ballot(c < n && b.segEnd[c] > b.segStart[c]);
             if (m != 0ull)
-                return __builtin_amdgcn_readfirstlane((int)(t + (__ffsll((long long)m) - 1) * W));
+                return __builtin_amdgcn_readfirstlane((int)(t + (unsigned)(__ffsll((long long)m) - 1) * W));
         }
         return items;
     }
@@ -1176,8 +1178,8 @@ namespace
     {
         __shared__ float4 acc[kWaves * kScatterTiles][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1481,8 +1483,8 @@ namespace
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubSize, b.gSubMax); tg < items;
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubSize, b.gSubMax))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubMax); tg < items;
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
@@ -1617,8 +1619,8 @@ namespace
         __shared__ float4 acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1718,8 +1720,8 @@ namespace
     {
         __shared__ float4 acc[kWaves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subSize, b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subSize, b.subMax))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -1796,8 +1798,8 @@ namespace
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubSize, b.gSubMax); tg < items;
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubSize, b.gSubMax))
+        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubMax); tg < items;
+             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
