@@ -126,7 +126,14 @@ namespace crhip
     // ------------------------------------------------------------------
     // 3x3 SVD, McAdams et al. 2011 (branch-free Jacobi on A^T A with
     // quaternion accumulation, 4 sweeps; sort; Givens QR), all selects,
-    // no divergent branches.
+    // no divergent branches.  The sequence of operations is that of the
+    // reference's svd3.h (include/math/svd3.h:74-1260), which derives from
+    // "Computing the Singular Value Decomposition of 3x3 matrices with minimal
+    // branching and elementary floating point operations" by A. McAdams,
+    // A. Selle, R. Tamstorf, J. Teran and E. Sifakis (University of
+    // Wisconsin-Madison technical report TR1690, 2011) and its BSD-licensed
+    // reference implementation (Copyright (c) 2011 the authors): kept so that
+    // PB-MPM results match the reference's bit for bit.
     // ------------------------------------------------------------------
     CR_DEV void svd_conj(float &sPP, float &sQQ, float &sPQ, float &sRR, float &sRP, float &sRQ,
                          float &qs, float &qR, float &qP, float &qQ)

@@ -3,7 +3,7 @@
 // Data layout in HBM
 //   engine-owned (bound):  positionMass float4[N], velocity float4[N], params0 float4[N],
 //                          type int[N], mask u8[N], shape/geometry/SDF tables
-//   solver-owned:          F, C  9 x float planes [9][stride] (column-major element k),
+//   solver-owned:          F, C  three float4 columns per particle (F[3p + c], w unused),
 //                          V0 float[N]
 //   grid:                  4x4x4-node blocks, 1 KiB each (float4 (momentum|velocity, mass) per
 //                          node), block b = bx + nbx * (by + nby * bz), node (lx,ly,lz) at
@@ -110,6 +110,11 @@ namespace
         int particleOffset;
         int numActive;
         int activeOffset; // sort item of the scene's first active particle
+        // A resting scene sits out the substep: no particle is processed and
+        // its grid blocks are neither cleared nor written, so its grid keeps
+        // the last update's state (an engine scene that has taken its
+        // substeps while the others of its batch finish).
+        int resting;
     };
 
     // Shape tables bound to a scene (engine-owned), read when a step starts.
@@ -369,7 +374,7 @@ namespace
     __global__ __launch_bounds__(kWG) void k_prepare(int totalBlocks, int blockWGs, float4 *grid, float4 *gridForce,
                                                      unsigned char *dirty, int *segStart, int *segEnd,
                                                      const SceneShapes *shapes, const SceneRec *scenes,
-                                                     int *activeCount)
+                                                     const int *blockScene, int numScenes, int *activeCount)
     {
         const int lane = threadIdx.x & 63;
         if ((int)blockIdx.x >= blockWGs)
@@ -404,8 +409,13 @@ namespace
         bool wasDirty = false;
         if (b < totalBlocks)
         {
-            wasDirty = dirty[b] != 0;
-            dirty[b] = 0;
+            // a resting scene's blocks keep their contents and their flags
+            const int sc = numScenes == 1 ? 0 : blockScene[b];
+            if (!scenes[sc].resting)
+            {
+                wasDirty = dirty[b] != 0;
+                dirty[b] = 0;
+            }
             segStart[b] = 0;
             segEnd[b] = 0;
         }
@@ -2360,6 +2370,7 @@ namespace
         int maxParticles = 0;
         int particleOffset = 0;
         unsigned numActive = 0;
+        bool resting = false;
         int numShapes = 0;
         int *shapeIds = nullptr;
         CrHipShapeData shapes{};
@@ -2564,14 +2575,16 @@ namespace
             rec[s].sv = shape_view(b, s);
             rec[s].blockOffset = h.blockOffset;
             rec[s].particleOffset = h.particleOffset;
-            rec[s].numActive = (int)h.numActive;
+            const unsigned na = h.resting ? 0u : h.numActive;
+            rec[s].numActive = (int)na;
             rec[s].activeOffset = active;
+            rec[s].resting = h.resting ? 1 : 0;
             sh[s].numShapes = h.numShapes;
             sh[s].shapeIds = h.shapeIds;
             sh[s].sd = h.shapes;
             sh[s].gd = h.geoms;
             prefix[s] = active;
-            active += (int)h.numActive;
+            active += (int)na;
         }
         prefix[ns] = active;
         b->totalActive = active;
@@ -2715,7 +2728,7 @@ namespace
         const int blockWGs = grid1d(b->totalBlocks, 64 * kWaves); // 64 node blocks per wavefront
         launch(b, CRMPM_HIP_PROF_BINNING, k_prepare, blockWGs + b->numScenes, kWG, (int)b->totalBlocks, blockWGs, b->grid,
                b->gridForce, b->dirty, b->segStart, b->segEnd, (const SceneShapes *)b->dShapes, (const SceneRec *)b->dScenes,
-               b->activeCount);
+               (const int *)b->dBlockScene, b->numScenes, b->activeCount);
         const int n = b->totalActive;
         if (n == 0)
             return check_launch();
@@ -3204,7 +3217,10 @@ namespace
         char *backup = nullptr;
         if (dalloc(&backup, bytes))
             return CRMPM_HIP_ERR_ALLOC;
+        // the launch configuration in force, restored if a trial fails
+        const int prev[6] = {b->tileWaves, b->updateWaves, b->subSize, b->subMax, b->gSubSize, b->gSubMax};
         rc = tune_copy(b, backup, true);
+        const bool saved = rc == CRMPM_HIP_OK;
         // 1. sub-tile split: 64, 128 or 256 particles per sub-tile, or whole
         //    pblocks (one sub-tile each)
         const int splits[4][2] = {{64, b->subStride}, {128, b->subStride}, {256, b->subStride}, {1 << 20, 1}};
@@ -3271,10 +3287,24 @@ namespace
                 bestU = c;
             }
         }
-        if (!rc)
-            rc = tune_copy(b, backup, false);
+        // the state goes back to where it was whether or not every trial ran
+        if (saved)
+        {
+            const int rr = tune_copy(b, backup, false);
+            rc = rc ? rc : rr;
+        }
         (void)hipStreamSynchronize(b->stream);
         dfree(backup);
+        if (rc)
+        {
+            b->tileWaves = prev[0];
+            b->updateWaves = prev[1];
+            b->subSize = prev[2];
+            b->subMax = prev[3];
+            b->gSubSize = prev[4];
+            b->gSubMax = prev[5];
+            return rc;
+        }
         b->tileWaves = bestT;
         b->updateWaves = bestU;
         b->subSize = bestS;
@@ -3572,6 +3602,15 @@ int crHipBatchSetSceneNumActiveParticles(CrHipBatch *b, int scene, unsigned int 
     if (!valid_scene(b, scene) || n > (unsigned)b->scenes[scene].maxParticles)
         return CRMPM_HIP_ERR_ARG;
     b->scenes[scene].numActive = n;
+    b->scenesDirty = true;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchSetSceneResting(CrHipBatch *b, int scene, int resting)
+{
+    if (!valid_scene(b, scene))
+        return CRMPM_HIP_ERR_ARG;
+    b->scenes[scene].resting = resting != 0;
     b->scenesDirty = true;
     return CRMPM_HIP_OK;
 }

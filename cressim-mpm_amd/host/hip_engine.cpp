@@ -10,8 +10,9 @@
 // the substep loop: the reference steps each scene's solver in turn
 // (`while (t < dt) t += solver->step()`); here the scenes that share a solver
 // kind and iteration count are one CrHipBatch, advanced by one launch per
-// phase, and a scene whose float-accumulated time has reached dt sits out the
-// remaining substeps of the batch with no active particle.
+// phase, and a scene whose float-accumulated time has reached dt rests for
+// the remaining substeps of the batch: its particles are not processed and
+// its grid keeps the last update, as a solver that is not stepped does.
 //
 // advanceAll runs on a worker thread, as the reference submits it to its
 // thread manager; fetchResultsAll joins it.  Calls other than fetch are
@@ -313,19 +314,19 @@ namespace
                 for (int i : bt.scenes)
                 {
                     auto &s = e->scenes[i];
-                    if (!rc && s.substeps == k && s.numAllocated > 0)
+                    if (!rc && s.substeps == k)
                     {
-                        rc = crHipBatchSetSceneNumActiveParticles(bt.b, s.slot, 0);
+                        rc = crHipBatchSetSceneResting(bt.b, s.slot, 1);
                         resting.push_back(i);
                     }
                 }
                 if (!rc)
                     rc = crHipBatchStep(bt.b);
             }
-            // resting scenes get their particles back even after a failed step
+            // resting scenes take part again at the next advance, even after a failed step
             for (int i : resting)
             {
-                const int r = crHipBatchSetSceneNumActiveParticles(bt.b, e->scenes[i].slot, e->scenes[i].numAllocated);
+                const int r = crHipBatchSetSceneResting(bt.b, e->scenes[i].slot, 0);
                 rc = rc ? rc : r;
             }
             if (rc)

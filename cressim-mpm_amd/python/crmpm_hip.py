@@ -124,6 +124,7 @@ EXPORTS = {
     "crHipBatchSetSceneStepSize": [_vp, ctypes.c_int, ctypes.c_float],
     "crHipBatchSetSceneContactModel": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchSetSceneNumActiveParticles": [_vp, ctypes.c_int, ctypes.c_uint],
+    "crHipBatchSetSceneResting": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchBindSceneShapes": [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(ShapeData),
                                   ctypes.POINTER(GeometryData), ctypes.POINTER(GeometrySdfData)],
     "crHipBatchComputeInitialData": [_vp, ctypes.c_int, ctypes.c_uint, _vp],
@@ -904,9 +905,12 @@ class DifferentiableMls:
         hip_memcpy_d2d(C, self._zero.data_ptr(), self._zero.numel() * 4, sh)
 
     def forward(self, lam, mu, v0):
-        """Final positions (N, 3), float32; no graph (use rollout() for autograd)."""
+        """Final positions (N, 3), float32; no graph (use rollout() for autograd).
+        backward() differentiates the latest forward: each forward starts a
+        new generation, which a rollout's autograd node checks."""
         self._reset(lam, mu, v0)
         check(self.solver.lib.crHipAdjointForward(self.h), "adjoint forward")
+        self.generation = getattr(self, "generation", 0) + 1
         return self.solver.xm[:, :3].clone()
 
     def backward(self, grad_x, grad_v=None):
@@ -949,10 +953,17 @@ def _rollout_function():
         def forward(ctx, lam, mu, v0, sim):
             ctx.sim = sim
             ctx.dtypes = (lam.dtype, mu.dtype, v0.dtype)
-            return sim.forward(lam, mu, v0)
+            x = sim.forward(lam, mu, v0)
+            ctx.generation = sim.generation
+            return x
 
         @staticmethod
         def backward(ctx, grad_x):
+            # the adjoint recomputes from the checkpoints and parameters of the
+            # sim's latest forward: a later forward would give this graph its gradients
+            if ctx.sim.generation != ctx.generation:
+                raise RuntimeError("DifferentiableMls: another forward ran on this sim after this rollout; "
+                                   "backward of an earlier rollout is not available (one rollout per backward)")
             dlam, dmu, dv0 = ctx.sim.backward(grad_x)
             tl, tm, tv = ctx.dtypes
             return dlam.to(tl), dmu.to(tm), dv0.to(tv), None

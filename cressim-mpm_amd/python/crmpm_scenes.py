@@ -288,7 +288,7 @@ def config1(n_side=32, steps=200, seed=20261203 + 1):
     return sc
 
 
-def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, frame_substeps=10):
+def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, frame_substeps=10, embed=0.01):
     """Plain PB-MPM tissue slab: stratified nx x ny x nz particles, the full
     128 x 16 x 128 = 262,144-particle slab 0.961 x 0.2 x 0.961 m, smaller
     counts give a smaller slab at the same spacing, centred in x/z.
@@ -299,8 +299,14 @@ def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, 
     and a slab filling the whole 1 m box blows up in the reference solver
     itself within 3 substeps.  Neo-Hookean with PB parameters (alpha, r) =
     (0.5/dt, 0.9); a kinematic sphere indenter of radius 0.1 m (scaled with
-    the slab) starts 5 mm above the top and moves down at 0.05 m/s, friction
-    0.3; dt = 2e-3, 20 iterations per substep."""
+    the slab) starts with its lowest point `embed` (10 mm) below the top and
+    moves down at 0.05 m/s, friction 0.3; dt = 2e-3, 20 iterations per
+    substep.  The slab's top falls at nearly free fall for its first ~40
+    substeps (the PB constraint and the F update act on the dx-scaled affine
+    velocity C, src/solver/pb_mpm_algorithm.h:329,360,407, so the slab
+    offers little elastic support at dx = 1/128), faster than the indenter
+    descends: a sphere starting above the top would never touch it.  Starting
+    embedded, it is in contact with the slab for the first ~35 substeps."""
     rng = SplitMix64(seed)
     dx = 1.0 / 128
     dt = 2e-3
@@ -322,7 +328,7 @@ def config2(nx=128, ny=16, nz=128, steps=500, iterations=20, seed=20261203 + 2, 
     sc["type"][:] = NEOHOOKEAN
     top = y0 + slab[1]
     r = 0.1 * min(nx, nz) / 128
-    c0 = (0.5, top + 0.005 + r, 0.5)
+    c0 = (0.5, top - embed + r, 0.5)
     add_shape(sc, SPHERE, (r, 0, 0, 0), pos=c0, stype=KINEMATIC, params0=(0.0, 1.0, 0.3, 0.0))
     sc["steps"] = steps
     # Kinematic target per frame of `frame_substeps` substeps: the shape
@@ -408,14 +414,15 @@ def small_scene(kind, seed, n_side=8, shapes="plane", dx=1.0 / 32, jitter=0.25, 
     return sc
 
 
-def config3(n_scenes=64, kind=MLS, steps=100, seed=20261203 + 3, n_min=2000, n_max=200000, iterations=20):
+def config3(n_scenes=64, kind=MLS, steps=100, seed=20261203 + 3, n_min=2000, n_max=200000, iterations=20, gap=2.0):
     """Batched scenes: n_scenes independent soft bodies with particle counts
     log-uniform in [n_min, n_max] and grids of G = 4 round(U(8, 32)) cells per
     axis over the unit box.  Each body is a box or a sphere (equal odds) at
     8 particles per cell (lattice spacing dx / 2, enlarged when the body would
     exceed half the box), E ~ U(5e3, 5e4), nu ~ U(0.3, 0.45), falling at
     0.5 m/s onto one static SDF box (half extents U(0.05, 0.2)) or capsule
-    (radius U(0.02, 0.08), half height U(0.05, 0.2)) below it.  MLS uses the
+    (radius U(0.02, 0.08), half height U(0.05, 0.2)) below it, the body's
+    lowest lattice row `gap` cells above the obstacle's top.  MLS uses the
     co-rotational model (StVK, as the reference evaluates it); PB uses
     Neo-Hookean with (alpha, r) = (0.5 / dt, 0.9) and `iterations` iterations.
     dt = 1e-4."""
@@ -462,7 +469,7 @@ def config3(n_scenes=64, kind=MLS, steps=100, seed=20261203 + 3, n_min=2000, n_m
             a = math.radians(90.0)
             add_shape(sc, CAPSULE, (r, hh, 0, 0), pos=(0.5, obs_top - r, 0.5),
                       rot=(0.0, 0.0, math.sin(a / 2), math.cos(a / 2)), params0=(0.0, 1.0, 0.3, 0.0))
-        centre = np.array([0.5, obs_top + 2 * dx + half[1], 0.5])
+        centre = np.array([0.5, obs_top + gap * dx + half[1], 0.5])
         sc["xm"][:, :3] = centre + pts + jit
         sc["xm"][:, 3] = 1000.0 * h ** 3
         sc["v"][:, 1] = -0.5

@@ -222,6 +222,14 @@ int crHipBatchSetSceneStepSize(CrHipBatch *b, int scene, float dt);
 int crHipBatchSetSceneContactModel(CrHipBatch *b, int scene, int model);
 /* Particles [particleOffset, particleOffset + n) of the scene take part. */
 int crHipBatchSetSceneNumActiveParticles(CrHipBatch *b, int scene, unsigned int n);
+/* The scene sits out the following substeps while resting != 0: none of its
+ * particles is processed and its grid is neither cleared nor written, so the
+ * grid keeps its last update (what a solver that is not stepped holds, e.g.
+ * one that has taken its substeps in advanceAll,
+ * src/engine/simulation_factory_impl.cpp:1123-1136; computeInitialData then
+ * adds mass onto it, src/solver/gpu_mls_mpm_solver.cu:85-100).  No
+ * counterpart in the reference, whose solvers are stepped one by one. */
+int crHipBatchSetSceneResting(CrHipBatch *b, int scene, int resting);
 /* A scene's shapes: setNumShapes + bindShapeIds + bindShapeData + bindGeometryData
  * + bindGeometrySdfData of its solver (null table pointers keep the previous binding). */
 int crHipBatchBindSceneShapes(CrHipBatch *b, int scene, int numShapes, int *shapeIds, const CrHipShapeData *shapes,
@@ -337,12 +345,12 @@ int crHipEngineGetTables(CrHipEngine *e, int device, CrHipShapeData *shapes, CrH
 int crHipEngineMarkTablesDirty(CrHipEngine *e, unsigned int flags);
 /* SimulationFactory::advanceAll(dt) / fetchResultsAll(): each scene takes
  * the reference's substep count (dt accumulated in FP32 until it reaches
- * dt); within a batch a scene that has taken its count rests (no active
- * particle) while the others finish.  A resting scene's grid is cleared, so
- * computeInitialData queued for it at the next advance adds the new
- * particles' mass to an empty grid, where the reference's solver still holds
- * its last step's grid (Q10).  FetchResultsAll returns the advance's status;
- * CRMPM_HIP_ERR_STATE if no advance is in flight. */
+ * dt); within a batch a scene that has taken its count rests
+ * (crHipBatchSetSceneResting) while the others finish, so its grid keeps its
+ * last update, as the reference's solver that is not stepped does, and
+ * computeInitialData queued for it at the next advance adds mass onto that
+ * grid.  FetchResultsAll returns the advance's status; CRMPM_HIP_ERR_STATE
+ * if no advance is in flight. */
 int crHipEngineAdvanceAll(CrHipEngine *e, float dt);
 int crHipEngineFetchResultsAll(CrHipEngine *e);
 /* crHipBatchSetGraphMode / crHipBatchAutotune on every batch of the engine

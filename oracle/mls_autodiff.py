@@ -44,7 +44,7 @@ def substep(state, consts):
     dt, dx = consts["dt"], consts["dx"]
     lo, hi, dims = consts["lo"], consts["hi"], consts["dims"]
     g = consts["g"]
-    invdx = 1.0 / dx
+    invdx = consts.get("invdx", 1.0 / dx)
     n = x.shape[0]
     s = (x - lo) * invdx
     base = torch.trunc(s.detach() - 0.5)
@@ -115,4 +115,6 @@ def consts_from_scene(scene, V0, lam=None, mu=None):
     return dict(m=t(scene["xm"][:, 3]), V0=t(V0), lam=t(scene["params"][:, 0]) if lam is None else lam,
                 mu=t(scene["params"][:, 1]) if mu is None else mu, typ=torch.as_tensor(scene["type"]),
                 g=t(f32(scene["gravity"])), dt=float(f32(scene["dt"])), dx=float(f32(scene["dx"])),
+                # the solvers hold 1/dx as an FP32 value (src/solver/gpu_mls_mpm_solver.cu:40-52)
+                invdx=float(np.float32(1.0) / np.float32(scene["dx"])),
                 lo=t(scene["lo"]), hi=t(scene["hi"]), dims=torch.tensor(dims))

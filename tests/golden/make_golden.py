@@ -7,7 +7,8 @@ seeded scene builders in crmpm_scenes and pinned by a SHA-256 of the input
 arrays.  The mesh-SDF fixtures (sdf_*.npz) hold the mesh with the table, from
 the reference's host SDF builder; two of the meshes are the reference's own
 test assets (assets/*.obj), read here as data.
-Run from the repository root:  python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py [name ...]
+(with names, only those fixtures are written).
 """
 import copy
 import hashlib
@@ -110,7 +111,10 @@ def run_ref(sc, steps):
 def main():
     if not CC.ref_available():
         sys.exit("oracle/_ref/libcrmpm_ref.so is missing: build it with `make -C oracle ref`")
+    only = set(sys.argv[1:])
     for name, sc, steps in cases():
+        if only and name not in only:
+            continue
         st, v0, f, t = run_ref(sc, steps)
         out = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(out, xm=st["xm"].astype(np.float32), v=st["v"].astype(np.float32),
@@ -122,16 +126,22 @@ def main():
     # Mesh -> SDF tables from the reference's host builder (SdfBuilder,
     # src/geometry_utils/sdf_builder.cpp), mesh included.
     for name, pts, tris, cell, fatten in sdf_cases():
+        if only and name not in only:
+            continue
         dims, lower, grid = CC.mesh_sdf(pts, tris, cell, fatten, which="reference")
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), points=pts, tris=tris, cell=np.float32(cell),
                             fatten=np.float32(fatten), dims=dims, lower=lower, grid=grid)
         print(f"wrote {name}.npz", tuple(dims))
     for name, pts, tris, cell, fatten, spine in slicer_cases():
+        if only and name not in only:
+            continue
         dims, lower, rows = CC.mesh_slicer_sdf(pts, tris, cell, fatten, spine, which="reference")
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), points=pts, tris=tris, cell=np.float32(cell),
                             fatten=np.float32(fatten), spine=np.asarray(spine, np.float32), dims=dims, lower=lower,
                             rows=rows)
         print(f"wrote {name}.npz", tuple(dims))
+    if only and "svd3" not in only:
+        return
     # SVD: the matrix of the reference's own SVD test (tests/test_svd/main.cpp:36,
     # column-major constructor) plus random, repeated-singular-value, negative
     # determinant and near-singular matrices.

@@ -125,6 +125,64 @@ def test_autodiff_gradient_matches_finite_differences():
         assert abs(fdv - g_v0[p, 1]) <= 2e-3 * abs(fdv)
 
 
+def block_gradients_fp64(sc, V0, target, dtype=torch.float64):
+    """Loss and dL/dlambda, dL/dmu, dL/dv0 of the differentiable block through
+    the autodiff restatement (FP64, or FP32 to measure FP32's own effect)."""
+    import adjoint_block as AB
+
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64), dtype=dtype)  # noqa: E731
+    lam = t(sc["params"][:, 0]).requires_grad_(True)
+    mu = t(sc["params"][:, 1]).requires_grad_(True)
+    v0 = t(sc["v"][:, :3]).requires_grad_(True)
+    c = AD.consts_from_scene(sc, V0, lam=lam, mu=mu)
+    if dtype != torch.float64:
+        c = AD.consts_to(c, dtype)
+        c["lam"], c["mu"] = lam, mu
+    n = len(sc["xm"])
+    x, _, _, _ = AD.rollout(t(sc["xm"][:, :3]), v0, torch.eye(3, dtype=dtype).repeat(n, 1, 1),
+                            torch.zeros(n, 3, 3, dtype=dtype), c, AB.STEPS)
+    L = ((x - t(target)) ** 2).sum(1).mean()
+    L.backward()
+    return float(L.detach()), lam.grad.double().numpy(), mu.grad.double().numpy(), v0.grad.double().numpy()
+
+
+def fd_directional(fx, g_lam, g_mu, g_v):
+    """(predicted, finite-difference) changes of the loss over the fixture's
+    steps: log E steps as realised in FP32 lambda and mu, and v0 steps."""
+    idx = fx["idx"]
+    dl = fx["lam_pm"][:, 0].astype(np.float64) - fx["lam_pm"][:, 1]
+    dm = fx["mu_pm"][:, 0].astype(np.float64) - fx["mu_pm"][:, 1]
+    dv = fx["v_pm"][:, 0].astype(np.float64) - fx["v_pm"][:, 1]
+    pred_e = g_lam[idx] * dl + g_mu[idx] * dm
+    pred_v = (g_v[idx] * dv).sum(1)
+    return pred_e, fx["L_loge"][:, 0] - fx["L_loge"][:, 1], pred_v, fx["L_v"][:, 0] - fx["L_v"][:, 1]
+
+
+def test_autodiff_block_gradients_match_fd_fixture():
+    """The differentiable block at test size (4,096 particles, 96^3 grid, GRF
+    log E, 100 substeps, targets from a hidden field plus 1 mm noise): the
+    restatement's FP64 loss equals the oracle's, and its gradients predict the
+    oracle's FP64 central differences (h = 1e-3 in log E, 1e-4 m/s in v0) on
+    32 particles: v0 to 1e-5; log E to 2e-4, the FP64 rounding of the loss
+    (about 1e-20 against loss changes of 1e-17 to 2e-15 per step) being the
+    limit there, not the O(h^2) truncation."""
+    import adjoint_block as AB
+
+    fx = np.load(os.path.join(CC.ROOT, "tests", "golden", "adjoint_block4096_fd.npz"))
+    sc, _ = AB.scene()
+    V0 = initial_volume(sc)
+    L, g_lam, g_mu, g_v = block_gradients_fp64(sc, V0, fx["target"])
+    assert abs(L - float(fx["L0"])) <= 1e-10 * float(fx["L0"]), (L, float(fx["L0"]))
+    pe, de, pv, dv = fd_directional(fx, g_lam, g_mu, g_v)
+    ee = np.linalg.norm(pe - de) / np.linalg.norm(de)
+    ev = np.linalg.norm(pv - dv) / np.linalg.norm(dv)
+    print(f"loss {L:.6e}; autograd vs central differences: log E {ee:.2e}, v0 {ev:.2e}")
+    assert ee <= 2e-4 and ev <= 1e-5, (ee, ev)
+    # the steps move the loss far above its FP64 rounding (eps L)
+    eps_l = np.finfo(np.float64).eps * L
+    assert np.median(np.abs(de)) > 1e4 * eps_l and np.abs(dv).min() > 1e4 * eps_l
+
+
 def test_planted_stress_sign_error_is_caught():
     """Flipping the sign of the stress term in the restatement breaks the trajectory check."""
     sc = block_scene()
@@ -140,3 +198,36 @@ def test_planted_stress_sign_error_is_caught():
     finally:
         AD.stress = orig
     assert np.abs(x.numpy() - ref["xm"][:, :3]).max() > 1e-9
+
+
+class _FakeSim:
+    """Stands in for DifferentiableMls on the CPU: forward returns x = v0 and
+    bumps the generation as the real forward does."""
+
+    def __init__(self):
+        self.generation = 0
+
+    def forward(self, lam, mu, v0):
+        self.generation += 1
+        return v0.detach().clone().float()
+
+    def backward(self, grad_x):
+        n = grad_x.shape[0]
+        return torch.zeros(n, dtype=torch.float64), torch.zeros(n, dtype=torch.float64), grad_x.float()
+
+
+def test_rollout_backward_refuses_a_stale_graph():
+    """A rollout's backward after another forward on the same sim raises
+    instead of returning the later forward's gradients."""
+    import crmpm_hip as H
+
+    sim = _FakeSim()
+    lam = torch.ones(4, dtype=torch.float64, requires_grad=True)
+    v0 = torch.ones(4, 3, dtype=torch.float64, requires_grad=True)
+    x1 = H._MlsRollout.apply(lam, lam, v0, sim)
+    x1.sum().backward()  # the latest rollout: fine
+    assert torch.equal(v0.grad, torch.ones(4, 3, dtype=torch.float64))
+    x2 = H._MlsRollout.apply(lam, lam, v0, sim)
+    sim.forward(lam, lam, v0)  # e.g. an evaluation forward in between
+    with pytest.raises(RuntimeError, match="another forward"):
+        x2.sum().backward()

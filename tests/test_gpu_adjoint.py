@@ -10,6 +10,14 @@ Tolerances (relative, in the 2-norm over particles, 30 substeps):
     stress terms; FP32-rounded states alone, with FP64 arithmetic, leave
     about 4e-3), and within 2e-2;
   * the gradient does not depend on the checkpoint interval (bitwise).
+The differentiable block at test size (4,096 particles of the 65,536-particle
+GRF block's 96^3 grid, 100 substeps, checkpoints every 10, targets from a
+hidden field plus 1 mm noise) is checked against the FP64 autograd
+restatement with the same dL/dx (log E no further than 1.5 times an FP32
+evaluation of the restatement and at most 1e-2; v0 within 1e-4) and against the
+FP64 oracle's central differences on 32 particles (tests/golden/
+adjoint_block4096_fd.npz): 1e-2 in log E, 1e-3 in v0, where the FP32
+restatement itself is off by 3.6e-3 and 1.5e-4.
 """
 import os
 import sys
@@ -23,7 +31,7 @@ import cpu_checkers as CC
 import crmpm_scenes as S
 sys.path.insert(0, os.path.join(CC.ROOT, "oracle"))
 import mls_autodiff as AD  # noqa: E402
-from test_adjoint_cpu import STEPS, block_scene, torch_rollout
+from test_adjoint_cpu import STEPS, block_scene, fd_directional, torch_rollout
 
 pytestmark = pytest.mark.gpu
 
@@ -128,3 +136,61 @@ def test_rollout_is_a_torch_autograd_function():
     assert relnorm(logE.grad.cpu().numpy(), g_loge) < 1e-12
     assert relnorm(v0.grad.cpu().numpy(), g_v0) < 1e-12
     sim.close()
+
+
+def test_block_gradients_against_fp64_autograd_and_central_differences():
+    import adjoint_block as AB
+    import crmpm_hip as H
+
+    fx = np.load(os.path.join(CC.ROOT, "tests", "golden", "adjoint_block4096_fd.npz"))
+    sc, _ = AB.scene()
+    n = len(sc["xm"])
+    print(f"adjoint block: {n} particles, {AB.STEPS} substeps, checkpoints every {AB.EVERY}", flush=True)
+    sim = H.DifferentiableMls(sc, AB.STEPS, checkpoint_every=AB.EVERY)
+    dev = sim.solver.device
+    lam = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
+    mu = torch.as_tensor(sc["params"][:, 1], dtype=torch.float64, device=dev)
+    v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
+    V0 = sim.solver.state()["V0"]
+    target = torch.as_tensor(fx["target"], dtype=torch.float64)
+    x = sim.forward(lam, mu, v0).double()
+    gx_dev = 2.0 * (x - target.to(dev)) / n
+    dlam, dmu, dv0 = sim.backward(gx_dev)
+    sim.close()
+    g_lam, g_mu, g_v = dlam.cpu().numpy(), dmu.cpu().numpy(), dv0.double().cpu().numpy()
+    lam_c, mu_c = lam.cpu().numpy(), mu.cpu().numpy()
+    g_loge = g_lam * lam_c + g_mu * mu_c
+    loss = float(((x.cpu() - target) ** 2).sum(1).mean())
+    # FP64 and FP32 vector-Jacobian products of the restatement with the GPU's dL/dx
+    gx = gx_dev.cpu()
+
+    def vjp(dtype):
+        c = AD.consts_from_scene(sc, V0)
+        if dtype != torch.float64:
+            c = AD.consts_to(c, dtype)
+        ls = torch.zeros(n, dtype=dtype, requires_grad=True)
+        c["lam"] = c["lam"] * torch.exp(ls)
+        c["mu"] = c["mu"] * torch.exp(ls)
+        vv = v0.cpu().to(dtype).clone().requires_grad_(True)
+        xr, _, _, _ = AD.rollout(torch.as_tensor(sc["xm"][:, :3], dtype=dtype), vv,
+                                 torch.eye(3, dtype=dtype).repeat(n, 1, 1), torch.zeros(n, 3, 3, dtype=dtype), c,
+                                 AB.STEPS)
+        gl, gv = torch.autograd.grad(xr, (ls, vv), grad_outputs=gx.to(dtype))
+        return xr.detach().double().numpy(), gl.double().numpy(), gv.double().numpy()
+
+    x64, r_loge, r_v = vjp(torch.float64)
+    _, r32_loge, r32_v = vjp(torch.float32)
+    ex = float(np.abs(x.cpu().numpy() - x64).max())
+    el, ev = relnorm(g_loge, r_loge), relnorm(g_v, r_v)
+    e32l, e32v = relnorm(r32_loge, r_loge), relnorm(r32_v, r_v)
+    pe, de, pv, dv = fd_directional(fx, g_lam, g_mu, g_v)
+    fe, fv = relnorm(pe, de), relnorm(pv, dv)
+    print(f"  loss {loss:.4e} (FP64 oracle {float(fx['L0']):.4e}); forward max |x - x64| {ex:.2e} m", flush=True)
+    print(f"  vs FP64 autograd: log E {el:.2e} (FP32 restatement {e32l:.2e}), v0 {ev:.2e} (FP32 {e32v:.2e})",
+          flush=True)
+    print(f"  vs FP64 central differences on 32 particles: log E {fe:.2e}, v0 {fv:.2e}", flush=True)
+    assert ex < 1e-6
+    assert abs(loss - float(fx["L0"])) <= 1e-3 * float(fx["L0"])
+    assert el <= max(1.5 * e32l, 1e-4) and el < 1e-2, (el, e32l)
+    assert ev < 1e-4, (ev, e32v)
+    assert fe < 1e-2 and fv < 1e-3, (fe, fv)

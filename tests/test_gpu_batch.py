@@ -5,6 +5,10 @@ when it runs alone, so its trajectory must match the single-scene solver's;
 the tolerance (relative 1e-6 after 10 substeps) only absorbs the float
 atomics of computeInitialData's mass scatter, whose order is not fixed.
 Graph replay and autotuned launch sizes must not change results at all.
+The batched workload's generator at test size (8 scenes, particle counts
+scaled down 50x, the same 32^3-128^3 grids, box and capsule obstacles in
+contact) runs autotuned and graph-replayed and is checked scene by scene
+against the oracle: relative error <= 1e-4 on x, v and F.
 """
 import copy
 
@@ -183,3 +187,53 @@ def test_single_solver_graph_with_kinematic_frames():
     for a, b in zip(runs[False], runs[True]):
         for k in FIELDS:
             assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("kind,steps", [("mls", 100), ("pb", 20)])
+def test_batched_workload_matches_oracle(kind, steps):
+    """The 64-scene generator (crmpm_scenes.config3) at test size: its first 8
+    scenes with 40-4,000 particles (log-uniform), grids of 32^3-128^3 cells,
+    each body starting half a cell above its box or capsule obstacle so that
+    contact begins within the run; autotuned, graph-replayed batch against
+    the oracle run scene by scene (the reference's advanceAll steps each
+    scene's solver in turn, src/engine/simulation_factory_impl.cpp:1106-1150).
+    PB bodies come to rest on their obstacle within 20 substeps (max |v| from
+    0.43 to 0.004 m/s), where a one-ulp change of the oracle's input positions
+    alone moves the velocities by 2.7e-3 of their max: velocity errors are
+    measured against the larger of the field's max and the bodies' initial
+    speed, 0.5 m/s."""
+    import crmpm_hip as H
+    import cpu_checkers as CC
+    from parity_bounds import TOL_100
+
+    scenes = S.config3(n_scenes=8, kind=KINDS[kind], steps=steps, n_min=40, n_max=4000, gap=0.5)
+    print(f"batched workload {kind}: {len(scenes)} scenes, {sum(len(s['xm']) for s in scenes)} particles, "
+          f"{steps} substeps", flush=True)
+    b = H.HipBatch(copy.deepcopy(scenes))
+    b.compute_initial_data()
+    b.step(1)
+    b.autotune(steps_per_trial=2)
+    b.set_graph_mode(True)
+    b.step(steps - 1)
+    for i, sc in enumerate(scenes):
+        o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+        o.compute_initial_data()
+        o.step(steps)
+        os_ = o.state()
+        o.close()
+        d = CC.CpuSolver(copy.deepcopy(sc), "oracle", fp64=True)
+        d.compute_initial_data()
+        d.step(steps)
+        od = d.state()
+        d.close()
+        st = b.state(i)
+        errs = {k: rel(st[k], os_[k]) for k in ("xm", "v", "F")}
+        errs["v"] = float(np.abs(st["v"] - os_["v"]).max()) / max(float(np.abs(os_["v"]).max()), 0.5)
+        print(f"  scene {i} ({len(sc['xm'])} particles, {round(1 / sc['dx'])}^3 cells, "
+              f"{'box' if sc['geoms'][0][0] == S.BOX else 'capsule'}): gpu-vs-oracle "
+              + " ".join(f"{k}={v:.2e}" for k, v in errs.items())
+              + " | fp64 replay: gpu " + " ".join(f"{k}={rel(st[k], od[k]):.2e}" for k in ("xm", "v", "F"))
+              + " cpu " + " ".join(f"{k}={rel(os_[k], od[k]):.2e}" for k in ("xm", "v", "F")), flush=True)
+        for k in ("xm", "v", "F"):
+            assert errs[k] <= TOL_100, (i, k, errs[k])
+    b.close()

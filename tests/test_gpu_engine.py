@@ -48,12 +48,20 @@ def test_engine_advance_all_matches_single_solvers():
     import crmpm_hip as H
 
     scenes = _scenes()
-    # scene 2 (half the step size: it never rests) starts with its last 100
-    # particles unallocated; a resting scene's grid is cleared, so initial
-    # data queued for it would not see its last step's grid mass
-    held = {2: 100}
+    # scenes 1 and 2 start with their last 100 particles unallocated.  Scene 2
+    # (half the step size) never rests; scene 1 takes half the batch's
+    # substeps and rests for the others, so its grid must keep its last
+    # update: the initial data queued for it at the next frame adds the new
+    # particles' mass onto that grid (Q10), as the reference's solver, which
+    # is not stepped, holds it
+    held = {1: 100, 2: 100}
     frame = 4 * scenes[0]["dt"]
     eng, ids, tabs = H.engine_from_scenes(copy.deepcopy(scenes), held)
+    # the oracle runs scene 1 through the same frames
+    n1 = len(scenes[1]["xm"])
+    orc = CC.CpuSolver(copy.deepcopy(scenes[1]), "oracle")
+    orc.fn["set_num_active"](orc.h, n1 - 100)
+    orc.compute_initial_data(np.arange(n1 - 100))
 
     solvers = []
     for i, sc in enumerate(scenes):
@@ -71,12 +79,26 @@ def test_engine_advance_all_matches_single_solvers():
             hd["v"][:, :3] *= 0.5
             eng.mark(1, H.SCENE_VELOCITY)
             solvers[1].v[:, :3] *= 0.5
-            n2 = len(scenes[2]["xm"])
-            eng.set_num_particles(2, n2)
-            eng.add_initial_data(2, np.arange(n2 - 100, n2))
-            g = solvers[2]
-            H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, n2), "num active")
-            g.compute_initial_data(np.arange(n2 - 100, n2))
+            vo = np.zeros((n1, 4), np.float32)
+            vo[:, :3] = orc.state()["v"].astype(np.float32) * np.float32(0.5)
+            orc.fn["set_particles"](orc.h, n1, None, CC.P(vo, CC._f), None, None, None)
+            for i in (1, 2):
+                ni = len(scenes[i]["xm"])
+                eng.set_num_particles(i, ni)
+                eng.add_initial_data(i, np.arange(ni - 100, ni))
+                g = solvers[i]
+                H.check(g.lib.crHipSolverSetNumActiveParticles(g.h, ni), "num active")
+                g.compute_initial_data(np.arange(ni - 100, ni))
+            orc.fn["set_num_active"](orc.h, n1)
+            orc.compute_initial_data(np.arange(n1 - 100, n1))
+            # the kept grid matters: on an emptied grid the added particles' V0 differ
+            fresh = CC.CpuSolver(copy.deepcopy(scenes[1]), "oracle")
+            xo = np.ascontiguousarray(orc.state()["xm"].astype(np.float32))
+            fresh.fn["set_particles"](fresh.h, n1, CC.P(xo, CC._f), None, None, None, None)
+            fresh.compute_initial_data(np.arange(n1 - 100, n1))
+            v_kept, v_empty = orc.state()["V0"][n1 - 100:], fresh.state()["V0"][n1 - 100:]
+            fresh.close()
+            assert np.abs(v_kept - v_empty).max() > 1e-3 * np.abs(v_kept).max()
             # the solvers' coupling forces restart per frame, as the engine's do
             for g in solvers:
                 g.force.zero_()
@@ -87,6 +109,15 @@ def test_engine_advance_all_matches_single_solvers():
             k = substeps(frame, sc["dt"])
             assert eng.substeps(i) == k, (f, i, eng.substeps(i), k)
             solvers[i].step(k)
+        assert eng.substeps(1) < eng.substeps(2), "scene 1 must rest in its batch"
+        orc.step(substeps(frame, scenes[1]["dt"]))
+        es, os_ = eng.state(1), orc.state()
+        hd = eng.host(1)
+        for key, a, b in (("xm", hd["xm"][:, :3], os_["xm"][:, :3]), ("v", hd["v"][:, :3], os_["v"]),
+                          ("F", es["F"], os_["F"]), ("V0", es["V0"], os_["V0"])):
+            err = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+            print(f"frame {f} scene 1 vs oracle: {key} rel {err:.2e}", flush=True)
+            assert err <= 1e-4, (f, key, err)
         for i, (sid, s0, k, _, _) in enumerate(ids):
             st = solvers[i].state()
             hd = eng.host(sid)
@@ -100,6 +131,7 @@ def test_engine_advance_all_matches_single_solvers():
             np.testing.assert_array_equal(tabs["force"][s0:s0 + k], gf[:k].astype(np.float32))
             np.testing.assert_array_equal(tabs["torque"][s0:s0 + k], gt[:k].astype(np.float32))
     assert np.abs(tabs["force"][ids[2][1] + 1, :3]).max() > 0, "the dynamic capsule receives no force"
+    orc.close()
     for g in solvers:
         g.close()
     eng.close()

@@ -1,12 +1,14 @@
 """GPU parity: the HIP solver (through the C-ABI) against the CPU oracle.
 
-Tolerances (FP32):
+Tolerances (FP32, fixed, from tests/parity_bounds.py):
   * binning keys and sort permutation: bit-exact against the oracle's base cells
     and a stable sort;
-  * positions, velocities, F after 100 substeps: relative error <= 1e-4 (max
-    abs difference over the field's max magnitude); after 10 substeps 2e-5;
+  * positions, velocities, F: relative error (max abs difference over the
+    field's max magnitude) <= 5e-5 after 10 substeps, <= 1e-4 after 100 or
+    more;
   * the HIP path must be bitwise reproducible run to run.
-The FP64 replay error of both FP32 paths is printed for the record.
+The error of both FP32 paths against the oracle's FP64 replay is printed for
+the record; it does not widen the bounds.
 """
 import copy
 
@@ -15,22 +17,9 @@ import pytest
 
 import cpu_checkers as CC
 import crmpm_scenes as S
+from parity_bounds import TOL_10, TOL_100, rel
 
 pytestmark = pytest.mark.gpu
-
-TOL_100 = 1e-4
-TOL_10 = 2e-5
-# Where the scene amplifies rounding (cut pairs reflecting velocities, PB
-# iterations), the FP32 oracle itself departs from its FP64 replay by more than
-# TOL_10.  Two FP32 evaluations that differ only in summation order may then
-# differ by up to the sum of their errors: the bound for such a field is twice
-# the FP32 oracle's own error against FP64.
-FP32_NOISE_FACTOR = 2.0
-
-
-def rel(a, b):
-    scale = max(float(np.abs(b).max()), 1e-30)
-    return float(np.abs(a - b).max()) / scale
 
 
 def hip_run(sc, steps):
@@ -114,24 +103,28 @@ def test_parity_10_substeps(kind, shapes):
     od = cpu_run(sc, 10, fp64=True)
     errs = report(f"{kind} {shapes} 10 steps", gs, os_, od)
     for k in ("xm", "v", "F"):
-        tol = max(TOL_10, FP32_NOISE_FACTOR * rel(os_[k], od[k]))
-        assert errs[k] <= tol, (k, errs[k], tol)
+        assert errs[k] <= TOL_10, (k, errs[k])
 
 
-def test_parity_mls_cube_100_substeps():
-    """Elastic cube on the ground plane, 16^3 particles, 100 substeps."""
-    sc = S.config1(n_side=16, steps=100)
-    gs = hip_run(sc, 100)
-    os_ = cpu_run(sc, 100)
-    od = cpu_run(sc, 100, fp64=True)
-    errs = report("mls cube 16^3 100 steps", gs, os_, od)
+def test_parity_mls_cube_200_substeps():
+    """The elastic cube at full size: 32^3 = 32,768 particles on 65^3 nodes
+    falling onto the ground plane, 200 substeps."""
+    sc = S.config1()
+    print("mls cube 32^3, 200 substeps", flush=True)
+    gs = hip_run(sc, 200)
+    os_ = cpu_run(sc, 200)
+    od = cpu_run(sc, 200, fp64=True)
+    errs = report("mls cube 32^3 200 steps", gs, os_, od)
     for k in ("xm", "v", "F"):
         assert errs[k] <= TOL_100, (k, errs[k])
 
 
 def test_parity_pb_slab_100_substeps():
-    """PB-MPM slab with the sphere indenter, 32x8x32 particles, 20 iterations, 100 substeps."""
+    """PB-MPM slab with the sphere indenter pressing into it from the first
+    substep (tests/test_oracle.py checks the contact), 32x8x32 particles,
+    20 iterations, 100 substeps."""
     sc = S.config2(nx=32, ny=8, nz=32, steps=100, iterations=20)
+    print("pb slab 32x8x32, 100 substeps", flush=True)
     gs = hip_run(sc, 100)
     os_ = cpu_run(sc, 100)
     od = cpu_run(sc, 100, fp64=True)
@@ -145,9 +138,11 @@ def test_parity_mls_cutting_30_substeps():
     and the growing quad slicer sweeping at 1 m/s (so the cut crosses many
     particles within 30 substeps), on the full 257 x 129 x 257-node grid."""
     sc = S.config5(nx=24, ny=8, nz=24, steps=30, speed=1.0)
+    print("mls cutting 24x8x24, 30 substeps", flush=True)
     gs = hip_run(sc, 30)
     os_ = cpu_run(sc, 30)
-    errs = report("mls cutting 24x8x24 30 steps", gs, os_)
+    od = cpu_run(sc, 30, fp64=True)
+    errs = report("mls cutting 24x8x24 30 steps", gs, os_, od)
     for k in ("xm", "v", "F"):
         assert errs[k] <= TOL_10, (k, errs[k])
 
@@ -365,5 +360,4 @@ def test_parity_dense_cells(kind):
     od = cpu_run(sc, 10, fp64=True)
     errs = report(f"{kind} dense cells (max {per_cell.max()} per cell) 10 steps", gs, os_, od)
     for k in ("xm", "v", "F"):
-        tol = max(TOL_10, FP32_NOISE_FACTOR * rel(os_[k], od[k]))
-        assert errs[k] <= tol, (k, errs[k], tol)
+        assert errs[k] <= TOL_10, (k, errs[k])

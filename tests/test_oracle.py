@@ -289,6 +289,51 @@ def test_config3_generator():
         assert G % 4 == 0 and 32 <= G <= 128
 
 
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_planted_friction_error_exceeds_parity_bound(kind):
+    """The fixed GPU parity bounds catch a contact bug: the capsule scene of
+    the 10-substep parity test with the capsule's friction 10% too high (a
+    mistake planted in the oracle's input) moves the velocities far beyond
+    TOL_10."""
+    from parity_bounds import TOL_10, rel
+
+    sc = S.small_scene(KINDS[kind], 7 + KINDS[kind], shapes="plane+capsule")
+    a = run(sc, "oracle", 10)
+    p0 = list(sc["shape_list"][1]["params0"])
+    p0[2] *= 1.1
+    sc["shape_list"][1]["params0"] = tuple(p0)
+    b = run(sc, "oracle", 10)
+    assert rel(b["v"], a["v"]) > 20 * TOL_10, rel(b["v"], a["v"])
+
+
+def _nodes_in_sphere(o, scene, shape=0):
+    """Grid nodes holding mass whose position lies inside the scene's sphere."""
+    dims, dx = o.dims, scene["dx"]
+    k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    pos = np.stack([i.ravel(), j.ravel(), k.ravel()], 1) * dx + np.asarray(scene["lo"], np.float64)
+    c = np.asarray(o.scene["shape_list"][shape]["pos"][:3])
+    r = o.scene["geoms"][o.scene["shape_list"][shape]["geom"]][1][0]
+    return int(((np.linalg.norm(pos - c, axis=1) < r) & (o.grid()[:, 3] > 0)).sum())
+
+
+def test_config2_indenter_contacts_the_slab():
+    """The PB slab's sphere indenter starts embedded and presses on the slab
+    through the first 25 substeps (the bench's timed window and the parity
+    test's start): grid nodes holding mass lie inside the sphere."""
+    sc = S.config2(nx=32, ny=8, nz=32, steps=25, iterations=20)
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle")
+    o.compute_initial_data()
+    counts = []
+    for fr in sc["frames"]:
+        S.apply_frame(o.scene, fr)
+        o.upload_shapes()
+        o.step(fr["substeps"])
+        counts.append(_nodes_in_sphere(o, sc))
+    o.close()
+    print("nodes inside the indenter after substeps 10, 20, 25:", counts)
+    assert min(counts) > 0, counts
+
+
 def test_config5_generator():
     """Cutting scene: 2,097,152 particles on 257 x 129 x 257 nodes, the block
     inset from the lower faces, the slicer's half extent growing by the swept
