@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # touched node.
 BYTES_PER_PARTICLE_FUSED = 164
 BYTES_PER_TOUCHED_NODE_FUSED = 32
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pb_fused_slab262k_traffic.json")
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILE = os.path.join(PROFILES, "pb_slab262k_counters.json")
 # the fused iteration kernel (template Mode 1; either launch form)
 FUSED_KERNEL = "k_pb_tile<1,"
 # what the library's autotuner picked: launch sizes, the P2G sub-tile split
@@ -44,6 +45,7 @@ FUSED_KERNEL = "k_pb_tile<1,"
 TUNED_KEYS = ("tile_waves", "update_waves", "sub_size", "sub_max", "gather_sub_size", "gather_sub_max")
 PROFILE_EVERY = 10
 COPY_CAL_BYTES = 256 << 20
+LAUNCH = None  # --launch: a fixed launch configuration in place of the autotuner's
 
 
 def copy_calibration():
@@ -100,20 +102,27 @@ def valu_issue_us(instructions):
 
 
 # What --summarize covers per profiled workload: the bench invocation that
-# rocprofv3 wraps, the summary's title and file name, and whether the HBM
-# counter passes (FETCH_SIZE, WRITE_SIZE with their copy calibration, SQ
-# instruction counts) belong to it or the kernel trace alone.
+# rocprofv3 wraps (with the launch configuration the timed run's autotuner
+# picked, so no trial dispatch enters the statistics), the summary's title and
+# file name, and whether the counter passes belong to it or the kernel trace alone.
+PB_LAUNCH = "0,16384,1048576,1,1048576,1"
+MLS_CUTTING_LAUNCH = "0,16384,1048576,1,1048576,1"
+MLS_CUBE_LAUNCH = "0,0,128,8,128,8"
 PROFILE_TARGETS = {
-    "pb_slab": dict(run="python3 bench.py --workload pb_slab --cpu-substeps 0", counters=True,
-                    title="Kernel trace and HBM traffic: PB-MPM slab, 262,144 particles, 20 iterations, 500 substeps",
+    "pb_slab": dict(run=f"python3 bench.py --workload pb_slab --cpu-substeps 0 --launch {PB_LAUNCH}", counters=True,
+                    title="Kernel trace and counters: PB-MPM slab, 262,144 particles, 20 iterations, 510 substeps",
                     md="pb_slab262k_kernel_trace.md"),
-    "mls": dict(run="python3 bench.py --workload mls", counters=True,
-                title="Kernel trace and HBM traffic: MLS-MPM cutting block, 2,097,152 particles, 500 substeps; "
-                      "then the 32,768-particle cube, 200 substeps",
-                md="mls_cutting2m_kernel_trace.md"),
+    "mls_cutting": dict(run=f"python3 bench.py --workload mls_cutting --cpu-substeps 0 --launch {MLS_CUTTING_LAUNCH}",
+                        counters=True,
+                        title="Kernel trace and counters: MLS-MPM cutting block, 2,097,152 particles, 510 substeps",
+                        md="mls_cutting2m_kernel_trace.md"),
+    "mls_cube": dict(run=f"python3 bench.py --workload mls_cube --cpu-substeps 0 --launch {MLS_CUBE_LAUNCH}",
+                     counters=True,
+                     title="Kernel trace and counters: MLS-MPM elastic cube, 32,768 particles, 205 substeps",
+                     md="mls_cube32k_kernel_trace.md"),
     "batched": dict(run="python3 bench.py --workload batched", counters=False,
                     title="Kernel trace: 64 batched scenes (MLS 100 substeps, PB 20 substeps) and the same scenes "
-                          "one at a time",
+                          "as single solvers",
                     md="batched64_kernel_trace.md"),
     "meshsdf": dict(run="python3 bench.py --workload meshsdf", counters=True,
                     title="Kernel trace and counters: mesh -> SDF table, 20,480-triangle icosphere on 128^3 nodes",
@@ -122,10 +131,51 @@ PROFILE_TARGETS = {
                    title="Kernel trace: engine advanceAll, 64 MLS scenes in frames of 10 substeps with host readback, "
                          "then the same batch stepped alone",
                    md="engine64_kernel_trace.md"),
-    "adjoint": dict(run="python3 bench.py --workload adjoint", counters=False,
+    "adjoint": dict(run="python3 bench.py --workload adjoint --cpu-substeps 0", counters=False,
                     title="Kernel trace: differentiable MLS block, 65,536 particles, 100 substeps, forward + adjoint",
                     md="adjoint65k_kernel_trace.md"),
 }
+
+# Counter passes: (directory, counters, calibrated by the copy kernel).  Each
+# pass fits gfx950's per-block limits (at most 4 TCC counters: FETCH_SIZE
+# takes 3, WRITE_SIZE 2; at most 8 SQ counters).
+COUNTER_PASSES = (
+    ("fetch", "FETCH_SIZE", True),
+    ("write", "WRITE_SIZE", True),
+    ("sq", "SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_ACTIVE_INST_VALU "
+           "SQ_INSTS_VMEM_RD", False),
+    ("tcc", "TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum TCC_EA0_ATOMIC_sum", False),
+    ("dram", "TCC_EA0_RDREQ_DRAM_32B_sum TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", True),
+)
+
+
+def counters_file(target):
+    return PROFILE_TARGETS[target]["md"].replace("_kernel_trace.md", "_counters.json")
+
+
+def profiled_row(target, kernel):
+    """The committed counter figures of one kernel of a profiled workload
+    (written by --summarize), or None."""
+    path = os.path.join(PROFILES, counters_file(target))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(kernel)
+
+
+def roofline_counters(row, avg_s):
+    """Counter figures of a profiled kernel for a roofline line: HBM-side
+    traffic per launch, the L2 hit rate, DRAM-path bytes, atomics, and the
+    VALU issue time over the measured duration."""
+    if row is None:
+        return {"traffic": None}
+    vi = row.get("valu_instructions")
+    rd, wr = row.get("dram_path_read_bytes"), row.get("dram_path_write_bytes")
+    return {"traffic": row.get("bytes_per_launch"), "l2_hit_rate": row.get("l2_hit_rate"),
+            "dram_path_bytes": None if rd is None or wr is None else rd + wr,
+            "atomics_per_launch": row.get("atomics_l2"),
+            "valu_issue_frac": None if vi is None else valu_issue_us(vi) / (avg_s * 1e6),
+            "profile_avg_launch_us": row.get("avg_us")}
 
 
 def profile_commands(target):
@@ -133,76 +183,142 @@ def profile_commands(target):
     run = t["run"]
     cmds = [f"rocprofv3 --kernel-trace --stats --output-format csv -d DIR/trace -o run -- {run}"]
     if t["counters"]:
-        for c in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = c.split("_")[0].lower()
-            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d} -o run -- {run}")
-            cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d DIR/{d}_copy -o cp -- "
-                        "python3 bench.py --copy-calibration")
-        cmds.append("rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES "
-                    f"SQ_ACTIVE_INST_VALU SQ_INSTS_VMEM_RD --output-format csv -d DIR/sq -o run -- {run}")
+        for d, counters, cal in COUNTER_PASSES:
+            cmds.append(f"rocprofv3 --pmc {counters} --output-format csv -d DIR/{d} -o run -- {run}")
+            if cal:
+                cmds.append(f"rocprofv3 --pmc {counters} --output-format csv -d DIR/{d}_copy -o cp -- "
+                            "python3 bench.py --copy-calibration")
     cmds.append(f"python3 bench.py --summarize DIR --profile-target {target}")
     return cmds
 
 
+def _copy_reading(directory, counter):
+    _, per = _counter_per_launch(directory, counter)
+    return max(per["__amd_rocclr_copyBuffer"])
+
+
 def summarize(out_dir, title, commands, counters=True):
     """Turn the rocprofv3 outputs of one profiling session into the committed
-    summary: kernel-trace statistics and, with counters, FETCH_SIZE /
-    WRITE_SIZE per dispatch of each kernel, the copy-kernel calibration that
-    scales them to bytes, and the VALU instruction count."""
+    summary: kernel-trace statistics and, with counters, per dispatch of each
+    kernel: L2-to-memory bytes (FETCH_SIZE + WRITE_SIZE), bytes on the L2's
+    DRAM path (requests the L2 sends towards memory: served by the MALL or by
+    HBM), each scaled by what the same counter reads for a device copy of known
+    size; the L2 hit rate; atomic requests at the L2 and beyond it; the VALU
+    instruction count."""
     stats = _read_csv(os.path.join(out_dir, "trace"), "kernel_stats.csv")
-    fetch = write = valu = {}
-    f_fac = w_fac = None
+    c = {}
+    fac = {}
     if counters:
-        fetch, _ = _counter_per_launch(os.path.join(out_dir, "fetch"))
-        write, _ = _counter_per_launch(os.path.join(out_dir, "write"))
-        _, fetch_cp = _counter_per_launch(os.path.join(out_dir, "fetch_copy"))
-        _, write_cp = _counter_per_launch(os.path.join(out_dir, "write_copy"))
-        valu = _counter_per_launch(os.path.join(out_dir, "sq"), "SQ_INSTS_VALU")[0]
         kib = COPY_CAL_BYTES / 1024.0
-        f_fac = kib / max(fetch_cp["__amd_rocclr_copyBuffer"])
-        w_fac = kib / max(write_cp["__amd_rocclr_copyBuffer"])
+        c["fetch"] = _counter_per_launch(os.path.join(out_dir, "fetch"))[0]
+        c["write"] = _counter_per_launch(os.path.join(out_dir, "write"))[0]
+        fac["fetch"] = kib / _copy_reading(os.path.join(out_dir, "fetch_copy"), "FETCH_SIZE")
+        fac["write"] = kib / _copy_reading(os.path.join(out_dir, "write_copy"), "WRITE_SIZE")
+        c["valu"] = _counter_per_launch(os.path.join(out_dir, "sq"), "SQ_INSTS_VALU")[0]
+        for name in ("TCC_HIT_sum", "TCC_MISS_sum", "TCC_ATOMIC_sum", "TCC_EA0_ATOMIC_sum"):
+            c[name] = _counter_per_launch(os.path.join(out_dir, "tcc"), name)[0]
+        for name in ("TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"):
+            c[name] = _counter_per_launch(os.path.join(out_dir, "dram"), name)[0]
+            # 32-byte requests -> bytes, scaled so the copy reads (writes) its 256 MiB
+            fac[name] = COPY_CAL_BYTES / (32.0 * _copy_reading(os.path.join(out_dir, "dram_copy"), name))
     lines = [f"# {title}", "", "Commands (repository root, rocprofv3 on one MI355X):", "", "```"]
     lines += commands + ["```", ""]
     if counters:
-        lines += ["| kernel | calls | avg µs | total % | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
-                  "| HBM MB/dispatch (calibrated) | SQ_INSTS_VALU/dispatch | VALU issue µs |",
-                  "|---|---|---|---|---|---|---|---|---|"]
+        lines += ["| kernel | calls | avg µs | total % | FETCH_SIZE KiB | WRITE_SIZE KiB | L2↔memory MB (calibrated) "
+                  "| DRAM-path read MB | DRAM-path write MB | L2 hit % | atomics at L2 | atomics past L2 "
+                  "| SQ_INSTS_VALU | VALU issue µs |",
+                  "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
     else:
         lines += ["| kernel | calls | avg µs | total % |", "|---|---|---|---|"]
     merged = {}
     for r in stats:
         k = _short_name(r["Name"])
-        c, t = merged.get(k, (0, 0.0))
-        merged[k] = (c + int(r["Calls"]), t + float(r["TotalDurationNs"]))
+        n_, t_ = merged.get(k, (0, 0.0))
+        merged[k] = (n_ + int(r["Calls"]), t_ + float(r["TotalDurationNs"]))
     total = sum(t for _, t in merged.values())
-    for k, (c, t) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
-        fr, wr = fetch.get(k), write.get(k)
-        mb = "" if fr is None or wr is None else f"{(fr * f_fac + wr * w_fac) * 1024 / 1e6:.2f}"
-        vi = valu.get(k)
-        row = f"| `{k}` | {c} | {t / c / 1e3:.1f} | {100 * t / total:.2f} |"
+
+    def per_kernel(k):
+        if not counters:
+            return {}
+        g = lambda key: c[key].get(k)  # noqa: E731
+        out = {"fetch_kib_raw": g("fetch"), "write_kib_raw": g("write"), "valu_instructions": g("valu")}
+        if out["fetch_kib_raw"] is not None and out["write_kib_raw"] is not None:
+            out["bytes_per_launch"] = 1024.0 * (out["fetch_kib_raw"] * fac["fetch"] + out["write_kib_raw"] * fac["write"])
+        h, m = g("TCC_HIT_sum"), g("TCC_MISS_sum")
+        out["l2_hit_rate"] = None if h is None or m is None or h + m == 0 else h / (h + m)
+        out["atomics_l2"], out["atomics_past_l2"] = g("TCC_ATOMIC_sum"), g("TCC_EA0_ATOMIC_sum")
+        rd, wr = g("TCC_EA0_RDREQ_DRAM_32B_sum"), g("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
+        out["dram_path_read_bytes"] = None if rd is None else 32.0 * rd * fac["TCC_EA0_RDREQ_DRAM_32B_sum"]
+        out["dram_path_write_bytes"] = None if wr is None else 32.0 * wr * fac["TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"]
+        return out
+
+    f = lambda v, fmt: "" if v is None else format(v, fmt)  # noqa: E731
+    for k, (n_, t_) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
+        row = f"| `{k}` | {n_} | {t_ / n_ / 1e3:.1f} | {100 * t_ / total:.2f} |"
         if counters:
-            row += (f" {'' if fr is None else f'{fr:,.0f}'} | {'' if wr is None else f'{wr:,.0f}'} | {mb} | "
-                    f"{'' if vi is None else f'{vi:,.0f}'} | {'' if vi is None else f'{valu_issue_us(vi):.1f}'} |")
+            q = per_kernel(k)
+            mb = lambda key: None if q.get(key) is None else q[key] / 1e6  # noqa: E731
+            row += (f" {f(q['fetch_kib_raw'], ',.0f')} | {f(q['write_kib_raw'], ',.0f')} | {f(mb('bytes_per_launch'), '.2f')}"
+                    f" | {f(mb('dram_path_read_bytes'), '.2f')} | {f(mb('dram_path_write_bytes'), '.2f')}"
+                    f" | {f(None if q['l2_hit_rate'] is None else 100 * q['l2_hit_rate'], '.1f')}"
+                    f" | {f(q['atomics_l2'], ',.0f')} | {f(q['atomics_past_l2'], ',.0f')}"
+                    f" | {f(q['valu_instructions'], ',.0f')}"
+                    f" | {f(None if q['valu_instructions'] is None else valu_issue_us(q['valu_instructions']), '.1f')} |")
         lines.append(row)
-    lines += ["", "Kernel names carry their template arguments: the tile kernels' last one selects the persistent "
-              "launch form, which appears in the autotuner's trial substeps."]
+    lines += ["", "Counter columns are per dispatch, averaged over the kernel's dispatches. Kernel names carry their "
+              "template arguments (the tile kernels' last one selects the persistent launch form)."]
     if not counters:
         return "\n".join(lines) + "\n", None
     lines += ["", f"Calibration: a device-to-device copy of {COPY_CAL_BYTES >> 20} MiB reads FETCH_SIZE "
-              f"{max(fetch_cp['__amd_rocclr_copyBuffer']):,.0f} KiB and WRITE_SIZE {max(write_cp['__amd_rocclr_copyBuffer']):,.0f} KiB "
-              f"per dispatch, so bytes = FETCH_SIZE x {f_fac:.3f} KiB + WRITE_SIZE x {w_fac:.3f} KiB.",
+              f"{COPY_CAL_BYTES / 1024 / fac['fetch']:,.0f} KiB and WRITE_SIZE {COPY_CAL_BYTES / 1024 / fac['write']:,.0f} KiB "
+              f"per dispatch, so L2↔memory bytes = FETCH_SIZE x {fac['fetch']:.3f} KiB + WRITE_SIZE x {fac['write']:.3f} KiB; "
+              f"its TCC_EA0_RDREQ_DRAM_32B_sum and TCC_EA0_WRREQ_WRITE_DRAM_32B_sum read "
+              f"{fac['TCC_EA0_RDREQ_DRAM_32B_sum']:.3f} and {fac['TCC_EA0_WRREQ_WRITE_DRAM_32B_sum']:.3f} of its bytes "
+              "/ 32 inverted, the scale applied to the DRAM-path columns.",
+              "", "DRAM-path bytes are the L2's requests towards memory (TCC_EA0_*_DRAM): the 256 MB MALL (Infinity "
+              "Cache) sits behind the L2 and serves those it holds, so they bound HBM traffic from above; no counter "
+              "here separates MALL hits from HBM.",
+              "", "L2 hit % = TCC_HIT / (TCC_HIT + TCC_MISS). Atomics at L2 = TCC_ATOMIC_sum (atomic requests of all "
+              "types reaching the L2); past L2 = TCC_EA0_ATOMIC_sum.",
               "", "VALU issue µs = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz): the time the VALU pipes need "
               "for the kernel's vector instructions; a kernel whose duration is close to it is bound by VALU issue, not HBM."]
-    fused = max((k for k in merged if k.startswith(FUSED_KERNEL)), key=lambda k: merged[k][0], default=None)
-    result = {"kernel": fused, "fetch_kib_raw": fetch.get(fused), "write_kib_raw": write.get(fused),
-              "fetch_factor": f_fac, "write_factor": w_fac, "valu_instructions": valu.get(fused)}
-    if result["fetch_kib_raw"] is not None and result["write_kib_raw"] is not None:
-        result["bytes_per_launch"] = 1024.0 * (result["fetch_kib_raw"] * f_fac + result["write_kib_raw"] * w_fac)
+    result = {}
+    for k in merged:
+        q = per_kernel(k)
+        n_, t_ = merged[k]
+        q["calls"] = n_
+        q["avg_us"] = t_ / n_ / 1e3
+        result[k] = q
+    result["_calibration"] = fac
     return "\n".join(lines) + "\n", result
 
 
-def cpu_baseline(scene, substeps):
-    """Reference CPU solver (compiled from its sources) or the oracle, one thread."""
+_CPU_INFO = None
+
+
+def host_cpu():
+    """The host CPU's model name and the cores this process may use."""
+    global _CPU_INFO
+    if _CPU_INFO is None:
+        model = platform.processor() or platform.machine()
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        _CPU_INFO = {"cpu_model": model, "nproc": usable}
+    return _CPU_INFO
+
+
+def cpu_baseline(scene, substeps, label=None):
+    """The reference's own CPU solver (compiled from its sources, oracle/_ref)
+    or, where that build is absent, the oracle's restatement of it, one
+    thread as the reference runs, on `substeps` substeps of the scene (its
+    first kinematic frame applied)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cpu_checkers as CC
 
@@ -210,27 +326,37 @@ def cpu_baseline(scene, substeps):
     sc = copy.deepcopy(scene)
     o = CC.CpuSolver(sc, which)
     o.compute_initial_data()
-    S.apply_frame(sc, sc["frames"][0])
-    o.upload_shapes()
+    if sc["frames"]:
+        S.apply_frame(sc, sc["frames"][0])
+        o.upload_shapes()
     t0 = time.perf_counter()
     o.step(substeps)
     dt = time.perf_counter() - t0
     o.close()
     n = len(scene["xm"])
+    info = host_cpu()
     return {
         "value": n * substeps / dt,
         "unit": "particle-substeps/s",
         "cores": 1,
         "kind": "reference" if which == "ref" else "port",
-        "sample": f"{substeps} substep(s) of the full 262,144-particle scene, 20 iterations, "
-                  f"{dt:.1f} s on 1 core of {platform.processor() or platform.machine()}",
+        "sample": f"{substeps} substep(s) of the full {n}-particle scene{label or ''}, {dt:.1f} s on 1 core",
+        **info,
     }
 
 
-def batched(kind_name, steps, warmup, tune=True, graph=True):
-    """64 scenes of uneven size (config3) advanced as one batch, against the sum
-    of the same scenes run one at a time through the single-scene solver
-    (same kernels).  Returns the batch throughput and the speedup."""
+def batched(kind_name, steps, warmup, tune=True, graph=True, n_streams=4):
+    """64 scenes of uneven size (config3) advanced as one batch, against the
+    same scenes as 64 single-scene solvers (same kernels, each autotuned and
+    graph-replayed like the batch):
+      * sum of plain: each solver's substeps timed alone, summed;
+      * advanceAll analogue: every solver's substeps enqueued solver by
+        solver, as SimulationFactoryImpl::_advanceAllTaskFn steps them
+        (src/engine/simulation_factory_impl.cpp:1106-1150), on one stream
+        (the reference's default, numCudaStreams = 1) and on `n_streams`
+        streams taken by load (getCudaStreamByLoad, :1093-1104), timed from
+        the first enqueue to the last stream's end.
+    Returns the batch throughput and the speedups."""
     import torch
 
     import crmpm_hip as H
@@ -253,22 +379,55 @@ def batched(kind_name, steps, warmup, tune=True, graph=True):
     t_batch = time.perf_counter() - t0
     finite = all(np.isfinite(b.state(i)["xm"]).all() for i in (0, len(scenes) - 1))
     b.close()
-    t_plain = 0.0
+    streams = [torch.cuda.Stream() for _ in range(n_streams)]
+    solvers = []
     for sc in scenes:
-        g = H.HipSolver(copy.deepcopy(sc))
+        g = H.HipSolver(copy.deepcopy(sc), stream=streams[0])
         g.compute_initial_data()
         g.step(warmup)
-        torch.cuda.synchronize()
+        if tune:
+            g.autotune(steps_per_trial=2)
+        g.set_graph_mode(graph)
+        g.step(1)  # capture
+        solvers.append(g)
+    torch.cuda.synchronize()
+    t_plain = 0.0
+    for g in solvers:
         t0 = time.perf_counter()
         g.step(steps)
         torch.cuda.synchronize()
         t_plain += time.perf_counter() - t0
+
+    def advance_all():
+        t0 = time.perf_counter()
+        for g in solvers:
+            g.step(steps)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t_one = advance_all()
+    # n streams by load: each solver takes the least-loaded stream when it is created
+    load = [0] * n_streams
+    for g in solvers:
+        k = min(range(n_streams), key=lambda i: load[i])
+        load[k] += 1
+        g.set_stream(streams[k])
+    torch.cuda.synchronize()
+    t_multi = advance_all()
+    for g in solvers:
         g.close()
     return {"workload": f"{len(scenes)} {kind_name.upper()} scenes, log-uniform 2k-200k particles, grids 32^3-128^3",
             "particles": n_total, "substeps": steps, "iterations": scenes[0]["iterations"] if kind == S.PB else None,
             "value": n_total * steps / t_batch, "unit": "particle-substeps/s",
-            "batch_ms_per_substep": t_batch / steps * 1e3, "sum_of_plain_ms_per_substep": t_plain / steps * 1e3,
+            "batch_ms_per_substep": t_batch / steps * 1e3,
+            "sum_of_plain_ms_per_substep": t_plain / steps * 1e3,
             "speedup_vs_sum_of_plain": t_plain / t_batch,
+            "advance_all_1_stream_ms_per_substep": t_one / steps * 1e3,
+            "speedup_vs_advance_all_1_stream": t_one / t_batch,
+            f"advance_all_{n_streams}_streams_ms_per_substep": t_multi / steps * 1e3,
+            f"speedup_vs_advance_all_{n_streams}_streams": t_multi / t_batch,
+            "plain_solvers": "autotuned, hipGraph replay" if tune and graph else
+                             ("autotuned" if tune else ("hipGraph replay" if graph else "eager")),
             "autotuned": None if tuned is None else dict(zip(TUNED_KEYS, tuned)),
             "hip_graph": graph, "finite": bool(finite)}
 
@@ -341,7 +500,7 @@ def engine(frames=10, substeps=10, warmup=1):
             "readback_GBps": 32.0 * n_total * frames / max(t_eng - t_batch, 1e-9) / 1e9, "finite": finite}
 
 
-def adjoint(steps=100, every=10):
+def adjoint(steps=100, every=10, cpu=True):
     """Differentiable MLS block (config4: 65,536 particles, 96^3 grid, GRF log E):
     time of a forward rollout with checkpoints, of forward + adjoint, and of the
     plain solver's forward, for the same substeps."""
@@ -350,7 +509,7 @@ def adjoint(steps=100, every=10):
     import crmpm_hip as H
 
     sc, log_e = S.config4(steps=steps)
-    hidden, log_e_true = S.config4(steps=steps, field_seed=20261203 + 44)
+    hidden, log_e_true = S.config4(steps=steps, field_seed=S.HIDDEN_FIELD_SEED)
     sim = H.DifferentiableMls(sc, steps, checkpoint_every=every, tune=True)
     dev = sim.solver.device
     nu = 0.3
@@ -361,7 +520,9 @@ def adjoint(steps=100, every=10):
         return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
 
     lt, mt = lame_t(log_e_true)
-    target = sim.forward(lt, mt, v0).double()
+    # final positions under the hidden field plus 1 mm measurement noise
+    noise = torch.as_tensor(S.gaussian_noise(len(log_e), S.TARGET_NOISE_SEED, S.TARGET_NOISE), device=dev)
+    target = sim.forward(lt, mt, v0).double() + noise
     lam, mu = lame_t(log_e)
     n = sim.n
 
@@ -396,6 +557,7 @@ def adjoint(steps=100, every=10):
     plain = H.HipSolver(copy.deepcopy(sc))
     plain.compute_initial_data()
     plain.autotune(steps_per_trial=2)
+    plain.set_graph_mode(True)
     plain.step(5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -408,9 +570,45 @@ def adjoint(steps=100, every=10):
             "plain_forward_ms_per_substep": t_p / steps * 1e3,
             "checkpointed_forward_ms_per_substep": t_f / steps * 1e3,
             "forward_adjoint_ms_per_substep": t_fa / steps * 1e3,
-            "forward_adjoint_vs_plain_forward": t_fa / t_p, "loss": loss,
-            "autotuned": dict(zip(TUNED_KEYS, sim_tuned)),
-            "grad_logE_abs_max": float(g.abs().max()), "finite": finite}
+            "forward_adjoint_vs_plain_forward": t_fa / t_p, "plain_forward": "autotuned, hipGraph replay",
+            "loss": loss, "autotuned": dict(zip(TUNED_KEYS, sim_tuned)),
+            "grad_logE_abs_max": float(g.abs().max()), "grad_v0_abs_max": float(dv0.abs().max()),
+            "cpu_baseline": cpu_baseline(sc, 20, label=" (forward only: the reference has no adjoint)") if cpu else None,
+            "cpu_forward_adjoint": adjoint_cpu_rate(sc, 10) if cpu else None,
+            "finite": finite}
+
+
+def adjoint_cpu_rate(sc, substeps):
+    """The FP64 autodiff restatement of the MLS substep (oracle/mls_autodiff.py,
+    torch on the host, the checker of the GPU adjoint) timed on `substeps`
+    substeps of forward + reverse of the block: the nearest CPU counterpart
+    of the differentiable form, which the reference lacks."""
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as CC
+    import mls_autodiff as AD
+
+    n = len(sc["xm"])
+    o = CC.CpuSolver(copy.deepcopy(sc), "oracle", fp64=True)
+    o.compute_initial_data()
+    V0 = o.state()["V0"]
+    o.close()
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64))  # noqa: E731
+    lam = t(sc["params"][:, 0]).requires_grad_(True)
+    mu = t(sc["params"][:, 1]).requires_grad_(True)
+    v0 = t(sc["v"][:, :3]).requires_grad_(True)
+    threads = torch.get_num_threads()
+    t0 = time.perf_counter()
+    c = AD.consts_from_scene(sc, V0, lam=lam, mu=mu)
+    x, _, _, _ = AD.rollout(t(sc["xm"][:, :3]), v0, torch.eye(3, dtype=torch.float64).repeat(n, 1, 1),
+                            torch.zeros(n, 3, 3, dtype=torch.float64), c, substeps)
+    (x ** 2).sum().backward()
+    dt = time.perf_counter() - t0
+    return {"value": n * substeps / dt, "unit": "particle-substeps/s (forward + adjoint)", "cores": threads,
+            "kind": "port", "sample": f"{substeps} substeps of the {n}-particle block, forward + reverse, FP64 torch "
+                                      f"autograd on {threads} threads, {dt:.1f} s", **host_cpu()}
 
 
 def mls_run(scene, steps, warmup, tune=True):
@@ -435,7 +633,12 @@ def mls_run(scene, steps, warmup, tune=True):
             solver.step(1)
 
     run(0, warmup)
-    tuned = solver.autotune(steps_per_trial=2) + solver.gather_split() if tune else None
+    if LAUNCH:
+        tuned = solver.set_launch(LAUNCH)
+    else:
+        tuned = solver.autotune(steps_per_trial=2) + solver.gather_split() if tune else None
+    # substeps replay a captured hipGraph (the sampled ones, which record events, run eagerly)
+    solver.set_graph_mode(True)
     torch.cuda.synchronize()
     grid = solver.grid()
     touched = int(np.count_nonzero(grid[:, 3] > 0))
@@ -452,7 +655,7 @@ def mls_run(scene, steps, warmup, tune=True):
     return elapsed, prof, touched, tuned, st
 
 
-def mls_kernel_roofline(n, touched, prof, slicer):
+def mls_kernel_roofline(n, touched, prof, slicer, target=None, persistent=False):
     """Achieved HBM rate of the MLS P2G and G2P kernels on their algorithmic
     bytes: P2G reads x, v, F, C, V0, (lambda, mu) = 108 B per particle and
     writes 16 B per touched node; G2P reads x, F (and x again per slicer test,
@@ -464,13 +667,15 @@ def mls_kernel_roofline(n, touched, prof, slicer):
             continue
         avg = ms / k / 1e3
         achieved = (per_p * n + 16 * touched) / avg / 1e9
-        out[name] = {"avg_launch_us": avg * 1e6, "launches": k, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "algorithmic_bytes_per_launch": per_p * n + 16 * touched}
+        row = f"k_{name}<{'true' if persistent else 'false'}>"
+        out[name] = {"kernel": row, "avg_launch_us": avg * 1e6, "launches": k, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_launch": per_p * n + 16 * touched,
+                     **(roofline_counters(profiled_row(target, row), avg) if target else {})}
     return out
 
 
-def mls_cutting(steps=500, warmup=10):
+def mls_cutting(steps=500, warmup=10, cpu=True):
     """MLS at scale with cutting (config5: 2,097,152 particles, 257 x 129 x 257 nodes)."""
     sc = S.config5(steps=steps + warmup)
     n = len(sc["xm"])
@@ -478,8 +683,10 @@ def mls_cutting(steps=500, warmup=10):
     return {"workload": "MLS block 256x32x256 = 2,097,152 particles, 257x129x257 nodes, capsule tool + growing quad "
                         "slicer cutting at 0.02 m/s", "particles": n, "substeps": steps,
             "value": n * steps / elapsed, "unit": "particle-substeps/s", "ms_per_substep": elapsed / steps * 1e3,
-            "touched_nodes": touched, "roofline": mls_kernel_roofline(n, touched, prof, slicer=True),
+            "touched_nodes": touched, "roofline": mls_kernel_roofline(n, touched, prof, slicer=True, target="mls_cutting",
+                                            persistent=tuned[0] > 0),
             "autotuned": dict(zip(TUNED_KEYS, tuned)),
+            "cpu_baseline": cpu_baseline(sc, 2) if cpu else None,
             "finite": bool(np.isfinite(st["xm"]).all())}
 
 
@@ -561,15 +768,31 @@ def mesh_sdf(reps=3):
     return res
 
 
-def mls_cube(steps=200, warmup=5):
+def mls_cube(steps=200, warmup=5, cpu=True):
     """The reference's CPU-runnable case (config1: 32^3 = 32,768 particles, 64^3 cells, 200 substeps)."""
     sc = S.config1(steps=steps + warmup)
     n = len(sc["xm"])
     elapsed, prof, touched, tuned, st = mls_run(sc, steps, warmup)
     return {"workload": "MLS elastic cube 32^3 = 32,768 particles on 65^3 nodes, ground plane", "particles": n,
             "substeps": steps, "value": n * steps / elapsed, "unit": "particle-substeps/s",
-            "ms_per_substep": elapsed / steps * 1e3, "roofline": mls_kernel_roofline(n, touched, prof, slicer=False),
+            "ms_per_substep": elapsed / steps * 1e3, "roofline": mls_kernel_roofline(n, touched, prof, slicer=False, target="mls_cube",
+                                            persistent=tuned[0] > 0),
+            "autotuned": dict(zip(TUNED_KEYS, tuned)),
+            "cpu_baseline": cpu_baseline(sc, 50) if cpu else None,
             "finite": bool(np.isfinite(st["xm"]).all())}
+
+
+def nodes_in_indenter(solver, grid):
+    """Grid nodes (after a substep's update) holding mass whose position lies
+    inside the scene's sphere indenter (shape 0)."""
+    sc = solver.scene
+    nx, ny, nz = solver.dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([i.ravel(), j.ravel(), k.ravel()], 1) * sc["dx"] + np.asarray(sc["lo"], np.float64)
+    sh = sc["shape_list"][0]
+    r = sc["geoms"][sh["geom"]][1][0]
+    inside = np.linalg.norm(pos - np.asarray(sh["pos"][:3]), axis=1) < r
+    return int((inside & (grid[:, 3] > 0)).sum())
 
 
 def main():
@@ -579,8 +802,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=4,
                     help="substeps in the CPU baseline sample, about 3 s each on one core (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "lattice", "meshsdf", "engine"], default="all",
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "mls_cutting", "mls_cube",
+                                           "lattice", "meshsdf", "engine"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
+    ap.add_argument("--launch", metavar="T,U,S,M,GS,GM",
+                    help="use this launch configuration (tile waves, update waves, sub-tile size, sub-tiles, gather "
+                         "sub-tile size, gather sub-tiles) instead of the autotuner's (PB slab and MLS workloads): "
+                         "profiled runs pass the one the timed run picked, so no autotuner trial enters their stats")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
     ap.add_argument("--summarize", metavar="DIR",
@@ -590,15 +818,20 @@ def main():
     ap.add_argument("--summary-out", metavar="DIR", default=os.path.join(ROOT, "profiles"),
                     help="where --summarize writes its .md and .json (default: profiles/)")
     args = ap.parse_args()
+    global LAUNCH
+    if args.launch:
+        LAUNCH = tuple(int(x) for x in args.launch.split(","))
+        assert len(LAUNCH) == 6, "--launch takes 6 integers"
     if args.summarize:
         tgt = PROFILE_TARGETS[args.profile_target]
         md, traffic = summarize(args.summarize, tgt["title"], profile_commands(args.profile_target), tgt["counters"])
         os.makedirs(args.summary_out, exist_ok=True)
         with open(os.path.join(args.summary_out, tgt["md"]), "w") as f:
             f.write(md)
-        if args.profile_target == "pb_slab":
-            with open(os.path.join(args.summary_out, os.path.basename(TRAFFIC_FILE)), "w") as f:
-                json.dump(traffic, f, indent=1)
+        if traffic is not None:
+            # per-kernel counter figures, which bench.py's roofline lines quote
+            with open(os.path.join(args.summary_out, counters_file(args.profile_target)), "w") as f:
+                json.dump(traffic, f, indent=1, sort_keys=True)
         print(md)
         return
     if args.copy_calibration:
@@ -614,8 +847,14 @@ def main():
 
     import crmpm_hip as H
 
-    if args.workload == "mls":
-        print(json.dumps({"mls_cutting": mls_cutting(), "mls_cube": mls_cube()}), flush=True)
+    cpu = args.cpu_substeps > 0
+    if args.workload in ("mls", "mls_cutting", "mls_cube"):
+        out = {}
+        if args.workload != "mls_cube":
+            out["mls_cutting"] = mls_cutting(cpu=cpu)
+        if args.workload != "mls_cutting":
+            out["mls_cube"] = mls_cube(cpu=cpu)
+        print(json.dumps(out), flush=True)
         return
     if args.workload == "lattice":
         print(json.dumps({"lattice": lattice()}), flush=True)
@@ -627,7 +866,7 @@ def main():
         print(json.dumps({"mesh_sdf": mesh_sdf()}), flush=True)
         return
     if args.workload == "adjoint":
-        print(json.dumps({"adjoint": adjoint()}), flush=True)
+        print(json.dumps({"adjoint": adjoint(cpu=cpu)}), flush=True)
         return
     if args.workload == "batched":
         print(json.dumps({"batched_mls": batched("mls", 100, 5), "batched_pb": batched("pb", 20, 2)}), flush=True)
@@ -652,11 +891,12 @@ def main():
 
     run(0, W)
     # the library's autotuner picks the tile split and launch sizes on this state (which it leaves unchanged)
-    tuned = solver.autotune(steps_per_trial=2) + solver.gather_split()
+    tuned = solver.set_launch(LAUNCH) if LAUNCH else solver.autotune(steps_per_trial=2) + solver.gather_split()
     torch.cuda.synchronize()
     # touched nodes of one substep (grid nodes holding mass), for the grid term of the roofline
     grid = solver.grid()
     touched = int(np.count_nonzero(grid[:, 3] > 0))
+    contact_start = nodes_in_indenter(solver, grid)
 
     # Every PROFILE_EVERY-th substep records HIP events around each of its
     # launches (a recorded launch costs a few microseconds of dispatch, so
@@ -677,13 +917,8 @@ def main():
     avg_fused_s = fused_ms / max(fused_n, 1) / 1e3
     bytes_fused = BYTES_PER_PARTICLE_FUSED * n + BYTES_PER_TOUCHED_NODE_FUSED * touched
     achieved = bytes_fused / avg_fused_s / 1e9
-    traffic = None
-    valu_insts = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            prof_summary = json.load(f)
-        traffic = prof_summary.get("bytes_per_launch")
-        valu_insts = prof_summary.get("valu_instructions")
+    fused_row = f"k_pb_tile<1, {'true' if tuned[0] > 0 else 'false'}>"
+    counters = roofline_counters(profiled_row("pb_slab", fused_row), avg_fused_s)
     out = {
         "metric": METRIC,
         "value": value,
@@ -701,20 +936,27 @@ def main():
                    "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
                    "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu",
                    "autotuned": dict(zip(TUNED_KEYS, tuned))},
-        "roofline": {"bound": "hbm", "kernel": "k_pb_tile<1, ...> (fused G2P+P2G)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "roofline": {"bound": "hbm", "kernel": fused_row + " (fused G2P+P2G)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
                      "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched,
-                     # the fused kernel's VALU issue time (rocprofv3 SQ_INSTS_VALU of this workload) over its duration
-                     "valu_issue_frac": None if valu_insts is None else valu_issue_us(valu_insts) / (avg_fused_s * 1e6)},
+                     # HBM-side traffic, L2 hit rate, DRAM-path bytes, atomics and VALU issue of the
+                     # same kernel from the committed rocprofv3 counter passes of this workload
+                     # (profiles/pb_slab262k_counters.json)
+                     **counters},
     }
     st = solver.state()
     out["state_check"] = {"nonfinite_particles": int((~np.isfinite(st["xm"])).any(axis=1).sum()),
                           "max_speed": float(np.abs(st["v"]).max()),
-                          "y_range": [float(st["xm"][:, 1].min()), float(st["xm"][:, 1].max())]}
+                          "y_range": [float(st["xm"][:, 1].min()), float(st["xm"][:, 1].max())],
+                          # grid nodes holding mass inside the sphere indenter at the first and the
+                          # last substep of the timed window: the contact the window exercises
+                          "indenter_contact_nodes": [contact_start, nodes_in_indenter(solver, solver.grid())],
+                          "indenter_bottom_y": float(solver.scene["shape_list"][0]["pos"][1]
+                                                     - solver.scene["geoms"][0][1][0])}
     solver.close()
     if args.cpu_substeps > 0:
-        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps, label=", 20 iterations")
     else:
         out["cpu_baseline"] = None
     if args.workload == "all":

@@ -3904,6 +3904,16 @@ int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *
     return s ? batch_autotune(s->b, stepsPerTrial, tileWaves, updateWaves, subSize, subMax) : CRMPM_HIP_ERR_ARG;
 }
 
+int crHipSolverSetLaunchConfig(CrHipSolver *s, int tileWaves, int updateWaves, int subSize, int subMax,
+                               int gatherSubSize, int gatherSubMax)
+{
+    if (!s)
+        return CRMPM_HIP_ERR_ARG;
+    int rc = crHipBatchSetTileSplit(s->b, subSize, subMax);
+    rc = rc ? rc : crHipBatchSetGatherSplit(s->b, gatherSubSize, gatherSubMax);
+    return rc ? rc : crHipBatchSetLaunchSizes(s->b, tileWaves, updateWaves);
+}
+
 int crHipSolverSetGraphMode(CrHipSolver *s, int enabled)
 {
     return s ? crHipBatchSetGraphMode(s->b, enabled) : CRMPM_HIP_ERR_ARG;

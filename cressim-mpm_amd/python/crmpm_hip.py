@@ -137,6 +137,7 @@ EXPORTS = {
     "crHipBatchSetGatherSplit": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipSolverGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverSetLaunchConfig": [_vp] + [ctypes.c_int] * 6,
     "crHipSolverSetGraphMode": [_vp, ctypes.c_int],
     "crHipBatchFetchResults": [_vp],
     "crHipBatchGetStateArrays": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -391,6 +392,12 @@ class HipSolver:
         self._idx = torch.from_numpy(np.ascontiguousarray(idx, np.uint32).view(np.int32)).to(self.device)
         check(self.lib.crHipSolverComputeInitialData(self.h, len(idx), self._idx.data_ptr()), "initial data")
 
+    def set_stream(self, stream):
+        """MpmSolverGpu::setCudaStream: later work goes to `stream` (a torch
+        stream); the caller orders it after the work already on the old one."""
+        self.stream = stream
+        check(self.lib.crHipSolverSetStream(self.h, _vp(stream.cuda_stream)), "stream")
+
     def step(self, n=1):
         dt = ctypes.c_float()
         for _ in range(n):
@@ -455,6 +462,13 @@ class HipSolver:
     def set_graph_mode(self, on):
         """Replay each substep as a hipGraph (captured on first use)."""
         check(self.lib.crHipSolverSetGraphMode(self.h, 1 if on else 0), "graph mode")
+
+    def set_launch(self, cfg):
+        """Set a launch configuration (tile waves, update waves, sub-tile size,
+        sub-tiles per pblock, gather sub-tile size, gather sub-tiles) without
+        timing trials; returns it as autotune() + gather_split() would."""
+        check(self.lib.crHipSolverSetLaunchConfig(self.h, *[int(c) for c in cfg]), "launch config")
+        return tuple(int(c) for c in cfg)
 
     def gather_split(self):
         """(sub-tile size, sub-tiles per pblock) of the G2P kernels."""

@@ -499,6 +499,22 @@ def grf(rng, points, length=0.1, features=256):
     return np.sqrt(2.0 / features) * np.cos(np.asarray(points, np.float64) @ w.T + b).sum(1)
 
 
+def gaussian_noise(n, seed, sigma):
+    """(n, 3) Gaussian noise of standard deviation sigma, Box-Muller over the
+    SplitMix64 stream (measurement noise on the differentiable block's targets)."""
+    rng = SplitMix64(seed)
+    u1 = np.maximum(rng.uniform(3 * n), 1e-300)
+    u2 = rng.uniform(3 * n)
+    return (sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(n, 3)
+
+
+# targets of the differentiable block: final positions under the hidden log E
+# field, plus 1 mm of measurement noise per coordinate
+HIDDEN_FIELD_SEED = 20261203 + 44
+TARGET_NOISE_SEED = 20261203 + 45
+TARGET_NOISE = 1e-3
+
+
 def config4(nx=64, ny=32, nz=32, steps=100, seed=20261203 + 4, field_seed=None):
     """Differentiable MLS block: nx x ny x nz = 65,536 particles at spacing dx/2
     on a 96^3-cell unit box (dx = 1/96), per-particle log E from a smooth

@@ -148,6 +148,13 @@ int crHipSolverBinParticles(CrHipSolver *s, unsigned int *keysOut, unsigned int 
 /* crHipBatchAutotune for the solver's one-scene batch (no counterpart in the
  * reference): picks the tile split and launch sizes on the current state. */
 int crHipSolverAutotune(CrHipSolver *s, int stepsPerTrial, int *tileWaves, int *updateWaves, int *subSize, int *subMax);
+/* A launch configuration as crHipSolverAutotune reports it (tile split,
+ * gather split, launch sizes), set without timing trials: crHipBatchSetTileSplit,
+ * crHipBatchSetGatherSplit and crHipBatchSetLaunchSizes for the solver's
+ * one-scene batch (e.g. to profile the configuration a timed run used).  No
+ * counterpart in the reference. */
+int crHipSolverSetLaunchConfig(CrHipSolver *s, int tileWaves, int updateWaves, int subSize, int subMax,
+                               int gatherSubSize, int gatherSubMax);
 /* crHipBatchSetGraphMode for the solver's one-scene batch: replay the
  * substep as a hipGraph (no counterpart in the reference). */
 int crHipSolverSetGraphMode(CrHipSolver *s, int enabled);

@@ -19,9 +19,7 @@ import crmpm_scenes as S
 N_SIDE = 16
 STEPS = 100
 EVERY = 10
-HIDDEN_FIELD_SEED = 20261203 + 44
-TARGET_NOISE = 1e-3
-NOISE_SEED = 20261203 + 45
+HIDDEN_FIELD_SEED = S.HIDDEN_FIELD_SEED
 
 
 def scene():
@@ -33,12 +31,8 @@ def hidden_scene():
     return S.config4(nx=N_SIDE, ny=N_SIDE, nz=N_SIDE, steps=STEPS, field_seed=HIDDEN_FIELD_SEED)
 
 
-def measurement_noise(n, seed=NOISE_SEED, sigma=TARGET_NOISE):
-    """(n, 3) Gaussian noise by Box-Muller from the SplitMix64 stream."""
-    rng = S.SplitMix64(seed)
-    u1 = np.maximum(rng.uniform(3 * n), 1e-300)
-    u2 = rng.uniform(3 * n)
-    return (sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(n, 3)
+def measurement_noise(n):
+    return S.gaussian_noise(n, S.TARGET_NOISE_SEED, S.TARGET_NOISE)
 
 
 def oracle_final_positions(sc, steps=STEPS):
