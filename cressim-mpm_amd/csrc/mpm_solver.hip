@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "crmpm_hip.h"
@@ -3455,6 +3456,112 @@ namespace
             ++idx;
         }
     }
+
+    // Argument checks and launch arguments shared by the lattice entry points:
+    // bounds fattened by 2 spacings (Bounds3::fattenFast, include/math/bounds3.h:85-94);
+    // lattice dimensions ((max - min) / spacing / invScale).cast<int>().  blocks = 0
+    // for an empty lattice (*numParticles is then 0).
+    int lattice_setup(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf, const float boundsMin[3],
+                      const float boundsMax[3], float particleSpacing, const float position[3], const float rotation[4],
+                      const float invScale[3], float particleMass, const float *positionMassOut, unsigned capacity,
+                      unsigned *numParticles, LatticeArgs &a, int &blocks)
+    {
+        blocks = 0;
+        if (!params0 || !boundsMin || !boundsMax || !position || !rotation || !invScale || !numParticles ||
+            (capacity > 0 && !positionMassOut) || geomType < gBox || geomType > gArc || !(particleSpacing > 0.0f))
+            return CRMPM_HIP_ERR_ARG;
+        const bool table = geomType == gTriangleMesh || geomType == gTriangleMeshSlicer || geomType == gLines;
+        if (table && (!sdf || !sdf->dimension || !sdf->lowerBoundCellSize || !sdf->gradientSignedDistance))
+            return CRMPM_HIP_ERR_ARG;
+        float lo[3], dimf[3];
+        int dims[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            if (!(invScale[k] > 0.0f) || !std::isfinite(boundsMin[k]) || !std::isfinite(boundsMax[k]))
+                return CRMPM_HIP_ERR_ARG;
+            const float d = 2.0f * particleSpacing;
+            lo[k] = boundsMin[k] - d;
+            const float hi = boundsMax[k] + d;
+            dimf[k] = (hi - lo[k]) / particleSpacing / invScale[k];
+            if (!(dimf[k] < 2147483647.0f))
+                return CRMPM_HIP_ERR_CAPACITY;
+            dims[k] = std::max(0, (int)dimf[k]);
+        }
+        const long long total = (long long)dims[0] * dims[1] * dims[2];
+        if (total >= (1ll << 31))
+            return CRMPM_HIP_ERR_CAPACITY;
+        *numParticles = 0;
+        if (total == 0)
+            return CRMPM_HIP_OK;
+        if (table)
+        {
+            a.sv.sdfDims = reinterpret_cast<const int4 *>(sdf->dimension);
+            a.sv.sdfLbcs = reinterpret_cast<const float4 *>(sdf->lowerBoundCellSize);
+            a.sv.sdfData = reinterpret_cast<const float4 *>(sdf->gradientSignedDistance);
+        }
+        a.p0 = make_float4(params0[0], params0[1], params0[2], params0[3]);
+        a.rot = make_float4(rotation[0], rotation[1], rotation[2], rotation[3]);
+        a.lo = make_float3(lo[0], lo[1], lo[2]);
+        a.invScale = make_float3(invScale[0], invScale[1], invScale[2]);
+        a.pos = make_float3(position[0], position[1], position[2]);
+        a.spacing = particleSpacing;
+        a.mass = particleMass;
+        a.gtype = geomType;
+        a.ny = dims[1];
+        a.nz = dims[2];
+        a.total = total;
+        blocks = (int)((total + kLatticeBlock - 1) / kLatticeBlock);
+        return CRMPM_HIP_OK;
+    }
+
+    // Count, scan and write passes with the per-block counts in `counts`
+    // (blocks + 1 entries, each written before it is read), then the total back
+    // to the host.
+    int lattice_run(const LatticeArgs &a, int blocks, unsigned *counts, float *positionMassOut, unsigned capacity,
+                    unsigned *numParticles, hipStream_t st)
+    {
+        k_lattice_count<<<blocks, kWG, 0, st>>>(a, counts);
+        k_lattice_scan<<<1, kWG, 0, st>>>(blocks, counts);
+        k_lattice_write<<<blocks, kWG, 0, st>>>(a, counts, reinterpret_cast<float4 *>(positionMassOut), capacity);
+        unsigned n = 0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&n, counts + blocks, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return CRMPM_HIP_ERR_LAUNCH;
+        *numParticles = n;
+        return CRMPM_HIP_OK;
+    }
+
+    // Scratch of crHipParticleLatticeNoAlloc: the per-block counts and their
+    // total for lattices of up to kLatticeScratchBlocks blocks of kLatticeBlock
+    // points (2^28 points), 512 KiB per GPU, held until the process ends.  In
+    // host code the name stands for no device address: kernels and copies take
+    // the per-device address from lattice_scratch().
+    constexpr int kLatticeScratchBlocks = 1 << 17;
+    static __device__ __align__(256) unsigned scratch[kLatticeScratchBlocks + 1];
+
+    constexpr int kMaxScratchDevices = 64;
+    std::mutex gScratchLock;
+    void *gScratchAddr[kMaxScratchDevices] = {};
+
+    // The current device's copy of `scratch`: looked up once per device (with
+    // that device current) by hipGetSymbolAddress, then kept.
+    int lattice_scratch(unsigned **out)
+    {
+        int dev = -1;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+            return CRMPM_HIP_ERR_LAUNCH;
+        std::lock_guard<std::mutex> lock(gScratchLock);
+        if (!gScratchAddr[dev])
+        {
+            void *p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+                return CRMPM_HIP_ERR_LAUNCH;
+            gScratchAddr[dev] = p;
+        }
+        *out = static_cast<unsigned *>(gScratchAddr[dev]);
+        return CRMPM_HIP_OK;
+    }
 } // namespace
 
 extern "C" {
@@ -4250,70 +4357,38 @@ int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeomet
                          const float rotation[4], const float invScale[3], float particleMass, float *positionMassOut,
                          unsigned int capacity, unsigned int *numParticles, void *stream)
 {
-    if (!params0 || !boundsMin || !boundsMax || !position || !rotation || !invScale || !numParticles ||
-        (capacity > 0 && !positionMassOut) || geomType < gBox || geomType > gArc || !(particleSpacing > 0.0f))
-        return CRMPM_HIP_ERR_ARG;
-    const bool table = geomType == gTriangleMesh || geomType == gTriangleMeshSlicer || geomType == gLines;
-    if (table && (!sdf || !sdf->dimension || !sdf->lowerBoundCellSize || !sdf->gradientSignedDistance))
-        return CRMPM_HIP_ERR_ARG;
-    // bounds fattened by 2 spacings (Bounds3::fattenFast, include/math/bounds3.h:85-94); lattice
-    // dimensions ((max - min) / spacing / invScale).cast<int>()
-    float lo[3], dimf[3];
-    int dims[3];
-    for (int k = 0; k < 3; ++k)
-    {
-        if (!(invScale[k] > 0.0f) || !std::isfinite(boundsMin[k]) || !std::isfinite(boundsMax[k]))
-            return CRMPM_HIP_ERR_ARG;
-        const float d = 2.0f * particleSpacing;
-        lo[k] = boundsMin[k] - d;
-        const float hi = boundsMax[k] + d;
-        dimf[k] = (hi - lo[k]) / particleSpacing / invScale[k];
-        if (!(dimf[k] < 2147483647.0f))
-            return CRMPM_HIP_ERR_CAPACITY;
-        dims[k] = std::max(0, (int)dimf[k]);
-    }
-    const long long total = (long long)dims[0] * dims[1] * dims[2];
-    if (total >= (1ll << 31))
-        return CRMPM_HIP_ERR_CAPACITY;
-    *numParticles = 0;
-    if (total == 0)
-        return CRMPM_HIP_OK;
     LatticeArgs a{};
-    if (table)
-    {
-        a.sv.sdfDims = reinterpret_cast<const int4 *>(sdf->dimension);
-        a.sv.sdfLbcs = reinterpret_cast<const float4 *>(sdf->lowerBoundCellSize);
-        a.sv.sdfData = reinterpret_cast<const float4 *>(sdf->gradientSignedDistance);
-    }
-    a.p0 = make_float4(params0[0], params0[1], params0[2], params0[3]);
-    a.rot = make_float4(rotation[0], rotation[1], rotation[2], rotation[3]);
-    a.lo = make_float3(lo[0], lo[1], lo[2]);
-    a.invScale = make_float3(invScale[0], invScale[1], invScale[2]);
-    a.pos = make_float3(position[0], position[1], position[2]);
-    a.spacing = particleSpacing;
-    a.mass = particleMass;
-    a.gtype = geomType;
-    a.ny = dims[1];
-    a.nz = dims[2];
-    a.total = total;
-    const int blocks = (int)((total + kLatticeBlock - 1) / kLatticeBlock);
-    hipStream_t st = static_cast<hipStream_t>(stream);
+    int blocks = 0;
+    int rc = lattice_setup(geomType, params0, sdf, boundsMin, boundsMax, particleSpacing, position, rotation, invScale,
+                           particleMass, positionMassOut, capacity, numParticles, a, blocks);
+    if (rc || blocks == 0)
+        return rc;
     unsigned *counts = nullptr;
     if (dalloc(&counts, (size_t)blocks + 1))
         return CRMPM_HIP_ERR_ALLOC;
-    int rc = CRMPM_HIP_OK;
-    k_lattice_count<<<blocks, kWG, 0, st>>>(a, counts);
-    k_lattice_scan<<<1, kWG, 0, st>>>(blocks, counts);
-    k_lattice_write<<<blocks, kWG, 0, st>>>(a, counts, reinterpret_cast<float4 *>(positionMassOut), capacity);
-    unsigned n = 0;
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&n, counts + blocks, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        rc = CRMPM_HIP_ERR_LAUNCH;
+    rc = lattice_run(a, blocks, counts, positionMassOut, capacity, numParticles, static_cast<hipStream_t>(stream));
     dfree(counts);
-    if (!rc)
-        *numParticles = n;
     return rc;
+}
+
+int crHipParticleLatticeNoAlloc(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf,
+                                const float boundsMin[3], const float boundsMax[3], float particleSpacing,
+                                const float position[3], const float rotation[4], const float invScale[3],
+                                float particleMass, float *positionMassOut, unsigned int capacity,
+                                unsigned int *numParticles, void *stream)
+{
+    LatticeArgs a{};
+    int blocks = 0;
+    int rc = lattice_setup(geomType, params0, sdf, boundsMin, boundsMax, particleSpacing, position, rotation, invScale,
+                           particleMass, positionMassOut, capacity, numParticles, a, blocks);
+    if (rc || blocks == 0)
+        return rc;
+    if (blocks > kLatticeScratchBlocks)
+        return CRMPM_HIP_ERR_CAPACITY; // more than 2^28 lattice points: the scratch array holds no more
+    unsigned *counts = nullptr;
+    if ((rc = lattice_scratch(&counts)))
+        return rc;
+    return lattice_run(a, blocks, counts, positionMassOut, capacity, numParticles, static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"

@@ -52,6 +52,8 @@ _f4 = ctypes.c_float * 4
 EXPORTS = {
     "crHipParticleLattice": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3, _f4, _f3,
                              ctypes.c_float, _vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), _vp],
+    "crHipParticleLatticeNoAlloc": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3,
+                                    _f4, _f3, ctypes.c_float, _vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), _vp],
     "crHipMeshSdfDesc": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "crHipMeshSdf": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
@@ -183,13 +185,15 @@ def check(rc, what):
         raise RuntimeError(f"{what} failed with status {rc}")
 
 
-def particle_lattice(obj, device="cuda:0", capacity=None):
-    """ParticleObjectImpl::initialize on the device (crHipParticleLattice).
-    obj: a dict from crmpm_scenes.lattice_object.  Returns (count, (m, 4)
-    float32 tensor of position|mass) with m = min(count, capacity)."""
+def particle_lattice(obj, device="cuda:0", capacity=None, no_alloc=False):
+    """ParticleObjectImpl::initialize on the device (crHipParticleLattice, or
+    crHipParticleLatticeNoAlloc with no_alloc).  obj: a dict from
+    crmpm_scenes.lattice_object.  Returns (count, (m, 4) float32 tensor of
+    position|mass) with m = min(count, capacity)."""
     import torch
 
     lib = load_library()
+    fn = lib.crHipParticleLatticeNoAlloc if no_alloc else lib.crHipParticleLattice
     dev = torch.device(device)
     stream = torch.cuda.current_stream(dev).cuda_stream
     sdf_t = [torch.as_tensor(np.ascontiguousarray(obj[k], dt), device=dev)
@@ -199,10 +203,10 @@ def particle_lattice(obj, device="cuda:0", capacity=None):
             _f3(*obj["bounds_max"]), float(obj["spacing"]), _f3(*obj["position"]), _f4(*obj["rotation"]),
             _f3(*obj["inv_scale"]), float(obj["mass"]))
     n = ctypes.c_uint(0)
-    check(lib.crHipParticleLattice(*args, None, 0, ctypes.byref(n), _vp(stream)), "particle lattice count")
+    check(fn(*args, None, 0, ctypes.byref(n), _vp(stream)), "particle lattice count")
     m = n.value if capacity is None else min(n.value, capacity)
     out = torch.empty((max(m, 1), 4), dtype=torch.float32, device=dev)
-    check(lib.crHipParticleLattice(*args, _vp(out.data_ptr()), m, ctypes.byref(n), _vp(stream)), "particle lattice")
+    check(fn(*args, _vp(out.data_ptr()), m, ctypes.byref(n), _vp(stream)), "particle lattice")
     return n.value, out[:m]
 
 

@@ -438,6 +438,19 @@ int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeomet
                          const float boundsMax[3], float particleSpacing, const float position[3],
                          const float rotation[4], const float invScale[3], float particleMass, float *positionMassOut,
                          unsigned int capacity, unsigned int *numParticles, void *stream);
+/* crHipParticleLattice without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors, except that the per-block counts live in a
+ * fixed device array of 512 KiB per GPU (taken on the first call on a device
+ * and held until the process ends): lattices of more than 2^28 points return
+ * CRMPM_HIP_ERR_CAPACITY.  Calls that overlap on one GPU (other threads,
+ * other streams) share that array and race: serialise them. */
+int crHipParticleLatticeNoAlloc(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf,
+                                const float boundsMin[3], const float boundsMax[3], float particleSpacing,
+                                const float position[3], const float rotation[4], const float invScale[3],
+                                float particleMass, float *positionMassOut, unsigned int capacity,
+                                unsigned int *numParticles, void *stream);
 
 /* ------------------------------------------------------------------------
  * Differentiable MLS-MPM (one scene, no rigid shapes).  The reference has no
