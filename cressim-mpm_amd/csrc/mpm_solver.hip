@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -857,11 +858,24 @@ namespace
     // Accumulation tiles per wavefront of MLS P2G and the fused PB iteration kernel.
     constexpr int kScatterTiles = 2;
 
+    // Momentum-only accumulation node (the fused PB iteration kernel: a
+    // pblock's node masses do not change over a substep's iterations).
+    struct F3
+    {
+        float x, y, z;
+    };
+
     template <int Tiles>
     CR_DEV void zero_tiles(float4 *A, int lane)
     {
         for (int i = lane; i < Tiles * kTileNodes; i += 64)
             A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    template <int Tiles>
+    CR_DEV void zero_tiles(F3 *A, int lane)
+    {
+        for (int i = lane; i < Tiles * kTileNodes; i += 64)
+            A[i] = F3{0.f, 0.f, 0.f};
     }
     // out = sum of the Tiles accumulation tiles, in tile order.
     template <int Tiles>
@@ -882,6 +896,25 @@ namespace
             out[i] = v;
         }
     }
+    // Momentum tiles: out's xyz = the sum in tile order; out's w (the node
+    // mass the substep's first iteration stored there) is kept.
+    template <int Tiles>
+    CR_DEV void store_tile_sum(const F3 *A, float4 *out, int lane)
+    {
+        for (int i = lane; i < kTileNodes; i += 64)
+        {
+            F3 v = A[i];
+#pragma unroll
+            for (int j = 1; j < Tiles; ++j)
+            {
+                const F3 u = A[j * kTileNodes + i];
+                v.x += u.x;
+                v.y += u.y;
+                v.z += u.z;
+            }
+            *reinterpret_cast<F3 *>(out + i) = v;
+        }
+    }
 
     CR_DEV void lds_add(float4 *A, int li, float x, float y, float z, float w)
     {
@@ -890,6 +923,15 @@ namespace
         cur.y += y;
         cur.z += z;
         cur.w += w;
+        A[li] = cur;
+        compiler_fence();
+    }
+    CR_DEV void lds_add(F3 *A, int li, float x, float y, float z, float)
+    {
+        F3 cur = A[li];
+        cur.x += x;
+        cur.y += y;
+        cur.z += z;
         A[li] = cur;
         compiler_fence();
     }
@@ -1161,7 +1203,8 @@ namespace
         }
     }
     // Scatter (term * w, w * mfac) to the 27 nodes, w = wxm[a] wy[b] wz[c].
-    CR_DEV void scatter_separable(float4 *A, int l0, const ScatterTerms &t, float mfac)
+    template <typename Acc>
+    CR_DEV void scatter_separable(Acc *A, int l0, const ScatterTerms &t, float mfac)
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -1574,8 +1617,8 @@ namespace
 
     // Scatter of one PB particle into the LDS tiles A0[0 .. kScatterTiles)
     // (pbMpmParticleToGrid loop)
-    template <int Tiles>
-    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, float4 *A0, int nSlicers, V3 x, float m, V3 vel,
+    template <int Tiles, typename Acc>
+    CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, Acc *A0, int nSlicers, V3 x, float m, V3 vel,
                            const M3 &Cm)
     {
         const GridGeom &g = c.g;
@@ -1588,7 +1631,7 @@ namespace
         const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
         in_conflict_free_passes<Tiles>(cell_key(ti, bx, by, bz), [&](int tile) {
-            float4 *A = A0 + tile * kTileNodes;
+            Acc *A = A0 + tile * kTileNodes;
             if (fast)
             {
                 ScatterTerms tt = terms;
@@ -1622,12 +1665,17 @@ namespace
     // pbMpmIntegrateParticle, results written back in particle order.
     // Mode 0 scatters into 3 accumulation tiles (it stages no grid tile),
     // Mode 1 into 2 (a third one, and its occupancy and register cost, did not
-    // pay for the scatter passes it saves there).
+    // pay for the scatter passes it saves there).  Positions and masses are
+    // fixed over a substep's iterations, so every iteration scatters the same
+    // node masses: Mode 0 stores them with the momenta in each pblock's partial
+    // box, Mode 1 accumulates and stores momentum only (12 of 16 bytes per
+    // node) and leaves Mode 0's masses in the boxes for the grid update.
     template <int Mode, bool Persistent>
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
         constexpr int kAcc = (Mode == 0) ? 3 : kScatterTiles;
-        __shared__ float4 acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
+        using Acc = typename std::conditional<Mode == 0, float4, F3>::type;
+        __shared__ Acc acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
@@ -1641,7 +1689,7 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            float4 *A = acc[(Mode < 2) ? wave * kAcc : 0];
+            Acc *A = acc[(Mode < 2) ? wave * kAcc : 0];
             float4 *G = lg[(Mode > 0) ? wave : 0];
             if (Mode < 2)
                 zero_tiles<kAcc>(A, lane);

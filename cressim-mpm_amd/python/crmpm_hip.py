@@ -160,11 +160,13 @@ PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g"
 _LIB = None
 
 
-def load_library(path=LIB_PATH):
-    """Load libcrmpm_hip.so and declare every C-ABI entry point."""
+def load_library(path=None):
+    """Load libcrmpm_hip.so (the in-tree build, or the file CRMPM_HIP_LIB
+    names) and declare every C-ABI entry point."""
     global _LIB
     if _LIB is not None:
         return _LIB
+    path = path or os.environ.get("CRMPM_HIP_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"libcrmpm_hip.so not found at {path}; run __graft_entry__.build()")
     # torch first: its HIP runtime (soname libamdhip64.so.7) then satisfies the
