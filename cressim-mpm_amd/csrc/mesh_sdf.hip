@@ -418,7 +418,10 @@ namespace crhip
         // candidate it accepts on the way falls on the boundary
         // (aabb_tree.cpp:724-760); here only the closest one counts, so the
         // nodes kept are a superset of the reference's.
-        template <bool Slicer>
+        // Unsigned: the distance only (the reference rule signs it with
+        // k_mesh_sdf_sign_tree afterwards); the slices then carry the closest
+        // triangle's index, as for the slicer.
+        template <bool Slicer, bool Unsigned = false>
         __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_combine(MeshSdfArgs a, int slices, float *dist,
                                                                     unsigned char *onMesh)
         {
@@ -440,13 +443,18 @@ namespace crhip
                     bestP = v3(p);
                     bestT = __float_as_int(a.partWn[o]);
                 }
-                if (!Slicer)
+                if (!Slicer && !Unsigned)
                 {
                     wn += a.partWn[o];
                     err += a.partErr[o];
                 }
             }
             const float d = norm(bestP - q);
+            if (Unsigned)
+            {
+                dist[node] = d;
+                return;
+            }
             if (!Slicer)
             {
                 // where the far-field sum cannot decide the side of 0.5 the
@@ -507,6 +515,197 @@ namespace crhip
             }
             if (need && wn > 0.5f)
                 dist[node] = -dist[node];
+        }
+
+        // ------------------------------------------------------------------
+        // The reference's rules (the default), over its AABB tree built on the
+        // host (build_ref_tree below).  Node i of the tree: lo = (box minimum,
+        // data bits), hi = (box maximum, bits of the node after i's subtree in
+        // a positive-child-first walk, -1 after the last); data is the
+        // reference's flattened node word (leaf: triangle << 5 | 2 | 1,
+        // internal: positive child << 1, negative child next to it).
+        // ------------------------------------------------------------------
+        struct RefTreeArgs
+        {
+            const float4 *lo, *hi;
+            const float4 *cluC; // per node: area-weighted centroid, radius
+            const float4 *cluN; // per node: area-weighted normal sum
+        };
+
+        CR_DEV unsigned tree_data(const RefTreeArgs &t, int i) { return __float_as_uint(t.lo[i].w); }
+
+        // evaluateExact (include/geometry_utils/winding_number.h:79-93) in the
+        // reference's own FP32 operations: IEEE square roots, no
+        // contraction, the math library's atan2f.
+        CR_DEV float winding_term_ref(V3 a, V3 b, V3 c, V3 p)
+        {
+            const float k = float(0.5 / 3.141592653589793238462643383);
+            a = a - p;
+            b = b - p;
+            c = c - p;
+            const float la = sqrtf(dot(a, a)), lb = sqrtf(dot(b, b)), lc = sqrtf(dot(c, c));
+            const float y = a.x * b.y * c.z - a.x * b.z * c.y - a.y * b.x * c.z + a.y * b.z * c.x + a.z * b.x * c.y -
+                            a.z * b.y * c.x;
+            const float x = (la * lb * lc + (a.x * b.x + a.y * b.y + a.z * b.z) * lc +
+                             (b.x * c.x + b.y * c.y + b.z * c.z) * la + (c.x * a.x + c.y * a.y + c.z * a.z) * lb);
+            return k * atan2f(y, x);
+        }
+
+        // SdfBuilder::computeSdf's sign (sdf_builder.cpp:148-155):
+        // computeWindingNumberApprox with beta = 2 (winding_number.h:128-133,
+        // winding_number.cpp:149-171) -- exact terms at the leaves, a
+        // cluster's first-order term once the node is farther than two radii
+        // from its centroid, summed in the walk's order (positive child
+        // first, so the escape links replace the walk's stack) -- and the
+        // distance negated where the sum exceeds 0.5.  One thread per node.
+        __global__ __launch_bounds__(kSdfWG) void k_mesh_sdf_sign_tree(MeshSdfArgs a, RefTreeArgs t, float *dist)
+        {
+            const long long node = (long long)blockIdx.x * kSdfWG + threadIdx.x;
+            if (node >= a.count)
+                return;
+            const int x = (int)(node % a.dx), y = (int)((node / a.dx) % a.dy), z = (int)(node / ((long long)a.dx * a.dy));
+            const V3 q = node_point(a, x, y, z);
+            float w = 0.0f;
+            int i = 0;
+            while (i >= 0)
+            {
+                const unsigned data = tree_data(t, i);
+                const int next = __float_as_int(t.hi[i].w);
+                if (data & 1u)
+                {
+                    const int tri = (int)(data >> 5);
+                    w += winding_term_ref(v3(a.tri[4 * tri]), v3(a.tri[4 * tri + 1]), v3(a.tri[4 * tri + 2]), q);
+                    i = next;
+                    continue;
+                }
+                const float4 cc = t.cluC[i];
+                const V3 cen = v3(cc);
+                const V3 dq = q - cen;
+                const float d2 = dot(dq, dq);
+                const float th = 2.0f * cc.w;
+                if (d2 > th * th)
+                {
+                    // ((0.25 / pi) / (l l l)) n . dir: the double constant makes it a double expression
+                    const V3 dir = cen - q;
+                    const float l = sqrtf(dot(dir, dir));
+                    const float ll = l * l * l;
+                    w += (float)(((0.25 / 3.141592653589793238462643383) / (double)ll) * (double)dot(v3(t.cluN[i]), dir));
+                    i = next;
+                    continue;
+                }
+                i = (int)(data >> 1);
+            }
+            if (w > 0.5f)
+                dist[node] = -dist[node];
+        }
+
+        constexpr int kWalkStack = 64;
+        // passes of the slicer walk before its chain of warm starts must have
+        // settled (a node's walk depends on its predecessor's only where
+        // triangles tie, so chains are short)
+        constexpr int kWalkPasses = 64;
+
+        CR_DEV float box_dist2(const RefTreeArgs &t, int i, V3 q)
+        {
+            const V3 cpt = vmax(v3(t.lo[i]), vmin(v3(t.hi[i]), q));
+            const V3 d = cpt - q;
+            return dot(d, d);
+        }
+
+        // SdfBuilder::computeModSdf's walk (sdf_builder.cpp:187-217;
+        // ClosestDistanceProjectionToTrimeshTraversalController, aabb_tree.cpp:724-779)
+        // for node `node`, warm-started from triangle warm[node - 1] (the
+        // previous node's closest triangle in row order; none for node 0):
+        // the nearer child first, a subtree pruned when its box is no nearer
+        // than the best so far, a leaf accepted when its distance is <= the
+        // best.  The on-surface flag falls for good when an accepted
+        // candidate's closest point lies on a boundary edge or boundary vertex.
+        // Writes the walk's closest triangle, the distance |closest point - q|
+        // (0 when the walk accepts nothing past the warm start: the
+        // reference's side stays 0) and the flag with the spine override.
+        __global__ __launch_bounds__(kSdfWG) void k_mesh_slicer_walk(MeshSdfArgs a, RefTreeArgs t, const int *warm,
+                                                                     int *closest, float *dist, unsigned char *onMesh)
+        {
+            const long long node = (long long)blockIdx.x * kSdfWG + threadIdx.x;
+            if (node >= a.count)
+                return;
+            const int x = (int)(node % a.dx), y = (int)((node / a.dx) % a.dy), z = (int)(node / ((long long)a.dx * a.dy));
+            const V3 q = node_point(a, x, y, z);
+            float best = kFltMax;
+            int bestT = -1;
+            V3 bestP = v3(0.0f, 0.0f, 0.0f);
+            bool on = true, accepted = false;
+            const int w0 = node > 0 ? warm[node - 1] : -1;
+            if (w0 >= 0)
+            {
+                best = tri_closest_unit(q, v3(a.tri[4 * w0]), v3(a.tri[4 * w0 + 1]), v3(a.tri[4 * w0 + 2]), bestP);
+                bestT = w0;
+            }
+            int stack[kWalkStack];
+            int sp = 0, i = 0;
+            while (true)
+            {
+                const unsigned data = tree_data(t, i);
+                int go = 0; // 0: no deeper, 1: positive child first, 2: negative child first
+                if (box_dist2(t, i, q) < best)
+                {
+                    if (data & 1u)
+                    {
+                        const int tri = (int)(data >> 5);
+                        V3 cp;
+                        int region;
+                        const float d = tri_closest_unit(q, v3(a.tri[4 * tri]), v3(a.tri[4 * tri + 1]),
+                                                         v3(a.tri[4 * tri + 2]), cp, region);
+                        if (d <= best)
+                        {
+                            best = d;
+                            bestT = tri;
+                            bestP = cp;
+                            accepted = true;
+                            const int f = a.flags[tri];
+                            if (region != kRegFace && ((f >> (region < 3 ? region + 3 : region - 3)) & 1))
+                                on = false;
+                        }
+                    }
+                    else
+                    {
+                        const int pos = (int)(data >> 1);
+                        const float dp = box_dist2(t, pos, q), dn = box_dist2(t, pos + 1, q);
+                        go = dp < dn ? (dp < best ? 1 : 0) : (dn < best ? 2 : 0);
+                    }
+                }
+                if (go != 0)
+                {
+                    const int pos = (int)(data >> 1);
+                    stack[sp++] = go == 1 ? pos + 1 : pos; // depth checked on the host
+                    i = go == 1 ? pos : pos + 1;
+                    continue;
+                }
+                if (sp == 0)
+                    break;
+                i = stack[--sp];
+            }
+            closest[node] = bestT;
+            dist[node] = accepted ? norm(bestP - q) : 0.0f;
+            if (!isnan(a.spine.x))
+            {
+                V3 n = v3(a.spine.x, a.spine.y, a.spine.z);
+                const float s2 = dot(n, n);
+                n = s2 > 0.0f ? n * rsq(s2) : v3(0.0f, 0.0f, 0.0f);
+                if (dot(q, n) + a.spine.w > 0.0f)
+                    on = true;
+            }
+            onMesh[node] = on ? 1 : 0;
+        }
+
+        // *changed = 1 when some node's closest triangle differs from the warm
+        // start the walk gave the next node.
+        __global__ __launch_bounds__(kSdfWG) void k_mesh_slicer_changed(long long count, const int *warm,
+                                                                        const int *closest, int *changed)
+        {
+            const long long node = (long long)blockIdx.x * kSdfWG + threadIdx.x;
+            if (node + 1 < count && warm[node] != closest[node])
+                *changed = 1;
         }
 
         // computeSdfGradient (sdf_builder.cpp:242-330): central differences
@@ -605,18 +804,30 @@ int crHipMeshSdfDesc(const float *points, int numPoints, const unsigned int *tri
     // by 1e-4 (buildTree, include/geometry_utils/winding_number.h:148-161),
     // then grown by fatten; dims = int(size / cellSize + 1)
     // (SdfBuilder::precomputeSdfDesc, src/geometry_utils/sdf_builder.cpp:84-101).
+    // A triangle's box starts as the zero-initialised Bounds3 and grows to the
+    // corners, so the union always holds the origin.
     float lo[3] = {kFltMax, kFltMax, kFltMax}, hi[3] = {-kFltMax, -kFltMax, -kFltMax};
-    for (long long t = 0; t < 3ll * numTriangles; ++t)
+    for (long long t = 0; t < numTriangles; ++t)
     {
-        if (triangles[t] >= (unsigned)numPoints)
-            return CRMPM_HIP_ERR_ARG;
+        float bl[3] = {0.0f, 0.0f, 0.0f}, bh[3] = {0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < 3; ++j)
+        {
+            const unsigned i = triangles[3 * t + j];
+            if (i >= (unsigned)numPoints)
+                return CRMPM_HIP_ERR_ARG;
+            for (int k = 0; k < 3; ++k)
+            {
+                const float v = points[3ll * i + k];
+                if (!std::isfinite(v))
+                    return CRMPM_HIP_ERR_ARG;
+                bl[k] = std::fmin(bl[k], v);
+                bh[k] = std::fmax(bh[k], v);
+            }
+        }
         for (int k = 0; k < 3; ++k)
         {
-            const float v = points[3ll * triangles[t] + k];
-            if (!std::isfinite(v))
-                return CRMPM_HIP_ERR_ARG;
-            lo[k] = std::fmin(lo[k], v - 1e-4f);
-            hi[k] = std::fmax(hi[k], v + 1e-4f);
+            lo[k] = std::fmin(lo[k], bl[k] - 1e-4f);
+            hi[k] = std::fmax(hi[k], bh[k] + 1e-4f);
         }
     }
     long long count = 1;
@@ -636,16 +847,226 @@ int crHipMeshSdfDesc(const float *points, int numPoints, const unsigned int *tri
 
 namespace
 {
+    // The reference's AABB tree over the triangles and the cluster data of its
+    // winding-number approximation, as SdfBuilder::precomputeSdfDesc builds
+    // them (buildTree, include/geometry_utils/winding_number.h:148-161;
+    // buildAABBTree / flattenTree, src/geometry_utils/aabb_tree.cpp:97-333;
+    // precomputeClusterInformation, src/geometry_utils/winding_number.cpp:44-124),
+    // in the device layout of RefTreeArgs.  FP32 host arithmetic in the
+    // reference's order of operations, so the tree, its boxes and the
+    // clusters are the reference's.
+    struct RefTree
+    {
+        std::vector<float4> lo, hi, cluC, cluN;
+        int height = 0; // longest root-to-leaf path, in edges
+    };
+
+    struct H3
+    {
+        float x, y, z;
+        float operator[](int k) const { return k == 0 ? x : (k == 1 ? y : z); }
+    };
+    H3 operator+(H3 a, H3 b) { return H3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+    H3 operator-(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+    H3 operator*(H3 a, H3 b) { return H3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+    H3 operator*(H3 a, float s) { return H3{a.x * s, a.y * s, a.z * s}; }
+    H3 operator/(H3 a, float s) { return H3{a.x / s, a.y / s, a.z / s}; }
+    H3 hmin(H3 a, H3 b) { return H3{std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
+    H3 hmax(H3 a, H3 b) { return H3{std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
+    float hsqn(H3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+
+    RefTree build_ref_tree(const float *points, const unsigned *tris, int numTris)
+    {
+        auto P = [&](unsigned i) { return H3{points[3 * (size_t)i], points[3 * (size_t)i + 1], points[3 * (size_t)i + 2]}; };
+        // triangle boxes: the zero box grown to the corners, fattened by 1e-4; centres
+        std::vector<H3> blo(numTris), bhi(numTris), centre(numTris);
+        for (int t = 0; t < numTris; ++t)
+        {
+            H3 l{0.0f, 0.0f, 0.0f}, h{0.0f, 0.0f, 0.0f};
+            for (int j = 0; j < 3; ++j)
+            {
+                const H3 p = P(tris[3 * (size_t)t + j]);
+                l = hmin(l, p);
+                h = hmax(h, p);
+            }
+            blo[t] = H3{l.x - 1e-4f, l.y - 1e-4f, l.z - 1e-4f};
+            bhi[t] = H3{h.x + 1e-4f, h.y + 1e-4f, h.z + 1e-4f};
+            centre[t] = (bhi[t] + blo[t]) * 0.5f;
+        }
+        // depth-first build, one triangle per leaf: split at the box centre
+        // along the axis of largest centre variance, positive side first in
+        // the index list, 50-50 when one side is empty; children in pairs
+        struct BN
+        {
+            H3 lo, hi;
+            int pos = -1, depth = 0;
+            unsigned first = 0, count = 0;
+        };
+        std::vector<unsigned> idx(numTris);
+        for (int i = 0; i < numTris; ++i)
+            idx[i] = (unsigned)i;
+        std::vector<BN> pool;
+        pool.reserve(2 * (size_t)numTris);
+        pool.push_back(BN{});
+        pool[0].count = (unsigned)numTris;
+        std::vector<int> stack{0};
+        RefTree tree;
+        while (!stack.empty())
+        {
+            const int ni = stack.back();
+            stack.pop_back();
+            unsigned *prims = idx.data() + pool[ni].first;
+            const unsigned nb = pool[ni].count;
+            H3 lo = blo[prims[0]], hi = bhi[prims[0]], mean = centre[prims[0]];
+            for (unsigned i = 1; i < nb; ++i)
+            {
+                mean = mean + centre[prims[i]];
+                lo = hmin(lo, blo[prims[i]]);
+                hi = hmax(hi, bhi[prims[i]]);
+            }
+            pool[ni].lo = lo;
+            pool[ni].hi = hi;
+            tree.height = std::max(tree.height, pool[ni].depth);
+            if (nb <= 1)
+                continue;
+            mean = mean * (1.0f / float(nb));
+            H3 var{0.0f, 0.0f, 0.0f};
+            for (unsigned i = 0; i < nb; ++i)
+            {
+                H3 c = centre[prims[i]] - mean;
+                c = c * c;
+                var = var + c;
+            }
+            var = var * (1.0f / float(nb - 1));
+            const float ax = std::fabs(var.x), ay = std::fabs(var.y), az = std::fabs(var.z);
+            const int axis = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+            const float split = (lo[axis] + hi[axis]) * 0.5f;
+            unsigned nPos = 0;
+            for (unsigned i = 0; i < nb; ++i)
+            {
+                const unsigned p = prims[i];
+                if (centre[p][axis] > split)
+                {
+                    prims[i] = prims[nPos];
+                    prims[nPos] = p;
+                    ++nPos;
+                }
+            }
+            if (nPos == 0 || nPos == nb)
+                nPos = nb >> 1;
+            const int c0 = (int)pool.size();
+            pool.push_back(BN{});
+            pool.push_back(BN{});
+            pool[ni].pos = c0;
+            for (int k = 0; k < 2; ++k)
+                pool[c0 + k].depth = pool[ni].depth + 1;
+            pool[c0].first = pool[ni].first;
+            pool[c0].count = nPos;
+            pool[c0 + 1].first = pool[ni].first + nPos;
+            pool[c0 + 1].count = nb - nPos;
+            stack.push_back(c0 + 1);
+            stack.push_back(c0);
+        }
+        const size_t n = pool.size();
+        std::vector<unsigned> data(n);
+        std::vector<int> next(n, -1);
+        for (size_t i = 0; i < n; ++i)
+            data[i] = pool[i].pos < 0 ? (idx[pool[i].first] << 5) | ((pool[i].count & 15u) << 1) | 1u
+                                      : (unsigned)pool[i].pos << 1;
+        // escape links of the positive-child-first walk
+        for (size_t i = 0; i < n; ++i)
+            if (pool[i].pos >= 0)
+            {
+                next[pool[i].pos] = pool[i].pos + 1;
+                next[pool[i].pos + 1] = next[i];
+            }
+        // children come after their parent in the pool: the loop above sees a
+        // parent's link before setting its children's
+        tree.lo.resize(n);
+        tree.hi.resize(n);
+        for (size_t i = 0; i < n; ++i)
+        {
+            tree.lo[i] = make_float4(pool[i].lo.x, pool[i].lo.y, pool[i].lo.z, __builtin_bit_cast(float, data[i]));
+            tree.hi[i] = make_float4(pool[i].hi.x, pool[i].hi.y, pool[i].hi.z, __builtin_bit_cast(float, next[i]));
+        }
+        // clusters: per triangle area-weighted normal, area and centroid; each
+        // internal node's triangles in the order of a negative-child-first walk
+        std::vector<float> area(numTris);
+        std::vector<H3> nrmA(numTris), cen(numTris);
+        for (int t = 0; t < numTris; ++t)
+        {
+            const H3 a = P(tris[3 * (size_t)t]), b = P(tris[3 * (size_t)t + 1]), c = P(tris[3 * (size_t)t + 2]);
+            const H3 u = b - a, v = c - a;
+            nrmA[t] = H3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x} * 0.5f;
+            area[t] = std::sqrt(hsqn(nrmA[t]));
+            cen[t] = (a + b + c) * float(1.0 / 3.0);
+        }
+        tree.cluC.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        tree.cluN.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        std::vector<int> todo{0}, order;
+        std::vector<std::pair<int, int>> sections;
+        while (!todo.empty())
+        {
+            int id = todo.back();
+            todo.pop_back();
+            if (id >= 0)
+            {
+                if (data[id] & 1u)
+                {
+                    order.push_back((int)(data[id] >> 5));
+                    sections.push_back({(int)order.size() - 1, (int)order.size()});
+                    continue;
+                }
+                const int pos = (int)(data[id] >> 1);
+                todo.push_back(-id - 1);
+                todo.push_back(pos);
+                todo.push_back(pos + 1);
+                continue;
+            }
+            const std::pair<int, int> sa = sections.back();
+            sections.pop_back();
+            const std::pair<int, int> sb = sections.back();
+            sections.pop_back();
+            const std::pair<int, int> sum{sb.first, sa.second};
+            id = -id - 1;
+            H3 wc{0.0f, 0.0f, 0.0f}, wn{0.0f, 0.0f, 0.0f};
+            float as = 0.0f;
+            for (int i = sum.first; i < sum.second; ++i)
+            {
+                const int t = order[i];
+                as += area[t];
+                wc = wc + cen[t] * area[t];
+                wn = wn + nrmA[t];
+            }
+            wc = wc / as;
+            float r2 = 0.0f;
+            for (int i = sum.first; i < sum.second; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    const float d2 = hsqn(wc - P(tris[3 * (size_t)order[i] + j]));
+                    if (d2 > r2)
+                        r2 = d2;
+                }
+            tree.cluC[id] = make_float4(wc.x, wc.y, wc.z, std::sqrt(r2));
+            tree.cluN[id] = make_float4(wn.x, wn.y, wn.z, as);
+            sections.push_back(sum);
+        }
+        return tree;
+    }
+
     // Both tables: the triangle-mesh SDF (slicer false: signed distance,
     // rows at out) and the mesh-slicer table (spine row at out, then the
     // grid).  Synchronous on stream.
-    int build_mesh_table(bool slicer, const float *points, int numPoints, const unsigned *triangles,
+    int build_mesh_table(bool slicer, int rule, const float *points, int numPoints, const unsigned *triangles,
                          int numTriangles, float cellSize, float fatten, const float *spine, int dims[3],
                          float lowerBound[3], float *out, unsigned long long capacity, void *stream)
     {
+        if (rule != CRMPM_HIP_MESH_RULE_REFERENCE && rule != CRMPM_HIP_MESH_RULE_EXACT)
+            return CRMPM_HIP_ERR_ARG;
         int rc = crHipMeshSdfDesc(points, numPoints, triangles, numTriangles, cellSize, fatten, dims, lowerBound);
         if (rc || !out)
             return rc;
+        const bool refRule = rule == CRMPM_HIP_MESH_RULE_REFERENCE;
         // one node along an axis leaves the reference's one-sided difference
         // reading outside the table; such tables are refused
         if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2)
@@ -735,6 +1156,14 @@ namespace
         std::vector<int> flags;
         if (slicer)
             flags = boundary_flags(triangles, numTriangles);
+        RefTree tree;
+        if (refRule)
+        {
+            tree = build_ref_tree(points, triangles, numTriangles);
+            // the slicer walk keeps the deferred subtrees of one root-to-leaf path
+            if (slicer && tree.height > kWalkStack)
+                return CRMPM_HIP_ERR_CAPACITY;
+        }
         const float spineRow[4] = {slicer ? spine[0] : 0.0f, slicer ? spine[1] : 0.0f, slicer ? spine[2] : 0.0f,
                                    slicer ? spine[3] : 0.0f};
 
@@ -743,6 +1172,10 @@ namespace
         float *dWn = nullptr, *dErr = nullptr, *dDist = nullptr;
         int *dFlags = nullptr;
         unsigned char *dMask = nullptr; // per node: slicer on-surface flag, or SDF sign left undecided
+        float4 *dTree = nullptr;        // reference rule: node lo, node hi, cluster centroid, cluster normal
+        int *dWalk = nullptr;           // reference slicer: two closest-triangle arrays and a flag
+        const size_t tn = tree.lo.size();
+        int walkRc = CRMPM_HIP_OK;
         rc = CRMPM_HIP_ERR_ALLOC;
         if (hipMalloc(&dTri, tri.size() * sizeof(float4)) == hipSuccess &&
             hipMalloc(&dCen, cen.size() * sizeof(float4)) == hipSuccess &&
@@ -752,7 +1185,9 @@ namespace
             hipMalloc(&dWn, (size_t)usedSlices * count * sizeof(float)) == hipSuccess &&
             hipMalloc(&dDist, (size_t)count * sizeof(float)) == hipSuccess &&
             hipMalloc(&dMask, (size_t)count) == hipSuccess &&
-            (!slicer || hipMalloc(&dFlags, flags.size() * sizeof(int)) == hipSuccess))
+            (!slicer || hipMalloc(&dFlags, flags.size() * sizeof(int)) == hipSuccess) &&
+            (!refRule || hipMalloc(&dTree, 4 * tn * sizeof(float4)) == hipSuccess) &&
+            (!(refRule && slicer) || hipMalloc(&dWalk, (2 * (size_t)count + 1) * sizeof(int)) == hipSuccess))
         {
             rc = CRMPM_HIP_ERR_LAUNCH;
             a.tri = dTri;
@@ -774,12 +1209,56 @@ namespace
                        (hipMemcpyAsync(dFlags, flags.data(), flags.size() * sizeof(int), hipMemcpyHostToDevice, st) ==
                             hipSuccess &&
                         hipMemcpyAsync(out, spineRow, sizeof(spineRow), hipMemcpyHostToDevice, st) == hipSuccess));
+            RefTreeArgs ta{};
+            if (ok && refRule)
+            {
+                ta = RefTreeArgs{dTree, dTree + tn, dTree + 2 * tn, dTree + 3 * tn};
+                const std::vector<float4> *parts[4] = {&tree.lo, &tree.hi, &tree.cluC, &tree.cluN};
+                for (int k = 0; k < 4 && ok; ++k)
+                    ok = hipMemcpyAsync(dTree + k * tn, parts[k]->data(), tn * sizeof(float4), hipMemcpyHostToDevice,
+                                        st) == hipSuccess;
+            }
+            if (ok && refRule && slicer)
+            {
+                // the reference's warm-started walk, node after node: run it for
+                // all nodes from the previous pass's closest triangles (none at
+                // first) until every node's warm start is its predecessor's result
+                int *warm = dWalk, *closest = dWalk + count, *changed = dWalk + 2 * count;
+                ok = hipMemsetAsync(warm, 0xff, (size_t)count * sizeof(int), st) == hipSuccess;
+                int pass = 0;
+                for (; ok && pass < kWalkPasses; ++pass)
+                {
+                    int flag = 0;
+                    k_mesh_slicer_walk<<<nodeWGs, kSdfWG, 0, st>>>(a, ta, warm, closest, dDist, dMask);
+                    ok = hipMemsetAsync(changed, 0, sizeof(int), st) == hipSuccess;
+                    k_mesh_slicer_changed<<<nodeWGs, kSdfWG, 0, st>>>(count, warm, closest, changed);
+                    ok = ok && hipGetLastError() == hipSuccess &&
+                         hipMemcpyAsync(&flag, changed, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+                         hipStreamSynchronize(st) == hipSuccess;
+                    if (!ok || !flag)
+                        break;
+                    std::swap(warm, closest);
+                }
+                if (ok && pass == kWalkPasses)
+                    walkRc = CRMPM_HIP_ERR_STATE; // the chain of warm starts did not settle
+            }
             if (ok)
             {
-                if (slicer)
+                if (slicer && refRule)
+                {
+                    // distances and flags come from the walk
+                }
+                else if (slicer)
                 {
                     k_mesh_sdf_slices<true><<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
                     k_mesh_sdf_combine<true><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, dMask);
+                }
+                else if (refRule)
+                {
+                    // closest distances from the slices, sign from the reference's winding walk
+                    k_mesh_sdf_slices<true><<<a.groups * usedSlices, kSdfWG, 0, st>>>(a);
+                    k_mesh_sdf_combine<false, true><<<nodeWGs, kSdfWG, 0, st>>>(a, usedSlices, dDist, dMask);
+                    k_mesh_sdf_sign_tree<<<nodeWGs, kSdfWG, 0, st>>>(a, ta, dDist);
                 }
                 else
                 {
@@ -794,7 +1273,7 @@ namespace
             }
             // the copies read host memory: wait for them whatever failed
             if (hipStreamSynchronize(st) == hipSuccess && ok)
-                rc = CRMPM_HIP_OK;
+                rc = walkRc;
         }
         (void)hipFree(dTri);
         (void)hipFree(dCen);
@@ -805,6 +1284,8 @@ namespace
         (void)hipFree(dDist);
         (void)hipFree(dFlags);
         (void)hipFree(dMask);
+        (void)hipFree(dTree);
+        (void)hipFree(dWalk);
         return rc;
     }
 } // namespace
@@ -815,7 +1296,15 @@ int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangl
                  float fatten, int dims[3], float lowerBound[3], float *gradientSignedDistance,
                  unsigned long long capacity, void *stream)
 {
-    return build_mesh_table(false, points, numPoints, triangles, numTriangles, cellSize, fatten, nullptr, dims,
+    return build_mesh_table(false, CRMPM_HIP_MESH_RULE_REFERENCE, points, numPoints, triangles, numTriangles, cellSize,
+                            fatten, nullptr, dims, lowerBound, gradientSignedDistance, capacity, stream);
+}
+
+int crHipMeshSdfWithRule(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                         float cellSize, float fatten, int rule, int dims[3], float lowerBound[3],
+                         float *gradientSignedDistance, unsigned long long capacity, void *stream)
+{
+    return build_mesh_table(false, rule, points, numPoints, triangles, numTriangles, cellSize, fatten, nullptr, dims,
                             lowerBound, gradientSignedDistance, capacity, stream);
 }
 
@@ -825,7 +1314,17 @@ int crHipMeshSlicerSdf(const float *points, int numPoints, const unsigned int *t
 {
     if (!spineArea)
         return CRMPM_HIP_ERR_ARG;
-    return build_mesh_table(true, points, numPoints, triangles, numTriangles, cellSize, fatten, spineArea, dims,
+    return build_mesh_table(true, CRMPM_HIP_MESH_RULE_REFERENCE, points, numPoints, triangles, numTriangles, cellSize,
+                            fatten, spineArea, dims, lowerBound, rows, capacity, stream);
+}
+
+int crHipMeshSlicerSdfWithRule(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                               float cellSize, float fatten, const float spineArea[4], int rule, int dims[3],
+                               float lowerBound[3], float *rows, unsigned long long capacity, void *stream)
+{
+    if (!spineArea)
+        return CRMPM_HIP_ERR_ARG;
+    return build_mesh_table(true, rule, points, numPoints, triangles, numTriangles, cellSize, fatten, spineArea, dims,
                             lowerBound, rows, capacity, stream);
 }
 

@@ -62,6 +62,12 @@ EXPORTS = {
                            ctypes.POINTER(ctypes.c_float),
                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong,
                            _vp],
+    "crHipMeshSdfWithRule": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong,
+                             _vp],
+    "crHipMeshSlicerSdfWithRule": [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_float), _vp, ctypes.c_ulonglong, _vp],
     "crHipEngineCreate": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
     "crHipEngineDestroy": [_vp],
     "crHipEngineCreateScene": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _f3, _f3, ctypes.POINTER(ctypes.c_int)],
@@ -231,11 +237,15 @@ def load_obj(path):
     return np.asarray(pts, np.float32).reshape(-1, 3), np.asarray(tris, np.uint32).reshape(-1, 3)
 
 
-def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0"):
-    """SdfBuilder's mesh -> SDF table on the device (crHipMeshSdf).  points
-    (n, 3) float32 and triangles (m, 3) uint32 on the host.  Returns dims,
-    lower bound (numpy) and a (dx*dy*dz, 4) float32 device tensor of rows
-    (gradient, signed distance)."""
+MESH_RULES = {"reference": 0, "exact": 1}
+
+
+def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0", rule="reference"):
+    """SdfBuilder's mesh -> SDF table on the device (crHipMeshSdf, or
+    crHipMeshSdfWithRule for rule "exact": signs by the exact winding
+    number).  points (n, 3) float32 and triangles (m, 3) uint32 on the host.
+    Returns dims, lower bound (numpy) and a (dx*dy*dz, 4) float32 device
+    tensor of rows (gradient, signed distance)."""
     import torch
 
     lib = load_library()
@@ -250,14 +260,16 @@ def mesh_sdf(points, triangles, cell_size, fatten, device="cuda:0"):
     check(lib.crHipMeshSdfDesc(*args), "mesh sdf desc")
     n = dims[0] * dims[1] * dims[2]
     out = torch.empty((n, 4), dtype=torch.float32, device=dev)
-    check(lib.crHipMeshSdf(*args, _vp(out.data_ptr()), n, _vp(stream)), "mesh sdf")
+    check(lib.crHipMeshSdfWithRule(*args[:6], MESH_RULES[rule], *args[6:], _vp(out.data_ptr()), n, _vp(stream)),
+          "mesh sdf")
     return np.array(dims[:], np.int32), np.array(lower[:], np.float32), out
 
 
-def mesh_slicer_sdf(points, triangles, cell_size, fatten, spine, device="cuda:0"):
-    """The mesh-slicer table on the device (crHipMeshSlicerSdf): dims, lower
-    bound and a (dx*dy*dz + 1, 4) float32 device tensor, the spine area in
-    row 0."""
+def mesh_slicer_sdf(points, triangles, cell_size, fatten, spine, device="cuda:0", rule="reference"):
+    """The mesh-slicer table on the device (crHipMeshSlicerSdf, or
+    crHipMeshSlicerSdfWithRule for rule "exact": rows dropped by the final
+    closest point only): dims, lower bound and a (dx*dy*dz + 1, 4) float32
+    device tensor, the spine area in row 0."""
     import torch
 
     lib = load_library()
@@ -271,9 +283,9 @@ def mesh_slicer_sdf(points, triangles, cell_size, fatten, spine, device="cuda:0"
                                float(fatten), dims, lower), "mesh sdf desc")
     n = dims[0] * dims[1] * dims[2] + 1
     out = torch.empty((n, 4), dtype=torch.float32, device=dev)
-    check(lib.crHipMeshSlicerSdf(_vp(pts.ctypes.data), len(pts), _vp(tris.ctypes.data), len(tris), float(cell_size),
-                                 float(fatten), _f4(*spine), dims, lower, _vp(out.data_ptr()), n, _vp(stream)),
-          "mesh slicer sdf")
+    check(lib.crHipMeshSlicerSdfWithRule(_vp(pts.ctypes.data), len(pts), _vp(tris.ctypes.data), len(tris),
+                                         float(cell_size), float(fatten), _f4(*spine), MESH_RULES[rule], dims, lower,
+                                         _vp(out.data_ptr()), n, _vp(stream)), "mesh slicer sdf")
     return np.array(dims[:], np.int32), np.array(lower[:], np.float32), out
 
 

@@ -381,24 +381,38 @@ int crHipEngineGetSceneStateArrays(CrHipEngine *e, int scene, float **F, float *
  * crHipMeshSdfDesc replaces SdfBuilder::precomputeSdfDesc
  * (src/geometry_utils/sdf_builder.cpp:84-101): from host arrays points
  * (float xyz per vertex) and triangles (3 vertex indices each), the table's
- * dims (int(size / cellSize + 1) per axis) and lower bound (the triangles'
- * bounding box grown by 1e-4 and by fatten).
+ * dims (int(size / cellSize + 1) per axis) and lower bound (the union of
+ * the triangles' boxes -- each the reference's zero-initialised box grown to
+ * its corners, so holding the origin -- grown by 1e-4 and by fatten).
  *
  * crHipMeshSdf replaces SdfBuilder::computeSdf + computeSdfGradient
  * (sdf_builder.cpp:103-160, 242-330): writes dims and lowerBound as above
  * and, when gradientSignedDistance (device float4[count], the geometry's rows
  * of CrHipGeometrySdfData::gradientSignedDistance) is not NULL, fills row
  * z*dx*dy + y*dx + x with the normalised gradient (xyz) and signed distance
- * (w) at lowerBound + (x, y, z) * cellSize; negative inside (winding number
- * above 0.5).  Synchronous on stream.  Errors: CRMPM_HIP_ERR_ARG for a missing
- * pointer, no triangles, cellSize <= 0, fatten < 0, a vertex index beyond
- * numPoints, a non-finite vertex, or a table with one node along an axis;
- * CRMPM_HIP_ERR_CAPACITY for capacity < dx*dy*dz rows or 2^31 rows or more. */
+ * (w) at lowerBound + (x, y, z) * cellSize; negative inside: where the
+ * reference's approximate winding number (its AABB tree's clusters, beta = 2,
+ * src/geometry_utils/winding_number.cpp:136-171) exceeds 0.5.  Synchronous on
+ * stream.  Errors: CRMPM_HIP_ERR_ARG for a missing pointer, no triangles,
+ * cellSize <= 0, fatten < 0, a vertex index beyond numPoints, a non-finite
+ * vertex, or a table with one node along an axis; CRMPM_HIP_ERR_CAPACITY for
+ * capacity < dx*dy*dz rows or 2^31 rows or more.
+ *
+ * crHipMeshSdfWithRule: the same with rule CRMPM_HIP_MESH_RULE_REFERENCE
+ * (crHipMeshSdf) or CRMPM_HIP_MESH_RULE_EXACT: the sign from the exact
+ * winding number (computeWindingNumberExact, winding_number.h:136-146), which
+ * differs from the reference's approximation near 0.5 (nodes on a closed
+ * surface, near the rim of an open one).  No counterpart in the reference. */
+#define CRMPM_HIP_MESH_RULE_REFERENCE 0
+#define CRMPM_HIP_MESH_RULE_EXACT 1
 int crHipMeshSdfDesc(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
                      float cellSize, float fatten, int dims[3], float lowerBound[3]);
 int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles, float cellSize,
                  float fatten, int dims[3], float lowerBound[3], float *gradientSignedDistance,
                  unsigned long long capacity, void *stream);
+int crHipMeshSdfWithRule(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                         float cellSize, float fatten, int rule, int dims[3], float lowerBound[3],
+                         float *gradientSignedDistance, unsigned long long capacity, void *stream);
 
 /* The mesh-slicer table the engine builds for a triangle-mesh slicer geometry
  * (src/engine/simulation_factory_impl.cpp:653-690): SdfBuilder::computeModSdf
@@ -406,14 +420,26 @@ int crHipMeshSdf(const float *points, int numPoints, const unsigned int *triangl
  * 242-357).  rows (device float4[dx*dy*dz + 1]) receives the spine area
  * (normal xyz, offset w; x NaN for none) in row 0, then per node the unsigned
  * distance (w) and its normalised gradient (xyz), all four NaN where the
+ * reference's warm-started tree walk (each node starting from the previous
+ * node's closest triangle, aabb_tree.cpp:724-779) accepted a candidate whose
  * closest point lies on a boundary edge or boundary vertex of the mesh (a
- * boundary edge has one adjacent triangle) unless the node is on the positive
- * side of the spine plane.  The reference also drops nodes whose tree walk
- * passed a boundary candidate before the closest one; the rows kept here are
- * a superset of its.  Errors as crHipMeshSdf; capacity counts the spine row. */
+ * boundary edge has one adjacent triangle), unless the node is on the
+ * positive side of the spine plane; a node whose walk accepts nothing past
+ * its warm start gets distance 0, as in the reference.  Errors as
+ * crHipMeshSdf; capacity counts the spine row; CRMPM_HIP_ERR_CAPACITY also
+ * for a tree deeper than 64 levels; CRMPM_HIP_ERR_STATE if the chain of warm
+ * starts has not settled after 16 passes.
+ *
+ * crHipMeshSlicerSdfWithRule with CRMPM_HIP_MESH_RULE_EXACT drops a row only
+ * where the final closest point (the first closest triangle in index order)
+ * lies on the boundary: a superset of the reference's rows.  No counterpart
+ * in the reference. */
 int crHipMeshSlicerSdf(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
                        float cellSize, float fatten, const float spineArea[4], int dims[3], float lowerBound[3],
                        float *rows, unsigned long long capacity, void *stream);
+int crHipMeshSlicerSdfWithRule(const float *points, int numPoints, const unsigned int *triangles, int numTriangles,
+                               float cellSize, float fatten, const float spineArea[4], int rule, int dims[3],
+                               float lowerBound[3], float *rows, unsigned long long capacity, void *stream);
 
 /* ------------------------------------------------------------------------
  * Particle lattice of a particle object on the device:

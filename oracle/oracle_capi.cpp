@@ -7,6 +7,7 @@
 // ShapeData/GeometryData/GeometrySdfData tables (include/engine/shape.h:58-89,
 // include/engine/geometry.h:65-118), as flat host arrays.
 #include "mpm_oracle.hpp"
+#include "mesh_sdf_ref.hpp"
 
 #include <algorithm>
 
@@ -281,7 +282,22 @@ extern "C"
         for (int t = 0; t < 3 * numTris; ++t)
             if (tris[t] >= static_cast<unsigned>(numPoints))
                 return -1;
-        const int n = orc::mesh_sdf_desc(pts, tris, numTris, cell, fatten, dims3, lower3);
+        const int n = orc::mesh_sdf_desc_ref(pts, tris, numTris, cell, fatten, dims3, lower3);
+        if (grid4 && n <= capacity)
+            orc::mesh_sdf_ref(pts, tris, numTris, cell, dims3, lower3, grid4);
+        return n;
+    }
+
+    // The same table with the exact winding number for the sign and the
+    // closest triangle in index order for the distance (the builders'
+    // optional exact rule).
+    int orc_mesh_sdf_exact(const float *pts, int numPoints, const unsigned *tris, int numTris, float cell,
+                           float fatten, int *dims3, float *lower3, float *grid4, int capacity)
+    {
+        for (int t = 0; t < 3 * numTris; ++t)
+            if (tris[t] >= static_cast<unsigned>(numPoints))
+                return -1;
+        const int n = orc::mesh_sdf_desc_ref(pts, tris, numTris, cell, fatten, dims3, lower3);
         if (grid4 && n <= capacity)
             orc::mesh_sdf(pts, tris, numTris, cell, dims3, lower3, grid4);
         return n;
@@ -297,7 +313,23 @@ extern "C"
         for (int t = 0; t < 3 * numTris; ++t)
             if (tris[t] >= static_cast<unsigned>(numPoints))
                 return -1;
-        const int n = orc::mesh_sdf_desc(pts, tris, numTris, cell, fatten, dims3, lower3);
+        const int n = orc::mesh_sdf_desc_ref(pts, tris, numTris, cell, fatten, dims3, lower3);
+        if (rows4 && n + 1 <= capacity)
+            orc::mesh_slicer_sdf_ref(pts, tris, numTris, cell, dims3, lower3, spine4, rows4);
+        return n;
+    }
+
+    // The mesh-slicer table with the exact rule: a node is off the surface
+    // when its closest triangle (first in index order) puts the closest point
+    // on a boundary edge or vertex.
+    int orc_mesh_slicer_sdf_exact(const float *pts, int numPoints, const unsigned *tris, int numTris, float cell,
+                                  float fatten, const float *spine4, int *dims3, float *lower3, float *rows4,
+                                  int capacity)
+    {
+        for (int t = 0; t < 3 * numTris; ++t)
+            if (tris[t] >= static_cast<unsigned>(numPoints))
+                return -1;
+        const int n = orc::mesh_sdf_desc_ref(pts, tris, numTris, cell, fatten, dims3, lower3);
         if (rows4 && n + 1 <= capacity)
             orc::mesh_slicer_sdf(pts, tris, numTris, cell, dims3, lower3, spine4, rows4);
         return n;

@@ -56,6 +56,10 @@ def _declare(lib, pre):
                                     ctypes.c_int]),
         "mesh_slicer_sdf": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f,
                                            _i, _f, _f, ctypes.c_int]),
+        "mesh_sdf_exact": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i, _f,
+                                          _f, ctypes.c_int]),
+        "mesh_slicer_sdf_exact": (ctypes.c_int, [_f, ctypes.c_int, _u, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                                 _f, _i, _f, _f, ctypes.c_int]),
         "stress": (None, [ctypes.c_int, _f, _i, _f, _d]),
         "pb_constraint": (None, [ctypes.c_int, ctypes.c_float, _f, _i, _f, _f, _d]),
     }
@@ -237,33 +241,37 @@ def particle_lattice(obj, cap=None):
     return n, out[:m]
 
 
-def mesh_sdf(points, tris, cell, fatten, which="oracle"):
+def mesh_sdf(points, tris, cell, fatten, which="oracle", rule="reference"):
     """Triangle mesh -> SDF table through the oracle or the reference's own
     host builder (oracle/_ref): returns dims, lower bound and the (n, 4)
-    float32 rows (gradient, signed distance)."""
+    float32 rows (gradient, signed distance).  The oracle's rule "exact" signs
+    by the exact winding number instead of the reference's approximation."""
     fn = load(which)
     if fn is None:
         raise RuntimeError("reference CPU build not present")
+    name = "mesh_sdf" if rule == "reference" else "mesh_sdf_exact"
     pts = np.ascontiguousarray(points, np.float32)
     tr = np.ascontiguousarray(tris, np.uint32)
     dims = np.zeros(3, np.int32)
     lower = np.zeros(3, np.float32)
     args = (P(pts, _f), len(pts), P(tr, _u), len(tr), float(cell), float(fatten), P(dims, _i), P(lower, _f))
-    n = fn["mesh_sdf"](*args, None, 0)
+    n = fn[name](*args, None, 0)
     if n < 0:
         raise ValueError("triangle vertex index out of range")
     grid = np.zeros((n, 4), np.float32)
-    fn["mesh_sdf"](*args, P(grid, _f), n)
+    fn[name](*args, P(grid, _f), n)
     return dims, lower, grid
 
 
-def mesh_slicer_sdf(points, tris, cell, fatten, spine, which="oracle"):
+def mesh_slicer_sdf(points, tris, cell, fatten, spine, which="oracle", rule="reference"):
     """Triangle mesh -> mesh-slicer table through the oracle or the
     reference's host builder: dims, lower bound and (n + 1, 4) float32 rows
-    (spine area, then gradient|unsigned distance, NaN off the surface)."""
+    (spine area, then gradient|unsigned distance, NaN off the surface).  The
+    oracle's rule "exact" drops a node only by its final closest point."""
     fn = load(which)
     if fn is None:
         raise RuntimeError("reference CPU build not present")
+    name = "mesh_slicer_sdf" if rule == "reference" else "mesh_slicer_sdf_exact"
     pts = np.ascontiguousarray(points, np.float32)
     tr = np.ascontiguousarray(tris, np.uint32)
     sp = np.ascontiguousarray(spine, np.float32)
@@ -271,11 +279,11 @@ def mesh_slicer_sdf(points, tris, cell, fatten, spine, which="oracle"):
     lower = np.zeros(3, np.float32)
     args = (P(pts, _f), len(pts), P(tr, _u), len(tr), float(cell), float(fatten), P(sp, _f), P(dims, _i),
             P(lower, _f))
-    n = fn["mesh_slicer_sdf"](*args, None, 0)
+    n = fn[name](*args, None, 0)
     if n < 0:
         raise ValueError("triangle vertex index out of range")
     rows = np.zeros((n + 1, 4), np.float32)
-    fn["mesh_slicer_sdf"](*args, P(rows, _f), n + 1)
+    fn[name](*args, P(rows, _f), n + 1)
     return dims, lower, rows
 
 

@@ -68,6 +68,14 @@ def sdf_cases():
     pts, tris = S.icosphere(0.1, 2)
     tris = np.vstack([tris, tris[:5], [[0, 0, 1]], [[3, 3, 3]]]).astype(np.uint32)
     yield "sdf_degenerate", pts, tris, 0.02, 0.03
+    # a mesh whose box excludes the origin: the reference's triangle boxes
+    # start from a zero box, so its table still spans the origin
+    yield "sdf_icosphere_offorigin", *_off_origin_sphere(), 0.02, 0.01
+
+
+def _off_origin_sphere():
+    pts, tris = S.icosphere(0.05, 2)
+    return (pts + np.array([0.25, 0.1, -0.2], np.float32)).astype(np.float32), tris
 
 
 def slicer_cases():
@@ -82,6 +90,9 @@ def slicer_cases():
         (float("nan"), 0.0, 0.0, 0.0)
     yield "slicer_curved_plane_obj", *load_obj(os.path.join(REF_ASSETS, "curved_plane.obj")), 0.1, 0.1, \
         (0.0, 0.0, 1.0, -0.5)
+    # an open cap of the off-origin sphere (its first 150 triangles) with a spine
+    pts, tris = _off_origin_sphere()
+    yield "slicer_sphere_cap_offorigin", pts, tris[:150], 0.02, 0.01, (0.0, 0.0, 1.0, -0.2)
 
 
 def input_digest(sc):

@@ -1,17 +1,24 @@
 """Mesh -> SDF tables (SdfBuilder, src/geometry_utils/sdf_builder.cpp), CPU only:
 the triangle-mesh SDF and the mesh-slicer table.
 
-The oracle's restatement (oracle/mpm_oracle.hpp, mesh_sdf) is pinned
-1. against the reference's own host builder, compiled from its sources
-   (oracle/_ref) and committed as fixtures (tests/golden/sdf_*.npz), on the
-   reference's test meshes (assets/sphere.obj at its test's cell size, the
-   open assets/hemisphere.obj, the winding-number test cube) and generated
-   ones (subdivided icosahedron, torus);
+The oracle's restatement of the reference's builder (oracle/mesh_sdf_ref.hpp:
+its AABB tree, the beta = 2 winding approximation for the sign, the
+warm-started closest-point walks, the slicer's on-surface flag) is pinned
+1. bit for bit against the reference's own host builder, compiled from its
+   sources (oracle/_ref) and committed as fixtures (tests/golden/sdf_*.npz,
+   slicer_*.npz), on the reference's test meshes (assets/sphere.obj at its
+   test's cell size, the open assets/hemisphere.obj, the winding-number test
+   cube, assets/plane.obj and assets/curved_plane.obj as slicers) and
+   generated ones (subdivided icosahedron, torus, degenerate triangles, a
+   sphere away from the origin whose table still spans the origin);
 2. against closed forms: the signed distance of a box for the cube mesh and a
    brute-force sampled distance to a single triangle.
-A mesh with its winding flipped (a planted mistake in the oracle's input)
-flips every sign and is caught by (1).  The library's host-side entry points
-(table size, argument errors) are checked here too, without a GPU.
+The oracle's exact rule (exact winding number; a slicer row dropped only by
+its final closest point) is checked against the same fixtures up to the
+documented differences.  A mesh with its winding flipped (a planted mistake in
+the oracle's input) flips every sign and is caught by (1).  The library's
+host-side entry points (table size, argument errors) are checked here too,
+without a GPU.
 """
 import ctypes
 import os
@@ -23,13 +30,19 @@ import cpu_checkers as CC
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = ["sdf_sphere_obj", "sdf_sphere_obj_fine", "sdf_hemisphere_obj", "sdf_cube", "sdf_icosphere", "sdf_torus",
-            "sdf_curved_plane_obj", "sdf_degenerate"]
-# The reference signs with a far-field approximation of the winding number
-# (beta = 2) and the oracle with the exact sum: on an open mesh they may
-# disagree where the winding number is near 0.5.  Measured: 3 of 6,877 nodes
-# past the hemisphere's rim; 58 of 4,761 around the curved sheet, whose
-# winding number crosses 0.5 away from the surface on its concave side.
+            "sdf_curved_plane_obj", "sdf_degenerate", "sdf_icosphere_offorigin"]
+# The exact rule signs with the exact winding number where the reference
+# thresholds its far-field approximation (beta = 2): on an open mesh they
+# may disagree where the winding number is near 0.5.  Measured: 3 of 6,877
+# nodes past the hemisphere's rim; 58 of 4,761 around the curved sheet,
+# whose winding number crosses 0.5 away from the surface on its concave side.
 OPEN_SIGN_MISMATCHES = {"sdf_hemisphere_obj": 3, "sdf_curved_plane_obj": 58}
+
+
+def bit_equal(a, b):
+    """Same float32 bits, NaN rows included."""
+    return a.shape == b.shape and np.array_equal(np.asarray(a, np.float32).view(np.uint32),
+                                                 np.asarray(b, np.float32).view(np.uint32))
 
 
 def fixture(name):
@@ -78,6 +91,14 @@ needs_ref = pytest.mark.skipif(not CC.ref_available(), reason="reference CPU bui
 def test_oracle_matches_reference_fixture(name):
     g = fixture(name)
     dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]))
+    assert np.array_equal(dims, g["dims"]) and np.array_equal(lower, g["lower"])
+    assert bit_equal(grid, g["grid"]), name
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_exact_rule_oracle_against_reference_fixture(name):
+    g = fixture(name)
+    dims, lower, grid = CC.mesh_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]), rule="exact")
     dd = compare_tables(name, g, dims, lower, grid)
     # the same arithmetic: most distances are bit-identical
     assert np.mean(dd == 0) > 0.95, np.mean(dd == 0)
@@ -147,12 +168,13 @@ def test_single_triangle_against_sampling():
 
 # ---------------- mesh-slicer tables (computeModSdf) ----------------
 
-SLICER_FIXTURES = ["slicer_plane_obj", "slicer_hemisphere_obj", "slicer_curved_plane_obj"]
-# Rows the reference drops although their closest point is on the surface:
-# its tree walk marks a node off the surface when any candidate accepted on
-# the way lies on the boundary (src/geometry_utils/aabb_tree.cpp:724-760).
-# Measured on the fixtures: 0, 812 and 44 nodes; bounded here per fixture.
-SLICER_EXTRA_ROWS = {"slicer_plane_obj": 0, "slicer_hemisphere_obj": 812, "slicer_curved_plane_obj": 44}
+SLICER_FIXTURES = ["slicer_plane_obj", "slicer_hemisphere_obj", "slicer_curved_plane_obj", "slicer_sphere_cap_offorigin"]
+# Rows the reference drops although their closest point is on the surface
+# (its tree walk marks a node off the surface when any candidate accepted on
+# the way lies on the boundary, src/geometry_utils/aabb_tree.cpp:724-760) and
+# the exact rule keeps.  Measured on the fixtures: 0, 812, 44 and 63 nodes.
+SLICER_EXTRA_ROWS = {"slicer_plane_obj": 0, "slicer_hemisphere_obj": 812, "slicer_curved_plane_obj": 44,
+                     "slicer_sphere_cap_offorigin": 63}
 
 
 def compare_slicer(name, g, dims, lower, rows):
@@ -176,6 +198,15 @@ def compare_slicer(name, g, dims, lower, rows):
 def test_slicer_oracle_matches_reference_fixture(name):
     g = fixture(name)
     dims, lower, rows = CC.mesh_slicer_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]), g["spine"])
+    assert np.array_equal(dims, g["dims"]) and np.array_equal(lower, g["lower"])
+    assert bit_equal(rows, g["rows"]), name
+
+
+@pytest.mark.parametrize("name", SLICER_FIXTURES)
+def test_exact_rule_slicer_oracle_against_reference_fixture(name):
+    g = fixture(name)
+    dims, lower, rows = CC.mesh_slicer_sdf(g["points"], g["tris"], float(g["cell"]), float(g["fatten"]), g["spine"],
+                                           rule="exact")
     compare_slicer(name, g, dims, lower, rows)
 
 
