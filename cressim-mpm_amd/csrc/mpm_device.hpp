@@ -542,6 +542,29 @@ namespace crhip
         int effectiveMass;
     };
 
+    // Shape records, slicer lists, SDF table headers and polyline points are
+    // written by the host or by an earlier launch, never by the kernel that
+    // reads them: they are read through the constant address space, so a
+    // wavefront-uniform read is a scalar load, which neither occupies the
+    // vector memory pipe nor joins its in-order completion counter (a
+    // particle loop's loads would otherwise wait behind the previous chunk's
+    // stores for every shape field it reads).  (The attribute only exists in
+    // the device compilation pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+    template <class T>
+    using ConstMem = const __attribute__((address_space(4))) T;
+#else
+    template <class T>
+    using ConstMem = const T;
+#endif
+    template <class T>
+    CR_DEV T const_load(const T *p, int i)
+    {
+        return ((ConstMem<T> *)(size_t)p)[i];
+    }
+    CR_DEV int shape_count(const ShapeView &sv) { return const_load(sv.counts, 0); }
+    CR_DEV int slicer_count(const ShapeView &sv) { return const_load(sv.counts, 1); }
+
     CR_DEV int f2i(float f) { return __float_as_int(f); }
 
     CR_DEV float sdf_trilinear(const ShapeView &sv, int4 dim, int offset, int bx, int by, int bz, V3 fr)
@@ -635,8 +658,8 @@ namespace crhip
         case gTriangleMesh:
         {
             const int off = f2i(p0.x), id = f2i(p0.y);
-            const float4 lb = sv.sdfLbcs[id];
-            const int4 dim = sv.sdfDims[id];
+            const float4 lb = const_load(sv.sdfLbcs, id);
+            const int4 dim = const_load(sv.sdfDims, id);
             const V3 sp = (lp - v3(lb)) / lb.w;
             const int mx = (int)(sp.x + 0.5f), my = (int)(sp.y + 0.5f), mz = (int)(sp.z + 0.5f);
             if (mx < 0 || mx >= dim.x || my < 0 || my >= dim.y || mz < 0 || mz >= dim.z)
@@ -677,8 +700,8 @@ namespace crhip
             g = V3{0, 1, 0};
             d = __int_as_float(0x7fc00000);
             const int off = f2i(p0.x) + 1, id = f2i(p0.y);
-            const float4 lb = sv.sdfLbcs[id];
-            const int4 dim = sv.sdfDims[id];
+            const float4 lb = const_load(sv.sdfLbcs, id);
+            const int4 dim = const_load(sv.sdfDims, id);
             const V3 sp = (lp - v3(lb)) / lb.w;
             const int mx = (int)(sp.x + 0.5f), my = (int)(sp.y + 0.5f), mz = (int)(sp.z + 0.5f);
             if (mx < 1 || mx >= dim.x - 1 || my < 1 || my >= dim.y - 1 || mz < 1 || mz >= dim.z - 1)
@@ -696,7 +719,7 @@ namespace crhip
             d = fabsf(iv);
             if (qv.w * iv < 0.0f)
                 g = V3{-g.x, -g.y, -g.z};
-            const float4 spn = sv.sdfData[off - 1];
+            const float4 spn = const_load(sv.sdfData, off - 1);
             if (isnan(spn.x))
                 break;
             const V3 sn = v3(spn);
@@ -714,8 +737,8 @@ namespace crhip
             g = V3{0, 1, 0};
             d = __int_as_float(0x7fc00000);
             const int off = f2i(p0.x), id = f2i(p0.y), size = f2i(p0.z);
-            const V3 bmin = v3(sv.sdfLbcs[id]);
-            const V3 bmax = v3(sv.sdfData[off]);
+            const V3 bmin = v3(const_load(sv.sdfLbcs, id));
+            const V3 bmax = v3(const_load(sv.sdfData, off));
             if (lp.x < bmin.x || lp.x > bmax.x || lp.y < bmin.y || lp.y > bmax.y || lp.z < bmin.z || lp.z > bmax.z)
                 break;
             int closestIdx = 0;
@@ -724,8 +747,8 @@ namespace crhip
             V3 pa{0, 0, 0}, pb{0, 0, 0};
             for (int p = off + 1; p < off + size - 1; ++p)
             {
-                pa = v3(sv.sdfData[p]);
-                pb = v3(sv.sdfData[p + 1]);
+                pa = v3(const_load(sv.sdfData, p));
+                pb = v3(const_load(sv.sdfData, p + 1));
                 const V3 lv = pb - pa;
                 const V3 pv = lp - pa;
                 float t = dot(pv, lv) / sqn(lv);
@@ -739,7 +762,7 @@ namespace crhip
                     closestIdx = p;
                 }
             }
-            pa = v3(sv.sdfData[off + 1]);
+            pa = v3(const_load(sv.sdfData, off + 1));
             if (sqn(closest - pa) < kFltEps || sqn(closest - pb) < kFltEps)
                 break;
             V3 gg = lp - closest;
@@ -772,7 +795,7 @@ namespace crhip
             g = V3{0, 1, 0};
             d = __int_as_float(0x7fc00000);
             int off = f2i(p0.x);
-            const float4 spn = sv.sdfData[off];
+            const float4 spn = const_load(sv.sdfData, off);
             if (isnan(spn.x))
                 return false;
             const float side = dot(lp, v3(spn)) + spn.w;
@@ -780,8 +803,8 @@ namespace crhip
                 return false;
             off++;
             const int id = f2i(p0.y);
-            const float4 lb = sv.sdfLbcs[id];
-            const int4 dim = sv.sdfDims[id];
+            const float4 lb = const_load(sv.sdfLbcs, id);
+            const int4 dim = const_load(sv.sdfDims, id);
             const V3 sp = (lp - v3(lb)) / lb.w;
             const int mx = (int)(sp.x + 0.5f), my = (int)(sp.y + 0.5f), mz = (int)(sp.z + 0.5f);
             if (mx < 1 || mx >= dim.x - 1 || my < 1 || my >= dim.y - 1 || mz < 1 || mz >= dim.z - 1)
@@ -811,7 +834,7 @@ namespace crhip
     {
         for (int k = 0; k < nSlicers; ++k)
         {
-            const ShapeRec &r = sv.recs[sv.slicers[k]];
+            const ShapeRec r = const_load(sv.recs, const_load(sv.slicers, k));
             const V3 is = v3(r.invScale);
             const V3 sp = v3(r.pos);
             const V3 r1 = qrot_inv(r.rot, p1 - sp) * is;
@@ -845,7 +868,7 @@ namespace crhip
         const float reach = 2.5980762f * dx; // 1.5 * sqrt(3) * dx
         for (int k = 0; k < nSlicers; ++k)
         {
-            const ShapeRec &r = sv.recs[sv.slicers[k]];
+            const ShapeRec r = const_load(sv.recs, const_load(sv.slicers, k));
             if (r.geomType != gQuadSlicer)
                 return false;
             const V3 is = v3(r.invScale);
@@ -896,7 +919,7 @@ namespace crhip
     {
         for (int si = 0; si < nShapes; ++si)
         {
-            const ShapeRec &r = sv.recs[si];
+            const ShapeRec r = const_load(sv.recs, si);
             const int gtype = r.geomType;
             if (IsParticle && (gtype == gLines || gtype == gArc))
                 continue;

@@ -785,6 +785,13 @@ namespace
     // reproducible.
     // ------------------------------------------------------------------
     CR_DEV void compiler_fence() { asm volatile("" ::: "memory"); }
+    // Makes the wavefront wait for the load of v here.  A particle loop that
+    // prefetches the next chunk's index settles the first one before the
+    // loop: otherwise the wait at the loop header, which serves the entry and
+    // the back edge alike, is sized for the entry (nothing issued after the
+    // load) and waits for every outstanding access, the previous chunk's
+    // stores included, on each iteration.
+    CR_DEV void settle(int v) { asm volatile("" ::"v"(v)); }
 
     // Makes values opaque to the optimiser at this point (no instruction), so
     // the per-node work of a scatter pass is not hoisted out of the pass loop.
@@ -1246,15 +1253,16 @@ namespace
             float4 *A0 = acc[wave * kScatterTiles];
             zero_tiles<kScatterTiles>(A0, lane);
             wave_sync();
-            const int nSlicers = c.sv.counts[1];
+            const int nSlicers = slicer_count(c.sv);
             const GridGeom &g = c.g;
             // the next chunk's particle index is loaded one chunk ahead, and all
             // of a particle's state is loaded before the velocity store
-            int pNext = (ti.start + lane < ti.end) ? (int)c.order[ti.start + lane] : 0;
+            int pNext = (int)c.order[min(ti.start + lane, ti.end - 1)];
+            settle(pNext);
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
                 const int p = pNext;
-                pNext = (k + 64 < ti.end) ? (int)c.order[k + 64] : 0;
+                pNext = (int)c.order[min(k + 64, ti.end - 1)];
                 const float4 xm = c.P.xm[p];
                 const float4 vv = c.P.v[p];
                 const M3 F = load_cols(c.S.F, p);
@@ -1476,7 +1484,7 @@ namespace
                 np = np + v3(s) * c.dt;
             }
             // all lanes take part (coupling momentum is summed per wavefront); light nodes are left as they are
-            resolve_rigid<false>(c.sv, c.sv.counts[0], s, np, heavy);
+            resolve_rigid<false>(c.sv, shape_count(c.sv), s, np, heavy);
             if (heavy)
             {
                 if (i < 2 || i > g.nx - 3)
@@ -1551,15 +1559,16 @@ namespace
             float4 *G = lg[wave];
             load_tile_grid(c.g, ti, c.grid, G, lane);
             wave_sync();
-            const int nSlicers = c.sv.counts[1];
-            const int nShapes = c.sv.counts[0];
+            const int nSlicers = slicer_count(c.sv);
+            const int nShapes = shape_count(c.sv);
             // the next chunk's particle index is loaded one chunk ahead, and a
             // particle's x, v and F are loaded together before any store
-            int pNext = (ti.start + lane < ti.end) ? (int)c.gatherOrder[ti.start + lane] : 0;
+            int pNext = (int)c.gatherOrder[min(ti.start + lane, ti.end - 1)];
+            settle(pNext);
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
                 const int p = pNext;
-                pNext = (k + 64 < ti.end) ? (int)c.gatherOrder[k + 64] : 0;
+                pNext = (int)c.gatherOrder[min(k + 64, ti.end - 1)];
                 const float4 xm = c.P.xm[p];
                 const float4 vp = c.P.v[p];
                 const M3 F = load_cols(c.S.F, p);
@@ -1696,8 +1705,8 @@ namespace
             if (Mode > 0)
                 load_tile_grid(c.g, ti, c.grid, G, lane);
             wave_sync();
-            const int nSlicers = c.sv.counts[1];
-            const int nShapes = c.sv.counts[0];
+            const int nSlicers = slicer_count(c.sv);
+            const int nShapes = shape_count(c.sv);
             const int S = st.stride;
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
@@ -1764,7 +1773,7 @@ namespace
         V3 x = clamp_bounds(c.g, v3(xm) + vel * c.dt);
         vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
         float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
-        resolve_rigid<true>(c.sv, c.sv.counts[0], vv, x);
+        resolve_rigid<true>(c.sv, shape_count(c.sv), vv, x);
         c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
         c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
         const M3 F = load_m3(st.Fs, S, k);
@@ -1872,7 +1881,7 @@ namespace
             load_tile_grid(c.g, ti, c.grid, G, lane);
             wave_sync();
             const GridGeom &g = c.g;
-            const int nShapes = c.sv.counts[0];
+            const int nShapes = shape_count(c.sv);
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
                 const int p = (int)c.gatherOrder[k];
