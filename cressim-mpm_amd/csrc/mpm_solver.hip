@@ -1713,6 +1713,19 @@ namespace
                 const float4 xm = st.xs[k];
                 V3 x = v3(xm);
                 V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
+                // the constraint's inputs are requested with x and v, so their
+                // latency overlaps the gather's (the material type decides a
+                // branch: loaded after the gather it would cost a second wait)
+                const M3 F = load_m3(st.Fs, S, k);
+                float2 pr;
+                int type = 0, p = 0;
+                if (Mode < 2)
+                {
+                    pr = st.ps[k];
+                    type = st.ts[k];
+                }
+                else
+                    p = (int)c.order[k];
                 M3 Cm;
                 if (Mode == 0)
                     Cm = load_m3(st.Cs, S, k);
@@ -1724,9 +1737,7 @@ namespace
                 }
                 if (Mode < 2)
                 {
-                    const M3 F = load_m3(st.Fs, S, k);
-                    const float2 pr = st.ps[k];
-                    pb_constraint(c.dt, pr.x, pr.y, st.ts[k], F, Cm);
+                    pb_constraint(c.dt, pr.x, pr.y, type, F, Cm);
                     if (Mode == 1 && nSlicers > 0)
                     {
                         st.vs[k] = vel.x;
@@ -1738,7 +1749,6 @@ namespace
                 else
                 {
                     // pbMpmIntegrateParticle (src/solver/pb_mpm_algorithm.h:363-408)
-                    const int p = (int)c.order[k];
                     x = clamp_bounds(c.g, x + vel * c.dt);
                     vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
                     float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
@@ -1746,7 +1756,6 @@ namespace
                     c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                     c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
                     store_cols(c.S.C, p, Cm);
-                    const M3 F = load_m3(st.Fs, S, k);
                     store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
                 }
             }
