@@ -45,6 +45,10 @@ namespace
     constexpr int kWaves = kWG / 64;  // tiles per workgroup
     constexpr int kRadixItems = 8;
     constexpr int kRadixTile = kWG * kRadixItems;
+    // Radix digits are 8 to 10 bits wide: the fewest passes that cover the
+    // key range (pblock index and sentinel), each as narrow as that allows.
+    constexpr int kMaxDigitBits = 10;
+    constexpr int kMaxDigits = 1 << kMaxDigitBits;
 
     struct GridGeom
     {
@@ -467,10 +471,15 @@ namespace
         return off + inc - v;
     }
 
-    __global__ __launch_bounds__(kWG) void k_radix_hist(const unsigned *keys, int n, int shift, unsigned *counts, int numTiles)
+    // Histogram of one tile's digits (digit width `bits`), digit-major.
+    template <int Bits>
+    __global__ __launch_bounds__(kWG) void k_radix_hist(const unsigned *keys, int n, int shift, unsigned *counts,
+                                                        int numTiles)
     {
-        __shared__ unsigned h[256];
-        h[threadIdx.x] = 0;
+        constexpr int digits = 1 << Bits;
+        __shared__ unsigned h[digits];
+        for (int d = threadIdx.x; d < digits; d += kWG)
+            h[d] = 0;
         __syncthreads();
         const int base = blockIdx.x * kRadixTile;
 #pragma unroll
@@ -478,10 +487,11 @@ namespace
         {
             const int idx = base + r * kWG + threadIdx.x;
             if (idx < n)
-                atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+                atomicAdd(&h[(keys[idx] >> shift) & (unsigned)(digits - 1)], 1u);
         }
         __syncthreads();
-        counts[threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
+        for (int d = threadIdx.x; d < digits; d += kWG)
+            counts[d * numTiles + blockIdx.x] = h[d];
     }
 
     // Sort item i of a step is active particle i - activeOffset of the scene
@@ -504,11 +514,14 @@ namespace
     // totalBlocks when the particle does not take part), the particle index,
     // and the histogram of the first radix digit: k_radix_hist fused with the
     // binning pass.
+    template <int Bits>
     __global__ __launch_bounds__(kWG) void k_bin_hist(BatchArgs b, const int *activePrefix, int n, int useMask,
                                                       unsigned *keys, unsigned *vals, unsigned *counts, int numTiles)
     {
-        __shared__ unsigned h[256];
-        h[threadIdx.x] = 0;
+        constexpr int digits = 1 << Bits;
+        __shared__ unsigned h[digits];
+        for (int d = threadIdx.x; d < digits; d += kWG)
+            h[d] = 0;
         __syncthreads();
         const int base = blockIdx.x * kRadixTile;
 #pragma unroll
@@ -531,11 +544,12 @@ namespace
                 }
                 keys[i] = key;
                 vals[i] = (unsigned)p;
-                atomicAdd(&h[key & 255u], 1u);
+                atomicAdd(&h[key & (unsigned)(digits - 1)], 1u);
             }
         }
         __syncthreads();
-        counts[threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
+        for (int d = threadIdx.x; d < digits; d += kWG)
+            counts[d * numTiles + blockIdx.x] = h[d];
     }
 
     // One workgroup per digit: exclusive scan over the tiles, digit total out.
@@ -558,34 +572,51 @@ namespace
             digitTotals[d] = carry;
     }
 
+    // Stable scatter of one tile by the digit (key >> shift) & (2^bits - 1):
+    // the tile's items go in rounds of 256; within a round, items of equal
+    // digit keep their order (rank within the wavefront by ballots, then
+    // wavefronts in order).  Thread t keeps the digits [t R, t R + R).
+    template <int Bits>
     __global__ __launch_bounds__(kWG) void k_radix_scatter(const unsigned *keysIn, const unsigned *valsIn, int n, int shift,
-                                                           const unsigned *counts, int numTiles, const unsigned *digitTotals,
-                                                           unsigned *keysOut, unsigned *valsOut)
+                                                           const unsigned *counts, int numTiles,
+                                                           const unsigned *digitTotals, unsigned *keysOut,
+                                                           unsigned *valsOut)
     {
-        __shared__ unsigned digitOff[256];
-        __shared__ unsigned waveCnt[kWaves][256];
-        __shared__ unsigned roundTot[256];
+        constexpr int digits = 1 << Bits, R = digits / kWG;
+        __shared__ unsigned digitOff[digits];
+        __shared__ unsigned waveCnt[kWaves][digits];
+        __shared__ unsigned roundTot[digits];
         __shared__ unsigned sh[kWaves];
         const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-        unsigned tot;
-        const unsigned ex = block_exclusive_scan(digitTotals[tid], tot, sh);
-        digitOff[tid] = ex + counts[tid * numTiles + blockIdx.x];
+        const int d0 = tid * R;
+        unsigned mine = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w)
-            waveCnt[w][tid] = 0;
+        for (int r = 0; r < R; ++r)
+            mine += digitTotals[d0 + r];
+        unsigned tot;
+        unsigned ex = block_exclusive_scan(mine, tot, sh);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            digitOff[d0 + r] = ex + counts[(d0 + r) * numTiles + blockIdx.x];
+            ex += digitTotals[d0 + r];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w)
+                waveCnt[w][d0 + r] = 0;
+        }
         __syncthreads();
         const unsigned long long ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
         const int base = blockIdx.x * kRadixTile;
-        for (int r = 0; r < kRadixItems; ++r)
+        for (int rd = 0; rd < kRadixItems; ++rd)
         {
-            const int idx = base + r * kWG + tid;
+            const int idx = base + rd * kWG + tid;
             const bool valid = idx < n;
             const unsigned key = valid ? keysIn[idx] : 0u;
             const unsigned val = valid ? valsIn[idx] : 0u;
-            const unsigned d = (key >> shift) & 255u;
+            const unsigned d = (key >> shift) & (unsigned)(digits - 1);
             unsigned long long peers = __ballot(valid);
 #pragma unroll
-            for (int bit = 0; bit < 8; ++bit)
+            for (int bit = 0; bit < Bits; ++bit)
             {
                 const bool set = (d >> bit) & 1u;
                 const unsigned long long bal = __ballot(valid && set);
@@ -595,15 +626,19 @@ namespace
             if (valid && rankInWave == 0)
                 waveCnt[wave][d] = (unsigned)__popcll(peers);
             __syncthreads();
-            unsigned run = 0;
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w)
+            for (int r = 0; r < R; ++r)
             {
-                const unsigned c = waveCnt[w][tid];
-                waveCnt[w][tid] = run;
-                run += c;
+                unsigned run = 0;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w)
+                {
+                    const unsigned c = waveCnt[w][d0 + r];
+                    waveCnt[w][d0 + r] = run;
+                    run += c;
+                }
+                roundTot[d0 + r] = run;
             }
-            roundTot[tid] = run;
             __syncthreads();
             if (valid)
             {
@@ -612,10 +647,14 @@ namespace
                 valsOut[pos] = val;
             }
             __syncthreads();
-            digitOff[tid] += roundTot[tid];
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w)
-                waveCnt[w][tid] = 0;
+            for (int r = 0; r < R; ++r)
+            {
+                digitOff[d0 + r] += roundTot[d0 + r];
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w)
+                    waveCnt[w][d0 + r] = 0;
+            }
             __syncthreads();
         }
     }
@@ -2723,21 +2762,32 @@ namespace
         int bits = 0;
         while ((1ll << bits) <= b->totalBlocks)
             ++bits;
-        const int passes = (bits + 7) / 8;
+        const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+        const int digitBits = std::max(8, (bits + passes - 1) / passes);
         const int numTiles = grid1d(n, kRadixTile);
         const BatchArgs a = batch_args(b, nullptr);
         unsigned *kin = b->keysA, *vin = b->valsA, *kout = b->keysB, *vout = b->valsB;
         for (int pass = 0; pass < passes; ++pass)
         {
-            const int shift = 8 * pass;
-            if (pass == 0)
-                launch(b, CRMPM_HIP_PROF_BINNING, k_bin_hist, numTiles, kWG, a, b->dActivePrefix, n, useMask, kin, vin,
-                       b->radixCounts, numTiles);
+            const int shift = digitBits * pass;
+            auto passKernels = [&](auto bitsTag) {
+                constexpr int B = decltype(bitsTag)::value;
+                if (pass == 0)
+                    launch(b, CRMPM_HIP_PROF_BINNING, k_bin_hist<B>, numTiles, kWG, a, b->dActivePrefix, n, useMask, kin,
+                           vin, b->radixCounts, numTiles);
+                else
+                    launch(b, CRMPM_HIP_PROF_BINNING, k_radix_hist<B>, numTiles, kWG, kin, n, shift, b->radixCounts,
+                           numTiles);
+                launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scan, 1 << B, kWG, b->radixCounts, numTiles, b->digitTotals);
+                launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scatter<B>, numTiles, kWG, kin, vin, n, shift, b->radixCounts,
+                       numTiles, b->digitTotals, kout, vout);
+            };
+            if (digitBits == 8)
+                passKernels(std::integral_constant<int, 8>());
+            else if (digitBits == 9)
+                passKernels(std::integral_constant<int, 9>());
             else
-                launch(b, CRMPM_HIP_PROF_BINNING, k_radix_hist, numTiles, kWG, kin, n, shift, b->radixCounts, numTiles);
-            launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scan, 256, kWG, b->radixCounts, numTiles, b->digitTotals);
-            launch(b, CRMPM_HIP_PROF_BINNING, k_radix_scatter, numTiles, kWG, kin, vin, n, shift, b->radixCounts, numTiles,
-                   b->digitTotals, kout, vout);
+                passKernels(std::integral_constant<int, 10>());
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
@@ -2992,8 +3042,8 @@ namespace
         rc |= dalloc(&b->keysB, N);
         rc |= dalloc(&b->valsA, N);
         rc |= dalloc(&b->valsB, N);
-        rc |= dalloc(&b->radixCounts, (size_t)radixTiles * 256);
-        rc |= dalloc(&b->digitTotals, 256);
+        rc |= dalloc(&b->radixCounts, (size_t)radixTiles * kMaxDigits);
+        rc |= dalloc(&b->digitTotals, kMaxDigits);
         // partial boxes: subStride per pblock (up to 8, fewer for very large
         // batches so the boxes stay within 8 GiB)
         const size_t box = (size_t)kTileNodes * sizeof(float4);
