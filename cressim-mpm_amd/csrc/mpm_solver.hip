@@ -330,6 +330,44 @@ namespace
         __builtin_amdgcn_wave_barrier();
     }
 
+    // A chunk's solver-owned matrices, written through LDS.  A store of one
+    // 48-byte record per lane writes 16 bytes every 48 and touches three
+    // times as many cache lines per instruction as 64 consecutive 16-byte
+    // pieces, and the L2's write requests then bound the gather kernels.  So
+    // each lane puts its particle's C and F columns into the wavefront's
+    // stage in chunk order, and lane j stores the chunk's j-th column: where
+    // the chunk's particles are consecutive (index order), neighbouring lanes
+    // fill neighbouring bytes.  All 64 lanes call it; p < 0: no particle.
+    struct ChunkStage
+    {
+        float4 col[2][3 * 64]; // C, F: column c of the chunk's particle q at 3 q + c
+        int p[64];
+    };
+    CR_DEV void store_chunk_cols(ChunkStage &S, int lane, int p, const M3 &Cm, const M3 &Fm, float4 *C, float4 *F)
+    {
+        S.p[lane] = p;
+        const V3 *cc = &Cm.c0, *fc = &Fm.c0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            S.col[0][3 * lane + k] = make_float4(cc[k].x, cc[k].y, cc[k].z, 0.0f);
+            S.col[1][3 * lane + k] = make_float4(fc[k].x, fc[k].y, fc[k].z, 0.0f);
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            const int j = 64 * i + lane, q = j / 3, k = j - 3 * q;
+            const int pq = S.p[q];
+            if (pq >= 0)
+            {
+                C[3 * pq + k] = S.col[0][j];
+                F[3 * pq + k] = S.col[1][j];
+            }
+        }
+        wave_sync(); // the next chunk reuses the stage
+    }
+
     struct Weights
     {
         float w[3][3]; // w[i][axis]
@@ -524,27 +562,42 @@ namespace
             h[d] = 0;
         __syncthreads();
         const int base = blockIdx.x * kRadixTile;
-#pragma unroll
-        for (int r = 0; r < kRadixItems; ++r)
-        {
-            const int i = base + r * kWG + threadIdx.x;
-            if (i < n)
+        auto bin = [&](int i, int sc) {
+            ConstSceneRec &rec = ((ConstSceneRec *)(size_t)b.scenes)[sc];
+            const int p = rec.particleOffset + (i - const_load(activePrefix, sc));
+            unsigned key = (unsigned)b.totalBlocks;
+            if (!useMask || b.P.mask[p])
             {
-                const int sc = scene_of_item(activePrefix, b.numScenes, i);
-                const SceneRec &rec = b.scenes[sc];
-                const int p = rec.particleOffset + (i - activePrefix[sc]);
-                unsigned key = (unsigned)b.totalBlocks;
-                if (!useMask || b.P.mask[p])
-                {
-                    const float4 x = b.P.xm[p];
-                    int bx, by, bz;
-                    V3 fx;
-                    cell_of(rec.g, v3(x), bx, by, bz, fx);
-                    key = (unsigned)(rec.blockOffset + pblock_key(rec.g, bx, by, bz));
-                }
-                keys[i] = key;
-                vals[i] = (unsigned)p;
-                atomicAdd(&h[key & (unsigned)(digits - 1)], 1u);
+                const float4 x = b.P.xm[p];
+                const GridGeom g = rec.g;
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, v3(x), bx, by, bz, fx);
+                key = (unsigned)(rec.blockOffset + pblock_key(g, bx, by, bz));
+            }
+            keys[i] = key;
+            vals[i] = (unsigned)p;
+            atomicAdd(&h[key & (unsigned)(digits - 1)], 1u);
+        };
+        // one scene: its record's fields are wavefront-uniform scalar loads
+        if (b.numScenes == 1)
+        {
+#pragma unroll
+            for (int r = 0; r < kRadixItems; ++r)
+            {
+                const int i = base + r * kWG + threadIdx.x;
+                if (i < n)
+                    bin(i, 0);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < kRadixItems; ++r)
+            {
+                const int i = base + r * kWG + threadIdx.x;
+                if (i < n)
+                    bin(i, scene_of_item(activePrefix, b.numScenes, i));
             }
         }
         __syncthreads();
@@ -1582,6 +1635,7 @@ namespace
     __global__ __launch_bounds__(kWG) void k_mls_g2p(BatchArgs b)
     {
         __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ ChunkStage stage[kWaves];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
         for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubMax); tg < items;
@@ -1601,29 +1655,35 @@ namespace
             const int nSlicers = slicer_count(c.sv);
             const int nShapes = shape_count(c.sv);
             // the next chunk's particle index is loaded one chunk ahead, and a
-            // particle's x, v and F are loaded together before any store
+            // particle's x, v and F are loaded together before any store; all
+            // lanes run every chunk (the staged C and F stores need them)
             int pNext = (int)c.gatherOrder[min(ti.start + lane, ti.end - 1)];
             settle(pNext);
-            for (int k = ti.start + lane; k < ti.end; k += 64)
+            for (int k0 = ti.start; k0 < ti.end; k0 += 64)
             {
+                const int k = k0 + lane;
+                const bool valid = k < ti.end;
                 const int p = pNext;
                 pNext = (int)c.gatherOrder[min(k + 64, ti.end - 1)];
-                const float4 xm = c.P.xm[p];
-                const float4 vp = c.P.v[p];
-                const M3 F = load_cols(c.S.F, p);
-                V3 x = v3(xm);
-                V3 vel;
-                M3 Cm;
-                // the particle velocity after P2G's gravity update (P2G does not store it)
-                const V3 vpg = v3(vp) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                gather_vc(c, ti, G, nSlicers, x, vpg, vel, Cm);
-                x = clamp_bounds(c.g, x + vel * c.dt);
-                float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                resolve_rigid<true>(c.sv, nShapes, vv, x);
-                c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
-                c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                store_cols(c.S.C, p, Cm);
-                store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
+                M3 Cm = m3_identity(), Fn = m3_identity();
+                if (valid)
+                {
+                    const float4 xm = c.P.xm[p];
+                    const float4 vp = c.P.v[p];
+                    const M3 F = load_cols(c.S.F, p);
+                    V3 x = v3(xm);
+                    V3 vel;
+                    // the particle velocity after P2G's gravity update (P2G does not store it)
+                    const V3 vpg = v3(vp) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                    gather_vc(c, ti, G, nSlicers, x, vpg, vel, Cm);
+                    x = clamp_bounds(c.g, x + vel * c.dt);
+                    float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                    resolve_rigid<true>(c.sv, nShapes, vv, x);
+                    c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+                    c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+                    Fn = mul(m3_identity() + Cm * c.dt, F);
+                }
+                store_chunk_cols(stage[wave], lane, valid ? p : -1, Cm, Fn, c.S.C, c.S.F);
             }
         }
     }
