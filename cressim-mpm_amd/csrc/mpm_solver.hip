@@ -954,7 +954,7 @@ namespace
         }
     }
 
-    // Accumulation tiles per wavefront of MLS P2G and the fused PB iteration kernel.
+    // Accumulation tiles per wavefront of MLS P2G.
     constexpr int kScatterTiles = 2;
 
     // Momentum-only accumulation node (the fused PB iteration kernel: a
@@ -1723,7 +1723,7 @@ namespace
         st.ts[k] = P.type[p];
     }
 
-    // Scatter of one PB particle into the LDS tiles A0[0 .. kScatterTiles)
+    // Scatter of one PB particle into the LDS tiles A0[0 .. Tiles)
     // (pbMpmParticleToGrid loop)
     template <int Tiles, typename Acc>
     CR_DEV void pb_scatter(const StepCommon &c, const TileInfo &ti, Acc *A0, int nSlicers, V3 x, float m, V3 vel,
@@ -1771,9 +1771,9 @@ namespace
     // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
     // iteration fused with P2G of this one.  Mode 2: last G2P fused with
     // pbMpmIntegrateParticle, results written back in particle order.
-    // Mode 0 scatters into 3 accumulation tiles (it stages no grid tile),
-    // Mode 1 into 2 (a third one, and its occupancy and register cost, did not
-    // pay for the scatter passes it saves there).  Positions and masses are
+    // Both scatter into 3 accumulation tiles: Mode 0 stages no grid tile,
+    // and Mode 1's momentum-only tiles (12 bytes per node) leave room for a
+    // third within the LDS of 3 wavefronts per SIMD.  Positions and masses are
     // fixed over a substep's iterations, so every iteration scatters the same
     // node masses: Mode 0 stores them with the momenta in each pblock's partial
     // box, Mode 1 accumulates and stores momentum only (12 of 16 bytes per
@@ -1781,7 +1781,7 @@ namespace
     template <int Mode, bool Persistent>
     __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
     {
-        constexpr int kAcc = (Mode == 0) ? 3 : kScatterTiles;
+        constexpr int kAcc = 3;
         using Acc = typename std::conditional<Mode == 0, float4, F3>::type;
         __shared__ Acc acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
         __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
