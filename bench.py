@@ -146,6 +146,8 @@ COUNTER_PASSES = (
            "SQ_INSTS_VMEM_RD", False),
     ("tcc", "TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum TCC_EA0_ATOMIC_sum", False),
     ("dram", "TCC_EA0_RDREQ_DRAM_32B_sum TCC_EA0_WRREQ_WRITE_DRAM_32B_sum", True),
+    ("stall", "SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS "
+              "SQ_LDS_BANK_CONFLICT SQ_INSTS_VMEM_WR SQ_INSTS_LDS", False),
 )
 
 
@@ -221,13 +223,17 @@ def summarize(out_dir, title, commands, counters=True):
             c[name] = _counter_per_launch(os.path.join(out_dir, "dram"), name)[0]
             # 32-byte requests -> bytes, scaled so the copy reads (writes) its 256 MiB
             fac[name] = COPY_CAL_BYTES / (32.0 * _copy_reading(os.path.join(out_dir, "dram_copy"), name))
+        for name in ("SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS",
+                     "SQ_LDS_BANK_CONFLICT"):
+            c["stall_" + name] = _counter_per_launch(os.path.join(out_dir, "stall"), name)[0]
     lines = [f"# {title}", "", "Commands (repository root, rocprofv3 on one MI355X):", "", "```"]
     lines += commands + ["```", ""]
     if counters:
         lines += ["| kernel | calls | avg µs | total % | FETCH_SIZE KiB | WRITE_SIZE KiB | L2↔memory MB (calibrated) "
                   "| DRAM-path read MB | DRAM-path write MB | L2 hit % | atomics at L2 | atomics past L2 "
-                  "| SQ_INSTS_VALU | VALU issue µs |",
-                  "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+                  "| SQ_INSTS_VALU | VALU issue µs | waiting % | issue-blocked % | VALU active % | LDS active % "
+                  "| LDS bank conflict % |",
+                  "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
     else:
         lines += ["| kernel | calls | avg µs | total % |", "|---|---|---|---|"]
     merged = {}
@@ -250,6 +256,12 @@ def summarize(out_dir, title, commands, counters=True):
         rd, wr = g("TCC_EA0_RDREQ_DRAM_32B_sum"), g("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
         out["dram_path_read_bytes"] = None if rd is None else 32.0 * rd * fac["TCC_EA0_RDREQ_DRAM_32B_sum"]
         out["dram_path_write_bytes"] = None if wr is None else 32.0 * wr * fac["TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"]
+        wc = g("stall_SQ_WAVE_CYCLES")
+        for key, name in (("wait_frac", "SQ_WAIT_ANY"), ("issue_wait_frac", "SQ_WAIT_INST_ANY"),
+                          ("valu_active_frac", "SQ_ACTIVE_INST_VALU"), ("lds_active_frac", "SQ_ACTIVE_INST_LDS"),
+                          ("lds_bank_conflict_frac", "SQ_LDS_BANK_CONFLICT")):
+            v = g("stall_" + name)
+            out[key] = None if v is None or not wc else v / wc
         return out
 
     f = lambda v, fmt: "" if v is None else format(v, fmt)  # noqa: E731
@@ -263,7 +275,10 @@ def summarize(out_dir, title, commands, counters=True):
                     f" | {f(None if q['l2_hit_rate'] is None else 100 * q['l2_hit_rate'], '.1f')}"
                     f" | {f(q['atomics_l2'], ',.0f')} | {f(q['atomics_past_l2'], ',.0f')}"
                     f" | {f(q['valu_instructions'], ',.0f')}"
-                    f" | {f(None if q['valu_instructions'] is None else valu_issue_us(q['valu_instructions']), '.1f')} |")
+                    f" | {f(None if q['valu_instructions'] is None else valu_issue_us(q['valu_instructions']), '.1f')}"
+                    + "".join(f" | {f(None if q[key] is None else 100 * q[key], '.1f')}"
+                              for key in ("wait_frac", "issue_wait_frac", "valu_active_frac", "lds_active_frac",
+                                          "lds_bank_conflict_frac")) + " |")
         lines.append(row)
     lines += ["", "Counter columns are per dispatch, averaged over the kernel's dispatches. Kernel names carry their "
               "template arguments (the tile kernels' last one selects the persistent launch form)."]
@@ -281,7 +296,13 @@ def summarize(out_dir, title, commands, counters=True):
               "", "L2 hit % = TCC_HIT / (TCC_HIT + TCC_MISS). Atomics at L2 = TCC_ATOMIC_sum (atomic requests of all "
               "types reaching the L2); past L2 = TCC_EA0_ATOMIC_sum.",
               "", "VALU issue µs = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz): the time the VALU pipes need "
-              "for the kernel's vector instructions; a kernel whose duration is close to it is bound by VALU issue, not HBM."]
+              "for the kernel's vector instructions; a kernel whose duration is close to it is bound by VALU issue, not HBM.",
+              "", "Wave-state columns are fractions of the kernel's wavefront cycles (SQ_WAVE_CYCLES): waiting = "
+              "SQ_WAIT_ANY (at an s_waitcnt: memory or LDS results outstanding), issue-blocked = SQ_WAIT_INST_ANY "
+              "(an instruction ready but not issued: its unit busy with other wavefronts or a dependency), VALU and "
+              "LDS active = SQ_ACTIVE_INST_VALU and SQ_ACTIVE_INST_LDS, LDS bank conflict = SQ_LDS_BANK_CONFLICT "
+              "cycles. Summed over the wavefronts resident on a SIMD, the VALU-active fraction approaches 1 when "
+              "the SIMD's VALU is saturated."]
     result = {}
     for k in merged:
         q = per_kernel(k)
