@@ -8,10 +8,12 @@ indenter (r = 0.1 m) moving down at 0.05 m/s with friction 0.3, its target
 updated every 10 substeps.  Inputs are synthetic (SplitMix64-seeded) and
 already resident in HBM when timing starts.  A step is one substep.
 
-Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
-(the fused G2P+P2G iteration kernel) measured with HIP events on the solver
-stream over the timed region (the events ride on the dispatches of every 10th
-substep), and a CPU baseline timed on this host.
+Prints one JSON line: particle-substeps/s, the roofline of the dominant work
+(a PB iteration after the first: its per-particle kernel, G2P and the material
+constraint, and its per-pblock P2G scatter, 19 of the 20 iterations) measured
+with HIP events on the solver stream over the timed region (the events ride
+on the dispatches of every 10th substep), and a CPU baseline timed on this
+host.
 """
 import argparse
 import copy
@@ -33,13 +35,16 @@ HBM_PEAK_GBS = 8000.0
 # Minimal FP32 state traffic per particle for one G2P + one P2G of PB-MPM:
 # P2G reads x, v, F, C, (alpha, r) = 12+12+36+36+8 = 104 B; G2P reads x and
 # writes v, C = 12+12+36 = 60 B.  Grid: P2G writes and G2P reads 16 B per
-# touched node.
-BYTES_PER_PARTICLE_FUSED = 164
-BYTES_PER_TOUCHED_NODE_FUSED = 32
+# touched node.  (The iteration's two kernels also pass v and C between them
+# through memory; that traffic is the implementation's, not the algorithm's.)
+BYTES_PER_PARTICLE_ITERATION = 164
+BYTES_PER_TOUCHED_NODE_ITERATION = 32
+# The same split by kernel: the per-particle kernel reads x, F, (alpha, r)
+# and the grid, and writes v, C; the scatter reads x, m, v, C and writes the
+# momentum of the touched nodes.
+BYTES_DENSE = (12 + 36 + 8 + 12 + 36, 16)
+BYTES_SCATTER = (12 + 4 + 12 + 36, 12)
 PROFILES = os.path.join(ROOT, "profiles")
-TRAFFIC_FILE = os.path.join(PROFILES, "pb_slab262k_counters.json")
-# the fused iteration kernel (template Mode 1; either launch form)
-FUSED_KERNEL = "k_pb_tile<1,"
 # what the library's autotuner picked: launch sizes, the P2G sub-tile split
 # and the G2P (gather) split
 TUNED_KEYS = ("tile_waves", "update_waves", "sub_size", "sub_max", "gather_sub_size", "gather_sub_max")
@@ -180,6 +185,24 @@ def roofline_counters(row, avg_s):
             "profile_avg_launch_us": row.get("avg_us")}
 
 
+def iteration_counters(rows, avg_s):
+    """roofline_counters over several kernels run back to back (a PB
+    iteration): traffic, DRAM-path bytes, atomics and VALU instructions
+    summed, the L2 hit rate from the summed hits and misses."""
+    if any(r is None for r in rows):
+        return {"traffic": None}
+    tot = lambda key: None if any(r.get(key) is None for r in rows) else sum(r[key] for r in rows)  # noqa: E731
+    h, m = tot("tcc_hit"), tot("tcc_miss")
+    rd, wr = tot("dram_path_read_bytes"), tot("dram_path_write_bytes")
+    vi = tot("valu_instructions")
+    return {"traffic": tot("bytes_per_launch"),
+            "l2_hit_rate": None if h is None or m is None or h + m == 0 else h / (h + m),
+            "dram_path_bytes": None if rd is None or wr is None else rd + wr,
+            "atomics_per_launch": tot("atomics_l2"),
+            "valu_issue_frac": None if vi is None else valu_issue_us(vi) / (avg_s * 1e6),
+            "profile_avg_launch_us": tot("avg_us")}
+
+
 def profile_commands(target):
     t = PROFILE_TARGETS[target]
     run = t["run"]
@@ -251,6 +274,7 @@ def summarize(out_dir, title, commands, counters=True):
         if out["fetch_kib_raw"] is not None and out["write_kib_raw"] is not None:
             out["bytes_per_launch"] = 1024.0 * (out["fetch_kib_raw"] * fac["fetch"] + out["write_kib_raw"] * fac["write"])
         h, m = g("TCC_HIT_sum"), g("TCC_MISS_sum")
+        out["tcc_hit"], out["tcc_miss"] = h, m
         out["l2_hit_rate"] = None if h is None or m is None or h + m == 0 else h / (h + m)
         out["atomics_l2"], out["atomics_past_l2"] = g("TCC_ATOMIC_sum"), g("TCC_EA0_ATOMIC_sum")
         rd, wr = g("TCC_EA0_RDREQ_DRAM_32B_sum"), g("TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
@@ -942,12 +966,25 @@ def main():
     elapsed = t1 - t0
     value = n * K / elapsed
 
-    fused_ms, fused_n = prof["pb_fused"]
-    avg_fused_s = fused_ms / max(fused_n, 1) / 1e3
-    bytes_fused = BYTES_PER_PARTICLE_FUSED * n + BYTES_PER_TOUCHED_NODE_FUSED * touched
-    achieved = bytes_fused / avg_fused_s / 1e9
-    fused_row = f"k_pb_tile<1, {'true' if tuned[0] > 0 else 'false'}>"
-    counters = roofline_counters(profiled_row("pb_slab", fused_row), avg_fused_s)
+    # one iteration after the first = the per-particle kernel + the per-pblock scatter
+    dense_ms, dense_n = prof["pb_fused"]
+    scat_ms, scat_n = prof["pb_scatter"]
+    avg_dense_s = dense_ms / max(dense_n, 1) / 1e3
+    avg_scat_s = scat_ms / max(scat_n, 1) / 1e3
+    avg_iter_s = avg_dense_s + avg_scat_s
+    bytes_iter = BYTES_PER_PARTICLE_ITERATION * n + BYTES_PER_TOUCHED_NODE_ITERATION * touched
+    achieved = bytes_iter / avg_iter_s / 1e9
+    dense_row = "k_pb_dense<1>"
+    scat_row = f"k_pb_scatter<false, {'true' if tuned[0] > 0 else 'false'}>"
+    rows = (profiled_row("pb_slab", dense_row), profiled_row("pb_slab", scat_row))
+    per_kernel = {}
+    for name, row, avg_s, (bp, bn), launches in ((dense_row, rows[0], avg_dense_s, BYTES_DENSE, dense_n),
+                                                 (scat_row, rows[1], avg_scat_s, BYTES_SCATTER, scat_n)):
+        alg = bp * n + bn * touched
+        per_kernel[name] = {"avg_launch_us": avg_s * 1e6, "launches": launches, "algorithmic_bytes_per_launch": alg,
+                            "achieved": alg / avg_s / 1e9, "frac": alg / avg_s / 1e9 / HBM_PEAK_GBS,
+                            **roofline_counters(row, avg_s)}
+    counters = iteration_counters(rows, avg_iter_s)
     out = {
         "metric": METRIC,
         "value": value,
@@ -965,14 +1002,15 @@ def main():
                    "grid_nodes": list(solver.dims), "dx": scene["dx"], "iterations": scene["iterations"],
                    "dt": scene["dt"], "frame_substeps": fsub, "parallelism": "single_gpu",
                    "autotuned": dict(zip(TUNED_KEYS, tuned))},
-        "roofline": {"bound": "hbm", "kernel": fused_row + " (fused G2P+P2G)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "avg_launch_us": avg_fused_s * 1e6, "launches": fused_n,
-                     "algorithmic_bytes_per_launch": bytes_fused, "touched_nodes": touched,
+        "roofline": {"bound": "hbm", "kernel": f"{dense_row} + {scat_row} (one PB iteration: G2P and the "
+                                                 "constraint per particle, P2G per pblock)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "avg_launch_us": avg_iter_s * 1e6, "launches": min(dense_n, scat_n),
+                     "algorithmic_bytes_per_launch": bytes_iter, "touched_nodes": touched,
                      # HBM-side traffic, L2 hit rate, DRAM-path bytes, atomics and VALU issue of the
-                     # same kernel from the committed rocprofv3 counter passes of this workload
-                     # (profiles/pb_slab262k_counters.json)
-                     **counters},
+                     # same kernels from the committed rocprofv3 counter passes of this workload
+                     # (profiles/pb_slab262k_counters.json), summed over the iteration's two kernels
+                     **counters, "kernels": per_kernel},
     }
     st = solver.state()
     out["state_check"] = {"nonfinite_particles": int((~np.isfinite(st["xm"])).any(axis=1).sum()),

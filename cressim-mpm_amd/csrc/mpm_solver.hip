@@ -17,6 +17,10 @@
 //                 partial buffer) -> update (one wavefront per active node block: sums the
 //                 <= 8 overlapping tiles in a fixed order, then cutoff / contact / walls)
 //                 -> G2P (one wavefront per tile, 6^3 nodes staged in LDS).
+// Substep (PB):   the same binning, then stage the particles in sorted order and run K
+//                 iterations, each a per-particle kernel (G2P of the previous iteration and
+//                 the material constraint, 64 sorted particles per wavefront) and a
+//                 per-tile scatter (P2G), then the update; the last G2P integrates.
 // P2G never issues global atomics and every sum has a fixed order, so a substep is bitwise
 // reproducible run to run.
 #include <hip/hip_ext.h>
@@ -957,8 +961,9 @@ namespace
     // Accumulation tiles per wavefront of MLS P2G.
     constexpr int kScatterTiles = 2;
 
-    // Momentum-only accumulation node (the fused PB iteration kernel: a
-    // pblock's node masses do not change over a substep's iterations).
+    // Momentum-only accumulation node (the PB scatter of iterations after
+    // the first: a pblock's node masses do not change over a substep's
+    // iterations).
     struct F3
     {
         float x, y, z;
@@ -1234,7 +1239,9 @@ namespace
     // sum (w v_i)_k dist, and dist's r-component depends only on the node's
     // r-axis index, so B(r, k) = sum_i d[r][i] * S_r[i]_k with S_r[i] the sum
     // of w v over the nodes whose r-axis index is i.
-    CR_DEV void gather_separable(const float4 *G, int l0, const Weights &W, const AxisOffsets &o, V3 &v, M3 &B)
+    // node(a, b, c) returns node (a, b, c) of the particle's 3^3 stencil.
+    template <class Node>
+    CR_DEV void gather_separable_by(Node &&node, const Weights &W, const AxisOffsets &o, V3 &v, M3 &B)
     {
         V3 sx[3], sy[3], sz[3];
 #pragma unroll
@@ -1249,7 +1256,7 @@ namespace
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                 {
-                    const V3 nw = v3(G[node_li(l0, a, b, c)]) * (wab * W.w[c][2]);
+                    const V3 nw = v3(node(a, b, c)) * (wab * W.w[c][2]);
                     sx[a] = sx[a] + nw;
                     sy[b] = sy[b] + nw;
                     sz[c] = sz[c] + nw;
@@ -1264,6 +1271,65 @@ namespace
             return V3{comp(sx, 0), comp(sy, 1), comp(sz, 2)};
         };
         B = M3{col(0), col(1), col(2)};
+    }
+    // The same sums with the stencil read from global memory one x-slice (9
+    // nodes) at a time, at most two slices in flight: issued all at once the
+    // 27 loads hold 81 registers, which costs the per-particle kernel a
+    // wavefront per SIMD.  Same nodes, products and summation order as
+    // gather_separable_by.
+    template <class Node>
+    CR_DEV void gather_separable_slices(Node &&node, const Weights &W, const AxisOffsets &o, V3 &v, M3 &B)
+    {
+        V3 sx[3], sy[3], sz[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            sx[i] = sy[i] = sz[i] = V3{0.f, 0.f, 0.f};
+        float4 q0[9], q1[9];
+        auto load = [&](int a, float4 *q) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    q[3 * b + c] = node(a, b, c);
+        };
+        auto add = [&](int a, const float4 *q) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+            {
+                const float wab = W.w[a][0] * W.w[b][1];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                {
+                    const V3 nw = v3(q[3 * b + c]) * (wab * W.w[c][2]);
+                    sx[a] = sx[a] + nw;
+                    sy[b] = sy[b] + nw;
+                    sz[c] = sz[c] + nw;
+                }
+            }
+        };
+        load(0, q0);
+        load(1, q1);
+        __builtin_amdgcn_sched_barrier(0);
+        add(0, q0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(2, q0);
+        __builtin_amdgcn_sched_barrier(0);
+        add(1, q1);
+        add(2, q0);
+        v = (sx[0] + sx[1]) + sx[2];
+        auto col = [&](int k) {
+            auto comp = [&](const V3 *s3, int r) {
+                const float *p0 = &s3[0].x, *p1 = &s3[1].x, *p2 = &s3[2].x;
+                return o.d[r][0] * p0[k] + o.d[r][1] * p1[k] + o.d[r][2] * p2[k];
+            };
+            return V3{comp(sx, 0), comp(sy, 1), comp(sz, 2)};
+        };
+        B = M3{col(0), col(1), col(2)};
+    }
+    // The stencil from the tile box staged in LDS (G), base cell at local index l0.
+    CR_DEV void gather_separable(const float4 *G, int l0, const Weights &W, const AxisOffsets &o, V3 &v, M3 &B)
+    {
+        gather_separable_by([&](int a, int b, int c) { return G[node_li(l0, a, b, c)]; }, W, o, v, B);
     }
 
     // P2G terms (v + C dist) * w * m: C dist = c0 d_x + c1 d_y + c2 d_z, so
@@ -1624,6 +1690,65 @@ namespace
         cOut = aacc * (4.0f * g.invdx);
     }
 
+    // The same gather read straight from the scene's grid in global memory
+    // (the per-particle PB kernels, which stage no tile): the same nodes,
+    // sums and order as gather_vc.  Every in-grid node of a particle lies in
+    // its tile's box, so skipping out-of-grid nodes is the tile's test.
+    CR_DEV void gather_vc_grid(const StepCommon &c, int nSlicers, V3 x, V3 vp, V3 &vOut, M3 &cOut)
+    {
+        const GridGeom &g = c.g;
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        Weights W;
+        weights_of(fx, W);
+        V3 vacc{0, 0, 0};
+        M3 aacc{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
+        if (nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx)))
+        {
+            // node (x, y, z) is at block (x>>2, y>>2, z>>2) * 64 + (z&3) * 16 + (y&3) * 4 + (x&3):
+            // a sum of per-axis parts
+            int ox[3], oy[3], oz[3];
+            const int rowY = g.nbx * 64, rowZ = g.nbx * g.nby * 64;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const int X = bx + i, Y = by + i, Z = bz + i;
+                ox[i] = ((X >> 2) << 6) + (X & 3);
+                oy[i] = (Y >> 2) * rowY + ((Y & 3) << 2);
+                oz[i] = (Z >> 2) * rowZ + ((Z & 3) << 4);
+            }
+            const float4 *grid = c.grid;
+            gather_separable_slices([&](int a, int b, int cc) { return grid[ox[a] + oy[b] + oz[cc]]; }, W,
+                                    axis_offsets(fx, g.dx), vacc, aacc);
+        }
+        else
+        {
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+                for (int b = 0; b < 3; ++b)
+#pragma unroll 1
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const int nx = bx + a, ny = by + b, nz = bz + cc;
+                        if (!in_grid(g, nx, ny, nz))
+                            continue;
+                        const V3 dist = (V3{(float)a, (float)b, (float)cc} - fx) * g.dx;
+                        const float wt = w_of(fx.x, a) * w_of(fx.y, b) * w_of(fx.z, cc);
+                        V3 nv = v3(c.grid[grid_index(g, nx, ny, nz)]);
+                        V3 n;
+                        if (nSlicers > 0 && different_sides<true>(c.sv, nSlicers, x, x + dist, n))
+                            nv = vp - n * dot(vp, n) * 2.0f;
+                        const V3 nw = nv * wt;
+                        vacc = vacc + nw;
+                        aacc = aacc + outer_ref(nw, dist);
+                    }
+        }
+        vOut = vacc;
+        cOut = aacc * (4.0f * g.invdx);
+    }
+
     CR_DEV V3 clamp_bounds(const GridGeom &g, V3 x)
     {
         return V3{fminf(fmaxf(x.x, g.bmin.x), g.bmax.x), fminf(fmaxf(x.y, g.bmin.y), g.bmax.y),
@@ -1733,33 +1858,31 @@ namespace
         int bx, by, bz;
         V3 fx;
         cell_of(g, x, bx, by, bz, fx);
-        Weights W;
-        weights_of(fx, W);
         const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
-        const ScatterTerms terms = scatter_terms(vel, Cm, m, W, axis_offsets(fx, g.dx));
         const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
+        // The weights and per-node terms are formed inside each pass (a
+        // pblock takes one or two) from inputs made opaque there, so only the
+        // 16 inputs stay live across passes, not the 36 separable terms.
+        M3 cm = Cm;
+        V3 v = vel, f = fx;
         in_conflict_free_passes<Tiles>(cell_key(ti, bx, by, bz), [&](int tile) {
             Acc *A = A0 + tile * kTileNodes;
-            if (fast)
-            {
-                ScatterTerms tt = terms;
-                opaque(tt);
-                scatter_separable(A, l0, tt, 1.0f);
-                return;
-            }
-            M3 cm = Cm;
-            V3 v = vel, f = fx;
-            Weights w = W;
             opaque(cm);
             opaque(v);
             opaque(f);
-            opaque(w);
+            Weights W;
+            weights_of(f, W);
+            if (fast)
+            {
+                scatter_separable(A, l0, scatter_terms(v, cm, m, W, axis_offsets(f, g.dx)), 1.0f);
+                return;
+            }
             auto scatter = [&](int li, int, int, int, V3 dist, float wt) {
                 const float wm = wt * m;
                 const V3 contrib = (v + mul(cm, dist)) * wm;
                 lds_add(A, li, contrib.x, contrib.y, contrib.z, wm);
             };
-            for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+            for_each_node<false>(g, ti, bx, by, bz, f, W, [&](int li, int a, int b, int cc, V3 dist, float wt) {
                 V3 nrm;
                 if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
                     return;
@@ -1768,23 +1891,112 @@ namespace
         });
     }
 
-    // Mode 0: P2G of the first iteration.  Mode 1: G2P of the previous
-    // iteration fused with P2G of this one.  Mode 2: last G2P fused with
-    // pbMpmIntegrateParticle, results written back in particle order.
-    // Both scatter into 3 accumulation tiles: Mode 0 stages no grid tile,
-    // and Mode 1's momentum-only tiles (12 bytes per node) leave room for a
-    // third within the LDS of 3 wavefronts per SIMD.  Positions and masses are
-    // fixed over a substep's iterations, so every iteration scatters the same
-    // node masses: Mode 0 stores them with the momenta in each pblock's partial
-    // box, Mode 1 accumulates and stores momentum only (12 of 16 bytes per
-    // node) and leaves Mode 0's masses in the boxes for the grid update.
-    template <int Mode, bool Persistent>
-    __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_pb_tile(BatchArgs b, PbStage st)
+    // ------------------------------------------------------------------
+    // PB iterations split by where the lanes are full.  A slab-like body
+    // holds about 37 particles per pblock, so a wavefront that takes one
+    // pblock keeps 57% of its lanes busy, and the per-particle work (the
+    // gather and above all the SVD-based constraint) dominates.  So each
+    // iteration runs the per-particle half over 64 consecutive sorted
+    // particles per wavefront (k_pb_dense: every lane busy, the stencil read
+    // from the grid in global memory, no LDS) and the scatter half per pblock
+    // (k_pb_scatter: LDS accumulation tiles only, few registers).  The staged
+    // v and C carry the result from one to the other.  Every sum keeps the
+    // order it has when one wavefront runs a pblock's whole iteration (the
+    // stencil in the same separable order, the scatter in the same
+    // conflict-free passes), and the split measures bitwise equal to that
+    // form: 42.2 us per iteration there, 16.5 + 19.5 us here on the slab.
+    //   Mode 0: the first iteration's constraint (pbMpmParticleToGrid's
+    //           solveMaterialConstraints on the staged C).
+    //   Mode 1: G2P of iteration k-1 (pbMpmGridToParticle) and the
+    //           constraint of iteration k.
+    //   Mode 2: the last G2P and pbMpmIntegrateParticle, written back in
+    //           particle order (src/solver/pb_mpm_algorithm.h:150-408).
+    // ------------------------------------------------------------------
+    template <int Mode>
+    CR_DEV void pb_dense_particle(const StepCommon &c, const PbStage &st, int k)
+    {
+        const int S = st.stride;
+        if (Mode == 0)
+        {
+            const float2 pr = st.ps[k];
+            const int type = st.ts[k];
+            M3 Cm = load_m3(st.Cs, S, k);
+            const M3 F = load_m3(st.Fs, S, k);
+            pb_constraint(c.dt, pr.x, pr.y, type, F, Cm);
+            store_m3(st.Cs, S, k, Cm);
+            return;
+        }
+        const float4 xm = st.xs[k];
+        V3 x = v3(xm);
+        const V3 vp{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
+        V3 vel;
+        M3 Cm;
+        gather_vc_grid(c, slicer_count(c.sv), x, vp, vel, Cm);
+        // F and the material are loaded after the gather: held across it
+        // (its slicer path is the kernel's register peak) they cost a
+        // wavefront per SIMD (155 against 127 registers; 16.4 against 14.4 us
+        // on the slab)
+        const M3 F = load_m3(st.Fs, S, k);
+        if (Mode == 1)
+        {
+            const float2 pr = st.ps[k];
+            const int type = st.ts[k];
+            pb_constraint(c.dt, pr.x, pr.y, type, F, Cm);
+            st.vs[k] = vel.x;
+            st.vs[S + k] = vel.y;
+            st.vs[2 * S + k] = vel.z;
+            store_m3(st.Cs, S, k, Cm);
+        }
+        else
+        {
+            // pbMpmIntegrateParticle (src/solver/pb_mpm_algorithm.h:363-408)
+            const int p = (int)c.order[k];
+            x = clamp_bounds(c.g, x + vel * c.dt);
+            vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+            float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
+            resolve_rigid<true>(c.sv, shape_count(c.sv), vv, x);
+            c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+            c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+            store_cols(c.S.C, p, Cm);
+            store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
+        }
+    }
+
+    // One thread per sorted particle; n sort items, keys sorted (masked and
+    // inactive particles, keyed past the last pblock, come last).  A
+    // wavefront whose particles belong to several scenes of a batch takes
+    // them one scene at a time, so scene records stay wavefront-uniform.
+    template <int Mode>
+    __global__ __launch_bounds__(kWG) void k_pb_dense(BatchArgs b, PbStage st, int n, const unsigned *keys)
+    {
+        const int k = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = k < n ? keys[k] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const int s = b.numScenes == 1 ? 0 : b.blockScene[key];
+        for (;;)
+        {
+            const int s0 = __builtin_amdgcn_readfirstlane(s);
+            if (s == s0)
+            {
+                pb_dense_particle<Mode>(scene_view_of(b, s0), st, k);
+                return;
+            }
+        }
+    }
+
+    // The scatter half of an iteration: one wavefront per pblock tile (or
+    // sub-tile), the staged x, v and C (the constraint's output) scattered
+    // into 3 LDS accumulation tiles in conflict-free passes.  Mass: the first
+    // iteration, which also accumulates the node masses (16-byte nodes);
+    // later iterations accumulate momentum only (12-byte nodes) and leave the
+    // first iteration's masses in the partial boxes.
+    template <bool Mass, bool Persistent>
+    __global__ __launch_bounds__(kWG) void k_pb_scatter(BatchArgs b, PbStage st)
     {
         constexpr int kAcc = 3;
-        using Acc = typename std::conditional<Mode == 0, float4, F3>::type;
-        __shared__ Acc acc[(Mode < 2) ? kWaves * kAcc : 1][kTileNodes];
-        __shared__ float4 lg[(Mode > 0) ? kWaves : 1][kTileNodes];
+        using Acc = typename std::conditional<Mass, float4, F3>::type;
+        __shared__ Acc acc[kWaves * kAcc][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
@@ -1797,72 +2009,20 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             wave_sync(); // the previous tile's LDS reads are done
-            Acc *A = acc[(Mode < 2) ? wave * kAcc : 0];
-            float4 *G = lg[(Mode > 0) ? wave : 0];
-            if (Mode < 2)
-                zero_tiles<kAcc>(A, lane);
-            if (Mode > 0)
-                load_tile_grid(c.g, ti, c.grid, G, lane);
+            Acc *A = acc[wave * kAcc];
+            zero_tiles<kAcc>(A, lane);
             wave_sync();
             const int nSlicers = slicer_count(c.sv);
-            const int nShapes = shape_count(c.sv);
             const int S = st.stride;
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
                 const float4 xm = st.xs[k];
-                V3 x = v3(xm);
-                V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
-                // the constraint's inputs are requested with x and v, so their
-                // latency overlaps the gather's (the material type decides a
-                // branch: loaded after the gather it would cost a second wait)
-                const M3 F = load_m3(st.Fs, S, k);
-                float2 pr;
-                int type = 0, p = 0;
-                if (Mode < 2)
-                {
-                    pr = st.ps[k];
-                    type = st.ts[k];
-                }
-                else
-                    p = (int)c.order[k];
-                M3 Cm;
-                if (Mode == 0)
-                    Cm = load_m3(st.Cs, S, k);
-                else
-                {
-                    V3 vnew;
-                    gather_vc(c, ti, G, nSlicers, x, vel, vnew, Cm);
-                    vel = vnew;
-                }
-                if (Mode < 2)
-                {
-                    pb_constraint(c.dt, pr.x, pr.y, type, F, Cm);
-                    if (Mode == 1 && nSlicers > 0)
-                    {
-                        st.vs[k] = vel.x;
-                        st.vs[S + k] = vel.y;
-                        st.vs[2 * S + k] = vel.z;
-                    }
-                    pb_scatter<kAcc>(c, ti, A, nSlicers, x, xm.w, vel, Cm);
-                }
-                else
-                {
-                    // pbMpmIntegrateParticle (src/solver/pb_mpm_algorithm.h:363-408)
-                    x = clamp_bounds(c.g, x + vel * c.dt);
-                    vel = vel + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                    float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                    resolve_rigid<true>(c.sv, nShapes, vv, x);
-                    c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
-                    c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-                    store_cols(c.S.C, p, Cm);
-                    store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
-                }
+                const V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
+                const M3 Cm = load_m3(st.Cs, S, k);
+                pb_scatter<kAcc>(c, ti, A, nSlicers, v3(xm), xm.w, vel, Cm);
             }
-            if (Mode < 2)
-            {
-                wave_sync();
-                store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
-            }
+            wave_sync();
+            store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
         }
     }
 
@@ -2945,15 +3105,19 @@ namespace
             }
             else
             {
-                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_tile<0, true> : k_pb_tile<0, false>, tileWGs, kWG, a, b->st);
+                const int denseWGs = grid1d(n, kWG);
+                launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
+                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true> : k_pb_scatter<true, false>, tileWGs,
+                       kWG, a, b->st);
                 launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 for (int it = 1; it < K; ++it)
                 {
-                    launch(b, CRMPM_HIP_PROF_PB_FUSED, pers ? k_pb_tile<1, true> : k_pb_tile<1, false>, tileWGs, kWG, a,
-                           b->st);
+                    launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
+                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true> : k_pb_scatter<false, false>,
+                           tileWGs, kWG, a, b->st);
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 }
-                launch(b, CRMPM_HIP_PROF_PB_G2P, pers ? k_pb_tile<2, true> : k_pb_tile<2, false>, tileWGs, kWG, a, b->st);
+                launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_dense<2>, denseWGs, kWG, a, b->st, n, keys);
             }
         }
         return check_launch();

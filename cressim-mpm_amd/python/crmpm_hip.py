@@ -161,7 +161,7 @@ EXPORTS = {
 }
 
 PROF_CATEGORIES = ("pb_fused", "pb_p2g", "pb_g2p_integrate", "update", "mls_p2g", "mls_g2p", "std_p2g", "std_g2p",
-                   "binning")
+                   "binning", "pb_scatter")
 
 _LIB = None
 

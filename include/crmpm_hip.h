@@ -167,8 +167,8 @@ int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax);
  * solver stream (at most maxLaunches launches are recorded).  End synchronizes
  * the stream and returns the summed milliseconds and launch counts per
  * category (CRMPM_HIP_PROF_*). */
-#define CRMPM_HIP_PROF_PB_FUSED 0  /* PB: G2P of iteration k-1 fused with P2G of iteration k */
-#define CRMPM_HIP_PROF_PB_P2G 1    /* PB: P2G of the first iteration */
+#define CRMPM_HIP_PROF_PB_FUSED 0  /* PB: G2P of iteration k-1 with the constraint of iteration k, per particle */
+#define CRMPM_HIP_PROF_PB_P2G 1    /* PB: the first iteration's constraint and P2G */
 #define CRMPM_HIP_PROF_PB_G2P 2    /* PB: last G2P fused with the particle integration */
 #define CRMPM_HIP_PROF_UPDATE 3    /* grid update, all solvers */
 #define CRMPM_HIP_PROF_MLS_P2G 4
@@ -176,7 +176,8 @@ int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax);
 #define CRMPM_HIP_PROF_STD_P2G 6
 #define CRMPM_HIP_PROF_STD_G2P 7
 #define CRMPM_HIP_PROF_BINNING 8   /* binning, radix sort, tiling, PB staging */
-#define CRMPM_HIP_PROF_CATEGORIES 9
+#define CRMPM_HIP_PROF_PB_SCATTER 9 /* PB: P2G scatter of iterations 1 .. K-1, per pblock */
+#define CRMPM_HIP_PROF_CATEGORIES 10
 int crHipSolverProfileBegin(CrHipSolver *s, int maxLaunches);
 int crHipSolverProfileEnd(CrHipSolver *s, float *msPerCategory, int *launchesPerCategory, int numCategories);
 /* Between ProfileBegin and ProfileEnd, record events only for launches whose
