@@ -110,7 +110,7 @@ def valu_issue_us(instructions):
 # rocprofv3 wraps (with the launch configuration the timed run's autotuner
 # picked, so no trial dispatch enters the statistics), the summary's title and
 # file name, and whether the counter passes belong to it or the kernel trace alone.
-PB_LAUNCH = "0,16384,1048576,1,1048576,1"
+PB_LAUNCH = "0,4096,1048576,1,1048576,1"
 MLS_CUTTING_LAUNCH = "0,16384,1048576,1,1048576,1"
 MLS_CUBE_LAUNCH = "0,0,128,8,128,8"
 PROFILE_TARGETS = {

@@ -3747,6 +3747,33 @@ namespace
         return f;
     }
 
+    // svd3 on n column-major matrices, one per thread (crHipSvd3).
+    __global__ __launch_bounds__(kWG) void k_svd3(const float *A9, int n, float *U9, float *Vt9, float *S3)
+    {
+        const int i = blockIdx.x * kWG + threadIdx.x;
+        if (i >= n)
+            return;
+        const float *a = A9 + 9 * (size_t)i;
+        const M3 A{V3{a[0], a[1], a[2]}, V3{a[3], a[4], a[5]}, V3{a[6], a[7], a[8]}};
+        M3 u, vt;
+        V3 sig;
+        svd3(A, u, vt, sig);
+        float *uo = U9 + 9 * (size_t)i, *vo = Vt9 + 9 * (size_t)i;
+        const V3 *uc = &u.c0, *vc = &vt.c0;
+        for (int c = 0; c < 3; ++c)
+        {
+            uo[3 * c] = uc[c].x;
+            uo[3 * c + 1] = uc[c].y;
+            uo[3 * c + 2] = uc[c].z;
+            vo[3 * c] = vc[c].x;
+            vo[3 * c + 1] = vc[c].y;
+            vo[3 * c + 2] = vc[c].z;
+        }
+        S3[3 * (size_t)i] = sig.x;
+        S3[3 * (size_t)i + 1] = sig.y;
+        S3[3 * (size_t)i + 2] = sig.z;
+    }
+
     __global__ __launch_bounds__(kWG) void k_lattice_count(LatticeArgs a, unsigned *blockCounts)
     {
         const long long first = (long long)blockIdx.x * kLatticeBlock + (long long)threadIdx.x * kLatticePts;
@@ -4691,6 +4718,16 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
     return rc;
 }
 
+
+int crHipSvd3(const float *A9, int n, float *U9, float *Vt9, float *S3, void *stream)
+{
+    if (n < 0 || (n > 0 && (!A9 || !U9 || !Vt9 || !S3)))
+        return CRMPM_HIP_ERR_ARG;
+    if (n == 0)
+        return CRMPM_HIP_OK;
+    k_svd3<<<grid1d(n, kWG), kWG, 0, (hipStream_t)stream>>>(A9, n, U9, Vt9, S3);
+    return check_launch();
+}
 
 int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeometrySdfData *sdf, const float boundsMin[3],
                          const float boundsMax[3], float particleSpacing, const float position[3],

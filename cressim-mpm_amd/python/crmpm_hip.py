@@ -50,6 +50,7 @@ class SceneDesc(ctypes.Structure):
 _f4 = ctypes.c_float * 4
 
 EXPORTS = {
+    "crHipSvd3": [_vp, ctypes.c_int, _vp, _vp, _vp, _vp],
     "crHipParticleLattice": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3, _f4, _f3,
                              ctypes.c_float, _vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint), _vp],
     "crHipParticleLatticeNoAlloc": [ctypes.c_int, _f4, ctypes.POINTER(GeometrySdfData), _f3, _f3, ctypes.c_float, _f3,
@@ -216,6 +217,28 @@ def particle_lattice(obj, device="cuda:0", capacity=None, no_alloc=False):
     out = torch.empty((max(m, 1), 4), dtype=torch.float32, device=dev)
     check(fn(*args, _vp(out.data_ptr()), m, ctypes.byref(n), _vp(stream)), "particle lattice")
     return n.value, out[:m]
+
+
+def svd3(A, device="cuda:0"):
+    """The device 3x3 SVD (crHipSvd3) of row-major matrices A (n, 3, 3):
+    returns U, Vt (n, 3, 3) and S (n, 3) as numpy float32 arrays with
+    U diag(S) Vt = A^T, as the reference's svd() factors the transpose."""
+    import torch
+
+    lib = load_library()
+    A = np.asarray(A, np.float32)
+    n = len(A)
+    dev = torch.device(device)
+    a = torch.as_tensor(np.ascontiguousarray(A.transpose(0, 2, 1).reshape(n, 9)), device=dev)
+    U = torch.empty((max(n, 1), 9), dtype=torch.float32, device=dev)
+    Vt = torch.empty_like(U)
+    S = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    check(lib.crHipSvd3(_vp(a.data_ptr()), n, _vp(U.data_ptr()), _vp(Vt.data_ptr()), _vp(S.data_ptr()), _vp(stream)),
+          "svd3")
+    torch.cuda.synchronize(dev)
+    to = lambda t: t[:n].cpu().numpy().reshape(n, 3, 3).transpose(0, 2, 1)  # noqa: E731
+    return to(U), to(Vt), S[:n].cpu().numpy()
 
 
 def load_obj(path):

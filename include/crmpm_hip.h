@@ -465,6 +465,16 @@ int crHipParticleLattice(int geomType, const float params0[4], const CrHipGeomet
                          const float boundsMax[3], float particleSpacing, const float position[3],
                          const float rotation[4], const float invScale[3], float particleMass, float *positionMassOut,
                          unsigned int capacity, unsigned int *numParticles, void *stream);
+/* The device 3x3 SVD that PB-MPM's material constraint runs per particle
+ * and iteration, on n matrices: svd() of include/math/svd3.h:1265-1275
+ * (McAdams et al., 4 Jacobi sweeps), which factors the transpose:
+ * u diag(sigma) vt = A^T.  Matrices are column-major, element (r, c) of
+ * matrix i at [9 i + 3 c + r], the reference Mat3x3f's element order; A9,
+ * U9, Vt9 and S3 (n x 3) are device arrays.  The reference runs this only
+ * inside its solvers; the entry point exists so the device SVD can be
+ * checked on its own.  Asynchronous on stream.  Errors: CRMPM_HIP_ERR_ARG
+ * for n < 0 or a missing array. */
+int crHipSvd3(const float *A9, int n, float *U9, float *Vt9, float *S3, void *stream);
 /* crHipParticleLattice without any allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under
  * stream capture, tight loops, harnesses that forbid hipMalloc).  Same
