@@ -1433,8 +1433,6 @@ namespace
                 int bx, by, bz;
                 V3 fx;
                 cell_of(g, x, bx, by, bz, fx);
-                Weights W;
-                weights_of(fx, W);
                 // v += g dt before the scatter (Q9).  The reference writes it
                 // back; only G2P's slicer reflection reads it before G2P
                 // overwrites v, so G2P forms the same sum instead and P2G
@@ -1445,25 +1443,26 @@ namespace
                 const M3 affine = stress * sc + Cm * m;
                 const V3 vm = vel * m;
                 const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
-                // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
-                const ScatterTerms terms = scatter_terms(vm, affine, 1.0f, W, axis_offsets(fx, g.dx));
                 const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
+                // The weights and the separable terms are formed inside each
+                // pass from inputs made opaque there (a chunk takes one pass
+                // in the rank-pair order), so 15 inputs stay live across the
+                // passes instead of the 36 terms.
+                M3 af = affine;
+                V3 v_m = vm, f = fx;
                 in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
                     float4 *A = A0 + tile * kTileNodes;
-                    if (fast)
-                    {
-                        ScatterTerms tt = terms;
-                        opaque(tt);
-                        scatter_separable(A, l0, tt, m);
-                        return;
-                    }
-                    M3 af = affine;
-                    V3 v_m = vm, f = fx;
-                    Weights w = W;
                     opaque(af);
                     opaque(v_m);
                     opaque(f);
-                    opaque(w);
+                    Weights w;
+                    weights_of(f, w);
+                    if (fast)
+                    {
+                        // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
+                        scatter_separable(A, l0, scatter_terms(v_m, af, 1.0f, w, axis_offsets(f, g.dx)), m);
+                        return;
+                    }
                     auto sc = [&](int li, int, int, int, V3 dist, float wt) {
                         const V3 contrib = (v_m + mul(af, dist)) * wt;
                         lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
