@@ -2445,10 +2445,13 @@ namespace
     }
 
     // Grid adjoint: sum the tile partials of the velocity adjoint (fixed
-    // order), then through u = p / m (or u = p below the mass cutoff) to the
-    // momentum and mass adjoints (p_bar xyz, m_bar w).
+    // order), then through u = p / m (or u = p below the mass cutoff: 1e-4
+    // for MLS, Kind 0; 1e-8 for PB, Kind 1) to the momentum and mass
+    // adjoints (p_bar xyz, m_bar w).
+    template <int Kind>
     __global__ __launch_bounds__(kWG) void k_adj_grid(BatchArgs b, const float4 *grid, float4 *gridAdj)
     {
+        const float cutoff = Kind == 1 ? 1e-8f : 1e-4f;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg += tile_stride())
         {
@@ -2489,7 +2492,7 @@ namespace
             const size_t gi = (size_t)(c.grid - b.grid) + (size_t)nb * 64 + lane;
             const float4 u = grid[gi];
             float4 r;
-            if (u.w < 1e-4f)
+            if (u.w < cutoff)
                 r = make_float4(ub.x, ub.y, ub.z, 0.0f);
             else
             {
@@ -2677,6 +2680,317 @@ namespace
             }
         }
     }
+
+    // ------------------------------------------------------------------
+    // Adjoint of the PB-MPM substep (reverse mode), one scene without rigid
+    // shapes.  Forward (src/solver/pb_mpm_algorithm.h:150-408; positions,
+    // and so weights and node masses M, are fixed over the K iterations):
+    //   iteration k = 1..K   Cc = constraint(C_{k-1}, F)                (per particle)
+    //                        P_i = sum_p w m (v_{k-1} + Cc d),  U_i = P_i / M_i with the
+    //                        wall band zeroed, or P_i where M_i < 1e-8
+    //                        v_k = sum_i w U_i,  C_k = (4 / dx) sum_i w d U_i^T
+    //   integration          x+ = clamp(x + dt v_K), v+ = v_K + g dt,
+    //                        F+ = (I + dt C_K) F, C+ = C_K
+    // The constraint (constitutive_models.h:123-154): T = (I + dt C) F,
+    // R = u vt of svd3(T) (the polar rotation of T^T), Q = T / s with
+    // s = sign(det T) sqrt(clamp(|det T|, 0.1, 1000)), B = r R + (1 - r) Q,
+    // Cc = C + alpha ((B F^-T - I) - dt C).
+    // The driver re-runs each substep with every iteration's grid U_k kept
+    // (the tape), so v_{k-1} and C_{k-1} are re-gathered exactly as the
+    // forward formed them.  The reverse kernels work in the substep's tile
+    // order on its staged particle arrays (x|m, F, (alpha, r), type):
+    //   k_pbadj_integrate    the integration's adjoint, per particle
+    //   k = K..1             k_pbadj_g2p (per tile: the velocity adjoint
+    //                        scattered into tile partials, the weights' x
+    //                        adjoint), k_adj_grid<1> (momentum and mass
+    //                        adjoints), k_pbadj_p2g (per tile: those gathered,
+    //                        then the constraint's adjoint through the SVD)
+    //   k_pbadj_finalize     the adjoint of F^-T; back to particle order
+    // The constraint's adjoint runs in FP64, and the polar rotation's
+    // derivative divides by sigma_i + sigma_j there.
+    // ------------------------------------------------------------------
+
+    // Adjoint state of the substep being reversed, in tile (sorted) order.
+    struct PbAdjRef
+    {
+        float4 *x;  // x adjoint (xyz), accumulated over the substep
+        float4 *v;  // adjoint of v_k
+        float4 *C;  // adjoint of C_k, 3 columns
+        float4 *F;  // adjoint of F, 3 columns, accumulated
+        float4 *Fi; // adjoint of F^-T, 3 columns, accumulated
+    };
+
+    CR_DEV D3 d3_scale(const D3 &a, double s) { return d3_comb(a, s, a, 0.0, 0.0); }
+    CR_DEV D3 d3_add(const D3 &a, const D3 &b) { return d3_comb(a, 1.0, b, 1.0, 0.0); }
+    // Cofactor matrix: d det(T) / dT.
+    CR_DEV D3 d3_cof(const D3 &t)
+    {
+        D3 c;
+        c.d[0][0] = t.d[1][1] * t.d[2][2] - t.d[1][2] * t.d[2][1];
+        c.d[0][1] = t.d[1][2] * t.d[2][0] - t.d[1][0] * t.d[2][2];
+        c.d[0][2] = t.d[1][0] * t.d[2][1] - t.d[1][1] * t.d[2][0];
+        c.d[1][0] = t.d[0][2] * t.d[2][1] - t.d[0][1] * t.d[2][2];
+        c.d[1][1] = t.d[0][0] * t.d[2][2] - t.d[0][2] * t.d[2][0];
+        c.d[1][2] = t.d[0][1] * t.d[2][0] - t.d[0][0] * t.d[2][1];
+        c.d[2][0] = t.d[0][1] * t.d[1][2] - t.d[0][2] * t.d[1][1];
+        c.d[2][1] = t.d[0][2] * t.d[1][0] - t.d[0][0] * t.d[1][2];
+        c.d[2][2] = t.d[0][0] * t.d[1][1] - t.d[0][1] * t.d[1][0];
+        return c;
+    }
+
+    // Adjoint of pb_constraint for an active material: Ccb is the adjoint of
+    // the constraint's output; returns the adjoint of its input C and adds
+    // to those of F, of F^-T (Fi) and of alpha and the ratio.  The forward's
+    // intermediates are recomputed in FP32 exactly as pb_constraint forms
+    // them (the same SVD).
+    CR_DEV D3 pb_constraint_adjoint(float dt, float alpha, float ratio, const M3 &F, const M3 &Fi, const M3 &C,
+                                    const D3 &Ccb, D3 &Fb, D3 &Fib, double &ab, double &rb)
+    {
+        const M3 T = mul(m3_identity() + C * dt, F);
+        M3 u, vt;
+        V3 sig;
+        svd3(T, u, vt, sig);
+        const float detT = det(T);
+        const float adet = fabsf(detT);
+        const float cl = fminf(fmaxf(adet, 0.1f), 1000.0f);
+        const float s = copysignf(1.0f, detT) * sqrtf(cl);
+        const M3 Q = T * (1.0f / s);
+        const M3 R = mul(u, vt);
+        const M3 B = R * ratio + Q * (1.0f - ratio);
+        const D3 Bd = d3_of(B), Fid = d3_of(Fi), Cd = d3_of(C), Td = d3_of(T), Rd = d3_of(R), Qd = d3_of(Q);
+        // Cc = C + alpha ((B Fi - I) - dt C)
+        ab += d3_dot(Ccb, d3_comb(d3_mul(Bd, Fid), 1.0, Cd, -(double)dt, -1.0));
+        const D3 Bb = d3_scale(d3_mul(Ccb, d3_t(Fid)), alpha);
+        Fib = d3_add(Fib, d3_scale(d3_mul(d3_t(Bd), Ccb), alpha));
+        // B = r R + (1 - r) Q
+        rb += d3_dot(Bb, d3_comb(Rd, 1.0, Qd, -1.0, 0.0));
+        const D3 Rb = d3_scale(Bb, ratio), Qb = d3_scale(Bb, 1.0 - (double)ratio);
+        // Q = T / s, s = sign(d) sqrt(clamp(|d|, 0.1, 1000)), d = det T
+        const double sd = s;
+        D3 Tb = d3_scale(Qb, 1.0 / sd);
+        if (adet > 0.1f && adet < 1000.0f)
+        {
+            const double sb = -d3_dot(Qb, Td) / (sd * sd);
+            Tb = d3_add(Tb, d3_scale(d3_cof(Td), sb * 0.5 / sqrt((double)adet)));
+        }
+        // R = U V^T for T^T = U diag(sigma) V^T (U = u, V^T = vt):
+        // dL/d(T^T) = U K V^T, K_ij = (G_ij - G_ji) / (sigma_i + sigma_j), G = U^T Rb V
+        const D3 Ud = d3_of(u), Vtd = d3_of(vt);
+        const D3 G = d3_mul(d3_mul(d3_t(Ud), Rb), d3_t(Vtd));
+        const double sg[3] = {sig.x, sig.y, sig.z};
+        D3 K = d3_zero();
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (i != j)
+                {
+                    const double den = sg[i] + sg[j];
+                    K.d[i][j] = fabs(den) > 1e-12 ? (G.d[i][j] - G.d[j][i]) / den : 0.0;
+                }
+        Tb = d3_add(Tb, d3_t(d3_mul(d3_mul(Ud, K), Vtd)));
+        // T = (I + dt C) F
+        const D3 Fd = d3_of(F);
+        Fb = d3_add(Fb, d3_mul(d3_t(d3_comb(Cd, (double)dt, Cd, 0.0, 1.0)), Tb));
+        return d3_add(d3_scale(Ccb, 1.0 - (double)alpha * dt), d3_scale(d3_mul(Tb, d3_t(Fd)), dt));
+    }
+
+    // Adjoint of the integration (pbMpmIntegrateParticle without shapes),
+    // one thread per sorted particle: the adjoint of the substep's end state
+    // (particle order, ad) to that of (v_K, C_K) and the direct x and F
+    // terms (tile order, pa).  The live particle state is the substep's end
+    // state; F is the staged start F.
+    __global__ void k_pbadj_integrate(BatchArgs b, PbStage st, int n, const unsigned *keys, AdjRef ad, PbAdjRef pa)
+    {
+        const int q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n || keys[q] >= (unsigned)b.totalBlocks)
+            return;
+        const StepCommon c = scene_view_of(b, b.numScenes == 1 ? 0 : b.blockScene[keys[q]]);
+        const GridGeom &g = c.g;
+        const int p = (int)c.order[q];
+        const V3 xe = v3(c.P.xm[p]);
+        const M3 CK = load_cols(c.S.C, p);
+        const M3 F = load_m3(st.Fs, st.stride, q);
+        const V3 xb = v3(ad.x[p]), vb = v3(ad.v[p]);
+        const M3 Fb = load_cols(ad.F, p), Cb = load_cols(ad.C, p);
+        // x+ = clamp(x + dt v_K): the adjoint passes where the clamp is inactive
+        const V3 keep{(xe.x > g.bmin.x && xe.x < g.bmax.x) ? 1.f : 0.f, (xe.y > g.bmin.y && xe.y < g.bmax.y) ? 1.f : 0.f,
+                      (xe.z > g.bmin.z && xe.z < g.bmax.z) ? 1.f : 0.f};
+        const V3 xbk = xb * keep;
+        const V3 vK = vb + xbk * c.dt;
+        pa.x[q] = make_float4(xbk.x, xbk.y, xbk.z, 0.0f);
+        pa.v[q] = make_float4(vK.x, vK.y, vK.z, 0.0f);
+        // F+ = (I + dt C_K) F, C+ = C_K
+        store_cols(pa.C, q, Cb + mul(Fb, transpose(F)) * c.dt);
+        store_cols(pa.F, q, mul(transpose(m3_identity() + CK * c.dt), Fb));
+        const M3 z{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
+        store_cols(pa.Fi, q, z);
+    }
+
+    // G2P adjoint of iteration k: U is the iteration's updated grid (the
+    // tape); the adjoint of (v_k, C_k) comes from pa; the velocity adjoint
+    // of the grid goes to the tile partials; the weights' x adjoint is added
+    // to pa.x.
+    __global__ __launch_bounds__(kWG) void k_pbadj_g2p(BatchArgs b, PbStage st, const float4 *U, PbAdjRef pa)
+    {
+        __shared__ float4 acc[kWaves][kTileNodes];
+        __shared__ float4 lg[kWaves][kTileNodes];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        {
+            if (!subtile_used(b, tg))
+                continue;
+            int t;
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
+            TileInfo ti;
+            if (!subtile_info(c, t, tg % b.subMax, ti))
+                continue;
+            const GridGeom &g = c.g;
+            float4 *A = acc[wave];
+            float4 *G = lg[wave];
+            for (int i = lane; i < kTileNodes; i += 64)
+                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            load_tile_grid(g, ti, U + (c.grid - b.grid), G, lane);
+            wave_sync();
+            for (int q = ti.start + lane; q < ti.end; q += 64)
+            {
+                const V3 x = v3(st.xs[q]);
+                const V3 vb = v3(pa.v[q]);
+                const M3 Bb = load_cols(pa.C, q) * (4.0f * g.invdx);
+                const M3 BbT = transpose(Bb);
+                V3 xacc{0.f, 0.f, 0.f};
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, x, bx, by, bz, fx);
+                NodeW nw;
+                node_weights(fx, nw);
+                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
+                    for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
+                        const V3 u = v3(G[li]);
+                        const V3 ub = (vb + mul(BbT, dist)) * wt;
+                        lds_add(A, li, ub.x, ub.y, ub.z, 0.0f);
+                        const V3 Bu = mul(Bb, u);
+                        xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
+                    });
+                });
+                const V3 xs = v3(pa.x[q]) + xacc;
+                pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+            }
+            wave_sync();
+            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
+            for (int i = lane; i < kTileNodes; i += 64)
+                out[i] = A[i];
+        }
+    }
+
+    // P2G adjoint of iteration k: gathers the momentum and mass adjoints
+    // (gridAdj) and takes them through the scatter and the constraint.  The
+    // constraint's input (v_{k-1}, C_{k-1}) is re-gathered from the previous
+    // iteration's grid Uprev, or for k = 1 read from the substep's start
+    // state s0.  Writes the adjoint of (v_{k-1}, C_{k-1}) to pa; adds to the
+    // x, F and F^-T adjoints and, in FP64 per particle, to those of alpha
+    // (ad.lambdaBar) and the ratio (ad.muBar).
+    __global__ __launch_bounds__(kWG) void k_pbadj_p2g(BatchArgs b, PbStage st, const float4 *gridAdj,
+                                                       const float4 *Uprev, StateRef s0, PbAdjRef pa, AdjRef ad)
+    {
+        __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ float4 lu[kWaves][kTileNodes];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        {
+            if (!subtile_used(b, tg))
+                continue;
+            int t;
+            const StepCommon c = scene_view(b, tg / b.subMax, t);
+            TileInfo ti;
+            if (!subtile_info(c, t, tg % b.subMax, ti))
+                continue;
+            const GridGeom &g = c.g;
+            float4 *G = lg[wave];
+            float4 *H = lu[wave];
+            load_tile_grid(g, ti, gridAdj + (c.grid - b.grid), G, lane);
+            if (Uprev)
+                load_tile_grid(g, ti, Uprev + (c.grid - b.grid), H, lane);
+            wave_sync();
+            const int S = st.stride;
+            for (int q = ti.start + lane; q < ti.end; q += 64)
+            {
+                const float4 xm = st.xs[q];
+                const V3 x = v3(xm);
+                const float m = xm.w;
+                V3 vel;
+                M3 Cm;
+                if (Uprev)
+                    gather_vc(c, ti, H, 0, x, V3{0.f, 0.f, 0.f}, vel, Cm);
+                else
+                {
+                    const int p = (int)c.order[q];
+                    vel = v3(s0.v[p]);
+                    Cm = load_cols(s0.C, p);
+                }
+                const M3 F = load_m3(st.Fs, S, q);
+                const float2 pr = st.ps[q];
+                const int type = st.ts[q];
+                M3 Cc = Cm;
+                pb_constraint(c.dt, pr.x, pr.y, type, F, Cc);
+                const M3 CcT = transpose(Cc);
+                V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
+                D3 Ccb = d3_zero();
+                int bx, by, bz;
+                V3 fx;
+                cell_of(g, x, bx, by, bz, fx);
+                NodeW nw;
+                node_weights(fx, nw);
+                for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
+                    const float4 gq = G[li];
+                    const V3 pb = v3(gq);
+                    const float wm = wt * m;
+                    vpb = vpb + pb * wm;
+                    const double pr3[3] = {pb.x, pb.y, pb.z}, dc[3] = {dist.x, dist.y, dist.z};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc)
+                            Ccb.d[r][cc] += (double)wm * pr3[r] * dc[cc];
+                    const float wbar = m * (dot(pb, vel + mul(Cc, dist)) + gq.w);
+                    xacc = xacc + gw * wbar - mul(CcT, pb) * wm;
+                });
+                const V3 xs = v3(pa.x[q]) + xacc;
+                pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+                pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
+                D3 Cb = Ccb;
+                if (type == mNeoHookean || type == mCoRotational)
+                {
+                    D3 Fb = d3_of(load_cols(pa.F, q)), Fib = d3_of(load_cols(pa.Fi, q));
+                    double ab = 0.0, rb = 0.0;
+                    Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, inverse_t(F), Cm, Ccb, Fb, Fib, ab, rb);
+                    store_cols(pa.F, q, m3_of(Fb));
+                    store_cols(pa.Fi, q, m3_of(Fib));
+                    const int p = (int)c.order[q];
+                    ad.lambdaBar[p] += ab;
+                    ad.muBar[p] += rb;
+                }
+                store_cols(pa.C, q, m3_of(Cb));
+            }
+        }
+    }
+
+    // End of a substep's reverse: the adjoint of F^-T added to F's
+    // (d F^-T = -F^-T dF^T F^-T), and the substep's start-state adjoint
+    // written back in particle order.
+    __global__ void k_pbadj_finalize(BatchArgs b, PbStage st, int n, const unsigned *keys, PbAdjRef pa, AdjRef ad)
+    {
+        const int q = blockIdx.x * blockDim.x + threadIdx.x;
+        if (q >= n || keys[q] >= (unsigned)b.totalBlocks)
+            return;
+        const int p = (int)b.order[q];
+        const M3 Fi = inverse_t(load_m3(st.Fs, st.stride, q));
+        const M3 Fb = load_cols(pa.F, q) - mul(mul(Fi, transpose(load_cols(pa.Fi, q))), Fi);
+        ad.x[p] = pa.x[q];
+        ad.v[p] = pa.v[q];
+        store_cols(ad.F, p, Fb);
+        store_cols(ad.C, p, load_cols(pa.C, q));
+    }
 } // namespace
 
 // ----------------------------------------------------------------------
@@ -2778,8 +3092,12 @@ struct CrHipBatch
     // the allocated partial boxes per pblock)
     int subSize = 64, subMax = 1, subStride = 1;
     int gSubSize = 1 << 20, gSubMax = 1; // gather kernels' split
-    // tile-ordered particle permutation of the last enqueued substep
+    // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
+    const unsigned *lastKeys = nullptr;
+    // PB: when set, every iteration's updated grid is copied to
+    // pbTape + iteration * nodes (the adjoint's re-run of a substep)
+    float4 *pbTape = nullptr;
     const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
 
     // hipGraph replay of the substep
@@ -3074,6 +3392,7 @@ namespace
         if (rc)
             return rc;
         b->lastOrder = order;
+        b->lastKeys = keys;
         const BatchArgs a = batch_args(b, order);
         const int tileWGs = pblock_wgs(b->totalBlocks * b->subMax, b->tileWaves);
         const int gatherWGs = pblock_wgs(b->totalBlocks * b->gSubMax, b->tileWaves);
@@ -3105,17 +3424,28 @@ namespace
             else
             {
                 const int denseWGs = grid1d(n, kWG);
+                const size_t nodes = (size_t)b->totalBlocks * 64;
+                bool taped = true;
+                auto tape = [&](int it) {
+                    if (b->pbTape)
+                        taped = taped && hipMemcpyAsync(b->pbTape + (size_t)it * nodes, b->grid, nodes * sizeof(float4),
+                                                        hipMemcpyDeviceToDevice, b->stream) == hipSuccess;
+                };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
                 launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true> : k_pb_scatter<true, false>, tileWGs,
                        kWG, a, b->st);
                 launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
+                tape(0);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
                     launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true> : k_pb_scatter<false, false>,
                            tileWGs, kWG, a, b->st);
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
+                    tape(it);
                 }
+                if (!taped)
+                    return CRMPM_HIP_ERR_LAUNCH;
                 launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_dense<2>, denseWGs, kWG, a, b->st, n, keys);
             }
         }
@@ -3174,6 +3504,21 @@ namespace
             std::copy(key, key + 9, b->graphKey);
         }
         return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+    }
+
+    // One PB substep, launched eagerly, with every iteration's updated grid
+    // copied to tape (K grids of the batch's node count).
+    int batch_step_taped(CrHipBatch *b, float4 *tape)
+    {
+        int rc = validate_for_step(b);
+        if (!rc)
+            rc = upload_scenes(b);
+        if (rc)
+            return rc;
+        b->pbTape = tape;
+        rc = enqueue_step(b);
+        b->pbTape = nullptr;
+        return rc;
     }
 
     int batch_destroy(CrHipBatch *b)
@@ -4481,6 +4826,11 @@ struct CrHipAdjoint
     float4 *gridAdj = nullptr;
     float4 *xTmp = nullptr;
     float4 *FTmp = nullptr;
+    // PB: the iterations' grids of the substep being reversed, and its
+    // adjoint state in tile order
+    float4 *tape = nullptr;
+    int tapeIters = 0;
+    float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
 };
 
 namespace
@@ -4514,6 +4864,70 @@ namespace
     }
     StateRef state_ref(const StateBuf &s, int stride) { return StateRef{s.x, s.v, s.F, s.C, stride}; }
 
+    // Reverse sweep of a PB rollout.  Per segment: re-run it from its
+    // checkpoint keeping every substep's start state; then, last substep
+    // first, re-run the substep from its start state with the iterations'
+    // grids taped and apply the reverse kernels (tile order of that run).
+    int pb_reverse(CrHipAdjoint *a, const AdjRef &ad)
+    {
+        CrHipBatch *b = a->solver->b;
+        hipStream_t st = b->stream;
+        const size_t nodes = (size_t)b->totalBlocks * 64;
+        const int K = b->iterations;
+        if (K > a->tapeIters)
+        {
+            (void)hipStreamSynchronize(st);
+            dfree(a->tape);
+            a->tapeIters = 0;
+            if (dalloc(&a->tape, (size_t)K * nodes))
+                return CRMPM_HIP_ERR_ALLOC;
+            a->tapeIters = K;
+        }
+        const PbAdjRef pa{a->paX, a->paV, a->paC, a->paF, a->paFi};
+        const int stride = b->capacity;
+        int rc = CRMPM_HIP_OK;
+        for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
+        {
+            const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
+            rc = state_copy(b, a->ckpt[sgm], false);
+            for (int t = t0; t < t1 && !rc; ++t)
+            {
+                rc = state_copy(b, a->seg[t - t0], true);
+                if (!rc && t + 1 < t1)
+                    rc = batch_step(b);
+            }
+            for (int t = t1 - 1; t >= t0 && !rc; --t)
+            {
+                const int j = t - t0;
+                rc = state_copy(b, a->seg[j], false);
+                if (!rc)
+                    rc = batch_step_taped(b, a->tape);
+                const int n = b->totalActive;
+                if (rc || n == 0)
+                    continue;
+                BatchArgs args = batch_args(b, b->lastOrder);
+                args.gatherOrder = args.order;
+                args.subSize = kAdjSubSize;
+                args.subMax = b->subStride;
+                const int subWgs = grid1d(b->totalBlocks * args.subMax, kWaves);
+                const int blockWgs = grid1d(b->totalBlocks, kWaves);
+                const unsigned *keys = b->lastKeys;
+                k_pbadj_integrate<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, ad, pa);
+                for (int k = K; k >= 1; --k)
+                {
+                    const float4 *U = a->tape + (size_t)(k - 1) * nodes;
+                    k_pbadj_g2p<<<subWgs, kWG, 0, st>>>(args, b->st, U, pa);
+                    k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(args, U, a->gridAdj);
+                    k_pbadj_p2g<<<subWgs, kWG, 0, st>>>(args, b->st, a->gridAdj, k > 1 ? U - nodes : nullptr,
+                                                         state_ref(a->seg[j], stride), pa, ad);
+                }
+                k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, pa, ad);
+                rc = check_launch();
+            }
+        }
+        return rc;
+    }
+
     int adjoint_destroy(CrHipAdjoint *a)
     {
         if (a->solver && a->solver->b->initialized)
@@ -4542,6 +4956,12 @@ namespace
         dfree(a->gridAdj);
         dfree(a->xTmp);
         dfree(a->FTmp);
+        dfree(a->tape);
+        dfree(a->paX);
+        dfree(a->paV);
+        dfree(a->paC);
+        dfree(a->paF);
+        dfree(a->paFi);
         delete a;
         return CRMPM_HIP_OK;
     }
@@ -4554,8 +4974,9 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
     if (!s || !out || numSteps <= 0 || checkpointEvery <= 0)
         return CRMPM_HIP_ERR_ARG;
     CrHipBatch *b = s->b;
-    if (b->kind != CRMPM_HIP_MLS || !b->initialized)
+    if ((b->kind != CRMPM_HIP_MLS && b->kind != CRMPM_HIP_PB) || !b->initialized)
         return CRMPM_HIP_ERR_STATE;
+    const bool mls = b->kind == CRMPM_HIP_MLS;
     CrHipAdjoint *a = new (std::nothrow) CrHipAdjoint();
     if (!a)
         return CRMPM_HIP_ERR_ALLOC;
@@ -4576,7 +4997,9 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= state_alloc(a->ckpt[i], N);
     for (int i = 0; !rc && i < a->every + 1; ++i)
         rc |= state_alloc(a->seg[i], N);
-    for (int i = 0; !rc && i < a->every; ++i)
+    // the MLS reverse keeps each substep's grid and tile order from the
+    // segment's re-run; the PB reverse re-runs each substep with its tape
+    for (int i = 0; !rc && mls && i < a->every; ++i)
     {
         rc |= dalloc(&a->segGrid[i], nodes);
         rc |= dalloc(&a->segOrder[i], N);
@@ -4588,6 +5011,14 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->gridAdj, nodes);
         rc |= dalloc(&a->xTmp, N);
         rc |= dalloc(&a->FTmp, 3 * N);
+    }
+    if (!rc && !mls)
+    {
+        rc |= dalloc(&a->paX, N);
+        rc |= dalloc(&a->paV, N);
+        rc |= dalloc(&a->paC, 3 * N);
+        rc |= dalloc(&a->paF, 3 * N);
+        rc |= dalloc(&a->paFi, 3 * N);
     }
     if (rc)
     {
@@ -4660,7 +5091,9 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
         rc = CRMPM_HIP_ERR_LAUNCH;
     const AdjRef ad{xb, vb, Fb, Cb, a->xTmp, a->FTmp, dLdLambda, dLdMu, b->capacity};
     const int stride = b->capacity;
-    for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
+    if (!rc && b->kind == CRMPM_HIP_PB)
+        rc = pb_reverse(a, ad);
+    for (int sgm = a->segments - 1; b->kind == CRMPM_HIP_MLS && sgm >= 0 && !rc; --sgm)
     {
         const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
         // recompute the segment from its checkpoint, keeping every substep's
@@ -4702,7 +5135,7 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
             k_adj_g2p<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
                                                a->segGrid[j], ad);
-            k_adj_grid<<<blockWgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
+            k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
             k_adj_p2g<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
             rc = check_launch();
         }

@@ -922,11 +922,13 @@ class DifferentiableMls:
     V0 comes from computeInitialData on the initial positions once; every
     rollout restarts from those positions with F = I and C = 0."""
 
+    KIND = S.MLS
+
     def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0", tune=False):
         import torch
 
-        if scene["kind"] != S.MLS or scene["shape_list"]:
-            raise ValueError("the adjoint covers MLS scenes without rigid shapes")
+        if scene["kind"] != self.KIND or scene["shape_list"]:
+            raise ValueError(f"{type(self).__name__} covers scenes of solver kind {self.KIND} without rigid shapes")
         self.torch = torch
         self.solver = HipSolver(scene, device=device)
         self.solver.compute_initial_data()
@@ -1037,6 +1039,18 @@ class _LazyRollout:
 
 
 _MlsRollout = _LazyRollout
+
+
+class DifferentiablePb(DifferentiableMls):
+    """Differentiable rollout of one PB-MPM scene without rigid shapes: the
+    per-particle inputs are the constraint's relaxation alpha (params0.x)
+    and rotation ratio (params0.y) and the initial velocity; forward(alpha,
+    ratio, v0) gives the final positions and backward(dL/dx) returns
+    dL/dalpha, dL/dratio (float64) and dL/dv0.  The adjoint runs the SVD's
+    polar-rotation derivative (crHipAdjoint* on a PB solver); the solver
+    iterations are the scene's."""
+
+    KIND = S.PB
 
 
 _HIPRT = None

@@ -490,17 +490,22 @@ int crHipParticleLatticeNoAlloc(int geomType, const float params0[4], const CrHi
                                 unsigned int *numParticles, void *stream);
 
 /* ------------------------------------------------------------------------
- * Differentiable MLS-MPM (one scene, no rigid shapes).  The reference has no
- * adjoint; this is the differentiable form of its MLS substep
- * (src/solver/mls_mpm_algorithm.h:147-390, same arithmetic and quirks).
+ * Differentiable MLS-MPM and PB-MPM (one scene, no rigid shapes).  The
+ * reference has no adjoint; this is the differentiable form of its MLS
+ * substep (src/solver/mls_mpm_algorithm.h:147-390) and of its PB substep
+ * (src/solver/pb_mpm_algorithm.h:150-408 with the SVD-based constraint of
+ * src/solver/constitutive_models.h:123-154), same arithmetic and quirks.
  * Forward runs numSteps substeps from the solver's current state and keeps
  * the state every checkpointEvery substeps; Backward takes dL/dx (and
  * optionally dL/dv) of the final state, recomputes each segment from its
- * checkpoint and sweeps the adjoint kernels back to substep 0.  Outputs
- * (device arrays, particle order): dL/dlambda and dL/dmu per particle
- * (FP64, summed over the substeps), dL/dv and dL/dx of the initial state
+ * checkpoint and sweeps the adjoint kernels back to substep 0 (PB: each
+ * substep is re-run once more with its iterations' grids kept).  Outputs
+ * (device arrays, particle order): the gradients w.r.t. params0.x and
+ * params0.y per particle (FP64, summed over the substeps) -- dL/dlambda and
+ * dL/dmu for MLS, dL/dalpha and dL/dratio (the constraint's relaxation and
+ * rotation ratio) for PB -- and dL/dv and dL/dx of the initial state
  * (float4, xyz used; dLdx0 may be null).  Backward leaves the solver in the
- * state Forward ended in. */
+ * state Forward ended in.  Other solver kinds: CRMPM_HIP_ERR_STATE. */
 typedef struct CrHipAdjoint CrHipAdjoint;
 int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipAdjoint **out);
 int crHipAdjointForward(CrHipAdjoint *a);
