@@ -623,6 +623,90 @@ def adjoint(steps=100, every=10, cpu=True):
             "finite": finite}
 
 
+def pb_block_scene(steps=20):
+    """Differentiable PB block: the config-4 block's 65,536 particles, 96^3
+    grid and initial velocities as a Neo-Hookean PB-MPM body, 20 iterations,
+    dt = 2e-3, alpha = 0.5 / dt, ratio 0.9 (the slab's constraint
+    parameters)."""
+    sc, _ = S.config4(steps=steps)
+    sc["kind"] = S.PB
+    sc["iterations"] = 20
+    sc["dt"] = 2e-3
+    sc["params"][:, 0] = 0.5 / sc["dt"]
+    sc["params"][:, 1] = 0.9
+    sc["type"][:] = S.NEOHOOKEAN
+    return sc
+
+
+def pb_adjoint(steps=20, every=10):
+    """Differentiable PB block (pb_block_scene): forward + adjoint (the
+    constraint's adjoint through the SVD) against the plain solver's
+    forward, for the same substeps; the loss is the MSE to the final
+    positions plus 1 mm of Gaussian noise (the constraint's parameters
+    barely move the positions over 20 substeps, so a rollout with other
+    parameters would give a loss near zero)."""
+    import torch
+
+    import crmpm_hip as H
+
+    sc = pb_block_scene(steps)
+    sim = H.DifferentiablePb(sc, steps, checkpoint_every=every, tune=True)
+    dev = sim.solver.device
+    alpha = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
+    ratio = torch.as_tensor(sc["params"][:, 1], dtype=torch.float64, device=dev)
+    v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
+    n = sim.n
+    noise = torch.as_tensor(S.gaussian_noise(n, S.TARGET_NOISE_SEED, S.TARGET_NOISE), device=dev)
+    target = sim.forward(alpha, ratio, v0).double() + noise
+
+    def fwd():
+        return sim.forward(alpha, ratio, v0)
+
+    def fwd_adj():
+        x = sim.forward(alpha, ratio, v0)
+        return sim.backward(2.0 * (x.double() - target) / n)
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    tuned = sim.tuned
+    t_f = timed(fwd)
+    t_fa = timed(fwd_adj)
+    loss = float(((fwd().double() - target) ** 2).sum(1).mean())
+    da, dr, dv0 = fwd_adj()
+    finite = bool(torch.isfinite(da).all() and torch.isfinite(dr).all() and torch.isfinite(dv0).all())
+    sim.close()
+    plain = H.HipSolver(copy.deepcopy(sc))
+    plain.compute_initial_data()
+    plain.autotune(steps_per_trial=2)
+    plain.set_graph_mode(True)
+    plain.step(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.step(steps)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t0
+    plain.close()
+    return {"workload": "PB block 64x32x32 = 65,536 particles, 96^3 grid, Neo-Hookean, 20 iterations, dt 2e-3, "
+                        "loss MSE to the final positions plus 1 mm noise",
+            "particles": n, "substeps": steps, "iterations": 20, "checkpoint_every": every,
+            "plain_forward_ms_per_substep": t_p / steps * 1e3,
+            "checkpointed_forward_ms_per_substep": t_f / steps * 1e3,
+            "forward_adjoint_ms_per_substep": t_fa / steps * 1e3,
+            "forward_adjoint_vs_plain_forward": t_fa / t_p, "plain_forward": "autotuned, hipGraph replay",
+            "loss": loss, "autotuned": dict(zip(TUNED_KEYS, tuned)),
+            "grad_alpha_abs_max": float(da.abs().max()), "grad_ratio_abs_max": float(dr.abs().max()),
+            "grad_v0_abs_max": float(dv0.abs().max()), "finite": finite}
+
+
 def adjoint_cpu_rate(sc, substeps):
     """The FP64 autodiff restatement of the MLS substep (oracle/mls_autodiff.py,
     torch on the host, the checker of the GPU adjoint) timed on `substeps`
@@ -855,7 +939,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-substeps", type=int, default=4,
                     help="substeps in the CPU baseline sample, about 3 s each on one core (0 = skip)")
-    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "mls", "mls_cutting", "mls_cube",
+    ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "pb_adjoint", "mls", "mls_cutting", "mls_cube",
                                            "lattice", "meshsdf", "engine"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
     ap.add_argument("--launch", metavar="T,U,S,M,GS,GM",
@@ -920,6 +1004,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps({"adjoint": adjoint(cpu=cpu)}), flush=True)
+        return
+    if args.workload == "pb_adjoint":
+        print(json.dumps({"pb_adjoint": pb_adjoint()}), flush=True)
         return
     if args.workload == "batched":
         print(json.dumps({"batched_mls": batched("mls", 100, 5), "batched_pb": batched("pb", 20, 2)}), flush=True)
@@ -1033,7 +1120,7 @@ def main():
                 out[key] = batched(kind, steps, warm)
             except Exception as e:  # reported, not fatal: the headline line still prints
                 out[key] = {"error": repr(e)}
-        for key, fn in (("adjoint", adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube),
+        for key, fn in (("adjoint", adjoint), ("pb_adjoint", pb_adjoint), ("mls_cutting", mls_cutting), ("mls_cube", mls_cube),
                         ("lattice", lattice), ("mesh_sdf", mesh_sdf), ("engine", engine)):
             try:
                 out[key] = fn()

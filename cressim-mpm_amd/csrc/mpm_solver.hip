@@ -2447,31 +2447,31 @@ namespace
     // Grid adjoint: sum the tile partials of the velocity adjoint (fixed
     // order), then through u = p / m (or u = p below the mass cutoff: 1e-4
     // for MLS, Kind 0; 1e-8 for PB, Kind 1) to the momentum and mass
-    // adjoints (p_bar xyz, m_bar w).
+    // adjoints (p_bar xyz, m_bar w).  One wavefront per node block of the
+    // substep's active list (the blocks that received tile partials, with
+    // the mask of their source pblocks holding particles).
     template <int Kind>
     __global__ __launch_bounds__(kWG) void k_adj_grid(BatchArgs b, const float4 *grid, float4 *gridAdj)
     {
         const float cutoff = Kind == 1 ? 1e-8f : 1e-4f;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int nbg = first_tile(wave); nbg < b.totalBlocks; nbg += tile_stride())
+        const int nActive = __builtin_amdgcn_readfirstlane(*b.activeCount);
+        for (int item = first_tile(wave); item < nActive; item += tile_stride())
         {
+            const unsigned entry = __builtin_amdgcn_readfirstlane(b.activeList[item]);
+            const int nbg = (int)(entry >> 8);
             int nb;
             const StepCommon c = scene_view(b, nbg, nb);
             const GridGeom &g = c.g;
             int bx, by, bz;
             decode_block(g, nb, bx, by, bz);
             int nsub[8];
-            bool any = false;
 #pragma unroll
             for (int d = 0; d < 8; ++d)
             {
-                const int px = bx - (d & 1), py = by - ((d >> 1) & 1), pz = bz - (d >> 2);
-                const int key = (px >= 0 && py >= 0 && pz >= 0) ? block_key(g, px, py, pz) : -1;
+                const int key = ((entry >> d) & 1u) ? block_key(g, bx - (d & 1), by - ((d >> 1) & 1), bz - (d >> 2)) : -1;
                 nsub[d] = key >= 0 ? subtiles_of(c.segEnd[key] - c.segStart[key], c.subSize, c.subMax) : 0;
-                any = any || nsub[d] > 0;
             }
-            if (!any)
-                continue;
             const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
             V3 ub{0.f, 0.f, 0.f};
 #pragma unroll
@@ -2703,11 +2703,15 @@ namespace
     //   k = K..1             k_pbadj_g2p (per tile: the velocity adjoint
     //                        scattered into tile partials, the weights' x
     //                        adjoint), k_adj_grid<1> (momentum and mass
-    //                        adjoints), k_pbadj_p2g (per tile: those gathered,
-    //                        then the constraint's adjoint through the SVD)
+    //                        adjoints), k_pbadj_prep (per particle: the
+    //                        scatter's inputs re-formed), k_pbadj_scatter (per
+    //                        tile: the momentum and mass adjoints gathered),
+    //                        k_pbadj_constraint (per particle: the constraint's
+    //                        adjoint through the SVD's polar rotation)
     //   k_pbadj_finalize     the adjoint of F^-T; back to particle order
-    // The constraint's adjoint runs in FP64, and the polar rotation's
-    // derivative divides by sigma_i + sigma_j there.
+    // As in the forward, the per-particle work (gather, SVD) runs 64 sorted
+    // particles per wavefront and the per-tile kernels only scatter or
+    // gather through LDS.
     // ------------------------------------------------------------------
 
     // Adjoint state of the substep being reversed, in tile (sorted) order.
@@ -2720,31 +2724,30 @@ namespace
         float4 *Fi; // adjoint of F^-T, 3 columns, accumulated
     };
 
-    CR_DEV D3 d3_scale(const D3 &a, double s) { return d3_comb(a, s, a, 0.0, 0.0); }
-    CR_DEV D3 d3_add(const D3 &a, const D3 &b) { return d3_comb(a, 1.0, b, 1.0, 0.0); }
-    // Cofactor matrix: d det(T) / dT.
-    CR_DEV D3 d3_cof(const D3 &t)
+    // Cofactor matrix of m (d det(m) / dm): the adjugate's transpose.
+    CR_DEV M3 cofactor(const M3 &m)
     {
-        D3 c;
-        c.d[0][0] = t.d[1][1] * t.d[2][2] - t.d[1][2] * t.d[2][1];
-        c.d[0][1] = t.d[1][2] * t.d[2][0] - t.d[1][0] * t.d[2][2];
-        c.d[0][2] = t.d[1][0] * t.d[2][1] - t.d[1][1] * t.d[2][0];
-        c.d[1][0] = t.d[0][2] * t.d[2][1] - t.d[0][1] * t.d[2][2];
-        c.d[1][1] = t.d[0][0] * t.d[2][2] - t.d[0][2] * t.d[2][0];
-        c.d[1][2] = t.d[0][1] * t.d[2][0] - t.d[0][0] * t.d[2][1];
-        c.d[2][0] = t.d[0][1] * t.d[1][2] - t.d[0][2] * t.d[1][1];
-        c.d[2][1] = t.d[0][2] * t.d[1][0] - t.d[0][0] * t.d[1][2];
-        c.d[2][2] = t.d[0][0] * t.d[1][1] - t.d[0][1] * t.d[1][0];
-        return c;
+        const float a = m.c0.x, b = m.c1.x, c = m.c2.x, d = m.c0.y, e = m.c1.y, f = m.c2.y, g = m.c0.z, h = m.c1.z, i = m.c2.z;
+        return M3{V3{(e * i - f * h), -(b * i - c * h), (b * f - c * e)},
+                  V3{-(d * i - f * g), (a * i - c * g), -(a * f - c * d)},
+                  V3{(d * h - e * g), -(a * h - b * g), (a * e - b * d)}};
+    }
+    CR_DEV float m3_at(const M3 &m, int r, int c)
+    {
+        const V3 col = c == 0 ? m.c0 : (c == 1 ? m.c1 : m.c2);
+        return r == 0 ? col.x : (r == 1 ? col.y : col.z);
     }
 
     // Adjoint of pb_constraint for an active material: Ccb is the adjoint of
     // the constraint's output; returns the adjoint of its input C and adds
     // to those of F, of F^-T (Fi) and of alpha and the ratio.  The forward's
     // intermediates are recomputed in FP32 exactly as pb_constraint forms
-    // them (the same SVD).
-    CR_DEV D3 pb_constraint_adjoint(float dt, float alpha, float ratio, const M3 &F, const M3 &Fi, const M3 &C,
-                                    const D3 &Ccb, D3 &Fb, D3 &Fib, double &ab, double &rb)
+    // them (the same SVD).  FP64 where it matters: the parameter reductions
+    // (the residual (B F^-T - I) - dt C is a small difference of O(1)
+    // terms) and the polar rotation's derivative, whose terms divide by
+    // sigma_i + sigma_j.
+    CR_DEV M3 pb_constraint_adjoint(float dt, float alpha, float ratio, const M3 &F, const M3 &C, const M3 &Ccb,
+                                    M3 &Fb, M3 &Fib, double &ab, double &rb)
     {
         const M3 T = mul(m3_identity() + C * dt, F);
         M3 u, vt;
@@ -2757,42 +2760,41 @@ namespace
         const M3 Q = T * (1.0f / s);
         const M3 R = mul(u, vt);
         const M3 B = R * ratio + Q * (1.0f - ratio);
-        const D3 Bd = d3_of(B), Fid = d3_of(Fi), Cd = d3_of(C), Td = d3_of(T), Rd = d3_of(R), Qd = d3_of(Q);
+        const M3 Fi = inverse_t(F);
         // Cc = C + alpha ((B Fi - I) - dt C)
-        ab += d3_dot(Ccb, d3_comb(d3_mul(Bd, Fid), 1.0, Cd, -(double)dt, -1.0));
-        const D3 Bb = d3_scale(d3_mul(Ccb, d3_t(Fid)), alpha);
-        Fib = d3_add(Fib, d3_scale(d3_mul(d3_t(Bd), Ccb), alpha));
+        ab += d3_dot(d3_of(Ccb), d3_comb(d3_mul(d3_of(B), d3_of(Fi)), 1.0, d3_of(C), -(double)dt, -1.0));
+        const M3 Bb = mul(Ccb, transpose(Fi)) * alpha;
+        Fib = Fib + mul(transpose(B), Ccb) * alpha;
         // B = r R + (1 - r) Q
-        rb += d3_dot(Bb, d3_comb(Rd, 1.0, Qd, -1.0, 0.0));
-        const D3 Rb = d3_scale(Bb, ratio), Qb = d3_scale(Bb, 1.0 - (double)ratio);
+        rb += d3_dot(d3_of(Bb), d3_comb(d3_of(R), 1.0, d3_of(Q), -1.0, 0.0));
+        const M3 Rb = Bb * ratio;
         // Q = T / s, s = sign(d) sqrt(clamp(|d|, 0.1, 1000)), d = det T
-        const double sd = s;
-        D3 Tb = d3_scale(Qb, 1.0 / sd);
+        M3 Tb = Bb * ((1.0f - ratio) / s);
         if (adet > 0.1f && adet < 1000.0f)
         {
-            const double sb = -d3_dot(Qb, Td) / (sd * sd);
-            Tb = d3_add(Tb, d3_scale(d3_cof(Td), sb * 0.5 / sqrt((double)adet)));
+            const float sb = -(1.0f - ratio) * mdot(Bb, T) / (s * s);
+            Tb = Tb + cofactor(T) * (sb * 0.5f / sqrtf(adet));
         }
         // R = U V^T for T^T = U diag(sigma) V^T (U = u, V^T = vt):
         // dL/d(T^T) = U K V^T, K_ij = (G_ij - G_ji) / (sigma_i + sigma_j), G = U^T Rb V
-        const D3 Ud = d3_of(u), Vtd = d3_of(vt);
-        const D3 G = d3_mul(d3_mul(d3_t(Ud), Rb), d3_t(Vtd));
+        const M3 G = mul(mul(transpose(u), Rb), transpose(vt));
         const double sg[3] = {sig.x, sig.y, sig.z};
-        D3 K = d3_zero();
+        float k[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                if (i != j)
-                {
-                    const double den = sg[i] + sg[j];
-                    K.d[i][j] = fabs(den) > 1e-12 ? (G.d[i][j] - G.d[j][i]) / den : 0.0;
-                }
-        Tb = d3_add(Tb, d3_t(d3_mul(d3_mul(Ud, K), Vtd)));
+            {
+                const double den = sg[i] + sg[j];
+                k[i][j] = (i != j && fabs(den) > 1e-12)
+                              ? (float)(((double)m3_at(G, i, j) - (double)m3_at(G, j, i)) / den)
+                              : 0.0f;
+            }
+        const M3 K{V3{k[0][0], k[1][0], k[2][0]}, V3{k[0][1], k[1][1], k[2][1]}, V3{k[0][2], k[1][2], k[2][2]}};
+        Tb = Tb + transpose(mul(mul(u, K), vt));
         // T = (I + dt C) F
-        const D3 Fd = d3_of(F);
-        Fb = d3_add(Fb, d3_mul(d3_t(d3_comb(Cd, (double)dt, Cd, 0.0, 1.0)), Tb));
-        return d3_add(d3_scale(Ccb, 1.0 - (double)alpha * dt), d3_scale(d3_mul(Tb, d3_t(Fd)), dt));
+        Fb = Fb + mul(transpose(m3_identity() + C * dt), Tb);
+        return Ccb * (1.0f - alpha * dt) + mul(Tb, transpose(F)) * dt;
     }
 
     // Adjoint of the integration (pbMpmIntegrateParticle without shapes),
@@ -2828,12 +2830,16 @@ namespace
     }
 
     // G2P adjoint of iteration k: U is the iteration's updated grid (the
-    // tape); the adjoint of (v_k, C_k) comes from pa; the velocity adjoint
-    // of the grid goes to the tile partials; the weights' x adjoint is added
-    // to pa.x.
+    // tape); the adjoint of (v_k, C_k) comes from pa.  The weights' x
+    // adjoint is a gather over the staged grid tile (no write conflicts);
+    // the grid velocity adjoint, w (v_bar + B_bar^T d) with B_bar =
+    // (4 / dx) C_bar, has the form of the forward scatter's momentum and
+    // goes through it (pb_scatter: separable node sums, three accumulation
+    // tiles in conflict-free passes) into the tile partials.
     __global__ __launch_bounds__(kWG) void k_pbadj_g2p(BatchArgs b, PbStage st, const float4 *U, PbAdjRef pa)
     {
-        __shared__ float4 acc[kWaves][kTileNodes];
+        constexpr int kAcc = 3;
+        __shared__ F3 acc[kWaves * kAcc][kTileNodes];
         __shared__ float4 lg[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
@@ -2846,10 +2852,9 @@ namespace
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
             const GridGeom &g = c.g;
-            float4 *A = acc[wave];
+            F3 *A = acc[wave * kAcc];
             float4 *G = lg[wave];
-            for (int i = lane; i < kTileNodes; i += 64)
-                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            zero_tiles<kAcc>(A, lane);
             load_tile_grid(g, ti, U + (c.grid - b.grid), G, lane);
             wave_sync();
             for (int q = ti.start + lane; q < ti.end; q += 64)
@@ -2857,44 +2862,76 @@ namespace
                 const V3 x = v3(st.xs[q]);
                 const V3 vb = v3(pa.v[q]);
                 const M3 Bb = load_cols(pa.C, q) * (4.0f * g.invdx);
-                const M3 BbT = transpose(Bb);
                 V3 xacc{0.f, 0.f, 0.f};
-                int bx, by, bz;
-                V3 fx;
-                cell_of(g, x, bx, by, bz, fx);
-                NodeW nw;
-                node_weights(fx, nw);
-                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
+                {
+                    int bx, by, bz;
+                    V3 fx;
+                    cell_of(g, x, bx, by, bz, fx);
+                    NodeW nw;
+                    node_weights(fx, nw);
                     for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
                         const V3 u = v3(G[li]);
-                        const V3 ub = (vb + mul(BbT, dist)) * wt;
-                        lds_add(A, li, ub.x, ub.y, ub.z, 0.0f);
                         const V3 Bu = mul(Bb, u);
                         xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
                     });
-                });
+                }
                 const V3 xs = v3(pa.x[q]) + xacc;
                 pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+                pb_scatter<kAcc>(c, ti, A, 0, x, 1.0f, vb, transpose(Bb));
             }
             wave_sync();
-            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
-            for (int i = lane; i < kTileNodes; i += 64)
-                out[i] = A[i];
+            store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
         }
     }
 
-    // P2G adjoint of iteration k: gathers the momentum and mass adjoints
-    // (gridAdj) and takes them through the scatter and the constraint.  The
-    // constraint's input (v_{k-1}, C_{k-1}) is re-gathered from the previous
-    // iteration's grid Uprev, or for k = 1 read from the substep's start
-    // state s0.  Writes the adjoint of (v_{k-1}, C_{k-1}) to pa; adds to the
-    // x, F and F^-T adjoints and, in FP64 per particle, to those of alpha
-    // (ad.lambdaBar) and the ratio (ad.muBar).
-    __global__ __launch_bounds__(kWG) void k_pbadj_p2g(BatchArgs b, PbStage st, const float4 *gridAdj,
-                                                       const float4 *Uprev, StateRef s0, PbAdjRef pa, AdjRef ad)
+    // The forward's per-particle inputs of iteration k's scatter, re-formed
+    // for its reverse (tile order).
+    struct PbAdjPrep
+    {
+        float4 *v;  // v_{k-1}
+        float4 *C;  // C_{k-1} (the constraint's input), 3 columns
+        float4 *Cc; // the constraint's output, 3 columns
+    };
+
+    // The per-particle half of forward iteration k again (k_pb_dense's
+    // arithmetic): (v_{k-1}, C_{k-1}) gathered from the previous iteration's
+    // grid -- b.grid is that grid -- or, for k = 1, read from the substep's
+    // start state s0, and the constraint applied.
+    __global__ __launch_bounds__(kWG) void k_pbadj_prep(BatchArgs b, PbStage st, int n, const unsigned *keys,
+                                                        int first, StateRef s0, PbAdjPrep pp)
+    {
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const StepCommon c = scene_view_of(b, 0);
+        V3 vel;
+        M3 Cm;
+        if (first)
+        {
+            const int p = (int)c.order[q];
+            vel = v3(s0.v[p]);
+            Cm = load_cols(s0.C, p);
+        }
+        else
+            gather_vc_grid(c, 0, v3(st.xs[q]), V3{0.f, 0.f, 0.f}, vel, Cm);
+        const M3 F = load_m3(st.Fs, st.stride, q);
+        const float2 pr = st.ps[q];
+        M3 Cc = Cm;
+        pb_constraint(c.dt, pr.x, pr.y, st.ts[q], F, Cc);
+        pp.v[q] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+        store_cols(pp.C, q, Cm);
+        store_cols(pp.Cc, q, Cc);
+    }
+
+    // P2G adjoint of iteration k (per tile): gathers the momentum and mass
+    // adjoints (gridAdj) into the adjoint of v_{k-1} (pa.v) and of the
+    // constraint's output (pa.C; accumulated in FP64, a first moment with
+    // cancellation), and adds the weights' x adjoint.
+    __global__ __launch_bounds__(kWG) void k_pbadj_scatter(BatchArgs b, PbStage st, const float4 *gridAdj,
+                                                           PbAdjPrep pp, PbAdjRef pa)
     {
         __shared__ float4 lg[kWaves][kTileNodes];
-        __shared__ float4 lu[kWaves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
@@ -2907,32 +2944,15 @@ namespace
                 continue;
             const GridGeom &g = c.g;
             float4 *G = lg[wave];
-            float4 *H = lu[wave];
             load_tile_grid(g, ti, gridAdj + (c.grid - b.grid), G, lane);
-            if (Uprev)
-                load_tile_grid(g, ti, Uprev + (c.grid - b.grid), H, lane);
             wave_sync();
-            const int S = st.stride;
             for (int q = ti.start + lane; q < ti.end; q += 64)
             {
                 const float4 xm = st.xs[q];
                 const V3 x = v3(xm);
                 const float m = xm.w;
-                V3 vel;
-                M3 Cm;
-                if (Uprev)
-                    gather_vc(c, ti, H, 0, x, V3{0.f, 0.f, 0.f}, vel, Cm);
-                else
-                {
-                    const int p = (int)c.order[q];
-                    vel = v3(s0.v[p]);
-                    Cm = load_cols(s0.C, p);
-                }
-                const M3 F = load_m3(st.Fs, S, q);
-                const float2 pr = st.ps[q];
-                const int type = st.ts[q];
-                M3 Cc = Cm;
-                pb_constraint(c.dt, pr.x, pr.y, type, F, Cc);
+                const V3 vel = v3(pp.v[q]);
+                const M3 Cc = load_cols(pp.Cc, q);
                 const M3 CcT = transpose(Cc);
                 V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
                 D3 Ccb = d3_zero();
@@ -2958,21 +2978,38 @@ namespace
                 const V3 xs = v3(pa.x[q]) + xacc;
                 pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
                 pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
-                D3 Cb = Ccb;
-                if (type == mNeoHookean || type == mCoRotational)
-                {
-                    D3 Fb = d3_of(load_cols(pa.F, q)), Fib = d3_of(load_cols(pa.Fi, q));
-                    double ab = 0.0, rb = 0.0;
-                    Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, inverse_t(F), Cm, Ccb, Fb, Fib, ab, rb);
-                    store_cols(pa.F, q, m3_of(Fb));
-                    store_cols(pa.Fi, q, m3_of(Fib));
-                    const int p = (int)c.order[q];
-                    ad.lambdaBar[p] += ab;
-                    ad.muBar[p] += rb;
-                }
-                store_cols(pa.C, q, m3_of(Cb));
+                store_cols(pa.C, q, m3_of(Ccb));
             }
         }
+    }
+
+    // The constraint's adjoint of iteration k, one thread per sorted
+    // particle: pa.C holds the adjoint of the constraint's output and
+    // receives that of its input C_{k-1}; the F and F^-T adjoints are
+    // accumulated in pa, those of alpha and the ratio in FP64 per particle
+    // (ad.lambdaBar, ad.muBar).  Other materials pass the adjoint through.
+    __global__ __launch_bounds__(kWG) void k_pbadj_constraint(BatchArgs b, PbStage st, int n, const unsigned *keys,
+                                                              PbAdjPrep pp, PbAdjRef pa, AdjRef ad)
+    {
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const int type = st.ts[q];
+        if (type != mNeoHookean && type != mCoRotational)
+            return;
+        const StepCommon c = scene_view_of(b, 0);
+        const float2 pr = st.ps[q];
+        const M3 F = load_m3(st.Fs, st.stride, q);
+        M3 Fb = load_cols(pa.F, q), Fib = load_cols(pa.Fi, q);
+        double ab = 0.0, rb = 0.0;
+        const M3 Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, load_cols(pp.C, q), load_cols(pa.C, q), Fb, Fib, ab, rb);
+        store_cols(pa.C, q, Cb);
+        store_cols(pa.F, q, Fb);
+        store_cols(pa.Fi, q, Fib);
+        const int p = (int)c.order[q];
+        ad.lambdaBar[p] += ab;
+        ad.muBar[p] += rb;
     }
 
     // End of a substep's reverse: the adjoint of F^-T added to F's
@@ -4823,6 +4860,7 @@ struct CrHipAdjoint
     float4 **segGrid = nullptr;
     unsigned **segOrder = nullptr;
     int **segStart = nullptr, **segEnd = nullptr;
+    unsigned **segActive = nullptr; // active node block list and its count ([totalBlocks] then the count)
     float4 *gridAdj = nullptr;
     float4 *xTmp = nullptr;
     float4 *FTmp = nullptr;
@@ -4831,6 +4869,7 @@ struct CrHipAdjoint
     float4 *tape = nullptr;
     int tapeIters = 0;
     float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
+    float4 *ppV = nullptr, *ppC = nullptr, *ppCc = nullptr;
 };
 
 namespace
@@ -4884,6 +4923,7 @@ namespace
             a->tapeIters = K;
         }
         const PbAdjRef pa{a->paX, a->paV, a->paC, a->paF, a->paFi};
+        const PbAdjPrep pp{a->ppV, a->ppC, a->ppCc};
         const int stride = b->capacity;
         int rc = CRMPM_HIP_OK;
         for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
@@ -4912,14 +4952,18 @@ namespace
                 const int subWgs = grid1d(b->totalBlocks * args.subMax, kWaves);
                 const int blockWgs = grid1d(b->totalBlocks, kWaves);
                 const unsigned *keys = b->lastKeys;
+                const int denseWgs = grid1d(n, kWG);
                 k_pbadj_integrate<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, ad, pa);
                 for (int k = K; k >= 1; --k)
                 {
                     const float4 *U = a->tape + (size_t)(k - 1) * nodes;
                     k_pbadj_g2p<<<subWgs, kWG, 0, st>>>(args, b->st, U, pa);
                     k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(args, U, a->gridAdj);
-                    k_pbadj_p2g<<<subWgs, kWG, 0, st>>>(args, b->st, a->gridAdj, k > 1 ? U - nodes : nullptr,
-                                                         state_ref(a->seg[j], stride), pa, ad);
+                    BatchArgs prev = args;
+                    prev.grid = const_cast<float4 *>(U - nodes); // unused for k = 1
+                    k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, b->st, n, keys, k == 1, state_ref(a->seg[j], stride), pp);
+                    k_pbadj_scatter<<<subWgs, kWG, 0, st>>>(args, b->st, a->gridAdj, pp, pa);
+                    k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, pp, pa, ad);
                 }
                 k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, pa, ad);
                 rc = check_launch();
@@ -4946,6 +4990,8 @@ namespace
                 dfree(a->segStart[i]);
             if (a->segEnd)
                 dfree(a->segEnd[i]);
+            if (a->segActive)
+                dfree(a->segActive[i]);
         }
         delete[] a->ckpt;
         delete[] a->seg;
@@ -4953,6 +4999,7 @@ namespace
         delete[] a->segOrder;
         delete[] a->segStart;
         delete[] a->segEnd;
+        delete[] a->segActive;
         dfree(a->gridAdj);
         dfree(a->xTmp);
         dfree(a->FTmp);
@@ -4962,6 +5009,9 @@ namespace
         dfree(a->paC);
         dfree(a->paF);
         dfree(a->paFi);
+        dfree(a->ppV);
+        dfree(a->ppC);
+        dfree(a->ppCc);
         delete a;
         return CRMPM_HIP_OK;
     }
@@ -4992,7 +5042,10 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
     a->segOrder = new (std::nothrow) unsigned *[a->every]();
     a->segStart = new (std::nothrow) int *[a->every]();
     a->segEnd = new (std::nothrow) int *[a->every]();
-    int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd) ? 0 : CRMPM_HIP_ERR_ALLOC;
+    a->segActive = new (std::nothrow) unsigned *[a->every]();
+    int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd && a->segActive)
+                 ? 0
+                 : CRMPM_HIP_ERR_ALLOC;
     for (int i = 0; !rc && i < a->segments + 1; ++i)
         rc |= state_alloc(a->ckpt[i], N);
     for (int i = 0; !rc && i < a->every + 1; ++i)
@@ -5005,6 +5058,7 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->segOrder[i], N);
         rc |= dalloc(&a->segStart[i], (size_t)b->totalBlocks);
         rc |= dalloc(&a->segEnd[i], (size_t)b->totalBlocks);
+        rc |= dalloc(&a->segActive[i], (size_t)b->totalBlocks + 1);
     }
     if (!rc)
     {
@@ -5019,6 +5073,9 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->paC, 3 * N);
         rc |= dalloc(&a->paF, 3 * N);
         rc |= dalloc(&a->paFi, 3 * N);
+        rc |= dalloc(&a->ppV, N);
+        rc |= dalloc(&a->ppC, 3 * N);
+        rc |= dalloc(&a->ppCc, 3 * N);
     }
     if (rc)
     {
@@ -5112,6 +5169,10 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
                  hipMemcpyAsync(a->segStart[j], b->segStart, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess ||
                  hipMemcpyAsync(a->segEnd[j], b->segEnd, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
+                                st) != hipSuccess ||
+                 hipMemcpyAsync(a->segActive[j], b->activeList, (size_t)b->totalBlocks * sizeof(unsigned),
+                                hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                 hipMemcpyAsync(a->segActive[j] + b->totalBlocks, b->activeCount, sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess))
                 rc = CRMPM_HIP_ERR_LAUNCH;
         }
@@ -5125,6 +5186,8 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             args.gatherOrder = args.order;
             args.segStart = a->segStart[j];
             args.segEnd = a->segEnd[j];
+            args.activeList = a->segActive[j];
+            args.activeCount = reinterpret_cast<const int *>(a->segActive[j] + b->totalBlocks);
             // The reverse kernels use their own split, as fine as the partial
             // slots allow: their per-particle work is several times the
             // forward's, and the forward's tuned split (often whole pblocks)
