@@ -213,6 +213,16 @@ namespace
     // w, w + W, w + 2W, ... (W = wavefronts in the launch), so a launch of any
     // size covers the batch and the autotuner picks the size.
     CR_DEV int first_tile(int wave) { return __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWaves + wave); }
+    // The same for a launch of Waves wavefronts per workgroup.  Launches with
+    // one tile per wavefront use one-wavefront workgroups: a workgroup's LDS
+    // is returned when all of its wavefronts are done, so in a four-wavefront
+    // workgroup one tile with many particles would hold the other three
+    // wavefronts' LDS (PB scatter on the slab: 23.3 -> 19.8 us).
+    template <int Waves>
+    CR_DEV int first_tile_of(int wave)
+    {
+        return __builtin_amdgcn_readfirstlane((int)blockIdx.x * Waves + wave);
+    }
     CR_DEV int tile_stride() { return (int)gridDim.x * kWaves; }
     CR_DEV int num_subtiles(const BatchArgs &b) { return b.totalBlocks * b.subMax; }
     // Whether sub-tile tg (global pblock tg / subMax, index tg % subMax) has
@@ -1392,12 +1402,13 @@ namespace
     // MLS-MPM P2G (src/solver/mls_mpm_algorithm.h:147-241)
     // ------------------------------------------------------------------
 
-    template <bool Persistent>
-    __global__ __launch_bounds__(kWG) void k_mls_p2g(BatchArgs b)
+    template <bool Persistent, int Waves = kWaves>
+    __global__ __launch_bounds__(64 * Waves) void k_mls_p2g(BatchArgs b)
     {
-        __shared__ float4 acc[kWaves * kScatterTiles][kTileNodes];
+        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
+        __shared__ float4 acc[Waves * kScatterTiles][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
@@ -1755,14 +1766,15 @@ namespace
     }
 
     // MLS G2P (src/solver/mls_mpm_algorithm.h:296-390)
-    template <bool Persistent>
-    __global__ __launch_bounds__(kWG) void k_mls_g2p(BatchArgs b)
+    template <bool Persistent, int Waves = kWaves>
+    __global__ __launch_bounds__(64 * Waves) void k_mls_g2p(BatchArgs b)
     {
-        __shared__ float4 lg[kWaves][kTileNodes];
-        __shared__ ChunkStage stage[kWaves];
+        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
+        __shared__ float4 lg[Waves][kTileNodes];
+        __shared__ ChunkStage stage[Waves];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubMax); tg < items;
+        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), items, b.gSubMax); tg < items;
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
@@ -1990,14 +2002,15 @@ namespace
     // iteration, which also accumulates the node masses (16-byte nodes);
     // later iterations accumulate momentum only (12-byte nodes) and leave the
     // first iteration's masses in the partial boxes.
-    template <bool Mass, bool Persistent>
-    __global__ __launch_bounds__(kWG) void k_pb_scatter(BatchArgs b, PbStage st)
+    template <bool Mass, bool Persistent, int Waves = kWaves>
+    __global__ __launch_bounds__(64 * Waves) void k_pb_scatter(BatchArgs b, PbStage st)
     {
+        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
         constexpr int kAcc = 3;
         using Acc = typename std::conditional<Mass, float4, F3>::type;
-        __shared__ Acc acc[kWaves * kAcc][kTileNodes];
+        __shared__ Acc acc[Waves * kAcc][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
@@ -2050,12 +2063,13 @@ namespace
     // ------------------------------------------------------------------
     // Standard MPM (src/solver/standard_mpm_algorithm.h:146-352)
     // ------------------------------------------------------------------
-    template <bool Persistent>
-    __global__ __launch_bounds__(kWG) void k_std_p2g(BatchArgs b)
+    template <bool Persistent, int Waves = kWaves>
+    __global__ __launch_bounds__(64 * Waves) void k_std_p2g(BatchArgs b)
     {
-        __shared__ float4 acc[kWaves][2][kTileNodes];
+        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
+        __shared__ float4 acc[Waves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
+        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
         {
             if (!subtile_used(b, tg))
@@ -2127,13 +2141,14 @@ namespace
         }
     }
 
-    template <bool Persistent>
-    __global__ __launch_bounds__(kWG) void k_std_g2p(BatchArgs b)
+    template <bool Persistent, int Waves = kWaves>
+    __global__ __launch_bounds__(64 * Waves) void k_std_g2p(BatchArgs b)
     {
-        __shared__ float4 lg[kWaves][kTileNodes];
+        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
+        __shared__ float4 lg[Waves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile(wave), items, b.gSubMax); tg < items;
+        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), items, b.gSubMax); tg < items;
              tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
         {
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
@@ -2376,12 +2391,12 @@ namespace
     // its end; grid: the substep's updated grid; the adjoint of s1 comes in
     // through ad and the G2P parts of the x and F adjoints go to ad.xTmp /
     // ad.FTmp; the grid velocity adjoint goes to the tile partials.
-    __global__ __launch_bounds__(kWG) void k_adj_g2p(BatchArgs b, StateRef s0, StateRef s1, const float4 *grid, AdjRef ad)
+    __global__ __launch_bounds__(64) void k_adj_g2p(BatchArgs b, StateRef s0, StateRef s1, const float4 *grid, AdjRef ad)
     {
-        __shared__ float4 acc[kWaves][kTileNodes];
-        __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ float4 acc[1][kTileNodes];
+        __shared__ float4 lg[1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -2586,11 +2601,11 @@ namespace
     // P2G adjoint of one substep: gathers (p_bar, m_bar) from the LDS tile,
     // then the affine, stress and parameter adjoints; writes the adjoint of
     // the substep's start state.
-    __global__ __launch_bounds__(kWG) void k_adj_p2g(BatchArgs b, StateRef s0, const float4 *gridAdj, AdjRef ad)
+    __global__ __launch_bounds__(64) void k_adj_p2g(BatchArgs b, StateRef s0, const float4 *gridAdj, AdjRef ad)
     {
-        __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ float4 lg[1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -2836,13 +2851,13 @@ namespace
     // (4 / dx) C_bar, has the form of the forward scatter's momentum and
     // goes through it (pb_scatter: separable node sums, three accumulation
     // tiles in conflict-free passes) into the tile partials.
-    __global__ __launch_bounds__(kWG) void k_pbadj_g2p(BatchArgs b, PbStage st, const float4 *U, PbAdjRef pa)
+    __global__ __launch_bounds__(64) void k_pbadj_g2p(BatchArgs b, PbStage st, const float4 *U, PbAdjRef pa)
     {
         constexpr int kAcc = 3;
-        __shared__ F3 acc[kWaves * kAcc][kTileNodes];
-        __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ F3 acc[kAcc][kTileNodes];
+        __shared__ float4 lg[1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -2928,12 +2943,12 @@ namespace
     // adjoints (gridAdj) into the adjoint of v_{k-1} (pa.v) and of the
     // constraint's output (pa.C; accumulated in FP64, a first moment with
     // cancellation), and adds the weights' x adjoint.
-    __global__ __launch_bounds__(kWG) void k_pbadj_scatter(BatchArgs b, PbStage st, const float4 *gridAdj,
+    __global__ __launch_bounds__(64) void k_pbadj_scatter(BatchArgs b, PbStage st, const float4 *gridAdj,
                                                            PbAdjPrep pp, PbAdjRef pa)
     {
-        __shared__ float4 lg[kWaves][kTileNodes];
+        __shared__ float4 lg[1][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -3436,9 +3451,16 @@ namespace
         const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
         const bool multi = b->subMax > 1, persU = b->updateWaves > 0;
+        // PB scatter launched one tile per wavefront: one-wavefront workgroups, one per item
+        const long long tileItems = (long long)b->totalBlocks * b->subMax;
+        const dim3 tileGrid = pers ? dim3(tileWGs) : dim3((unsigned)std::max(1ll, tileItems));
+        const dim3 tileBlock = pers ? dim3(kWG) : dim3(64);
         auto upd1 = update_kernel<1>(multi, persU);
         if (b->kind == CRMPM_HIP_MLS)
         {
+            // MLS and standard MPM keep four-wavefront workgroups: their sparse
+            // grids have many empty items, whose workgroups cost dispatch time
+            // (one-wavefront groups: MLS cube substep 0.109 -> 0.117 ms)
             launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, a, b->dirty);
             launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, gatherWGs, kWG, a);
@@ -3469,15 +3491,15 @@ namespace
                                                         hipMemcpyDeviceToDevice, b->stream) == hipSuccess;
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
-                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true> : k_pb_scatter<true, false>, tileWGs,
-                       kWG, a, b->st);
+                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true> : k_pb_scatter<true, false, 1>, tileGrid,
+                       tileBlock, a, b->st);
                 launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 tape(0);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
-                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true> : k_pb_scatter<false, false>,
-                           tileWGs, kWG, a, b->st);
+                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true> : k_pb_scatter<false, false, 1>,
+                           tileGrid, tileBlock, a, b->st);
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                     tape(it);
                 }
@@ -4949,7 +4971,7 @@ namespace
                 args.gatherOrder = args.order;
                 args.subSize = kAdjSubSize;
                 args.subMax = b->subStride;
-                const int subWgs = grid1d(b->totalBlocks * args.subMax, kWaves);
+                const int subItems = std::max(1, (int)b->totalBlocks * args.subMax); // one-wavefront workgroups, one per sub-tile
                 const int blockWgs = grid1d(b->totalBlocks, kWaves);
                 const unsigned *keys = b->lastKeys;
                 const int denseWgs = grid1d(n, kWG);
@@ -4957,12 +4979,12 @@ namespace
                 for (int k = K; k >= 1; --k)
                 {
                     const float4 *U = a->tape + (size_t)(k - 1) * nodes;
-                    k_pbadj_g2p<<<subWgs, kWG, 0, st>>>(args, b->st, U, pa);
+                    k_pbadj_g2p<<<subItems, 64, 0, st>>>(args, b->st, U, pa);
                     k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(args, U, a->gridAdj);
                     BatchArgs prev = args;
                     prev.grid = const_cast<float4 *>(U - nodes); // unused for k = 1
                     k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, b->st, n, keys, k == 1, state_ref(a->seg[j], stride), pp);
-                    k_pbadj_scatter<<<subWgs, kWG, 0, st>>>(args, b->st, a->gridAdj, pp, pa);
+                    k_pbadj_scatter<<<subItems, 64, 0, st>>>(args, b->st, a->gridAdj, pp, pa);
                     k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, pp, pa, ad);
                 }
                 k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, pa, ad);
@@ -5194,12 +5216,12 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             // would leave them one wavefront per dense pblock.
             args.subSize = kAdjSubSize;
             args.subMax = b->subStride;
-            const int subWgs = grid1d(b->totalBlocks * args.subMax, kWaves);
+            const int subItems = std::max(1, (int)b->totalBlocks * args.subMax); // one-wavefront workgroups, one per sub-tile
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
-            k_adj_g2p<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
+            k_adj_g2p<<<subItems, 64, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
                                                a->segGrid[j], ad);
             k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
-            k_adj_p2g<<<subWgs, kWG, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
+            k_adj_p2g<<<subItems, 64, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
             rc = check_launch();
         }
     }
