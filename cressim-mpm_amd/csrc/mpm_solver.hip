@@ -78,8 +78,8 @@ namespace
     // to few cache lines.
     struct SolverState
     {
-        float4 *F;  // [3 * stride]: column c of particle p at 3 p + c
-        float4 *C;
+        float *F;  // [9 * stride]: element (r, c) of particle p at 9 p + 3 c + r
+        float *C;
         float *V0;
         int stride;
     };
@@ -224,6 +224,11 @@ namespace
         return __builtin_amdgcn_readfirstlane((int)blockIdx.x * Waves + wave);
     }
     CR_DEV int tile_stride() { return (int)gridDim.x * kWaves; }
+    template <int Waves>
+    CR_DEV int tile_stride_of()
+    {
+        return (int)gridDim.x * Waves;
+    }
     CR_DEV int num_subtiles(const BatchArgs &b) { return b.totalBlocks * b.subMax; }
     // Whether sub-tile tg (global pblock tg / subMax, index tg % subMax) has
     // particles, from the global segment arrays alone (an empty sub-tile
@@ -243,12 +248,12 @@ namespace
     // (subMax 1) and W <= 2^24, where t + 64 W stays below 2^32; otherwise,
     // and in a non-persistent launch (one item per wavefront), the loop body
     // keeps its own test.
-    template <bool Persistent>
+    template <bool Persistent, int Waves = kWaves>
     CR_DEV int skip_empty(const BatchArgs &b, int tg, int items, int subMax)
     {
-        if (!Persistent || subMax != 1 || tile_stride() > (1 << 24))
+        if (!Persistent || subMax != 1 || tile_stride_of<Waves>() > (1 << 24))
             return tg;
-        const unsigned W = (unsigned)tile_stride(), n = (unsigned)items;
+        const unsigned W = (unsigned)tile_stride_of<Waves>(), n = (unsigned)items;
         const unsigned lane = threadIdx.x & 63u;
         for (unsigned t = (unsigned)tg; t < n; t += 64u * W)
         {
@@ -261,10 +266,10 @@ namespace
     }
     // Next pblock of a tile loop; a non-persistent launch (one wavefront per
     // pblock) compiles to straight-line code, which needs fewer registers.
-    template <bool Persistent>
+    template <bool Persistent, int Waves = kWaves>
     CR_DEV int next_tile(int tg, int totalBlocks)
     {
-        return Persistent ? tg + tile_stride() : totalBlocks;
+        return Persistent ? tg + tile_stride_of<Waves>() : totalBlocks;
     }
 
     CR_DEV int block_key(const GridGeom &g, int bx, int by, int bz) { return bx + g.nbx * (by + g.nby * bz); }
@@ -307,6 +312,27 @@ namespace
         bz = key / (g.nbx * g.nby);
     }
 
+    // Solver-owned particle matrices (F, C): 9 floats per particle, element
+    // (r, c) of particle p at 9 p + 3 c + r (36 bytes, the columns packed).
+    CR_DEV M3 load_pm(const float *m, int p)
+    {
+        const float *a = m + 9 * (size_t)p;
+        return M3{V3{a[0], a[1], a[2]}, V3{a[3], a[4], a[5]}, V3{a[6], a[7], a[8]}};
+    }
+    CR_DEV void store_pm(float *m, int p, const M3 &v)
+    {
+        float *a = m + 9 * (size_t)p;
+        a[0] = v.c0.x;
+        a[1] = v.c0.y;
+        a[2] = v.c0.z;
+        a[3] = v.c1.x;
+        a[4] = v.c1.y;
+        a[5] = v.c1.z;
+        a[6] = v.c2.x;
+        a[7] = v.c2.y;
+        a[8] = v.c2.z;
+    }
+    // Three float4 columns per particle (the adjoints' own scratch arrays).
     CR_DEV M3 load_cols(const float4 *m, int p)
     {
         const float4 a = m[3 * p], b = m[3 * p + 1], c = m[3 * p + 2];
@@ -345,38 +371,38 @@ namespace
     }
 
     // A chunk's solver-owned matrices, written through LDS.  A store of one
-    // 48-byte record per lane writes 16 bytes every 48 and touches three
-    // times as many cache lines per instruction as 64 consecutive 16-byte
-    // pieces, and the L2's write requests then bound the gather kernels.  So
-    // each lane puts its particle's C and F columns into the wavefront's
-    // stage in chunk order, and lane j stores the chunk's j-th column: where
-    // the chunk's particles are consecutive (index order), neighbouring lanes
-    // fill neighbouring bytes.  All 64 lanes call it; p < 0: no particle.
+    // 36-byte record per lane touches a cache line per lane and instruction;
+    // so each lane puts its particle's C and F into the wavefront's stage in
+    // chunk order, and lane j stores the chunk's j-th float: where the
+    // chunk's particles are consecutive (index order), each store
+    // instruction writes 256 contiguous bytes.  All 64 lanes call it;
+    // p < 0: no particle.
     struct ChunkStage
     {
-        float4 col[2][3 * 64]; // C, F: column c of the chunk's particle q at 3 q + c
+        float m[2][9 * 64]; // C, F: element e of the chunk's particle q at 9 q + e
         int p[64];
     };
-    CR_DEV void store_chunk_cols(ChunkStage &S, int lane, int p, const M3 &Cm, const M3 &Fm, float4 *C, float4 *F)
+    CR_DEV void store_chunk_mats(ChunkStage &S, int lane, int p, const M3 &Cm, const M3 &Fm, float *C, float *F)
     {
         S.p[lane] = p;
-        const V3 *cc = &Cm.c0, *fc = &Fm.c0;
+        const float cv[9] = {Cm.c0.x, Cm.c0.y, Cm.c0.z, Cm.c1.x, Cm.c1.y, Cm.c1.z, Cm.c2.x, Cm.c2.y, Cm.c2.z};
+        const float fv[9] = {Fm.c0.x, Fm.c0.y, Fm.c0.z, Fm.c1.x, Fm.c1.y, Fm.c1.z, Fm.c2.x, Fm.c2.y, Fm.c2.z};
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (int e = 0; e < 9; ++e)
         {
-            S.col[0][3 * lane + k] = make_float4(cc[k].x, cc[k].y, cc[k].z, 0.0f);
-            S.col[1][3 * lane + k] = make_float4(fc[k].x, fc[k].y, fc[k].z, 0.0f);
+            S.m[0][9 * lane + e] = cv[e];
+            S.m[1][9 * lane + e] = fv[e];
         }
         wave_sync();
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 9; ++i)
         {
-            const int j = 64 * i + lane, q = j / 3, k = j - 3 * q;
+            const int j = 64 * i + lane, q = j / 9, e = j - 9 * q;
             const int pq = S.p[q];
             if (pq >= 0)
             {
-                C[3 * pq + k] = S.col[0][j];
-                F[3 * pq + k] = S.col[1][j];
+                C[9 * (size_t)pq + e] = S.m[0][j];
+                F[9 * (size_t)pq + e] = S.m[1][j];
             }
         }
         wave_sync(); // the next chunk reuses the stage
@@ -1434,8 +1460,8 @@ namespace
                 pNext = (int)c.order[min(k + 64, ti.end - 1)];
                 const float4 xm = c.P.xm[p];
                 const float4 vv = c.P.v[p];
-                const M3 F = load_cols(c.S.F, p);
-                const M3 Cm = load_cols(c.S.C, p);
+                const M3 F = load_pm(c.S.F, p);
+                const M3 Cm = load_pm(c.S.C, p);
                 const float4 prm = c.P.params[p];
                 const int type = c.P.type[p];
                 const float v0 = c.S.V0[p];
@@ -1806,7 +1832,7 @@ namespace
                 {
                     const float4 xm = c.P.xm[p];
                     const float4 vp = c.P.v[p];
-                    const M3 F = load_cols(c.S.F, p);
+                    const M3 F = load_pm(c.S.F, p);
                     V3 x = v3(xm);
                     V3 vel;
                     // the particle velocity after P2G's gravity update (P2G does not store it)
@@ -1819,7 +1845,7 @@ namespace
                     c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
                     Fn = mul(m3_identity() + Cm * c.dt, F);
                 }
-                store_chunk_cols(stage[wave], lane, valid ? p : -1, Cm, Fn, c.S.C, c.S.F);
+                store_chunk_mats(stage[wave], lane, valid ? p : -1, Cm, Fn, c.S.C, c.S.F);
             }
         }
     }
@@ -1852,8 +1878,8 @@ namespace
         st.vs[k] = v.x;
         st.vs[st.stride + k] = v.y;
         st.vs[2 * st.stride + k] = v.z;
-        store_m3(st.Fs, st.stride, k, load_cols(S.F, p));
-        store_m3(st.Cs, st.stride, k, load_cols(S.C, p));
+        store_m3(st.Fs, st.stride, k, load_pm(S.F, p));
+        store_m3(st.Cs, st.stride, k, load_pm(S.C, p));
         const float4 prm = P.params[p];
         st.ps[k] = make_float2(prm.x, prm.y);
         st.ts[k] = P.type[p];
@@ -1968,8 +1994,8 @@ namespace
             resolve_rigid<true>(c.sv, shape_count(c.sv), vv, x);
             c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
             c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-            store_cols(c.S.C, p, Cm);
-            store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
+            store_pm(c.S.C, p, Cm);
+            store_pm(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
         }
     }
 
@@ -2005,13 +2031,14 @@ namespace
     template <bool Mass, bool Persistent, int Waves = kWaves>
     __global__ __launch_bounds__(64 * Waves) void k_pb_scatter(BatchArgs b, PbStage st)
     {
-        static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
         constexpr int kAcc = 3;
         using Acc = typename std::conditional<Mass, float4, F3>::type;
         __shared__ Acc acc[Waves * kAcc][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
+        for (int tg = skip_empty<Persistent, Waves>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax);
+             tg < num_subtiles(b);
+             tg = skip_empty<Persistent, Waves>(b, next_tile<Persistent, Waves>(tg, num_subtiles(b)), num_subtiles(b),
+                                                b.subMax))
         {
             if (!subtile_used(b, tg))
                 continue;
@@ -2057,7 +2084,7 @@ namespace
         c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
         c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
         const M3 F = load_m3(st.Fs, S, k);
-        store_cols(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
+        store_pm(c.S.F, p, mul(m3_identity() + Cm * c.dt, F));
     }
 
     // ------------------------------------------------------------------
@@ -2101,7 +2128,7 @@ namespace
                 Weights W;
                 weights_of(fx, W);
                 const float4 prm = c.P.params[p];
-                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_cols(c.S.F, p));
+                const M3 stress = stress_kirchhoff(prm.x, prm.y, c.P.type[p], load_pm(c.S.F, p));
                 const V3 vel = v3(c.P.v[p]);
                 const float nv0 = -c.S.V0[p];
                 const bool fast = nodes_inside(g, bx, by, bz);
@@ -2170,7 +2197,7 @@ namespace
                 if (!c.P.mask[p])
                     continue;
                 const float4 xm = c.P.xm[p];
-                const M3 F = load_cols(c.S.F, p); // loaded with x, before the stores
+                const M3 F = load_pm(c.S.F, p); // loaded with x, before the stores
                 int bx, by, bz;
                 V3 fx;
                 cell_of(g, v3(xm), bx, by, bz, fx);
@@ -2194,7 +2221,7 @@ namespace
                 resolve_rigid<true>(c.sv, nShapes, vv, x);
                 c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
                 c.P.v[p] = make_float4(vacc.x, vacc.y, vacc.z, 0.0f);
-                store_cols(c.S.F, p, mul(m3_identity() + gv * c.dt, F));
+                store_pm(c.S.F, p, mul(m3_identity() + gv * c.dt, F));
             }
         }
     }
@@ -2219,7 +2246,7 @@ namespace
         const StepCommon c = scene_view_of(b, s);
         const GridGeom &g = c.g;
         const int p = b.scenes[s].particleOffset + (int)idx[i];
-        store_cols(c.S.F, p, m3_identity());
+        store_pm(c.S.F, p, m3_identity());
         const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
@@ -2264,7 +2291,7 @@ namespace
         const StepCommon c = scene_view_of(b, s);
         const GridGeom &g = c.g;
         const int p = b.scenes[s].particleOffset + (int)idx[i];
-        store_cols(c.S.F, p, m3_identity());
+        store_pm(c.S.F, p, m3_identity());
         const float4 x = c.P.xm[p];
         int bx, by, bz;
         V3 fx;
@@ -2325,8 +2352,8 @@ namespace
     {
         const float4 *x;
         const float4 *v;
-        const float4 *F; // columns, as SolverState
-        const float4 *C;
+        const float *F; // packed, as SolverState
+        const float *C;
         int stride;
     };
 
@@ -2417,8 +2444,8 @@ namespace
                 const int p = (int)c.order[k];
                 const V3 x = v3(s0.x[p]);
                 const V3 vn = v3(s1.v[p]);
-                const M3 Cn = load_cols(s1.C, p);
-                const M3 F = load_cols(s0.F, p);
+                const M3 Cn = load_pm(s1.C, p);
+                const M3 F = load_pm(s0.F, p);
                 const V3 xb = v3(ad.x[p]);
                 const V3 vb = v3(ad.v[p]);
                 const M3 Fb = load_cols(ad.F, p);
@@ -2625,8 +2652,8 @@ namespace
                 const V3 x = v3(xm);
                 const float m = xm.w;
                 const V3 vel = v3(s0.v[p]) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                const M3 F = load_cols(s0.F, p);
-                const M3 Cm = load_cols(s0.C, p);
+                const M3 F = load_pm(s0.F, p);
+                const M3 Cm = load_pm(s0.C, p);
                 const float4 prm = c.P.params[p];
                 const int type = c.P.type[p];
                 const M3 tau = stress_kirchhoff(prm.x, prm.y, type, F);
@@ -2826,7 +2853,7 @@ namespace
         const GridGeom &g = c.g;
         const int p = (int)c.order[q];
         const V3 xe = v3(c.P.xm[p]);
-        const M3 CK = load_cols(c.S.C, p);
+        const M3 CK = load_pm(c.S.C, p);
         const M3 F = load_m3(st.Fs, st.stride, q);
         const V3 xb = v3(ad.x[p]), vb = v3(ad.v[p]);
         const M3 Fb = load_cols(ad.F, p), Cb = load_cols(ad.C, p);
@@ -2926,7 +2953,7 @@ namespace
         {
             const int p = (int)c.order[q];
             vel = v3(s0.v[p]);
-            Cm = load_cols(s0.C, p);
+            Cm = load_pm(s0.C, p);
         }
         else
             gather_vc_grid(c, 0, v3(st.xs[q]), V3{0.f, 0.f, 0.f}, vel, Cm);
@@ -3110,7 +3137,7 @@ struct CrHipBatch
     ParticleBind P{};
 
     // owned
-    float4 *F = nullptr, *C = nullptr;
+    float *F = nullptr, *C = nullptr; // 9 floats per particle (SolverState)
     float *V0 = nullptr;
     float4 *grid = nullptr, *gridForce = nullptr;
     unsigned char *dirty = nullptr;
@@ -3451,10 +3478,11 @@ namespace
         const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
         const bool multi = b->subMax > 1, persU = b->updateWaves > 0;
-        // PB scatter launched one tile per wavefront: one-wavefront workgroups, one per item
+        // PB scatter: one-wavefront workgroups, one per item or, persistent,
+        // one per launched wavefront
         const long long tileItems = (long long)b->totalBlocks * b->subMax;
-        const dim3 tileGrid = pers ? dim3(tileWGs) : dim3((unsigned)std::max(1ll, tileItems));
-        const dim3 tileBlock = pers ? dim3(kWG) : dim3(64);
+        const dim3 tileGrid((unsigned)std::max(1ll, pers ? std::min<long long>(b->tileWaves, tileItems) : tileItems));
+        const dim3 tileBlock(64);
         auto upd1 = update_kernel<1>(multi, persU);
         if (b->kind == CRMPM_HIP_MLS)
         {
@@ -3491,14 +3519,14 @@ namespace
                                                         hipMemcpyDeviceToDevice, b->stream) == hipSuccess;
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
-                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true> : k_pb_scatter<true, false, 1>, tileGrid,
+                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
                        tileBlock, a, b->st);
                 launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                 tape(0);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
-                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true> : k_pb_scatter<false, false, 1>,
+                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true, 1> : k_pb_scatter<false, false, 1>,
                            tileGrid, tileBlock, a, b->st);
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                     tape(it);
@@ -3654,8 +3682,8 @@ namespace
         const int radixTiles = grid1d((long long)N, kRadixTile);
         const int ns = b->numScenes;
         int rc = 0;
-        rc |= dalloc(&b->F, 3 * N);
-        rc |= dalloc(&b->C, 3 * N);
+        rc |= dalloc(&b->F, 9 * N);
+        rc |= dalloc(&b->C, 9 * N);
         rc |= dalloc(&b->V0, N);
         rc |= dalloc(&b->grid, nodes);
         if (b->kind == CRMPM_HIP_STANDARD)
@@ -3730,8 +3758,8 @@ namespace
         if (mc != hipSuccess)
             return CRMPM_HIP_ERR_LAUNCH;
         hipStream_t st = b->stream;
-        rc |= hipMemsetAsync(b->F, 0, 3 * N * sizeof(float4), st);
-        rc |= hipMemsetAsync(b->C, 0, 3 * N * sizeof(float4), st);
+        rc |= hipMemsetAsync(b->F, 0, 9 * N * sizeof(float), st);
+        rc |= hipMemsetAsync(b->C, 0, 9 * N * sizeof(float), st);
         rc |= hipMemsetAsync(b->V0, 0, N * sizeof(float), st);
         rc |= hipMemsetAsync(b->grid, 0, nodes * sizeof(float4), st);
         if (b->gridForce)
@@ -3871,8 +3899,8 @@ namespace
         };
         add(b->P.xm, N * sizeof(float4));
         add(b->P.v, N * sizeof(float4));
-        add(b->F, 3 * N * sizeof(float4));
-        add(b->C, 3 * N * sizeof(float4));
+        add(b->F, 9 * N * sizeof(float));
+        add(b->C, 9 * N * sizeof(float));
         add(b->V0, N * sizeof(float));
         add(b->grid, (size_t)b->totalBlocks * 64 * sizeof(float4));
         add(b->gridForce, (size_t)b->totalBlocks * 64 * sizeof(float4));
@@ -4589,9 +4617,9 @@ int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, in
     if (!b->initialized)
         return CRMPM_HIP_ERR_STATE;
     if (F)
-        *F = reinterpret_cast<float *>(b->F);
+        *F = b->F;
     if (C)
-        *C = reinterpret_cast<float *>(b->C);
+        *C = b->C;
     if (V0)
         *V0 = b->V0;
     if (stride)
@@ -4865,7 +4893,7 @@ namespace
     struct StateBuf
     {
         float4 *x = nullptr, *v = nullptr;
-        float4 *F = nullptr, *C = nullptr;
+        float *F = nullptr, *C = nullptr; // as SolverState
     };
 } // namespace
 
@@ -4900,8 +4928,8 @@ namespace
     {
         int rc = dalloc(&s.x, N);
         rc |= dalloc(&s.v, N);
-        rc |= dalloc(&s.F, 3 * N);
-        rc |= dalloc(&s.C, 3 * N);
+        rc |= dalloc(&s.F, 9 * N);
+        rc |= dalloc(&s.C, 9 * N);
         return rc;
     }
     void state_free(StateBuf &s)
@@ -4920,7 +4948,7 @@ namespace
                    hipSuccess;
         };
         const bool ok = cp(b->P.xm, s.x, N * sizeof(float4)) && cp(b->P.v, s.v, N * sizeof(float4)) &&
-                        cp(b->F, s.F, 3 * N * sizeof(float4)) && cp(b->C, s.C, 3 * N * sizeof(float4));
+                        cp(b->F, s.F, 9 * N * sizeof(float)) && cp(b->C, s.C, 9 * N * sizeof(float));
         return ok ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
     StateRef state_ref(const StateBuf &s, int stride) { return StateRef{s.x, s.v, s.F, s.C, stride}; }

@@ -314,13 +314,13 @@ def mesh_slicer_sdf(points, triangles, cell_size, fatten, spine, device="cuda:0"
 
 def _read_state(torch, device, stream, F, C, V0, stride, a, e):
     """F, C as (n, 9) with element 3 * column + row, and V0 (n,), of particles
-    a..e-1, float64.  The solver keeps F and C as three float4 columns per
-    particle and V0 as one float per particle."""
+    a..e-1, float64.  The solver keeps F and C as 9 floats per particle in
+    that order and V0 as one float per particle."""
     out = {}
     for name, ptr in (("F", F), ("C", C)):
-        t = torch.empty((stride, 3, 4), dtype=torch.float32, device=device)
+        t = torch.empty((stride, 9), dtype=torch.float32, device=device)
         hip_memcpy_d2d(t.data_ptr(), ptr, t.numel() * 4, stream)
-        out[name] = t[a:e, :, :3].reshape(e - a, 9).cpu().numpy().astype(np.float64)
+        out[name] = t[a:e].cpu().numpy().astype(np.float64)
     t = torch.empty(stride, dtype=torch.float32, device=device)
     hip_memcpy_d2d(t.data_ptr(), V0, stride * 4, stream)
     out["V0"] = t[a:e].cpu().numpy().astype(np.float64)
@@ -943,10 +943,10 @@ class DifferentiableMls:
         F, C, V0, stride = self.solver._state_ptrs
         self.stride = stride
         dev = self.solver.device
-        eye = torch.zeros((stride, 3, 4), dtype=torch.float32, device=dev)
-        eye[:, 0, 0] = eye[:, 1, 1] = eye[:, 2, 2] = 1.0
+        eye = torch.zeros((stride, 9), dtype=torch.float32, device=dev)
+        eye[:, 0] = eye[:, 4] = eye[:, 8] = 1.0
         self._eye = eye
-        self._zero = torch.zeros((stride, 3, 4), dtype=torch.float32, device=dev)
+        self._zero = torch.zeros((stride, 9), dtype=torch.float32, device=dev)
 
     def _reset(self, lam, mu, v0):
         s = self.solver

@@ -131,10 +131,9 @@ int crHipSolverFetchResults(CrHipSolver *s);
 
 /* Solver-owned per-particle state, which the reference keeps private
  * (src/solver/gpu_mls_mpm_solver.cuh:61-77).  Device pointers into the
- * solver's arrays.  F and C hold three float4 columns per particle, as the
- * reference's Mat3x3f: element (row r, column c) of particle p is
- * F[12 * p + 4 * c + r] (the fourth float of a column is padding); same for
- * C; V0[p].  stride is the number of particle slots.  Used for checkpoints,
+ * solver's arrays.  F and C hold 9 floats per particle, the reference
+ * Mat3x3f's element order with its column padding dropped: element (row r,
+ * column c) of particle p is F[9 * p + 3 * c + r]; same for C; V0[p].  stride is the number of particle slots.  Used for checkpoints,
  * the adjoint and parity checks. */
 int crHipSolverGetStateArrays(CrHipSolver *s, float **F, float **C, float **V0, int *stride);
 /* Binning stage of a substep, exposed for verification; it has no counterpart
