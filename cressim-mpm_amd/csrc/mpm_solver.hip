@@ -154,6 +154,10 @@ namespace
         // particles), activeCount entries.
         const unsigned *activeList;
         const int *activeCount;
+        // Occupied pblocks (global keys), compacted, tileCount entries: the
+        // items of the persistent tile loops.
+        const unsigned *tileList;
+        const int *tileCount;
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
@@ -240,29 +244,24 @@ namespace
         return count > 0 && (j == 0 || count > j * subSize);
     }
     CR_DEV bool subtile_used(const BatchArgs &b, int tg) { return split_used(b, tg, b.subSize, b.subMax); }
-    // First occupied item at or after tg of a persistent tile loop's items
-    // tg, tg + W, tg + 2W, ... (W = wavefronts in the launch): the 64 lanes
-    // test 64 candidates with one load each, so a wavefront passes the empty
-    // pblocks of sparse grids without a dependent load per pblock and visits
-    // the same items in the same order.  Taken when items are whole pblocks
-    // (subMax 1) and W <= 2^24, where t + 64 W stays below 2^32; otherwise,
-    // and in a non-persistent launch (one item per wavefront), the loop body
-    // keeps its own test.
-    template <bool Persistent, int Waves = kWaves>
-    CR_DEV int skip_empty(const BatchArgs &b, int tg, int items, int subMax)
+    // Items of a tile loop.  Non-persistent (a launch of one wavefront per
+    // slot): every (sub-)tile slot of the batch, each item its slot.
+    // Persistent (a launch of any size whose wavefronts stride over the
+    // items): the occupied pblocks' sub-tiles, from the compacted list
+    // k_active_blocks builds, so no wavefront visits an empty pblock.
+    template <bool Persistent>
+    CR_DEV int loop_items(const BatchArgs &b, int slots, int subMax)
     {
-        if (!Persistent || subMax != 1 || tile_stride_of<Waves>() > (1 << 24))
-            return tg;
-        const unsigned W = (unsigned)tile_stride_of<Waves>(), n = (unsigned)items;
-        const unsigned lane = threadIdx.x & 63u;
-        for (unsigned t = (unsigned)tg; t < n; t += 64u * W)
-        {
-            const unsigned c = t + lane * W;
-            const unsigned long long m = __ballot(c < n && b.segEnd[c] > b.segStart[c]);
-            if (m != 0ull)
-                return __builtin_amdgcn_readfirstlane((int)(t + (unsigned)(__ffsll((long long)m) - 1) * W));
-        }
-        return items;
+        return Persistent ? __builtin_amdgcn_readfirstlane(*b.tileCount) * subMax : slots;
+    }
+    // The slot (global pblock * subMax + sub-tile) of item it.
+    template <bool Persistent>
+    CR_DEV int loop_tile(const BatchArgs &b, int it, int subMax)
+    {
+        if (!Persistent)
+            return it;
+        const int i = it / subMax;
+        return __builtin_amdgcn_readfirstlane((int)b.tileList[i]) * subMax + (it - i * subMax);
     }
     // Next pblock of a tile loop; a non-persistent launch (one wavefront per
     // pblock) compiles to straight-line code, which needs fewer registers.
@@ -489,7 +488,10 @@ namespace
         const int b0 = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * 64;
         const int b = b0 + lane;
         if (b == 0)
-            *activeCount = 0;
+        {
+            activeCount[0] = 0; // active node blocks
+            activeCount[1] = 0; // occupied pblocks
+        }
         bool wasDirty = false;
         if (b < totalBlocks)
         {
@@ -1434,9 +1436,10 @@ namespace
         static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
         __shared__ float4 acc[Waves * kScatterTiles][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
+        const int nIt = loop_items<Persistent>(b, num_subtiles(b), b.subMax);
+        for (int it = first_tile_of<Waves>(wave); it < nIt; it = next_tile<Persistent, Waves>(it, nIt))
         {
+            const int tg = loop_tile<Persistent>(b, it, b.subMax);
             if (!subtile_used(b, tg))
                 continue;
             int t;
@@ -1527,7 +1530,9 @@ namespace
     // grid updates that follow (20 per substep in PB-MPM).  One thread per
     // global block; each wavefront appends its blocks with one atomic, so the
     // list's order varies between runs while each block's result does not.
-    __global__ __launch_bounds__(256) void k_active_blocks(BatchArgs b, unsigned *list, int *count)
+    // Also appends each occupied pblock (bit 0 of its own node block's mask)
+    // to tiles (count[1] entries): the items of the persistent tile loops.
+    __global__ __launch_bounds__(256) void k_active_blocks(BatchArgs b, unsigned *list, int *count, unsigned *tiles)
     {
         const int nbg = blockIdx.x * blockDim.x + threadIdx.x;
         const int lane = threadIdx.x & 63;
@@ -1557,11 +1562,16 @@ namespace
         if (lane == 0 && act != 0ull)
             base = atomicAdd(count, __popcll(act));
         base = __shfl(base, 0, 64);
+        const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
         if (mask != 0u)
-        {
-            const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
             list[base + __popcll(act & below)] = ((unsigned)nbg << 8) | mask;
-        }
+        const unsigned long long occ = __ballot((mask & 1u) != 0u);
+        int tbase = 0;
+        if (lane == 0 && occ != 0ull)
+            tbase = atomicAdd(count + 1, __popcll(occ));
+        tbase = __shfl(tbase, 0, 64);
+        if (mask & 1u)
+            tiles[tbase + __popcll(occ & below)] = (unsigned)nbg;
     }
 
     // Multi: pblocks may have more than one sub-tile (subMax > 1); the
@@ -1799,10 +1809,10 @@ namespace
         __shared__ float4 lg[Waves][kTileNodes];
         __shared__ ChunkStage stage[Waves];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), items, b.gSubMax); tg < items;
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
+        const int nIt = loop_items<Persistent>(b, b.totalBlocks * b.gSubMax, b.gSubMax);
+        for (int it = first_tile_of<Waves>(wave); it < nIt; it = next_tile<Persistent, Waves>(it, nIt))
         {
+            const int tg = loop_tile<Persistent>(b, it, b.gSubMax);
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
             int t;
@@ -2035,11 +2045,10 @@ namespace
         using Acc = typename std::conditional<Mass, float4, F3>::type;
         __shared__ Acc acc[Waves * kAcc][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent, Waves>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax);
-             tg < num_subtiles(b);
-             tg = skip_empty<Persistent, Waves>(b, next_tile<Persistent, Waves>(tg, num_subtiles(b)), num_subtiles(b),
-                                                b.subMax))
+        const int nIt = loop_items<Persistent>(b, num_subtiles(b), b.subMax);
+        for (int it = first_tile_of<Waves>(wave); it < nIt; it = next_tile<Persistent, Waves>(it, nIt))
         {
+            const int tg = loop_tile<Persistent>(b, it, b.subMax);
             if (!subtile_used(b, tg))
                 continue;
             int t;
@@ -2096,9 +2105,10 @@ namespace
         static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
         __shared__ float4 acc[Waves][2][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), num_subtiles(b), b.subMax); tg < num_subtiles(b);
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, num_subtiles(b)), num_subtiles(b), b.subMax))
+        const int nIt = loop_items<Persistent>(b, num_subtiles(b), b.subMax);
+        for (int it = first_tile_of<Waves>(wave); it < nIt; it = next_tile<Persistent, Waves>(it, nIt))
         {
+            const int tg = loop_tile<Persistent>(b, it, b.subMax);
             if (!subtile_used(b, tg))
                 continue;
             int t;
@@ -2174,10 +2184,10 @@ namespace
         static_assert(Persistent ? Waves == kWaves : true, "persistent tile loops stride by kWaves per workgroup");
         __shared__ float4 lg[Waves][kTileNodes];
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int items = b.totalBlocks * b.gSubMax;
-        for (int tg = skip_empty<Persistent>(b, first_tile_of<Waves>(wave), items, b.gSubMax); tg < items;
-             tg = skip_empty<Persistent>(b, next_tile<Persistent>(tg, items), items, b.gSubMax))
+        const int nIt = loop_items<Persistent>(b, b.totalBlocks * b.gSubMax, b.gSubMax);
+        for (int it = first_tile_of<Waves>(wave); it < nIt; it = next_tile<Persistent, Waves>(it, nIt))
         {
+            const int tg = loop_tile<Persistent>(b, it, b.gSubMax);
             if (!split_used(b, tg, b.gSubSize, b.gSubMax))
                 continue;
             int t;
@@ -3143,7 +3153,8 @@ struct CrHipBatch
     unsigned char *dirty = nullptr;
     int *segStart = nullptr, *segEnd = nullptr;
     unsigned *activeList = nullptr; // node blocks with partials this substep
-    int *activeCount = nullptr;
+    int *activeCount = nullptr;     // [0] active node blocks, [1] occupied pblocks
+    unsigned *tileList = nullptr;   // occupied pblocks this substep
     unsigned *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
     unsigned *radixCounts = nullptr, *digitTotals = nullptr;
     float4 *tilebuf = nullptr, *tileForce = nullptr;
@@ -3352,6 +3363,8 @@ namespace
         a.gatherOrder = b->gatherOrder;
         a.activeList = b->activeList;
         a.activeCount = b->activeCount;
+        a.tileList = b->tileList;
+        a.tileCount = b->activeCount + 1;
         a.tilebuf = b->tilebuf;
         a.tileForce = b->tileForce;
         a.grid = b->grid;
@@ -3422,7 +3435,7 @@ namespace
         launch(b, CRMPM_HIP_PROF_BINNING, k_segments, grid1d(n, 256), 256, n, (int)b->totalBlocks, *keysOut, b->segStart,
                b->segEnd);
         launch(b, CRMPM_HIP_PROF_BINNING, k_active_blocks, grid1d(b->totalBlocks, 256), 256, batch_args(b, nullptr),
-               b->activeList, b->activeCount);
+               b->activeList, b->activeCount, b->tileList);
         // the sort's spare ping-pong buffers take the cell ranks and the tile-ordered permutation
         unsigned *tmp = (*keysOut == b->keysA) ? b->keysB : b->keysA;
         unsigned *order2 = (*orderOut == b->valsA) ? b->valsB : b->valsA;
@@ -3621,6 +3634,7 @@ namespace
         dfree(b->dirty);
         dfree(b->activeList);
         dfree(b->activeCount);
+        dfree(b->tileList);
         dfree(b->segStart);
         dfree(b->segEnd);
         dfree(b->keysA);
@@ -3690,7 +3704,8 @@ namespace
             rc |= dalloc(&b->gridForce, nodes);
         rc |= dalloc(&b->dirty, blocks);
         rc |= dalloc(&b->activeList, blocks);
-        rc |= dalloc(&b->activeCount, 1);
+        rc |= dalloc(&b->activeCount, 2);
+        rc |= dalloc(&b->tileList, blocks);
         rc |= dalloc(&b->segStart, blocks);
         rc |= dalloc(&b->segEnd, blocks);
         rc |= dalloc(&b->keysA, N);
