@@ -125,6 +125,10 @@ PROFILE_TARGETS = {
                      counters=True,
                      title="Kernel trace and counters: MLS-MPM elastic cube, 32,768 particles, 205 substeps",
                      md="mls_cube32k_kernel_trace.md"),
+    "pb_adjoint": dict(run="python3 bench.py --workload pb_adjoint", counters=False,
+                       title="Kernel trace: differentiable PB block, 65,536 particles, 20 iterations, 20 substeps, "
+                             "forward + adjoint",
+                       md="pb_adjoint65k_kernel_trace.md"),
     "batched": dict(run="python3 bench.py --workload batched", counters=False,
                     title="Kernel trace: 64 batched scenes (MLS 100 substeps, PB 20 substeps) and the same scenes "
                           "as single solvers",
