@@ -4913,6 +4913,8 @@ namespace
 } // namespace
 
 // Particles per sub-tile of the reverse (adjoint) kernels.
+// (128 and 256 measured slower for both adjoints: the reverse kernels' per-particle
+// work needs the wavefronts more than the grid adjoint needs fewer partials)
 constexpr int kAdjSubSize = 64;
 
 struct CrHipAdjoint

@@ -75,7 +75,9 @@ def reference_grads(sc, steps, gx, dtype=torch.float64):
         x, _, _, _ = torch_rollout(sc, alpha, ratio, v0, steps=steps)
     else:
         x, _, _, _ = fp32_rollout(sc, alpha, ratio, v0, steps)
-    ga, gr, gv = torch.autograd.grad(x, (alpha, ratio, v0), grad_outputs=torch.as_tensor(gx, dtype=dtype))
+    grads = torch.autograd.grad(x, (alpha, ratio, v0), grad_outputs=torch.as_tensor(gx, dtype=dtype), allow_unused=True)
+    # without solver iterations alpha and the ratio do not enter the rollout
+    ga, gr, gv = (torch.zeros_like(t) if g is None else g for g, t in zip(grads, (alpha, ratio, v0)))
     return x.detach().double().numpy(), ga.double().numpy(), gr.double().numpy(), gv.double().numpy()
 
 
@@ -119,3 +121,41 @@ def test_pb_adjoint_checkpoint_interval_independent():
     b = run_case(sc, steps, steps)
     for u, w in zip(a[2:], b[2:]):
         assert np.array_equal(u, w)
+
+
+def test_pb_adjoint_without_iterations_is_ballistic():
+    """No solver iterations: a substep is the integration alone, so the final
+    positions are x0 + steps dt v0 + const and dL/dv0 = steps dt dL/dx
+    exactly (closed form), with zero alpha and ratio gradients."""
+    sc = pb_block(seed=13, iterations=0)
+    steps = 6
+    x, gx, da, dr, dv = run_case(sc, steps, 4)
+    dt = float(np.float32(sc["dt"]))
+    assert np.all(da == 0.0) and np.all(dr == 0.0)
+    np.testing.assert_allclose(dv, gx * steps * dt, rtol=1e-5, atol=1e-12)
+    xr, ra, rr, rv = reference_grads(sc, steps, gx)
+    assert float(np.abs(x - xr).max()) < 1e-6
+    assert relnorm(dv, rv) < 1e-5
+
+
+def test_pb_adjoint_masked_particles():
+    """Masked particles neither move nor take part (the PB solver honours the
+    mask): their v0, alpha and ratio gradients are zero when the loss is on
+    positions, and the active particles' gradients equal those of the same
+    block without the masked particles (FP64 restatement), at the bounds above."""
+    sc = pb_block(seed=5)
+    sc["mask"][::5] = 0
+    steps = 8
+    act = sc["mask"] != 0
+    x, gx, da, dr, dv = run_case(sc, steps, 3)
+    assert np.all(dv[~act] == 0.0) and np.all(da[~act] == 0.0) and np.all(dr[~act] == 0.0)
+    np.testing.assert_array_equal(x[~act], sc["xm"][~act, :3].astype(np.float64))
+    sub = {k: (v[act] if isinstance(v, np.ndarray) and v.shape[:1] == act.shape else v) for k, v in sc.items()}
+    xr, ra, rr, rv = reference_grads(sub, steps, gx[act])
+    _, fa, fr, fv = reference_grads(sub, steps, gx[act], torch.float32)
+    assert float(np.abs(x[act] - xr).max()) < 1e-5
+    for name, g, r, f, cap in (("alpha", da[act], ra, fa, 1e-1), ("ratio", dr[act], rr, fr, 1e-1),
+                               ("v0", dv[act], rv, fv, 1e-3)):
+        eg, e32 = relnorm(g, r), relnorm(f, r)
+        print(f"  masked block d/d{name}: GPU vs FP64 {eg:.3e}, FP32 restatement vs FP64 {e32:.3e}", flush=True)
+        assert eg <= min(cap, max(2.0 * e32, 1e-5)), (name, eg, e32)
