@@ -1965,11 +1965,22 @@ namespace
         const int S = st.stride;
         if (Mode == 0)
         {
-            const float2 pr = st.ps[k];
-            const int type = st.ts[k];
-            M3 Cm = load_m3(st.Cs, S, k);
-            const M3 F = load_m3(st.Fs, S, k);
-            pb_constraint(c.dt, pr.x, pr.y, type, F, Cm);
+            // the substep's staging (k_pb_stage's gather into sorted order),
+            // then the first iteration's constraint on the staged C
+            const int p = (int)c.order[k];
+            st.xs[k] = c.P.xm[p];
+            const float4 v = c.P.v[p];
+            st.vs[k] = v.x;
+            st.vs[S + k] = v.y;
+            st.vs[2 * S + k] = v.z;
+            const M3 F = load_pm(c.S.F, p);
+            M3 Cm = load_pm(c.S.C, p);
+            const float4 prm = c.P.params[p];
+            const int type = c.P.type[p];
+            store_m3(st.Fs, S, k, F);
+            st.ps[k] = make_float2(prm.x, prm.y);
+            st.ts[k] = type;
+            pb_constraint(c.dt, prm.x, prm.y, type, F, Cm);
             store_m3(st.Cs, S, k, Cm);
             return;
         }
@@ -3514,11 +3525,11 @@ namespace
         }
         else
         {
-            launch(b, CRMPM_HIP_PROF_BINNING, k_pb_stage, grid1d(n, 256), 256, n, (int)b->totalBlocks, keys, order, b->P,
-                   a.S, b->st);
             const int K = b->iterations;
             if (K == 0)
             {
+                launch(b, CRMPM_HIP_PROF_BINNING, k_pb_stage, grid1d(n, 256), 256, n, (int)b->totalBlocks, keys, order, b->P,
+                       a.S, b->st);
                 launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_integrate_only, grid1d(n, 256), 256, a, n, keys, b->st);
             }
             else
