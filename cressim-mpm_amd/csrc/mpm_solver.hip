@@ -2763,18 +2763,20 @@ namespace
     // forward formed them.  The reverse kernels work in the substep's tile
     // order on its staged particle arrays (x|m, F, (alpha, r), type):
     //   k_pbadj_integrate    the integration's adjoint, per particle
-    //   k = K..1             k_pbadj_g2p (per tile: the velocity adjoint
-    //                        scattered into tile partials, the weights' x
-    //                        adjoint), k_adj_grid<1> (momentum and mass
-    //                        adjoints), k_pbadj_prep (per particle: the
-    //                        scatter's inputs re-formed), k_pbadj_scatter (per
-    //                        tile: the momentum and mass adjoints gathered),
+    //   k_pbadj_g2p_x        the weights' x adjoint of iteration K's G2P
+    //   k = K..1             k_pbadj_uscatter (per pblock: the velocity
+    //                        adjoint scattered into tile partials),
+    //                        k_adj_grid<1> (momentum and mass adjoints),
+    //                        k_pbadj_prep (per particle: the scatter's inputs
+    //                        re-formed), k_pbadj_gather (per particle: the
+    //                        momentum and mass adjoints gathered),
     //                        k_pbadj_constraint (per particle: the constraint's
-    //                        adjoint through the SVD's polar rotation)
+    //                        adjoint through the SVD's polar rotation, then
+    //                        iteration k-1's x adjoint of the weights)
     //   k_pbadj_finalize     the adjoint of F^-T; back to particle order
-    // As in the forward, the per-particle work (gather, SVD) runs 64 sorted
-    // particles per wavefront and the per-tile kernels only scatter or
-    // gather through LDS.
+    // As in the forward, the per-particle work (gathers, SVD) runs 64 sorted
+    // particles per wavefront, reading the grids from global memory, and
+    // only the scatter is per pblock, through LDS.
     // ------------------------------------------------------------------
 
     // Adjoint state of the substep being reversed, in tile (sorted) order.
@@ -2892,59 +2894,87 @@ namespace
         store_cols(pa.Fi, q, z);
     }
 
-    // G2P adjoint of iteration k: U is the iteration's updated grid (the
-    // tape); the adjoint of (v_k, C_k) comes from pa.  The weights' x
-    // adjoint is a gather over the staged grid tile (no write conflicts);
-    // the grid velocity adjoint, w (v_bar + B_bar^T d) with B_bar =
-    // (4 / dx) C_bar, has the form of the forward scatter's momentum and
-    // goes through it (pb_scatter: separable node sums, three accumulation
-    // tiles in conflict-free passes) into the tile partials.
-    __global__ __launch_bounds__(64) void k_pbadj_g2p(BatchArgs b, PbStage st, const float4 *U, PbAdjRef pa)
+    // The weights' x adjoint of iteration k's G2P for one particle:
+    // v_k = sum_i w U_i, C_k = (4 / dx) sum_i w d U_i^T, so with B_bar =
+    // (4 / dx) C_bar it is sum_i grad w (v_bar . U_i + d . B_bar U_i) -
+    // w B_bar U_i.  U (the taped grid of the particle's scene) is read from
+    // global memory: every in-grid node of a particle lies in its pblock's
+    // box, and out-of-grid nodes are skipped, as in the forward's gather.
+    CR_DEV V3 pbadj_g2p_x(const GridGeom &g, const float4 *U, V3 x, V3 vb, const M3 &Bb)
+    {
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        NodeW nw;
+        node_weights(fx, nw);
+        V3 xacc{0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+            for (int b = 0; b < 3; ++b)
+#pragma unroll 3
+                for (int c = 0; c < 3; ++c)
+                {
+                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    if (!in_grid(g, nx, ny, nz))
+                        continue;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, c);
+                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, b), dc = dw_of(fx.z, c);
+                    const float wt = wa * wb * wc;
+                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                    const V3 dist = (V3{(float)a, (float)b, (float)c} - fx) * g.dx;
+                    const V3 u = v3(U[grid_index(g, nx, ny, nz)]);
+                    const V3 Bu = mul(Bb, u);
+                    xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
+                }
+        return xacc;
+    }
+
+    // The weights' x adjoint of the last iteration's G2P (the earlier
+    // iterations' is added by k_pbadj_constraint), one thread per sorted
+    // particle; U is iteration K's taped grid (the batch's layout).
+    __global__ __launch_bounds__(kWG) void k_pbadj_g2p_x(BatchArgs b, PbStage st, int n, const unsigned *keys,
+                                                         const float4 *U, PbAdjRef pa)
+    {
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const StepCommon c = scene_view_of(b, 0);
+        const M3 Bb = load_cols(pa.C, q) * (4.0f * c.g.invdx);
+        const V3 xs = v3(pa.x[q]) + pbadj_g2p_x(c.g, U + (c.grid - b.grid), v3(st.xs[q]), v3(pa.v[q]), Bb);
+        pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+    }
+
+    // The grid velocity adjoint of iteration k's G2P, w (v_bar + B_bar^T d)
+    // per node: the form of the forward scatter's momentum with m = 1, so it
+    // goes through pb_scatter (separable node sums, three accumulation tiles
+    // in conflict-free passes) into the tile partials; one pblock per
+    // one-wavefront workgroup, so the grid adjoint sums at most 8 partials
+    // per node.
+    __global__ __launch_bounds__(64) void k_pbadj_uscatter(BatchArgs b, PbStage st, PbAdjRef pa)
     {
         constexpr int kAcc = 3;
         __shared__ F3 acc[kAcc][kTileNodes];
-        __shared__ float4 lg[1][kTileNodes];
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        const int lane = threadIdx.x & 63;
+        const int tg = first_tile_of<1>(0);
+        if (tg >= num_subtiles(b) || !subtile_used(b, tg))
+            return;
+        int t;
+        const StepCommon c = scene_view(b, tg / b.subMax, t);
+        TileInfo ti;
+        if (!subtile_info(c, t, tg % b.subMax, ti))
+            return;
+        F3 *A = acc[0];
+        zero_tiles<kAcc>(A, lane);
+        wave_sync();
+        for (int q = ti.start + lane; q < ti.end; q += 64)
         {
-            if (!subtile_used(b, tg))
-                continue;
-            int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
-            TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
-                continue;
-            const GridGeom &g = c.g;
-            F3 *A = acc[wave * kAcc];
-            float4 *G = lg[wave];
-            zero_tiles<kAcc>(A, lane);
-            load_tile_grid(g, ti, U + (c.grid - b.grid), G, lane);
-            wave_sync();
-            for (int q = ti.start + lane; q < ti.end; q += 64)
-            {
-                const V3 x = v3(st.xs[q]);
-                const V3 vb = v3(pa.v[q]);
-                const M3 Bb = load_cols(pa.C, q) * (4.0f * g.invdx);
-                V3 xacc{0.f, 0.f, 0.f};
-                {
-                    int bx, by, bz;
-                    V3 fx;
-                    cell_of(g, x, bx, by, bz, fx);
-                    NodeW nw;
-                    node_weights(fx, nw);
-                    for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
-                        const V3 u = v3(G[li]);
-                        const V3 Bu = mul(Bb, u);
-                        xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
-                    });
-                }
-                const V3 xs = v3(pa.x[q]) + xacc;
-                pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
-                pb_scatter<kAcc>(c, ti, A, 0, x, 1.0f, vb, transpose(Bb));
-            }
-            wave_sync();
-            store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
+            const M3 Bb = load_cols(pa.C, q) * (4.0f * c.g.invdx);
+            pb_scatter<kAcc>(c, ti, A, 0, v3(st.xs[q]), 1.0f, v3(pa.v[q]), transpose(Bb));
         }
+        wave_sync();
+        store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
     }
 
     // The forward's per-particle inputs of iteration k's scatter, re-formed
@@ -2987,63 +3017,66 @@ namespace
         store_cols(pp.Cc, q, Cc);
     }
 
-    // P2G adjoint of iteration k (per tile): gathers the momentum and mass
-    // adjoints (gridAdj) into the adjoint of v_{k-1} (pa.v) and of the
-    // constraint's output (pa.C; accumulated in FP64, a first moment with
-    // cancellation), and adds the weights' x adjoint.
-    __global__ __launch_bounds__(64) void k_pbadj_scatter(BatchArgs b, PbStage st, const float4 *gridAdj,
-                                                           PbAdjPrep pp, PbAdjRef pa)
+    // P2G adjoint of iteration k, one thread per sorted particle: the
+    // momentum and mass adjoints (gridAdj, read from global memory)
+    // gathered into the adjoint of v_{k-1} (pa.v) and of the constraint's
+    // output (pa.C; accumulated in FP64, a first moment with cancellation),
+    // and the weights' x adjoint added to pa.x.
+    __global__ __launch_bounds__(kWG) void k_pbadj_gather(BatchArgs b, PbStage st, int n, const unsigned *keys,
+                                                          const float4 *gridAdj, PbAdjPrep pp, PbAdjRef pa)
     {
-        __shared__ float4 lg[1][kTileNodes];
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
-        {
-            if (!subtile_used(b, tg))
-                continue;
-            int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
-            TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
-                continue;
-            const GridGeom &g = c.g;
-            float4 *G = lg[wave];
-            load_tile_grid(g, ti, gridAdj + (c.grid - b.grid), G, lane);
-            wave_sync();
-            for (int q = ti.start + lane; q < ti.end; q += 64)
-            {
-                const float4 xm = st.xs[q];
-                const V3 x = v3(xm);
-                const float m = xm.w;
-                const V3 vel = v3(pp.v[q]);
-                const M3 Cc = load_cols(pp.Cc, q);
-                const M3 CcT = transpose(Cc);
-                V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
-                D3 Ccb = d3_zero();
-                int bx, by, bz;
-                V3 fx;
-                cell_of(g, x, bx, by, bz, fx);
-                NodeW nw;
-                node_weights(fx, nw);
-                for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
-                    const float4 gq = G[li];
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const StepCommon c = scene_view_of(b, 0);
+        const GridGeom &g = c.g;
+        const float4 *G = gridAdj + (c.grid - b.grid);
+        const float4 xm = st.xs[q];
+        const V3 x = v3(xm);
+        const float m = xm.w;
+        const V3 vel = v3(pp.v[q]);
+        const M3 Cc = load_cols(pp.Cc, q);
+        const M3 CcT = transpose(Cc);
+        V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
+        D3 Ccb = d3_zero();
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        NodeW nw;
+        node_weights(fx, nw);
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll 3
+                for (int cc = 0; cc < 3; ++cc)
+                {
+                    const int nx = bx + a, ny = by + bb, nz = bz + cc;
+                    if (!in_grid(g, nx, ny, nz))
+                        continue;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, bb), wc = w_of(fx.z, cc);
+                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, bb), dc = dw_of(fx.z, cc);
+                    const float wt = wa * wb * wc;
+                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                    const V3 dist = (V3{(float)a, (float)bb, (float)cc} - fx) * g.dx;
+                    const float4 gq = G[grid_index(g, nx, ny, nz)];
                     const V3 pb = v3(gq);
                     const float wm = wt * m;
                     vpb = vpb + pb * wm;
-                    const double pr3[3] = {pb.x, pb.y, pb.z}, dc[3] = {dist.x, dist.y, dist.z};
+                    const double pr3[3] = {pb.x, pb.y, pb.z}, dcs[3] = {dist.x, dist.y, dist.z};
 #pragma unroll
                     for (int r = 0; r < 3; ++r)
 #pragma unroll
-                        for (int cc = 0; cc < 3; ++cc)
-                            Ccb.d[r][cc] += (double)wm * pr3[r] * dc[cc];
+                        for (int k2 = 0; k2 < 3; ++k2)
+                            Ccb.d[r][k2] += (double)wm * pr3[r] * dcs[k2];
                     const float wbar = m * (dot(pb, vel + mul(Cc, dist)) + gq.w);
                     xacc = xacc + gw * wbar - mul(CcT, pb) * wm;
-                });
-                const V3 xs = v3(pa.x[q]) + xacc;
-                pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
-                pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
-                store_cols(pa.C, q, m3_of(Ccb));
-            }
-        }
+                }
+        const V3 xs = v3(pa.x[q]) + xacc;
+        pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+        pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
+        store_cols(pa.C, q, m3_of(Ccb));
     }
 
     // The constraint's adjoint of iteration k, one thread per sorted
@@ -3051,28 +3084,39 @@ namespace
     // receives that of its input C_{k-1}; the F and F^-T adjoints are
     // accumulated in pa, those of alpha and the ratio in FP64 per particle
     // (ad.lambdaBar, ad.muBar).  Other materials pass the adjoint through.
+    // Then, for k > 1, the weights' x adjoint of iteration k-1's G2P
+    // (pbadj_g2p_x with the adjoint of (v_{k-1}, C_{k-1}) just formed and the
+    // taped grid Uprev).
     __global__ __launch_bounds__(kWG) void k_pbadj_constraint(BatchArgs b, PbStage st, int n, const unsigned *keys,
-                                                              PbAdjPrep pp, PbAdjRef pa, AdjRef ad)
+                                                              PbAdjPrep pp, PbAdjRef pa, AdjRef ad, const float4 *Uprev)
     {
         const int q = blockIdx.x * kWG + threadIdx.x;
         const unsigned key = q < n ? keys[q] : 0xffffffffu;
         if (key >= (unsigned)b.totalBlocks)
             return;
-        const int type = st.ts[q];
-        if (type != mNeoHookean && type != mCoRotational)
-            return;
         const StepCommon c = scene_view_of(b, 0);
-        const float2 pr = st.ps[q];
-        const M3 F = load_m3(st.Fs, st.stride, q);
-        M3 Fb = load_cols(pa.F, q), Fib = load_cols(pa.Fi, q);
-        double ab = 0.0, rb = 0.0;
-        const M3 Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, load_cols(pp.C, q), load_cols(pa.C, q), Fb, Fib, ab, rb);
-        store_cols(pa.C, q, Cb);
-        store_cols(pa.F, q, Fb);
-        store_cols(pa.Fi, q, Fib);
-        const int p = (int)c.order[q];
-        ad.lambdaBar[p] += ab;
-        ad.muBar[p] += rb;
+        const int type = st.ts[q];
+        M3 Cb = load_cols(pa.C, q);
+        if (type == mNeoHookean || type == mCoRotational)
+        {
+            const float2 pr = st.ps[q];
+            const M3 F = load_m3(st.Fs, st.stride, q);
+            M3 Fb = load_cols(pa.F, q), Fib = load_cols(pa.Fi, q);
+            double ab = 0.0, rb = 0.0;
+            Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, load_cols(pp.C, q), Cb, Fb, Fib, ab, rb);
+            store_cols(pa.C, q, Cb);
+            store_cols(pa.F, q, Fb);
+            store_cols(pa.Fi, q, Fib);
+            const int p = (int)c.order[q];
+            ad.lambdaBar[p] += ab;
+            ad.muBar[p] += rb;
+        }
+        if (Uprev)
+        {
+            const V3 xs = v3(pa.x[q]) + pbadj_g2p_x(c.g, Uprev + (c.grid - b.grid), v3(st.xs[q]), v3(pa.v[q]),
+                                                    Cb * (4.0f * c.g.invdx));
+            pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+        }
     }
 
     // End of a substep's reverse: the adjoint of F^-T added to F's
@@ -5025,23 +5069,29 @@ namespace
                     continue;
                 BatchArgs args = batch_args(b, b->lastOrder);
                 args.gatherOrder = args.order;
-                args.subSize = kAdjSubSize;
-                args.subMax = b->subStride;
-                const int subItems = std::max(1, (int)b->totalBlocks * args.subMax); // one-wavefront workgroups, one per sub-tile
+                // the velocity-adjoint scatter and the grid adjoint take whole
+                // pblocks (the grid adjoint then sums at most 8 partials per node)
+                BatchArgs whole = args;
+                whole.subSize = 1 << 30;
+                whole.subMax = 1;
+                const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
                 const int blockWgs = grid1d(b->totalBlocks, kWaves);
                 const unsigned *keys = b->lastKeys;
                 const int denseWgs = grid1d(n, kWG);
                 k_pbadj_integrate<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, ad, pa);
+                if (K > 0)
+                    k_pbadj_g2p_x<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, a->tape + (size_t)(K - 1) * nodes, pa);
                 for (int k = K; k >= 1; --k)
                 {
                     const float4 *U = a->tape + (size_t)(k - 1) * nodes;
-                    k_pbadj_g2p<<<subItems, 64, 0, st>>>(args, b->st, U, pa);
-                    k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(args, U, a->gridAdj);
+                    const float4 *Uprev = k > 1 ? U - nodes : nullptr;
+                    k_pbadj_uscatter<<<wholeItems, 64, 0, st>>>(whole, b->st, pa);
+                    k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
                     BatchArgs prev = args;
-                    prev.grid = const_cast<float4 *>(U - nodes); // unused for k = 1
+                    prev.grid = const_cast<float4 *>(Uprev); // unused for k = 1
                     k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, b->st, n, keys, k == 1, state_ref(a->seg[j], stride), pp);
-                    k_pbadj_scatter<<<subItems, 64, 0, st>>>(args, b->st, a->gridAdj, pp, pa);
-                    k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, pp, pa, ad);
+                    k_pbadj_gather<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, a->gridAdj, pp, pa);
+                    k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, pp, pa, ad, Uprev);
                 }
                 k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, pa, ad);
                 rc = check_launch();
