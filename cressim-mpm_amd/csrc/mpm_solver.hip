@@ -2334,6 +2334,21 @@ namespace
         c.S.V0[p] = (density > 1e-8f) ? x.w / density : 1e-8f;
     }
 
+    // The node blocks of the substep's active list copied to dst (the
+    // batch's grid layout; other blocks are left as they are): what a later
+    // reader of a particle's stencil needs, since every in-grid node of a
+    // particle's stencil lies in a block that received partials.
+    __global__ __launch_bounds__(kWG) void k_copy_active_blocks(BatchArgs b, const float4 *src, float4 *dst)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int nActive = __builtin_amdgcn_readfirstlane(*b.activeCount);
+        for (int item = first_tile(wave); item < nActive; item += tile_stride())
+        {
+            const size_t nbg = __builtin_amdgcn_readfirstlane(b.activeList[item]) >> 8;
+            dst[nbg * 64 + lane] = src[nbg * 64 + lane];
+        }
+    }
+
     __global__ void k_copy_grid(BatchArgs b, int s, float4 *dst)
     {
         const StepCommon c = scene_view_of(b, s);
@@ -2865,9 +2880,10 @@ namespace
     // Adjoint of the integration (pbMpmIntegrateParticle without shapes),
     // one thread per sorted particle: the adjoint of the substep's end state
     // (particle order, ad) to that of (v_K, C_K) and the direct x and F
-    // terms (tile order, pa).  The live particle state is the substep's end
-    // state; F is the staged start F.
-    __global__ void k_pbadj_integrate(BatchArgs b, PbStage st, int n, const unsigned *keys, AdjRef ad, PbAdjRef pa)
+    // terms (tile order, pa).  s1 is the substep's end state; F is the
+    // staged start F.
+    __global__ void k_pbadj_integrate(BatchArgs b, PbStage st, int n, const unsigned *keys, StateRef s1, AdjRef ad,
+                                      PbAdjRef pa)
     {
         const int q = blockIdx.x * blockDim.x + threadIdx.x;
         if (q >= n || keys[q] >= (unsigned)b.totalBlocks)
@@ -2875,8 +2891,8 @@ namespace
         const StepCommon c = scene_view_of(b, b.numScenes == 1 ? 0 : b.blockScene[keys[q]]);
         const GridGeom &g = c.g;
         const int p = (int)c.order[q];
-        const V3 xe = v3(c.P.xm[p]);
-        const M3 CK = load_pm(c.S.C, p);
+        const V3 xe = v3(s1.x[p]);
+        const M3 CK = load_pm(s1.C, p);
         const M3 F = load_m3(st.Fs, st.stride, q);
         const V3 xb = v3(ad.x[p]), vb = v3(ad.v[p]);
         const M3 Fb = load_cols(ad.F, p), Cb = load_cols(ad.C, p);
@@ -3240,8 +3256,9 @@ struct CrHipBatch
     // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
     const unsigned *lastKeys = nullptr;
-    // PB: when set, every iteration's updated grid is copied to
-    // pbTape + iteration * nodes (the adjoint's re-run of a substep)
+    // PB: when set, every iteration's updated grid (its active node blocks,
+    // in the grid's layout) is copied to pbTape + iteration * nodes (the
+    // adjoint's re-run of a substep)
     float4 *pbTape = nullptr;
     const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
 
@@ -3580,11 +3597,10 @@ namespace
             {
                 const int denseWGs = grid1d(n, kWG);
                 const size_t nodes = (size_t)b->totalBlocks * 64;
-                bool taped = true;
                 auto tape = [&](int it) {
                     if (b->pbTape)
-                        taped = taped && hipMemcpyAsync(b->pbTape + (size_t)it * nodes, b->grid, nodes * sizeof(float4),
-                                                        hipMemcpyDeviceToDevice, b->stream) == hipSuccess;
+                        launch(b, CRMPM_HIP_PROF_UPDATE, k_copy_active_blocks, updWGs, kWG, a, (const float4 *)b->grid,
+                               b->pbTape + (size_t)it * nodes);
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
                 launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
@@ -3599,8 +3615,7 @@ namespace
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
                     tape(it);
                 }
-                if (!taped)
-                    return CRMPM_HIP_ERR_LAUNCH;
+
                 launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_dense<2>, denseWGs, kWG, a, b->st, n, keys);
             }
         }
@@ -4986,10 +5001,11 @@ struct CrHipAdjoint
     float4 *gridAdj = nullptr;
     float4 *xTmp = nullptr;
     float4 *FTmp = nullptr;
-    // PB: the iterations' grids of the substep being reversed, and its
-    // adjoint state in tile order
+    // PB: the tape (a segment's substeps, or the one substep being reversed;
+    // see pb_reverse), its size and budget, and the adjoint state in tile order
     float4 *tape = nullptr;
-    int tapeIters = 0;
+    size_t tapeBytes = 0;
+    size_t tapeBudget = size_t(8) << 30;
     float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
     float4 *ppV = nullptr, *ppC = nullptr, *ppCc = nullptr;
 };
@@ -5025,76 +5041,201 @@ namespace
     }
     StateRef state_ref(const StateBuf &s, int stride) { return StateRef{s.x, s.v, s.F, s.C, stride}; }
 
-    // Reverse sweep of a PB rollout.  Per segment: re-run it from its
-    // checkpoint keeping every substep's start state; then, last substep
-    // first, re-run the substep from its start state with the iterations'
-    // grids taped and apply the reverse kernels (tile order of that run).
-    int pb_reverse(CrHipAdjoint *a, const AdjRef &ad)
+    // What the reverse kernels of one PB substep read of its forward run:
+    // the iterations' grids, the sort products and the staged particle
+    // arrays (x|m, F, (alpha, ratio), type in tile order).
+    struct PbSubstepTape
+    {
+        const float4 *U; // K grids of the batch's node count
+        const unsigned *order, *keys;
+        const int *segStart, *segEnd;
+        const unsigned *activeList;
+        const int *activeCount;
+        PbStage st;
+    };
+
+    // The reverse kernels of one PB substep; s0 and s1 are its start and
+    // end states.
+    int pb_reverse_substep(CrHipAdjoint *a, const AdjRef &ad, const PbSubstepTape &tp, StateRef s0, StateRef s1)
     {
         CrHipBatch *b = a->solver->b;
         hipStream_t st = b->stream;
         const size_t nodes = (size_t)b->totalBlocks * 64;
-        const int K = b->iterations;
-        if (K > a->tapeIters)
-        {
-            (void)hipStreamSynchronize(st);
-            dfree(a->tape);
-            a->tapeIters = 0;
-            if (dalloc(&a->tape, (size_t)K * nodes))
-                return CRMPM_HIP_ERR_ALLOC;
-            a->tapeIters = K;
-        }
+        const int K = b->iterations, n = b->totalActive;
+        if (n == 0)
+            return CRMPM_HIP_OK;
         const PbAdjRef pa{a->paX, a->paV, a->paC, a->paF, a->paFi};
         const PbAdjPrep pp{a->ppV, a->ppC, a->ppCc};
+        BatchArgs args = batch_args(b, tp.order);
+        args.gatherOrder = tp.order;
+        args.segStart = tp.segStart;
+        args.segEnd = tp.segEnd;
+        args.activeList = tp.activeList;
+        args.activeCount = tp.activeCount;
+        // the velocity-adjoint scatter and the grid adjoint take whole pblocks
+        // (the grid adjoint then sums at most 8 partials per node)
+        BatchArgs whole = args;
+        whole.subSize = 1 << 30;
+        whole.subMax = 1;
+        const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
+        const int blockWgs = grid1d(b->totalBlocks, kWaves);
+        const int denseWgs = grid1d(n, kWG);
+        const unsigned *keys = tp.keys;
+        k_pbadj_integrate<<<grid1d(n, 256), 256, 0, st>>>(args, tp.st, n, keys, s1, ad, pa);
+        if (K > 0)
+            k_pbadj_g2p_x<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, tp.U + (size_t)(K - 1) * nodes, pa);
+        for (int k = K; k >= 1; --k)
+        {
+            const float4 *U = tp.U + (size_t)(k - 1) * nodes;
+            const float4 *Uprev = k > 1 ? U - nodes : nullptr;
+            k_pbadj_uscatter<<<wholeItems, 64, 0, st>>>(whole, tp.st, pa);
+            k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
+            BatchArgs prev = args;
+            prev.grid = const_cast<float4 *>(Uprev); // unused for k = 1
+            k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, tp.st, n, keys, k == 1, s0, pp);
+            k_pbadj_gather<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, a->gridAdj, pp, pa);
+            k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, pp, pa, ad, Uprev);
+        }
+        k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, tp.st, n, keys, pa, ad);
+        return check_launch();
+    }
+
+    // Bytes per substep of a whole segment's tape: the iterations' grids,
+    // the sort products (order, keys, segments, active list and count) and
+    // the staged x|m, F, (alpha, ratio) and type.
+    size_t pb_substep_tape_bytes(const CrHipBatch *b, int K)
+    {
+        const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
+        return (size_t)K * B * 64 * sizeof(float4) + N * (2 * sizeof(unsigned) + sizeof(float4) + 9 * sizeof(float) +
+                                                          sizeof(float2) + sizeof(int)) +
+               B * (2 * sizeof(int) + sizeof(unsigned)) + 2 * sizeof(int);
+    }
+
+    int pb_ensure_tape(CrHipAdjoint *a, int K, int substeps)
+    {
+        CrHipBatch *b = a->solver->b;
+        const size_t need = pb_substep_tape_bytes(b, K) * (size_t)substeps;
+        if (a->tape && a->tapeBytes >= need)
+            return CRMPM_HIP_OK;
+        (void)hipStreamSynchronize(b->stream);
+        dfree(a->tape);
+        a->tapeBytes = 0;
+        if (dalloc(reinterpret_cast<char **>(&a->tape), need))
+            return CRMPM_HIP_ERR_ALLOC;
+        a->tapeBytes = need;
+        return CRMPM_HIP_OK;
+    }
+
+    // The tape of substep j of a segment, carved from a->tape.
+    PbSubstepTape pb_tape_slot(CrHipAdjoint *a, int K, int j)
+    {
+        CrHipBatch *b = a->solver->b;
+        const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
+        char *m = reinterpret_cast<char *>(a->tape) + pb_substep_tape_bytes(b, K) * (size_t)j;
+        PbSubstepTape t{};
+        auto take = [&](size_t bytes) {
+            char *r = m;
+            m += bytes;
+            return r;
+        };
+        t.U = reinterpret_cast<const float4 *>(take((size_t)K * B * 64 * sizeof(float4)));
+        t.st.xs = reinterpret_cast<float4 *>(take(N * sizeof(float4)));
+        t.st.Fs = reinterpret_cast<float *>(take(9 * N * sizeof(float)));
+        t.st.ps = reinterpret_cast<float2 *>(take(N * sizeof(float2)));
+        t.order = reinterpret_cast<const unsigned *>(take(N * sizeof(unsigned)));
+        t.keys = reinterpret_cast<const unsigned *>(take(N * sizeof(unsigned)));
+        t.st.ts = reinterpret_cast<int *>(take(N * sizeof(int)));
+        t.segStart = reinterpret_cast<const int *>(take(B * sizeof(int)));
+        t.segEnd = reinterpret_cast<const int *>(take(B * sizeof(int)));
+        t.activeList = reinterpret_cast<const unsigned *>(take(B * sizeof(unsigned)));
+        t.activeCount = reinterpret_cast<const int *>(take(2 * sizeof(int)));
+        t.st.vs = nullptr;
+        t.st.Cs = nullptr;
+        t.st.stride = (int)N;
+        return t;
+    }
+
+    // Copy of what the reverse kernels read of the substep just run (its
+    // grids were taped by batch_step_taped) into a segment tape slot.
+    int pb_save_substep(CrHipBatch *b, const PbSubstepTape &t)
+    {
+        const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
+        const int n = b->totalActive;
+        hipStream_t st = b->stream;
+        auto cp = [&](const void *dst, const void *src, size_t bytes) {
+            return hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
+        };
+        const bool ok = n == 0 ||
+                        (cp(t.st.xs, b->st.xs, N * sizeof(float4)) && cp(t.st.Fs, b->st.Fs, 9 * N * sizeof(float)) &&
+                         cp(t.st.ps, b->st.ps, N * sizeof(float2)) && cp(t.st.ts, b->st.ts, N * sizeof(int)) &&
+                         cp(t.order, b->lastOrder, N * sizeof(unsigned)) && cp(t.keys, b->lastKeys, N * sizeof(unsigned)) &&
+                         cp(t.segStart, b->segStart, B * sizeof(int)) && cp(t.segEnd, b->segEnd, B * sizeof(int)) &&
+                         cp(t.activeList, b->activeList, B * sizeof(unsigned)) &&
+                         cp(t.activeCount, b->activeCount, 2 * sizeof(int)));
+        return ok ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
+    }
+
+    // Reverse sweep of a PB rollout.  Per segment (checkpoint interval):
+    // when a whole segment's tape fits the adjoint's tape budget, the
+    // segment is re-run once from its checkpoint with every substep's
+    // iteration grids, sort products and staged particle arrays kept, and
+    // then reversed; otherwise the re-run keeps only every substep's start
+    // state and each substep is re-run once more, last first, with its own
+    // tape before its reverse.  Both give bitwise the same gradients.
+    int pb_reverse(CrHipAdjoint *a, const AdjRef &ad)
+    {
+        CrHipBatch *b = a->solver->b;
+        const int K = b->iterations;
         const int stride = b->capacity;
-        int rc = CRMPM_HIP_OK;
+        const bool whole = pb_substep_tape_bytes(b, K) * (size_t)a->every <= a->tapeBudget;
+        int rc = pb_ensure_tape(a, K, whole ? a->every : 1);
+        const StateRef live{b->P.xm, b->P.v, b->F, b->C, stride};
         for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
         {
             const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
             rc = state_copy(b, a->ckpt[sgm], false);
             for (int t = t0; t < t1 && !rc; ++t)
             {
-                rc = state_copy(b, a->seg[t - t0], true);
-                if (!rc && t + 1 < t1)
+                const int j = t - t0;
+                rc = state_copy(b, a->seg[j], true);
+                if (rc)
+                    break;
+                if (whole)
+                {
+                    const PbSubstepTape tp = pb_tape_slot(a, K, j);
+                    rc = batch_step_taped(b, const_cast<float4 *>(tp.U));
+                    if (!rc)
+                        rc = pb_save_substep(b, tp);
+                }
+                else if (t + 1 < t1)
                     rc = batch_step(b);
             }
+            if (!rc && whole)
+                rc = state_copy(b, a->seg[t1 - t0], true);
             for (int t = t1 - 1; t >= t0 && !rc; --t)
             {
                 const int j = t - t0;
+                if (whole)
+                {
+                    rc = pb_reverse_substep(a, ad, pb_tape_slot(a, K, j), state_ref(a->seg[j], stride),
+                                            state_ref(a->seg[j + 1], stride));
+                    continue;
+                }
                 rc = state_copy(b, a->seg[j], false);
+                PbSubstepTape tp{};
+                tp.U = a->tape;
                 if (!rc)
                     rc = batch_step_taped(b, a->tape);
-                const int n = b->totalActive;
-                if (rc || n == 0)
-                    continue;
-                BatchArgs args = batch_args(b, b->lastOrder);
-                args.gatherOrder = args.order;
-                // the velocity-adjoint scatter and the grid adjoint take whole
-                // pblocks (the grid adjoint then sums at most 8 partials per node)
-                BatchArgs whole = args;
-                whole.subSize = 1 << 30;
-                whole.subMax = 1;
-                const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
-                const int blockWgs = grid1d(b->totalBlocks, kWaves);
-                const unsigned *keys = b->lastKeys;
-                const int denseWgs = grid1d(n, kWG);
-                k_pbadj_integrate<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, ad, pa);
-                if (K > 0)
-                    k_pbadj_g2p_x<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, a->tape + (size_t)(K - 1) * nodes, pa);
-                for (int k = K; k >= 1; --k)
-                {
-                    const float4 *U = a->tape + (size_t)(k - 1) * nodes;
-                    const float4 *Uprev = k > 1 ? U - nodes : nullptr;
-                    k_pbadj_uscatter<<<wholeItems, 64, 0, st>>>(whole, b->st, pa);
-                    k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
-                    BatchArgs prev = args;
-                    prev.grid = const_cast<float4 *>(Uprev); // unused for k = 1
-                    k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, b->st, n, keys, k == 1, state_ref(a->seg[j], stride), pp);
-                    k_pbadj_gather<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, a->gridAdj, pp, pa);
-                    k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, b->st, n, keys, pp, pa, ad, Uprev);
-                }
-                k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, b->st, n, keys, pa, ad);
-                rc = check_launch();
+                if (rc)
+                    break;
+                tp.order = b->lastOrder;
+                tp.keys = b->lastKeys;
+                tp.segStart = b->segStart;
+                tp.segEnd = b->segEnd;
+                tp.activeList = b->activeList;
+                tp.activeCount = b->activeCount;
+                tp.st = b->st;
+                rc = pb_reverse_substep(a, ad, tp, state_ref(a->seg[j], stride), live);
             }
         }
         return rc;
@@ -5211,6 +5352,14 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         return CRMPM_HIP_ERR_ALLOC;
     }
     *out = a;
+    return CRMPM_HIP_OK;
+}
+
+int crHipAdjointSetTapeBudget(CrHipAdjoint *a, unsigned long long bytes)
+{
+    if (!a)
+        return CRMPM_HIP_ERR_ARG;
+    a->tapeBudget = (size_t)bytes;
     return CRMPM_HIP_OK;
 }
 

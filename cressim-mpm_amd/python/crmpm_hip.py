@@ -158,6 +158,7 @@ EXPORTS = {
     "crHipAdjointCreate": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
     "crHipAdjointForward": [_vp],
     "crHipAdjointBackward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "crHipAdjointSetTapeBudget": [_vp, ctypes.c_ulonglong],
     "crHipAdjointDestroy": [_vp],
 }
 
@@ -1051,6 +1052,12 @@ class DifferentiablePb(DifferentiableMls):
     iterations are the scene's."""
 
     KIND = S.PB
+
+    def set_tape_budget(self, nbytes):
+        """Device memory the backward pass may take for a whole checkpoint
+        segment's tape (default 8 GiB); below it each substep is re-run
+        twice in the backward pass instead of once."""
+        check(self.solver.lib.crHipAdjointSetTapeBudget(self.h, int(nbytes)), "adjoint tape budget")
 
 
 _HIPRT = None

@@ -510,6 +510,15 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
 int crHipAdjointForward(CrHipAdjoint *a);
 int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, double *dLdLambda, double *dLdMu,
                          float *dLdx0, float *dLdv0);
+/* PB only (no counterpart in the reference; for callers that trade device
+ * memory for time): the device memory the backward pass may take for a
+ * whole checkpoint segment's tape -- every substep's iteration grids, sort
+ * products and staged particle arrays, about
+ * iterations x nodes x 16 B + 72 B x particles per substep -- so that each
+ * substep is re-run once in the backward pass instead of twice.  Default
+ * 8 GiB; a segment that does not fit is reversed substep by substep.  The
+ * gradients are bitwise the same either way. */
+int crHipAdjointSetTapeBudget(CrHipAdjoint *a, unsigned long long bytes);
 int crHipAdjointDestroy(CrHipAdjoint *a);
 
 #ifdef __cplusplus
