@@ -2782,12 +2782,11 @@ namespace
     //   k = K..1             k_pbadj_uscatter (per pblock: the velocity
     //                        adjoint scattered into tile partials),
     //                        k_adj_grid<1> (momentum and mass adjoints),
-    //                        k_pbadj_prep (per particle: the scatter's inputs
-    //                        re-formed), k_pbadj_gather (per particle: the
-    //                        momentum and mass adjoints gathered),
-    //                        k_pbadj_constraint (per particle: the constraint's
-    //                        adjoint through the SVD's polar rotation, then
-    //                        iteration k-1's x adjoint of the weights)
+    //                        k_pbadj_particle (per particle: the scatter's
+    //                        inputs re-formed, the momentum and mass adjoints
+    //                        gathered, the constraint's adjoint through the
+    //                        SVD's polar rotation, iteration k-1's x adjoint
+    //                        of the weights)
     //   k_pbadj_finalize     the adjoint of F^-T; back to particle order
     // As in the forward, the per-particle work (gathers, SVD) runs 64 sorted
     // particles per wavefront, reading the grids from global memory, and
@@ -2947,7 +2946,7 @@ namespace
     }
 
     // The weights' x adjoint of the last iteration's G2P (the earlier
-    // iterations' is added by k_pbadj_constraint), one thread per sorted
+    // iterations' is added by k_pbadj_particle), one thread per sorted
     // particle; U is iteration K's taped grid (the batch's layout).
     __global__ __launch_bounds__(kWG) void k_pbadj_g2p_x(BatchArgs b, PbStage st, int n, const unsigned *keys,
                                                          const float4 *U, PbAdjRef pa)
@@ -2993,146 +2992,118 @@ namespace
         store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
     }
 
-    // The forward's per-particle inputs of iteration k's scatter, re-formed
-    // for its reverse (tile order).
-    struct PbAdjPrep
-    {
-        float4 *v;  // v_{k-1}
-        float4 *C;  // C_{k-1} (the constraint's input), 3 columns
-        float4 *Cc; // the constraint's output, 3 columns
-    };
-
-    // The per-particle half of forward iteration k again (k_pb_dense's
-    // arithmetic): (v_{k-1}, C_{k-1}) gathered from the previous iteration's
-    // grid -- b.grid is that grid -- or, for k = 1, read from the substep's
-    // start state s0, and the constraint applied.
-    __global__ __launch_bounds__(kWG) void k_pbadj_prep(BatchArgs b, PbStage st, int n, const unsigned *keys,
-                                                        int first, StateRef s0, PbAdjPrep pp)
+    // The per-particle part of reverse iteration k, one thread per sorted
+    // particle (the phases run in one kernel: at the sizes of a
+    // differentiable scene a launch holds about one wavefront per SIMD, so
+    // each launch costs its latency):
+    //  1. the forward's inputs of iteration k's scatter re-formed
+    //     (k_pb_dense's arithmetic): (v_{k-1}, C_{k-1}) gathered from the
+    //     previous iteration's taped grid Uprev -- or, for k = 1, read from
+    //     the substep's start state s0 -- and the constraint applied;
+    //  2. P2G adjoint: the momentum and mass adjoints (gridAdj) gathered into
+    //     the adjoint of v_{k-1} and of the constraint's output (the latter
+    //     accumulated in FP64, a first moment with cancellation), and the
+    //     weights' x adjoint;
+    //  3. the constraint's adjoint (active materials; others pass the
+    //     adjoint through): the adjoint of C_{k-1}, the F and F^-T adjoints
+    //     accumulated in pa, those of alpha and the ratio in FP64 per particle
+    //     (ad.lambdaBar, ad.muBar);
+    //  4. for k > 1, the weights' x adjoint of iteration k-1's G2P.
+    // Grids are read from global memory (every in-grid node of a particle's
+    // stencil lies in a block that received partials).
+    __global__ __launch_bounds__(kWG) void k_pbadj_particle(BatchArgs b, PbStage st, int n, const unsigned *keys,
+                                                            StateRef s0, const float4 *gridAdj, const float4 *Uprev,
+                                                            PbAdjRef pa, AdjRef ad)
     {
         const int q = blockIdx.x * kWG + threadIdx.x;
         const unsigned key = q < n ? keys[q] : 0xffffffffu;
         if (key >= (unsigned)b.totalBlocks)
             return;
-        const StepCommon c = scene_view_of(b, 0);
+        StepCommon c = scene_view_of(b, 0);
+        const GridGeom &g = c.g;
+        const size_t off = (size_t)(c.grid - b.grid); // the scene's first node in the batch's layout
+        const float4 xm = st.xs[q];
+        const V3 x = v3(xm);
+        const float m = xm.w;
+        // 1. the scatter's inputs
         V3 vel;
         M3 Cm;
-        if (first)
+        if (!Uprev)
         {
             const int p = (int)c.order[q];
             vel = v3(s0.v[p]);
             Cm = load_pm(s0.C, p);
         }
         else
-            gather_vc_grid(c, 0, v3(st.xs[q]), V3{0.f, 0.f, 0.f}, vel, Cm);
+        {
+            c.grid = const_cast<float4 *>(Uprev) + off;
+            gather_vc_grid(c, 0, x, V3{0.f, 0.f, 0.f}, vel, Cm);
+        }
         const M3 F = load_m3(st.Fs, st.stride, q);
         const float2 pr = st.ps[q];
+        const int type = st.ts[q];
         M3 Cc = Cm;
-        pb_constraint(c.dt, pr.x, pr.y, st.ts[q], F, Cc);
-        pp.v[q] = make_float4(vel.x, vel.y, vel.z, 0.0f);
-        store_cols(pp.C, q, Cm);
-        store_cols(pp.Cc, q, Cc);
-    }
-
-    // P2G adjoint of iteration k, one thread per sorted particle: the
-    // momentum and mass adjoints (gridAdj, read from global memory)
-    // gathered into the adjoint of v_{k-1} (pa.v) and of the constraint's
-    // output (pa.C; accumulated in FP64, a first moment with cancellation),
-    // and the weights' x adjoint added to pa.x.
-    __global__ __launch_bounds__(kWG) void k_pbadj_gather(BatchArgs b, PbStage st, int n, const unsigned *keys,
-                                                          const float4 *gridAdj, PbAdjPrep pp, PbAdjRef pa)
-    {
-        const int q = blockIdx.x * kWG + threadIdx.x;
-        const unsigned key = q < n ? keys[q] : 0xffffffffu;
-        if (key >= (unsigned)b.totalBlocks)
-            return;
-        const StepCommon c = scene_view_of(b, 0);
-        const GridGeom &g = c.g;
-        const float4 *G = gridAdj + (c.grid - b.grid);
-        const float4 xm = st.xs[q];
-        const V3 x = v3(xm);
-        const float m = xm.w;
-        const V3 vel = v3(pp.v[q]);
-        const M3 Cc = load_cols(pp.Cc, q);
+        pb_constraint(c.dt, pr.x, pr.y, type, F, Cc);
+        // 2. P2G adjoint
+        const float4 *G = gridAdj + off;
         const M3 CcT = transpose(Cc);
         V3 vpb{0.f, 0.f, 0.f}, xacc{0.f, 0.f, 0.f};
         D3 Ccb = d3_zero();
-        int bx, by, bz;
-        V3 fx;
-        cell_of(g, x, bx, by, bz, fx);
-        NodeW nw;
-        node_weights(fx, nw);
+        {
+            int bx, by, bz;
+            V3 fx;
+            cell_of(g, x, bx, by, bz, fx);
+            NodeW nw;
+            node_weights(fx, nw);
 #pragma unroll 1
-        for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll 1
-            for (int bb = 0; bb < 3; ++bb)
+                for (int bb = 0; bb < 3; ++bb)
 #pragma unroll 3
-                for (int cc = 0; cc < 3; ++cc)
-                {
-                    const int nx = bx + a, ny = by + bb, nz = bz + cc;
-                    if (!in_grid(g, nx, ny, nz))
-                        continue;
-                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, bb), wc = w_of(fx.z, cc);
-                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, bb), dc = dw_of(fx.z, cc);
-                    const float wt = wa * wb * wc;
-                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
-                    const V3 dist = (V3{(float)a, (float)bb, (float)cc} - fx) * g.dx;
-                    const float4 gq = G[grid_index(g, nx, ny, nz)];
-                    const V3 pb = v3(gq);
-                    const float wm = wt * m;
-                    vpb = vpb + pb * wm;
-                    const double pr3[3] = {pb.x, pb.y, pb.z}, dcs[3] = {dist.x, dist.y, dist.z};
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const int nx = bx + a, ny = by + bb, nz = bz + cc;
+                        if (!in_grid(g, nx, ny, nz))
+                            continue;
+                        const float wa = w_of(fx.x, a), wb = w_of(fx.y, bb), wc = w_of(fx.z, cc);
+                        const float da = dw_of(fx.x, a), db = dw_of(fx.y, bb), dc = dw_of(fx.z, cc);
+                        const float wt = wa * wb * wc;
+                        const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                        const V3 dist = (V3{(float)a, (float)bb, (float)cc} - fx) * g.dx;
+                        const float4 gq = G[grid_index(g, nx, ny, nz)];
+                        const V3 pb = v3(gq);
+                        const float wm = wt * m;
+                        vpb = vpb + pb * wm;
+                        const double pr3[3] = {pb.x, pb.y, pb.z}, dcs[3] = {dist.x, dist.y, dist.z};
 #pragma unroll
-                    for (int r = 0; r < 3; ++r)
+                        for (int r = 0; r < 3; ++r)
 #pragma unroll
-                        for (int k2 = 0; k2 < 3; ++k2)
-                            Ccb.d[r][k2] += (double)wm * pr3[r] * dcs[k2];
-                    const float wbar = m * (dot(pb, vel + mul(Cc, dist)) + gq.w);
-                    xacc = xacc + gw * wbar - mul(CcT, pb) * wm;
-                }
-        const V3 xs = v3(pa.x[q]) + xacc;
-        pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
-        pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
-        store_cols(pa.C, q, m3_of(Ccb));
-    }
-
-    // The constraint's adjoint of iteration k, one thread per sorted
-    // particle: pa.C holds the adjoint of the constraint's output and
-    // receives that of its input C_{k-1}; the F and F^-T adjoints are
-    // accumulated in pa, those of alpha and the ratio in FP64 per particle
-    // (ad.lambdaBar, ad.muBar).  Other materials pass the adjoint through.
-    // Then, for k > 1, the weights' x adjoint of iteration k-1's G2P
-    // (pbadj_g2p_x with the adjoint of (v_{k-1}, C_{k-1}) just formed and the
-    // taped grid Uprev).
-    __global__ __launch_bounds__(kWG) void k_pbadj_constraint(BatchArgs b, PbStage st, int n, const unsigned *keys,
-                                                              PbAdjPrep pp, PbAdjRef pa, AdjRef ad, const float4 *Uprev)
-    {
-        const int q = blockIdx.x * kWG + threadIdx.x;
-        const unsigned key = q < n ? keys[q] : 0xffffffffu;
-        if (key >= (unsigned)b.totalBlocks)
-            return;
-        const StepCommon c = scene_view_of(b, 0);
-        const int type = st.ts[q];
-        M3 Cb = load_cols(pa.C, q);
+                            for (int k2 = 0; k2 < 3; ++k2)
+                                Ccb.d[r][k2] += (double)wm * pr3[r] * dcs[k2];
+                        const float wbar = m * (dot(pb, vel + mul(Cc, dist)) + gq.w);
+                        xacc = xacc + gw * wbar - mul(CcT, pb) * wm;
+                    }
+        }
+        // 3. the constraint's adjoint
+        M3 Cb = m3_of(Ccb);
         if (type == mNeoHookean || type == mCoRotational)
         {
-            const float2 pr = st.ps[q];
-            const M3 F = load_m3(st.Fs, st.stride, q);
             M3 Fb = load_cols(pa.F, q), Fib = load_cols(pa.Fi, q);
             double ab = 0.0, rb = 0.0;
-            Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, load_cols(pp.C, q), Cb, Fb, Fib, ab, rb);
-            store_cols(pa.C, q, Cb);
+            Cb = pb_constraint_adjoint(c.dt, pr.x, pr.y, F, Cm, Cb, Fb, Fib, ab, rb);
             store_cols(pa.F, q, Fb);
             store_cols(pa.Fi, q, Fib);
             const int p = (int)c.order[q];
             ad.lambdaBar[p] += ab;
             ad.muBar[p] += rb;
         }
+        // 4. iteration k-1's weights' x adjoint
         if (Uprev)
-        {
-            const V3 xs = v3(pa.x[q]) + pbadj_g2p_x(c.g, Uprev + (c.grid - b.grid), v3(st.xs[q]), v3(pa.v[q]),
-                                                    Cb * (4.0f * c.g.invdx));
-            pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
-        }
+            xacc = xacc + pbadj_g2p_x(g, Uprev + off, x, vpb, Cb * (4.0f * g.invdx));
+        const V3 xs = v3(pa.x[q]) + xacc;
+        pa.x[q] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+        pa.v[q] = make_float4(vpb.x, vpb.y, vpb.z, 0.0f);
+        store_cols(pa.C, q, Cb);
     }
 
     // End of a substep's reverse: the adjoint of F^-T added to F's
@@ -5007,7 +4978,6 @@ struct CrHipAdjoint
     size_t tapeBytes = 0;
     size_t tapeBudget = size_t(8) << 30;
     float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
-    float4 *ppV = nullptr, *ppC = nullptr, *ppCc = nullptr;
 };
 
 namespace
@@ -5065,7 +5035,6 @@ namespace
         if (n == 0)
             return CRMPM_HIP_OK;
         const PbAdjRef pa{a->paX, a->paV, a->paC, a->paF, a->paFi};
-        const PbAdjPrep pp{a->ppV, a->ppC, a->ppCc};
         BatchArgs args = batch_args(b, tp.order);
         args.gatherOrder = tp.order;
         args.segStart = tp.segStart;
@@ -5090,11 +5059,7 @@ namespace
             const float4 *Uprev = k > 1 ? U - nodes : nullptr;
             k_pbadj_uscatter<<<wholeItems, 64, 0, st>>>(whole, tp.st, pa);
             k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
-            BatchArgs prev = args;
-            prev.grid = const_cast<float4 *>(Uprev); // unused for k = 1
-            k_pbadj_prep<<<denseWgs, kWG, 0, st>>>(prev, tp.st, n, keys, k == 1, s0, pp);
-            k_pbadj_gather<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, a->gridAdj, pp, pa);
-            k_pbadj_constraint<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, pp, pa, ad, Uprev);
+            k_pbadj_particle<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, s0, a->gridAdj, Uprev, pa, ad);
         }
         k_pbadj_finalize<<<grid1d(n, 256), 256, 0, st>>>(args, tp.st, n, keys, pa, ad);
         return check_launch();
@@ -5278,9 +5243,6 @@ namespace
         dfree(a->paC);
         dfree(a->paF);
         dfree(a->paFi);
-        dfree(a->ppV);
-        dfree(a->ppC);
-        dfree(a->ppCc);
         delete a;
         return CRMPM_HIP_OK;
     }
@@ -5342,9 +5304,6 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->paC, 3 * N);
         rc |= dalloc(&a->paF, 3 * N);
         rc |= dalloc(&a->paFi, 3 * N);
-        rc |= dalloc(&a->ppV, N);
-        rc |= dalloc(&a->ppC, 3 * N);
-        rc |= dalloc(&a->ppCc, 3 * N);
     }
     if (rc)
     {
