@@ -158,6 +158,9 @@ namespace
         // items of the persistent tile loops.
         const unsigned *tileList;
         const int *tileCount;
+        // When set, the grid update also writes each updated node here (the
+        // batch's grid layout): the adjoint's tape of a PB iteration.
+        float4 *tapeGrid;
         float4 *tilebuf;
         float4 *tileForce;
         float4 *grid;
@@ -1699,7 +1702,11 @@ namespace
                     s.z = 0.0f;
             }
             if (inGrid)
+            {
                 c.grid[nb * 64 + lane] = s;
+                if (b.tapeGrid)
+                    b.tapeGrid[(size_t)nbg * 64 + lane] = s;
+            }
         }
     }
 
@@ -2332,21 +2339,6 @@ namespace
                     density += nd * wt;
                 }
         c.S.V0[p] = (density > 1e-8f) ? x.w / density : 1e-8f;
-    }
-
-    // The node blocks of the substep's active list copied to dst (the
-    // batch's grid layout; other blocks are left as they are): what a later
-    // reader of a particle's stencil needs, since every in-grid node of a
-    // particle's stencil lies in a block that received partials.
-    __global__ __launch_bounds__(kWG) void k_copy_active_blocks(BatchArgs b, const float4 *src, float4 *dst)
-    {
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int nActive = __builtin_amdgcn_readfirstlane(*b.activeCount);
-        for (int item = first_tile(wave); item < nActive; item += tile_stride())
-        {
-            const size_t nbg = __builtin_amdgcn_readfirstlane(b.activeList[item]) >> 8;
-            dst[nbg * 64 + lane] = src[nbg * 64 + lane];
-        }
     }
 
     __global__ void k_copy_grid(BatchArgs b, int s, float4 *dst)
@@ -3227,9 +3219,9 @@ struct CrHipBatch
     // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
     const unsigned *lastKeys = nullptr;
-    // PB: when set, every iteration's updated grid (its active node blocks,
-    // in the grid's layout) is copied to pbTape + iteration * nodes (the
-    // adjoint's re-run of a substep)
+    // PB: when set, every iteration's grid update also writes its nodes (the
+    // active node blocks, in the grid's layout) to pbTape + iteration * nodes
+    // (the adjoint's re-run of a substep)
     float4 *pbTape = nullptr;
     const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
 
@@ -3408,6 +3400,7 @@ namespace
         a.activeCount = b->activeCount;
         a.tileList = b->tileList;
         a.tileCount = b->activeCount + 1;
+        a.tapeGrid = nullptr;
         a.tilebuf = b->tilebuf;
         a.tileForce = b->tileForce;
         a.grid = b->grid;
@@ -3568,23 +3561,23 @@ namespace
             {
                 const int denseWGs = grid1d(n, kWG);
                 const size_t nodes = (size_t)b->totalBlocks * 64;
-                auto tape = [&](int it) {
-                    if (b->pbTape)
-                        launch(b, CRMPM_HIP_PROF_UPDATE, k_copy_active_blocks, updWGs, kWG, a, (const float4 *)b->grid,
-                               b->pbTape + (size_t)it * nodes);
+                // the adjoint's re-run of a substep: each iteration's update
+                // also writes its nodes to the tape
+                auto taped = [&](int it) {
+                    BatchArgs t = a;
+                    t.tapeGrid = b->pbTape ? b->pbTape + (size_t)it * nodes : nullptr;
+                    return t;
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
                 launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
                        tileBlock, a, b->st);
-                launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
-                tape(0);
+                launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, taped(0), b->dirty);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
                     launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true, 1> : k_pb_scatter<false, false, 1>,
                            tileGrid, tileBlock, a, b->st);
-                    launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, a, b->dirty);
-                    tape(it);
+                    launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, taped(it), b->dirty);
                 }
 
                 launch(b, CRMPM_HIP_PROF_PB_G2P, k_pb_dense<2>, denseWGs, kWG, a, b->st, n, keys);

@@ -925,7 +925,7 @@ class DifferentiableMls:
 
     KIND = S.MLS
 
-    def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0", tune=False):
+    def __init__(self, scene, steps, checkpoint_every=10, device="cuda:0", tune=False, graph=True):
         import torch
 
         if scene["kind"] != self.KIND or scene["shape_list"]:
@@ -935,6 +935,9 @@ class DifferentiableMls:
         self.solver.compute_initial_data()
         if tune:
             self.tuned = self.solver.autotune(steps_per_trial=2)
+        # the forward pass (and the backward's re-runs that keep no tape)
+        # replay each substep as a hipGraph
+        self.solver.set_graph_mode(graph)
         self.steps = steps
         self.n = self.solver.n
         h = _vp()
