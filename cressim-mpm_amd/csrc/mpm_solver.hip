@@ -2442,76 +2442,138 @@ namespace
                 }
     }
 
-    // G2P adjoint of one substep.  s0: state at the substep's start, s1: at
-    // its end; grid: the substep's updated grid; the adjoint of s1 comes in
-    // through ad and the G2P parts of the x and F adjoints go to ad.xTmp /
-    // ad.FTmp; the grid velocity adjoint goes to the tile partials.
-    __global__ __launch_bounds__(64) void k_adj_g2p(BatchArgs b, StateRef s0, StateRef s1, const float4 *grid, AdjRef ad)
+    // The same visit for a particle whose stencil is read from a grid in
+    // global memory: fn(node index in the scene's grid, d, w, grad w).
+    template <class Fn>
+    CR_DEV void for_each_node_grad_global(const GridGeom &g, int bx, int by, int bz, V3 fx, const NodeW &n, Fn &&fn)
     {
-        __shared__ float4 acc[1][kTileNodes];
-        __shared__ float4 lg[1][kTileNodes];
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
-        {
-            if (!subtile_used(b, tg))
-                continue;
-            int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
-            TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
-                continue;
-            const GridGeom &g = c.g;
-            float4 *A = acc[wave];
-            float4 *G = lg[wave];
-            for (int i = lane; i < kTileNodes; i += 64)
-                A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            load_tile_grid(g, ti, grid + (c.grid - b.grid), G, lane);
-            wave_sync();
-            for (int k = ti.start + lane; k < ti.end; k += 64)
-            {
-                const int p = (int)c.order[k];
-                const V3 x = v3(s0.x[p]);
-                const V3 vn = v3(s1.v[p]);
-                const M3 Cn = load_pm(s1.C, p);
-                const M3 F = load_pm(s0.F, p);
-                const V3 xb = v3(ad.x[p]);
-                const V3 vb = v3(ad.v[p]);
-                const M3 Fb = load_cols(ad.F, p);
-                const M3 Cb = load_cols(ad.C, p);
-                // x+ = clamp(x + dt v+): the adjoint passes where the clamp is inactive
-                const V3 xp = x + vn * c.dt;
-                const V3 keep{(xp.x > g.bmin.x && xp.x < g.bmax.x) ? 1.f : 0.f,
-                              (xp.y > g.bmin.y && xp.y < g.bmax.y) ? 1.f : 0.f,
-                              (xp.z > g.bmin.z && xp.z < g.bmax.z) ? 1.f : 0.f};
-                const V3 xbk = xb * keep;
-                const V3 vaccb = vb + xbk * c.dt;
-                // F+ = (I + dt C+) F
-                const M3 Bb = (Cb + mul(Fb, transpose(F)) * c.dt) * (4.0f * g.invdx);
-                store_cols(ad.FTmp, p, mul(transpose(m3_identity() + Cn * c.dt), Fb));
-                V3 xacc = xbk;
-                int bx, by, bz;
-                V3 fx;
-                cell_of(g, x, bx, by, bz, fx);
-                NodeW nw;
-                node_weights(fx, nw);
-                const M3 BbT = transpose(Bb);
-                in_conflict_free_passes<1>(cell_key(ti, bx, by, bz), [&](int) {
-                    for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
-                        const V3 u = v3(G[li]);
-                        const V3 ub = (vaccb + mul(BbT, dist)) * wt;
-                        lds_add(A, li, ub.x, ub.y, ub.z, 0.0f);
-                        const V3 Bu = mul(Bb, u);
-                        const float wbar = dot(vaccb, u) + dot(dist, Bu);
-                        xacc = xacc + gw * wbar - Bu * wt;
-                    });
-                });
-                ad.xTmp[p] = make_float4(xacc.x, xacc.y, xacc.z, 0.0f);
-            }
-            wave_sync();
-            float4 *out = c.tilebuf + (size_t)ti.slot * kTileNodes;
-            for (int i = lane; i < kTileNodes; i += 64)
-                out[i] = A[i];
-        }
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+            for (int b = 0; b < 3; ++b)
+#pragma unroll 3
+                for (int c = 0; c < 3; ++c)
+                {
+                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    if (!in_grid(g, nx, ny, nz))
+                        continue;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, c);
+                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, b), dc = dw_of(fx.z, c);
+                    const float wt = wa * wb * wc;
+                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                    const V3 dist = (V3{(float)a, (float)b, (float)c} - fx) * g.dx;
+                    fn(grid_index(g, nx, ny, nz), dist, wt, gw);
+                }
+    }
+
+    // The weights' x adjoint of a G2P (MLS, or PB iteration k) for one particle:
+    // v_k = sum_i w U_i, C_k = (4 / dx) sum_i w d U_i^T, so with B_bar =
+    // (4 / dx) C_bar it is sum_i grad w (v_bar . U_i + d . B_bar U_i) -
+    // w B_bar U_i.  U (the taped grid of the particle's scene) is read from
+    // global memory: every in-grid node of a particle lies in its pblock's
+    // box, and out-of-grid nodes are skipped, as in the forward's gather.
+    CR_DEV V3 pbadj_g2p_x(const GridGeom &g, const float4 *U, V3 x, V3 vb, const M3 &Bb)
+    {
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        NodeW nw;
+        node_weights(fx, nw);
+        V3 xacc{0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+            for (int b = 0; b < 3; ++b)
+#pragma unroll 3
+                for (int c = 0; c < 3; ++c)
+                {
+                    const int nx = bx + a, ny = by + b, nz = bz + c;
+                    if (!in_grid(g, nx, ny, nz))
+                        continue;
+                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, c);
+                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, b), dc = dw_of(fx.z, c);
+                    const float wt = wa * wb * wc;
+                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
+                    const V3 dist = (V3{(float)a, (float)b, (float)c} - fx) * g.dx;
+                    const V3 u = v3(U[grid_index(g, nx, ny, nz)]);
+                    const V3 Bu = mul(Bb, u);
+                    xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
+                }
+        return xacc;
+    }
+
+    // What the MLS G2P adjoint's scatter needs per particle (tile order):
+    // the adjoint of the gathered velocity and of the gathered affine matrix.
+    struct MlsScatterIn
+    {
+        float4 *v; // v_bar (xyz)
+        float4 *B; // B_bar = (4 / dx) (C_bar + dt F_bar F^T), 3 columns
+    };
+
+    // G2P adjoint of one substep, per particle (one thread per sorted
+    // particle).  s0: state at the substep's start, s1: at its end; grid: the
+    // substep's updated grid (read from global memory: every in-grid node of
+    // a particle's stencil lies in a block that received partials).  The
+    // adjoint of s1 comes in through ad; the G2P parts of the x and F
+    // adjoints go to ad.xTmp / ad.FTmp, and the inputs of the velocity
+    // adjoint's scatter to si.
+    __global__ __launch_bounds__(kWG) void k_adj_g2p(BatchArgs b, int n, const unsigned *keys, StateRef s0, StateRef s1,
+                                                     const float4 *grid, AdjRef ad, MlsScatterIn si)
+    {
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
+        const StepCommon c = scene_view_of(b, 0);
+        const GridGeom &g = c.g;
+        const int p = (int)c.order[q];
+        const V3 x = v3(s0.x[p]);
+        const V3 vn = v3(s1.v[p]);
+        const M3 Cn = load_pm(s1.C, p);
+        const M3 F = load_pm(s0.F, p);
+        const V3 xb = v3(ad.x[p]);
+        const V3 vb = v3(ad.v[p]);
+        const M3 Fb = load_cols(ad.F, p);
+        const M3 Cb = load_cols(ad.C, p);
+        // x+ = clamp(x + dt v+): the adjoint passes where the clamp is inactive
+        const V3 xp = x + vn * c.dt;
+        const V3 keep{(xp.x > g.bmin.x && xp.x < g.bmax.x) ? 1.f : 0.f, (xp.y > g.bmin.y && xp.y < g.bmax.y) ? 1.f : 0.f,
+                      (xp.z > g.bmin.z && xp.z < g.bmax.z) ? 1.f : 0.f};
+        const V3 xbk = xb * keep;
+        const V3 vaccb = vb + xbk * c.dt;
+        // F+ = (I + dt C+) F
+        const M3 Bb = (Cb + mul(Fb, transpose(F)) * c.dt) * (4.0f * g.invdx);
+        store_cols(ad.FTmp, p, mul(transpose(m3_identity() + Cn * c.dt), Fb));
+        // v+ = sum w u_i, C+ = (4 / dx) sum w d u_i^T: the weights' x adjoint
+        const V3 xacc = xbk + pbadj_g2p_x(g, grid + (c.grid - b.grid), x, vaccb, Bb);
+        ad.xTmp[p] = make_float4(xacc.x, xacc.y, xacc.z, 0.0f);
+        si.v[q] = make_float4(vaccb.x, vaccb.y, vaccb.z, 0.0f);
+        store_cols(si.B, q, Bb);
+    }
+
+    // The grid velocity adjoint of the G2P, w (v_bar + B_bar^T d) per node,
+    // through the forward's separable scatter (pb_scatter with m = 1) into the
+    // tile partials; one pblock per one-wavefront workgroup.
+    __global__ __launch_bounds__(64) void k_adj_uscatter(BatchArgs b, StateRef s0, MlsScatterIn si)
+    {
+        constexpr int kAcc = 3;
+        __shared__ F3 acc[kAcc][kTileNodes];
+        const int lane = threadIdx.x & 63;
+        const int tg = first_tile_of<1>(0);
+        if (tg >= num_subtiles(b) || !subtile_used(b, tg))
+            return;
+        int t;
+        const StepCommon c = scene_view(b, tg / b.subMax, t);
+        TileInfo ti;
+        if (!subtile_info(c, t, tg % b.subMax, ti))
+            return;
+        F3 *A = acc[0];
+        zero_tiles<kAcc>(A, lane);
+        wave_sync();
+        for (int q = ti.start + lane; q < ti.end; q += 64)
+            pb_scatter<kAcc>(c, ti, A, 0, v3(s0.x[c.order[q]]), 1.0f, v3(si.v[q]), transpose(load_cols(si.B, q)));
+        wave_sync();
+        store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
     }
 
     // Grid adjoint: sum the tile partials of the velocity adjoint (fixed
@@ -2653,29 +2715,23 @@ namespace
         return o;
     }
 
-    // P2G adjoint of one substep: gathers (p_bar, m_bar) from the LDS tile,
-    // then the affine, stress and parameter adjoints; writes the adjoint of
-    // the substep's start state.
-    __global__ __launch_bounds__(64) void k_adj_p2g(BatchArgs b, StateRef s0, const float4 *gridAdj, AdjRef ad)
+    // P2G adjoint of one substep, one thread per sorted particle: gathers
+    // (p_bar, m_bar) from the grid adjoint in global memory, then the affine,
+    // stress and parameter adjoints; writes the adjoint of the substep's
+    // start state.
+    __global__ __launch_bounds__(kWG) void k_adj_p2g(BatchArgs b, int n, const unsigned *keys, StateRef s0,
+                                                     const float4 *gridAdj, AdjRef ad)
     {
-        __shared__ float4 lg[1][kTileNodes];
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int tg = first_tile_of<1>(wave); tg < num_subtiles(b); tg = next_tile<false>(tg, num_subtiles(b)))
+        const int q = blockIdx.x * kWG + threadIdx.x;
+        const unsigned key = q < n ? keys[q] : 0xffffffffu;
+        if (key >= (unsigned)b.totalBlocks)
+            return;
         {
-            if (!subtile_used(b, tg))
-                continue;
-            int t;
-            const StepCommon c = scene_view(b, tg / b.subMax, t);
-            TileInfo ti;
-            if (!subtile_info(c, t, tg % b.subMax, ti))
-                continue;
+            const StepCommon c = scene_view_of(b, 0);
             const GridGeom &g = c.g;
-            float4 *G = lg[wave];
-            load_tile_grid(g, ti, gridAdj + (c.grid - b.grid), G, lane);
-            wave_sync();
-            for (int k = ti.start + lane; k < ti.end; k += 64)
+            const float4 *G = gridAdj + (c.grid - b.grid);
             {
-                const int p = (int)c.order[k];
+                const int p = (int)c.order[q];
                 const float4 xm = s0.x[p];
                 const V3 x = v3(xm);
                 const float m = xm.w;
@@ -2696,8 +2752,8 @@ namespace
                 cell_of(g, x, bx, by, bz, fx);
                 NodeW nw;
                 node_weights(fx, nw);
-                for_each_node_grad(g, ti, bx, by, bz, fx, nw, [&](int li, V3 dist, float wt, V3 gw) {
-                    const float4 q = G[li];
+                for_each_node_grad_global(g, bx, by, bz, fx, nw, [&](int gi, V3 dist, float wt, V3 gw) {
+                    const float4 q = G[gi];
                     const V3 pb = v3(q);
                     vpb = vpb + pb * (wt * m);
                     const double pr[3] = {pb.x, pb.y, pb.z}, dc[3] = {dist.x, dist.y, dist.z};
@@ -2899,42 +2955,6 @@ namespace
         store_cols(pa.F, q, mul(transpose(m3_identity() + CK * c.dt), Fb));
         const M3 z{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
         store_cols(pa.Fi, q, z);
-    }
-
-    // The weights' x adjoint of iteration k's G2P for one particle:
-    // v_k = sum_i w U_i, C_k = (4 / dx) sum_i w d U_i^T, so with B_bar =
-    // (4 / dx) C_bar it is sum_i grad w (v_bar . U_i + d . B_bar U_i) -
-    // w B_bar U_i.  U (the taped grid of the particle's scene) is read from
-    // global memory: every in-grid node of a particle lies in its pblock's
-    // box, and out-of-grid nodes are skipped, as in the forward's gather.
-    CR_DEV V3 pbadj_g2p_x(const GridGeom &g, const float4 *U, V3 x, V3 vb, const M3 &Bb)
-    {
-        int bx, by, bz;
-        V3 fx;
-        cell_of(g, x, bx, by, bz, fx);
-        NodeW nw;
-        node_weights(fx, nw);
-        V3 xacc{0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int a = 0; a < 3; ++a)
-#pragma unroll 1
-            for (int b = 0; b < 3; ++b)
-#pragma unroll 3
-                for (int c = 0; c < 3; ++c)
-                {
-                    const int nx = bx + a, ny = by + b, nz = bz + c;
-                    if (!in_grid(g, nx, ny, nz))
-                        continue;
-                    const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, c);
-                    const float da = dw_of(fx.x, a), db = dw_of(fx.y, b), dc = dw_of(fx.z, c);
-                    const float wt = wa * wb * wc;
-                    const V3 gw = V3{da * wb * wc, wa * db * wc, wa * wb * dc} * g.invdx;
-                    const V3 dist = (V3{(float)a, (float)b, (float)c} - fx) * g.dx;
-                    const V3 u = v3(U[grid_index(g, nx, ny, nz)]);
-                    const V3 Bu = mul(Bb, u);
-                    xacc = xacc + gw * (dot(vb, u) + dot(dist, Bu)) - Bu * wt;
-                }
-        return xacc;
     }
 
     // The weights' x adjoint of the last iteration's G2P (the earlier
@@ -3219,10 +3239,10 @@ struct CrHipBatch
     // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
     const unsigned *lastKeys = nullptr;
-    // PB: when set, every iteration's grid update also writes its nodes (the
-    // active node blocks, in the grid's layout) to pbTape + iteration * nodes
-    // (the adjoint's re-run of a substep)
-    float4 *pbTape = nullptr;
+    // When set, the grid update also writes its nodes (the active node
+    // blocks, in the grid's layout) to tape (MLS) or tape + iteration * nodes
+    // (PB): the adjoint's re-run of a substep
+    float4 *tape = nullptr;
     const unsigned *gatherOrder = nullptr; // this substep's tiles in particle index order
 
     // hipGraph replay of the substep
@@ -3539,7 +3559,9 @@ namespace
             // grids have many empty items, whose workgroups cost dispatch time
             // (one-wavefront groups: MLS cube substep 0.109 -> 0.117 ms)
             launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
-            launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, a, b->dirty);
+            BatchArgs au = a;
+            au.tapeGrid = b->tape;
+            launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, au, b->dirty);
             launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, gatherWGs, kWG, a);
         }
         else if (b->kind == CRMPM_HIP_STANDARD)
@@ -3565,7 +3587,7 @@ namespace
                 // also writes its nodes to the tape
                 auto taped = [&](int it) {
                     BatchArgs t = a;
-                    t.tapeGrid = b->pbTape ? b->pbTape + (size_t)it * nodes : nullptr;
+                    t.tapeGrid = b->tape ? b->tape + (size_t)it * nodes : nullptr;
                     return t;
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
@@ -3640,8 +3662,9 @@ namespace
         return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
 
-    // One PB substep, launched eagerly, with every iteration's updated grid
-    // copied to tape (K grids of the batch's node count).
+    // One substep, launched eagerly, with each grid update also writing its
+    // active node blocks to tape (MLS: one grid of the batch's node count;
+    // PB: K, one per iteration).
     int batch_step_taped(CrHipBatch *b, float4 *tape)
     {
         int rc = validate_for_step(b);
@@ -3649,9 +3672,9 @@ namespace
             rc = upload_scenes(b);
         if (rc)
             return rc;
-        b->pbTape = tape;
+        b->tape = tape;
         rc = enqueue_step(b);
-        b->pbTape = nullptr;
+        b->tape = nullptr;
         return rc;
     }
 
@@ -4946,11 +4969,6 @@ namespace
     };
 } // namespace
 
-// Particles per sub-tile of the reverse (adjoint) kernels.
-// (128 and 256 measured slower for both adjoints: the reverse kernels' per-particle
-// work needs the wavefronts more than the grid adjoint needs fewer partials)
-constexpr int kAdjSubSize = 64;
-
 struct CrHipAdjoint
 {
     CrHipSolver *solver = nullptr;
@@ -4962,6 +4980,7 @@ struct CrHipAdjoint
     unsigned **segOrder = nullptr;
     int **segStart = nullptr, **segEnd = nullptr;
     unsigned **segActive = nullptr; // active node block list and its count ([totalBlocks] then the count)
+    unsigned **segKeys = nullptr;   // sorted pblock keys
     float4 *gridAdj = nullptr;
     float4 *xTmp = nullptr;
     float4 *FTmp = nullptr;
@@ -5219,6 +5238,8 @@ namespace
                 dfree(a->segEnd[i]);
             if (a->segActive)
                 dfree(a->segActive[i]);
+            if (a->segKeys)
+                dfree(a->segKeys[i]);
         }
         delete[] a->ckpt;
         delete[] a->seg;
@@ -5227,6 +5248,7 @@ namespace
         delete[] a->segStart;
         delete[] a->segEnd;
         delete[] a->segActive;
+        delete[] a->segKeys;
         dfree(a->gridAdj);
         dfree(a->xTmp);
         dfree(a->FTmp);
@@ -5267,7 +5289,8 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
     a->segStart = new (std::nothrow) int *[a->every]();
     a->segEnd = new (std::nothrow) int *[a->every]();
     a->segActive = new (std::nothrow) unsigned *[a->every]();
-    int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd && a->segActive)
+    a->segKeys = new (std::nothrow) unsigned *[a->every]();
+    int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd && a->segActive && a->segKeys)
                  ? 0
                  : CRMPM_HIP_ERR_ALLOC;
     for (int i = 0; !rc && i < a->segments + 1; ++i)
@@ -5283,6 +5306,7 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->segStart[i], (size_t)b->totalBlocks);
         rc |= dalloc(&a->segEnd[i], (size_t)b->totalBlocks);
         rc |= dalloc(&a->segActive[i], (size_t)b->totalBlocks + 1);
+        rc |= dalloc(&a->segKeys[i], N);
     }
     if (!rc)
     {
@@ -5290,11 +5314,16 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
         rc |= dalloc(&a->xTmp, N);
         rc |= dalloc(&a->FTmp, 3 * N);
     }
+    // PB: the reverse state in tile order; MLS: paV and paC hold the G2P
+    // adjoint's scatter inputs
+    if (!rc)
+    {
+        rc |= dalloc(&a->paV, N);
+        rc |= dalloc(&a->paC, 3 * N);
+    }
     if (!rc && !mls)
     {
         rc |= dalloc(&a->paX, N);
-        rc |= dalloc(&a->paV, N);
-        rc |= dalloc(&a->paC, 3 * N);
         rc |= dalloc(&a->paF, 3 * N);
         rc |= dalloc(&a->paFi, 3 * N);
     }
@@ -5383,18 +5412,18 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
     {
         const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
         // recompute the segment from its checkpoint, keeping every substep's
-        // start state, updated grid and tiles
+        // start state, updated grid (its active blocks, written by the grid
+        // update) and tiles
         rc = state_copy(b, a->ckpt[sgm], false);
         for (int t = t0; t < t1 && !rc; ++t)
         {
             const int j = t - t0;
             rc = state_copy(b, a->seg[j], true);
             if (!rc)
-                rc = batch_step(b);
+                rc = batch_step_taped(b, a->segGrid[j]);
             if (!rc &&
-                (hipMemcpyAsync(a->segGrid[j], b->grid, (size_t)b->totalBlocks * 64 * sizeof(float4), hipMemcpyDeviceToDevice,
-                                st) != hipSuccess ||
-                 hipMemcpyAsync(a->segOrder[j], b->lastOrder, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                (hipMemcpyAsync(a->segOrder[j], b->lastOrder, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                 hipMemcpyAsync(a->segKeys[j], b->lastKeys, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
                  hipMemcpyAsync(a->segStart[j], b->segStart, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess ||
                  hipMemcpyAsync(a->segEnd[j], b->segEnd, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
@@ -5408,7 +5437,8 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
         if (!rc)
             rc = state_copy(b, a->seg[t1 - t0], true);
         // reverse sweep over the segment
-        for (int t = t1 - 1; t >= t0 && !rc; --t)
+        const int n = b->totalActive;
+        for (int t = t1 - 1; t >= t0 && !rc && n > 0; --t)
         {
             const int j = t - t0;
             BatchArgs args = batch_args(b, a->segOrder[j]);
@@ -5417,18 +5447,22 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             args.segEnd = a->segEnd[j];
             args.activeList = a->segActive[j];
             args.activeCount = reinterpret_cast<const int *>(a->segActive[j] + b->totalBlocks);
-            // The reverse kernels use their own split, as fine as the partial
-            // slots allow: their per-particle work is several times the
-            // forward's, and the forward's tuned split (often whole pblocks)
-            // would leave them one wavefront per dense pblock.
-            args.subSize = kAdjSubSize;
-            args.subMax = b->subStride;
-            const int subItems = std::max(1, (int)b->totalBlocks * args.subMax); // one-wavefront workgroups, one per sub-tile
+            // the velocity adjoint's scatter and the grid adjoint take whole
+            // pblocks (at most 8 partials per node); the per-particle kernels
+            // run 64 sorted particles per wavefront
+            BatchArgs whole = args;
+            whole.subSize = 1 << 30;
+            whole.subMax = 1;
+            const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
-            k_adj_g2p<<<subItems, 64, 0, st>>>(args, state_ref(a->seg[j], stride), state_ref(a->seg[j + 1], stride),
-                                               a->segGrid[j], ad);
-            k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(args, a->segGrid[j], a->gridAdj);
-            k_adj_p2g<<<subItems, 64, 0, st>>>(args, state_ref(a->seg[j], stride), a->gridAdj, ad);
+            const int denseWgs = grid1d(n, kWG);
+            const MlsScatterIn si{a->paV, a->paC};
+            const StateRef s0 = state_ref(a->seg[j], stride);
+            k_adj_g2p<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, state_ref(a->seg[j + 1], stride),
+                                                a->segGrid[j], ad, si);
+            k_adj_uscatter<<<wholeItems, 64, 0, st>>>(whole, s0, si);
+            k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(whole, a->segGrid[j], a->gridAdj);
+            k_adj_p2g<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, a->gridAdj, ad);
             rc = check_launch();
         }
     }
