@@ -4986,7 +4986,7 @@ struct CrHipAdjoint
     float4 *FTmp = nullptr;
     // PB: the tape (a segment's substeps, or the one substep being reversed;
     // see pb_reverse), its size and budget, and the adjoint state in tile order
-    float4 *tape = nullptr;
+    char *tape = nullptr; // the PB tape's bytes (see pb_tape_slot)
     size_t tapeBytes = 0;
     size_t tapeBudget = size_t(8) << 30;
     float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
@@ -5097,7 +5097,7 @@ namespace
         (void)hipStreamSynchronize(b->stream);
         dfree(a->tape);
         a->tapeBytes = 0;
-        if (dalloc(reinterpret_cast<char **>(&a->tape), need))
+        if (dalloc(&a->tape, need))
             return CRMPM_HIP_ERR_ALLOC;
         a->tapeBytes = need;
         return CRMPM_HIP_OK;
@@ -5108,7 +5108,7 @@ namespace
     {
         CrHipBatch *b = a->solver->b;
         const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
-        char *m = reinterpret_cast<char *>(a->tape) + pb_substep_tape_bytes(b, K) * (size_t)j;
+        char *m = a->tape + pb_substep_tape_bytes(b, K) * (size_t)j;
         PbSubstepTape t{};
         auto take = [&](size_t bytes) {
             char *r = m;
@@ -5200,9 +5200,9 @@ namespace
                 }
                 rc = state_copy(b, a->seg[j], false);
                 PbSubstepTape tp{};
-                tp.U = a->tape;
+                tp.U = reinterpret_cast<const float4 *>(a->tape);
                 if (!rc)
-                    rc = batch_step_taped(b, a->tape);
+                    rc = batch_step_taped(b, reinterpret_cast<float4 *>(a->tape));
                 if (rc)
                     break;
                 tp.order = b->lastOrder;
