@@ -111,8 +111,8 @@ def valu_issue_us(instructions):
 # picked, so no trial dispatch enters the statistics), the summary's title and
 # file name, and whether the counter passes belong to it or the kernel trace alone.
 PB_LAUNCH = "0,4096,1048576,1,1048576,1"
-MLS_CUTTING_LAUNCH = "0,16384,1048576,1,1048576,1"
-MLS_CUBE_LAUNCH = "0,0,128,8,128,8"
+MLS_CUTTING_LAUNCH = "16384,4096,1048576,1,1048576,0"
+MLS_CUBE_LAUNCH = "4096,0,128,8,1048576,0"
 PROFILE_TARGETS = {
     "pb_slab": dict(run=f"python3 bench.py --workload pb_slab --cpu-substeps 0 --launch {PB_LAUNCH}", counters=True,
                     title="Kernel trace and counters: PB-MPM slab, 262,144 particles, 20 iterations, 510 substeps",
@@ -788,19 +788,24 @@ def mls_run(scene, steps, warmup, tune=True):
     return elapsed, prof, touched, tuned, st
 
 
-def mls_kernel_roofline(n, touched, prof, slicer, target=None, persistent=False):
+def mls_kernel_roofline(n, touched, prof, slicer, target=None, tuned=None):
     """Achieved HBM rate of the MLS P2G and G2P kernels on their algorithmic
     bytes: P2G reads x, v, F, C, V0, (lambda, mu) = 108 B per particle and
     writes 16 B per touched node; G2P reads x, F (and x again per slicer test,
     12 B) and writes x, v, F, C = 96 B per particle and reads 16 B per node."""
     out = {}
+    pers = "true" if tuned and tuned[0] > 0 else "false"
+    # the kernels' names as the profiler lists them; gather split 0 is the
+    # per-particle G2P
+    rows = {"mls_p2g": f"k_mls_p2g<{pers}, 4>",
+            "mls_g2p": "k_g2p_dense<0>" if tuned and tuned[5] == 0 else f"k_mls_g2p<{pers}, 4>"}
     for name, per_p in (("mls_p2g", 108), ("mls_g2p", 48 + (12 if slicer else 0) + 96)):
         ms, k = prof[name]
         if not k:
             continue
         avg = ms / k / 1e3
         achieved = (per_p * n + 16 * touched) / avg / 1e9
-        row = f"k_{name}<{'true' if persistent else 'false'}>"
+        row = rows[name]
         out[name] = {"kernel": row, "avg_launch_us": avg * 1e6, "launches": k, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algorithmic_bytes_per_launch": per_p * n + 16 * touched,
@@ -817,7 +822,7 @@ def mls_cutting(steps=500, warmup=10, cpu=True):
                         "slicer cutting at 0.02 m/s", "particles": n, "substeps": steps,
             "value": n * steps / elapsed, "unit": "particle-substeps/s", "ms_per_substep": elapsed / steps * 1e3,
             "touched_nodes": touched, "roofline": mls_kernel_roofline(n, touched, prof, slicer=True, target="mls_cutting",
-                                            persistent=tuned[0] > 0),
+                                            tuned=tuned),
             "autotuned": dict(zip(TUNED_KEYS, tuned)),
             "cpu_baseline": cpu_baseline(sc, 2) if cpu else None,
             "finite": bool(np.isfinite(st["xm"]).all())}
@@ -917,7 +922,7 @@ def mls_cube(steps=200, warmup=5, cpu=True):
     return {"workload": "MLS elastic cube 32^3 = 32,768 particles on 65^3 nodes, ground plane", "particles": n,
             "substeps": steps, "value": n * steps / elapsed, "unit": "particle-substeps/s",
             "ms_per_substep": elapsed / steps * 1e3, "roofline": mls_kernel_roofline(n, touched, prof, slicer=False, target="mls_cube",
-                                            persistent=tuned[0] > 0),
+                                            tuned=tuned),
             "autotuned": dict(zip(TUNED_KEYS, tuned)),
             "cpu_baseline": cpu_baseline(sc, 50) if cpu else None,
             "finite": bool(np.isfinite(st["xm"]).all())}
@@ -1066,7 +1071,8 @@ def main():
     bytes_iter = BYTES_PER_PARTICLE_ITERATION * n + BYTES_PER_TOUCHED_NODE_ITERATION * touched
     achieved = bytes_iter / avg_iter_s / 1e9
     dense_row = "k_pb_dense<1>"
-    scat_row = f"k_pb_scatter<false, {'true' if tuned[0] > 0 else 'false'}>"
+    # the names as the profiler lists them (the scatter runs one-wavefront workgroups)
+    scat_row = f"k_pb_scatter<false, {'true' if tuned[0] > 0 else 'false'}, 1>"
     rows = (profiled_row("pb_slab", dense_row), profiled_row("pb_slab", scat_row))
     per_kernel = {}
     for name, row, avg_s, (bp, bn), launches in ((dense_row, rows[0], avg_dense_s, BYTES_DENSE, dense_n),

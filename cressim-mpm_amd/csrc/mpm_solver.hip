@@ -3,7 +3,7 @@
 // Data layout in HBM
 //   engine-owned (bound):  positionMass float4[N], velocity float4[N], params0 float4[N],
 //                          type int[N], mask u8[N], shape/geometry/SDF tables
-//   solver-owned:          F, C  three float4 columns per particle (F[3p + c], w unused),
+//   solver-owned:          F, C  9 floats per particle, column-major (F[9p + 3c + r]),
 //                          V0 float[N]
 //   grid:                  4x4x4-node blocks, 1 KiB each (float4 (momentum|velocity, mass) per
 //                          node), block b = bx + nbx * (by + nby * bz), node (lx,ly,lz) at
@@ -1802,6 +1802,56 @@ namespace
         cOut = aacc * (4.0f * g.invdx);
     }
 
+    // Standard MPM's gather (v and the velocity gradient) read straight from
+    // the scene's grid in global memory: the same nodes, terms and order as
+    // k_std_g2p's LDS gather.
+    CR_DEV void gather_vgrad_grid(const StepCommon &c, V3 x, V3 &vOut, M3 &gvOut)
+    {
+        const GridGeom &g = c.g;
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        Weights W;
+        weights_of(fx, W);
+        V3 vacc{0, 0, 0};
+        M3 gv{V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}};
+        auto gather = [&](V3 nv, int a, int b, int cc, float wt) {
+            vacc = vacc + nv * wt;
+            const float wa = w_of(fx.x, a), wb = w_of(fx.y, b), wc = w_of(fx.z, cc);
+            const V3 wg{dw_of(fx.x, a) * wb * wc, wa * dw_of(fx.y, b) * wc, wa * wb * dw_of(fx.z, cc)};
+            gv = gv + outer_ref(wg, nv);
+        };
+        if (nodes_inside(g, bx, by, bz))
+        {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+                        gather(v3(c.grid[grid_index(g, bx + a, by + b, bz + cc)]), a, b, cc,
+                               W.w[a][0] * W.w[b][1] * W.w[cc][2]);
+        }
+        else
+        {
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a)
+#pragma unroll 1
+                for (int b = 0; b < 3; ++b)
+#pragma unroll 1
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const int nx = bx + a, ny = by + b, nz = bz + cc;
+                        if (!in_grid(g, nx, ny, nz))
+                            continue;
+                        gather(v3(c.grid[grid_index(g, nx, ny, nz)]), a, b, cc,
+                               w_of(fx.x, a) * w_of(fx.y, b) * w_of(fx.z, cc));
+                    }
+        }
+        vOut = vacc;
+        gvOut = gv;
+    }
+
     CR_DEV V3 clamp_bounds(const GridGeom &g, V3 x)
     {
         return V3{fminf(fmaxf(x.x, g.bmin.x), g.bmax.x), fminf(fmaxf(x.y, g.bmin.y), g.bmax.y),
@@ -1865,6 +1915,64 @@ namespace
                 store_chunk_mats(stage[wave], lane, valid ? p : -1, Cm, Fn, c.S.C, c.S.F);
             }
         }
+    }
+
+    // G2P per particle in particle index order, for a batch of one scene
+    // (the gather split "per particle"): 64 consecutive particles per
+    // wavefront, the stencil read from the grid in global memory
+    // (gather_vc_grid: the same nodes, sums and order as the tile kernels'
+    // LDS gathers), so every particle array is read and written in whole
+    // cache lines instead of the tile's rows of 8 particles.  The same
+    // particles take part as in the tile kernels (the binning's test:
+    // MLS: in the active mask; standard: P2G takes every particle, G2P the
+    // masked-in ones), and the results are bitwise those of the tile kernels.
+    // i counts the scene's particles; n is 0 for a resting scene.
+    template <int Kind> // 0 MLS, 2 standard
+    __global__ __launch_bounds__(kWG) void k_g2p_dense(BatchArgs b, int n)
+    {
+        __shared__ ChunkStage stage[kWaves];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int i = blockIdx.x * kWG + threadIdx.x;
+        if (__builtin_amdgcn_readfirstlane(i - lane) >= n)
+            return; // the whole wavefront is past the end
+        const StepCommon c = scene_view_of(b, 0);
+        const GridGeom &g = c.g;
+        const int p = ((ConstSceneRec *)(size_t)b.scenes)[0].particleOffset + i;
+        const bool valid = i < n && c.P.mask[p];
+        const int nSlicers = slicer_count(c.sv);
+        const int nShapes = shape_count(c.sv);
+        M3 Cm = m3_identity(), Fn = m3_identity();
+        if (valid)
+        {
+            const float4 xm = c.P.xm[p];
+            // the incoming velocity only enters a slicer's reflection
+            const float4 vp = (Kind == 0 && nSlicers > 0) ? c.P.v[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const M3 F = load_pm(c.S.F, p);
+            V3 x = v3(xm);
+            V3 vel;
+            if (Kind == 0)
+            {
+                // the particle velocity after P2G's gravity update (P2G does not store it)
+                const V3 vpg = v3(vp) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+                gather_vc_grid(c, nSlicers, x, vpg, vel, Cm);
+                Fn = mul(m3_identity() + Cm * c.dt, F);
+            }
+            else
+            {
+                M3 gv;
+                gather_vgrad_grid(c, x, vel, gv);
+                Fn = mul(m3_identity() + gv * c.dt, F);
+            }
+            x = clamp_bounds(g, x + vel * c.dt);
+            float4 vv = make_float4(vel.x, vel.y, vel.z, 0.0f);
+            resolve_rigid<true>(c.sv, nShapes, vv, x);
+            c.P.xm[p] = make_float4(x.x, x.y, x.z, xm.w);
+            c.P.v[p] = make_float4(vel.x, vel.y, vel.z, 0.0f);
+        }
+        if (Kind == 0)
+            store_chunk_mats(stage[wave], lane, valid ? p : -1, Cm, Fn, c.S.C, c.S.F);
+        else if (valid)
+            store_pm(c.S.F, p, Fn);
     }
 
     // ------------------------------------------------------------------
@@ -3544,6 +3652,8 @@ namespace
         const BatchArgs a = batch_args(b, order);
         const int tileWGs = pblock_wgs(b->totalBlocks * b->subMax, b->tileWaves);
         const int gatherWGs = pblock_wgs(b->totalBlocks * b->gSubMax, b->tileWaves);
+        // gather split 0: G2P per particle in particle index order (one scene)
+        const bool gDense = b->gSubMax == 0;
         const int updWGs = pblock_wgs(b->totalBlocks, b->updateWaves);
         const bool pers = b->tileWaves > 0;
         const bool multi = b->subMax > 1, persU = b->updateWaves > 0;
@@ -3562,13 +3672,19 @@ namespace
             BatchArgs au = a;
             au.tapeGrid = b->tape;
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, au, b->dirty);
-            launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, gatherWGs, kWG, a);
+            if (gDense)
+                launch(b, CRMPM_HIP_PROF_MLS_G2P, k_g2p_dense<0>, grid1d(n, kWG), kWG, a, n);
+            else
+                launch(b, CRMPM_HIP_PROF_MLS_G2P, pers ? k_mls_g2p<true> : k_mls_g2p<false>, gatherWGs, kWG, a);
         }
         else if (b->kind == CRMPM_HIP_STANDARD)
         {
             launch(b, CRMPM_HIP_PROF_STD_P2G, pers ? k_std_p2g<true> : k_std_p2g<false>, tileWGs, kWG, a);
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<2>(multi, persU), updWGs, kWG, a, b->dirty);
-            launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, gatherWGs, kWG, a);
+            if (gDense)
+                launch(b, CRMPM_HIP_PROF_STD_G2P, k_g2p_dense<2>, grid1d(n, kWG), kWG, a, n);
+            else
+                launch(b, CRMPM_HIP_PROF_STD_G2P, pers ? k_std_g2p<true> : k_std_g2p<false>, gatherWGs, kWG, a);
         }
         else
         {
@@ -4083,9 +4199,10 @@ namespace
                 bestM = sp[1];
             }
         }
-        // 2. gather (G2P) split: whole pblocks, or sub-tiles of 256 or 128
-        //    particles (no partials to sum, only more wavefronts)
-        const int gsplits[3][2] = {{1 << 20, 1}, {256, 4}, {128, 8}};
+        // 2. gather (G2P) split: whole pblocks, sub-tiles of 256 or 128
+        //    particles (no partials to sum, only more wavefronts), or, for a
+        //    batch of one scene, per particle in particle index order
+        const int gsplits[4][2] = {{1 << 20, 1}, {256, 4}, {128, 8}, {1 << 20, 0}};
         int bestGS = 1 << 20, bestGM = 1;
         best = 3.4e38f;
         for (const auto &gs : gsplits)
@@ -4093,6 +4210,8 @@ namespace
             float ms = 0.0f;
             if (rc || b->kind == CRMPM_HIP_PB) // PB has no separate gather kernel
                 break;
+            if (gs[1] == 0 && b->numScenes != 1)
+                continue;
             rc = tune_trial(b, backup, steps, 0, 0, bestS, bestM, gs[0], gs[1], &ms);
             if (!rc && ms < best)
             {
@@ -4650,7 +4769,9 @@ int crHipBatchSetTileSplit(CrHipBatch *b, int subSize, int subMax)
 
 int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax)
 {
-    if (!b || subSize < 64 || subSize % 64 != 0 || subMax < 1 || subMax > 64)
+    if (!b || subSize < 64 || subSize % 64 != 0 || subMax < 0 || subMax > 64)
+        return CRMPM_HIP_ERR_ARG;
+    if (subMax == 0 && b->numScenes != 1) // per particle: batches of one scene
         return CRMPM_HIP_ERR_ARG;
     b->gSubSize = subSize;
     b->gSubMax = subMax;

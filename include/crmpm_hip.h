@@ -265,8 +265,11 @@ int crHipBatchAutotune(CrHipBatch *b, int stepsPerTrial, int *tileWaves, int *up
 int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
 /* Split of the gather kernels (G2P), which write no partial boxes: a
  * pblock's particles in sub-tiles of subSize (a multiple of 64), at most
- * subMax (1..64) of them, the last taking the rest.  Default whole pblocks;
- * crHipBatchAutotune also picks it.  No counterpart in the reference. */
+ * subMax (1..64) of them, the last taking the rest.  subMax 0, for a batch of
+ * one scene only (else CRMPM_HIP_ERR_ARG): G2P per particle in particle index
+ * order, 64 consecutive particles per wavefront, with the same results.
+ * Default whole pblocks; crHipBatchAutotune also picks it.  No counterpart in
+ * the reference. */
 int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax);
 int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax);
 int crHipBatchFetchResults(CrHipBatch *b);

@@ -154,6 +154,40 @@ def test_gather_split_is_bitwise_neutral(kind):
                 assert np.array_equal(r[i][k], runs[0][i][k]), (i, k)
 
 
+@pytest.mark.parametrize("kind", ["mls", "std"])
+@pytest.mark.parametrize("shapes", ["plane+sphere+box", "plane+slicer", "plane+capsule+meshslicer"])
+def test_per_particle_gather_is_bitwise_neutral(kind, shapes):
+    """G2P per particle in particle index order (gather split 0, one scene)
+    reads the same nodes in the same order as the tile kernels and takes the
+    same particles (masked ones left alone): results bitwise equal, also
+    replayed as a graph.  A batch of several scenes refuses that split."""
+    import crmpm_hip as H
+
+    sc = S.small_scene(KINDS[kind], 7, n_side=10, dx=1 / 32, shapes=shapes)
+    sc["mask"][::7] = 0
+    print(f"per-particle gather {kind} {shapes}: {len(sc['xm'])} particles", flush=True)
+    runs = []
+    for cfg, graph in (((0, 0, 1 << 20, 1, 1 << 20, 1), False), ((0, 0, 1 << 20, 1, 1 << 20, 0), False),
+                       ((2048, 0, 1 << 20, 1, 1 << 20, 0), True)):
+        g = H.HipSolver(copy.deepcopy(sc))
+        g.set_launch(cfg)
+        assert g.gather_split() == cfg[4:]
+        g.set_graph_mode(graph)
+        g.compute_initial_data()
+        g.step(12)
+        runs.append(g.state())
+        g.close()
+    moved = np.abs(runs[0]["xm"] - sc["xm"]).max()
+    assert moved > 0.0
+    for r in runs[1:]:
+        for k in FIELDS:
+            assert np.array_equal(r[k], runs[0][k]), k
+    b = H.HipBatch(mixed_scenes(KINDS[kind])[:2])
+    with pytest.raises(RuntimeError):
+        b.set_gather_split(1 << 20, 0)
+    b.close()
+
+
 def test_single_solver_graph_with_kinematic_frames():
     """The single-scene solver replaying its substep as a hipGraph, with the
     indenter's target uploaded between frames and every third substep run
