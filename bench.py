@@ -47,7 +47,7 @@ BYTES_SCATTER = (12 + 4 + 12 + 36, 12)
 PROFILES = os.path.join(ROOT, "profiles")
 # what the library's autotuner picked: launch sizes, the P2G sub-tile split
 # and the G2P (gather) split
-TUNED_KEYS = ("tile_waves", "update_waves", "sub_size", "sub_max", "gather_sub_size", "gather_sub_max")
+TUNED_KEYS = ("tile_waves", "update_waves", "sub_size", "sub_max", "gather_sub_size", "gather_sub_max", "scatter_waves")
 PROFILE_EVERY = 10
 COPY_CAL_BYTES = 256 << 20
 LAUNCH = None  # --launch: a fixed launch configuration in place of the autotuner's
@@ -418,7 +418,7 @@ def batched(kind_name, steps, warmup, tune=True, graph=True, n_streams=4):
     b.step(warmup)
     tuned = None
     if tune:
-        tuned = b.autotune(steps_per_trial=2) + b.gather_split()
+        tuned = b.autotune(steps_per_trial=2) + b.gather_split() + (b.scatter_waves(),)
     b.set_graph_mode(graph)
     b.step(1)  # capture
     torch.cuda.synchronize()
@@ -769,7 +769,7 @@ def mls_run(scene, steps, warmup, tune=True):
     if LAUNCH:
         tuned = solver.set_launch(LAUNCH)
     else:
-        tuned = solver.autotune(steps_per_trial=2) + solver.gather_split() if tune else None
+        tuned = solver.autotune(steps_per_trial=2) + solver.gather_split() + (solver.scatter_waves(),) if tune else None
     # substeps replay a captured hipGraph (the sampled ones, which record events, run eagerly)
     solver.set_graph_mode(True)
     torch.cuda.synchronize()
@@ -951,9 +951,10 @@ def main():
     ap.add_argument("--workload", choices=["all", "pb_slab", "batched", "adjoint", "pb_adjoint", "mls", "mls_cutting", "mls_cube",
                                            "lattice", "meshsdf", "engine"], default="all",
                     help="all: the PB slab headline line plus the batched, adjoint and MLS workloads")
-    ap.add_argument("--launch", metavar="T,U,S,M,GS,GM",
+    ap.add_argument("--launch", metavar="T,U,S,M,GS,GM[,W]",
                     help="use this launch configuration (tile waves, update waves, sub-tile size, sub-tiles, gather "
-                         "sub-tile size, gather sub-tiles) instead of the autotuner's (PB slab and MLS workloads): "
+                         "sub-tile size, gather sub-tiles, scatter wavefronts per tile (default 1)) instead of the "
+                         "autotuner's (PB slab and MLS workloads): "
                          "profiled runs pass the one the timed run picked, so no autotuner trial enters their stats")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel that calibrates rocprofv3's byte counters")
@@ -967,7 +968,7 @@ def main():
     global LAUNCH
     if args.launch:
         LAUNCH = tuple(int(x) for x in args.launch.split(","))
-        assert len(LAUNCH) == 6, "--launch takes 6 integers"
+        assert len(LAUNCH) in (6, 7), "--launch takes 6 or 7 integers"
     if args.summarize:
         tgt = PROFILE_TARGETS[args.profile_target]
         md, traffic = summarize(args.summarize, tgt["title"], profile_commands(args.profile_target), tgt["counters"])
@@ -1040,7 +1041,8 @@ def main():
 
     run(0, W)
     # the library's autotuner picks the tile split and launch sizes on this state (which it leaves unchanged)
-    tuned = solver.set_launch(LAUNCH) if LAUNCH else solver.autotune(steps_per_trial=2) + solver.gather_split()
+    tuned = (solver.set_launch(LAUNCH) if LAUNCH else
+             solver.autotune(steps_per_trial=2) + solver.gather_split() + (solver.scatter_waves(),))
     torch.cuda.synchronize()
     # touched nodes of one substep (grid nodes holding mass), for the grid term of the roofline
     grid = solver.grid()

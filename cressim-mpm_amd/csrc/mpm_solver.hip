@@ -1022,11 +1022,12 @@ namespace
         for (int i = lane; i < Tiles * kTileNodes; i += 64)
             A[i] = F3{0.f, 0.f, 0.f};
     }
-    // out = sum of the Tiles accumulation tiles, in tile order.
-    template <int Tiles>
-    CR_DEV void store_tile_sum(const float4 *A, float4 *out, int lane)
+    // out = sum of the Tiles accumulation tiles, in tile order (Threads
+    // threads share the nodes; tid is the thread's index among them).
+    template <int Tiles, int Threads = 64>
+    CR_DEV void store_tile_sum(const float4 *A, float4 *out, int tid)
     {
-        for (int i = lane; i < kTileNodes; i += 64)
+        for (int i = tid; i < kTileNodes; i += Threads)
         {
             float4 v = A[i];
 #pragma unroll
@@ -1043,10 +1044,10 @@ namespace
     }
     // Momentum tiles: out's xyz = the sum in tile order; out's w (the node
     // mass the substep's first iteration stored there) is kept.
-    template <int Tiles>
-    CR_DEV void store_tile_sum(const F3 *A, float4 *out, int lane)
+    template <int Tiles, int Threads = 64>
+    CR_DEV void store_tile_sum(const F3 *A, float4 *out, int tid)
     {
-        for (int i = lane; i < kTileNodes; i += 64)
+        for (int i = tid; i < kTileNodes; i += Threads)
         {
             F3 v = A[i];
 #pragma unroll
@@ -1059,6 +1060,26 @@ namespace
             }
             *reinterpret_cast<F3 *>(out + i) = v;
         }
+    }
+
+    // The scatter of one (sub-)tile by the W wavefronts of a workgroup, for
+    // dense pblocks (which otherwise run all their chunks on one wavefront):
+    // wavefront w takes chunks w, w + W, ... into its own KAcc accumulation
+    // tiles, and the workgroup writes the sum over wavefronts, then tiles, in
+    // that fixed order to the tile's partial box.  A tile still writes one
+    // partial box, so the grid update's work does not grow with W.
+    // scatterOne(A, k) scatters sorted particle k into the tiles at A.
+    template <int W, int KAcc, typename Acc, typename Fn>
+    CR_DEV void wg_tile_scatter(Acc *acc, const TileInfo &ti, float4 *out, Fn &&scatterOne)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        Acc *A = acc + wave * KAcc * kTileNodes;
+        zero_tiles<KAcc>(A, lane);
+        wave_sync();
+        for (int k = ti.start + 64 * wave + lane; k < ti.end; k += 64 * W)
+            scatterOne(A, k);
+        __syncthreads();
+        store_tile_sum<W * KAcc, 64 * W>(acc, out, (int)threadIdx.x);
     }
 
     CR_DEV void lds_add(float4 *A, int li, float x, float y, float z, float w)
@@ -2200,6 +2221,32 @@ namespace
         }
     }
 
+    // k_pb_scatter with W wavefronts per (sub-)tile (wg_tile_scatter), one
+    // workgroup per tile slot.
+    template <bool Mass, int W>
+    __global__ __launch_bounds__(64 * W) void k_pb_scatter_wg(BatchArgs b, PbStage st)
+    {
+        constexpr int kAcc = 3;
+        using Acc = typename std::conditional<Mass, float4, F3>::type;
+        __shared__ Acc acc[W * kAcc][kTileNodes];
+        const int tg = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+        if (tg >= num_subtiles(b) || !subtile_used(b, tg))
+            return;
+        int t;
+        const StepCommon c = scene_view(b, tg / b.subMax, t);
+        TileInfo ti;
+        if (!subtile_info(c, t, tg % b.subMax, ti))
+            return;
+        const int nSlicers = slicer_count(c.sv);
+        const int S = st.stride;
+        wg_tile_scatter<W, kAcc>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](Acc *A, int k) {
+            const float4 xm = st.xs[k];
+            const V3 vel{st.vs[k], st.vs[S + k], st.vs[2 * S + k]};
+            const M3 Cm = load_m3(st.Cs, S, k);
+            pb_scatter<kAcc>(c, ti, A, nSlicers, v3(xm), xm.w, vel, Cm);
+        });
+    }
+
     // Zero iterations: integrate with the current v and C.
     __global__ void k_pb_integrate_only(BatchArgs b, int n, const unsigned *keys, PbStage st)
     {
@@ -2557,7 +2604,7 @@ namespace
     {
 #pragma unroll 1
         for (int a = 0; a < 3; ++a)
-#pragma unroll 1
+#pragma unroll
             for (int b = 0; b < 3; ++b)
 #pragma unroll 3
                 for (int c = 0; c < 3; ++c)
@@ -2590,7 +2637,7 @@ namespace
         V3 xacc{0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int a = 0; a < 3; ++a)
-#pragma unroll 1
+#pragma unroll
             for (int b = 0; b < 3; ++b)
 #pragma unroll 3
                 for (int c = 0; c < 3; ++c)
@@ -2662,12 +2709,13 @@ namespace
     // The grid velocity adjoint of the G2P, w (v_bar + B_bar^T d) per node,
     // through the forward's separable scatter (pb_scatter with m = 1) into the
     // tile partials; one pblock per one-wavefront workgroup.
-    __global__ __launch_bounds__(64) void k_adj_uscatter(BatchArgs b, StateRef s0, MlsScatterIn si)
+    // W wavefronts per tile as in the forward's scatter (wg_tile_scatter).
+    template <int W>
+    __global__ __launch_bounds__(64 * W) void k_adj_uscatter(BatchArgs b, StateRef s0, MlsScatterIn si)
     {
         constexpr int kAcc = 3;
-        __shared__ F3 acc[kAcc][kTileNodes];
-        const int lane = threadIdx.x & 63;
-        const int tg = first_tile_of<1>(0);
+        __shared__ F3 acc[W * kAcc][kTileNodes];
+        const int tg = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
         if (tg >= num_subtiles(b) || !subtile_used(b, tg))
             return;
         int t;
@@ -2675,13 +2723,9 @@ namespace
         TileInfo ti;
         if (!subtile_info(c, t, tg % b.subMax, ti))
             return;
-        F3 *A = acc[0];
-        zero_tiles<kAcc>(A, lane);
-        wave_sync();
-        for (int q = ti.start + lane; q < ti.end; q += 64)
+        wg_tile_scatter<W, kAcc>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](F3 *A, int q) {
             pb_scatter<kAcc>(c, ti, A, 0, v3(s0.x[c.order[q]]), 1.0f, v3(si.v[q]), transpose(load_cols(si.B, q)));
-        wave_sync();
-        store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
+        });
     }
 
     // Grid adjoint: sum the tile partials of the velocity adjoint (fixed
@@ -3087,12 +3131,12 @@ namespace
     // in conflict-free passes) into the tile partials; one pblock per
     // one-wavefront workgroup, so the grid adjoint sums at most 8 partials
     // per node.
-    __global__ __launch_bounds__(64) void k_pbadj_uscatter(BatchArgs b, PbStage st, PbAdjRef pa)
+    template <int W>
+    __global__ __launch_bounds__(64 * W) void k_pbadj_uscatter(BatchArgs b, PbStage st, PbAdjRef pa)
     {
         constexpr int kAcc = 3;
-        __shared__ F3 acc[kAcc][kTileNodes];
-        const int lane = threadIdx.x & 63;
-        const int tg = first_tile_of<1>(0);
+        __shared__ F3 acc[W * kAcc][kTileNodes];
+        const int tg = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
         if (tg >= num_subtiles(b) || !subtile_used(b, tg))
             return;
         int t;
@@ -3100,16 +3144,10 @@ namespace
         TileInfo ti;
         if (!subtile_info(c, t, tg % b.subMax, ti))
             return;
-        F3 *A = acc[0];
-        zero_tiles<kAcc>(A, lane);
-        wave_sync();
-        for (int q = ti.start + lane; q < ti.end; q += 64)
-        {
+        wg_tile_scatter<W, kAcc>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](F3 *A, int q) {
             const M3 Bb = load_cols(pa.C, q) * (4.0f * c.g.invdx);
             pb_scatter<kAcc>(c, ti, A, 0, v3(st.xs[q]), 1.0f, v3(pa.v[q]), transpose(Bb));
-        }
-        wave_sync();
-        store_tile_sum<kAcc>(A, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
+        });
     }
 
     // The per-particle part of reverse iteration k, one thread per sorted
@@ -3177,7 +3215,7 @@ namespace
             node_weights(fx, nw);
 #pragma unroll 1
             for (int a = 0; a < 3; ++a)
-#pragma unroll 1
+#pragma unroll
                 for (int bb = 0; bb < 3; ++bb)
 #pragma unroll 3
                     for (int cc = 0; cc < 3; ++cc)
@@ -3344,6 +3382,9 @@ struct CrHipBatch
     // the allocated partial boxes per pblock)
     int subSize = 64, subMax = 1, subStride = 1;
     int gSubSize = 1 << 20, gSubMax = 1; // gather kernels' split
+    // wavefronts per tile of the PB scatter and the adjoints' velocity
+    // scatters (1, 2 or 4; wg_tile_scatter)
+    int scatterWaves = 1;
     // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
     const unsigned *lastKeys = nullptr;
@@ -3357,7 +3398,7 @@ struct CrHipBatch
     bool graphMode = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    long long graphKey[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+    long long graphKey[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     long long bindEpoch = 0;
 
     // optional per-launch event timing (ProfileBegin/End)
@@ -3706,15 +3747,36 @@ namespace
                     t.tapeGrid = b->tape ? b->tape + (size_t)it * nodes : nullptr;
                     return t;
                 };
+                // the scatter: one wavefront per tile (persistent or not), or
+                // scatterWaves wavefronts per tile, one workgroup per slot
+                const dim3 wgGrid((unsigned)std::max(1ll, tileItems));
+                auto scatter = [&](bool mass, int prof) {
+                    switch (b->scatterWaves)
+                    {
+                    case 2:
+                        launch(b, prof, mass ? k_pb_scatter_wg<true, 2> : k_pb_scatter_wg<false, 2>, wgGrid, dim3(128), a,
+                               b->st);
+                        break;
+                    case 4:
+                        launch(b, prof, mass ? k_pb_scatter_wg<true, 4> : k_pb_scatter_wg<false, 4>, wgGrid, dim3(256), a,
+                               b->st);
+                        break;
+                    default:
+                        if (mass)
+                            launch(b, prof, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
+                                   tileBlock, a, b->st);
+                        else
+                            launch(b, prof, pers ? k_pb_scatter<false, true, 1> : k_pb_scatter<false, false, 1>, tileGrid,
+                                   tileBlock, a, b->st);
+                    }
+                };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
-                launch(b, CRMPM_HIP_PROF_PB_P2G, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
-                       tileBlock, a, b->st);
+                scatter(true, CRMPM_HIP_PROF_PB_P2G);
                 launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, taped(0), b->dirty);
                 for (int it = 1; it < K; ++it)
                 {
                     launch(b, CRMPM_HIP_PROF_PB_FUSED, k_pb_dense<1>, denseWGs, kWG, a, b->st, n, keys);
-                    launch(b, CRMPM_HIP_PROF_PB_SCATTER, pers ? k_pb_scatter<false, true, 1> : k_pb_scatter<false, false, 1>,
-                           tileGrid, tileBlock, a, b->st);
+                    scatter(false, CRMPM_HIP_PROF_PB_SCATTER);
                     launch(b, CRMPM_HIP_PROF_UPDATE, upd1, updWGs, kWG, taped(it), b->dirty);
                 }
 
@@ -3751,9 +3813,9 @@ namespace
         const bool useGraph = b->graphMode && (!b->profEvents || b->profMask == 0u) && b->stream != nullptr;
         if (!useGraph)
             return enqueue_step(b);
-        const long long key[9] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
-                                  b->subSize,       b->subMax,     b->gSubSize,  b->gSubMax};
-        if (!b->graphExec || !std::equal(key, key + 9, b->graphKey))
+        const long long key[10] = {b->totalActive, b->iterations, b->tileWaves, b->updateWaves, b->bindEpoch,
+                                   b->subSize,       b->subMax,     b->gSubSize,  b->gSubMax,    b->scatterWaves};
+        if (!b->graphExec || !std::equal(key, key + 10, b->graphKey))
         {
             drop_graph(b);
             if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -3773,7 +3835,7 @@ namespace
                 drop_graph(b);
                 return CRMPM_HIP_ERR_LAUNCH;
             }
-            std::copy(key, key + 9, b->graphKey);
+            std::copy(key, key + 10, b->graphKey);
         }
         return hipGraphLaunch(b->graphExec, b->stream) == hipSuccess ? CRMPM_HIP_OK : CRMPM_HIP_ERR_LAUNCH;
     }
@@ -4178,7 +4240,9 @@ namespace
         if (dalloc(&backup, bytes))
             return CRMPM_HIP_ERR_ALLOC;
         // the launch configuration in force, restored if a trial fails
-        const int prev[6] = {b->tileWaves, b->updateWaves, b->subSize, b->subMax, b->gSubSize, b->gSubMax};
+        const int prev[7] = {b->tileWaves, b->updateWaves, b->subSize,     b->subMax,
+                             b->gSubSize,  b->gSubMax,    b->scatterWaves};
+        b->scatterWaves = 1; // steps 1-4 time one wavefront per scatter tile
         rc = tune_copy(b, backup, true);
         const bool saved = rc == CRMPM_HIP_OK;
         // 1. sub-tile split: 64, 128 or 256 particles per sub-tile, or whole
@@ -4250,6 +4314,21 @@ namespace
                 bestU = c;
             }
         }
+        // 5. PB: 2 or 4 wavefronts per scatter tile (dense pblocks)
+        int bestW = 1;
+        for (int w : {2, 4})
+        {
+            float ms = 0.0f;
+            if (rc || b->kind != CRMPM_HIP_PB)
+                break;
+            b->scatterWaves = w;
+            rc = tune_trial(b, backup, steps, bestT, bestU, bestS, bestM, bestGS, bestGM, &ms);
+            if (!rc && ms < best)
+            {
+                best = ms;
+                bestW = w;
+            }
+        }
         // the state goes back to where it was whether or not every trial ran
         if (saved)
         {
@@ -4266,6 +4345,7 @@ namespace
             b->subMax = prev[3];
             b->gSubSize = prev[4];
             b->gSubMax = prev[5];
+            b->scatterWaves = prev[6];
             return rc;
         }
         b->tileWaves = bestT;
@@ -4274,6 +4354,7 @@ namespace
         b->subMax = bestM;
         b->gSubSize = bestGS;
         b->gSubMax = bestGM;
+        b->scatterWaves = bestW;
         if (tileWavesOut)
             *tileWavesOut = bestT;
         if (updateWavesOut)
@@ -4778,6 +4859,22 @@ int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax)
     return CRMPM_HIP_OK;
 }
 
+int crHipBatchSetScatterWaves(CrHipBatch *b, int waves)
+{
+    if (!b || (waves != 1 && waves != 2 && waves != 4))
+        return CRMPM_HIP_ERR_ARG;
+    b->scatterWaves = waves;
+    return CRMPM_HIP_OK;
+}
+
+int crHipBatchGetScatterWaves(const CrHipBatch *b, int *waves)
+{
+    if (!b || !waves)
+        return CRMPM_HIP_ERR_ARG;
+    *waves = b->scatterWaves;
+    return CRMPM_HIP_OK;
+}
+
 int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax)
 {
     if (!b || !subSize || !subMax)
@@ -5017,6 +5114,16 @@ int crHipSolverSetGraphMode(CrHipSolver *s, int enabled)
     return s ? crHipBatchSetGraphMode(s->b, enabled) : CRMPM_HIP_ERR_ARG;
 }
 
+int crHipSolverSetScatterWaves(CrHipSolver *s, int waves)
+{
+    return s ? crHipBatchSetScatterWaves(s->b, waves) : CRMPM_HIP_ERR_ARG;
+}
+
+int crHipSolverGetScatterWaves(const CrHipSolver *s, int *waves)
+{
+    return s ? crHipBatchGetScatterWaves(s->b, waves) : CRMPM_HIP_ERR_ARG;
+}
+
 int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax)
 {
     return s ? crHipBatchGetGatherSplit(s->b, subSize, subMax) : CRMPM_HIP_ERR_ARG;
@@ -5111,6 +5218,14 @@ struct CrHipAdjoint
     size_t tapeBytes = 0;
     size_t tapeBudget = size_t(8) << 30;
     float4 *paX = nullptr, *paV = nullptr, *paC = nullptr, *paF = nullptr, *paFi = nullptr;
+    // PB, two segment tapes (pb_reverse): the second set of segment states,
+    // the reverse's own partial boxes and stream, and the events that order
+    // the re-runs (batch stream) against the reverses (reverse stream)
+    StateBuf *seg2 = nullptr;
+    float4 *revTiles = nullptr;
+    size_t revTilesCount = 0;
+    hipStream_t revStream = nullptr;
+    hipEvent_t ran[2] = {nullptr, nullptr}, rev[2] = {nullptr, nullptr}, start = nullptr;
 };
 
 namespace
@@ -5159,10 +5274,12 @@ namespace
 
     // The reverse kernels of one PB substep; s0 and s1 are its start and
     // end states.
-    int pb_reverse_substep(CrHipAdjoint *a, const AdjRef &ad, const PbSubstepTape &tp, StateRef s0, StateRef s1)
+    // st: the stream of the reverse kernels; tiles: the partial boxes of the
+    // velocity-adjoint scatter (null: the batch's own)
+    int pb_reverse_substep(CrHipAdjoint *a, const AdjRef &ad, const PbSubstepTape &tp, StateRef s0, StateRef s1,
+                           hipStream_t st, float4 *tiles)
     {
         CrHipBatch *b = a->solver->b;
-        hipStream_t st = b->stream;
         const size_t nodes = (size_t)b->totalBlocks * 64;
         const int K = b->iterations, n = b->totalActive;
         if (n == 0)
@@ -5179,7 +5296,9 @@ namespace
         BatchArgs whole = args;
         whole.subSize = 1 << 30;
         whole.subMax = 1;
-        const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
+        if (tiles)
+            whole.tilebuf = tiles;
+        const int wholeItems = std::max(1, (int)b->totalBlocks); // one workgroup per pblock
         const int blockWgs = grid1d(b->totalBlocks, kWaves);
         const int denseWgs = grid1d(n, kWG);
         const unsigned *keys = tp.keys;
@@ -5190,7 +5309,12 @@ namespace
         {
             const float4 *U = tp.U + (size_t)(k - 1) * nodes;
             const float4 *Uprev = k > 1 ? U - nodes : nullptr;
-            k_pbadj_uscatter<<<wholeItems, 64, 0, st>>>(whole, tp.st, pa);
+            if (b->scatterWaves == 4)
+                k_pbadj_uscatter<4><<<wholeItems, 256, 0, st>>>(whole, tp.st, pa);
+            else if (b->scatterWaves == 2)
+                k_pbadj_uscatter<2><<<wholeItems, 128, 0, st>>>(whole, tp.st, pa);
+            else
+                k_pbadj_uscatter<1><<<wholeItems, 64, 0, st>>>(whole, tp.st, pa);
             k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
             k_pbadj_particle<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, s0, a->gridAdj, Uprev, pa, ad);
         }
@@ -5280,45 +5404,165 @@ namespace
     // then reversed; otherwise the re-run keeps only every substep's start
     // state and each substep is re-run once more, last first, with its own
     // tape before its reverse.  Both give bitwise the same gradients.
+    // What two segment tapes need beside the tape: a second set of segment
+    // states, partial boxes of the reverse's own, a stream (default
+    // priority) and the events ordering the two streams.
+    int pb_ensure_overlap(CrHipAdjoint *a)
+    {
+        CrHipBatch *b = a->solver->b;
+        // the partial boxes follow the batch's layout, which a later tile
+        // split may widen
+        const size_t tiles = (size_t)b->totalBlocks * b->subStride * kTileNodes;
+        if (a->revTilesCount < tiles)
+        {
+            if (a->revStream)
+                (void)hipStreamSynchronize(a->revStream);
+            dfree(a->revTiles);
+            a->revTilesCount = 0;
+            if (dalloc(&a->revTiles, tiles))
+                return CRMPM_HIP_ERR_ALLOC;
+            a->revTilesCount = tiles;
+        }
+        if (a->revStream)
+            return CRMPM_HIP_OK;
+        const size_t N = (size_t)std::max(b->capacity, 1);
+        if (!a->seg2)
+        {
+            a->seg2 = new (std::nothrow) StateBuf[a->every + 1]();
+            if (!a->seg2)
+                return CRMPM_HIP_ERR_ALLOC;
+            int rc = 0;
+            for (int i = 0; i < a->every + 1; ++i)
+                rc |= state_alloc(a->seg2[i], N);
+            if (rc)
+                return CRMPM_HIP_ERR_ALLOC; // freed with the adjoint
+        }
+        for (hipEvent_t *e : {&a->ran[0], &a->ran[1], &a->rev[0], &a->rev[1], &a->start})
+            if (!*e && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+            {
+                *e = nullptr;
+                return CRMPM_HIP_ERR_ALLOC;
+            }
+        if (hipStreamCreateWithFlags(&a->revStream, hipStreamNonBlocking) != hipSuccess)
+        {
+            a->revStream = nullptr;
+            return CRMPM_HIP_ERR_ALLOC;
+        }
+        return CRMPM_HIP_OK;
+    }
+
+    // Reverse sweep of a PB rollout, three ways by the tape budget:
+    //  * two segment tapes fit: segment s is reversed on the adjoint's stream
+    //    while segment s - 1 is re-run (taped) on the batch's stream, the
+    //    two sets of tapes and segment states alternating.  The reverse
+    //    kernels hold about one wavefront per SIMD at the sizes of a
+    //    differentiable scene, so the re-run mostly fills idle SIMDs;
+    //  * one fits: each segment is re-run, then reversed;
+    //  * none: each substep is re-run once more before it is reversed.
+    // The gradients are bitwise the same in all three (each reversed
+    // substep reads the same taped values).
     int pb_reverse(CrHipAdjoint *a, const AdjRef &ad)
     {
         CrHipBatch *b = a->solver->b;
         const int K = b->iterations;
         const int stride = b->capacity;
-        const bool whole = pb_substep_tape_bytes(b, K) * (size_t)a->every <= a->tapeBudget;
-        int rc = pb_ensure_tape(a, K, whole ? a->every : 1);
+        const size_t segBytes = pb_substep_tape_bytes(b, K) * (size_t)a->every;
+        const bool overlap = a->segments > 1 && 2 * segBytes <= a->tapeBudget;
+        const bool whole = overlap || segBytes <= a->tapeBudget;
+        int rc = pb_ensure_tape(a, K, whole ? a->every * (overlap ? 2 : 1) : 1);
+        if (!rc && overlap)
+            rc = pb_ensure_overlap(a);
+        if (rc)
+            return rc;
         const StateRef live{b->P.xm, b->P.v, b->F, b->C, stride};
+        auto states = [&](int set) { return set ? a->seg2 : a->seg; };
+        // re-run segment sgm from its checkpoint into tape set `set`
+        auto rerun = [&](int sgm, int set) {
+            const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
+            StateBuf *S = states(set);
+            int r = state_copy(b, a->ckpt[sgm], false);
+            for (int t = t0; t < t1 && !r; ++t)
+            {
+                const int j = t - t0;
+                r = state_copy(b, S[j], true);
+                if (r)
+                    break;
+                const PbSubstepTape tp = pb_tape_slot(a, K, set * a->every + j);
+                r = batch_step_taped(b, const_cast<float4 *>(tp.U));
+                if (!r)
+                    r = pb_save_substep(b, tp);
+            }
+            return r ? r : state_copy(b, S[t1 - t0], true);
+        };
+        auto reverse = [&](int sgm, int set, hipStream_t st, float4 *tiles) {
+            const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
+            StateBuf *S = states(set);
+            int r = CRMPM_HIP_OK;
+            for (int t = t1 - 1; t >= t0 && !r; --t)
+            {
+                const int j = t - t0;
+                r = pb_reverse_substep(a, ad, pb_tape_slot(a, K, set * a->every + j), state_ref(S[j], stride),
+                                       state_ref(S[j + 1], stride), st, tiles);
+            }
+            return r;
+        };
+        if (overlap)
+        {
+            auto ok = [](hipError_t e) { return e == hipSuccess; };
+            hipStream_t rs = a->revStream;
+            // the reverse stream starts after the work enqueued so far
+            if (!ok(hipEventRecord(a->start, b->stream)) || !ok(hipStreamWaitEvent(rs, a->start, 0)))
+                return CRMPM_HIP_ERR_LAUNCH;
+            const int last = a->segments - 1;
+            rc = rerun(last, last & 1);
+            if (!rc && !ok(hipEventRecord(a->ran[last & 1], b->stream)))
+                rc = CRMPM_HIP_ERR_LAUNCH;
+            for (int sgm = last; sgm >= 0 && !rc; --sgm)
+            {
+                const int set = sgm & 1, other = set ^ 1;
+                if (!ok(hipStreamWaitEvent(rs, a->ran[set], 0)))
+                    rc = CRMPM_HIP_ERR_LAUNCH;
+                if (!rc && sgm > 0)
+                {
+                    // set `other` was read by the reverse of segment sgm + 1
+                    if (sgm < last && !ok(hipStreamWaitEvent(b->stream, a->rev[other], 0)))
+                        rc = CRMPM_HIP_ERR_LAUNCH;
+                    if (!rc)
+                        rc = rerun(sgm - 1, other);
+                    if (!rc && !ok(hipEventRecord(a->ran[other], b->stream)))
+                        rc = CRMPM_HIP_ERR_LAUNCH;
+                }
+                if (!rc)
+                    rc = reverse(sgm, set, rs, a->revTiles);
+                if (!rc && !ok(hipEventRecord(a->rev[set], rs)))
+                    rc = CRMPM_HIP_ERR_LAUNCH;
+            }
+            // the batch stream goes on after the last reverse (or, on an
+            // error, after whatever the reverse stream holds)
+            if (ok(hipEventRecord(a->start, rs)))
+                (void)hipStreamWaitEvent(b->stream, a->start, 0);
+            return rc;
+        }
         for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
         {
+            if (whole)
+            {
+                rc = rerun(sgm, 0);
+                if (!rc)
+                    rc = reverse(sgm, 0, b->stream, nullptr);
+                continue;
+            }
             const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
             rc = state_copy(b, a->ckpt[sgm], false);
             for (int t = t0; t < t1 && !rc; ++t)
             {
-                const int j = t - t0;
-                rc = state_copy(b, a->seg[j], true);
-                if (rc)
-                    break;
-                if (whole)
-                {
-                    const PbSubstepTape tp = pb_tape_slot(a, K, j);
-                    rc = batch_step_taped(b, const_cast<float4 *>(tp.U));
-                    if (!rc)
-                        rc = pb_save_substep(b, tp);
-                }
-                else if (t + 1 < t1)
+                rc = state_copy(b, a->seg[t - t0], true);
+                if (!rc && t + 1 < t1)
                     rc = batch_step(b);
             }
-            if (!rc && whole)
-                rc = state_copy(b, a->seg[t1 - t0], true);
             for (int t = t1 - 1; t >= t0 && !rc; --t)
             {
                 const int j = t - t0;
-                if (whole)
-                {
-                    rc = pb_reverse_substep(a, ad, pb_tape_slot(a, K, j), state_ref(a->seg[j], stride),
-                                            state_ref(a->seg[j + 1], stride));
-                    continue;
-                }
                 rc = state_copy(b, a->seg[j], false);
                 PbSubstepTape tp{};
                 tp.U = reinterpret_cast<const float4 *>(a->tape);
@@ -5333,7 +5577,7 @@ namespace
                 tp.activeList = b->activeList;
                 tp.activeCount = b->activeCount;
                 tp.st = b->st;
-                rc = pb_reverse_substep(a, ad, tp, state_ref(a->seg[j], stride), live);
+                rc = pb_reverse_substep(a, ad, tp, state_ref(a->seg[j], stride), live, b->stream, nullptr);
             }
         }
         return rc;
@@ -5379,6 +5623,17 @@ namespace
         dfree(a->paC);
         dfree(a->paF);
         dfree(a->paFi);
+        if (a->revStream)
+            (void)hipStreamSynchronize(a->revStream);
+        for (int i = 0; a->seg2 && i < a->every + 1; ++i)
+            state_free(a->seg2[i]);
+        delete[] a->seg2;
+        dfree(a->revTiles);
+        for (hipEvent_t e : {a->ran[0], a->ran[1], a->rev[0], a->rev[1], a->start})
+            if (e)
+                (void)hipEventDestroy(e);
+        if (a->revStream)
+            (void)hipStreamDestroy(a->revStream);
         delete a;
         return CRMPM_HIP_OK;
     }
@@ -5462,6 +5717,15 @@ int crHipAdjointSetTapeBudget(CrHipAdjoint *a, unsigned long long bytes)
     if (!a)
         return CRMPM_HIP_ERR_ARG;
     a->tapeBudget = (size_t)bytes;
+    return CRMPM_HIP_OK;
+}
+
+int crHipAdjointGetSegmentTapeBytes(const CrHipAdjoint *a, unsigned long long *bytes)
+{
+    if (!a || !bytes)
+        return CRMPM_HIP_ERR_ARG;
+    const CrHipBatch *b = a->solver->b;
+    *bytes = b->kind == CRMPM_HIP_PB ? (unsigned long long)(pb_substep_tape_bytes(b, b->iterations) * (size_t)a->every) : 0ull;
     return CRMPM_HIP_OK;
 }
 
@@ -5574,14 +5838,19 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
             BatchArgs whole = args;
             whole.subSize = 1 << 30;
             whole.subMax = 1;
-            const int wholeItems = std::max(1, (int)b->totalBlocks); // one-wavefront workgroups, one per pblock
+            const int wholeItems = std::max(1, (int)b->totalBlocks); // one workgroup per pblock
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
             const int denseWgs = grid1d(n, kWG);
             const MlsScatterIn si{a->paV, a->paC};
             const StateRef s0 = state_ref(a->seg[j], stride);
             k_adj_g2p<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, state_ref(a->seg[j + 1], stride),
                                                 a->segGrid[j], ad, si);
-            k_adj_uscatter<<<wholeItems, 64, 0, st>>>(whole, s0, si);
+            if (b->scatterWaves == 4)
+                k_adj_uscatter<4><<<wholeItems, 256, 0, st>>>(whole, s0, si);
+            else if (b->scatterWaves == 2)
+                k_adj_uscatter<2><<<wholeItems, 128, 0, st>>>(whole, s0, si);
+            else
+                k_adj_uscatter<1><<<wholeItems, 64, 0, st>>>(whole, s0, si);
             k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(whole, a->segGrid[j], a->gridAdj);
             k_adj_p2g<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, a->gridAdj, ad);
             rc = check_launch();

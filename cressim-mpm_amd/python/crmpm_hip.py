@@ -146,6 +146,10 @@ EXPORTS = {
     "crHipBatchSetGatherSplit": [_vp, ctypes.c_int, ctypes.c_int],
     "crHipBatchGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "crHipSolverGetGatherSplit": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "crHipBatchSetScatterWaves": [_vp, ctypes.c_int],
+    "crHipBatchGetScatterWaves": [_vp, ctypes.POINTER(ctypes.c_int)],
+    "crHipSolverSetScatterWaves": [_vp, ctypes.c_int],
+    "crHipSolverGetScatterWaves": [_vp, ctypes.POINTER(ctypes.c_int)],
     "crHipSolverSetLaunchConfig": [_vp] + [ctypes.c_int] * 6,
     "crHipSolverSetGraphMode": [_vp, ctypes.c_int],
     "crHipBatchFetchResults": [_vp],
@@ -159,6 +163,7 @@ EXPORTS = {
     "crHipAdjointForward": [_vp],
     "crHipAdjointBackward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "crHipAdjointSetTapeBudget": [_vp, ctypes.c_ulonglong],
+    "crHipAdjointGetSegmentTapeBytes": [_vp, ctypes.POINTER(ctypes.c_ulonglong)],
     "crHipAdjointDestroy": [_vp],
 }
 
@@ -507,10 +512,24 @@ class HipSolver:
 
     def set_launch(self, cfg):
         """Set a launch configuration (tile waves, update waves, sub-tile size,
-        sub-tiles per pblock, gather sub-tile size, gather sub-tiles) without
-        timing trials; returns it as autotune() + gather_split() would."""
-        check(self.lib.crHipSolverSetLaunchConfig(self.h, *[int(c) for c in cfg]), "launch config")
-        return tuple(int(c) for c in cfg)
+        sub-tiles per pblock, gather sub-tile size, gather sub-tiles[, scatter
+        wavefronts per tile, default 1]) without timing trials; returns it as
+        autotune() + gather_split() + (scatter_waves(),) would."""
+        cfg = [int(c) for c in cfg]
+        waves = cfg[6] if len(cfg) > 6 else 1
+        check(self.lib.crHipSolverSetLaunchConfig(self.h, *cfg[:6]), "launch config")
+        self.set_scatter_waves(waves)
+        return tuple(cfg[:6]) + (waves,)
+
+    def set_scatter_waves(self, waves):
+        """Wavefronts per tile of the PB scatter and the adjoints' velocity
+        scatters (1, 2 or 4)."""
+        check(self.lib.crHipSolverSetScatterWaves(self.h, int(waves)), "scatter waves")
+
+    def scatter_waves(self):
+        w = ctypes.c_int()
+        check(self.lib.crHipSolverGetScatterWaves(self.h, ctypes.byref(w)), "scatter waves")
+        return w.value
 
     def gather_split(self):
         """(sub-tile size, sub-tiles per pblock) of the G2P kernels."""
@@ -871,6 +890,14 @@ class HipBatch:
     def set_launch_sizes(self, tile_waves, update_waves):
         check(self.lib.crHipBatchSetLaunchSizes(self.h, tile_waves, update_waves), "launch sizes")
 
+    def set_scatter_waves(self, waves):
+        check(self.lib.crHipBatchSetScatterWaves(self.h, int(waves)), "scatter waves")
+
+    def scatter_waves(self):
+        w = ctypes.c_int()
+        check(self.lib.crHipBatchGetScatterWaves(self.h, ctypes.byref(w)), "scatter waves")
+        return w.value
+
     def sync(self):
         check(self.lib.crHipBatchFetchResults(self.h), "fetch results")
 
@@ -1057,10 +1084,17 @@ class DifferentiablePb(DifferentiableMls):
     KIND = S.PB
 
     def set_tape_budget(self, nbytes):
-        """Device memory the backward pass may take for a whole checkpoint
-        segment's tape (default 8 GiB); below it each substep is re-run
-        twice in the backward pass instead of once."""
+        """Device memory the backward pass may take for checkpoint segments'
+        tapes (default 8 GiB): two segments' tapes let the re-run of one
+        segment overlap the reverse of the next; with one, each substep is
+        re-run once in the backward pass; below it, twice."""
         check(self.solver.lib.crHipAdjointSetTapeBudget(self.h, int(nbytes)), "adjoint tape budget")
+
+    def segment_tape_bytes(self):
+        """Bytes of one checkpoint segment's tape."""
+        v = ctypes.c_ulonglong()
+        check(self.solver.lib.crHipAdjointGetSegmentTapeBytes(self.h, ctypes.byref(v)), "segment tape bytes")
+        return v.value
 
 
 _HIPRT = None

@@ -160,6 +160,10 @@ int crHipSolverSetGraphMode(CrHipSolver *s, int enabled);
 /* crHipBatchGetGatherSplit for the solver's one-scene batch (the split the
  * autotuner picked for the G2P kernels). */
 int crHipSolverGetGatherSplit(const CrHipSolver *s, int *subSize, int *subMax);
+/* crHipBatchSetScatterWaves / crHipBatchGetScatterWaves for the solver's
+ * one-scene batch (no counterpart in the reference). */
+int crHipSolverSetScatterWaves(CrHipSolver *s, int waves);
+int crHipSolverGetScatterWaves(const CrHipSolver *s, int *waves);
 /* Launch timing, an addition of this solver (the reference times nothing on
  * the device).  Between Begin and End every particle-transfer, grid-update and
  * binning launch of crHipSolverStep is bracketed by a pair of hipEvents on the
@@ -272,6 +276,16 @@ int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
  * the reference. */
 int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax);
 int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax);
+/* Wavefronts per tile (1, 2 or 4; else CRMPM_HIP_ERR_ARG) of the PB-MPM
+ * scatter and of the adjoints' velocity-adjoint scatters: with more than one,
+ * a tile's chunks are shared by the wavefronts of one workgroup, each into its
+ * own LDS accumulation tiles, summed in a fixed order into the tile's one
+ * partial box (for dense pblocks, whose chunks otherwise run on one
+ * wavefront).  Deterministic for a given setting; the summation order, and so
+ * the rounding, depends on it.  Default 1; crHipBatchAutotune picks it for PB
+ * batches.  No counterpart in the reference. */
+int crHipBatchSetScatterWaves(CrHipBatch *b, int waves);
+int crHipBatchGetScatterWaves(const CrHipBatch *b, int *waves);
 int crHipBatchFetchResults(CrHipBatch *b);
 int crHipBatchGetStateArrays(CrHipBatch *b, float **F, float **C, float **V0, int *stride);
 int crHipBatchCopyGrid(CrHipBatch *b, int scene, float *dst);
@@ -514,14 +528,20 @@ int crHipAdjointForward(CrHipAdjoint *a);
 int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, double *dLdLambda, double *dLdMu,
                          float *dLdx0, float *dLdv0);
 /* PB only (no counterpart in the reference; for callers that trade device
- * memory for time): the device memory the backward pass may take for a
- * whole checkpoint segment's tape -- every substep's iteration grids, sort
+ * memory for time): the device memory the backward pass may take for
+ * checkpoint segments' tapes -- every substep's iteration grids, sort
  * products and staged particle arrays, about
- * iterations x nodes x 16 B + 72 B x particles per substep -- so that each
- * substep is re-run once in the backward pass instead of twice.  Default
- * 8 GiB; a segment that does not fit is reversed substep by substep.  The
- * gradients are bitwise the same either way. */
+ * iterations x nodes x 16 B + 72 B x particles per substep.  When two
+ * segments' tapes fit, segment s is reversed on a stream of the adjoint's
+ * own while segment s - 1 is re-run on the solver's stream (the backward's
+ * work is then complete when the solver's stream is); when one fits, each
+ * segment is re-run once, then reversed; otherwise each substep is re-run
+ * once more before it is reversed.  Default 8 GiB.  The gradients are
+ * bitwise the same in all three. */
 int crHipAdjointSetTapeBudget(CrHipAdjoint *a, unsigned long long bytes);
+/* PB: the bytes of one segment's tape (0 for MLS); no counterpart in the
+ * reference. */
+int crHipAdjointGetSegmentTapeBytes(const CrHipAdjoint *a, unsigned long long *bytes);
 int crHipAdjointDestroy(CrHipAdjoint *a);
 
 #ifdef __cplusplus

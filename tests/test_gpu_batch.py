@@ -115,12 +115,15 @@ def test_autotune_keeps_state_and_results():
     for i in range(len(scenes)):
         for k in FIELDS:
             assert np.array_equal(before[i][k], after[i][k]), (i, k)
-    # every launch size gives the same arithmetic (the tile split fixes the
-    # summation order): persistent launches against one wavefront per sub-tile
+    # every launch size gives the same arithmetic (the tile split and the
+    # wavefronts per scatter tile fix the summation order): persistent
+    # launches against one wavefront per sub-tile
     ref = H.HipBatch(copy.deepcopy(scenes))
     ref.compute_initial_data()
     ref.step(2)
     ref.set_tile_split(ss, sm)
+    # wavefronts per scatter tile fix a summation order as the split does
+    ref.set_scatter_waves(b.scatter_waves())
     b.set_launch_sizes(2048, 2048)
     b.step(3)
     ref.step(3)

@@ -48,13 +48,14 @@ def fp32_rollout(sc, alpha, ratio, v0, steps):
     return PA.rollout(x0, v0, F0, C0, c, steps)
 
 
-def run_case(sc, steps, every, tape_budget=None):
+def run_case(sc, steps, every, tape_budget=None, scatter_waves=1):
     import crmpm_hip as H
 
     n = len(sc["xm"])
     sim = H.DifferentiablePb(sc, steps, checkpoint_every=every)
     if tape_budget is not None:
         sim.set_tape_budget(tape_budget)
+    sim.solver.set_scatter_waves(scatter_waves)
     try:
         dev = sim.solver.device
         alpha = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
@@ -117,15 +118,24 @@ def test_pb_adjoint_matches_fp64_autograd(case):
 
 
 def test_pb_adjoint_segment_tape_equals_substep_reruns():
-    """The backward pass tapes a whole checkpoint segment when it fits the
-    tape budget and otherwise re-runs each substep before reversing it: the
-    same gradients, bitwise."""
+    """The backward pass reverses a segment while re-running the one before
+    it when two segments' tapes fit the tape budget, tapes one segment at a
+    time when one fits, and otherwise re-runs each substep before reversing
+    it: the same gradients, bitwise (7 substeps in segments of 3: three
+    segments, the last one short)."""
+    import crmpm_hip as H
+
     sc = pb_block(seed=21, iterations=6)
     steps = 7
-    a = run_case(sc, steps, 3)
-    b = run_case(sc, steps, 3, tape_budget=0)
-    for u, w in zip(a[2:], b[2:]):
-        assert np.array_equal(u, w)
+    probe = H.DifferentiablePb(sc, steps, checkpoint_every=3)
+    seg = probe.segment_tape_bytes()
+    probe.close()
+    assert seg > 0
+    runs = [run_case(sc, steps, 3), run_case(sc, steps, 3, tape_budget=seg + seg // 2),
+            run_case(sc, steps, 3, tape_budget=0)]
+    for r in runs[1:]:
+        for u, w in zip(r[2:], runs[0][2:]):
+            assert np.array_equal(u, w)
 
 
 def test_pb_adjoint_checkpoint_interval_independent():

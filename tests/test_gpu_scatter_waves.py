@@ -1,0 +1,146 @@
+"""Several wavefronts per scatter tile (crHipBatchSetScatterWaves): the PB
+scatter and the adjoints' velocity-adjoint scatters share a dense pblock's
+chunks among the wavefronts of one workgroup, each accumulating into its own
+LDS tiles, and sum those tiles in a fixed order into the tile's one partial
+box.  Only the summation order changes, so results are bitwise the same run to
+run, within FP32 rounding of one wavefront per tile, and on the oracle at the
+parity bound.  The scenes hold 8 particles per cell (up to 512 per pblock,
+eight chunks), so every setting splits real work.
+"""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+import cpu_checkers as CC
+import crmpm_scenes as S
+import test_gpu_pb_adjoint as PBA
+from test_adjoint_cpu import block_scene
+from test_pb_adjoint_cpu import pb_block
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("xm", "v", "F", "C")
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-30)
+
+
+def relnorm(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def pb_run(sc, waves, steps):
+    import crmpm_hip as H
+
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.set_scatter_waves(waves)
+    assert g.scatter_waves() == waves
+    g.compute_initial_data()
+    g.step(steps)
+    st = g.state()
+    g.close()
+    return st
+
+
+def test_pb_scatter_waves_forward():
+    sc = S.small_scene(S.PB, 11, n_side=16, dx=1 / 32, shapes="plane+sphere")
+    steps = 10
+    print(f"PB scatter waves: {len(sc['xm'])} particles, {steps} substeps", flush=True)
+    one = pb_run(sc, 1, steps)
+    o = CC.CpuSolver(sc, "oracle")
+    o.compute_initial_data()
+    o.step(steps)
+    ref = o.state()
+    o.close()
+    for w in (2, 4):
+        a, b = pb_run(sc, w, steps), pb_run(sc, w, steps)
+        for k in FIELDS:
+            assert np.array_equal(a[k], b[k]), (w, k)
+            e1 = rel(a[k], one[k])
+            print(f"  {w} wavefronts, {k}: against one wavefront {e1:.2e}", flush=True)
+            assert e1 <= 1e-5, (w, k)
+        for k in ("xm", "v"):
+            eo = rel(a[k][:, :3], ref[k][:, :3])
+            print(f"  {w} wavefronts, {k}: against the oracle {eo:.2e}", flush=True)
+            assert eo <= 1e-4, (w, k)
+
+
+def test_pb_autotune_may_pick_scatter_waves():
+    """The autotuner times 2 and 4 wavefronts per scatter tile for PB and
+    reports its pick; the state is unchanged by the trials."""
+    import crmpm_hip as H
+
+    sc = S.small_scene(S.PB, 12, n_side=16, dx=1 / 32, shapes="plane")
+    g = H.HipSolver(copy.deepcopy(sc))
+    g.compute_initial_data()
+    g.step(2)
+    before = g.state()
+    g.autotune(steps_per_trial=2)
+    w = g.scatter_waves()
+    print(f"autotune picked {w} wavefronts per scatter tile", flush=True)
+    assert w in (1, 2, 4)
+    after = g.state()
+    for k in FIELDS:
+        assert np.array_equal(before[k], after[k]), k
+    g.close()
+
+
+def test_pb_adjoint_scatter_waves():
+    """The PB adjoint with 2 and 4 wavefronts per scatter tile (forward and
+    velocity-adjoint scatters): bitwise reproducible, and against FP64
+    autograd of the restatement.  The alpha and ratio gradients contract the
+    constraint's small residual, so each FP32 summation order is another
+    sample of an error of the FP32 restatement's size (here about 1.4e-3 for
+    alpha; one wavefront per tile gives its own sample): the bound is three
+    times the restatement's error, capped as in tests/test_gpu_pb_adjoint.py,
+    where a wrong or missing term gives errors of order one."""
+    sc = pb_block(seed=9, n=(16, 12, 10), iterations=6)
+    steps = 6
+    _, gx, *one = PBA.run_case(sc, steps, 3)
+    _, ra, rr, rv = PBA.reference_grads(sc, steps, gx)
+    _, fa, fr, fv = PBA.reference_grads(sc, steps, gx, torch.float32)
+    for name, u, r in zip(("alpha", "ratio", "v0"), one, (ra, rr, rv)):
+        print(f"PB adjoint, 1 wavefront: d/d{name} against FP64 {PBA.relnorm(u, r):.2e}", flush=True)
+    for w in (2, 4):
+        a = PBA.run_case(sc, steps, 3, scatter_waves=w)
+        b = PBA.run_case(sc, steps, 3, scatter_waves=w)
+        assert np.array_equal(a[1], gx)
+        for name, u, v, r, f, cap in zip(("alpha", "ratio", "v0"), a[2:], b[2:], (ra, rr, rv), (fa, fr, fv),
+                                         (1e-1, 1e-1, 1e-3)):
+            assert np.array_equal(u, v), (w, name)
+            eg, e32 = PBA.relnorm(u, r), PBA.relnorm(f, r)
+            print(f"PB adjoint, {w} wavefronts: d/d{name} against FP64 {eg:.2e} (FP32 restatement {e32:.2e})",
+                  flush=True)
+            assert eg <= min(cap, max(3.0 * e32, 1e-5)), (w, name)
+        assert relnorm(a[4], one[2]) <= 1e-5  # v0: well conditioned
+
+
+def test_mls_adjoint_scatter_waves():
+    import crmpm_hip as H
+
+    sc = block_scene(seed=6, n=(16, 12, 10))
+    n = len(sc["xm"])
+    steps = 8
+    out = {}
+    for w in (1, 4, 4):
+        sim = H.DifferentiableMls(sc, steps, checkpoint_every=3)
+        sim.solver.set_scatter_waves(w)
+        dev = sim.solver.device
+        lam = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
+        mu = torch.as_tensor(sc["params"][:, 1], dtype=torch.float64, device=dev)
+        v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
+        x = sim.forward(lam, mu, v0)
+        gx = torch.as_tensor(np.random.default_rng(5).normal(0.0, 1.0, (n, 3)) / n, device=dev)
+        grads = [g.double().cpu().numpy() for g in sim.backward(gx)]
+        sim.close()
+        if w in out:
+            for u, v in zip(grads, out[w]):
+                assert np.array_equal(u, v)
+        out[w] = grads
+    for name, u, r in zip(("lambda", "mu", "v0"), out[4], out[1]):
+        e = relnorm(u, r)
+        print(f"MLS adjoint, 4 wavefronts: d/d{name} against one wavefront {e:.2e}", flush=True)
+        assert e <= 1e-5, name
