@@ -559,7 +559,7 @@ def adjoint(steps=100, every=10, cpu=True):
 
     sc, log_e = S.config4(steps=steps)
     hidden, log_e_true = S.config4(steps=steps, field_seed=S.HIDDEN_FIELD_SEED)
-    sim = H.DifferentiableMls(sc, steps, checkpoint_every=every, tune=True)
+    sim = H.DifferentiableMls(sc, steps, checkpoint_every=every, tune=LAUNCH or True)
     dev = sim.solver.device
     nu = 0.3
     v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
@@ -605,7 +605,10 @@ def adjoint(steps=100, every=10, cpu=True):
     sim.close()
     plain = H.HipSolver(copy.deepcopy(sc))
     plain.compute_initial_data()
-    plain.autotune(steps_per_trial=2)
+    if LAUNCH:
+        plain.set_launch(LAUNCH)
+    else:
+        plain.autotune(steps_per_trial=2)
     plain.set_graph_mode(True)
     plain.step(5)
     torch.cuda.synchronize()
@@ -654,7 +657,7 @@ def pb_adjoint(steps=20, every=10):
     import crmpm_hip as H
 
     sc = pb_block_scene(steps)
-    sim = H.DifferentiablePb(sc, steps, checkpoint_every=every, tune=True)
+    sim = H.DifferentiablePb(sc, steps, checkpoint_every=every, tune=LAUNCH or True)
     dev = sim.solver.device
     alpha = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
     ratio = torch.as_tensor(sc["params"][:, 1], dtype=torch.float64, device=dev)
@@ -690,7 +693,10 @@ def pb_adjoint(steps=20, every=10):
     sim.close()
     plain = H.HipSolver(copy.deepcopy(sc))
     plain.compute_initial_data()
-    plain.autotune(steps_per_trial=2)
+    if LAUNCH:
+        plain.set_launch(LAUNCH)
+    else:
+        plain.autotune(steps_per_trial=2)
     plain.set_graph_mode(True)
     plain.step(2)
     torch.cuda.synchronize()

@@ -172,6 +172,9 @@ namespace
         // Split of the gather kernels (G2P), which write no partials: a
         // pblock's particles in sub-tiles of gSubSize, at most gSubMax.
         int gSubSize, gSubMax;
+        // Tiles a scatter launch takes (tile_class_ok): 0 all, 1 those of at
+        // most 64 particles, 2 the others.
+        int tileClass;
     };
 
     // Scene records are never written by a kernel: reading them through the
@@ -1082,6 +1085,16 @@ namespace
         store_tile_sum<W * KAcc, 64 * W>(acc, out, (int)threadIdx.x);
     }
 
+    // Whether a scatter launch of tile class cls takes a (sub-)tile of count
+    // particles: with adaptive wavefronts per tile, one launch takes the
+    // tiles of one chunk on one wavefront each and another the larger ones
+    // on four; a tile's class depends on its count alone, so its summation
+    // order is fixed.
+    CR_DEV bool tile_class_ok(int cls, int count)
+    {
+        return cls == 0 || ((count <= 64) == (cls == 1));
+    }
+
     CR_DEV void lds_add(float4 *A, int li, float x, float y, float z, float w)
     {
         float4 cur = A[li];
@@ -1454,6 +1467,67 @@ namespace
     // MLS-MPM P2G (src/solver/mls_mpm_algorithm.h:147-241)
     // ------------------------------------------------------------------
 
+    // One particle of MLS P2G (src/solver/mls_mpm_algorithm.h:147-241):
+    // its momentum, mass and affine terms scattered into the tile's LDS
+    // accumulation tiles A0 in conflict-free passes.
+    CR_DEV void mls_p2g_particle(const StepCommon &c, const TileInfo &ti, float4 *A0, int nSlicers, int p)
+    {
+        const GridGeom &g = c.g;
+        const float4 xm = c.P.xm[p];
+        const float4 vv = c.P.v[p];
+        const M3 F = load_pm(c.S.F, p);
+        const M3 Cm = load_pm(c.S.C, p);
+        const float4 prm = c.P.params[p];
+        const int type = c.P.type[p];
+        const float v0 = c.S.V0[p];
+        const V3 x = v3(xm);
+        const float m = xm.w;
+        int bx, by, bz;
+        V3 fx;
+        cell_of(g, x, bx, by, bz, fx);
+        // v += g dt before the scatter (Q9).  The reference writes it
+        // back; only G2P's slicer reflection reads it before G2P
+        // overwrites v, so G2P forms the same sum instead and P2G
+        // issues no global store until its tile flush.
+        const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
+        const M3 stress = stress_kirchhoff(prm.x, prm.y, type, F);
+        const float sc = -c.dt * 4.0f * g.invdx * g.invdx * v0;
+        const M3 affine = stress * sc + Cm * m;
+        const V3 vm = vel * m;
+        const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
+        const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
+        // The weights and the separable terms are formed inside each
+        // pass from inputs made opaque there (a chunk takes one pass
+        // in the rank-pair order), so 15 inputs stay live across the
+        // passes instead of the 36 terms.
+        M3 af = affine;
+        V3 v_m = vm, f = fx;
+        in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
+            float4 *A = A0 + tile * kTileNodes;
+            opaque(af);
+            opaque(v_m);
+            opaque(f);
+            Weights w;
+            weights_of(f, w);
+            if (fast)
+            {
+                // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
+                scatter_separable(A, l0, scatter_terms(v_m, af, 1.0f, w, axis_offsets(f, g.dx)), m);
+                return;
+            }
+            auto sc = [&](int li, int, int, int, V3 dist, float wt) {
+                const V3 contrib = (v_m + mul(af, dist)) * wt;
+                lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
+            };
+            for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
+                V3 nrm;
+                if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
+                    return;
+                sc(li, a, b, cc, dist, wt);
+            });
+        });
+    }
+
     template <bool Persistent, int Waves = kWaves>
     __global__ __launch_bounds__(64 * Waves) void k_mls_p2g(BatchArgs b)
     {
@@ -1471,77 +1545,48 @@ namespace
             TileInfo ti;
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
+            if (!tile_class_ok(b.tileClass, ti.end - ti.start))
+                continue;
             wave_sync(); // the previous tile's LDS reads are done
             float4 *A0 = acc[wave * kScatterTiles];
             zero_tiles<kScatterTiles>(A0, lane);
             wave_sync();
             const int nSlicers = slicer_count(c.sv);
-            const GridGeom &g = c.g;
-            // the next chunk's particle index is loaded one chunk ahead, and all
-            // of a particle's state is loaded before the velocity store
+            // the next chunk's particle index is loaded one chunk ahead
             int pNext = (int)c.order[min(ti.start + lane, ti.end - 1)];
             settle(pNext);
             for (int k = ti.start + lane; k < ti.end; k += 64)
             {
                 const int p = pNext;
                 pNext = (int)c.order[min(k + 64, ti.end - 1)];
-                const float4 xm = c.P.xm[p];
-                const float4 vv = c.P.v[p];
-                const M3 F = load_pm(c.S.F, p);
-                const M3 Cm = load_pm(c.S.C, p);
-                const float4 prm = c.P.params[p];
-                const int type = c.P.type[p];
-                const float v0 = c.S.V0[p];
-                const V3 x = v3(xm);
-                const float m = xm.w;
-                int bx, by, bz;
-                V3 fx;
-                cell_of(g, x, bx, by, bz, fx);
-                // v += g dt before the scatter (Q9).  The reference writes it
-                // back; only G2P's slicer reflection reads it before G2P
-                // overwrites v, so G2P forms the same sum instead and P2G
-                // issues no global store until its tile flush.
-                const V3 vel = v3(vv) + v3(c.gravity.x, c.gravity.y, c.gravity.z) * c.dt;
-                const M3 stress = stress_kirchhoff(prm.x, prm.y, type, F);
-                const float sc = -c.dt * 4.0f * g.invdx * g.invdx * v0;
-                const M3 affine = stress * sc + Cm * m;
-                const V3 vm = vel * m;
-                const bool fast = nodes_inside(g, bx, by, bz) && (nSlicers == 0 || slicer_free(c.sv, nSlicers, x, g.dx));
-                const int l0 = fast ? base_li(ti, bx, by, bz) : 0;
-                // The weights and the separable terms are formed inside each
-                // pass from inputs made opaque there (a chunk takes one pass
-                // in the rank-pair order), so 15 inputs stay live across the
-                // passes instead of the 36 terms.
-                M3 af = affine;
-                V3 v_m = vm, f = fx;
-                in_conflict_free_passes<kScatterTiles>(cell_key(ti, bx, by, bz), [&](int tile) {
-                    float4 *A = A0 + tile * kTileNodes;
-                    opaque(af);
-                    opaque(v_m);
-                    opaque(f);
-                    Weights w;
-                    weights_of(f, w);
-                    if (fast)
-                    {
-                        // (v m + A d) w and w m: the separable form with terms v m + A d, mass factor m
-                        scatter_separable(A, l0, scatter_terms(v_m, af, 1.0f, w, axis_offsets(f, g.dx)), m);
-                        return;
-                    }
-                    auto sc = [&](int li, int, int, int, V3 dist, float wt) {
-                        const V3 contrib = (v_m + mul(af, dist)) * wt;
-                        lds_add(A, li, contrib.x, contrib.y, contrib.z, wt * m);
-                    };
-                    for_each_node<false>(g, ti, bx, by, bz, f, w, [&](int li, int a, int b, int cc, V3 dist, float wt) {
-                        V3 nrm;
-                        if (nSlicers > 0 && different_sides<false>(c.sv, nSlicers, x, x + dist, nrm))
-                            return;
-                        sc(li, a, b, cc, dist, wt);
-                    });
-                });
+                mls_p2g_particle(c, ti, A0, nSlicers, p);
             }
             wave_sync();
             store_tile_sum<kScatterTiles>(A0, c.tilebuf + (size_t)ti.slot * kTileNodes, lane);
         }
+    }
+
+    // k_mls_p2g with W wavefronts per (sub-)tile (wg_tile_scatter), one
+    // workgroup per tile slot: for dense pblocks, whose chunks otherwise run
+    // on one wavefront.
+    template <int W>
+    __global__ __launch_bounds__(64 * W) void k_mls_p2g_wg(BatchArgs b)
+    {
+        __shared__ float4 acc[W * kScatterTiles][kTileNodes];
+        const int tg = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+        if (tg >= num_subtiles(b) || !subtile_used(b, tg))
+            return;
+        int t;
+        const StepCommon c = scene_view(b, tg / b.subMax, t);
+        TileInfo ti;
+        if (!subtile_info(c, t, tg % b.subMax, ti))
+            return;
+        if (!tile_class_ok(b.tileClass, ti.end - ti.start))
+            return;
+        const int nSlicers = slicer_count(c.sv);
+        wg_tile_scatter<W, kScatterTiles>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](float4 *A, int k) {
+            mls_p2g_particle(c, ti, A, nSlicers, (int)c.order[k]);
+        });
     }
 
     // ------------------------------------------------------------------
@@ -2203,6 +2248,8 @@ namespace
             TileInfo ti;
             if (!subtile_info(c, t, tg % b.subMax, ti))
                 continue;
+            if (!tile_class_ok(b.tileClass, ti.end - ti.start))
+                continue;
             wave_sync(); // the previous tile's LDS reads are done
             Acc *A = acc[wave * kAcc];
             zero_tiles<kAcc>(A, lane);
@@ -2236,6 +2283,8 @@ namespace
         const StepCommon c = scene_view(b, tg / b.subMax, t);
         TileInfo ti;
         if (!subtile_info(c, t, tg % b.subMax, ti))
+            return;
+        if (!tile_class_ok(b.tileClass, ti.end - ti.start))
             return;
         const int nSlicers = slicer_count(c.sv);
         const int S = st.stride;
@@ -2723,6 +2772,8 @@ namespace
         TileInfo ti;
         if (!subtile_info(c, t, tg % b.subMax, ti))
             return;
+        if (!tile_class_ok(b.tileClass, ti.end - ti.start))
+            return;
         wg_tile_scatter<W, kAcc>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](F3 *A, int q) {
             pb_scatter<kAcc>(c, ti, A, 0, v3(s0.x[c.order[q]]), 1.0f, v3(si.v[q]), transpose(load_cols(si.B, q)));
         });
@@ -3144,6 +3195,8 @@ namespace
         TileInfo ti;
         if (!subtile_info(c, t, tg % b.subMax, ti))
             return;
+        if (!tile_class_ok(b.tileClass, ti.end - ti.start))
+            return;
         wg_tile_scatter<W, kAcc>(acc[0], ti, c.tilebuf + (size_t)ti.slot * kTileNodes, [&](F3 *A, int q) {
             const M3 Bb = load_cols(pa.C, q) * (4.0f * c.g.invdx);
             pb_scatter<kAcc>(c, ti, A, 0, v3(st.xs[q]), 1.0f, v3(pa.v[q]), transpose(Bb));
@@ -3382,8 +3435,9 @@ struct CrHipBatch
     // the allocated partial boxes per pblock)
     int subSize = 64, subMax = 1, subStride = 1;
     int gSubSize = 1 << 20, gSubMax = 1; // gather kernels' split
-    // wavefronts per tile of the PB scatter and the adjoints' velocity
-    // scatters (1, 2 or 4; wg_tile_scatter)
+    // wavefronts per tile of the PB and MLS scatters (P2G) and the
+    // adjoints' velocity scatters (1, 2 or 4; wg_tile_scatter), or 0:
+    // four for tiles of more than 64 particles, one for the others
     int scatterWaves = 1;
     // tile-ordered particle permutation of the last enqueued substep, and its sorted keys
     const unsigned *lastOrder = nullptr;
@@ -3578,6 +3632,7 @@ namespace
         a.subStride = b->subStride;
         a.gSubSize = b->gSubSize;
         a.gSubMax = b->gSubMax;
+        a.tileClass = 0;
         return a;
     }
 
@@ -3709,7 +3764,22 @@ namespace
             // MLS and standard MPM keep four-wavefront workgroups: their sparse
             // grids have many empty items, whose workgroups cost dispatch time
             // (one-wavefront groups: MLS cube substep 0.109 -> 0.117 ms)
-            launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
+            const dim3 wgGrid((unsigned)std::max(1ll, tileItems));
+            if (b->scatterWaves == 4)
+                launch(b, CRMPM_HIP_PROF_MLS_P2G, k_mls_p2g_wg<4>, wgGrid, dim3(256), a);
+            else if (b->scatterWaves == 2)
+                launch(b, CRMPM_HIP_PROF_MLS_P2G, k_mls_p2g_wg<2>, wgGrid, dim3(128), a);
+            else if (b->scatterWaves == 1)
+                launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, a);
+            else
+            {
+                // adaptive: tiles of one chunk on one wavefront, the others on four
+                BatchArgs small = a, large = a;
+                small.tileClass = 1;
+                large.tileClass = 2;
+                launch(b, CRMPM_HIP_PROF_MLS_P2G, pers ? k_mls_p2g<true> : k_mls_p2g<false>, tileWGs, kWG, small);
+                launch(b, CRMPM_HIP_PROF_MLS_P2G, k_mls_p2g_wg<4>, wgGrid, dim3(256), large);
+            }
             BatchArgs au = a;
             au.tapeGrid = b->tape;
             launch(b, CRMPM_HIP_PROF_UPDATE, update_kernel<0>(multi, persU), updWGs, kWG, au, b->dirty);
@@ -3750,25 +3820,35 @@ namespace
                 // the scatter: one wavefront per tile (persistent or not), or
                 // scatterWaves wavefronts per tile, one workgroup per slot
                 const dim3 wgGrid((unsigned)std::max(1ll, tileItems));
-                auto scatter = [&](bool mass, int prof) {
-                    switch (b->scatterWaves)
+                auto scatterW = [&](bool mass, int prof, int w, const BatchArgs &ar) {
+                    switch (w)
                     {
                     case 2:
-                        launch(b, prof, mass ? k_pb_scatter_wg<true, 2> : k_pb_scatter_wg<false, 2>, wgGrid, dim3(128), a,
+                        launch(b, prof, mass ? k_pb_scatter_wg<true, 2> : k_pb_scatter_wg<false, 2>, wgGrid, dim3(128), ar,
                                b->st);
                         break;
                     case 4:
-                        launch(b, prof, mass ? k_pb_scatter_wg<true, 4> : k_pb_scatter_wg<false, 4>, wgGrid, dim3(256), a,
+                        launch(b, prof, mass ? k_pb_scatter_wg<true, 4> : k_pb_scatter_wg<false, 4>, wgGrid, dim3(256), ar,
                                b->st);
                         break;
                     default:
                         if (mass)
                             launch(b, prof, pers ? k_pb_scatter<true, true, 1> : k_pb_scatter<true, false, 1>, tileGrid,
-                                   tileBlock, a, b->st);
+                                   tileBlock, ar, b->st);
                         else
                             launch(b, prof, pers ? k_pb_scatter<false, true, 1> : k_pb_scatter<false, false, 1>, tileGrid,
-                                   tileBlock, a, b->st);
+                                   tileBlock, ar, b->st);
                     }
+                };
+                // adaptive (0): tiles of one chunk on one wavefront, the others on four
+                auto scatter = [&](bool mass, int prof) {
+                    if (b->scatterWaves != 0)
+                        return scatterW(mass, prof, b->scatterWaves, a);
+                    BatchArgs small = a, large = a;
+                    small.tileClass = 1;
+                    large.tileClass = 2;
+                    scatterW(mass, prof, 1, small);
+                    scatterW(mass, prof, 4, large);
                 };
                 launch(b, CRMPM_HIP_PROF_PB_P2G, k_pb_dense<0>, denseWGs, kWG, a, b->st, n, keys);
                 scatter(true, CRMPM_HIP_PROF_PB_P2G);
@@ -4314,12 +4394,13 @@ namespace
                 bestU = c;
             }
         }
-        // 5. PB: 2 or 4 wavefronts per scatter tile (dense pblocks)
+        // 5. PB and MLS: 2 or 4 wavefronts per scatter tile, or 4 for the
+        //    tiles of more than one chunk only (dense pblocks)
         int bestW = 1;
-        for (int w : {2, 4})
+        for (int w : {2, 4, 0})
         {
             float ms = 0.0f;
-            if (rc || b->kind != CRMPM_HIP_PB)
+            if (rc || b->kind == CRMPM_HIP_STANDARD)
                 break;
             b->scatterWaves = w;
             rc = tune_trial(b, backup, steps, bestT, bestU, bestS, bestM, bestGS, bestGM, &ms);
@@ -4861,7 +4942,7 @@ int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax)
 
 int crHipBatchSetScatterWaves(CrHipBatch *b, int waves)
 {
-    if (!b || (waves != 1 && waves != 2 && waves != 4))
+    if (!b || (waves != 0 && waves != 1 && waves != 2 && waves != 4))
         return CRMPM_HIP_ERR_ARG;
     b->scatterWaves = waves;
     return CRMPM_HIP_OK;
@@ -5298,6 +5379,9 @@ namespace
         whole.subMax = 1;
         if (tiles)
             whole.tilebuf = tiles;
+        BatchArgs wholeSmall = whole, wholeLarge = whole; // adaptive wavefronts per tile
+        wholeSmall.tileClass = 1;
+        wholeLarge.tileClass = 2;
         const int wholeItems = std::max(1, (int)b->totalBlocks); // one workgroup per pblock
         const int blockWgs = grid1d(b->totalBlocks, kWaves);
         const int denseWgs = grid1d(n, kWG);
@@ -5313,8 +5397,13 @@ namespace
                 k_pbadj_uscatter<4><<<wholeItems, 256, 0, st>>>(whole, tp.st, pa);
             else if (b->scatterWaves == 2)
                 k_pbadj_uscatter<2><<<wholeItems, 128, 0, st>>>(whole, tp.st, pa);
-            else
+            else if (b->scatterWaves == 1)
                 k_pbadj_uscatter<1><<<wholeItems, 64, 0, st>>>(whole, tp.st, pa);
+            else
+            {
+                k_pbadj_uscatter<1><<<wholeItems, 64, 0, st>>>(wholeSmall, tp.st, pa);
+                k_pbadj_uscatter<4><<<wholeItems, 256, 0, st>>>(wholeLarge, tp.st, pa);
+            }
             k_adj_grid<1><<<blockWgs, kWG, 0, st>>>(whole, U, a->gridAdj);
             k_pbadj_particle<<<denseWgs, kWG, 0, st>>>(args, tp.st, n, keys, s0, a->gridAdj, Uprev, pa, ad);
         }
@@ -5849,8 +5938,16 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
                 k_adj_uscatter<4><<<wholeItems, 256, 0, st>>>(whole, s0, si);
             else if (b->scatterWaves == 2)
                 k_adj_uscatter<2><<<wholeItems, 128, 0, st>>>(whole, s0, si);
-            else
+            else if (b->scatterWaves == 1)
                 k_adj_uscatter<1><<<wholeItems, 64, 0, st>>>(whole, s0, si);
+            else
+            {
+                BatchArgs wholeSmall = whole, wholeLarge = whole;
+                wholeSmall.tileClass = 1;
+                wholeLarge.tileClass = 2;
+                k_adj_uscatter<1><<<wholeItems, 64, 0, st>>>(wholeSmall, s0, si);
+                k_adj_uscatter<4><<<wholeItems, 256, 0, st>>>(wholeLarge, s0, si);
+            }
             k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(whole, a->segGrid[j], a->gridAdj);
             k_adj_p2g<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, a->gridAdj, ad);
             rc = check_launch();

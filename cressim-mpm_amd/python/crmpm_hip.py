@@ -522,8 +522,9 @@ class HipSolver:
         return tuple(cfg[:6]) + (waves,)
 
     def set_scatter_waves(self, waves):
-        """Wavefronts per tile of the PB scatter and the adjoints' velocity
-        scatters (1, 2 or 4)."""
+        """Wavefronts per tile of the PB and MLS scatters and the adjoints'
+        velocity scatters (1, 2 or 4, or 0: four for tiles of more than 64
+        particles, one for the others)."""
         check(self.lib.crHipSolverSetScatterWaves(self.h, int(waves)), "scatter waves")
 
     def scatter_waves(self):
@@ -960,8 +961,13 @@ class DifferentiableMls:
         self.torch = torch
         self.solver = HipSolver(scene, device=device)
         self.solver.compute_initial_data()
-        if tune:
-            self.tuned = self.solver.autotune(steps_per_trial=2)
+        # tune: True runs the autotuner; a launch configuration (as
+        # HipSolver.set_launch takes) is set as given
+        if isinstance(tune, (tuple, list)):
+            self.tuned = self.solver.set_launch(tune)
+        elif tune:
+            self.tuned = (self.solver.autotune(steps_per_trial=2) + self.solver.gather_split() +
+                          (self.solver.scatter_waves(),))
         # the forward pass (and the backward's re-runs that keep no tape)
         # replay each substep as a hipGraph
         self.solver.set_graph_mode(graph)

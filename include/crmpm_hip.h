@@ -276,14 +276,15 @@ int crHipBatchSetLaunchSizes(CrHipBatch *b, int tileWaves, int updateWaves);
  * the reference. */
 int crHipBatchSetGatherSplit(CrHipBatch *b, int subSize, int subMax);
 int crHipBatchGetGatherSplit(const CrHipBatch *b, int *subSize, int *subMax);
-/* Wavefronts per tile (1, 2 or 4; else CRMPM_HIP_ERR_ARG) of the PB-MPM
- * scatter and of the adjoints' velocity-adjoint scatters: with more than one,
- * a tile's chunks are shared by the wavefronts of one workgroup, each into its
- * own LDS accumulation tiles, summed in a fixed order into the tile's one
- * partial box (for dense pblocks, whose chunks otherwise run on one
- * wavefront).  Deterministic for a given setting; the summation order, and so
+/* Wavefronts per tile (1, 2 or 4, or 0: four for tiles of more than 64
+ * particles and one for the others; else CRMPM_HIP_ERR_ARG) of the PB-MPM
+ * and MLS scatters (P2G) and of the adjoints' velocity-adjoint scatters:
+ * with more than one, a tile's chunks are shared by the wavefronts of one
+ * workgroup, each into its own LDS accumulation tiles, summed in a fixed
+ * order into the tile's one partial box (for dense pblocks, whose chunks
+ * otherwise run on one wavefront).  Deterministic for a given setting; the summation order, and so
  * the rounding, depends on it.  Default 1; crHipBatchAutotune picks it for PB
- * batches.  No counterpart in the reference. */
+ * and MLS batches.  No counterpart in the reference. */
 int crHipBatchSetScatterWaves(CrHipBatch *b, int waves);
 int crHipBatchGetScatterWaves(const CrHipBatch *b, int *waves);
 int crHipBatchFetchResults(CrHipBatch *b);

@@ -1,10 +1,11 @@
 """Several wavefronts per scatter tile (crHipBatchSetScatterWaves): the PB
-scatter and the adjoints' velocity-adjoint scatters share a dense pblock's
-chunks among the wavefronts of one workgroup, each accumulating into its own
-LDS tiles, and sum those tiles in a fixed order into the tile's one partial
-box.  Only the summation order changes, so results are bitwise the same run to
-run, within FP32 rounding of one wavefront per tile, and on the oracle at the
-parity bound.  The scenes hold 8 particles per cell (up to 512 per pblock,
+and MLS scatters (P2G) and the adjoints' velocity-adjoint scatters share a
+dense pblock's chunks among the wavefronts of one workgroup, each
+accumulating into its own LDS tiles, and sum those tiles in a fixed order
+into the tile's one partial box.  Setting 0 gives four wavefronts to tiles
+of more than 64 particles and one to the others.  Only the summation order
+changes, so results are bitwise the same run to run, within FP32 rounding
+of one wavefront per tile, and on the oracle at the parity bound.  The scenes hold 8 particles per cell (up to 512 per pblock,
 eight chunks), so every setting splits real work.
 """
 import copy
@@ -32,7 +33,7 @@ def relnorm(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def pb_run(sc, waves, steps):
+def run(sc, waves, steps):
     import crmpm_hip as H
 
     g = H.HipSolver(copy.deepcopy(sc))
@@ -45,18 +46,22 @@ def pb_run(sc, waves, steps):
     return st
 
 
-def test_pb_scatter_waves_forward():
-    sc = S.small_scene(S.PB, 11, n_side=16, dx=1 / 32, shapes="plane+sphere")
+@pytest.mark.parametrize("kind", ["pb", "mls"])
+def test_scatter_waves_forward(kind):
+    """PB and MLS P2G with 2 and 4 wavefronts per tile (MLS with a slicer
+    through the block, so the cut-aware scatter path runs too)."""
+    k = {"pb": S.PB, "mls": S.MLS}[kind]
+    sc = S.small_scene(k, 11, n_side=16, dx=1 / 32, shapes="plane+sphere" if kind == "pb" else "plane+sphere+slicer")
     steps = 10
-    print(f"PB scatter waves: {len(sc['xm'])} particles, {steps} substeps", flush=True)
-    one = pb_run(sc, 1, steps)
+    print(f"{kind} scatter waves: {len(sc['xm'])} particles, {steps} substeps", flush=True)
+    one = run(sc, 1, steps)
     o = CC.CpuSolver(sc, "oracle")
     o.compute_initial_data()
     o.step(steps)
     ref = o.state()
     o.close()
-    for w in (2, 4):
-        a, b = pb_run(sc, w, steps), pb_run(sc, w, steps)
+    for w in (2, 4, 0):
+        a, b = run(sc, w, steps), run(sc, w, steps)
         for k in FIELDS:
             assert np.array_equal(a[k], b[k]), (w, k)
             e1 = rel(a[k], one[k])
@@ -69,8 +74,8 @@ def test_pb_scatter_waves_forward():
 
 
 def test_pb_autotune_may_pick_scatter_waves():
-    """The autotuner times 2 and 4 wavefronts per scatter tile for PB and
-    reports its pick; the state is unchanged by the trials."""
+    """The autotuner times 2 and 4 wavefronts per scatter tile (PB and MLS)
+    and reports its pick; the state is unchanged by the trials."""
     import crmpm_hip as H
 
     sc = S.small_scene(S.PB, 12, n_side=16, dx=1 / 32, shapes="plane")
@@ -81,7 +86,7 @@ def test_pb_autotune_may_pick_scatter_waves():
     g.autotune(steps_per_trial=2)
     w = g.scatter_waves()
     print(f"autotune picked {w} wavefronts per scatter tile", flush=True)
-    assert w in (1, 2, 4)
+    assert w in (0, 1, 2, 4)
     after = g.state()
     for k in FIELDS:
         assert np.array_equal(before[k], after[k]), k
@@ -104,7 +109,7 @@ def test_pb_adjoint_scatter_waves():
     _, fa, fr, fv = PBA.reference_grads(sc, steps, gx, torch.float32)
     for name, u, r in zip(("alpha", "ratio", "v0"), one, (ra, rr, rv)):
         print(f"PB adjoint, 1 wavefront: d/d{name} against FP64 {PBA.relnorm(u, r):.2e}", flush=True)
-    for w in (2, 4):
+    for w in (2, 4, 0):
         a = PBA.run_case(sc, steps, 3, scatter_waves=w)
         b = PBA.run_case(sc, steps, 3, scatter_waves=w)
         assert np.array_equal(a[1], gx)
@@ -119,13 +124,20 @@ def test_pb_adjoint_scatter_waves():
 
 
 def test_mls_adjoint_scatter_waves():
+    """The MLS adjoint with 4 wavefronts per scatter tile (the forward's P2G
+    and the velocity-adjoint scatter): bitwise reproducible, and within
+    1e-3 (lambda, mu) and 1e-4 (v0) of one wavefront per tile.  The forward's
+    summation order changes too, and the lambda and mu gradients are
+    ill-conditioned in FP32: on the 120-particle block of
+    tests/test_gpu_adjoint.py an FP32 evaluation of the FP64 restatement is
+    1.2e-2 from it, so a wrong term (errors of order one) stands far out."""
     import crmpm_hip as H
 
     sc = block_scene(seed=6, n=(16, 12, 10))
     n = len(sc["xm"])
     steps = 8
     out = {}
-    for w in (1, 4, 4):
+    for w in (1, 4, 4, 0, 0):
         sim = H.DifferentiableMls(sc, steps, checkpoint_every=3)
         sim.solver.set_scatter_waves(w)
         dev = sim.solver.device
@@ -140,7 +152,8 @@ def test_mls_adjoint_scatter_waves():
             for u, v in zip(grads, out[w]):
                 assert np.array_equal(u, v)
         out[w] = grads
-    for name, u, r in zip(("lambda", "mu", "v0"), out[4], out[1]):
-        e = relnorm(u, r)
-        print(f"MLS adjoint, 4 wavefronts: d/d{name} against one wavefront {e:.2e}", flush=True)
-        assert e <= 1e-5, name
+    for w in (4, 0):
+        for name, u, r, bound in zip(("lambda", "mu", "v0"), out[w], out[1], (1e-3, 1e-3, 1e-4)):
+            e = relnorm(u, r)
+            print(f"MLS adjoint, wavefronts {w}: d/d{name} against one wavefront {e:.2e}", flush=True)
+            assert e <= bound, (w, name)
