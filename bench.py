@@ -113,6 +113,8 @@ def valu_issue_us(instructions):
 PB_LAUNCH = "0,4096,1048576,1,1048576,1"
 MLS_CUTTING_LAUNCH = "16384,4096,1048576,1,1048576,0"
 MLS_CUBE_LAUNCH = "4096,0,128,8,1048576,0"
+PB_ADJOINT_LAUNCH = "4096,2048,1048576,1,1048576,1,4"
+ADJOINT_LAUNCH = "0,4096,1048576,1,1048576,0,4"
 PROFILE_TARGETS = {
     "pb_slab": dict(run=f"python3 bench.py --workload pb_slab --cpu-substeps 0 --launch {PB_LAUNCH}", counters=True,
                     title="Kernel trace and counters: PB-MPM slab, 262,144 particles, 20 iterations, 510 substeps",
@@ -125,7 +127,7 @@ PROFILE_TARGETS = {
                      counters=True,
                      title="Kernel trace and counters: MLS-MPM elastic cube, 32,768 particles, 205 substeps",
                      md="mls_cube32k_kernel_trace.md"),
-    "pb_adjoint": dict(run="python3 bench.py --workload pb_adjoint", counters=False,
+    "pb_adjoint": dict(run=f"python3 bench.py --workload pb_adjoint --launch {PB_ADJOINT_LAUNCH}", counters=False,
                        title="Kernel trace: differentiable PB block, 65,536 particles, 20 iterations, 20 substeps, "
                              "forward + adjoint",
                        md="pb_adjoint65k_kernel_trace.md"),
@@ -140,7 +142,8 @@ PROFILE_TARGETS = {
                    title="Kernel trace: engine advanceAll, 64 MLS scenes in frames of 10 substeps with host readback, "
                          "then the same batch stepped alone",
                    md="engine64_kernel_trace.md"),
-    "adjoint": dict(run="python3 bench.py --workload adjoint --cpu-substeps 0", counters=False,
+    "adjoint": dict(run=f"python3 bench.py --workload adjoint --cpu-substeps 0 --launch {ADJOINT_LAUNCH}",
+                    counters=False,
                     title="Kernel trace: differentiable MLS block, 65,536 particles, 100 substeps, forward + adjoint",
                     md="adjoint65k_kernel_trace.md"),
 }
