@@ -5540,6 +5540,90 @@ namespace
         return CRMPM_HIP_OK;
     }
 
+    // MLS: the bytes of one set of per-segment buffers (grids, sort products,
+    // segments and active lists of a segment's substeps).
+    size_t mls_set_bytes(const CrHipAdjoint *a)
+    {
+        const CrHipBatch *b = a->solver->b;
+        const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
+        return (size_t)a->every * (B * 64 * sizeof(float4) + 2 * N * sizeof(unsigned) + (3 * B + 1) * sizeof(int));
+    }
+
+    int pb_ensure_overlap(CrHipAdjoint *a);
+
+    // MLS: the second set of per-segment buffers, beside what
+    // pb_ensure_overlap provides (second segment states, partial boxes,
+    // stream, events).
+    int mls_ensure_overlap(CrHipAdjoint *a)
+    {
+        CrHipBatch *b = a->solver->b;
+        const size_t N = (size_t)std::max(b->capacity, 1), B = (size_t)b->totalBlocks;
+        int rc = 0;
+        for (int q = a->every; q < 2 * a->every; ++q)
+        {
+            if (!a->segGrid[q])
+                rc |= dalloc(&a->segGrid[q], B * 64);
+            if (!a->segOrder[q])
+                rc |= dalloc(&a->segOrder[q], N);
+            if (!a->segStart[q])
+                rc |= dalloc(&a->segStart[q], B);
+            if (!a->segEnd[q])
+                rc |= dalloc(&a->segEnd[q], B);
+            if (!a->segActive[q])
+                rc |= dalloc(&a->segActive[q], B + 1);
+            if (!a->segKeys[q])
+                rc |= dalloc(&a->segKeys[q], N);
+        }
+        return rc ? CRMPM_HIP_ERR_ALLOC : pb_ensure_overlap(a);
+    }
+
+    // The overlapped backward schedule: segment s is reversed on the
+    // adjoint's stream while segment s - 1 is re-run on the batch's stream,
+    // the two sets of per-segment buffers alternating.  rerun(sgm, set)
+    // enqueues a segment's re-run on the batch's stream; reverse(sgm, set,
+    // stream) its reverse.  Events order each re-run after the reverse that
+    // last read its set and each reverse after its re-run; the batch's
+    // stream goes on after the last reverse.
+    template <class Rerun, class Reverse>
+    int overlapped_reverse(CrHipAdjoint *a, Rerun &&rerun, Reverse &&reverse)
+    {
+        CrHipBatch *b = a->solver->b;
+        auto ok = [](hipError_t e) { return e == hipSuccess; };
+        hipStream_t rs = a->revStream;
+        // the reverse stream starts after the work enqueued so far
+        if (!ok(hipEventRecord(a->start, b->stream)) || !ok(hipStreamWaitEvent(rs, a->start, 0)))
+            return CRMPM_HIP_ERR_LAUNCH;
+        const int last = a->segments - 1;
+        int rc = rerun(last, last & 1);
+        if (!rc && !ok(hipEventRecord(a->ran[last & 1], b->stream)))
+            rc = CRMPM_HIP_ERR_LAUNCH;
+        for (int sgm = last; sgm >= 0 && !rc; --sgm)
+        {
+            const int set = sgm & 1, other = set ^ 1;
+            if (!ok(hipStreamWaitEvent(rs, a->ran[set], 0)))
+                rc = CRMPM_HIP_ERR_LAUNCH;
+            if (!rc && sgm > 0)
+            {
+                // set `other` was read by the reverse of segment sgm + 1
+                if (sgm < last && !ok(hipStreamWaitEvent(b->stream, a->rev[other], 0)))
+                    rc = CRMPM_HIP_ERR_LAUNCH;
+                if (!rc)
+                    rc = rerun(sgm - 1, other);
+                if (!rc && !ok(hipEventRecord(a->ran[other], b->stream)))
+                    rc = CRMPM_HIP_ERR_LAUNCH;
+            }
+            if (!rc)
+                rc = reverse(sgm, set, rs);
+            if (!rc && !ok(hipEventRecord(a->rev[set], rs)))
+                rc = CRMPM_HIP_ERR_LAUNCH;
+        }
+        // the batch stream goes on after the last reverse (or, on an error,
+        // after whatever the reverse stream holds)
+        if (ok(hipEventRecord(a->start, rs)))
+            (void)hipStreamWaitEvent(b->stream, a->start, 0);
+        return rc;
+    }
+
     // Reverse sweep of a PB rollout, three ways by the tape budget:
     //  * two segment tapes fit: segment s is reversed on the adjoint's stream
     //    while segment s - 1 is re-run (taped) on the batch's stream, the
@@ -5596,42 +5680,9 @@ namespace
             return r;
         };
         if (overlap)
-        {
-            auto ok = [](hipError_t e) { return e == hipSuccess; };
-            hipStream_t rs = a->revStream;
-            // the reverse stream starts after the work enqueued so far
-            if (!ok(hipEventRecord(a->start, b->stream)) || !ok(hipStreamWaitEvent(rs, a->start, 0)))
-                return CRMPM_HIP_ERR_LAUNCH;
-            const int last = a->segments - 1;
-            rc = rerun(last, last & 1);
-            if (!rc && !ok(hipEventRecord(a->ran[last & 1], b->stream)))
-                rc = CRMPM_HIP_ERR_LAUNCH;
-            for (int sgm = last; sgm >= 0 && !rc; --sgm)
-            {
-                const int set = sgm & 1, other = set ^ 1;
-                if (!ok(hipStreamWaitEvent(rs, a->ran[set], 0)))
-                    rc = CRMPM_HIP_ERR_LAUNCH;
-                if (!rc && sgm > 0)
-                {
-                    // set `other` was read by the reverse of segment sgm + 1
-                    if (sgm < last && !ok(hipStreamWaitEvent(b->stream, a->rev[other], 0)))
-                        rc = CRMPM_HIP_ERR_LAUNCH;
-                    if (!rc)
-                        rc = rerun(sgm - 1, other);
-                    if (!rc && !ok(hipEventRecord(a->ran[other], b->stream)))
-                        rc = CRMPM_HIP_ERR_LAUNCH;
-                }
-                if (!rc)
-                    rc = reverse(sgm, set, rs, a->revTiles);
-                if (!rc && !ok(hipEventRecord(a->rev[set], rs)))
-                    rc = CRMPM_HIP_ERR_LAUNCH;
-            }
-            // the batch stream goes on after the last reverse (or, on an
-            // error, after whatever the reverse stream holds)
-            if (ok(hipEventRecord(a->start, rs)))
-                (void)hipStreamWaitEvent(b->stream, a->start, 0);
-            return rc;
-        }
+            return overlapped_reverse(a, rerun, [&](int sgm, int set, hipStream_t rs) {
+                return reverse(sgm, set, rs, a->revTiles);
+            });
         for (int sgm = a->segments - 1; sgm >= 0 && !rc; --sgm)
         {
             if (whole)
@@ -5680,7 +5731,7 @@ namespace
             state_free(a->ckpt[i]);
         for (int i = 0; a->seg && i < a->every + 1; ++i)
             state_free(a->seg[i]);
-        for (int i = 0; i < a->every; ++i)
+        for (int i = 0; i < 2 * a->every; ++i)
         {
             if (a->segGrid)
                 dfree(a->segGrid[i]);
@@ -5749,12 +5800,12 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
     const size_t nodes = (size_t)b->totalBlocks * 64;
     a->ckpt = new (std::nothrow) StateBuf[a->segments + 1];
     a->seg = new (std::nothrow) StateBuf[a->every + 1];
-    a->segGrid = new (std::nothrow) float4 *[a->every]();
-    a->segOrder = new (std::nothrow) unsigned *[a->every]();
-    a->segStart = new (std::nothrow) int *[a->every]();
-    a->segEnd = new (std::nothrow) int *[a->every]();
-    a->segActive = new (std::nothrow) unsigned *[a->every]();
-    a->segKeys = new (std::nothrow) unsigned *[a->every]();
+    a->segGrid = new (std::nothrow) float4 *[2 * a->every](); // a second set for overlapped reverses
+    a->segOrder = new (std::nothrow) unsigned *[2 * a->every]();
+    a->segStart = new (std::nothrow) int *[2 * a->every]();
+    a->segEnd = new (std::nothrow) int *[2 * a->every]();
+    a->segActive = new (std::nothrow) unsigned *[2 * a->every]();
+    a->segKeys = new (std::nothrow) unsigned *[2 * a->every]();
     int rc = (a->ckpt && a->seg && a->segGrid && a->segOrder && a->segStart && a->segEnd && a->segActive && a->segKeys)
                  ? 0
                  : CRMPM_HIP_ERR_ALLOC;
@@ -5814,7 +5865,8 @@ int crHipAdjointGetSegmentTapeBytes(const CrHipAdjoint *a, unsigned long long *b
     if (!a || !bytes)
         return CRMPM_HIP_ERR_ARG;
     const CrHipBatch *b = a->solver->b;
-    *bytes = b->kind == CRMPM_HIP_PB ? (unsigned long long)(pb_substep_tape_bytes(b, b->iterations) * (size_t)a->every) : 0ull;
+    *bytes = b->kind == CRMPM_HIP_PB ? (unsigned long long)(pb_substep_tape_bytes(b, b->iterations) * (size_t)a->every)
+                                     : (unsigned long long)mls_set_bytes(a);
     return CRMPM_HIP_OK;
 }
 
@@ -5882,75 +5934,99 @@ int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, 
     const int stride = b->capacity;
     if (!rc && b->kind == CRMPM_HIP_PB)
         rc = pb_reverse(a, ad);
-    for (int sgm = a->segments - 1; b->kind == CRMPM_HIP_MLS && sgm >= 0 && !rc; --sgm)
-    {
+    // MLS: re-run a segment from its checkpoint into buffer set `set`,
+    // keeping every substep's start state, updated grid (its active blocks,
+    // written by the grid update) and tiles
+    auto states = [&](int set) { return set ? a->seg2 : a->seg; };
+    auto mrerun = [&](int sgm, int set) {
         const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
-        // recompute the segment from its checkpoint, keeping every substep's
-        // start state, updated grid (its active blocks, written by the grid
-        // update) and tiles
-        rc = state_copy(b, a->ckpt[sgm], false);
-        for (int t = t0; t < t1 && !rc; ++t)
+        StateBuf *S = states(set);
+        int r = state_copy(b, a->ckpt[sgm], false);
+        for (int t = t0; t < t1 && !r; ++t)
         {
-            const int j = t - t0;
-            rc = state_copy(b, a->seg[j], true);
-            if (!rc)
-                rc = batch_step_taped(b, a->segGrid[j]);
-            if (!rc &&
-                (hipMemcpyAsync(a->segOrder[j], b->lastOrder, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
-                 hipMemcpyAsync(a->segKeys[j], b->lastKeys, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
-                 hipMemcpyAsync(a->segStart[j], b->segStart, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
+            const int j = t - t0, q = set * a->every + j;
+            r = state_copy(b, S[j], true);
+            if (!r)
+                r = batch_step_taped(b, a->segGrid[q]);
+            if (!r &&
+                (hipMemcpyAsync(a->segOrder[q], b->lastOrder, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                 hipMemcpyAsync(a->segKeys[q], b->lastKeys, N * sizeof(unsigned), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+                 hipMemcpyAsync(a->segStart[q], b->segStart, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess ||
-                 hipMemcpyAsync(a->segEnd[j], b->segEnd, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
+                 hipMemcpyAsync(a->segEnd[q], b->segEnd, (size_t)b->totalBlocks * sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess ||
-                 hipMemcpyAsync(a->segActive[j], b->activeList, (size_t)b->totalBlocks * sizeof(unsigned),
+                 hipMemcpyAsync(a->segActive[q], b->activeList, (size_t)b->totalBlocks * sizeof(unsigned),
                                 hipMemcpyDeviceToDevice, st) != hipSuccess ||
-                 hipMemcpyAsync(a->segActive[j] + b->totalBlocks, b->activeCount, sizeof(int), hipMemcpyDeviceToDevice,
+                 hipMemcpyAsync(a->segActive[q] + b->totalBlocks, b->activeCount, sizeof(int), hipMemcpyDeviceToDevice,
                                 st) != hipSuccess))
-                rc = CRMPM_HIP_ERR_LAUNCH;
+                r = CRMPM_HIP_ERR_LAUNCH;
         }
-        if (!rc)
-            rc = state_copy(b, a->seg[t1 - t0], true);
-        // reverse sweep over the segment
+        return r ? r : state_copy(b, S[t1 - t0], true);
+    };
+    // the reverse sweep over a segment, on stream rs; tiles: the velocity
+    // adjoint's partial boxes (null: the batch's own)
+    auto mreverse = [&](int sgm, int set, hipStream_t rs, float4 *tiles) {
+        const int t0 = sgm * a->every, t1 = std::min(t0 + a->every, a->steps);
+        StateBuf *S = states(set);
         const int n = b->totalActive;
-        for (int t = t1 - 1; t >= t0 && !rc && n > 0; --t)
+        int r = CRMPM_HIP_OK;
+        for (int t = t1 - 1; t >= t0 && !r && n > 0; --t)
         {
-            const int j = t - t0;
-            BatchArgs args = batch_args(b, a->segOrder[j]);
+            const int j = t - t0, q = set * a->every + j;
+            BatchArgs args = batch_args(b, a->segOrder[q]);
             args.gatherOrder = args.order;
-            args.segStart = a->segStart[j];
-            args.segEnd = a->segEnd[j];
-            args.activeList = a->segActive[j];
-            args.activeCount = reinterpret_cast<const int *>(a->segActive[j] + b->totalBlocks);
+            args.segStart = a->segStart[q];
+            args.segEnd = a->segEnd[q];
+            args.activeList = a->segActive[q];
+            args.activeCount = reinterpret_cast<const int *>(a->segActive[q] + b->totalBlocks);
             // the velocity adjoint's scatter and the grid adjoint take whole
             // pblocks (at most 8 partials per node); the per-particle kernels
             // run 64 sorted particles per wavefront
             BatchArgs whole = args;
             whole.subSize = 1 << 30;
             whole.subMax = 1;
+            if (tiles)
+                whole.tilebuf = tiles;
             const int wholeItems = std::max(1, (int)b->totalBlocks); // one workgroup per pblock
             const int blockWgs = grid1d(b->totalBlocks, kWaves);
             const int denseWgs = grid1d(n, kWG);
             const MlsScatterIn si{a->paV, a->paC};
-            const StateRef s0 = state_ref(a->seg[j], stride);
-            k_adj_g2p<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, state_ref(a->seg[j + 1], stride),
-                                                a->segGrid[j], ad, si);
+            const StateRef s0 = state_ref(S[j], stride);
+            k_adj_g2p<<<denseWgs, kWG, 0, rs>>>(args, n, a->segKeys[q], s0, state_ref(S[j + 1], stride), a->segGrid[q],
+                                                ad, si);
             if (b->scatterWaves == 4)
-                k_adj_uscatter<4><<<wholeItems, 256, 0, st>>>(whole, s0, si);
+                k_adj_uscatter<4><<<wholeItems, 256, 0, rs>>>(whole, s0, si);
             else if (b->scatterWaves == 2)
-                k_adj_uscatter<2><<<wholeItems, 128, 0, st>>>(whole, s0, si);
+                k_adj_uscatter<2><<<wholeItems, 128, 0, rs>>>(whole, s0, si);
             else if (b->scatterWaves == 1)
-                k_adj_uscatter<1><<<wholeItems, 64, 0, st>>>(whole, s0, si);
+                k_adj_uscatter<1><<<wholeItems, 64, 0, rs>>>(whole, s0, si);
             else
             {
                 BatchArgs wholeSmall = whole, wholeLarge = whole;
                 wholeSmall.tileClass = 1;
                 wholeLarge.tileClass = 2;
-                k_adj_uscatter<1><<<wholeItems, 64, 0, st>>>(wholeSmall, s0, si);
-                k_adj_uscatter<4><<<wholeItems, 256, 0, st>>>(wholeLarge, s0, si);
+                k_adj_uscatter<1><<<wholeItems, 64, 0, rs>>>(wholeSmall, s0, si);
+                k_adj_uscatter<4><<<wholeItems, 256, 0, rs>>>(wholeLarge, s0, si);
             }
-            k_adj_grid<0><<<blockWgs, kWG, 0, st>>>(whole, a->segGrid[j], a->gridAdj);
-            k_adj_p2g<<<denseWgs, kWG, 0, st>>>(args, n, a->segKeys[j], s0, a->gridAdj, ad);
-            rc = check_launch();
+            k_adj_grid<0><<<blockWgs, kWG, 0, rs>>>(whole, a->segGrid[q], a->gridAdj);
+            k_adj_p2g<<<denseWgs, kWG, 0, rs>>>(args, n, a->segKeys[q], s0, a->gridAdj, ad);
+            r = check_launch();
+        }
+        return r;
+    };
+    if (!rc && b->kind == CRMPM_HIP_MLS)
+    {
+        // two sets of segment buffers when the second fits the tape budget
+        const bool overlap = a->segments > 1 && mls_set_bytes(a) <= a->tapeBudget && mls_ensure_overlap(a) == CRMPM_HIP_OK;
+        if (overlap)
+            rc = overlapped_reverse(a, mrerun, [&](int sgm, int set, hipStream_t rs) {
+                return mreverse(sgm, set, rs, a->revTiles);
+            });
+        for (int sgm = a->segments - 1; !overlap && sgm >= 0 && !rc; --sgm)
+        {
+            rc = mrerun(sgm, 0);
+            if (!rc)
+                rc = mreverse(sgm, 0, st, nullptr);
         }
     }
     // leave the solver in the state the forward pass ended in

@@ -1065,6 +1065,19 @@ def _rollout_function():
     return MlsRollout
 
 
+def _set_tape_budget(self, nbytes):
+    """Device memory the backward pass may take for checkpoint segments'
+    tapes (default 8 GiB).  PB: two segments' tapes let the re-run of one
+    segment overlap the reverse of the next; with one, each substep is re-run
+    once in the backward pass; below it, twice.  MLS: a second set of
+    per-segment buffers within the budget lets the re-runs overlap the
+    reverses.  The gradients are bitwise the same either way."""
+    check(self.solver.lib.crHipAdjointSetTapeBudget(self.h, int(nbytes)), "adjoint tape budget")
+
+
+DifferentiableMls.set_tape_budget = _set_tape_budget
+
+
 class _LazyRollout:
     fn = None
 
@@ -1088,13 +1101,6 @@ class DifferentiablePb(DifferentiableMls):
     iterations are the scene's."""
 
     KIND = S.PB
-
-    def set_tape_budget(self, nbytes):
-        """Device memory the backward pass may take for checkpoint segments'
-        tapes (default 8 GiB): two segments' tapes let the re-run of one
-        segment overlap the reverse of the next; with one, each substep is
-        re-run once in the backward pass; below it, twice."""
-        check(self.solver.lib.crHipAdjointSetTapeBudget(self.h, int(nbytes)), "adjoint tape budget")
 
     def segment_tape_bytes(self):
         """Bytes of one checkpoint segment's tape."""

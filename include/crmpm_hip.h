@@ -528,20 +528,22 @@ int crHipAdjointCreate(CrHipSolver *s, int numSteps, int checkpointEvery, CrHipA
 int crHipAdjointForward(CrHipAdjoint *a);
 int crHipAdjointBackward(CrHipAdjoint *a, const float *dLdx, const float *dLdv, double *dLdLambda, double *dLdMu,
                          float *dLdx0, float *dLdv0);
-/* PB only (no counterpart in the reference; for callers that trade device
- * memory for time): the device memory the backward pass may take for
- * checkpoint segments' tapes -- every substep's iteration grids, sort
- * products and staged particle arrays, about
+/* No counterpart in the reference; for callers that trade device memory for
+ * time: the device memory the backward pass may take for checkpoint
+ * segments' tapes.  PB: a segment's tape is every substep's iteration grids,
+ * sort products and staged particle arrays, about
  * iterations x nodes x 16 B + 72 B x particles per substep.  When two
  * segments' tapes fit, segment s is reversed on a stream of the adjoint's
- * own while segment s - 1 is re-run on the solver's stream (the backward's
- * work is then complete when the solver's stream is); when one fits, each
- * segment is re-run once, then reversed; otherwise each substep is re-run
- * once more before it is reversed.  Default 8 GiB.  The gradients are
- * bitwise the same in all three. */
+ * own while segment s - 1 is re-run on the solver's stream; when one fits,
+ * each segment is re-run once, then reversed; otherwise each substep is
+ * re-run once more before it is reversed.  MLS: when a second set of
+ * per-segment buffers (grids and sort products, nodes x 16 B + 8 B x
+ * particles per substep) fits, the re-runs overlap the reverses the same way.
+ * With overlap the backward's work is complete when the solver's stream is.
+ * Default 8 GiB.  The gradients are bitwise the same in every mode. */
 int crHipAdjointSetTapeBudget(CrHipAdjoint *a, unsigned long long bytes);
-/* PB: the bytes of one segment's tape (0 for MLS); no counterpart in the
- * reference. */
+/* The bytes of one segment's tape (PB) or of one set of per-segment buffers
+ * (MLS); no counterpart in the reference. */
 int crHipAdjointGetSegmentTapeBytes(const CrHipAdjoint *a, unsigned long long *bytes);
 int crHipAdjointDestroy(CrHipAdjoint *a);
 

@@ -114,6 +114,32 @@ def test_checkpoint_interval_does_not_change_gradients():
             assert np.array_equal(a, b)
 
 
+def test_overlapped_reverse_equals_sequential():
+    """With a second set of segment buffers within the tape budget (the
+    default) each segment's re-run overlaps the next segment's reverse on
+    another stream; with none the segments run one after the other.  The
+    gradients are bitwise the same, backward after backward."""
+    import crmpm_hip as H
+
+    sc = block_scene(seed=8)
+    n = len(sc["xm"])
+    out = []
+    for budget in (None, 0):
+        sim = H.DifferentiableMls(sc, STEPS, checkpoint_every=4)
+        if budget is not None:
+            sim.set_tape_budget(budget)
+        dev = sim.solver.device
+        lam = torch.as_tensor(sc["params"][:, 0], dtype=torch.float64, device=dev)
+        mu = torch.as_tensor(sc["params"][:, 1], dtype=torch.float64, device=dev)
+        v0 = torch.as_tensor(sc["v"][:, :3], dtype=torch.float64, device=dev)
+        target = torch.full((n, 3), 0.5, dtype=torch.float64, device=dev)
+        out.append([gpu_grads(sim, lam, mu, v0, target) for _ in range(2)])
+        sim.close()
+    for a, b in zip(out[0] + out[0][:1], out[1] + out[0][1:]):
+        for u, w in zip(a, b):
+            assert np.array_equal(u, w)
+
+
 def test_rollout_is_a_torch_autograd_function():
     import crmpm_hip as H
 
